@@ -1,0 +1,135 @@
+/*
+ * cathy_gpu.h -- C ABI of the MI355X (gfx950) CATHY subsurface flow step.
+ *
+ * Every entry point takes its arguments BY REFERENCE, with Fortran's
+ * conventions for the arrays the CATHY host already holds (1-based node
+ * and entry ids, column-major 2-D arrays, REAL*8 / INTEGER*4), so that the
+ * Fortran host binds them with a plain ISO_C_BINDING interface block and
+ * passes its own arrays unchanged.  Return value: 0 on success, > 0 for a
+ * solver outcome the host handles (linear-solver failure), < 0 for errors.
+ * All calls are synchronous with respect to the host.
+ *
+ * Reference interfaces replaced (paths relative to the vendored CATHY
+ * source, pyCATHY/tests/weil_exemple/my_cathy_prj/src/):
+ *   cathy_gpu_symslv        SYMSLV   solscal-extended.f:4669-4699
+ *   cathy_gpu_flow3d_*      FLOW3D   flow3d.f:8-294 (Picard: picard.f:8)
+ *                           with the per-time-step host work of
+ *                           cathy_main.f:2882-3860 (NEUMANN free drainage,
+ *                           ATMNXT/ATMBAK, SWITCH_OLD, BKSTEP, CHVELO,
+ *                           STORCAL, TIMUPD) available as one call.
+ */
+#ifndef CATHY_GPU_H
+#define CATHY_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CATHY_GPU_OK 0
+#define CATHY_GPU_LSFAIL 1    /* PCG reached ITMXCG (picard.f:172-177) */
+#define CATHY_GPU_EINVAL (-1) /* bad argument / inconsistent sizes */
+#define CATHY_GPU_EDEVICE (-2) /* HIP runtime error */
+#define CATHY_GPU_ENOMEM (-3)
+#define CATHY_GPU_EHANDLE (-4) /* unknown or destroyed handle */
+
+/* preconditioner orderings (cathy_gpu_flow3d_create, *ordering) */
+#define CATHY_GPU_ORDER_NATURAL 0 /* the reference's IC(0): exact INCLDP order */
+#define CATHY_GPU_ORDER_COLUMN3 1 /* IC(0) in 3-colour column order (fast) */
+
+/* Human-readable text of the last error on this thread. */
+const char *cathy_gpu_last_error(void);
+
+/* Device 0 name and compute-unit count; returns <0 without a GPU. */
+int cathy_gpu_device_info(char *name, int *name_len, int *cu_count);
+
+/* ------------------------------------------------------------------ *
+ * SYMSLV (solscal-extended.f:4669): solve COEF1 * PNEW = TNOTI by IC(0)
+ * preconditioned CG exactly as INCLDP + PRODDP + GRADDP + RESSYM do
+ * (natural-order factor, x0 = M^-1 b, stop on ||r_free|| / ||b_free|| <=
+ * TOLCG or NITER >= ITMXCG).  COEF1/TOPOL/JA are the reference's upper
+ * CSR (diagonal first), NODDIR(1:NUMDIR) the Dirichlet rows whose
+ * diagonal carries the 1e-9*RMAX penalty.  FP64 throughout.
+ * Outputs: PNEW, NITER, ERR (GRADDP's last residual), RES (RESSYM).
+ * Returns CATHY_GPU_LSFAIL when NITER >= ITMXCG.
+ * ------------------------------------------------------------------ */
+int cathy_gpu_symslv(const int *n, const int *nterm, const int *numdir,
+                     int *niter, const int *itmxcg, const double *tolcg,
+                     const double *rmin, const int *noddir, const int *ja,
+                     const int *topol, double *pnew, const double *tnoti,
+                     const double *coef1, double *err, double *res);
+
+/* ------------------------------------------------------------------ *
+ * FLOW3D family.  Set-up takes the mesh and soil arrays the host builds
+ * once (GRDSYS / TPNODI / CHPARM outputs, cathy_main.f:2559-2704):
+ *   TETRA(5,NT)  1-based sorted vertices + zone      (gen3d.f, riord.f)
+ *   BI,CI,DI(4,NT) P1 coefficients /6, VOLU(NT), IVOL(NT)   (volbas.f)
+ *   VOLNOD(N), ARENOD(NNOD)                          (volbas.f, area2d.f)
+ *   PERMX/Y/Z(MAXSTR,NZONE), ELSTOR/POROS only through SNODI/PNODI(N)
+ *   VGN, VGRMC, VGPSAT(N) node vG parameters            (tpnodi.f)
+ *   KZNOD(N)                                  (cathy_main.f:2687-2694)
+ * with CATHY's node numbering (node = layer*NNOD + surface id).
+ * LUMP as in input/parm.  ORDERING: CATHY_GPU_ORDER_*.
+ * FP64 state; IC(0) factor stored in FP32.
+ * ------------------------------------------------------------------ */
+int cathy_gpu_flow3d_create(int64_t *handle, const int *n, const int *nt,
+                            const int *ntri, const int *nnod,
+                            const int *nstr, const int *nzone,
+                            const int *maxstr, const int *tetra,
+                            const double *bi, const double *ci,
+                            const double *di, const double *volu,
+                            const int *ivol, const double *volnod,
+                            const double *arenod, const double *permx,
+                            const double *permy, const double *permz,
+                            const double *snodi, const double *pnodi,
+                            const double *vgn, const double *vgrmc,
+                            const double *vgpsat, const double *kznod,
+                            const int *lump, const int *ordering);
+
+int cathy_gpu_flow3d_destroy(int64_t *handle);
+
+/* input/parm and soil-file scalars (datin.f:80-122, :421).
+ * iprm: [0] IOPT [1] L2NORM [2] ITUNS [3] ITUNS1 [4] ITUNS2 [5] ITMXCG
+ *       [6] IETO [7] free drainage at the base (NNEU<0) [8] HSPATM
+ * dprm: [0] TETAF [1] TOLUNS [2] TOLSWI [3] ERNLMX [4] TOLCG [5] DELTAT
+ *       [6] DTMIN [7] DTMAX [8] TMAX [9] DTMAGA [10] DTMAGM [11] DTREDS
+ *       [12] DTREDM [13] PMIN [14] PONDH_MIN [15] SCF */
+int cathy_gpu_flow3d_set_params(int64_t *handle, const int *iprm,
+                                const double *dprm);
+
+/* Atmospheric series (the atmbc file read by ATMONE/ATMNXT): NATM
+ * records at ATMTIM(NATM), values ATMVAL(NATM) if HSPATM=1 else
+ * ATMVAL(NNOD,NATM) (m/s, +ve rain). */
+int cathy_gpu_flow3d_set_forcing(int64_t *handle, const int *natm,
+                                 const double *atmtim, const double *atmval);
+
+/* Initial condition: PTIMEP(N) (INDP / ICV*) then the INITAL sequence
+ * (INIT1, ATMONE, MBINIT, CHVELO, STORCAL, NEUMANN) on the device. */
+int cathy_gpu_flow3d_init(int64_t *handle, const double *ptimep);
+
+/* One accepted time step of cathy_main.f's loop (back-steps included)
+ * with FLOW3D's Picard iterations on the device.  stats(32) receives:
+ *  [0] NSTEP [1] DELTAT [2] TIME [3] ITER [4] NITERT [5] back-steps
+ *  [6] STORE1 [7] STORE2 [8] DSTORE [9] CDSTOR [10] VIN [11] CVIN
+ *  [12] VOUT [13] CVOUT [14] VIN+VOUT [15] VTOT [16] ERRAS [17] ERREL
+ *  [18] CERRAS [19] CAERAS [20] PINF [21] ITRTOT [22] ITLIN [23] KLSFAI
+ *  [24] PL2 [25] FINF [26] FL2 [27] IKMAX (1-based) [28..31] reserved
+ * Returns 0 while the run continues, 1 at TMAX, 2 on NOBACK, <0 on error. */
+int cathy_gpu_flow3d_advance(int64_t *handle, double *stats);
+
+/* State read-back / overwrite (CATHY node order): PNEW, SW, CKRW (N),
+ * IFATM (NNOD).  Any pointer may be NULL. */
+int cathy_gpu_flow3d_get_state(int64_t *handle, double *pnew, double *sw,
+                               double *ckrw, int *ifatm);
+int cathy_gpu_flow3d_set_pnew(int64_t *handle, const double *pnew);
+
+/* Counters of the last advance: Picard its, PCG its, IC(0) breakdown
+ * fallbacks, kernel time of the linear solver (ms), total device ms. */
+int cathy_gpu_flow3d_counters(int64_t *handle, double *out6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1552 @@
+/*
+ * cathy_oracle.c -- CPU restatement of the CATHY subsurface Picard step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker of the HIP path; see the header).
+ * Each function cites the reference routine it restates as SRC/<file>:<line>
+ * where SRC = pyCATHY/tests/weil_exemple/my_cathy_prj/src.  Built with
+ * -ffp-contract=off so the arithmetic order matches the reference's.
+ */
+#include "cathy_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define VGF(vg, n, f, i) ((vg)[(size_t)(f) * (size_t)(n) + (size_t)(i)])
+
+/* ===================================================================== */
+/*                                 mesh                                  */
+/* ===================================================================== */
+
+void orc_riord(int nelt, int *eltcon, int nod) {
+  /* SRC/riord.f: exchange sort of the first nod ids of each record. */
+  for (int e = 0; e < nelt; ++e) {
+    int *c = eltcon + (size_t)e * (nod + 1);
+    for (int k = 0; k < nod - 1; ++k)
+      for (int j = k + 1; j < nod; ++j)
+        if (c[k] > c[j]) {
+          int t = c[k];
+          c[k] = c[j];
+          c[j] = t;
+        }
+  }
+}
+
+int orc_triangoli(int nrow, int ncol, double dx, double dy, double west,
+                  double south, const double *dem, const int *zone,
+                  double factor, int *nnod_out, int *ntri_out, double *x,
+                  double *y, double *z, int *triang) {
+  /* SRC/triangoli.f:25-150 with DOSTEP=1: a node exists at each corner of
+   * an active cell; nodes numbered row by row (north first); each cell
+   * (i,j) gives triangles (i,j)-(i+1,j)-(i+1,j+1) and
+   * (i,j)-(i+1,j+1)-(i,j+1). */
+  int nr1 = nrow + 1, nc1 = ncol + 1;
+  int *nodi = (int *)calloc((size_t)nr1 * nc1, sizeof(int));
+  double *eltria = (double *)malloc(sizeof(double) * 2 * (size_t)nrow * ncol);
+  if (!nodi || !eltria) return -1;
+  for (int i = 0; i < nrow; ++i)
+    for (int j = 0; j < ncol; ++j)
+      if (dem[(size_t)i * ncol + j] != 0.0) {
+        nodi[(size_t)i * nc1 + j] = 1;
+        nodi[(size_t)(i + 1) * nc1 + j] = 1;
+        nodi[(size_t)(i + 1) * nc1 + j + 1] = 1;
+        nodi[(size_t)i * nc1 + j + 1] = 1;
+      }
+  int inod = 0;
+  for (int i = 0; i < nr1; ++i)
+    for (int j = 0; j < nc1; ++j)
+      if (nodi[(size_t)i * nc1 + j] == 1) {
+        x[inod] = west + j * dx;
+        y[inod] = south + (nrow - i) * dy; /* (nrow-i+1) with 1-based i */
+        nodi[(size_t)i * nc1 + j] = inod + 1; /* keep 1-based marker */
+        ++inod;
+      }
+  int itria = 0;
+  for (int i = 0; i < nrow; ++i)
+    for (int j = 0; j < ncol; ++j) {
+      double d = dem[(size_t)i * ncol + j];
+      if (d == 0.0) continue;
+      double el = d * factor / 1.0; /* one non-null cell per triangle */
+      int a = nodi[(size_t)i * nc1 + j] - 1;
+      int b = nodi[(size_t)(i + 1) * nc1 + j] - 1;
+      int c = nodi[(size_t)(i + 1) * nc1 + j + 1] - 1;
+      int dd = nodi[(size_t)i * nc1 + j + 1] - 1;
+      int zn = zone ? zone[(size_t)i * ncol + j] : 1;
+      int *t = triang + (size_t)itria * 4;
+      t[0] = a; t[1] = b; t[2] = c; t[3] = zn;
+      eltria[itria++] = el;
+      t = triang + (size_t)itria * 4;
+      t[0] = a; t[1] = c; t[2] = dd; t[3] = zn;
+      eltria[itria++] = el;
+    }
+  /* SRC/tpnodi2d.f: surface z = mean DEM value of incident triangles. */
+  int *tp2d = (int *)calloc((size_t)inod, sizeof(int));
+  for (int k = 0; k < inod; ++k) z[k] = 0.0;
+  for (int k = 0; k < itria; ++k)
+    for (int ii = 0; ii < 3; ++ii) {
+      int nd = triang[(size_t)k * 4 + ii];
+      z[nd] = z[nd] + eltria[k];
+      tp2d[nd] += 1;
+    }
+  int bad = 0;
+  for (int k = 0; k < inod; ++k) {
+    if (tp2d[k] == 0) bad = 1;
+    else z[k] = z[k] / tp2d[k];
+  }
+  /* SRC/grdsys.f: RIORD(NTRI,TRIANG,3) before GEN3D. */
+  orc_riord(itria, triang, 3);
+  *nnod_out = inod;
+  *ntri_out = itria;
+  free(tp2d);
+  free(nodi);
+  free(eltria);
+  return bad ? -2 : 0;
+}
+
+void orc_gen3d(int nnod, int nstr, int ntri, int ivert, const int *triang,
+               const double *zratio, double base, const double *depth,
+               double *x, double *y, double *z, int *tetra, int sort_tetra) {
+  /* SRC/gen3d.f:24-45: three tets per prism (layer J between node planes
+   * J-1 and J). */
+  for (int J = 1; J <= nstr; ++J)
+    for (int I = 0; I < ntri; ++I) {
+      const int *tr = triang + (size_t)I * 4;
+      size_t nt0 = 3 * ((size_t)(J - 1) * ntri + I);
+      int *t1 = tetra + nt0 * 5, *t2 = t1 + 5, *t3 = t2 + 5;
+      int lo = (J - 1) * nnod, hi = J * nnod;
+      t1[0] = lo + tr[0]; t1[1] = lo + tr[1]; t1[2] = lo + tr[2];
+      t1[3] = hi + tr[0]; t1[4] = tr[3];
+      t2[0] = t1[3]; t2[1] = hi + tr[1]; t2[2] = hi + tr[2];
+      t2[3] = t1[2]; t2[4] = tr[3];
+      t3[0] = t1[1]; t3[1] = t1[2]; t3[2] = t2[1];
+      t3[3] = t1[3]; t3[4] = tr[3];
+    }
+  /* SRC/gen3d.f:53-75: vertical coordinates. */
+  double zmin = 1.7e100;
+  for (int i = 0; i < nnod; ++i)
+    if (z[i] < zmin) zmin = z[i];
+  for (int i = 0; i < nnod; ++i) {
+    double zthick = (z[i] - zmin) + base;
+    double zrsum = 0.0;
+    for (int J = 1; J <= nstr; ++J) {
+      size_t kk = (size_t)J * nnod + i;
+      x[kk] = x[i];
+      y[kk] = y[i];
+      zrsum = zrsum + zratio[J - 1];
+      if (ivert == 0) z[kk] = z[i] - zrsum * base;
+      else if (ivert == 1) z[kk] = z[i] - zrsum * zthick;
+      else if (ivert == 2) z[kk] = zmin - zrsum * base;
+      else if (ivert == 3) z[kk] = z[i] - zrsum * depth[i];
+      else if (ivert == 4) {
+        z[kk] = z[i] - zrsum * base;
+        if (J == nstr) z[kk] = zmin - base;
+      }
+    }
+  }
+  if (sort_tetra) orc_riord(3 * ntri * nstr, tetra, 4);
+}
+
+/* IP4 of SRC/cathy_main.f:2491 (column-major DATA), 0-based. */
+static const int IP4[4][4] = {
+    /* IP4(r, nn) stored as IP4c[nn][r] */
+    {0, 1, 2, 3}, {1, 2, 3, 0}, {2, 3, 0, 1}, {3, 0, 1, 2}};
+
+/* Build the 3x3 cyclic table of SRC/volums.f:22-33 / SRC/basis6.f:26-37
+ * for local vertex nn: column 0 = the other three vertices, columns 1 and 2
+ * their cyclic shifts. */
+static void ip3_for(int nn, int ip3[3][3]) {
+  for (int ii = 1; ii < 4; ++ii) ip3[ii - 1][0] = IP4[nn][ii];
+  for (int lc = 1; lc < 3; ++lc)
+    for (int lr = 0; lr < 3; ++lr)
+      ip3[lr][lc] = (lr == 2) ? ip3[0][lc - 1] : ip3[lr + 1][lc - 1];
+}
+
+int orc_volbas(int n, int nt, const int *tetra, const double *x,
+               const double *y, const double *z, double *ai, double *bi,
+               double *ci, double *di, double *volnod, double *volu,
+               double *volur, int *ivol) {
+  static const double amen4[4] = {-1.0, 1.0, -1.0, 1.0};
+  static const double amen5[5] = {-1.0, 1.0, -1.0, 1.0, -1.0};
+  for (int i = 0; i < n; ++i) volnod[i] = 0.0;
+  int bad = 0;
+  for (int e = 0; e < nt; ++e) {
+    const int *t = tetra + (size_t)e * 5;
+    int ip3[3][3];
+    /* SRC/volums.f: signed volume by cofactor expansion. */
+    double vol = 0.0;
+    for (int nn = 0; nn < 4; ++nn) {
+      ip3_for(nn, ip3);
+      double a2 = 0.0, a3 = 0.0;
+      for (int ii = 0; ii < 3; ++ii) {
+        int I = t[ip3[ii][0]], J = t[ip3[ii][1]], M = t[ip3[ii][2]];
+        a3 = y[I] * z[J] + a3;
+        a2 = y[I] * z[M] + a2;
+      }
+      vol = vol + x[t[nn]] * amen4[nn] * (a3 - a2) / 6.0;
+    }
+    /* SRC/basis6.f: cofactors / 6. */
+    for (int nn = 0; nn < 4; ++nn) {
+      ip3_for(nn, ip3);
+      double a3 = 0.0, a2;
+      for (int ii = 0; ii < 3; ++ii) {
+        int I = t[ip3[ii][0]], J = t[ip3[ii][1]], M = t[ip3[ii][2]];
+        a3 = a3 + x[M] * (y[I] * z[J] - z[I] * y[J]);
+      }
+      ai[(size_t)e * 4 + nn] = amen5[nn + 1] * a3 / 6.0;
+      a2 = 0.0; a3 = 0.0;
+      for (int ii = 0; ii < 3; ++ii) {
+        int I = t[ip3[ii][0]], J = t[ip3[ii][1]], M = t[ip3[ii][2]];
+        a3 = y[I] * z[J] + a3;
+        a2 = y[I] * z[M] + a2;
+      }
+      bi[(size_t)e * 4 + nn] = amen5[nn] * (a3 - a2) / 6.0;
+      a2 = 0.0; a3 = 0.0;
+      for (int ii = 0; ii < 3; ++ii) {
+        int I = t[ip3[ii][0]], J = t[ip3[ii][1]], M = t[ip3[ii][2]];
+        a3 = x[I] * z[J] + a3;
+        a2 = x[I] * z[M] + a2;
+      }
+      ci[(size_t)e * 4 + nn] = amen5[nn + 1] * (a3 - a2) / 6.0;
+      a2 = 0.0; a3 = 0.0;
+      for (int ii = 0; ii < 3; ++ii) {
+        int I = t[ip3[ii][0]], J = t[ip3[ii][1]], M = t[ip3[ii][2]];
+        a3 = x[I] * y[J] + a3;
+        a2 = x[I] * y[M] + a2;
+      }
+      di[(size_t)e * 4 + nn] = amen5[nn] * (a3 - a2) / 6.0;
+    }
+    if (vol == 0.0) bad = 1;
+    double vol4 = fabs(vol) * 0.25;
+    for (int k = 0; k < 4; ++k) volnod[t[k]] = volnod[t[k]] + vol4;
+    ivol[e] = 1;
+    if (vol < 0.0) {
+      ivol[e] = -1;
+      vol = -vol;
+    }
+    volu[e] = vol;
+    volur[e] = 1.0 / vol;
+  }
+  return bad ? -1 : 0;
+}
+
+int orc_area2d(int nnod, int ntri, const int *triang, const double *x,
+               const double *y, double *arenod) {
+  /* SRC/area2d.f + SRC/areas.f with IP3 = {1,2,3; 2,3,1; 3,1,2}. */
+  static const int ip3[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+  const double r3 = 1.0 / 3.0;
+  for (int i = 0; i < nnod; ++i) arenod[i] = 0.0;
+  int bad = 0;
+  for (int e = 0; e < ntri; ++e) {
+    const int *t = triang + (size_t)e * 4;
+    double a2 = 0.0, a3 = 0.0;
+    for (int ii = 0; ii < 3; ++ii) {
+      int I = t[ip3[ii][0]], J = t[ip3[ii][1]], M = t[ip3[ii][2]];
+      a3 = x[I] * y[J] + a3;
+      a2 = x[I] * y[M] + a2;
+    }
+    double are = 0.5 * (a3 - a2);
+    if (are == 0.0) bad = 1;
+    double are3 = fabs(are) * r3;
+    for (int k = 0; k < 3; ++k) arenod[t[k]] = arenod[t[k]] + are3;
+  }
+  return bad ? -1 : 0;
+}
+
+/* Row-wise sorted insertion of tet connectivity.  upper=1 reproduces the
+ * STRPIC pattern (diag first, then columns > row), upper=0 the STRNEW
+ * full pattern (all columns sorted, diag included). */
+static int build_pattern(int n, int nt, const int *tetra, int *topol, int *ja,
+                         int cap, int upper) {
+  const int W = 32; /* slots per row while building (N1) */
+  int *slot = (int *)malloc(sizeof(int) * (size_t)n * W);
+  int *cnt = (int *)calloc((size_t)n, sizeof(int));
+  if (!slot || !cnt) return -1;
+  if (upper)
+    for (int r = 0; r < n; ++r) {
+      slot[(size_t)r * W] = r;
+      cnt[r] = 1;
+    }
+  int ok = 1;
+  for (int e = 0; e < nt && ok; ++e) {
+    int v[4];
+    for (int k = 0; k < 4; ++k) v[k] = tetra[(size_t)e * 5 + k];
+    /* ascending order of the element's nodes (STRPIC/STRNEW selection) */
+    for (int k = 0; k < 3; ++k)
+      for (int j = k + 1; j < 4; ++j)
+        if (v[k] > v[j]) { int t = v[k]; v[k] = v[j]; v[j] = t; }
+    for (int a = 0; a < 4; ++a)
+      for (int b = upper ? a + 1 : 0; b < 4; ++b) {
+        int r = v[a], c = v[b];
+        int *s = slot + (size_t)r * W;
+        int m = cnt[r], pos = upper ? 1 : 0;
+        while (pos < m && s[pos] < c) ++pos;
+        if (pos < m && s[pos] == c) continue;
+        if (m >= W) { ok = 0; break; }
+        memmove(s + pos + 1, s + pos, sizeof(int) * (size_t)(m - pos));
+        s[pos] = c;
+        cnt[r] = m + 1;
+      }
+  }
+  int nterm = -1;
+  if (ok) {
+    size_t tot = 0;
+    for (int r = 0; r < n; ++r) tot += (size_t)cnt[r];
+    if (tot <= (size_t)cap) {
+      size_t m = 0;
+      for (int r = 0; r < n; ++r) {
+        topol[r] = (int)m;
+        for (int k = 0; k < cnt[r]; ++k) ja[m++] = slot[(size_t)r * W + k];
+      }
+      topol[n] = (int)m;
+      nterm = (int)m;
+    }
+  }
+  free(slot);
+  free(cnt);
+  return nterm;
+}
+
+int orc_strpic(int n, int nt, const int *tetra, int *topol, int *ja,
+               int cap) {
+  /* SRC/strpic.f:8-104. */
+  return build_pattern(n, nt, tetra, topol, ja, cap, 1);
+}
+
+int orc_strnew(int n, int nt, const int *tetra, int *topol, int *ja,
+               int cap) {
+  /* SRC/strnew.f. */
+  return build_pattern(n, nt, tetra, topol, ja, cap, 0);
+}
+
+static int find_in_row(const int *topol, const int *ja, int r, int c) {
+  for (int m = topol[r]; m < topol[r + 1]; ++m)
+    if (ja[m] == c) return m;
+  return -1;
+}
+
+int orc_tetpic(int nt, const int *tetra, const int *topol, const int *ja,
+               int *tetja) {
+  /* SRC/tetpic.f:19-30 (requires sorted TETRA). */
+  int bad = 0;
+  for (int e = 0; e < nt; ++e)
+    for (int k = 0; k < 4; ++k)
+      for (int l = 0; l < 4; ++l) {
+        int v = -1;
+        if (l >= k) {
+          v = find_in_row(topol, ja, tetra[(size_t)e * 5 + k],
+                          tetra[(size_t)e * 5 + l]);
+          if (v < 0) bad = 1;
+        }
+        tetja[(size_t)e * 16 + k * 4 + l] = v;
+      }
+  return bad ? -1 : 0;
+}
+
+int orc_tetnew(int nt, const int *tetra, const int *topol, const int *ja,
+               int *tetja) {
+  /* SRC/tetnew.f. */
+  int bad = 0;
+  for (int e = 0; e < nt; ++e)
+    for (int k = 0; k < 4; ++k)
+      for (int l = 0; l < 4; ++l) {
+        int v = find_in_row(topol, ja, tetra[(size_t)e * 5 + k],
+                            tetra[(size_t)e * 5 + l]);
+        if (v < 0) bad = 1;
+        tetja[(size_t)e * 16 + k * 4 + l] = v;
+      }
+  return bad ? -1 : 0;
+}
+
+void orc_locmas(double *lmass, int lump) {
+  /* SRC/locmas.f:13-48. */
+  const double r4 = 1.0 / 4.0, r10 = 1.0 / 10.0, r20 = 1.0 / 20.0;
+  for (int k = 0; k < 4; ++k)
+    for (int i = 0; i < 4; ++i) {
+      double v;
+      if (lump == 0) v = (k == i) ? r10 : r20;
+      else v = (k == i) ? r4 : 0.0;
+      lmass[k * 4 + i] = v;
+    }
+}
+
+/* ===================================================================== */
+/*                              soil params                              */
+/* ===================================================================== */
+
+int orc_tpnodi(int n, int nt, int ntri, const int *tetra, int nzone,
+               const double *poros, const double *elstor,
+               const double *vgncell, const double *vgrmccell,
+               const double *vgpsatcell, double *pnodi, double *snodi,
+               double *vg, double *pel, int *tp) {
+  /* SRC/tpnodi.f:28-70 with INIT1 zeroing (SRC/init1.f). */
+  for (int i = 0; i < n; ++i) {
+    pnodi[i] = 0.0;
+    snodi[i] = 0.0;
+    tp[i] = 0;
+    VGF(vg, n, VG_N, i) = 0.0;
+    VGF(vg, n, VG_RMC, i) = 0.0;
+    VGF(vg, n, VG_PSAT, i) = 0.0;
+  }
+  for (int e = 0; e < nt; ++e) {
+    int mt = tetra[(size_t)e * 5 + 4] - 1;
+    int istr = e / (3 * ntri);
+    size_t ix = (size_t)istr * nzone + mt;
+    for (int ii = 0; ii < 4; ++ii) {
+      int nd = tetra[(size_t)e * 5 + ii];
+      pnodi[nd] = pnodi[nd] + poros[ix];
+      snodi[nd] = snodi[nd] + elstor[ix];
+      VGF(vg, n, VG_N, nd) = VGF(vg, n, VG_N, nd) + vgncell[ix];
+      VGF(vg, n, VG_RMC, nd) = VGF(vg, n, VG_RMC, nd) + vgrmccell[ix];
+      VGF(vg, n, VG_PSAT, nd) = VGF(vg, n, VG_PSAT, nd) + vgpsatcell[ix];
+      tp[nd] += 1;
+    }
+  }
+  int bad = 0;
+  for (int k = 0; k < n; ++k) {
+    if (tp[k] == 0) { bad = 1; continue; }
+    pnodi[k] = pnodi[k] / tp[k];
+    snodi[k] = snodi[k] / tp[k];
+    VGF(vg, n, VG_N, k) = VGF(vg, n, VG_N, k) / tp[k];
+    VGF(vg, n, VG_PSAT, k) = VGF(vg, n, VG_PSAT, k) / tp[k];
+    VGF(vg, n, VG_RMC, k) = VGF(vg, n, VG_RMC, k) / tp[k];
+  }
+  for (int e = 0; e < nt; ++e) {
+    int mt = tetra[(size_t)e * 5 + 4] - 1;
+    int istr = e / (3 * ntri);
+    pel[e] = poros[(size_t)istr * nzone + mt];
+  }
+  return bad ? -1 : 0;
+}
+
+void orc_chparm(int n, const double *pnodi, double *vg) {
+  /* SRC/chparm.f:24-41 (IVGHU=0). */
+  for (int i = 0; i < n; ++i) {
+    double vgn = VGF(vg, n, VG_N, i);
+    double vgm = (vgn - 1.0) / vgn;
+    VGF(vg, n, VG_M, i) = vgm;
+    VGF(vg, n, VG_N1, i) = vgn - 1.0;
+    VGF(vg, n, VG_NR, i) = 1.0 / vgn;
+    VGF(vg, n, VG_MM1, i) = vgm - 1.0;
+    VGF(vg, n, VG_PSN, i) = pow(fabs(VGF(vg, n, VG_PSAT, i)), vgn);
+    VGF(vg, n, VG_MR, i) = 1.0 / vgm;
+    VGF(vg, n, VG_M52, i) = 2.5 * vgm;
+  }
+  for (int i = 0; i < n; ++i)
+    VGF(vg, n, VG_PNOT, i) = (pnodi[i] - VGF(vg, n, VG_RMC, i)) / pnodi[i];
+}
+
+void orc_eltnod(int n, int nt, const int *tp, const int *tetra,
+                const double *velt, double *vnod) {
+  /* SRC/eltnod.f. */
+  for (int i = 0; i < n; ++i) vnod[i] = 0.0;
+  for (int e = 0; e < nt; ++e)
+    for (int ii = 0; ii < 4; ++ii) {
+      int nd = tetra[(size_t)e * 5 + ii];
+      vnod[nd] = vnod[nd] + velt[e];
+    }
+  for (int i = 0; i < n; ++i) vnod[i] = vnod[i] / tp[i];
+}
+
+void orc_nodelt(int nt, const int *tetra, const double *vnod, double *velt) {
+  /* SRC/nodelt.f:9-29. */
+  const double r4 = 1.0 / 4.0;
+  for (int e = 0; e < nt; ++e) {
+    double v = 0.0;
+    for (int j = 0; j < 4; ++j) v = v + vnod[tetra[(size_t)e * 5 + j]];
+    velt[e] = v * r4;
+  }
+}
+
+/* ===================================================================== */
+/*                          van Genuchten (a2)                           */
+/* ===================================================================== */
+
+double orc_fvgse(double psi, const double *vg, int n, int i) {
+  /* SRC/fvgse.f:18-23. */
+  if (psi < -1.0e-14) {
+    double beta = pow(fabs(psi / VGF(vg, n, VG_PSAT, i)), VGF(vg, n, VG_N, i));
+    return pow(fabs(1.0 / (beta + 1.0)), VGF(vg, n, VG_M, i));
+  }
+  return 1.0;
+}
+
+double orc_fvgkr(double psi, double se, const double *vg, int n, int i) {
+  /* SRC/fvgkr.f:18-24. */
+  if (psi < -1.0e-14) {
+    double omega = pow(fabs(se), VGF(vg, n, VG_MR, i));
+    double v1 = 1.0 - pow(fabs(1.0 - omega), VGF(vg, n, VG_M, i));
+    return sqrt(se) * v1 * v1;
+  }
+  return 1.0;
+}
+
+double orc_fvgdse(double psi, const double *vg, int n, int i) {
+  /* SRC/fvgdse.f:18-26. */
+  if (psi < -1.0e-14) {
+    double beta = pow(fabs(psi / VGF(vg, n, VG_PSAT, i)), VGF(vg, n, VG_N, i));
+    double b1 = beta + 1.0;
+    double b1r = 1.0 / b1;
+    double vgn1 = VGF(vg, n, VG_N1, i);
+    return vgn1 * (pow(fabs(psi), vgn1) / VGF(vg, n, VG_PSN, i)) *
+           pow(fabs(b1), VGF(vg, n, VG_NR, i)) * b1r * b1r;
+  }
+  return 0.0;
+}
+
+void orc_chpic0(int n, const double *ptnew, const double *pnew,
+                const double *ptimep, double *swnew, double *swtimep,
+                const double *snodi, const double *pnodi, const double *vg,
+                double *sw, double *ckrw, double *etai, double *et1,
+                double *et2) {
+  /* SRC/chpic0.f:23-36. */
+  for (int i = 0; i < n; ++i) {
+    double pnot = VGF(vg, n, VG_PNOT, i), rmc = VGF(vg, n, VG_RMC, i);
+    swnew[i] = pnot * orc_fvgse(pnew[i], vg, n, i) + rmc / pnodi[i];
+    swtimep[i] = pnot * orc_fvgse(ptimep[i], vg, n, i) + rmc / pnodi[i];
+  }
+  for (int i = 0; i < n; ++i) {
+    double pnot = VGF(vg, n, VG_PNOT, i), rmc = VGF(vg, n, VG_RMC, i);
+    double psi = ptnew[i];
+    double se = orc_fvgse(psi, vg, n, i);
+    sw[i] = pnot * se + rmc / pnodi[i];
+    etai[i] = sw[i] * snodi[i] + pnodi[i] * pnot * orc_fvgdse(psi, vg, n, i);
+    et1[i] = sw[i] * snodi[i];
+    et2[i] = pnot * orc_fvgdse(psi, vg, n, i);
+    ckrw[i] = orc_fvgkr(psi, se, vg, n, i);
+  }
+}
+
+void orc_chvelo(int n, const double *pnew, const double *pnodi,
+                const double *vg, double *sw, double *ckrw) {
+  /* SRC/chvelo.f (IVGHU=0 branch). */
+  for (int i = 0; i < n; ++i) {
+    double psi = pnew[i];
+    double se = orc_fvgse(psi, vg, n, i);
+    sw[i] = VGF(vg, n, VG_PNOT, i) * se + VGF(vg, n, VG_RMC, i) / pnodi[i];
+    ckrw[i] = orc_fvgkr(psi, se, vg, n, i);
+  }
+}
+
+void orc_chmass(int n, const double *pnew, const double *snodi,
+                const double *pnodi, const double *vg, double *etai) {
+  /* SRC/chmass.f (IVGHU=0 branch). */
+  for (int i = 0; i < n; ++i) {
+    double psi = pnew[i];
+    double se = orc_fvgse(psi, vg, n, i);
+    double swi = VGF(vg, n, VG_PNOT, i) * se + VGF(vg, n, VG_RMC, i) / pnodi[i];
+    etai[i] = swi * snodi[i] +
+              pnodi[i] * VGF(vg, n, VG_PNOT, i) * orc_fvgdse(psi, vg, n, i);
+  }
+}
+
+/* ===================================================================== */
+/*                           assembly (a3-a7)                            */
+/* ===================================================================== */
+
+void orc_asspic(int nt, int ntri, int nzone, const int *tetra,
+                const int *tetja, const double *lmass, double *coef1,
+                double *coef2, double *coef4, const double *et1e,
+                const double *pel, const double *permx, const double *permy,
+                const double *permz, const double *ckrwe,
+                const double *volu, const double *volur, const double *bi,
+                const double *ci, const double *di) {
+  /* SRC/asspic.f:26-51. */
+  for (int e = 0; e < nt; ++e) {
+    int istr = e / (3 * ntri);
+    int mt = tetra[(size_t)e * 5 + 4] - 1;
+    size_t ix = (size_t)istr * nzone + mt;
+    double vk = volur[e] * ckrwe[e];
+    double ve1 = volu[e] * et1e[e];
+    double ve = volu[e] * pel[e];
+    double pvk1 = permx[ix] * vk, pvk2 = permy[ix] * vk, pvk3 = permz[ix] * vk;
+    const double *b = bi + (size_t)e * 4, *c = ci + (size_t)e * 4,
+                 *d = di + (size_t)e * 4;
+    for (int k = 0; k < 4; ++k) {
+      double pvkb = pvk1 * b[k], pvkc = pvk2 * c[k], pvkd = pvk3 * d[k];
+      for (int l = k; l < 4; ++l) {
+        int ind = tetja[(size_t)e * 16 + k * 4 + l];
+        coef1[ind] = coef1[ind] + pvkb * b[l] + pvkc * c[l] + pvkd * d[l];
+        coef2[ind] = coef2[ind] + ve1 * lmass[k * 4 + l];
+        coef4[ind] = coef4[ind] + ve * lmass[k * 4 + l];
+      }
+    }
+  }
+}
+
+void orc_rhspic(int n, const int *ja, const int *topol, double deltat,
+                const double *ptnew, const double *pnew,
+                const double *ptimep, const double *swnew,
+                const double *swtimep, const double *coef1,
+                const double *coef2, const double *coef4, double *tnoti) {
+  /* SRC/rhspic.f:24-37. */
+  for (int k = 0; k < n; ++k) tnoti[k] = 0.0;
+  double rdt = 1.0 / deltat;
+  for (int k = 0; k < n; ++k) {
+    int i = topol[k], j = topol[k + 1];
+    for (int m = i; m < j; ++m) {
+      int c = ja[m];
+      double coef = coef2[m] * rdt, coeff = coef4[m] * rdt;
+      tnoti[k] = tnoti[k] - coef1[m] * ptnew[c] - coef * (pnew[c] - ptimep[c]) -
+                 coeff * (swnew[c] - swtimep[c]);
+      if (m != i)
+        tnoti[c] = tnoti[c] - coef1[m] * ptnew[k] -
+                   coef * (pnew[k] - ptimep[k]) -
+                   coeff * (swnew[k] - swtimep[k]);
+    }
+  }
+}
+
+void orc_cfmatp(int n, int nterm, double tetaf, double deltat, double *coef1,
+                const double *coef2, const double *coef4, double *coef5,
+                const int *topol, const double *et2) {
+  /* SRC/cfmatp.f:21-26. */
+  double rdt = 1.0 / deltat;
+  for (int k = 0; k < n; ++k) coef5[topol[k]] = coef4[topol[k]] * et2[k];
+  for (int k = 0; k < nterm; ++k)
+    coef1[k] = tetaf * coef1[k] + coef2[k] * rdt + coef5[k] * rdt;
+}
+
+void orc_rhsgrv(int nt, int ntri, int nzone, const int *tetra,
+                double *tnoti, const double *di, const double *permz,
+                const int *ivol, const double *ckrwe) {
+  /* SRC/rhsgrv.f:22-31. */
+  for (int e = 0; e < nt; ++e) {
+    int istr = e / (3 * ntri);
+    int mt = tetra[(size_t)e * 5 + 4] - 1;
+    double t3 = ckrwe[e] * permz[(size_t)istr * nzone + mt];
+    for (int in = 0; in < 4; ++in) {
+      int nd = tetra[(size_t)e * 5 + in];
+      double t1 = t3 * di[(size_t)e * 4 + in] * (double)ivol[e];
+      tnoti[nd] = tnoti[nd] - t1;
+    }
+  }
+}
+
+int orc_bcpic(int np, int nq, const int *contp, const int *contq,
+              double *lhsp, const double *q, int nnod, int n,
+              double *lhsatm, const int *ifatm, const double *atmact,
+              const double *atmold, const int *topol, double *coef1,
+              double *tnoti, double tetaf, double rmax, int *noddir,
+              const double *qtranie) {
+  /* SRC/bcpic.f:34-87 (NSF=0). */
+  for (int k = 0; k < np; ++k) {
+    int j = contp[k], ind = topol[j];
+    tnoti[j] = 0.0;
+    lhsp[k] = coef1[ind];
+    coef1[ind] = 1.0e-9 * rmax;
+  }
+  int numdir = np;
+  for (int k = 0; k < nnod; ++k)
+    if (ifatm[k] == 1 || ifatm[k] == 2) {
+      noddir[numdir++] = k;
+      int ind = topol[k];
+      tnoti[k] = 0.0;
+      lhsatm[k] = coef1[ind];
+      coef1[ind] = 1.0e-9 * rmax;
+    }
+  for (int k = 0; k < nq; ++k) tnoti[contq[k]] = tnoti[contq[k]] + q[k];
+  for (int k = 0; k < nnod; ++k)
+    if (ifatm[k] == 0)
+      tnoti[k] = tnoti[k] + (tetaf * atmact[k] + (1.0 - tetaf) * atmold[k]);
+  for (int k = 0; k < n; ++k) tnoti[k] = tnoti[k] - qtranie[k];
+  return numdir;
+}
+
+void orc_shlpic(int np, const int *topol, const int *contp,
+                const double *presc, const double *lhsp, double *coef1,
+                double *pnew, const double *pold, int nnod, const int *ifatm,
+                const double *lhsatm) {
+  /* SRC/shlpic.f:30-58 (NSF=0). */
+  for (int i = 0; i < np; ++i) {
+    int j = contp[i];
+    pnew[j] = presc[i];
+    coef1[topol[j]] = lhsp[i];
+  }
+  for (int i = 0; i < nnod; ++i)
+    if (ifatm[i] == 1 || ifatm[i] == 2) {
+      pnew[i] = pold[i];
+      coef1[topol[i]] = lhsatm[i];
+    }
+}
+
+/* ===================================================================== */
+/*                            IC(0)-PCG (a8-a11)                         */
+/* ===================================================================== */
+
+int orc_incldp(const int *ja, const int *topol, const double *coefr,
+               double *lriga, int n, int nterm) {
+  /* SRC/solscal-extended.f:2152-2190 (right-looking row IC(0)). */
+  int nfall = 0;
+  for (int k = 0; k < nterm; ++k) lriga[k] = 0.0;
+  for (int kk = 0; kk < n; ++kk) {
+    int k = topol[kk];
+    double a = coefr[k] - lriga[k];
+    if (a <= 0.0) {
+      /* :2157-2160 breakdown fallback: previous diagonal squared */
+      a = (kk > 0) ? lriga[topol[kk - 1]] * lriga[topol[kk - 1]] : 1.0;
+      ++nfall;
+    }
+    lriga[k] = sqrt(a);
+    if (kk == n - 1) continue;
+    int i = topol[kk] + 1, j = topol[kk + 1] - 1;
+    if (j < i) continue;
+    for (int k1 = i; k1 <= j; ++k1)
+      lriga[k1] = (coefr[k1] - lriga[k1]) / lriga[k];
+    for (int k2 = i; k2 <= j; ++k2) {
+      int row = ja[k2];
+      int j1 = topol[row];
+      lriga[j1] = lriga[j1] + lriga[k2] * lriga[k2];
+      /* merge the rest of row kk (columns > ja[k2]) into row ja[k2] */
+      int i1 = k2 + 1, jj = j1 + 1, end = topol[row + 1];
+      while (i1 <= j && jj < end) {
+        if (ja[jj] < ja[i1]) ++jj;
+        else if (ja[jj] > ja[i1]) ++i1;
+        else {
+          lriga[jj] = lriga[jj] + lriga[k2] * lriga[i1];
+          ++i1;
+        }
+      }
+    }
+  }
+  return nfall;
+}
+
+void orc_proddp(double *x, const double *b, int n, const int *topol,
+                const int *ja, const double *coef2) {
+  /* SRC/solscal-extended.f:4175-4203: U^T y = b (column sweep), U x = y. */
+  for (int k = 0; k < n; ++k) x[k] = 0.0;
+  for (int k = 0; k < n; ++k) {
+    int i = topol[k], j = topol[k + 1];
+    for (int m = i; m < j; ++m) {
+      if (m == i) x[k] = (b[k] - x[k]) / coef2[m];
+      else x[ja[m]] = x[ja[m]] + coef2[m] * x[k];
+    }
+  }
+  double a = 0.0;
+  for (int kk = n - 1; kk >= 0; --kk) {
+    int i = topol[kk], j = topol[kk + 1] - 1;
+    for (int m = i; m <= j; ++m) {
+      int mm = j - m + i;
+      if (mm == i) {
+        x[kk] = (x[kk] - a) / coef2[mm];
+        a = 0.0;
+      } else {
+        a = a + coef2[mm] * x[ja[mm]];
+      }
+    }
+  }
+}
+
+/* symmetric upper-CSR product y (+)= A x exactly as GRADDP's loops do */
+int orc_graddp(int n, double *poten, double *scr, double *res, double *bv,
+               double *r, double *p, const int *topol, const int *ja,
+               const double *coef1, const double *coef2,
+               const double *tnoti, double tol, int imax, int np,
+               const int *contp, double *err) {
+  /* SRC/solscal-extended.f:1275-1385. */
+  int niter = 1;
+  for (int k = 0; k < n; ++k) {
+    scr[k] = tnoti[k];
+    r[k] = tnoti[k];
+  }
+  for (int i = 0; i < np; ++i) scr[contp[i]] = 0.0;
+  double xlung = 0.0;
+  for (int i = 0; i < n; ++i) xlung = xlung + scr[i] * scr[i];
+  for (int i = 0; i < np; ++i) scr[contp[i]] = tnoti[contp[i]];
+  for (int k = 0; k < n; ++k) {
+    int i = topol[k], j = topol[k + 1];
+    for (int m = i; m < j; ++m) {
+      r[k] = r[k] - coef1[m] * poten[ja[m]];
+      res[k] = r[k];
+      if (m != i) {
+        r[ja[m]] = r[ja[m]] - coef1[m] * poten[k];
+        res[ja[m]] = r[ja[m]];
+      }
+    }
+  }
+  orc_proddp(p, r, n, topol, ja, coef2);
+  for (;;) {
+    for (int k = 0; k < n; ++k) bv[k] = 0.0;
+    for (int k = 0; k < n; ++k) {
+      int i = topol[k], j = topol[k + 1];
+      for (int m = i; m < j; ++m) {
+        bv[k] = bv[k] + coef1[m] * p[ja[m]];
+        if (m != i) bv[ja[m]] = bv[ja[m]] + coef1[m] * p[k];
+      }
+    }
+    double a = 0.0, bb = 0.0;
+    for (int k = 0; k < n; ++k) {
+      a = a + p[k] * r[k];
+      bb = bb + p[k] * bv[k];
+    }
+    double alfa = a / bb;
+    for (int k = 0; k < n; ++k) {
+      r[k] = r[k] - alfa * bv[k];
+      poten[k] = poten[k] + alfa * p[k];
+    }
+    orc_proddp(scr, r, n, topol, ja, coef2);
+    double c = 0.0;
+    for (int k = 0; k < n; ++k) c = c + bv[k] * scr[k];
+    double beta = -c / bb;
+    for (int k = 0; k < n; ++k) p[k] = scr[k] + beta * p[k];
+    for (int k = 0; k < n; ++k) res[k] = r[k];
+    for (int k = 0; k < np; ++k) res[contp[k]] = 0.0;
+    a = 0.0;
+    for (int k = 0; k < n; ++k) a = a + res[k] * res[k];
+    *err = (xlung > 0.0) ? sqrt(a / xlung) : sqrt(a / n);
+    if (*err > tol && niter < imax) {
+      ++niter;
+      continue;
+    }
+    break;
+  }
+  return niter;
+}
+
+double orc_ressym(const int *topol, const int *ja, const double *coef1,
+                  const double *tnoti, const double *pnew, double *resv,
+                  int n, int numdir, const int *noddir, double rmin) {
+  /* SRC/solscal-extended.f:4400-4437. */
+  for (int k = 0; k < n; ++k) resv[k] = tnoti[k];
+  for (int k = 0; k < numdir; ++k) resv[noddir[k]] = 0.0;
+  double xlung = 0.0;
+  for (int k = 0; k < n; ++k) xlung = xlung + resv[k] * resv[k];
+  for (int k = 0; k < n; ++k) {
+    int m = topol[k], mm = topol[k + 1];
+    for (int i = m; i < mm; ++i) {
+      if (i > m) resv[ja[i]] = resv[ja[i]] - coef1[i] * pnew[k];
+      resv[k] = resv[k] - coef1[i] * pnew[ja[i]];
+    }
+  }
+  for (int k = 0; k < numdir; ++k) resv[noddir[k]] = 0.0;
+  double res = 0.0;
+  for (int k = 0; k < n; ++k) res = res + resv[k] * resv[k];
+  return (xlung <= rmin) ? sqrt(res / n) : sqrt(res / xlung);
+}
+
+int orc_symslv(int n, int nterm, int numdir, int itmxcg, double tolcg,
+               double rmin, const int *noddir, const int *ja,
+               const int *topol, double *pnew, const double *tnoti,
+               const double *coef1, double *coef2, double *scr, double *err,
+               double *res) {
+  /* SRC/solscal-extended.f:4679-4697. */
+  orc_incldp(ja, topol, coef1, coef2, n, nterm);
+  orc_proddp(pnew, tnoti, n, topol, ja, coef2);
+  int niter = orc_graddp(n, pnew, scr, scr + n, scr + 2 * (size_t)n,
+                         scr + 3 * (size_t)n, scr + 4 * (size_t)n, topol, ja,
+                         coef1, coef2, tnoti, tolcg, itmxcg, numdir, noddir,
+                         err);
+  *res = orc_ressym(topol, ja, coef1, tnoti, pnew, scr, n, numdir, noddir,
+                    rmin);
+  return niter;
+}
+
+/* ===================================================================== */
+/*                         mass balance (a13-a14)                        */
+/* ===================================================================== */
+
+void orc_bkpic(int n, int np, int nnod, double tetaf, const int *topol,
+               const int *ja, const int *contp, const int *ifatm,
+               const double *pdiff, double *qpnew, const double *qpold,
+               double *atmact, const double *atmold, const double *coef1,
+               const double *xt5, double *scr) {
+  /* SRC/bkpic.f:28-54 (NSF=0). */
+  double tetafr = 1.0 / tetaf;
+  for (int k = 0; k < n; ++k) scr[k] = 0.0;
+  for (int k = 0; k < n; ++k) {
+    int i = topol[k], j = topol[k + 1];
+    for (int m = i; m < j; ++m) {
+      int jm = ja[m];
+      scr[k] = scr[k] + coef1[m] * pdiff[jm];
+      if (m != i) scr[jm] = scr[jm] + coef1[m] * pdiff[k];
+    }
+    scr[k] = scr[k] - xt5[k];
+  }
+  for (int i = 0; i < np; ++i) {
+    int k = contp[i];
+    qpnew[i] = (scr[k] - (1.0 - tetaf) * qpold[i]) * tetafr;
+  }
+  for (int k = 0; k < nnod; ++k)
+    if (ifatm[k] == 1 || ifatm[k] == 2)
+      atmact[k] = (scr[k] - (1.0 - tetaf) * atmold[k]) * tetafr;
+}
+
+double orc_stormb(const double *pnew, const double *ptimep,
+                  const double *swnew, const double *swtimep, int n,
+                  const double *volnod, const double *snodi,
+                  const double *pnodi) {
+  /* SRC/stormb.f:20-25. */
+  double dstore = 0.0;
+  for (int k = 0; k < n; ++k)
+    dstore = dstore +
+             volnod[k] * (snodi[k] * (swnew[k] + swtimep[k]) * 0.5 *
+                              (pnew[k] - ptimep[k]) +
+                          pnodi[k] * (swnew[k] - swtimep[k]));
+  return dstore;
+}
+
+int orc_norms(int n, double *pikmax, double *pinf, double *pl2, double *finf,
+              double *fl2, const double *pnew, const double *pold,
+              const double *tnoti) {
+  /* SRC/norms.f:23-40 (ties keep the last index). */
+  int ikmax = 0;
+  double spl2 = 0.0, sfl2 = 0.0;
+  *pinf = 0.0;
+  *finf = 0.0;
+  *pikmax = 0.0;
+  for (int k = 0; k < n; ++k) {
+    double diff = pnew[k] - pold[k];
+    double difabs = fabs(diff);
+    spl2 = spl2 + diff * diff;
+    sfl2 = sfl2 + tnoti[k] * tnoti[k];
+    if (difabs >= *pinf) {
+      *pinf = difabs;
+      *pikmax = diff;
+      ikmax = k;
+    }
+    double fabs_ = fabs(tnoti[k]);
+    if (fabs_ >= *finf) *finf = fabs_;
+  }
+  *pl2 = sqrt(spl2);
+  *fl2 = sqrt(sfl2);
+  return ikmax;
+}
+
+void orc_switch_old(int nnod, int *ifatm, double *atmact,
+                    const double *atmpot, double *pnew, double pmin) {
+  /* SRC/switch_old.f:22-52. */
+  for (int i = 0; i < nnod; ++i) {
+    if (ifatm[i] == -1) continue;
+    if (ifatm[i] == 1 && pnew[i] >= 0.0) {
+      if (atmpot[i] < 0.0 || atmact[i] > atmpot[i]) {
+        ifatm[i] = 0;
+        atmact[i] = atmpot[i];
+        continue;
+      }
+    }
+    if (ifatm[i] == 1 && pnew[i] <= pmin) {
+      if (atmpot[i] > 0.0 || atmact[i] < atmpot[i]) {
+        ifatm[i] = 0;
+        atmact[i] = atmpot[i];
+        continue;
+      }
+    }
+    if (ifatm[i] == 0 && pnew[i] >= 0.0 && atmpot[i] >= 0.0) {
+      ifatm[i] = 1;
+      pnew[i] = 0.0;
+      continue;
+    }
+    if (ifatm[i] == 0 && pnew[i] <= pmin && atmpot[i] < 0.0) {
+      ifatm[i] = 1;
+      pnew[i] = pmin;
+      continue;
+    }
+  }
+}
+
+int orc_timupd(double *timep, double *time, double *deltat, double dtmin,
+               double dtmax, double tmax, double dtmaga, double dtmagm,
+               double dtreds, double dtredm, int iter, int ituns1,
+               int ituns2) {
+  /* SRC/timupd.f:21-40. */
+  *timep = *time;
+  if (iter < ituns1) {
+    *deltat = *deltat * dtmagm + dtmaga;
+    if (*deltat > dtmax) *deltat = dtmax;
+  }
+  if (iter >= ituns2) {
+    *deltat = *deltat * dtredm - dtreds;
+    if (*deltat < dtmin) *deltat = dtmin;
+  }
+  if ((*time + *deltat) >= tmax) {
+    *deltat = tmax - *time;
+    *time = tmax;
+  } else {
+    if ((*time + 2 * *deltat) > tmax) *deltat = (tmax - *time) / 2;
+    *time = *time + *deltat;
+  }
+  return (*deltat <= dtmin) ? 0 : 1;
+}
+
+void orc_icvdwt(int nnod, int nstr, double wtposition, double *ptimep,
+                const double *z) {
+  /* SRC/icvdwt.f:20-25. */
+  for (int i = 0; i < nnod; ++i)
+    for (int k = 0; k <= nstr; ++k) {
+      size_t kk = (size_t)k * nnod + i;
+      ptimep[kk] = z[i] - z[kk] - wtposition;
+    }
+}
+
+void orc_neumann_free(int nnod, const int *acontq, const double *arenod,
+                      const double *ckrw, const double *kznod, double *q) {
+  /* SRC/neumann.f:19-24. */
+  for (int i = 0; i < nnod; ++i) {
+    int nn = acontq[i];
+    q[i] = -1.0 * arenod[i] * ckrw[nn] * kznod[nn];
+  }
+}
+
+/* ===================================================================== */
+/*                            whole run driver                           */
+/* ===================================================================== */
+
+struct orc_run {
+  orc_params p;
+  int nnod, nstr, n, ntri, nt, nterm, nzone;
+  int *tetra, *topol, *ja, *tetja, *ivol, *tp;
+  double *x, *y, *z, *ai, *bi, *ci, *di, *volu, *volur, *volnod, *arenod;
+  double lmass[16];
+  double *permx, *permy, *permz;
+  double *pnodi, *snodi, *vg, *pel, *kznod;
+  /* nodal state */
+  double *pnew, *pold, *ptnew, *ptold, *ptimep, *pdiff;
+  double *sw, *ckrw, *ckrwp, *etai, *et1, *et2, *swnew, *swtimep;
+  double *ckrwe, *etae, *et1e;
+  double *coef1, *coef2, *coef4, *coef5, *tnoti, *xt5, *scr, *qtranie;
+  /* boundary conditions */
+  int nq;
+  int *contq;
+  double *q;
+  int *ifatm, *ifatmp, *noddir;
+  double *atmpot, *atmact, *atmold, *lhsatm;
+  int numdir;
+  /* atmbc series (file replaced by arrays) */
+  int n_atm, hspatm, atm_next, htiatm;
+  double *atm_times, *atm_vals;
+  double atmtim[3];
+  double *atminp; /* [3][nnod] */
+  /* time */
+  double time, timep, deltat;
+  int dtgmin, nstep, iter, nitert, itlin, itrtot, kbackt, klsfai;
+  /* mass balance */
+  double store1, store2, dstore, cdstor, cvin, cvout, vtot, cerras, caeras;
+  double adinp, adoutp, ndinp, ndoutp, aninp, anoutp, nninp, nnoutp;
+  double vin, vout, erras, errel;
+  double adin, adout, ndin, ndout, anin, anout, nnin, nnout;
+  double pinf, pl2, finf, fl2, pikmax;
+  int lsfail, noback, bckstp, finished;
+};
+
+static void *xdup(const void *src, size_t bytes) {
+  void *d = malloc(bytes ? bytes : 1);
+  if (d && src) memcpy(d, src, bytes);
+  return d;
+}
+static double *dalloc(size_t k) { return (double *)calloc(k ? k : 1, sizeof(double)); }
+static int *ialloc(size_t k) { return (int *)calloc(k ? k : 1, sizeof(int)); }
+#define DALLOC(k) dalloc((size_t)(k))
+#define IALLOC(k) ialloc((size_t)(k))
+
+/* ATMONE/ATMNXT file reader restated over arrays: the next record. */
+static int atm_read(orc_run *r, double *timein, double *vals) {
+  if (r->atm_next >= r->n_atm) return 0; /* END= */
+  int k = r->atm_next++;
+  *timein = r->atm_times[k];
+  if (r->hspatm == 0)
+    for (int i = 0; i < r->nnod; ++i)
+      vals[i] = r->atm_vals[(size_t)k * r->nnod + i];
+  else
+    for (int i = 0; i < r->nnod; ++i) vals[i] = r->atm_vals[k];
+  return 1;
+}
+
+static void atm_interp(orc_run *r, int set_act) {
+  /* SRC/atmnxt.f:41-73 (label 300). */
+  double *a2 = r->atminp + r->nnod, *a3 = r->atminp + 2 * (size_t)r->nnod;
+  for (int i = 0; i < r->nnod; ++i) {
+    if (r->atmtim[2] > r->atmtim[1]) {
+      double slope = (a3[i] - a2[i]) / (r->atmtim[2] - r->atmtim[1]);
+      if (r->p.ieto != 0) slope = 0.0;
+      r->atmpot[i] = (a2[i] + slope * (r->time - r->atmtim[1])) * r->arenod[i];
+    } else {
+      r->atmpot[i] = a3[i] * r->arenod[i];
+    }
+    if (set_act && r->ifatm[i] == 0)
+      r->atmact[i] = (r->atmpot[i] >= 0.0) ? r->atmpot[i]
+                                           : (1.0 - r->p.scf) * r->atmpot[i];
+  }
+}
+
+static void atm_shift_read(orc_run *r) {
+  /* the label-200 loop of SRC/atmnxt.f:30-40 / SRC/atmone.f:77-93 */
+  double *a1 = r->atminp, *a2 = a1 + r->nnod, *a3 = a2 + r->nnod;
+  while (!(r->time <= r->atmtim[2])) {
+    r->atmtim[0] = r->atmtim[1];
+    r->atmtim[1] = r->atmtim[2];
+    for (int i = 0; i < r->nnod; ++i) {
+      a1[i] = a2[i];
+      a2[i] = a3[i];
+    }
+    double tin;
+    if (!atm_read(r, &tin, a3)) {
+      r->htiatm = 1;
+      break;
+    }
+    r->atmtim[2] = tin;
+  }
+}
+
+static void atmnxt(orc_run *r) {
+  /* SRC/atmnxt.f:9-104 (no Dirichlet/Neumann surface nodes, NSF=0). */
+  if (r->htiatm != 0) return;
+  atm_shift_read(r);
+  atm_interp(r, 1);
+}
+
+static void atmbak(orc_run *r) {
+  /* SRC/atmbak.f. */
+  if (r->atmtim[0] >= r->atmtim[1]) return;
+  double *a1 = r->atminp, *a2 = a1 + r->nnod;
+  for (int i = 0; i < r->nnod; ++i) {
+    double slope = (a2[i] - a1[i]) / (r->atmtim[1] - r->atmtim[0]);
+    if (r->p.ieto != 0) slope = 0.0;
+    r->atmpot[i] = (a1[i] + slope * (r->time - r->atmtim[0])) * r->arenod[i];
+    if (r->ifatm[i] == 0)
+      r->atmact[i] = (r->atmpot[i] >= 0.0) ? r->atmpot[i]
+                                           : (1.0 - r->p.scf) * r->atmpot[i];
+  }
+}
+
+static void atmone(orc_run *r) {
+  /* SRC/atmone.f:39-163 (HSPATM read, NP=NQ=0 on the surface). */
+  int nnod = r->nnod;
+  double *a3 = r->atminp + 2 * (size_t)nnod;
+  r->htiatm = 0;
+  r->atmtim[0] = r->atmtim[1] = 0.0;
+  for (int i = 0; i < nnod; ++i) {
+    r->ifatm[i] = 0;
+    r->ifatmp[i] = 0;
+    r->atminp[i] = 0.0;
+    r->atminp[nnod + i] = 0.0;
+    r->atmpot[i] = r->atmact[i] = r->atmold[i] = 0.0;
+  }
+  double tin;
+  if (!atm_read(r, &tin, a3)) {
+    r->htiatm = 1;
+  } else {
+    r->atmtim[2] = tin;
+    if (r->atmtim[2] <= 0.0)
+      for (int i = 0; i < nnod; ++i) r->atmold[i] = a3[i] * r->arenod[i];
+    atm_shift_read(r);
+    atm_interp(r, 0);
+  }
+  for (int i = 0; i < nnod; ++i) {
+    if (r->ifatm[i] == -1) continue;
+    if (r->pnew[i] >= r->p.pondh_min) {
+      r->ifatm[i] = 2;
+      r->ifatmp[i] = 2;
+      continue;
+    }
+    if (r->pnew[i] >= 0.0 && r->atmpot[i] > 0.0) r->ifatm[i] = 1;
+    if (r->ptimep[i] >= 0.0 && r->atmold[i] > 0.0) r->ifatmp[i] = 1;
+    if (r->pnew[i] <= r->p.pmin && r->atmpot[i] < 0.0) {
+      r->pnew[i] = r->p.pmin;
+      r->ifatm[i] = 1;
+    }
+    if (r->ptimep[i] <= r->p.pmin && r->atmold[i] < 0.0) {
+      r->ptimep[i] = r->p.pmin;
+      r->ifatmp[i] = 1;
+    }
+  }
+  for (int i = 0; i < nnod; ++i) {
+    if (r->ifatm[i] == 0)
+      r->atmact[i] = (r->atmpot[i] >= 0.0) ? r->atmpot[i]
+                                           : (1.0 - r->p.scf) * r->atmpot[i];
+    else
+      r->atmact[i] = 0.0;
+    if (r->ifatmp[i] == 1 || r->ifatmp[i] == 2) r->atmold[i] = 0.0;
+  }
+}
+
+static void weight(int n, double teta, const double *vcurr,
+                   const double *vprev, double *vwght) {
+  /* SRC/weight.f. */
+  double teta1 = 1.0 - teta;
+  for (int k = 0; k < n; ++k) vwght[k] = teta * vcurr[k] + teta1 * vprev[k];
+}
+
+static void neumann_update(orc_run *r, const double *ckrw) {
+  if (r->nq > 0)
+    orc_neumann_free(r->nnod, r->contq, r->arenod, ckrw, r->kznod, r->q);
+}
+
+orc_run *orc_run_create(int nnod, int nstr, int ntri, const int *tetra,
+                        const double *x, const double *y, const double *z,
+                        const double *arenod, int nzone,
+                        const double *permx, const double *permy,
+                        const double *permz, const double *elstor,
+                        const double *poros, const double *vgn,
+                        const double *vgrmc, const double *vgpsat,
+                        const double *ptimep0, int n_atm, int hspatm,
+                        const double *atm_times, const double *atm_vals,
+                        const orc_params *prm) {
+  orc_run *r = (orc_run *)calloc(1, sizeof(orc_run));
+  if (!r) return NULL;
+  r->p = *prm;
+  r->nnod = nnod;
+  r->nstr = nstr;
+  r->ntri = ntri;
+  r->n = nnod * (nstr + 1);
+  r->nt = 3 * ntri * nstr;
+  r->nzone = nzone;
+  int n = r->n, nt = r->nt;
+  size_t ns = (size_t)nstr * nzone;
+  r->tetra = (int *)xdup(tetra, sizeof(int) * 5 * (size_t)nt);
+  r->x = (double *)xdup(x, sizeof(double) * n);
+  r->y = (double *)xdup(y, sizeof(double) * n);
+  r->z = (double *)xdup(z, sizeof(double) * n);
+  r->arenod = (double *)xdup(arenod, sizeof(double) * nnod);
+  r->permx = (double *)xdup(permx, sizeof(double) * ns);
+  r->permy = (double *)xdup(permy, sizeof(double) * ns);
+  r->permz = (double *)xdup(permz, sizeof(double) * ns);
+  /* geometry: SRC/grdsys.f -> VOLBAS, LOCMAS */
+  r->ai = DALLOC(4 * (size_t)nt); r->bi = DALLOC(4 * (size_t)nt);
+  r->ci = DALLOC(4 * (size_t)nt); r->di = DALLOC(4 * (size_t)nt);
+  r->volu = DALLOC(nt); r->volur = DALLOC(nt); r->ivol = IALLOC(nt);
+  r->volnod = DALLOC(n);
+  orc_volbas(n, nt, r->tetra, r->x, r->y, r->z, r->ai, r->bi, r->ci, r->di,
+             r->volnod, r->volu, r->volur, r->ivol);
+  orc_locmas(r->lmass, prm->lump);
+  /* pattern: SRC/cathy_main.f:2599-2601 STRPIC + TETPIC */
+  int cap = 16 * n;
+  r->topol = IALLOC(n + 1);
+  r->ja = IALLOC(cap);
+  r->nterm = orc_strpic(n, nt, r->tetra, r->topol, r->ja, cap);
+  r->tetja = IALLOC(16 * (size_t)nt);
+  orc_tetpic(nt, r->tetra, r->topol, r->ja, r->tetja);
+  /* soil: SRC/inital.f -> TPNODI, CHPARM */
+  r->pnodi = DALLOC(n); r->snodi = DALLOC(n);
+  r->vg = DALLOC((size_t)VG_NFIELD * n); r->pel = DALLOC(nt);
+  r->tp = IALLOC(n);
+  orc_tpnodi(n, nt, ntri, r->tetra, nzone, poros, elstor, vgn, vgrmc, vgpsat,
+             r->pnodi, r->snodi, r->vg, r->pel, r->tp);
+  orc_chparm(n, r->pnodi, r->vg);
+  /* state */
+  r->pnew = DALLOC(n); r->pold = DALLOC(n); r->ptnew = DALLOC(n);
+  r->ptold = DALLOC(n); r->ptimep = DALLOC(n); r->pdiff = DALLOC(n);
+  r->sw = DALLOC(n); r->ckrw = DALLOC(n); r->ckrwp = DALLOC(n);
+  r->etai = DALLOC(n); r->et1 = DALLOC(n); r->et2 = DALLOC(n);
+  r->swnew = DALLOC(n); r->swtimep = DALLOC(n);
+  r->ckrwe = DALLOC(nt); r->etae = DALLOC(nt); r->et1e = DALLOC(nt);
+  r->coef1 = DALLOC(r->nterm); r->coef2 = DALLOC(r->nterm);
+  r->coef4 = DALLOC(r->nterm); r->coef5 = DALLOC(r->nterm);
+  r->tnoti = DALLOC(n); r->xt5 = DALLOC(n); r->scr = DALLOC(5 * (size_t)n);
+  r->qtranie = DALLOC(n);
+  r->ifatm = IALLOC(nnod); r->ifatmp = IALLOC(nnod);
+  r->noddir = IALLOC(nnod + 1);
+  r->atmpot = DALLOC(nnod); r->atmact = DALLOC(nnod); r->atmold = DALLOC(nnod);
+  r->lhsatm = DALLOC(nnod); r->atminp = DALLOC(3 * (size_t)nnod);
+  r->n_atm = n_atm;
+  r->hspatm = hspatm;
+  r->atm_times = (double *)xdup(atm_times, sizeof(double) * (size_t)n_atm);
+  r->atm_vals = (double *)xdup(
+      atm_vals, sizeof(double) * (size_t)n_atm * (hspatm ? 1 : nnod));
+  /* initial condition and INIT1 (SRC/init1.f:37-60) */
+  memcpy(r->ptimep, ptimep0, sizeof(double) * n);
+  if (r->p.deltat > r->p.dtmax) r->p.deltat = r->p.dtmax;
+  if (r->p.deltat <= r->p.dtmin) {
+    r->p.deltat = r->p.dtmin;
+    r->dtgmin = 0;
+  } else {
+    r->dtgmin = 1;
+  }
+  r->deltat = r->p.deltat;
+  r->timep = 0.0;
+  r->time = r->deltat;
+  r->iter = 1;
+  r->nstep = 1;
+  memcpy(r->pnew, r->ptimep, sizeof(double) * n);
+  /* free drainage: all bottom nodes (BCONE with NNEU<0) */
+  r->nq = prm->free_drainage ? nnod : 0;
+  r->contq = IALLOC(nnod);
+  r->q = DALLOC(nnod);
+  for (int i = 0; i < nnod; ++i) r->contq[i] = nstr * nnod + i;
+  /* SRC/inital.f:216-223 ATMONE, POLD/PTNEW/PTOLD */
+  atmone(r);
+  memcpy(r->pold, r->pnew, sizeof(double) * n);
+  weight(n, r->p.tetaf, r->pnew, r->ptimep, r->ptnew);
+  memcpy(r->ptold, r->ptnew, sizeof(double) * n);
+  /* MBINIT (SRC/mbinit.f:52-74) */
+  r->adinp = r->adoutp = r->ndinp = r->ndoutp = 0.0;
+  r->aninp = r->anoutp = 0.0;
+  for (int k = 0; k < nnod; ++k)
+    if (r->ifatmp[k] == 0) {
+      if (r->atmold[k] > 0.0) r->aninp += r->atmold[k];
+      else r->anoutp += r->atmold[k];
+    }
+  /* SRC/cathy_main.f:2663-2704: CHVELO, STORCAL, KZNOD, NEUMANN */
+  orc_chvelo(n, r->ptimep, r->pnodi, r->vg, r->sw, r->ckrw);
+  r->store1 = 0.0;
+  for (int i = 0; i < n; ++i)
+    r->store1 = r->store1 + r->sw[i] * r->volnod[i] * r->pnodi[i];
+  r->store2 = r->store1;
+  double *kz = DALLOC(nt);
+  for (int e = 0; e < nt; ++e) {
+    int istr = e / (3 * ntri), mt = r->tetra[(size_t)e * 5 + 4] - 1;
+    kz[e] = r->permz[(size_t)istr * nzone + mt];
+  }
+  r->kznod = DALLOC(n);
+  orc_eltnod(n, nt, r->tp, r->tetra, kz, r->kznod);
+  free(kz);
+  r->nninp = r->nnoutp = 0.0;
+  if (r->nq > 0) {
+    memcpy(r->ckrwp, r->ckrw, sizeof(double) * n);
+    neumann_update(r, r->ckrw);
+    for (int k = 0; k < r->nq; ++k) {
+      if (r->q[k] > 0.0) r->nninp += r->q[k];
+      else r->nnoutp += r->q[k];
+    }
+  }
+  return r;
+}
+
+void orc_run_free(orc_run *r) {
+  if (!r) return;
+  void *ptrs[] = {r->tetra, r->topol, r->ja, r->tetja, r->ivol, r->tp,
+                  r->x, r->y, r->z, r->ai, r->bi, r->ci, r->di, r->volu,
+                  r->volur, r->volnod, r->arenod, r->permx, r->permy,
+                  r->permz, r->pnodi, r->snodi, r->vg, r->pel, r->kznod,
+                  r->pnew, r->pold, r->ptnew, r->ptold, r->ptimep, r->pdiff,
+                  r->sw, r->ckrw, r->ckrwp, r->etai, r->et1, r->et2,
+                  r->swnew, r->swtimep, r->ckrwe, r->etae, r->et1e,
+                  r->coef1, r->coef2, r->coef4, r->coef5, r->tnoti, r->xt5,
+                  r->scr, r->qtranie, r->contq, r->q, r->ifatm, r->ifatmp,
+                  r->noddir, r->atmpot, r->atmact, r->atmold, r->lhsatm,
+                  r->atm_times, r->atm_vals, r->atminp};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
+  free(r);
+}
+
+/* One Picard iteration: SRC/picard.f:64-203 (IPEAT=0, KSLOPE=0, IVGHU=0,
+ * no nudging, NSF=0). */
+static void picard(orc_run *r, double *err_out) {
+  int n = r->n, nt = r->nt;
+  orc_chpic0(n, r->ptnew, r->pnew, r->ptimep, r->swnew, r->swtimep,
+             r->snodi, r->pnodi, r->vg, r->sw, r->ckrw, r->etai, r->et1,
+             r->et2);
+  orc_nodelt(nt, r->tetra, r->ckrw, r->ckrwe);
+  orc_nodelt(nt, r->tetra, r->etai, r->etae);
+  orc_nodelt(nt, r->tetra, r->et1, r->et1e);
+  /* PEL here is NODELT(PNODI) (SRC/picuns.f:48) */
+  orc_nodelt(nt, r->tetra, r->pnodi, r->pel);
+  memset(r->coef1, 0, sizeof(double) * r->nterm);
+  memset(r->coef2, 0, sizeof(double) * r->nterm);
+  memset(r->coef4, 0, sizeof(double) * r->nterm);
+  memset(r->coef5, 0, sizeof(double) * r->nterm);
+  orc_asspic(nt, r->ntri, r->nzone, r->tetra, r->tetja, r->lmass, r->coef1,
+             r->coef2, r->coef4, r->et1e, r->pel, r->permx, r->permy,
+             r->permz, r->ckrwe, r->volu, r->volur, r->bi, r->ci, r->di);
+  orc_rhspic(n, r->ja, r->topol, r->deltat, r->ptnew, r->pnew, r->ptimep,
+             r->swnew, r->swtimep, r->coef1, r->coef2, r->coef4, r->tnoti);
+  orc_cfmatp(n, r->nterm, r->p.tetaf, r->deltat, r->coef1, r->coef2,
+             r->coef4, r->coef5, r->topol, r->et2);
+  orc_rhsgrv(nt, r->ntri, r->nzone, r->tetra, r->tnoti, r->di, r->permz,
+             r->ivol, r->ckrwe);
+  memcpy(r->xt5, r->tnoti, sizeof(double) * n);
+  r->numdir = orc_bcpic(0, r->nq, NULL, r->contq, NULL, r->q, r->nnod, n,
+                        r->lhsatm, r->ifatm, r->atmact, r->atmold, r->topol,
+                        r->coef1, r->tnoti, r->p.tetaf, 1.7e100, r->noddir,
+                        r->qtranie);
+  double err = 0.0, res = 0.0;
+  int niter = orc_symslv(n, r->nterm, r->numdir, r->p.itmxcg, r->p.tolcg,
+                         1.7e-100, r->noddir, r->ja, r->topol, r->pdiff,
+                         r->tnoti, r->coef1, r->coef2, r->scr, &err, &res);
+  r->itlin += niter;
+  r->nitert += niter;
+  if (niter >= r->p.itmxcg) {
+    r->lsfail = 1;
+    r->klsfai += 1;
+  } else {
+    r->lsfail = 0;
+  }
+  for (int k = 0; k < n; ++k) r->pnew[k] = r->pnew[k] + r->pdiff[k];
+  orc_shlpic(0, r->topol, NULL, NULL, NULL, r->coef1, r->pnew, r->pold,
+             r->nnod, r->ifatm, r->lhsatm);
+  if (err_out) *err_out = err;
+}
+
+/* MASBAL restated for IOPT=1, NP=0, NSF=0, no nudging
+ * (SRC/masbal.f:37-107 with SRC/fluxmb.f). */
+static void masbal(orc_run *r) {
+  int n = r->n, nnod = r->nnod;
+  orc_bkpic(n, 0, nnod, r->p.tetaf, r->topol, r->ja, NULL, r->ifatm,
+            r->pdiff, NULL, NULL, r->atmact, r->atmold, r->coef1, r->xt5,
+            r->scr);
+  /* CHMASS(PTIMEP) into SCR and CHMASS(PNEW) into ETAI: only ETAI's new
+   * value is used later (SRC/masbal.f:70-73). */
+  orc_chmass(n, r->ptimep, r->snodi, r->pnodi, r->vg, r->scr);
+  orc_chmass(n, r->pnew, r->snodi, r->pnodi, r->vg, r->etai);
+  double deltam = 0.5 * r->deltat;
+  r->dstore = orc_stormb(r->pnew, r->ptimep, r->swnew, r->swtimep, n,
+                         r->volnod, r->snodi, r->pnodi);
+  double ndin = 0.0, ndout = 0.0, nnin = 0.0, nnout = 0.0;
+  for (int k = 0; k < r->nq; ++k) {
+    if (r->q[k] > 0.0) nnin += r->q[k];
+    else nnout += r->q[k];
+  }
+  double adin = 0.0, adout = 0.0, anin = 0.0, anout = 0.0;
+  for (int k = 0; k < nnod; ++k) {
+    if (r->ifatm[k] == -1) continue;
+    if (r->ifatm[k] == 1 || r->ifatm[k] == 2) {
+      if (r->atmact[k] > 0.0) adin += r->atmact[k];
+      else adout += r->atmact[k];
+    } else {
+      if (r->atmact[k] > 0.0) anin += r->atmact[k];
+      else anout += r->atmact[k];
+    }
+  }
+  double vadin = (adin + r->adinp) * deltam, vadout = (adout + r->adoutp) * deltam;
+  double vndin = (ndin + r->ndinp) * deltam, vndout = (ndout + r->ndoutp) * deltam;
+  double vanin = (anin + r->aninp) * deltam, vanout = (anout + r->anoutp) * deltam;
+  double vnnin = (nnin + r->nninp) * deltam, vnnout = (nnout + r->nnoutp) * deltam;
+  r->vin = vadin + vndin + vanin + vnnin + 0.0;
+  r->vout = vadout + vndout + vanout + vnnout + 0.0 + 0.0;
+  r->erras = r->vin + r->vout - r->dstore;
+  r->errel = ((r->vin + r->vout) != 0.0) ? 100.0 * r->erras / (r->vin + r->vout)
+                                         : 0.0;
+  /* current-step fluxes, promoted to "previous" once the step is accepted
+   * (SRC/cathy_main.f:3283-3290) */
+  r->adin = adin; r->adout = adout; r->ndin = ndin; r->ndout = ndout;
+  r->anin = anin; r->anout = anout; r->nnin = nnin; r->nnout = nnout;
+}
+
+/* FLOW3D, Picard branch: SRC/flow3d.f:93-294 (NLRELX=0, NSF=0). */
+static void flow3d(orc_run *r) {
+  int n = r->n;
+  for (;;) {
+    picard(r, NULL);
+    masbal(r);
+    /* CONVER: SRC/conver.f:24-48 */
+    orc_norms(n, &r->pikmax, &r->pinf, &r->pl2, &r->finf, &r->fl2, r->pnew,
+              r->pold, r->tnoti);
+    if ((r->p.l2norm == 0 && r->pinf <= r->p.tolswi) ||
+        (r->p.l2norm != 0 && r->pl2 <= r->p.tolswi))
+      orc_switch_old(r->nnod, r->ifatm, r->atmact, r->atmpot, r->pnew,
+                     r->p.pmin);
+    r->bckstp = 0;
+    int itagen = r->iter < r->p.ituns;
+    int errgmx = (r->pl2 >= r->p.ernlmx || r->pinf >= r->p.ernlmx ||
+                  r->fl2 >= r->p.ernlmx || r->finf >= r->p.ernlmx);
+    int normcv = (r->p.l2norm == 0) ? (r->pinf <= r->p.toluns)
+                                    : (r->pl2 <= r->p.toluns);
+    if (!r->lsfail && !errgmx && !normcv && itagen) {
+      memcpy(r->pold, r->pnew, sizeof(double) * n);
+      memcpy(r->ptold, r->ptnew, sizeof(double) * n);
+      weight(n, r->p.tetaf, r->pnew, r->ptimep, r->ptnew);
+      r->iter += 1;
+      continue;
+    }
+    r->itrtot += r->iter;
+    if (!r->lsfail && !errgmx && normcv) {
+      r->noback = 0;
+    } else if (r->dtgmin) {
+      r->bckstp = 1;
+      r->noback = 0;
+    } else {
+      r->noback = 1;
+    }
+    return;
+  }
+}
+
+/* BKSTEP: SRC/bkstep.f:58-147 (constant non-atmospheric BCs, NSF=0). */
+static void bkstep(orc_run *r) {
+  int n = r->n, nnod = r->nnod;
+  memcpy(r->pnew, r->ptimep, sizeof(double) * n);
+  memcpy(r->ifatm, r->ifatmp, sizeof(int) * nnod);
+  memcpy(r->atmact, r->atmold, sizeof(double) * nnod);
+  r->time = r->timep;
+  r->deltat = r->deltat * r->p.dtredm - r->p.dtreds;
+  if (r->deltat <= r->p.dtmin) {
+    r->deltat = r->p.dtmin;
+    r->dtgmin = 0;
+  } else {
+    r->dtgmin = 1;
+  }
+  r->time = r->time + r->deltat;
+  r->kbackt += 1;
+  r->iter = 1;
+  r->nitert = 0;
+  if (r->time > r->atmtim[1]) atmnxt(r);
+  else atmbak(r);
+  orc_switch_old(nnod, r->ifatm, r->atmact, r->atmpot, r->pnew, r->p.pmin);
+  memcpy(r->pold, r->pnew, sizeof(double) * n);
+  weight(n, r->p.tetaf, r->pnew, r->ptimep, r->ptnew);
+  memcpy(r->ptold, r->ptnew, sizeof(double) * n);
+}
+
+int orc_run_step(orc_run *r, double *st) {
+  if (r->finished) return r->finished;
+  int n = r->n, nnod = r->nnod;
+  /* top of the time loop, SRC/cathy_main.f:2892-2980 */
+  neumann_update(r, r->ckrw);
+  atmnxt(r);
+  for (int i = 0; i < n; ++i) r->qtranie[i] = 0.0; /* ETRAN, no ET */
+  orc_switch_old(nnod, r->ifatm, r->atmact, r->atmpot, r->pnew, r->p.pmin);
+  memcpy(r->pold, r->pnew, sizeof(double) * n);
+  weight(n, r->p.tetaf, r->pnew, r->ptimep, r->ptnew);
+  memcpy(r->ptold, r->ptnew, sizeof(double) * n);
+  int backs = 0;
+  for (;;) { /* label 100 */
+    flow3d(r);
+    if (!r->bckstp) break;
+    bkstep(r);
+    neumann_update(r, r->ckrwp);
+    ++backs;
+  }
+  /* SRC/cathy_main.f:3233-3262: CHVELO, STORCAL, cumulative balance */
+  orc_chvelo(n, r->pnew, r->pnodi, r->vg, r->sw, r->ckrw);
+  r->store1 = 0.0;
+  for (int i = 0; i < n; ++i)
+    r->store1 = r->store1 + r->sw[i] * r->volnod[i] * r->pnodi[i];
+  r->store2 = r->store2 + r->dstore;
+  r->vtot = r->vtot + (r->vin + r->vout);
+  r->cvin = r->cvin + r->vin;
+  r->cvout = r->cvout + r->vout;
+  r->cdstor = r->cdstor + r->dstore;
+  r->cerras = r->cerras + r->erras;
+  r->caeras = r->caeras + fabs(r->erras);
+  if (st) {
+    st[0] = r->nstep; st[1] = r->deltat; st[2] = r->time; st[3] = r->iter;
+    st[4] = r->nitert; st[5] = backs; st[6] = r->store1; st[7] = r->store2;
+    st[8] = r->dstore; st[9] = r->cdstor; st[10] = r->vin; st[11] = r->cvin;
+    st[12] = r->vout; st[13] = r->cvout; st[14] = r->vin + r->vout;
+    st[15] = r->vtot; st[16] = r->erras; st[17] = r->errel;
+    st[18] = r->cerras; st[19] = r->caeras; st[20] = r->pinf;
+    st[21] = r->itrtot; st[22] = r->itlin; st[23] = r->klsfai;
+  }
+  r->adinp = r->adin; r->adoutp = r->adout; r->ndinp = r->ndin;
+  r->ndoutp = r->ndout; r->aninp = r->anin; r->anoutp = r->anout;
+  r->nninp = r->nnin; r->nnoutp = r->nnout;
+  /* SRC/cathy_main.f:3765-3770 */
+  memcpy(r->ifatmp, r->ifatm, sizeof(int) * nnod);
+  memcpy(r->atmold, r->atmact, sizeof(double) * nnod);
+  memcpy(r->ptimep, r->pnew, sizeof(double) * n);
+  memcpy(r->ckrwp, r->ckrw, sizeof(double) * n);
+  if (r->noback) return r->finished = 2;
+  if (fabs(r->time - r->p.tmax) <= 0.001 * r->deltat) return r->finished = 1;
+  r->dtgmin = orc_timupd(&r->timep, &r->time, &r->deltat, r->p.dtmin,
+                         r->p.dtmax, r->p.tmax, r->p.dtmaga, r->p.dtmagm,
+                         r->p.dtreds, r->p.dtredm, r->iter, r->p.ituns1,
+                         r->p.ituns2);
+  r->nstep += 1; /* TIMNXT */
+  r->iter = 1;
+  r->nitert = 0;
+  r->kbackt = 0;
+  return 0;
+}
+
+void orc_run_get_psi(const orc_run *r, double *psi) {
+  memcpy(psi, r->pnew, sizeof(double) * r->n);
+}
+void orc_run_get_sw(const orc_run *r, double *sw) {
+  memcpy(sw, r->sw, sizeof(double) * r->n);
+}
+void orc_run_get_ifatm(const orc_run *r, int *ifatm) {
+  memcpy(ifatm, r->ifatm, sizeof(int) * r->nnod);
+}
+int orc_run_n(const orc_run *r) { return r->n; }
+int orc_run_nterm(const orc_run *r) { return r->nterm; }
+const int *orc_run_topol(const orc_run *r) { return r->topol; }
+const int *orc_run_ja(const orc_run *r) { return r->ja; }

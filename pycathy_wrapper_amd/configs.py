@@ -1,0 +1,171 @@
+"""Synthetic workloads of the subsurface Picard step.
+
+Each builder returns a ``Problem``: mesh, per-(layer, zone) soil tables,
+initial pressure head, atmospheric forcing series and the ``input/parm``
+solver parameters.  Everything is generated from a seed; nothing is read
+from disk.
+
+* ``hillslope_small``  -- 21x21-node DEM (20x20 cells of 5 m) on a 2% planar
+  slope with N(0, 0.05^2) noise, 10 uniform layers to 5 m (4,851 nodes,
+  24,000 tets); homogeneous van Genuchten soil (n=1.46, alpha=1/m,
+  theta_s=0.45, theta_r=0.05, Ks=1e-5 m/s, Ss=1e-4 1/m); water table 1 m
+  deep; 1e-7 m/s rain; free drainage at the base; one simulated day.
+* ``hillslope_large``  -- 201x201-node DEM (200x200 cells of 2 m) with
+  0-20 m spectral (fractal) relief, 30 geometrically graded layers to 10 m
+  (1,252,431 nodes, 7.2M tets); lognormal Ks field (geometric mean 1e-5 m/s,
+  ln-sigma 1, exponential covariance length 20 m) per (layer, DEM cell)
+  zone, alpha and n perturbed +-20% per zone; 6 h of 5e-7 m/s rain then
+  drainage; two simulated days.  ``cells`` shrinks the DEM for tests.
+
+Soil values are rounded to 4 significant digits, the precision of the
+soil file pyCATHY writes (cathy_tools.py:3130, ``%1.3e``), so a run from
+files and a run from these arrays see the same numbers.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field, asdict
+
+import numpy as np
+
+from . import mesh as M
+
+
+@dataclass
+class Params:
+    """input/parm fields used by the subsurface step (datin.f:80-122), the
+    soil-file head (PMIN) and the forcing options."""
+    tetaf: float = 1.0
+    lump: int = 1
+    iopt: int = 1
+    l2norm: int = 0
+    toluns: float = 1e-6
+    tolswi: float = 1e30
+    ernlmx: float = 1e30
+    ituns: int = 10
+    ituns1: int = 5
+    ituns2: int = 7
+    itmxcg: int = 500
+    tolcg: float = 1e-10
+    deltat: float = 10.0
+    dtmin: float = 1.0
+    dtmax: float = 3600.0
+    tmax: float = 86400.0
+    dtmaga: float = 0.0
+    dtmagm: float = 1.1
+    dtreds: float = 0.0
+    dtredm: float = 0.5
+    pmin: float = -5.0
+    pondh_min: float = 0.0
+    scf: float = 0.0
+    ieto: int = 0
+    free_drainage: int = 1
+
+    def as_dict(self):
+        return asdict(self)
+
+
+@dataclass
+class Problem:
+    name: str
+    mesh: M.Mesh
+    soil: dict            # permx/permy/permz/elstor/poros/vgn/vgrmc/vgpsat: (nstr, nzone); nzone
+    psi0: np.ndarray
+    atm_times: np.ndarray
+    atm_vals: np.ndarray  # (n_atm,) homogeneous (HSPATM=1)
+    params: Params
+    seed: int = 0
+    meta: dict = field(default_factory=dict)
+
+
+def sig4(a: np.ndarray) -> np.ndarray:
+    """Round to 4 significant digits (the %1.3e of the soil file)."""
+    a = np.asarray(a, dtype=np.float64)
+    out = np.zeros_like(a)
+    nz = a != 0
+    e = np.floor(np.log10(np.abs(a[nz])))
+    out[nz] = np.round(a[nz] / 10.0 ** e, 3) * 10.0 ** e
+    # exact decimal round-trip, as a text file would give
+    return np.array([float(f"{v:.3e}") for v in out.ravel()]).reshape(a.shape)
+
+
+def icvdwt(mesh: M.Mesh, wt_depth: float) -> np.ndarray:
+    """icvdwt.f:20-25: hydrostatic head, water table wt_depth below surface."""
+    nnod = mesh.nnod
+    zs = np.tile(mesh.z[:nnod], mesh.nstr + 1)
+    return zs - mesh.z - wt_depth
+
+
+def homogeneous_soil(nstr: int, ks=1e-5, ss=1e-4, poros=0.45, vgn=1.46, vgrmc=0.05, vgpsat=1.0):
+    f = lambda v: np.full((nstr, 1), v, dtype=np.float64)
+    return dict(nzone=1, permx=f(ks), permy=f(ks), permz=f(ks), elstor=f(ss), poros=f(poros),
+                vgn=f(vgn), vgrmc=f(vgrmc), vgpsat=f(vgpsat))
+
+
+def hillslope_small_params(**kw) -> Params:
+    """Solver settings under which the small hillslope's day completes.
+    With DTMIN = 1 s and ITUNS = 10 the reference's own FLOW3D stalls near
+    the falling water table (free drainage from a saturated base) and stops
+    with NOBACK after ~100 s; DTMIN = 0.01 s and ITUNS = 30 let it back-step
+    through those steps."""
+    base = dict(toluns=1e-6, dtmin=0.01, ituns=30, ituns1=10, ituns2=20)
+    base.update(kw)
+    return Params(**base)
+
+
+def hillslope_small(seed: int = 1, params: Params | None = None, nstr: int = 10, cells: int = 20) -> Problem:
+    rng = np.random.default_rng(seed)
+    dx = 5.0
+    xc = (np.arange(cells) + 0.5) * dx
+    dem = 100.0 - 0.02 * xc[None, :] + rng.normal(0.0, 0.05, (cells, cells))
+    m = M.build_mesh(dem, dx, dx, nstr, [1.0 / nstr] * nstr, 5.0)
+    soil = homogeneous_soil(nstr)
+    p = params or hillslope_small_params()
+    return Problem("hillslope_small", m, soil, icvdwt(m, 1.0), np.array([0.0]), np.array([1e-7]), p, seed,
+                   dict(cells=cells, dx=dx))
+
+
+def _spectral_dem(n: int, dx: float, relief: float, rng, beta: float = 3.0) -> np.ndarray:
+    kx = np.fft.fftfreq(n, dx)
+    k = np.sqrt(kx[:, None] ** 2 + kx[None, :] ** 2)
+    k[0, 0] = 1.0
+    amp = k ** (-beta / 2.0)
+    amp[0, 0] = 0.0
+    ph = rng.uniform(0, 2 * np.pi, (n, n))
+    f = np.real(np.fft.ifft2(amp * np.exp(1j * ph)))
+    f = (f - f.min()) / (f.max() - f.min())
+    return f * relief
+
+
+def _grf_exponential(n: int, dx: float, corr: float, rng) -> np.ndarray:
+    """Stationary Gaussian field, exponential covariance, by circulant
+    embedding on a doubled periodic grid."""
+    m = 2 * n
+    ix = np.minimum(np.arange(m), m - np.arange(m)) * dx
+    r = np.sqrt(ix[:, None] ** 2 + ix[None, :] ** 2)
+    cov = np.exp(-r / corr)
+    lam = np.real(np.fft.fft2(cov))
+    lam = np.maximum(lam, 0.0)
+    w = rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m))
+    f = np.fft.fft2(np.sqrt(lam / (m * m)) * w)
+    return np.real(f)[:n, :n]
+
+
+def hillslope_large(seed: int = 2, params: Params | None = None, cells: int = 200, nstr: int = 30,
+                    dx: float = 2.0, ratio: float = 1.1) -> Problem:
+    rng = np.random.default_rng(seed)
+    dem = 100.0 + _spectral_dem(cells, dx, 20.0, rng)
+    zone = np.arange(1, cells * cells + 1, dtype=np.int32).reshape(cells, cells)
+    zr = ratio ** np.arange(nstr)
+    zr = zr / zr.sum()                     # geometric grading, thin at the top
+    m = M.build_mesh(dem, dx, dx, nstr, zr, 10.0, zone=zone)
+    nz = cells * cells
+    lnks = np.stack([_grf_exponential(cells, dx, 20.0, rng).ravel() for _ in range(nstr)])
+    ks = sig4(np.exp(np.log(1e-5) + 1.0 * lnks))
+    alpha = 1.0 * (1.0 + rng.uniform(-0.2, 0.2, (nstr, nz)))
+    vgn = sig4(1.46 * (1.0 + rng.uniform(-0.2, 0.2, (nstr, nz))))
+    soil = dict(nzone=nz, permx=ks, permy=ks.copy(), permz=ks.copy(), elstor=np.full((nstr, nz), 1e-4),
+                poros=np.full((nstr, nz), 0.45), vgn=vgn, vgrmc=np.full((nstr, nz), 0.05),
+                vgpsat=sig4(1.0 / alpha))
+    p = params or Params(tmax=172800.0)
+    return Problem("hillslope_large", m, soil, icvdwt(m, 1.0), np.array([0.0, 21600.0, 21600.0 + 1.0]),
+                   np.array([5e-7, 5e-7, 0.0]), p, seed, dict(cells=cells, dx=dx, ratio=ratio))

@@ -1,0 +1,1084 @@
+// cathy_gpu.hip -- C ABI and host orchestration of the CATHY subsurface
+// Picard step on one MI355X (gfx950).
+//
+// Layering:
+//   Flow (below)        one mesh: device-resident state, the Picard loop of
+//                       flow3d.f/picard.f, IC(0)-PCG of solscal-extended.f,
+//                       and the per-time-step host logic of cathy_main.f.
+//   extern "C" entries   the FLOW3D / SYMSLV drop-in boundary
+//                       (include/cathy_gpu.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "cathy_gpu.h"
+#include "flow_kernels.h"
+
+namespace cg {
+
+static thread_local std::string g_last_error;
+
+// ------------------------------------------------------------- schedule --
+struct Schedule {
+  std::vector<LevelDesc> lv;
+  DBuf<int> rows;  // row lists of list-form levels
+};
+
+// Level schedule of the natural elimination order on a full CSR (sorted
+// columns): level(r) = 1 + max level of its lower neighbours.
+static void natural_levels(int n, const std::vector<int> &rowptr, const std::vector<int> &col,
+                           Schedule &sch, hipStream_t st) {
+  std::vector<int> lev(n, 0);
+  int nlev = 0;
+  for (int r = 0; r < n; ++r) {
+    int l = 0;
+    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+      int j = col[p];
+      if (j >= r) break;
+      l = std::max(l, lev[j] + 1);
+    }
+    lev[r] = l;
+    nlev = std::max(nlev, l + 1);
+  }
+  std::vector<int> cnt(nlev + 1, 0), rows(n);
+  for (int r = 0; r < n; ++r) cnt[lev[r] + 1]++;
+  for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int r = 0; r < n; ++r) rows[fill[lev[r]]++] = r;
+  sch.lv.clear();
+  for (int l = 0; l < nlev; ++l)
+    sch.lv.push_back(LevelDesc{0, cnt[l + 1] - cnt[l], 0, 1, cnt[l]});
+  sch.rows.upload(rows, st);
+}
+
+// --------------------------------------------------------------- params --
+struct Params {
+  int iopt = 1, l2norm = 0, ituns = 10, ituns1 = 5, ituns2 = 7, itmxcg = 500, ieto = 0;
+  int free_drainage = 1, hspatm = 1;
+  double tetaf = 1.0, toluns = 1e-6, tolswi = 1e30, ernlmx = 1e30, tolcg = 1e-10;
+  double deltat = 10.0, dtmin = 1.0, dtmax = 3600.0, tmax = 86400.0;
+  double dtmaga = 0.0, dtmagm = 1.1, dtreds = 0.0, dtredm = 0.5;
+  double pmin = -5.0, pondh_min = 0.0, scf = 0.0;
+};
+
+// ------------------------------------------------------------------ Flow --
+struct Flow {
+  int n = 0, nt = 0, ntri = 0, nnod = 0, nstr = 0, nnz = 0, ordering = 0, lump = 1;
+  bool f32 = true;  // IC(0) factor precision
+  hipStream_t st = nullptr;
+  std::vector<int> perm, iperm;  // CATHY id -> device id, device -> CATHY
+
+  // mesh / pattern (device numbering)
+  DBuf<int> rowptr, col, diagpos, tpos, inc_ptr, inc, refid, row_surf, surf_row, bot_row;
+  DBuf<uint32_t> incoff;
+  DBuf<int4> enode;
+  DBuf<double> S, gz, volu, m4d, m4full, lmass, arenod;
+  DBuf<double> psat, vgn, vgm, vgn1, vgnr, vgmr, vgpsn, pnot, rmcp, snodi, pnodi, volnod, kznod;
+  Schedule sched;
+
+  // state
+  DBuf<double> pnew, pold, ptnew, ptold, ptimep, pdiff, sw, ckrw, ckrwp, et1, et2, swnew, swtimep;
+  DBuf<double> ckrwe, et1e, A, tnoti, xt5, bcflux, qnode, qbot;
+  DBuf<uint8_t> isdir;
+  DBuf<int> ifatm, ifatmp;
+  DBuf<double> atmpot, atmact, atmold;
+  DBuf<float> F32;
+  DBuf<double> F64, dinv;
+  DBuf<double> r, p, Bv, z;
+  DBuf<double> part;  // partial sums, kSlots slots of kMaxRedBlocks
+  DBuf<int> ipart;
+  DBuf<PcgScal> scal;
+  DBuf<int> flags;  // [0] PCG done, [1] IC(0) breakdown count
+  DBuf<double> res;
+  std::vector<double> h_res;
+
+  // host time / balance state (cathy_main.f)
+  Params prm;
+  std::vector<double> atm_t, atm_v;  // series
+  int atm_next = 0, htiatm = 0;
+  double atmtim[3] = {0, 0, 0};
+  std::vector<double> atminp;  // 3 x (1 or nnod) rates
+  double time = 0, timep = 0, deltat = 0;
+  int dtgmin = 1, nstep = 1, iter = 1, nitert = 0, itlin = 0, itrtot = 0, klsfai = 0;
+  double store1 = 0, store2 = 0, dstore = 0, cdstor = 0, cvin = 0, cvout = 0, vtot = 0, cerras = 0,
+         caeras = 0, vin = 0, vout = 0, erras = 0, errel = 0;
+  double adinp = 0, adoutp = 0, ndinp = 0, ndoutp = 0, aninp = 0, anoutp = 0, nninp = 0, nnoutp = 0;
+  double cur[8] = {0};  // adin adout ndin ndout anin anout nnin nnout of the last MASBAL
+  double pinf = 0, pl2 = 0, finf = 0, fl2 = 0, pikmax = 0;
+  int ikmax = 0, lsfail = 0, noback = 0, bckstp = 0, finished = 0, initialised = 0;
+  // counters
+  double c_picard = 0, c_pcg = 0, c_fall = 0, c_solve_ms = 0, c_dev_ms = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  ~Flow() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  SoilSoA soil() const {
+    return SoilSoA{psat.p, vgn.p, vgm.p, vgn1.p, vgnr.p, vgmr.p, vgpsn.p, pnot.p, rmcp.p, snodi.p, pnodi.p};
+  }
+  // partial-sum slots: 0 XLUNG, 1 (p,r), 2 (p,Ap), 3 r_free^2, 4 (Ap,z),
+  // 5-10 norms/DSTORE, 11-16 boundary fluxes, 17 storage
+  static constexpr int kSlots = 18;
+  double *P(int slot) { return part.p + (size_t)slot * kMaxRedBlocks; }
+  int red_grid(size_t work) const { return grid_for(work, kBlock, kMaxRedBlocks); }
+
+  // ------------------------------------------------------------ set-up --
+  void setup(int n_, int nt_, int ntri_, int nnod_, int nstr_, int nzone, int maxstr, const int *tetra,
+             const double *bi, const double *ci, const double *di, const double *volu_, const int *ivol,
+             const double *volnod_, const double *arenod_, const double *permx, const double *permy,
+             const double *permz, const double *snodi_, const double *pnodi_, const double *vgn_,
+             const double *vgrmc_, const double *vgpsat_, const double *kznod_, int lump_, int ordering_) {
+    n = n_; nt = nt_; ntri = ntri_; nnod = nnod_; nstr = nstr_; lump = lump_; ordering = ordering_;
+    CG_REQUIRE(n > 0 && nt > 0 && nnod > 0 && nstr > 0, "empty mesh");
+    CG_REQUIRE((long)nnod * (nstr + 1) == n, "N must equal NNOD*(NSTR+1) (gen3d.f numbering)");
+    CG_REQUIRE((long)3 * ntri * nstr == nt, "NT must equal 3*NTRI*NSTR (gen3d.f)");
+    CG_REQUIRE(nzone >= 1 && maxstr >= nstr, "NZONE/MAXSTR");
+    CG_REQUIRE(ordering == CATHY_GPU_ORDER_NATURAL || ordering == CATHY_GPU_ORDER_COLUMN3, "ORDERING");
+    f32 = ordering == CATHY_GPU_ORDER_COLUMN3;
+    CG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    CG_HIP(hipEventCreate(&ev0));
+    CG_HIP(hipEventCreate(&ev1));
+    for (long e = 0; e < (long)nt * 5; e += 5) {
+      for (int k = 0; k < 4; ++k)
+        CG_REQUIRE(tetra[e + k] >= 1 && tetra[e + k] <= n, "TETRA node id out of range");
+      CG_REQUIRE(tetra[e + 4] >= 1 && tetra[e + 4] <= nzone, "TETRA zone out of range");
+    }
+    build_ordering(tetra);
+    build_pattern(tetra);
+    build_elements(tetra, bi, ci, di, volu_, ivol, permx, permy, permz, maxstr, pnodi_);
+    build_nodes(snodi_, pnodi_, vgn_, vgrmc_, vgpsat_, volnod_, kznod_, arenod_);
+    alloc_state();
+    CG_HIP(hipStreamSynchronize(st));
+  }
+
+  // Surface 3-colouring by triangle propagation (a triangulated disc with
+  // even interior degrees is 3-colourable and the colouring is forced);
+  // greedy fallback otherwise.
+  std::vector<int> surface_colours(const int *tetra, int &ncol) {
+    std::vector<std::array<int, 3>> tris;
+    for (int e = 0; e < 3 * ntri; ++e) {  // layer-1 tets with 3 surface nodes
+      int v[4], m = 0;
+      for (int k = 0; k < 4; ++k)
+        if (tetra[5 * e + k] - 1 < nnod) v[m++] = tetra[5 * e + k] - 1;
+      if (m == 3) tris.push_back({v[0], v[1], v[2]});
+    }
+    std::vector<std::vector<int>> adj(nnod);
+    std::map<std::pair<int, int>, std::vector<int>> edge_tris;
+    for (int t = 0; t < (int)tris.size(); ++t)
+      for (int a = 0; a < 3; ++a) {
+        int u = tris[t][a], w = tris[t][(a + 1) % 3];
+        adj[u].push_back(w);
+        adj[w].push_back(u);
+        edge_tris[{std::min(u, w), std::max(u, w)}].push_back(t);
+      }
+    std::vector<int> c(nnod, -1);
+    bool ok = true;
+    std::vector<char> seen(tris.size(), 0);
+    for (int t0 = 0; t0 < (int)tris.size() && ok; ++t0) {
+      if (seen[t0]) continue;
+      std::vector<int> stack{t0};
+      seen[t0] = 1;
+      // seed the first triangle of a component
+      int used = 0;
+      for (int a = 0; a < 3; ++a)
+        if (c[tris[t0][a]] >= 0) used |= 1 << c[tris[t0][a]];
+      for (int a = 0; a < 3; ++a)
+        if (c[tris[t0][a]] < 0) {
+          int k = 0;
+          while (used & (1 << k)) ++k;
+          c[tris[t0][a]] = k;
+          used |= 1 << k;
+        }
+      while (!stack.empty() && ok) {
+        int t = stack.back();
+        stack.pop_back();
+        int cols = 0;
+        for (int a = 0; a < 3; ++a) cols |= 1 << c[tris[t][a]];
+        if (cols != 7) { ok = false; break; }
+        for (int a = 0; a < 3; ++a) {
+          int u = tris[t][a], w = tris[t][(a + 1) % 3];
+          for (int t2 : edge_tris[{std::min(u, w), std::max(u, w)}]) {
+            if (seen[t2]) continue;
+            seen[t2] = 1;
+            for (int b = 0; b < 3; ++b) {
+              int x = tris[t2][b];
+              if (x == u || x == w) continue;
+              int need = 7 & ~((1 << c[u]) | (1 << c[w]));
+              int want = need == 1 ? 0 : need == 2 ? 1 : 2;
+              if (c[x] < 0) c[x] = want;
+              else if (c[x] != want) ok = false;
+            }
+            stack.push_back(t2);
+          }
+        }
+      }
+    }
+    for (int s = 0; s < nnod && ok; ++s)
+      for (int w : adj[s])
+        if (c[s] == c[w] || c[s] < 0) ok = false;
+    if (!ok) {  // greedy colouring (any graph)
+      std::fill(c.begin(), c.end(), -1);
+      for (int s = 0; s < nnod; ++s) {
+        uint64_t used = 0;
+        for (int w : adj[s])
+          if (c[w] >= 0 && c[w] < 64) used |= 1ull << c[w];
+        int k = 0;
+        while (used & (1ull << k)) ++k;
+        c[s] = k;
+      }
+    }
+    ncol = 0;
+    for (int s = 0; s < nnod; ++s) ncol = std::max(ncol, c[s] + 1);
+    return c;
+  }
+
+  void build_ordering(const int *tetra) {
+    perm.assign(n, 0);
+    iperm.assign(n, 0);
+    sched.lv.clear();
+    if (ordering == CATHY_GPU_ORDER_NATURAL) {
+      for (int i = 0; i < n; ++i) perm[i] = iperm[i] = i;
+      return;
+    }
+    int ncolr = 0;
+    std::vector<int> c = surface_colours(tetra, ncolr);
+    std::vector<int> cnt(ncolr, 0), rank(nnod);
+    for (int s = 0; s < nnod; ++s) rank[s] = cnt[c[s]]++;
+    std::vector<long> base(ncolr + 1, 0);
+    for (int k = 0; k < ncolr; ++k) base[k + 1] = base[k] + (long)cnt[k] * (nstr + 1);
+    for (int s = 0; s < nnod; ++s)
+      for (int k = 0; k <= nstr; ++k) {
+        const long ref = (long)k * nnod + s;
+        const long dev = base[c[s]] + (long)k * cnt[c[s]] + rank[s];
+        perm[ref] = (int)dev;
+        iperm[dev] = (int)ref;
+      }
+    for (int k = 0; k < ncolr; ++k)
+      if (cnt[k]) sched.lv.push_back(LevelDesc{(int)base[k], cnt[k], cnt[k], nstr + 1, -1});
+  }
+
+  void build_pattern(const int *tetra) {
+    // full pattern in device numbering: sorted slots per row
+    const int W = kMaxRow;
+    std::vector<int> slot((size_t)n * W), cnt(n, 0);
+    auto ins = [&](int r, int c) {
+      int *s = &slot[(size_t)r * W];
+      int m = cnt[r], pos = 0;
+      while (pos < m && s[pos] < c) ++pos;
+      if (pos < m && s[pos] == c) return;
+      CG_REQUIRE(m < W, "row longer than kMaxRow in the full pattern");
+      std::memmove(s + pos + 1, s + pos, sizeof(int) * (m - pos));
+      s[pos] = c;
+      cnt[r] = m + 1;
+    };
+    for (int e = 0; e < nt; ++e) {
+      int v[4];
+      for (int k = 0; k < 4; ++k) v[k] = perm[tetra[5 * e + k] - 1];
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) ins(v[a], v[b]);
+    }
+    std::vector<int> hrp(n + 1, 0);
+    for (int r = 0; r < n; ++r) hrp[r + 1] = hrp[r] + cnt[r];
+    nnz = hrp[n];
+    std::vector<int> hcol(nnz), hdiag(n), htpos(nnz);
+    for (int r = 0; r < n; ++r)
+      for (int k = 0; k < cnt[r]; ++k) {
+        hcol[hrp[r] + k] = slot[(size_t)r * W + k];
+        if (slot[(size_t)r * W + k] == r) hdiag[r] = hrp[r] + k;
+      }
+    for (int r = 0; r < n; ++r)
+      for (int p = hrp[r]; p < hrp[r + 1]; ++p) {
+        int j = hcol[p];
+        const int *b = &hcol[hrp[j]], *e = &hcol[hrp[j + 1]];
+        const int *f = std::lower_bound(b, e, r);
+        CG_REQUIRE(f != e && *f == r, "pattern not symmetric");
+        htpos[p] = hrp[j] + (int)(f - b);
+      }
+    rowptr.upload(hrp, st);
+    col.upload(hcol, st);
+    diagpos.upload(hdiag, st);
+    tpos.upload(htpos, st);
+    if (ordering == CATHY_GPU_ORDER_NATURAL) natural_levels(n, hrp, hcol, sched, st);
+    h_rowptr = std::move(hrp);
+    h_col = std::move(hcol);
+  }
+  std::vector<int> h_rowptr, h_col;
+
+  void build_elements(const int *tetra, const double *bi, const double *ci, const double *di,
+                      const double *volu_, const int *ivol, const double *permx, const double *permy,
+                      const double *permz, int maxstr, const double *pnodi_) {
+    std::vector<int4> hen(nt);
+    std::vector<double> hS((size_t)nt * 10), hgz((size_t)nt * 4), hvol(nt);
+    std::vector<int> icnt(n + 1, 0);
+    std::vector<double> pel(nt);
+    for (int e = 0; e < nt; ++e) {
+      int v[4];
+      for (int k = 0; k < 4; ++k) v[k] = perm[tetra[5 * e + k] - 1];
+      hen[e] = make_int4(v[0], v[1], v[2], v[3]);
+      const int istr = e / (3 * ntri);
+      const int mt = tetra[5 * e + 4] - 1;
+      const size_t ix = (size_t)mt * maxstr + istr;  // PERMX(MAXSTR,NZONE)
+      const double vr = 1.0 / volu_[e];
+      const double k1 = permx[ix] * vr, k2 = permy[ix] * vr, k3 = permz[ix] * vr;
+      int q = 0;
+      for (int a = 0; a < 4; ++a) {
+        const double pb = k1 * bi[4 * e + a], pc = k2 * ci[4 * e + a], pd = k3 * di[4 * e + a];
+        for (int b = a; b < 4; ++b)
+          hS[(size_t)e * 10 + q++] = pb * bi[4 * e + b] + pc * ci[4 * e + b] + pd * di[4 * e + b];
+        hgz[(size_t)e * 4 + a] = permz[ix] * di[4 * e + a] * (double)ivol[e];
+      }
+      hvol[e] = volu_[e];
+      double s = 0.0;
+      for (int k = 0; k < 4; ++k) s = s + pnodi_[tetra[5 * e + k] - 1];
+      pel[e] = s * 0.25;  // picuns.f:48 NODELT(PNODI)
+      for (int k = 0; k < 4; ++k) icnt[v[k] + 1]++;
+    }
+    for (int r = 0; r < n; ++r) icnt[r + 1] += icnt[r];
+    std::vector<int> hinc((size_t)4 * nt), fill(icnt.begin(), icnt.end() - 1);
+    std::vector<uint32_t> hoff((size_t)4 * nt);
+    for (int e = 0; e < nt; ++e) {  // ascending e per row
+      const int v[4] = {hen[e].x, hen[e].y, hen[e].z, hen[e].w};
+      for (int k = 0; k < 4; ++k) {
+        const int r = v[k];
+        const int i = fill[r]++;
+        hinc[i] = e * 4 + k;
+        uint32_t pk = 0;
+        for (int l = 0; l < 4; ++l) {
+          const int *b = &h_col[h_rowptr[r]], *en = &h_col[h_rowptr[r + 1]];
+          const int *f = std::lower_bound(b, en, v[l]);
+          pk |= (uint32_t)(f - b) << (8 * l);
+        }
+        hoff[i] = pk;
+      }
+    }
+    // COEF4 (porosity mass, constant): lumped diagonal or full (locmas.f)
+    double lm[16];
+    for (int k = 0; k < 4; ++k)
+      for (int l = 0; l < 4; ++l)
+        lm[k * 4 + l] = lump ? (k == l ? 0.25 : 0.0) : (k == l ? 0.1 : 0.05);
+    std::vector<double> hm4d(n, 0.0), hm4f;
+    if (!lump) hm4f.assign(nnz, 0.0);
+    for (int r = 0; r < n; ++r)
+      for (int i = icnt[r]; i < icnt[r + 1]; ++i) {
+        const int e = hinc[i] >> 2, k = hinc[i] & 3;
+        const double ve = volu_[e] * pel[e];
+        if (lump) {
+          hm4d[r] = hm4d[r] + ve * lm[k * 4 + k];
+        } else {
+          for (int l = 0; l < 4; ++l) {
+            const int o = (hoff[i] >> (8 * l)) & 0xff;
+            hm4f[h_rowptr[r] + o] = hm4f[h_rowptr[r] + o] + ve * lm[k * 4 + l];
+          }
+        }
+      }
+    enode.upload(hen, st);
+    S.upload(hS, st);
+    gz.upload(hgz, st);
+    volu.upload(hvol, st);
+    inc_ptr.upload(icnt, st);
+    inc.upload(hinc, st);
+    incoff.upload(hoff, st);
+    m4d.upload(hm4d, st);
+    if (!lump) m4full.upload(hm4f, st);
+    else m4full.alloc(1);
+    lmass.upload(std::vector<double>(lm, lm + 16), st);
+    CG_HIP(hipStreamSynchronize(st));
+  }
+
+  void build_nodes(const double *snodi_, const double *pnodi_, const double *vgn_, const double *vgrmc_,
+                   const double *vgpsat_, const double *volnod_, const double *kznod_,
+                   const double *arenod_) {
+    std::vector<double> a_psat(n), a_vgn(n), a_vgm(n), a_vgn1(n), a_vgnr(n), a_vgmr(n), a_vgpsn(n),
+        a_pnot(n), a_rmcp(n), a_snodi(n), a_pnodi(n), a_volnod(n), a_kz(n);
+    std::vector<int> hrefid(n), hrow_surf(n, -1), hsurf_row(nnod), hbot_row(nnod);
+    for (int ref = 0; ref < n; ++ref) {
+      const int d = perm[ref];
+      const double g = vgn_[ref];
+      CG_REQUIRE(g > 1.0 && pnodi_[ref] > 0.0, "vG n must exceed 1 and porosity be positive");
+      const double m = (g - 1.0) / g;  // chparm.f:24-31
+      a_vgn[d] = g;
+      a_vgm[d] = m;
+      a_vgn1[d] = g - 1.0;
+      a_vgnr[d] = 1.0 / g;
+      a_vgmr[d] = 1.0 / m;
+      a_vgpsn[d] = std::pow(std::fabs(vgpsat_[ref]), g);
+      a_psat[d] = vgpsat_[ref];
+      a_pnot[d] = (pnodi_[ref] - vgrmc_[ref]) / pnodi_[ref];
+      a_rmcp[d] = vgrmc_[ref] / pnodi_[ref];
+      a_snodi[d] = snodi_[ref];
+      a_pnodi[d] = pnodi_[ref];
+      a_volnod[d] = volnod_[ref];
+      a_kz[d] = kznod_ ? kznod_[ref] : 0.0;
+      hrefid[d] = ref;
+      if (ref < nnod) hrow_surf[d] = ref;
+    }
+    for (int s = 0; s < nnod; ++s) {
+      hsurf_row[s] = perm[s];
+      hbot_row[s] = perm[(long)nstr * nnod + s];
+    }
+    psat.upload(a_psat, st); vgn.upload(a_vgn, st); vgm.upload(a_vgm, st); vgn1.upload(a_vgn1, st);
+    vgnr.upload(a_vgnr, st); vgmr.upload(a_vgmr, st); vgpsn.upload(a_vgpsn, st); pnot.upload(a_pnot, st);
+    rmcp.upload(a_rmcp, st); snodi.upload(a_snodi, st); pnodi.upload(a_pnodi, st);
+    volnod.upload(a_volnod, st); kznod.upload(a_kz, st);
+    refid.upload(hrefid, st); row_surf.upload(hrow_surf, st); surf_row.upload(hsurf_row, st);
+    bot_row.upload(hbot_row, st);
+    arenod.upload(std::vector<double>(arenod_, arenod_ + nnod), st);
+    CG_HIP(hipStreamSynchronize(st));
+  }
+
+  void alloc_state() {
+    for (DBuf<double> *b : {&pnew, &pold, &ptnew, &ptold, &ptimep, &pdiff, &sw, &ckrw, &ckrwp, &et1, &et2,
+                            &swnew, &swtimep, &tnoti, &xt5, &bcflux, &qnode, &dinv, &r, &p, &Bv, &z})
+      b->alloc(n);
+    ckrwe.alloc(nt);
+    et1e.alloc(nt);
+    A.alloc(nnz);
+    isdir.alloc(n);
+    ifatm.alloc(nnod); ifatmp.alloc(nnod);
+    atmpot.alloc(nnod); atmact.alloc(nnod); atmold.alloc(nnod); qbot.alloc(nnod);
+    if (f32) F32.alloc(nnz); else F64.alloc(nnz);
+    part.alloc((size_t)kSlots * kMaxRedBlocks);
+    ipart.alloc(kMaxRedBlocks);
+    scal.alloc(1);
+    flags.alloc(2);
+    res.alloc(16);
+    h_res.assign(16, 0.0);
+    CG_HIP(hipMemsetAsync(qnode.p, 0, sizeof(double) * n, st));
+    CG_HIP(hipMemsetAsync(flags.p, 0, sizeof(int) * 2, st));
+  }
+
+  // ------------------------------------------------------- kernels glue --
+  void precond_apply(const double *tin, double *out, const int *done) {
+    // z = (L L^T)^-1 t   (proddp.f)
+    for (size_t l = 0; l < sched.lv.size(); ++l) {
+      const LevelDesc &L = sched.lv[l];
+      const int g = grid_for(L.nchains, 64);
+      if (f32)
+        hipLaunchKernelGGL(k_sweep_fwd<float>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
+                           diagpos.p, F32.p, dinv.p, tin, out, done);
+      else
+        hipLaunchKernelGGL(k_sweep_fwd<double>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
+                           diagpos.p, F64.p, dinv.p, tin, out, done);
+    }
+    for (size_t l = sched.lv.size(); l-- > 0;) {
+      const LevelDesc &L = sched.lv[l];
+      const int g = grid_for(L.nchains, 64);
+      if (f32)
+        hipLaunchKernelGGL(k_sweep_bwd<float>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
+                           diagpos.p, F32.p, dinv.p, out, done);
+      else
+        hipLaunchKernelGGL(k_sweep_bwd<double>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
+                           diagpos.p, F64.p, dinv.p, out, done);
+    }
+  }
+
+  void factor() {
+    for (size_t l = 0; l < sched.lv.size(); ++l) {
+      const LevelDesc &L = sched.lv[l];
+      const int g = grid_for(L.nchains, 64);
+      if (f32)
+        hipLaunchKernelGGL(k_ic_factor<float>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
+                           diagpos.p, tpos.p, A.p, isdir.p, F32.p, dinv.p, flags.p + 1);
+      else
+        hipLaunchKernelGGL(k_ic_factor<double>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
+                           diagpos.p, tpos.p, A.p, isdir.p, F64.p, dinv.p, flags.p + 1);
+    }
+  }
+
+  // SYMSLV on the current A / TNOTI: x = pdiff.  Returns NITER; err out.
+  int pcg(double tol, int imax, double *err_out, int expect) {
+    const int G = red_grid(n);
+    factor();
+    int *done = flags.p;
+    precond_apply(tnoti.p, pdiff.p, nullptr);  // x0 = M^-1 b (symslv :4686)
+    hipLaunchKernelGGL(k_residual0, dim3(G), dim3(kBlock), 0, st, n, rowptr.p, col.p, A.p, pdiff.p,
+                       tnoti.p, isdir.p, r.p, P(0));
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kBlock), 0, st, G, P(0), scal.p, done, tol, imax);
+    precond_apply(r.p, p.p, nullptr);  // p = M^-1 r
+    int batch = std::max(1, std::min(expect, imax));
+    int launched = 0;
+    int h_done = 0;
+    PcgScal hs{};
+    while (true) {
+      for (int b = 0; b < batch; ++b) {
+        hipLaunchKernelGGL(k_spmv_dot2, dim3(G), dim3(kBlock), 0, st, n, rowptr.p, col.p, A.p, p.p, r.p,
+                           Bv.p, P(1), P(2), done);
+        hipLaunchKernelGGL(k_pcg_alpha, dim3(G), dim3(kBlock), 0, st, n, G, P(1), P(2), Bv.p, p.p,
+                           isdir.p, r.p, pdiff.p, P(3), scal.p, done);
+        precond_apply(r.p, z.p, done);
+        hipLaunchKernelGGL(k_dot, dim3(G), dim3(kBlock), 0, st, n, Bv.p, z.p, P(4), done);
+        hipLaunchKernelGGL(k_pcg_beta, dim3(G), dim3(kBlock), 0, st, n, G, P(4), P(3), z.p, p.p, scal.p,
+                           done);
+      }
+      launched += batch;
+      CG_HIP(hipMemcpyAsync(&h_done, done, sizeof(int), hipMemcpyDeviceToHost, st));
+      CG_HIP(hipMemcpyAsync(&hs, scal.p, sizeof(PcgScal), hipMemcpyDeviceToHost, st));
+      CG_HIP(hipStreamSynchronize(st));
+      if (h_done) break;
+      CG_REQUIRE(launched <= imax + 2, "PCG did not stop");
+      batch = std::max(2, std::min(8, imax - hs.niter + 1));
+    }
+    *err_out = hs.err;
+    return hs.niter;
+  }
+
+  // ------------------------------------------------ Picard iteration --
+  void picard_iteration(double &err) {
+    const int gn = grid_for(n), ge = grid_for(nt);
+    hipLaunchKernelGGL(k_constitutive, dim3(gn), dim3(kBlock), 0, st, n, ptnew.p, pnew.p, soil(), sw.p,
+                       ckrw.p, et1.p, et2.p, swnew.p);
+    hipLaunchKernelGGL(k_elem_avg, dim3(ge), dim3(kBlock), 0, st, nt, enode.p, ckrw.p, et1.p, ckrwe.p,
+                       et1e.p);
+    hipLaunchKernelGGL(k_bc_rows, dim3(gn), dim3(kBlock), 0, st, n, row_surf.p, qnode.p, ifatm.p, atmact.p,
+                       atmold.p, prm.tetaf, isdir.p, bcflux.p);
+    AsmArgs a{};
+    a.n = n; a.rowptr = rowptr.p; a.col = col.p; a.diagpos = diagpos.p; a.inc_ptr = inc_ptr.p;
+    a.inc = inc.p; a.incoff = incoff.p; a.S = S.p; a.gz = gz.p; a.volu = volu.p; a.ckrwe = ckrwe.p;
+    a.et1e = et1e.p; a.m4d = m4d.p; a.m4full = m4full.p; a.lmass = lmass.p; a.ptnew = ptnew.p;
+    a.pnew = pnew.p; a.ptimep = ptimep.p; a.swnew = swnew.p; a.swtimep = swtimep.p; a.et2 = et2.p;
+    a.isdir = isdir.p; a.bcflux = bcflux.p; a.tetaf = prm.tetaf; a.rdt = 1.0 / deltat; a.A = A.p;
+    a.tnoti = tnoti.p; a.xt5 = xt5.p;
+    const int ga = grid_for(n, kAsmBlock);
+    if (lump) hipLaunchKernelGGL(k_assemble<true>, dim3(ga), dim3(kAsmBlock), 0, st, a);
+    else hipLaunchKernelGGL(k_assemble<false>, dim3(ga), dim3(kAsmBlock), 0, st, a);
+    CG_HIP(hipEventRecord(ev0, st));
+    const int expect = last_pcg > 0 ? last_pcg : 8;
+    int niter = pcg(prm.tolcg, prm.itmxcg, &err, expect);
+    CG_HIP(hipEventRecord(ev1, st));
+    last_pcg = niter;
+    itlin += niter;
+    nitert += niter;
+    c_pcg += niter;
+    if (niter >= prm.itmxcg) {
+      lsfail = 1;
+      klsfai += 1;
+    } else {
+      lsfail = 0;
+    }
+    hipLaunchKernelGGL(k_update_pnew, dim3(gn), dim3(kBlock), 0, st, n, pdiff.p, row_surf.p, ifatm.p, pold.p,
+                       pnew.p);
+  }
+  int last_pcg = 0;
+
+  // MASBAL (masbal.f:37-107) + NORMS (norms.f); result in h_res.
+  void masbal_norms() {
+    const int gs = grid_for(nnod);
+    hipLaunchKernelGGL(k_bkpic, dim3(gs), dim3(kBlock), 0, st, nnod, surf_row.p, rowptr.p, col.p, A.p,
+                       pdiff.p, xt5.p, ifatm.p, atmold.p, prm.tetaf, atmact.p);
+    const int G = red_grid(n), Gf = red_grid(nnod);
+    NormPart np{P(5), P(6), P(7), P(8), P(9), P(10), ipart.p};
+    hipLaunchKernelGGL(k_norms, dim3(G), dim3(kBlock), 0, st, n, refid.p, pnew.p, pold.p, tnoti.p, swnew.p,
+                       swtimep.p, ptimep.p, volnod.p, snodi.p, pnodi.p, np);
+    hipLaunchKernelGGL(k_flux, dim3(Gf), dim3(kBlock), 0, st, nnod, ifatm.p, atmact.p, qbot.p,
+                       prm.free_drainage ? nnod : 0, P(11));
+    hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(kBlock), 0, st, G, Gf, np, P(11), res.p);
+    CG_HIP(hipMemcpyAsync(h_res.data(), res.p, sizeof(double) * 16, hipMemcpyDeviceToHost, st));
+    CG_HIP(hipStreamSynchronize(st));
+    const double *R = h_res.data();
+    dstore = R[0];
+    pl2 = R[1]; fl2 = R[2]; finf = R[3]; pinf = R[4]; pikmax = R[5]; ikmax = (int)R[6];
+    const double adin = R[7], adout = R[8], anin = R[9], anout = R[10], nnin = R[11], nnout = R[12];
+    const double ndin = 0.0, ndout = 0.0;
+    const double deltam = 0.5 * deltat;
+    const double vadin = (adin + adinp) * deltam, vadout = (adout + adoutp) * deltam;
+    const double vndin = (ndin + ndinp) * deltam, vndout = (ndout + ndoutp) * deltam;
+    const double vanin = (anin + aninp) * deltam, vanout = (anout + anoutp) * deltam;
+    const double vnnin = (nnin + nninp) * deltam, vnnout = (nnout + nnoutp) * deltam;
+    vin = vadin + vndin + vanin + vnnin;
+    vout = vadout + vndout + vanout + vnnout;
+    erras = vin + vout - dstore;
+    errel = (vin + vout) != 0.0 ? 100.0 * erras / (vin + vout) : 0.0;
+    cur[0] = adin; cur[1] = adout; cur[2] = ndin; cur[3] = ndout;
+    cur[4] = anin; cur[5] = anout; cur[6] = nnin; cur[7] = nnout;
+  }
+
+  void switch_old() {
+    hipLaunchKernelGGL(k_switch_old, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, surf_row.p, ifatm.p,
+                       atmact.p, atmpot.p, pnew.p, prm.pmin);
+  }
+
+  // FLOW3D, Picard branch (flow3d.f:93-294).
+  void flow3d() {
+    const int gn = grid_for(n);
+    while (true) {
+      double err = 0.0;
+      picard_iteration(err);
+      c_picard += 1;
+      masbal_norms();
+      if ((prm.l2norm == 0 && pinf <= prm.tolswi) || (prm.l2norm != 0 && pl2 <= prm.tolswi)) switch_old();
+      bckstp = 0;
+      const bool itagen = iter < prm.ituns;
+      const bool errgmx = pl2 >= prm.ernlmx || pinf >= prm.ernlmx || fl2 >= prm.ernlmx || finf >= prm.ernlmx;
+      const bool normcv = prm.l2norm == 0 ? pinf <= prm.toluns : pl2 <= prm.toluns;
+      if (!lsfail && !errgmx && !normcv && itagen) {
+        hipLaunchKernelGGL(k_next_iter, dim3(gn), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
+                           pold.p, ptnew.p, ptold.p);
+        iter += 1;
+        continue;
+      }
+      itrtot += iter;
+      if (!lsfail && !errgmx && normcv) {
+        noback = 0;
+      } else if (dtgmin) {
+        bckstp = 1;
+        noback = 0;
+      } else {
+        noback = 1;
+      }
+      return;
+    }
+  }
+
+  // ---------------------------------------------- forcing (atmbc file) --
+  int arow() const { return prm.hspatm ? 1 : nnod; }
+  bool atm_read(double &tin, double *vals) {
+    if (atm_next >= (int)atm_t.size()) return false;
+    const int k = atm_next++;
+    tin = atm_t[k];
+    const int w = arow();
+    for (int i = 0; i < w; ++i) vals[i] = atm_v[(size_t)k * w + i];
+    return true;
+  }
+  void atm_shift_read() {  // atmnxt.f:30-40
+    const int w = arow();
+    double *a1 = atminp.data(), *a2 = a1 + w, *a3 = a2 + w;
+    while (!(time <= atmtim[2])) {
+      atmtim[0] = atmtim[1];
+      atmtim[1] = atmtim[2];
+      for (int i = 0; i < w; ++i) { a1[i] = a2[i]; a2[i] = a3[i]; }
+      double tin;
+      if (!atm_read(tin, a3)) { htiatm = 1; break; }
+      atmtim[2] = tin;
+    }
+  }
+  // rate(s) at `time` from window slots i0,i1 (atmnxt.f:41-73 / atmbak.f)
+  void atm_apply(int i0, int i1, double t0, double t1, bool interp, int set_act) {
+    const int w = arow();
+    std::vector<double> rate(w);
+    const double *a = atminp.data() + (size_t)i0 * w, *b = atminp.data() + (size_t)i1 * w;
+    for (int i = 0; i < w; ++i) {
+      if (interp) {
+        double slope = (b[i] - a[i]) / (t1 - t0);
+        if (prm.ieto != 0) slope = 0.0;
+        rate[i] = a[i] + slope * (time - t0);
+      } else {
+        rate[i] = b[i];
+      }
+    }
+    const double *rn = nullptr;
+    if (w > 1) {
+      rate_dev.upload(rate, st);
+      rn = rate_dev.p;
+    }
+    hipLaunchKernelGGL(k_atm_update, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, rate[0], rn, arenod.p,
+                       ifatm.p, prm.scf, set_act, atmpot.p, atmact.p);
+  }
+  DBuf<double> rate_dev;
+  void atmnxt() {
+    if (htiatm != 0) return;
+    atm_shift_read();
+    if (atmtim[2] > atmtim[1]) atm_apply(1, 2, atmtim[1], atmtim[2], true, 1);
+    else atm_apply(2, 2, 0, 0, false, 1);
+  }
+  void atmbak() {
+    if (atmtim[0] >= atmtim[1]) return;
+    atm_apply(0, 1, atmtim[0], atmtim[1], true, 1);
+  }
+
+  void free_drainage(const double *ck) {
+    if (!prm.free_drainage) return;
+    hipLaunchKernelGGL(k_free_drainage, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, bot_row.p, arenod.p,
+                       ck, kznod.p, qbot.p, qnode.p);
+  }
+
+  double device_sum_store() {
+    const int G = red_grid(n);
+    hipLaunchKernelGGL(k_store, dim3(G), dim3(kBlock), 0, st, n, sw.p, volnod.p, pnodi.p, P(17));
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, st, G, P(17), res.p + 15);
+    double v = 0;
+    CG_HIP(hipMemcpyAsync(&v, res.p + 15, sizeof(double), hipMemcpyDeviceToHost, st));
+    CG_HIP(hipStreamSynchronize(st));
+    return v;
+  }
+
+  // -------------------------------------------------- run initialisation --
+  // inital.f (INIT1, ATMONE, MBINIT) + cathy_main.f:2663-2704.
+  void init(const double *ptimep_ref) {
+    CG_REQUIRE(!atm_t.empty(), "forcing not set");
+    std::vector<double> h(n);
+    for (int i = 0; i < n; ++i) h[perm[i]] = ptimep_ref[i];
+    ptimep.upload(h, st);
+    // INIT1 (init1.f:37-60)
+    deltat = prm.deltat;
+    if (deltat > prm.dtmax) deltat = prm.dtmax;
+    if (deltat <= prm.dtmin) { deltat = prm.dtmin; dtgmin = 0; } else dtgmin = 1;
+    timep = 0.0;
+    time = deltat;
+    iter = 1; nstep = 1; nitert = 0; itlin = 0; itrtot = 0; klsfai = 0;
+    finished = 0; noback = 0; bckstp = 0;
+    store1 = store2 = dstore = cdstor = cvin = cvout = vtot = cerras = caeras = 0.0;
+    CG_HIP(hipMemcpyAsync(pnew.p, ptimep.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    // ATMONE (atmone.f:39-163), host side on the surface values
+    const int w = arow();
+    atminp.assign((size_t)3 * w, 0.0);
+    atm_next = 0; htiatm = 0;
+    atmtim[0] = atmtim[1] = atmtim[2] = 0.0;
+    std::vector<double> hp(n);
+    pnew.download(hp.data(), n, st);
+    CG_HIP(hipStreamSynchronize(st));
+    std::vector<int> fa(nnod, 0), fap(nnod, 0);
+    std::vector<double> apot(nnod, 0.0), aact(nnod, 0.0), aold(nnod, 0.0);
+    std::vector<double> are(nnod);
+    arenod.download(are.data(), nnod, st);
+    CG_HIP(hipStreamSynchronize(st));
+    double *a3 = atminp.data() + 2 * (size_t)w;
+    double tin;
+    auto rate_at = [&](int slot, int s) { return atminp[(size_t)slot * w + (w > 1 ? s : 0)]; };
+    if (!atm_read(tin, a3)) {
+      htiatm = 1;
+    } else {
+      atmtim[2] = tin;
+      if (atmtim[2] <= 0.0)
+        for (int s = 0; s < nnod; ++s) aold[s] = rate_at(2, s) * are[s];
+      atm_shift_read();
+      for (int s = 0; s < nnod; ++s) {
+        double rt;
+        if (atmtim[2] > atmtim[1]) {
+          double slope = (rate_at(2, s) - rate_at(1, s)) / (atmtim[2] - atmtim[1]);
+          if (prm.ieto != 0) slope = 0.0;
+          rt = rate_at(1, s) + slope * (time - atmtim[1]);
+        } else {
+          rt = rate_at(2, s);
+        }
+        apot[s] = rt * are[s];
+      }
+    }
+    std::vector<double> hpt = hp;  // PTIMEP == PNEW here
+    for (int s = 0; s < nnod; ++s) {
+      const int d = perm[s];
+      if (hp[d] >= prm.pondh_min) { fa[s] = 2; fap[s] = 2; continue; }
+      if (hp[d] >= 0.0 && apot[s] > 0.0) fa[s] = 1;
+      if (hpt[d] >= 0.0 && aold[s] > 0.0) fap[s] = 1;
+      if (hp[d] <= prm.pmin && apot[s] < 0.0) { hp[d] = prm.pmin; fa[s] = 1; }
+      if (hpt[d] <= prm.pmin && aold[s] < 0.0) { hpt[d] = prm.pmin; fap[s] = 1; }
+    }
+    ptimep.upload(hpt, st);
+    for (int s = 0; s < nnod; ++s) {
+      if (fa[s] == 0) aact[s] = apot[s] >= 0.0 ? apot[s] : (1.0 - prm.scf) * apot[s];
+      else aact[s] = 0.0;
+      if (fap[s] == 1 || fap[s] == 2) aold[s] = 0.0;
+    }
+    pnew.upload(hp, st);
+    ifatm.upload(fa, st); ifatmp.upload(fap, st);
+    atmpot.upload(apot, st); atmact.upload(aact, st); atmold.upload(aold, st);
+    const int gn = grid_for(n);
+    hipLaunchKernelGGL(k_start_iter, dim3(gn), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p, pold.p,
+                       ptnew.p, ptold.p);
+    // MBINIT (mbinit.f:52-74)
+    adinp = adoutp = ndinp = ndoutp = 0.0;
+    aninp = anoutp = 0.0;
+    for (int s = 0; s < nnod; ++s)
+      if (fap[s] == 0) {
+        if (aold[s] > 0.0) aninp += aold[s]; else anoutp += aold[s];
+      }
+    // CHVELO(PTIMEP), STORCAL, NEUMANN
+    hipLaunchKernelGGL(k_sw_kr, dim3(gn), dim3(kBlock), 0, st, n, ptimep.p, soil(), sw.p, ckrw.p);
+    store1 = device_sum_store();
+    store2 = store1;
+    CG_HIP(hipMemcpyAsync(ckrwp.p, ckrw.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    nninp = nnoutp = 0.0;
+    if (prm.free_drainage) {
+      free_drainage(ckrw.p);
+      std::vector<double> q(nnod);
+      qbot.download(q.data(), nnod, st);
+      CG_HIP(hipStreamSynchronize(st));
+      for (int s = 0; s < nnod; ++s) { if (q[s] > 0.0) nninp += q[s]; else nnoutp += q[s]; }
+    }
+    // SWTIMEP is a function of PTIMEP only
+    hipLaunchKernelGGL(k_sw_kr, dim3(gn), dim3(kBlock), 0, st, n, ptimep.p, soil(), swtimep.p,
+                       (double *)nullptr);
+    CG_HIP(hipStreamSynchronize(st));
+    initialised = 1;
+  }
+
+  void bkstep() {  // bkstep.f:58-147 (constant non-atmospheric BCs, NSF=0)
+    CG_HIP(hipMemcpyAsync(pnew.p, ptimep.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    CG_HIP(hipMemcpyAsync(ifatm.p, ifatmp.p, sizeof(int) * nnod, hipMemcpyDeviceToDevice, st));
+    CG_HIP(hipMemcpyAsync(atmact.p, atmold.p, sizeof(double) * nnod, hipMemcpyDeviceToDevice, st));
+    time = timep;
+    deltat = deltat * prm.dtredm - prm.dtreds;
+    if (deltat <= prm.dtmin) { deltat = prm.dtmin; dtgmin = 0; } else dtgmin = 1;
+    time = time + deltat;
+    iter = 1;
+    nitert = 0;
+    if (time > atmtim[1]) atmnxt(); else atmbak();
+    switch_old();
+    hipLaunchKernelGGL(k_start_iter, dim3(grid_for(n)), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
+                       pold.p, ptnew.p, ptold.p);
+  }
+
+  // One accepted step of cathy_main.f:2882-3830.
+  int advance(double *stv) {
+    CG_REQUIRE(initialised, "flow3d_init not called");
+    if (finished) return finished;
+    const auto t0 = std::chrono::steady_clock::now();
+    const double pcg0 = c_pcg, pic0 = c_picard;
+    free_drainage(ckrw.p);
+    atmnxt();
+    switch_old();
+    hipLaunchKernelGGL(k_start_iter, dim3(grid_for(n)), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
+                       pold.p, ptnew.p, ptold.p);
+    int backs = 0;
+    while (true) {
+      flow3d();
+      if (!bckstp) break;
+      bkstep();
+      free_drainage(ckrwp.p);
+      ++backs;
+    }
+    // CHVELO, STORCAL, cumulative balance (cathy_main.f:3233-3262)
+    hipLaunchKernelGGL(k_sw_kr, dim3(grid_for(n)), dim3(kBlock), 0, st, n, pnew.p, soil(), sw.p, ckrw.p);
+    store1 = device_sum_store();
+    store2 = store2 + dstore;
+    vtot += vin + vout;
+    cvin += vin;
+    cvout += vout;
+    cdstor += dstore;
+    cerras += erras;
+    caeras += std::fabs(erras);
+    if (stv) {
+      const double v[28] = {(double)nstep, deltat, time, (double)iter, (double)nitert, (double)backs,
+                            store1, store2, dstore, cdstor, vin, cvin, vout, cvout, vin + vout, vtot, erras,
+                            errel, cerras, caeras, pinf, (double)itrtot, (double)itlin, (double)klsfai, pl2,
+                            finf, fl2, (double)(ikmax + 1)};
+      for (int i = 0; i < 28; ++i) stv[i] = v[i];
+      for (int i = 28; i < 32; ++i) stv[i] = 0.0;
+    }
+    adinp = cur[0]; adoutp = cur[1]; ndinp = cur[2]; ndoutp = cur[3];
+    aninp = cur[4]; anoutp = cur[5]; nninp = cur[6]; nnoutp = cur[7];
+    CG_HIP(hipMemcpyAsync(ifatmp.p, ifatm.p, sizeof(int) * nnod, hipMemcpyDeviceToDevice, st));
+    CG_HIP(hipMemcpyAsync(atmold.p, atmact.p, sizeof(double) * nnod, hipMemcpyDeviceToDevice, st));
+    CG_HIP(hipMemcpyAsync(ptimep.p, pnew.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    CG_HIP(hipMemcpyAsync(ckrwp.p, ckrw.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_sw_kr, dim3(grid_for(n)), dim3(kBlock), 0, st, n, ptimep.p, soil(), swtimep.p,
+                       (double *)nullptr);
+    CG_HIP(hipStreamSynchronize(st));
+    last_counters[0] = c_picard - pic0;
+    last_counters[1] = c_pcg - pcg0;
+    last_counters[4] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (noback) return finished = 2;
+    if (std::fabs(time - prm.tmax) <= 0.001 * deltat) return finished = 1;
+    dtgmin = timupd();
+    nstep += 1;
+    iter = 1;
+    nitert = 0;
+    return 0;
+  }
+  double last_counters[6] = {0};
+
+  int timupd() {  // timupd.f:21-40
+    timep = time;
+    if (iter < prm.ituns1) {
+      deltat = deltat * prm.dtmagm + prm.dtmaga;
+      if (deltat > prm.dtmax) deltat = prm.dtmax;
+    }
+    if (iter >= prm.ituns2) {
+      deltat = deltat * prm.dtredm - prm.dtreds;
+      if (deltat < prm.dtmin) deltat = prm.dtmin;
+    }
+    if ((time + deltat) >= prm.tmax) {
+      deltat = prm.tmax - time;
+      time = prm.tmax;
+    } else {
+      if ((time + 2 * deltat) > prm.tmax) deltat = (prm.tmax - time) / 2;
+      time = time + deltat;
+    }
+    return deltat <= prm.dtmin ? 0 : 1;
+  }
+
+  void get_state(double *pn, double *swo, double *ck, int *fa) {
+    std::vector<double> h(n);
+    auto fetch = [&](const DBuf<double> &b, double *out) {
+      b.download(h.data(), n, st);
+      CG_HIP(hipStreamSynchronize(st));
+      for (int i = 0; i < n; ++i) out[i] = h[perm[i]];
+    };
+    if (pn) fetch(pnew, pn);
+    if (swo) fetch(sw, swo);
+    if (ck) fetch(ckrw, ck);
+    if (fa) {
+      ifatm.download(fa, nnod, st);
+      CG_HIP(hipStreamSynchronize(st));
+    }
+  }
+};
+
+// ----------------------------------------------------------- handles --
+static std::mutex g_mu;
+static std::map<int64_t, std::unique_ptr<Flow>> g_flows;
+static int64_t g_next = 1;
+
+static Flow *lookup(const int64_t *h) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!h) throw Error(CATHY_GPU_EHANDLE, "null handle");
+  auto it = g_flows.find(*h);
+  if (it == g_flows.end()) throw Error(CATHY_GPU_EHANDLE, "unknown handle");
+  return it->second.get();
+}
+
+template <typename F>
+static int guarded(F &&f) {
+  try {
+    return f();
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return CATHY_GPU_ENOMEM;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return CATHY_GPU_EINVAL;
+  }
+}
+
+}  // namespace cg
+
+using namespace cg;
+
+extern "C" {
+
+const char *cathy_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int cathy_gpu_device_info(char *name, int *name_len, int *cu_count) {
+  return guarded([&] {
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt < 1) throw Error(CATHY_GPU_EDEVICE, "no HIP device");
+    hipDeviceProp_t pr;
+    CG_HIP(hipGetDeviceProperties(&pr, 0));
+    if (name && name_len && *name_len > 0) {
+      std::strncpy(name, pr.gcnArchName, *name_len - 1);
+      name[*name_len - 1] = 0;
+    }
+    if (cu_count) *cu_count = pr.multiProcessorCount;
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_create(int64_t *handle, const int *n, const int *nt, const int *ntri, const int *nnod,
+                            const int *nstr, const int *nzone, const int *maxstr, const int *tetra,
+                            const double *bi, const double *ci, const double *di, const double *volu,
+                            const int *ivol, const double *volnod, const double *arenod, const double *permx,
+                            const double *permy, const double *permz, const double *snodi,
+                            const double *pnodi, const double *vgn, const double *vgrmc,
+                            const double *vgpsat, const double *kznod, const int *lump,
+                            const int *ordering) {
+  return guarded([&] {
+    CG_REQUIRE(handle && n && nt && ntri && nnod && nstr && nzone && maxstr && tetra && bi && ci && di &&
+                   volu && ivol && volnod && arenod && permx && permy && permz && snodi && pnodi && vgn &&
+                   vgrmc && vgpsat && lump && ordering,
+               "null argument");
+    auto f = std::make_unique<Flow>();
+    f->setup(*n, *nt, *ntri, *nnod, *nstr, *nzone, *maxstr, tetra, bi, ci, di, volu, ivol, volnod, arenod,
+             permx, permy, permz, snodi, pnodi, vgn, vgrmc, vgpsat, kznod, *lump, *ordering);
+    std::lock_guard<std::mutex> lk(g_mu);
+    *handle = g_next++;
+    g_flows[*handle] = std::move(f);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_destroy(int64_t *handle) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!handle || !g_flows.count(*handle)) throw Error(CATHY_GPU_EHANDLE, "unknown handle");
+    g_flows.erase(*handle);
+    *handle = 0;
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_set_params(int64_t *handle, const int *ip, const double *dp) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(ip && dp, "null argument");
+    Params &p = f->prm;
+    p.iopt = ip[0]; p.l2norm = ip[1]; p.ituns = ip[2]; p.ituns1 = ip[3]; p.ituns2 = ip[4];
+    p.itmxcg = ip[5]; p.ieto = ip[6]; p.free_drainage = ip[7]; p.hspatm = ip[8];
+    p.tetaf = dp[0]; p.toluns = dp[1]; p.tolswi = dp[2]; p.ernlmx = dp[3]; p.tolcg = dp[4];
+    p.deltat = dp[5]; p.dtmin = dp[6]; p.dtmax = dp[7]; p.tmax = dp[8]; p.dtmaga = dp[9];
+    p.dtmagm = dp[10]; p.dtreds = dp[11]; p.dtredm = dp[12]; p.pmin = dp[13]; p.pondh_min = dp[14];
+    p.scf = dp[15];
+    CG_REQUIRE(p.iopt == 1, "IOPT=2 (Newton) is not available in this build");
+    CG_REQUIRE(p.tetaf > 0.0 && p.deltat > 0.0 && p.itmxcg >= 1 && p.ituns >= 1, "bad solver parameters");
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_set_forcing(int64_t *handle, const int *natm, const double *atmtim,
+                                 const double *atmval) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(natm && *natm >= 1 && atmtim && atmval, "forcing needs at least one record");
+    const int w = f->prm.hspatm ? 1 : f->nnod;
+    f->atm_t.assign(atmtim, atmtim + *natm);
+    f->atm_v.assign(atmval, atmval + (size_t)(*natm) * w);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_init(int64_t *handle, const double *ptimep) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(ptimep, "null argument");
+    f->init(ptimep);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_advance(int64_t *handle, double *stats) {
+  return guarded([&] { return lookup(handle)->advance(stats); });
+}
+
+int cathy_gpu_flow3d_get_state(int64_t *handle, double *pnew, double *sw, double *ckrw, int *ifatm) {
+  return guarded([&] {
+    lookup(handle)->get_state(pnew, sw, ckrw, ifatm);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_set_pnew(int64_t *handle, const double *pnew) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(pnew, "null argument");
+    std::vector<double> h(f->n);
+    for (int i = 0; i < f->n; ++i) h[f->perm[i]] = pnew[i];
+    f->pnew.upload(h, f->st);
+    CG_HIP(hipStreamSynchronize(f->st));
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_counters(int64_t *handle, double *out6) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(out6, "null argument");
+    for (int i = 0; i < 6; ++i) out6[i] = f->last_counters[i];
+    return 0;
+  });
+}
+
+}  // extern "C"

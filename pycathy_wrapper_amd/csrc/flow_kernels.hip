@@ -1,0 +1,693 @@
+// flow_kernels.hip -- gfx950 kernels of one Picard iteration and of the
+// IC(0)-PCG solve (device numbering, see kernels.h).
+#include "flow_kernels.h"
+
+namespace cg {
+
+// ----------------------------------------------- constitutive (a2) --
+// chpic0.f:23-36: Sw, kr, ET1 = Sw*Ss, ET2 = VGPNOT*dSe/dpsi at PTNEW, and
+// SWNEW at PNEW.
+__global__ void k_constitutive(int n, const double *__restrict__ ptnew,
+                               const double *__restrict__ pnew, SoilSoA s,
+                               double *__restrict__ sw, double *__restrict__ ckrw,
+                               double *__restrict__ et1, double *__restrict__ et2,
+                               double *__restrict__ swnew) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double psat = s.psat[i], vgn = s.vgn[i], vgm = s.vgm[i];
+  double pnot = s.pnot[i], rmcp = s.rmcp[i];
+  swnew[i] = pnot * dev_fvgse(pnew[i], psat, vgn, vgm) + rmcp;
+  double psi = ptnew[i];
+  double se = dev_fvgse(psi, psat, vgn, vgm);
+  double swv = pnot * se + rmcp;
+  sw[i] = swv;
+  et1[i] = swv * s.snodi[i];
+  et2[i] = pnot * dev_fvgdse(psi, psat, vgn, s.vgn1[i], s.vgpsn[i], s.vgnr[i]);
+  ckrw[i] = dev_fvgkr(psi, se, s.vgmr[i], vgm);
+}
+
+// Sw at one head vector (SWTIMEP of chpic0.f:24-27, Sw of chvelo.f) and,
+// optionally, kr (chvelo.f).
+__global__ void k_sw_kr(int n, const double *__restrict__ psi_in, SoilSoA s,
+                        double *__restrict__ sw, double *__restrict__ ckrw) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double psi = psi_in[i];
+  double se = dev_fvgse(psi, s.psat[i], s.vgn[i], s.vgm[i]);
+  sw[i] = s.pnot[i] * se + s.rmcp[i];
+  if (ckrw) ckrw[i] = dev_fvgkr(psi, se, s.vgmr[i], s.vgm[i]);
+}
+
+// nodelt.f: element means of kr and ET1.
+__global__ void k_elem_avg(int nt, const int4 *__restrict__ enode,
+                           const double *__restrict__ ckrw, const double *__restrict__ et1,
+                           double *__restrict__ ckrwe, double *__restrict__ et1e) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+  int4 v = enode[e];
+  double a = 0.0;
+  a = a + ckrw[v.x]; a = a + ckrw[v.y]; a = a + ckrw[v.z]; a = a + ckrw[v.w];
+  ckrwe[e] = a * 0.25;
+  double b = 0.0;
+  b = b + et1[v.x]; b = b + et1[v.y]; b = b + et1[v.z]; b = b + et1[v.w];
+  et1e[e] = b * 0.25;
+}
+
+// bcpic.f:34-87 classification of each row: Dirichlet (atmospheric nodes
+// with IFATM = 1 or 2) or the flux it receives (Neumann Q, atmospheric
+// TETAF*ATMACT + (1-TETAF)*ATMOLD for IFATM = 0).
+__global__ void k_bc_rows(int n, const int *__restrict__ row_surf,
+                          const double *__restrict__ qnode, const int *__restrict__ ifatm,
+                          const double *__restrict__ atmact, const double *__restrict__ atmold,
+                          double tetaf, uint8_t *__restrict__ isdir,
+                          double *__restrict__ bcflux) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint8_t d = 0;
+  double f = qnode[r];
+  int s = row_surf[r];
+  if (s >= 0) {
+    int fa = ifatm[s];
+    if (fa == 1 || fa == 2) d = 1;
+    else if (fa == 0) f = f + (tetaf * atmact[s] + (1.0 - tetaf) * atmold[s]);
+  }
+  isdir[r] = d;
+  bcflux[r] = f;
+}
+
+// Upper-pair slot of (k,l), k <= l, in an element's 10 stiffness values.
+__device__ __forceinline__ int upair(int k, int l) {
+  int a = k < l ? k : l, b = k < l ? l : k;
+  return a * 4 - (a * (a - 1)) / 2 + (b - a);
+}
+
+// Row-gather assembly + RHS + CFMATP + RHSGRV + BCPIC right-hand side.
+// asspic.f:26-51 scattered element matrices; here each thread owns one row
+// and gathers the contributions of its incident elements (ascending
+// element id, the reference's accumulation order) into an LDS row buffer,
+// so the assembly is deterministic and atomic-free.
+//   K      = sum_e kr_e * S_e                       (COEF1 before CFMATP)
+//   TNOTI  = -K PTNEW - M2/dt (PNEW-PTIMEP) - M4/dt (SWNEW-SWTIMEP) - grav
+//   A      = TETAF*K + (M2 + diag(M4_rr ET2_r))/dt  (cfmatp.f)
+// Dirichlet rows keep their true coefficients (BKPIC needs them) and get
+// TNOTI = 0; the preconditioner masks them.
+template <bool LUMPED>
+__global__ __launch_bounds__(kAsmBlock) void k_assemble(AsmArgs a) {
+  __shared__ double Ks[kMaxRow * kAsmBlock];
+  __shared__ double Ms[LUMPED ? 1 : kMaxRow * kAsmBlock];
+  int r = blockIdx.x * kAsmBlock + threadIdx.x;
+  if (r >= a.n) return;
+  const int beg = a.rowptr[r], len = a.rowptr[r + 1] - beg;
+  double *K = Ks + threadIdx.x;
+  double *M = LUMPED ? nullptr : Ms + threadIdx.x;
+  for (int j = 0; j < len; ++j) {
+    K[j * kAsmBlock] = 0.0;
+    if (!LUMPED) M[j * kAsmBlock] = 0.0;
+  }
+  double m2d = 0.0, grav = 0.0;
+  const int i1 = a.inc_ptr[r + 1];
+  for (int i = a.inc_ptr[r]; i < i1; ++i) {
+    const int ek = a.inc[i];
+    const int e = ek >> 2, k = ek & 3;
+    const uint32_t off = a.incoff[i];
+    const double kr = a.ckrwe[e];
+    const double *se = a.S + (size_t)e * 10;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int o = (off >> (8 * l)) & 0xff;
+      K[o * kAsmBlock] += kr * se[upair(k, l)];
+    }
+    const double ve1 = a.volu[e] * a.et1e[e];
+    if (LUMPED) {
+      m2d = m2d + ve1 * 0.25;
+    } else {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const int o = (off >> (8 * l)) & 0xff;
+        M[o * kAsmBlock] += ve1 * a.lmass[k * 4 + l];
+      }
+    }
+    grav = grav + kr * a.gz[(size_t)e * 4 + k];
+  }
+  const int dg = a.diagpos[r] - beg;
+  const double rdt = a.rdt;
+  double t = 0.0;
+  for (int j = 0; j < len; ++j) t = t - K[j * kAsmBlock] * a.ptnew[a.col[beg + j]];
+  double m4r;
+  if (LUMPED) {
+    m4r = a.m4d[r];
+    t = t - (m2d * rdt) * (a.pnew[r] - a.ptimep[r]) - (m4r * rdt) * (a.swnew[r] - a.swtimep[r]);
+  } else {
+    m4r = a.m4full[beg + dg];
+    for (int j = 0; j < len; ++j) {
+      const int c = a.col[beg + j];
+      t = t - (M[j * kAsmBlock] * rdt) * (a.pnew[c] - a.ptimep[c]) -
+          (a.m4full[beg + j] * rdt) * (a.swnew[c] - a.swtimep[c]);
+    }
+  }
+  t = t - grav;
+  a.xt5[r] = t;
+  a.tnoti[r] = a.isdir[r] ? 0.0 : t + a.bcflux[r];
+  const double tetaf = a.tetaf;
+  const double m5 = m4r * a.et2[r];
+  for (int j = 0; j < len; ++j) {
+    double v = tetaf * K[j * kAsmBlock];
+    if (LUMPED) {
+      if (j == dg) v = v + m2d * rdt + m5 * rdt;
+    } else {
+      v = v + M[j * kAsmBlock] * rdt;
+      if (j == dg) v = v + m5 * rdt;
+    }
+    a.A[beg + j] = v;
+  }
+}
+template __global__ void k_assemble<true>(AsmArgs);
+template __global__ void k_assemble<false>(AsmArgs);
+
+// ------------------------------------------------------------ IC(0) --
+// incldp.f restated as an up-looking IC(0) on the full pattern: for each
+// row r in elimination order, L_rj = (a_rj - sum_{m<j} L_rm L_jm) / L_jj for
+// its lower entries j (ascending), L_rr = sqrt(a_rr - sum L_rj^2).  A chain
+// is a run of rows one thread eliminates in order; chains of one level are
+// independent.  Dirichlet rows/columns are removed (identity, zero inverse
+// diagonal) -- what the 1e-9*RMAX diagonal penalty of bcpic.f does in exact
+// arithmetic.  The factor's transpose is mirrored into the upper half so
+// both sweeps read rows.
+template <typename FT>
+__global__ void k_ic_factor(LevelDesc L, const int *__restrict__ lvl_rows,
+                            const int *__restrict__ rowptr, const int *__restrict__ col,
+                            const int *__restrict__ diagpos, const int *__restrict__ tpos,
+                            const double *__restrict__ A, const uint8_t *__restrict__ isdir,
+                            FT *__restrict__ F, double *__restrict__ dinv, int *nfall) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.nchains) return;
+  for (int k = 0; k < L.len; ++k) {
+    const int r = L.rows_off >= 0 ? lvl_rows[L.rows_off + t] : L.base + t + k * L.stride;
+    const int beg = rowptr[r], d = diagpos[r];
+    if (isdir[r]) {
+      for (int p = beg; p < d; ++p) {
+        F[p] = FT(0);
+        F[tpos[p]] = FT(0);
+      }
+      F[d] = FT(1);
+      dinv[r] = 0.0;
+      continue;
+    }
+    double sumsq = 0.0;
+    for (int p = beg; p < d; ++p) {
+      const int j = col[p];
+      double l = 0.0;
+      if (!isdir[j]) {
+        double s = A[p];
+        int q = beg, u = rowptr[j];
+        const int ue = diagpos[j];
+        while (q < p && u < ue) {
+          const int cq = col[q], cu = col[u];
+          if (cq < cu) ++q;
+          else if (cq > cu) ++u;
+          else {
+            s -= (double)F[q] * (double)F[u];
+            ++q;
+            ++u;
+          }
+        }
+        l = s * dinv[j];
+      }
+      F[p] = FT(l);
+      F[tpos[p]] = FT(l);
+      sumsq += l * l;
+    }
+    double av = A[d] - sumsq;
+    if (!(av > 0.0)) {  // breakdown: keep the row's own diagonal
+      av = fabs(A[d]) > 0.0 ? fabs(A[d]) : 1.0;
+      atomicAdd(nfall, 1);
+    }
+    const double lrr = sqrt(av);
+    F[d] = FT(lrr);
+    dinv[r] = 1.0 / lrr;
+  }
+}
+
+// Forward sweep L y = t (proddp, solscal-extended.f:4175-4185): rows in chain order.
+template <typename FT>
+__global__ void k_sweep_fwd(LevelDesc L, const int *__restrict__ lvl_rows,
+                            const int *__restrict__ rowptr, const int *__restrict__ col,
+                            const int *__restrict__ diagpos, const FT *__restrict__ F,
+                            const double *__restrict__ dinv, const double *__restrict__ tin,
+                            double *__restrict__ y, const int *__restrict__ done) {
+  if (done && *done) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.nchains) return;
+  for (int k = 0; k < L.len; ++k) {
+    const int r = L.rows_off >= 0 ? lvl_rows[L.rows_off + t] : L.base + t + k * L.stride;
+    double s = tin[r];
+    const int d = diagpos[r];
+    for (int p = rowptr[r]; p < d; ++p) s -= (double)F[p] * y[col[p]];
+    y[r] = s * dinv[r];
+  }
+}
+
+// Backward sweep L^T x = y (proddp, solscal-extended.f:4186-4197), in place, chains and
+// levels in reverse.
+template <typename FT>
+__global__ void k_sweep_bwd(LevelDesc L, const int *__restrict__ lvl_rows,
+                            const int *__restrict__ rowptr, const int *__restrict__ col,
+                            const int *__restrict__ diagpos, const FT *__restrict__ F,
+                            const double *__restrict__ dinv, double *__restrict__ x,
+                            const int *__restrict__ done) {
+  if (done && *done) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.nchains) return;
+  for (int k = L.len - 1; k >= 0; --k) {
+    const int r = L.rows_off >= 0 ? lvl_rows[L.rows_off + t] : L.base + t + k * L.stride;
+    double s = x[r];
+    const int e = rowptr[r + 1];
+    for (int p = diagpos[r] + 1; p < e; ++p) s -= (double)F[p] * x[col[p]];
+    x[r] = s * dinv[r];
+  }
+}
+
+template __global__ void k_ic_factor<float>(LevelDesc, const int *, const int *, const int *, const int *,
+                                            const int *, const double *, const uint8_t *, float *, double *, int *);
+template __global__ void k_ic_factor<double>(LevelDesc, const int *, const int *, const int *, const int *,
+                                             const int *, const double *, const uint8_t *, double *, double *, int *);
+template __global__ void k_sweep_fwd<float>(LevelDesc, const int *, const int *, const int *, const int *,
+                                            const float *, const double *, const double *, double *, const int *);
+template __global__ void k_sweep_fwd<double>(LevelDesc, const int *, const int *, const int *, const int *,
+                                             const double *, const double *, const double *, double *, const int *);
+template __global__ void k_sweep_bwd<float>(LevelDesc, const int *, const int *, const int *, const int *,
+                                            const float *, const double *, double *, const int *);
+template __global__ void k_sweep_bwd<double>(LevelDesc, const int *, const int *, const int *, const int *,
+                                             const double *, const double *, double *, const int *);
+
+// --------------------------------------------------------------- PCG --
+// graddp.f:1275-1385 with the reference's recurrences:
+//   alpha = (p,r)/(p,Ap); r -= alpha Ap; x += alpha p; z = M^-1 r;
+//   beta = -(Ap,z)/(p,Ap); p = z + beta p; err = ||r_free|| / ||b_free||.
+// Dirichlet rows carry p = z = 0, so they drop out of every product.
+
+// Partial sum of b_free^2 (XLUNG) and r = b - A x0.
+__global__ void k_residual0(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                            const double *__restrict__ A, const double *__restrict__ x,
+                            const double *__restrict__ b, const uint8_t *__restrict__ isdir,
+                            double *__restrict__ r, double *__restrict__ part) {
+  __shared__ double sh[kBlock / 64 + 1];
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = b[i];
+    for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) s -= A[p] * x[col[p]];
+    r[i] = s;
+    if (!isdir[i]) acc += b[i] * b[i];
+  }
+  double t = block_sum<kBlock>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// B = A p with partial (p,r) and (p,B).
+__global__ void k_spmv_dot2(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                            const double *__restrict__ A, const double *__restrict__ p,
+                            const double *__restrict__ r, double *__restrict__ B,
+                            double *__restrict__ part_pr, double *__restrict__ part_pb,
+                            const int *__restrict__ done) {
+  if (*done) return;
+  __shared__ double sh[kBlock / 64 + 1];
+  double apr = 0.0, apb = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    const int e = rowptr[i + 1];
+    for (int q = rowptr[i]; q < e; ++q) s += A[q] * p[col[q]];
+    B[i] = s;
+    const double pi = p[i];
+    apr += pi * r[i];
+    apb += pi * s;
+  }
+  double t1 = block_sum<kBlock>(apr, sh);
+  double t2 = block_sum<kBlock>(apb, sh);
+  if (threadIdx.x == 0) {
+    part_pr[blockIdx.x] = t1;
+    part_pb[blockIdx.x] = t2;
+  }
+}
+
+// alpha from the partials; r -= alpha B; x += alpha p; partial r_free^2.
+__global__ void k_pcg_alpha(int n, int nparts, const double *__restrict__ part_pr,
+                            const double *__restrict__ part_pb, const double *__restrict__ B,
+                            const double *__restrict__ p, const uint8_t *__restrict__ isdir,
+                            double *__restrict__ r, double *__restrict__ x,
+                            double *__restrict__ part_rr, PcgScal *sc, const int *__restrict__ done) {
+  if (*done) return;
+  __shared__ double sh[kBlock / 64 + 2];
+  const double pr = reduce_partials<kBlock>(part_pr, nparts, sh);
+  const double pb = reduce_partials<kBlock>(part_pb, nparts, sh);
+  const double alfa = pr / pb;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->pb = pb;
+    sc->alpha = alfa;
+  }
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double ri = r[i] - alfa * B[i];
+    r[i] = ri;
+    x[i] = x[i] + alfa * p[i];
+    if (!isdir[i]) acc += ri * ri;
+  }
+  double t = block_sum<kBlock>(acc, sh);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
+}
+
+// partial (B, z)
+__global__ void k_dot(int n, const double *__restrict__ u, const double *__restrict__ v,
+                      double *__restrict__ part, const int *__restrict__ done) {
+  if (done && *done) return;
+  __shared__ double sh[kBlock / 64 + 1];
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    acc += u[i] * v[i];
+  double t = block_sum<kBlock>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// beta, p = z + beta p, err and the stopping test (graddp.f:1348-1377).
+__global__ void k_pcg_beta(int n, int nparts, const double *__restrict__ part_bz,
+                           const double *__restrict__ part_rr, const double *__restrict__ z,
+                           double *__restrict__ p, PcgScal *sc, int *done) {
+  if (*done) return;
+  __shared__ double sh[kBlock / 64 + 2];
+  const double c = reduce_partials<kBlock>(part_bz, nparts, sh);
+  const double rr = reduce_partials<kBlock>(part_rr, nparts, sh);
+  const double pb = sc->pb;
+  const double beta = -c / pb;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double xl = sc->xlung;
+    const double err = xl > 0.0 ? sqrt(rr / xl) : sqrt(rr / n);
+    sc->err = err;
+    if (err > sc->tol && sc->niter < sc->imax) {
+      sc->niter += 1;
+    } else {
+      *done = 1;
+    }
+  }
+}
+
+// XLUNG from k_residual0's partials; reset the iteration state.
+__global__ void k_pcg_init(int nparts, const double *__restrict__ part, PcgScal *sc, int *done,
+                           double tol, int imax) {
+  __shared__ double sh[kBlock / 64 + 2];
+  const double xl = reduce_partials<kBlock>(part, nparts, sh);
+  if (threadIdx.x == 0) {
+    sc->xlung = xl;
+    sc->tol = tol;
+    sc->imax = imax;
+    sc->niter = 1;
+    sc->err = 0.0;
+    *done = 0;
+  }
+}
+
+// ------------------------------------------------ Picard bookkeeping --
+// picard.f:185-200: PNEW += PDIFF, then SHLPIC (atmospheric Dirichlet
+// nodes keep POLD).
+__global__ void k_update_pnew(int n, const double *__restrict__ pdiff,
+                              const int *__restrict__ row_surf, const int *__restrict__ ifatm,
+                              const double *__restrict__ pold, double *__restrict__ pnew) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double v = pnew[r] + pdiff[r];
+  const int s = row_surf[r];
+  if (s >= 0) {
+    const int fa = ifatm[s];
+    if (fa == 1 || fa == 2) v = pold[r];
+  }
+  pnew[r] = v;
+}
+
+// bkpic.f:28-54 at atmospheric Dirichlet nodes: ATMACT = (A dpsi - XT5 -
+// (1-TETAF) ATMOLD) / TETAF with the true (restored) diagonal.
+__global__ void k_bkpic(int nnod, const int *__restrict__ surf_row, const int *__restrict__ rowptr,
+                        const int *__restrict__ col, const double *__restrict__ A,
+                        const double *__restrict__ pdiff, const double *__restrict__ xt5,
+                        const int *__restrict__ ifatm, const double *__restrict__ atmold,
+                        double tetaf, double *__restrict__ atmact) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnod) return;
+  const int fa = ifatm[s];
+  if (!(fa == 1 || fa == 2)) return;
+  const int r = surf_row[s];
+  double v = 0.0;
+  for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) v += A[p] * pdiff[col[p]];
+  v = v - xt5[r];
+  atmact[s] = (v - (1.0 - tetaf) * atmold[s]) * (1.0 / tetaf);
+}
+
+// norms.f (PINF with its last-index tie-break on CATHY ids, PL2, FINF,
+// FL2) and stormb.f's DSTORE, as per-block partials.
+__global__ void k_norms(int n, const int *__restrict__ refid, const double *__restrict__ pnew,
+                        const double *__restrict__ pold, const double *__restrict__ tnoti,
+                        const double *__restrict__ swnew, const double *__restrict__ swtimep,
+                        const double *__restrict__ ptimep, const double *__restrict__ volnod,
+                        const double *__restrict__ snodi, const double *__restrict__ pnodi,
+                        NormPart part) {
+  __shared__ double sh[kBlock / 64 + 1];
+  __shared__ double shv[kBlock / 64], shd[kBlock / 64];
+  __shared__ int shi[kBlock / 64];
+  double pl2 = 0.0, fl2 = 0.0, dst = 0.0, finf = 0.0, pinf = -1.0, pdif = 0.0;
+  int pidx = -1;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double diff = pnew[i] - pold[i];
+    const double a = fabs(diff);
+    pl2 += diff * diff;
+    const double t = tnoti[i];
+    fl2 += t * t;
+    finf = fmax(finf, fabs(t));
+    const int id = refid[i];
+    if (a > pinf || (a == pinf && id > pidx)) {
+      pinf = a;
+      pidx = id;
+      pdif = diff;
+    }
+    dst += volnod[i] * (snodi[i] * (swnew[i] + swtimep[i]) * 0.5 * (pnew[i] - ptimep[i]) +
+                        pnodi[i] * (swnew[i] - swtimep[i]));
+  }
+  double t1 = block_sum<kBlock>(pl2, sh);
+  double t2 = block_sum<kBlock>(fl2, sh);
+  double t3 = block_sum<kBlock>(dst, sh);
+  // max reductions (finf; pinf with index)
+  for (int o = 32; o > 0; o >>= 1) {
+    finf = fmax(finf, __shfl_down(finf, o, 64));
+    const double ov = __shfl_down(pinf, o, 64);
+    const int oi = __shfl_down(pidx, o, 64);
+    const double od = __shfl_down(pdif, o, 64);
+    if (ov > pinf || (ov == pinf && oi > pidx)) {
+      pinf = ov;
+      pidx = oi;
+      pdif = od;
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    shv[w] = pinf;
+    shi[w] = pidx;
+    shd[w] = pdif;
+    sh[w] = finf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bv = -1.0, bd = 0.0, bf = 0.0;
+    int bi = -1;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      if (shv[k] > bv || (shv[k] == bv && shi[k] > bi)) {
+        bv = shv[k];
+        bi = shi[k];
+        bd = shd[k];
+      }
+      bf = fmax(bf, sh[k]);
+    }
+    part.pl2[blockIdx.x] = t1;
+    part.fl2[blockIdx.x] = t2;
+    part.dstore[blockIdx.x] = t3;
+    part.finf[blockIdx.x] = bf;
+    part.pinf[blockIdx.x] = bv;
+    part.pidx[blockIdx.x] = bi;
+    part.pdif[blockIdx.x] = bd;
+  }
+}
+
+// fluxmb.f: atmospheric and Neumann in/out sums on the surface / base.
+__global__ void k_flux(int nnod, const int *__restrict__ ifatm, const double *__restrict__ atmact,
+                       const double *__restrict__ qbot, int nq, double *__restrict__ part6) {
+  __shared__ double sh[kBlock / 64 + 1];
+  double v[6] = {0, 0, 0, 0, 0, 0};  // adin adout anin anout nnin nnout
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nnod; s += gridDim.x * blockDim.x) {
+    const int fa = ifatm[s];
+    const double a = atmact[s];
+    if (fa == 1 || fa == 2) {
+      if (a > 0.0) v[0] += a; else v[1] += a;
+    } else if (fa == 0) {
+      if (a > 0.0) v[2] += a; else v[3] += a;
+    }
+    if (s < nq) {
+      const double q = qbot[s];
+      if (q > 0.0) v[4] += q; else v[5] += q;
+    }
+  }
+  for (int k = 0; k < 6; ++k) {
+    double t = block_sum<kBlock>(v[k], sh);
+    if (threadIdx.x == 0) part6[k * kMaxRedBlocks + blockIdx.x] = t;
+  }
+}
+
+// One work-group folds the partials of k_norms and k_flux into res[16]:
+// 0 DSTORE 1 PL2 2 FL2 3 FINF 4 PINF 5 PIKMAX 6 IKMAX 7 ADIN 8 ADOUT
+// 9 ANIN 10 ANOUT 11 NNIN 12 NNOUT
+__global__ void k_norms_final(int nb, int nbf, NormPart part, const double *__restrict__ part6,
+                              double *__restrict__ res) {
+  __shared__ double sh[kBlock / 64 + 2];
+  const double pl2 = reduce_partials<kBlock>(part.pl2, nb, sh);
+  const double fl2 = reduce_partials<kBlock>(part.fl2, nb, sh);
+  const double dst = reduce_partials<kBlock>(part.dstore, nb, sh);
+  double fx[6];
+  for (int k = 0; k < 6; ++k) fx[k] = reduce_partials<kBlock>(part6 + k * kMaxRedBlocks, nbf, sh);
+  if (threadIdx.x == 0) {
+    double bv = -1.0, bd = 0.0, bf = 0.0;
+    int bi = -1;
+    for (int k = 0; k < nb; ++k) {
+      if (part.pinf[k] > bv || (part.pinf[k] == bv && part.pidx[k] > bi)) {
+        bv = part.pinf[k];
+        bi = part.pidx[k];
+        bd = part.pdif[k];
+      }
+      bf = fmax(bf, part.finf[k]);
+    }
+    res[0] = dst;
+    res[1] = sqrt(pl2);
+    res[2] = sqrt(fl2);
+    res[3] = bf;
+    res[4] = bv;
+    res[5] = bd;
+    res[6] = (double)bi;
+    for (int k = 0; k < 6; ++k) res[7 + k] = fx[k];
+  }
+}
+
+// switch_old.f:22-52 on the surface nodes.
+__global__ void k_switch_old(int nnod, const int *__restrict__ surf_row, int *__restrict__ ifatm,
+                             double *__restrict__ atmact, const double *__restrict__ atmpot,
+                             double *__restrict__ pnew, double pmin) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnod) return;
+  int fa = ifatm[i];
+  if (fa == -1) return;
+  const int r = surf_row[i];
+  const double pn = pnew[r], ap = atmpot[i], aa = atmact[i];
+  if (fa == 1 && pn >= 0.0) {
+    if (ap < 0.0 || aa > ap) {
+      ifatm[i] = 0;
+      atmact[i] = ap;
+      return;
+    }
+  }
+  if (fa == 1 && pn <= pmin) {
+    if (ap > 0.0 || aa < ap) {
+      ifatm[i] = 0;
+      atmact[i] = ap;
+      return;
+    }
+  }
+  if (fa == 0 && pn >= 0.0 && ap >= 0.0) {
+    ifatm[i] = 1;
+    pnew[r] = 0.0;
+    return;
+  }
+  if (fa == 0 && pn <= pmin && ap < 0.0) {
+    ifatm[i] = 1;
+    pnew[r] = pmin;
+  }
+}
+
+// flow3d.f:235-240 / cathy_main.f:2976-2978: POLD = PNEW, PTOLD = PTNEW,
+// PTNEW = TETAF*PNEW + (1-TETAF)*PTIMEP (weight.f).
+__global__ void k_next_iter(int n, double tetaf, const double *__restrict__ pnew,
+                            const double *__restrict__ ptimep, double *__restrict__ pold,
+                            double *__restrict__ ptnew, double *__restrict__ ptold) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double pn = pnew[i];
+  pold[i] = pn;
+  ptold[i] = ptnew[i];
+  const double w = tetaf * pn + (1.0 - tetaf) * ptimep[i];
+  ptnew[i] = w;
+}
+
+// Start of a step: POLD = PNEW, PTNEW = weight, PTOLD = PTNEW.
+__global__ void k_start_iter(int n, double tetaf, const double *__restrict__ pnew,
+                             const double *__restrict__ ptimep, double *__restrict__ pold,
+                             double *__restrict__ ptnew, double *__restrict__ ptold) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double pn = pnew[i];
+  pold[i] = pn;
+  const double w = tetaf * pn + (1.0 - tetaf) * ptimep[i];
+  ptnew[i] = w;
+  ptold[i] = w;
+}
+
+// atmnxt.f:41-73 (label 300) for one interpolated rate (HSPATM=1) or a
+// per-node rate array: ATMPOT = rate*area, and ATMACT for IFATM = 0.
+__global__ void k_atm_update(int nnod, double rate, const double *__restrict__ rate_node,
+                             const double *__restrict__ arenod, const int *__restrict__ ifatm,
+                             double scf, int set_act, double *__restrict__ atmpot,
+                             double *__restrict__ atmact) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnod) return;
+  const double rt = rate_node ? rate_node[s] : rate;
+  const double ap = rt * arenod[s];
+  atmpot[s] = ap;
+  if (set_act && ifatm[s] == 0) atmact[s] = ap >= 0.0 ? ap : (1.0 - scf) * ap;
+}
+
+// neumann.f:19-24: free drainage Q = -A kr Kz at the base nodes.
+__global__ void k_free_drainage(int nnod, const int *__restrict__ bot_row,
+                                const double *__restrict__ arenod, const double *__restrict__ ckrw,
+                                const double *__restrict__ kznod, double *__restrict__ qbot,
+                                double *__restrict__ qnode) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnod) return;
+  const int r = bot_row[s];
+  const double q = -1.0 * arenod[s] * ckrw[r] * kznod[r];
+  qbot[s] = q;
+  qnode[r] = q;
+}
+
+// storcal.f partial: sum Sw * VOLNOD * PNODI.
+__global__ void k_store(int n, const double *__restrict__ sw, const double *__restrict__ volnod,
+                        const double *__restrict__ pnodi, double *__restrict__ part) {
+  __shared__ double sh[kBlock / 64 + 1];
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    acc += sw[i] * volnod[i] * pnodi[i];
+  double t = block_sum<kBlock>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_sum_final(int nb, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double sh[kBlock / 64 + 2];
+  const double v = reduce_partials<kBlock>(part, nb, sh);
+  if (threadIdx.x == 0) *out = v;
+}
+
+__global__ void k_copy(size_t n, const double *__restrict__ src, double *__restrict__ dst) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+__global__ void k_copy_int(int n, const int *__restrict__ src, int *__restrict__ dst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+__global__ void k_fill(size_t n, double v, double *__restrict__ dst) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
+}
+
+}  // namespace cg

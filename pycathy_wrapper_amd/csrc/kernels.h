@@ -1,0 +1,84 @@
+// kernels.h -- gfx950 kernels of the CATHY subsurface Picard step.
+//
+// Device numbering: node ids are permuted at set-up (ordering NATURAL keeps
+// CATHY's, COLUMN3 groups whole soil columns by a 3-colouring of the
+// surface mesh so IC(0) sweeps run column-parallel).  Every vector below is
+// in device numbering; the full CSR (rowptr/col, sorted columns) holds the
+// symmetric system matrix (FP64) and, in the same pattern, the IC(0) factor
+// (lower half = L_rj, upper half = transpose, diagonal = L_rr).
+#pragma once
+
+#include "common.h"
+
+namespace cg {
+
+// ------------------------------------------------------------------ vG --
+// van Genuchten / Mualem closed forms exactly as fvgse.f, fvgkr.f,
+// fvgdse.f (psi >= -1e-14 is saturated).
+__device__ __forceinline__ double dev_fvgse(double psi, double psat, double vgn, double vgm) {
+  if (psi < -1.0e-14) {
+    double beta = pow(fabs(psi / psat), vgn);
+    return pow(fabs(1.0 / (beta + 1.0)), vgm);
+  }
+  return 1.0;
+}
+__device__ __forceinline__ double dev_fvgkr(double psi, double se, double vgmr, double vgm) {
+  if (psi < -1.0e-14) {
+    double omega = pow(fabs(se), vgmr);
+    double v1 = 1.0 - pow(fabs(1.0 - omega), vgm);
+    return sqrt(se) * v1 * v1;
+  }
+  return 1.0;
+}
+__device__ __forceinline__ double dev_fvgdse(double psi, double psat, double vgn, double vgn1,
+                                             double vgpsn, double vgnr) {
+  if (psi < -1.0e-14) {
+    double beta = pow(fabs(psi / psat), vgn);
+    double b1 = beta + 1.0;
+    double b1r = 1.0 / b1;
+    return vgn1 * (pow(fabs(psi), vgn1) / vgpsn) * pow(fabs(b1), vgnr) * b1r * b1r;
+  }
+  return 0.0;
+}
+
+// Node soil table, structure of arrays (device numbering).
+struct SoilSoA {
+  const double *psat, *vgn, *vgm, *vgn1, *vgnr, *vgmr, *vgpsn, *pnot, *rmcp, *snodi, *pnodi;
+};
+
+// ------------------------------------------------- block reductions --
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Deterministic block sum: returns the total in thread 0.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < BS / 64; ++i) t += sh[i];
+  __syncthreads();
+  return t;
+}
+
+// Sum of `cnt` partials in a fixed order by one block (result in all
+// threads).  Used by consumers of a previous kernel's partial sums.
+template <int BS>
+__device__ __forceinline__ double reduce_partials(const double *part, int cnt, double *sh) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < cnt; i += BS) v += part[i];
+  v = block_sum<BS>(v, sh);
+  if (threadIdx.x == 0) sh[BS / 64] = v;
+  __syncthreads();
+  double r = sh[BS / 64];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace cg

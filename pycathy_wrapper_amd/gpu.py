@@ -1,0 +1,154 @@
+"""ctypes binding of libcathy_gpu.so (the gfx950 FLOW3D / SYMSLV drop-in).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (make -C
+pycathy_wrapper_amd/csrc).  Nothing here falls back to a CPU path: a missing
+library or device raises.
+
+``FlowGPU`` wraps the FLOW3D handle of include/cathy_gpu.h with the host
+arrays a CATHY run holds (1-based TETRA, column-major soil tables, ...),
+built from a ``configs.Problem``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import soil as SO
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcathy_gpu.so")
+
+ORDER_NATURAL = 0
+ORDER_COLUMN3 = 1
+
+STAT_NAMES = ["NSTEP", "DELTAT", "TIME", "ITER", "NITERT", "BACK", "STORE1", "STORE2", "DSTORE", "CDSTOR",
+              "VIN", "CVIN", "VOUT", "CVOUT", "VINOUT", "VTOT", "ERRAS", "ERREL", "CERRAS", "CAERAS", "PINF",
+              "ITRTOT", "ITLIN", "KLSFAI", "PL2", "FINF", "FL2", "IKMAX"]
+
+_lib = None
+
+
+class CathyGpuError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CathyGpuError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.cathy_gpu_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        msg = lib().cathy_gpu_last_error().decode(errors="replace")
+        raise CathyGpuError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def _i(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def device_info() -> tuple[str, int]:
+    buf = C.create_string_buffer(128)
+    ln = C.c_int(128)
+    cu = C.c_int(0)
+    _check(lib().cathy_gpu_device_info(buf, C.byref(ln), C.byref(cu)), "cathy_gpu_device_info")
+    return buf.value.decode(), cu.value
+
+
+class FlowGPU:
+    """Device-resident subsurface flow run (one mesh)."""
+
+    def __init__(self, problem, ordering: int = ORDER_COLUMN3):
+        m = problem.mesh
+        s = problem.soil
+        self.problem = problem
+        self.n, self.nnod = m.n, m.nnod
+        nodep = SO.node_params(m, s)
+        kz = SO.kz_nodes(m, s)
+        tet = np.ascontiguousarray(m.tetra, dtype=np.int32).copy()
+        tet[:, :4] += 1
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        # (nstr, nzone)[layer, zone] -> Fortran PERMX(MAXSTR=nstr, NZONE)
+        tab = {k: f64(np.asarray(s[k]).T) for k in ("permx", "permy", "permz")}
+        keep = dict(tet=tet, bi=f64(m.bi), ci=f64(m.ci), di=f64(m.di), volu=f64(m.volu),
+                    ivol=np.ascontiguousarray(m.ivol, dtype=np.int32), volnod=f64(m.volnod), arenod=f64(m.arenod),
+                    snodi=f64(nodep["snodi"]), pnodi=f64(nodep["pnodi"]), vgn=f64(nodep["vgn"]),
+                    vgrmc=f64(nodep["vgrmc"]), vgpsat=f64(nodep["vgpsat"]), kz=f64(kz), **tab)
+        self._keep = keep
+        self.node_params = nodep
+        h = C.c_int64(0)
+        rc = lib().cathy_gpu_flow3d_create(
+            C.byref(h), _i(m.n), _i(m.nt), _i(m.ntri), _i(m.nnod), _i(m.nstr), _i(s["nzone"]), _i(m.nstr),
+            _p(keep["tet"]), _p(keep["bi"]), _p(keep["ci"]), _p(keep["di"]), _p(keep["volu"]), _p(keep["ivol"]),
+            _p(keep["volnod"]), _p(keep["arenod"]), _p(keep["permx"]), _p(keep["permy"]), _p(keep["permz"]),
+            _p(keep["snodi"]), _p(keep["pnodi"]), _p(keep["vgn"]), _p(keep["vgrmc"]), _p(keep["vgpsat"]),
+            _p(keep["kz"]), _i(problem.params.lump), _i(ordering))
+        _check(rc, "cathy_gpu_flow3d_create")
+        self.h = h
+        self.set_params(problem.params)
+        self.set_forcing(problem.atm_times, problem.atm_vals)
+        self.init(problem.psi0)
+
+    def set_params(self, p):
+        ip = np.array([p.iopt, p.l2norm, p.ituns, p.ituns1, p.ituns2, p.itmxcg, p.ieto, p.free_drainage, 1],
+                      dtype=np.int32)
+        dp = np.array([p.tetaf, p.toluns, p.tolswi, p.ernlmx, p.tolcg, p.deltat, p.dtmin, p.dtmax, p.tmax,
+                       p.dtmaga, p.dtmagm, p.dtreds, p.dtredm, p.pmin, p.pondh_min, p.scf], dtype=np.float64)
+        _check(lib().cathy_gpu_flow3d_set_params(C.byref(self.h), _p(ip), _p(dp)), "set_params")
+
+    def set_forcing(self, times, vals):
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        _check(lib().cathy_gpu_flow3d_set_forcing(C.byref(self.h), _i(len(t)), _p(t), _p(v)), "set_forcing")
+
+    def init(self, psi0):
+        p = np.ascontiguousarray(psi0, dtype=np.float64)
+        _check(lib().cathy_gpu_flow3d_init(C.byref(self.h), _p(p)), "flow3d_init")
+
+    def advance(self):
+        st = np.zeros(32)
+        rc = _check(lib().cathy_gpu_flow3d_advance(C.byref(self.h), _p(st)), "flow3d_advance")
+        return rc, dict(zip(STAT_NAMES, st.tolist()))
+
+    def psi(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        _check(lib().cathy_gpu_flow3d_get_state(C.byref(self.h), _p(out), None, None, None), "get_state")
+        return out
+
+    def sw(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        _check(lib().cathy_gpu_flow3d_get_state(C.byref(self.h), None, _p(out), None, None), "get_state")
+        return out
+
+    def ifatm(self) -> np.ndarray:
+        out = np.zeros(self.nnod, dtype=np.int32)
+        _check(lib().cathy_gpu_flow3d_get_state(C.byref(self.h), None, None, None, _p(out)), "get_state")
+        return out
+
+    def counters(self) -> dict:
+        out = np.zeros(6)
+        _check(lib().cathy_gpu_flow3d_counters(C.byref(self.h), _p(out)), "counters")
+        return dict(picard=out[0], pcg=out[1], fallbacks=out[2], solve_ms=out[3], step_ms=out[4])
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().cathy_gpu_flow3d_destroy(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,390 @@
+"""Time loop around the reference's own FLOW3D (TEST INFRASTRUCTURE ONLY).
+
+The CATHY main program cannot be linked in this image (it needs LAPACK for
+the transport code), but FLOW3D and everything it calls are in
+oracle/_ref/libcathy_ref.so.  This module drives them through the
+subsurface-only time loop of cathy_main.f:2882-3860 (ISIMGR<=1, IOPT=1,
+homogeneous atmospheric forcing, free drainage at the base, no seepage faces,
+no nudging, no ET), calling the reference's FLOW3D, NEUMANN, SWITCH_OLD,
+WEIGHT, CHVELO, STORCAL, TIMUPD and ICVDWT routines directly.  The only
+host logic restated here is what reads input files (ATMONE/ATMNXT, the
+atmbc reader; BKSTEP's BC re-reads), which has no file to read.
+
+The mesh is given to the reference as the host builder makes it; the reference
+itself is sized by its CATHY.H (NMAX=7056, MAXSTR=15, MAXZON=1,
+MAXTRM=52111, NTRMAX=800), so runs here are config-1 sized.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from pyoracle import REF_MAXSTR, REF_NMAX, RefLib
+
+MB_FIELDS = ("NUDIN NUDOUT NUDINP NUDOUTP VNUDIN VNUDOUT VNUDTOT OVFLOW REFLOW SFFLW SFFLWP VSFFLW APOT AACT AACTP "
+             "AACTAV ADIN ADOUT ADINP ADOUTP NDIN NDOUT NDINP NDOUTP ANIN ANOUT ANINP ANOUTP NNIN NNOUT NNINP NNOUTP "
+             "VADIN VADOUT VNDIN VNDOUT VANIN VANOUT VNNIN VNNOUT VIN CVIN VOUT CVOUT VAPOT_T VAACT_T VSFTOT VNDTOT "
+             "VNNTOT VTOT STORE0 STORE1 STORE2 DSTORE CDSTOR ERRAS CERRAS CAERAS ERREL").split()
+
+# FLOW3D's dummy arguments in order (flow3d.f:8-30); letter = kind:
+# i int scalar, d real*8 scalar, l logical scalar, f real*4 scalar,
+# I int array, D real*8 array, F real*4 array
+FLOW3D_ARGS = [
+    ("IOPT", "i"), ("IPRT1", "i"), ("NLRELX", "i"), ("L2NORM", "i"), ("N", "i"), ("NT", "i"), ("NTRI", "i"),
+    ("NTERM", "i"), ("NSTR", "i"), ("NDIR", "I"), ("NDIRC", "I"), ("NP", "i"), ("NQ", "i"), ("HSPATM", "i"),
+    ("HTIATM", "i"), ("HTIDIR", "i"), ("HTINEU", "i"), ("ITER", "i"), ("ITUNS", "i"), ("NITER", "i"),
+    ("NITERT", "i"), ("ITLIN", "i"), ("ITRTOT", "i"), ("ISOLV", "i"), ("ITMXCG", "i"), ("IPEAT", "i"),
+    ("IVGHU", "i"), ("KSLOPE", "i"), ("ISFONE", "i"), ("KSFCV", "i"), ("KSFCVT", "i"), ("IBOT", "i"),
+    ("MINBOT", "i"), ("NNOD", "i"), ("NUMDIR", "i"), ("NSF", "i"), ("KLSFAI", "i"), ("NLKP", "i"),
+    ("NUDN", "i"), ("NUDC", "i"), ("NUDCTR", "i"), ("NUDFLAG", "i"), ("WFLAG", "i"),
+    ("TP", "I"), ("TETRA", "I"), ("TETJA", "I"), ("IA", "I"), ("JA", "I"), ("TOPOL", "I"), ("CONTP", "I"),
+    ("CONTQ", "I"), ("IVOL", "I"), ("INSYM", "I"), ("IFATM", "I"), ("IFATMP", "I"), ("NODDIR", "I"),
+    ("NSFNUM", "I"), ("NSFNOD", "I"), ("SFEX", "I"), ("SFEXIT", "I"), ("SFEXP", "I"), ("SFFLAG", "I"),
+    ("NUDTET", "I"), ("DUPUIT", "i"), ("SURF", "l"), ("PONDING", "l"),
+    ("DTGMIN", "l"), ("LSFAIL", "l"), ("ERRGMX", "l"), ("NORMCV", "l"), ("ITAGEN", "l"), ("SFCHEK", "l"),
+    ("KSFZER", "l"), ("NOBACK", "l"), ("BCKSTP", "l"), ("CPUVEC", "F"), ("CPUNL", "f"), ("TIME", "d"),
+    ("TIMEP", "d"), ("DELTAT", "d"), ("DTMIN", "d"), ("DTREDM", "d"), ("DTREDS", "d"), ("TETAF", "d"),
+    ("OMEGA", "d"), ("OMEGAP", "d"), ("RMAX", "d"), ("RMIN", "d"), ("TOLUNS", "d"), ("TOLSWI", "d"),
+    ("TOLCG", "d"), ("ERNLMX", "d"), ("PONDH_MIN", "d"), ("NUDG", "d"),
+    ("PONDNOD", "D"), ("OVFLNOD", "D"), ("PERMX", "D"), ("PERMY", "D"), ("PERMZ", "D"), ("INDE0", "D"),
+    ("SNODI", "D"), ("PNODI", "D"), ("PEL", "D"), ("PORE", "D"), ("INDE", "D"), ("PRESC", "D"), ("PTIM", "D"),
+    ("PINP", "D"), ("DEF", "D"), ("Q", "D"), ("QTIM", "D"), ("QINP", "D"), ("ATMTIM", "D"), ("ATMINP", "D"),
+    ("AI", "D"), ("BI", "D"), ("CI", "D"), ("DI", "D"), ("LMASS", "D"), ("ARENOD", "D"), ("VOLNOD", "D"),
+    ("VOLU", "D"), ("VOLUR", "D"), ("ETAI", "D"), ("ETAE", "D"), ("ET1", "D"), ("ET2", "D"), ("ET1E", "D"),
+    ("DETAI", "D"), ("DETAIE", "D"), ("SW", "D"), ("SWTIMEP", "D"), ("SWE", "D"), ("CKRW", "D"),
+    ("CKRWE", "D"), ("DCKRW", "D"), ("DCKRWE", "D"), ("SENODI", "D"), ("SEELT", "D"), ("POLD", "D"),
+    ("PDIFF", "D"), ("PTOLD", "D"), ("PNEW", "D"), ("PTNEW", "D"), ("PTIMEP", "D"), ("TNOTI", "D"),
+    ("XT5", "D"), ("LHSP", "D"), ("COEF1", "D"), ("COEF2", "D"), ("COEF3", "D"), ("COEF4", "D"), ("SCR", "D"),
+    ("RNSYM", "D"), ("ATMACT", "D"), ("ATMPOT", "D"), ("ATMOLD", "D"), ("LHSATM", "D"), ("LHSSF", "D"),
+    ("QPNEW", "D"), ("QPOLD", "D"), ("SFQ", "D"), ("SFQP", "D"), ("BKCFLOW_NODE", "D"), ("X", "D"), ("Y", "D"),
+    ("Z", "D"), ("NUDTIM", "D"), ("NUDRXY", "D"), ("NUDRZ", "D"), ("NUDX", "D"), ("NUDY", "D"), ("NUDZ", "D"),
+    ("NUDCUM", "D"), ("NUDEPS", "D"), ("NUDDIF", "D"), ("NUDSMC", "D"), ("NUDNOD", "D"), ("NUDVAL", "D"),
+    ("NUDTAU", "D"), ("PUNTDIRSF_NODE", "I"), ("QTRANIE", "D"),
+]
+
+
+class RefFlowRun:
+    """Reference-FLOW3D driven run.  Parameters follow orc_params."""
+
+    def __init__(self, ref: RefLib, mesh, soil, psi0, rain, prm: dict):
+        assert soil["nzone"] == 1, "reference CATHY.H has MAXZON=1"
+        assert mesh.nstr <= REF_MAXSTR and mesh.n <= REF_NMAX
+        self.ref = ref
+        self.p = dict(prm)
+        n, nt, nnod = mesh.n, mesh.nt, mesh.nnod
+        self.n, self.nnod = n, nnod
+        L = ref.L
+        self.mb = np.ctypeslib.as_array((C.c_double * len(MB_FIELDS)).in_dll(L, "mb_hgraph_"))
+        self.mb[:] = 0.0
+        ref.chord[0] = prm["pmin"]
+        from pycathy_wrapper_amd import mesh as M
+        topol, ja = M.sym_pattern(mesh.tetra, n)
+        nterm = len(ja)
+        tj = np.full((nt, 4, 4), 0, dtype=np.int32)  # TETJA(4,4,NT) col-major -> [e][l][k]
+        for idx, (k, l) in enumerate(M.UPPER_PAIRS):
+            tj[:, l, k] = M.tetja_sym(mesh.tetra, topol, ja, n)[:, idx] + 1
+        a = {}
+        I = lambda v: np.array([v], dtype=np.int32)
+        D = lambda v: np.array([v], dtype=np.float64)
+        big = max(n, nt, nterm) + 16
+        for name, kind in FLOW3D_ARGS:
+            if kind in "il":
+                a[name] = I(0)
+            elif kind == "d":
+                a[name] = D(0.0)
+            elif kind == "f":
+                a[name] = np.zeros(1, dtype=np.float32)
+            elif kind == "F":
+                a[name] = np.zeros(16, dtype=np.float32)
+            elif kind == "I":
+                a[name] = np.zeros(big, dtype=np.int32)
+            else:
+                a[name] = np.zeros(big, dtype=np.float64)
+        a["SCR"] = np.zeros(5 * n + 16)
+        tet = np.ascontiguousarray(mesh.tetra, dtype=np.int32).copy()
+        tet[:, :4] += 1
+        a["TETRA"] = tet.reshape(-1)
+        a["TETJA"] = tj.reshape(-1)
+        a["TOPOL"] = (topol + 1).astype(np.int32)
+        a["JA"] = (ja + 1).astype(np.int32)
+        a["IVOL"] = mesh.ivol.astype(np.int32).copy()
+        if prm["free_drainage"]:   # ACONTQ of BCONE with NNEU<0: the base nodes
+            a["CONTQ"][:nnod] = mesh.nstr * nnod + np.arange(nnod) + 1
+        for k, v in dict(IOPT=1, IPRT1=0, NLRELX=0, L2NORM=prm["l2norm"], N=n, NT=nt, NTRI=mesh.ntri,
+                         NTERM=nterm, NSTR=mesh.nstr, NP=0, NQ=nnod if prm["free_drainage"] else 0,
+                         HSPATM=1, ITER=1, ITUNS=prm["ituns"], ISOLV=2, ITMXCG=prm["itmxcg"], NNOD=nnod,
+                         NLKP=0).items():
+            a[k][0] = v
+        for k, v in dict(TETAF=prm["tetaf"], DTMIN=prm["dtmin"], DTREDM=prm["dtredm"], DTREDS=prm["dtreds"],
+                         RMAX=1.7e100, RMIN=1.7e-100, TOLUNS=prm["toluns"], TOLSWI=prm["tolswi"],
+                         TOLCG=prm["tolcg"], ERNLMX=prm["ernlmx"], PONDH_MIN=prm["pondh_min"]).items():
+            a[k][0] = v
+        perm = np.zeros(REF_MAXSTR)
+        for key, src in (("PERMX", "permx"), ("PERMY", "permy"), ("PERMZ", "permz")):
+            perm = np.zeros(REF_MAXSTR)
+            perm[:mesh.nstr] = soil[src][:, 0]
+            a[key] = perm.copy()
+        a["BI"] = np.ascontiguousarray(mesh.bi).reshape(-1).copy()
+        a["CI"] = np.ascontiguousarray(mesh.ci).reshape(-1).copy()
+        a["DI"] = np.ascontiguousarray(mesh.di).reshape(-1).copy()
+        a["VOLU"] = mesh.volu.copy()
+        a["VOLUR"] = 1.0 / mesh.volu
+        a["VOLNOD"] = mesh.volnod.copy()
+        a["ARENOD"] = np.zeros(big); a["ARENOD"][:nnod] = mesh.arenod
+        a["X"][:n] = mesh.x; a["Y"][:n] = mesh.y; a["Z"][:n] = mesh.z
+        lm = np.zeros(16)
+        L.locmas_(lm.ctypes.data_as(C.POINTER(C.c_double)), C.byref(C.c_int(prm["lump"])))
+        a["LMASS"] = lm
+        # soil: TPNODI + CHPARM (reference routines)
+        self._tpnodi(mesh, soil, a)
+        # initial condition (INDP=4 is done by the caller through psi0)
+        a["PTIMEP"][:n] = psi0
+        self.a = a
+        self.mesh = mesh
+        self.rain = rain
+        self.nterm = nterm
+        self._argv = [a[name].ctypes.data_as(C.c_void_p) for name, _ in FLOW3D_ARGS]
+        self._init_state(prm)
+
+    # -- reference set-up calls --------------------------------------------
+    def _tpnodi(self, mesh, soil, a):
+        L = self.ref.L
+        n, nt = mesh.n, mesh.nt
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        big = len(a["PNODI"])
+        tabs = {}
+        for k in ("poros", "elstor", "vgn", "vgrmc", "vgpsat"):
+            t = np.zeros(REF_MAXSTR); t[:mesh.nstr] = soil[k][:, 0]; tabs[k] = t
+        zeros = {k: np.zeros(big) for k in ("PORE", "INDE", "DEF", "SEL", "LEL", "KEL", "LAMBDANODI", "KDNODI")}
+        lam = np.zeros(REF_MAXSTR); kd = np.zeros(REF_MAXSTR)
+        self.ref.vg[:] = 0.0
+        a["TP"][:] = 0
+        L.tpnodi_(C.byref(C.c_int(n)), C.byref(C.c_int(nt)), C.byref(C.c_int(mesh.ntri)), ip(a["TETRA"]),
+                  ip(np.zeros(4, dtype=np.int32)), dp(a["PNODI"]), dp(a["SNODI"]), ip(a["TP"]),
+                  C.byref(C.c_int(0)), dp(tabs["poros"]), dp(tabs["elstor"]), dp(zeros["PORE"]),
+                  dp(zeros["INDE"]), dp(zeros["DEF"]), dp(a["PEL"]), dp(zeros["SEL"]), dp(tabs["vgn"]),
+                  dp(tabs["vgrmc"]), dp(tabs["vgpsat"]), dp(zeros["LEL"]), dp(lam), dp(zeros["LAMBDANODI"]),
+                  dp(zeros["KEL"]), dp(kd), dp(zeros["KDNODI"]))
+        L.chparm_(C.byref(C.c_int(n)), dp(a["SNODI"]), dp(a["PNODI"]), C.byref(C.c_int(0)))
+        kz = np.zeros(nt)
+        lay = mesh.layer_of_element()
+        kz[:] = soil["permz"][lay, 0]
+        self.kznod = np.zeros(big)
+        L.eltnod_(C.byref(C.c_int(n)), C.byref(C.c_int(nt)), ip(a["TP"]), ip(a["TETRA"]), dp(kz), dp(self.kznod))
+
+    def _call(self, name, *args):
+        return getattr(self.ref.L, name.lower() + "_")(*args)
+
+    def _init_state(self, prm):
+        a, n, nnod = self.a, self.n, self.nnod
+        # INIT1 (init1.f:37-60)
+        dt = prm["deltat"]
+        if dt > prm["dtmax"]:
+            dt = prm["dtmax"]
+        if dt <= prm["dtmin"]:
+            dt = prm["dtmin"]; a["DTGMIN"][0] = 0
+        else:
+            a["DTGMIN"][0] = 1
+        a["DELTAT"][0] = dt
+        a["TIMEP"][0] = 0.0
+        a["TIME"][0] = dt
+        a["ITER"][0] = 1
+        a["PNEW"][:n] = a["PTIMEP"][:n]
+        self.nstep = 1
+        # ATMONE restated for a single constant homogeneous record at t=0
+        ar = a["ARENOD"][:nnod]
+        a["IFATM"][:nnod] = 0; a["IFATMP"][:nnod] = 0
+        a["ATMOLD"][:nnod] = self.rain * ar  # atmtim(3) <= 0
+        a["ATMPOT"][:nnod] = self.rain * ar
+        for i in range(nnod):
+            if a["PNEW"][i] >= prm["pondh_min"]:
+                a["IFATM"][i] = 2; a["IFATMP"][i] = 2
+                continue
+            if a["PNEW"][i] >= 0.0 and a["ATMPOT"][i] > 0.0:
+                a["IFATM"][i] = 1
+            if a["PTIMEP"][i] >= 0.0 and a["ATMOLD"][i] > 0.0:
+                a["IFATMP"][i] = 1
+        for i in range(nnod):
+            a["ATMACT"][i] = a["ATMPOT"][i] if a["IFATM"][i] == 0 else 0.0
+            if a["IFATMP"][i] in (1, 2):
+                a["ATMOLD"][i] = 0.0
+        self._copy("POLD", "PNEW", n)
+        self._weight()
+        self._copy("PTOLD", "PTNEW", n)
+        # MBINIT
+        mb = self._mb()
+        for k in list(mb):
+            mb[k] = 0.0
+        anin = 0.0; anout = 0.0
+        for k in range(nnod):   # mbinit.f:60-67, sequential sum
+            if a["IFATMP"][k] == 0:
+                if a["ATMOLD"][k] > 0.0:
+                    anin += a["ATMOLD"][k]
+                else:
+                    anout += a["ATMOLD"][k]
+        mb["ANINP"] = anin; mb["ANOUTP"] = anout
+        self._set_mb(mb)
+        # CHVELO(PTIMEP), STORCAL, NEUMANN
+        self._chvelo("PTIMEP")
+        st1 = np.zeros(1); st2 = np.zeros(1)
+        self._storcal(st1, st2, 0.0)
+        mb = self._mb(); mb["STORE0"] = st1[0]; mb["STORE1"] = st1[0]; mb["STORE2"] = st1[0]
+        self.ckrwp = a["CKRW"][:n].copy()
+        self._neumann(self.ckrwp)
+        q = a["Q"][:nnod] if self.p["free_drainage"] else np.zeros(0)
+        nin = 0.0; nout = 0.0
+        for v in q:
+            if v > 0.0:
+                nin += v
+            else:
+                nout += v
+        mb["NNINP"] = nin; mb["NNOUTP"] = nout
+        self._set_mb(mb)
+        self.cum = dict(CVIN=0.0, CVOUT=0.0, CDSTOR=0.0, CERRAS=0.0, CAERAS=0.0, VTOT=0.0)
+        self.finished = 0
+
+    # -- helpers -------------------------------------------------------------
+    def _mb(self):
+        return dict(zip(MB_FIELDS, self.mb.tolist()))
+
+    def _set_mb(self, d):
+        for i, k in enumerate(MB_FIELDS):
+            self.mb[i] = d[k]
+
+    def _copy(self, dst, src, k):
+        self.a[dst][:k] = self.a[src][:k]
+
+    def _weight(self):
+        a = self.a
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        self.ref.L.weight_(C.byref(C.c_int(self.n)), dp(a["TETAF"]), dp(a["PNEW"]), dp(a["PTIMEP"]), dp(a["PTNEW"]))
+
+    def _chvelo(self, psi_name):
+        a = self.a
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        self.ref.L.chvelo_(C.byref(C.c_int(0)), C.byref(C.c_int(self.n)), C.byref(C.c_int(self.mesh.nt)),
+                           C.byref(C.c_int(self.mesh.ntri)), C.byref(C.c_int(0)), ip(a["TETRA"]), ip(a["TP"]),
+                           dp(a[psi_name]), dp(a["SNODI"]), dp(a["PNODI"]), dp(a["SW"]), dp(a["CKRW"]),
+                           dp(a["SWE"]), dp(a["CKRWE"]))
+
+    def _storcal(self, st1, st2, dstore):
+        a = self.a
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        self.ref.L.storcal_(C.byref(C.c_int(self.n)), dp(st1), dp(st2), C.byref(C.c_double(dstore)), dp(a["SW"]),
+                            dp(a["PNODI"]), dp(a["VOLNOD"]))
+
+    def _neumann(self, ckrw):
+        if not self.p["free_drainage"]:
+            return
+        a, nnod, nstr = self.a, self.nnod, self.mesh.nstr
+        acontq = (nstr * nnod + np.arange(nnod) + 1).astype(np.int32)
+        nneu = np.array([0, -1, 0], dtype=np.int32)
+        nneuc = np.zeros(3, dtype=np.int32)
+        qinp = np.zeros(3 * nnod + 3)
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        ck = np.zeros(len(a["CKRW"])); ck[:self.n] = ckrw[:self.n]
+        self.ref.L.neumann_(dp(a["TIME"]), ip(nneu), ip(nneuc), C.byref(C.c_int(nnod)), C.byref(C.c_int(nstr)),
+                            dp(ck), dp(self.kznod), dp(a["ARENOD"]), ip(acontq), dp(qinp), dp(a["Q"]))
+
+    def _switch_old(self):
+        a = self.a
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        self.ref.L.switch_old_(C.byref(C.c_int(self.nnod)), ip(a["IFATM"]), dp(a["ATMACT"]), dp(a["ATMPOT"]),
+                               dp(a["PNEW"]))
+
+    def _flow3d(self):
+        self.ref.L.flow3d_(*self._argv)
+
+    def _bkstep(self):
+        """bkstep.f:58-147 restated (constant BCs; constant forcing)."""
+        a, n, nnod, p = self.a, self.n, self.nnod, self.p
+        self._copy("PNEW", "PTIMEP", n)
+        a["IFATM"][:nnod] = a["IFATMP"][:nnod]
+        a["ATMACT"][:nnod] = a["ATMOLD"][:nnod]
+        a["TIME"][0] = a["TIMEP"][0]
+        dt = a["DELTAT"][0] * p["dtredm"] - p["dtreds"]
+        if dt <= p["dtmin"]:
+            dt = p["dtmin"]; a["DTGMIN"][0] = 0
+        else:
+            a["DTGMIN"][0] = 1
+        a["DELTAT"][0] = dt
+        a["TIME"][0] = a["TIME"][0] + dt
+        a["ITER"][0] = 1
+        a["NITERT"][0] = 0
+        # constant forcing: ATMNXT/ATMBAK leave ATMPOT unchanged
+        self._switch_old()
+        self._copy("POLD", "PNEW", n)
+        self._weight()
+        self._copy("PTOLD", "PTNEW", n)
+
+    def step(self):
+        """One accepted time step; returns (rc, stats dict) like OracleRun."""
+        if self.finished:
+            return self.finished, {}
+        a, n, nnod, p = self.a, self.n, self.nnod, self.p
+        self._neumann(a["CKRW"])
+        # ATMNXT: constant record -> ATMACT = ATMPOT for IFATM = 0 nodes
+        for i in range(nnod):
+            if a["IFATM"][i] == 0:
+                a["ATMACT"][i] = a["ATMPOT"][i]
+        self._switch_old()
+        self._copy("POLD", "PNEW", n)
+        self._weight()
+        self._copy("PTOLD", "PTNEW", n)
+        backs = 0
+        while True:
+            self._flow3d()
+            if not a["BCKSTP"][0]:
+                break
+            self._bkstep()
+            self._neumann(self.ckrwp)
+            backs += 1
+        self._chvelo("PNEW")
+        mb = self._mb()
+        st1 = np.zeros(1); st2 = np.array([mb["STORE2"]])
+        self._storcal(st1, st2, mb["DSTORE"])
+        mb["STORE1"], mb["STORE2"] = st1[0], st2[0]
+        c = self.cum
+        c["VTOT"] += mb["VIN"] + mb["VOUT"]; c["CVIN"] += mb["VIN"]; c["CVOUT"] += mb["VOUT"]
+        c["CDSTOR"] += mb["DSTORE"]; c["CERRAS"] += mb["ERRAS"]; c["CAERAS"] += abs(mb["ERRAS"])
+        stats = dict(NSTEP=self.nstep, DELTAT=a["DELTAT"][0], TIME=a["TIME"][0], ITER=int(a["ITER"][0]),
+                     NITERT=int(a["NITERT"][0]), BACK=backs, STORE1=mb["STORE1"], STORE2=mb["STORE2"],
+                     DSTORE=mb["DSTORE"], CDSTOR=c["CDSTOR"], VIN=mb["VIN"], CVIN=c["CVIN"], VOUT=mb["VOUT"],
+                     CVOUT=c["CVOUT"], VINOUT=mb["VIN"] + mb["VOUT"], VTOT=c["VTOT"], ERRAS=mb["ERRAS"],
+                     ERREL=mb["ERREL"], CERRAS=c["CERRAS"], CAERAS=c["CAERAS"])
+        for k in ("ADIN", "ADOUT", "NDIN", "NDOUT", "ANIN", "ANOUT", "NNIN", "NNOUT"):
+            mb[k + "P"] = mb[k]
+        self._set_mb(mb)
+        a["IFATMP"][:nnod] = a["IFATM"][:nnod]
+        a["ATMOLD"][:nnod] = a["ATMACT"][:nnod]
+        self._copy("PTIMEP", "PNEW", n)
+        self.ckrwp = a["CKRW"][:n].copy()
+        if a["NOBACK"][0]:
+            self.finished = 2
+            return 2, stats
+        if abs(a["TIME"][0] - p["tmax"]) <= 0.001 * a["DELTAT"][0]:
+            self.finished = 1
+            return 1, stats
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        self.ref.L.timupd_(dp(a["TIMEP"]), dp(a["TIME"]), dp(a["DELTAT"]), C.byref(C.c_double(p["dtmin"])),
+                           C.byref(C.c_double(p["dtmax"])), C.byref(C.c_double(p["tmax"])),
+                           C.byref(C.c_double(p["dtmaga"])), C.byref(C.c_double(p["dtmagm"])),
+                           C.byref(C.c_double(p["dtreds"])), C.byref(C.c_double(p["dtredm"])), ip(a["DTGMIN"]),
+                           ip(a["ITER"]), C.byref(C.c_int(p["ituns1"])), C.byref(C.c_int(p["ituns2"])))
+        self.nstep += 1
+        a["ITER"][0] = 1
+        a["NITERT"][0] = 0
+        return 0, stats
+
+    def psi(self):
+        return self.a["PNEW"][:self.n].copy()
+
+    def sw(self):
+        return self.a["SW"][:self.n].copy()
