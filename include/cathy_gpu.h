@@ -129,6 +129,20 @@ int cathy_gpu_flow3d_set_pnew(int64_t *handle, const double *pnew);
  * fallbacks, kernel time of the linear solver (ms), total device ms. */
 int cathy_gpu_flow3d_counters(int64_t *handle, double *out6);
 
+/* ------------------------------------------------------------------ *
+ * Instrumentation (no counterpart in the reference; used by bench.py).
+ * ------------------------------------------------------------------ */
+/* Enable/disable HIP-event timing of the PCG SpMV kernel (k_spmv_dot2) on
+ * the handle's stream and read the accumulated time / launch count. */
+int cathy_gpu_flow3d_profile(int64_t *handle, const int *enable,
+                             double *spmv_ms, int64_t *launches);
+/* Algorithmic bytes of one SpMV launch: 12*nnz + 4*(N+1) + 16*N. */
+int cathy_gpu_flow3d_spmv_bytes(int64_t *handle, double *bytes);
+/* Copy kernel over NBYTES (read NBYTES, write NBYTES), REPS launches;
+ * average ms per launch.  Calibrates HBM counters. */
+int cathy_gpu_calib_copy(const int64_t *nbytes, const int *reps,
+                         double *ms_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,8 +30,9 @@ static thread_local std::string g_last_error;
 
 // ------------------------------------------------------------- schedule --
 struct Schedule {
-  std::vector<LevelDesc> lv;
-  DBuf<int> rows;  // row lists of list-form levels
+  std::vector<ColDesc> cols;     // column ordering: colour classes
+  std::vector<ListLevel> lv;     // natural ordering: DAG levels
+  DBuf<int> rows;                // row lists of the natural levels
 };
 
 // Level schedule of the natural elimination order on a full CSR (sorted
@@ -56,8 +57,7 @@ static void natural_levels(int n, const std::vector<int> &rowptr, const std::vec
   std::vector<int> fill(cnt.begin(), cnt.end() - 1);
   for (int r = 0; r < n; ++r) rows[fill[lev[r]]++] = r;
   sch.lv.clear();
-  for (int l = 0; l < nlev; ++l)
-    sch.lv.push_back(LevelDesc{0, cnt[l + 1] - cnt[l], 0, 1, cnt[l]});
+  for (int l = 0; l < nlev; ++l) sch.lv.push_back(ListLevel{cnt[l], cnt[l + 1] - cnt[l]});
   sch.rows.upload(rows, st);
 }
 
@@ -79,7 +79,9 @@ struct Flow {
   std::vector<int> perm, iperm;  // CATHY id -> device id, device -> CATHY
 
   // mesh / pattern (device numbering)
-  DBuf<int> rowptr, col, diagpos, tpos, inc_ptr, inc, refid, row_surf, surf_row, bot_row;
+  DBuf<int> so, scol, slen, sdiag, stpos;  // SELL-64 pattern
+  int nnz_sell = 0;
+  DBuf<int> inc_ptr, inc, refid, row_surf, surf_row, bot_row;
   DBuf<uint32_t> incoff;
   DBuf<int4> enode;
   DBuf<double> S, gz, volu, m4d, m4full, lmass, arenod;
@@ -119,11 +121,37 @@ struct Flow {
   // counters
   double c_picard = 0, c_pcg = 0, c_fall = 0, c_solve_ms = 0, c_dev_ms = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // SpMV launch timing (HIP events on this stream around k_spmv_dot2)
+  bool prof = false;
+  std::vector<hipEvent_t> pev;
+  int pev_used = 0;
+  double spmv_ms = 0.0;
+  long spmv_launches = 0;
 
   ~Flow() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t e : pev) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
+  }
+
+  hipEvent_t next_event() {
+    if (pev_used == (int)pev.size()) {
+      hipEvent_t e;
+      CG_HIP(hipEventCreate(&e));
+      pev.push_back(e);
+    }
+    return pev[pev_used++];
+  }
+  // after a stream sync: fold recorded (start, stop) pairs into spmv_ms
+  void collect_events() {
+    for (int i = 0; i + 1 < pev_used; i += 2) {
+      float ms = 0.f;
+      CG_HIP(hipEventElapsedTime(&ms, pev[i], pev[i + 1]));
+      spmv_ms += ms;
+      spmv_launches += 1;
+    }
+    pev_used = 0;
   }
 
   SoilSoA soil() const {
@@ -134,6 +162,7 @@ struct Flow {
   static constexpr int kSlots = 18;
   double *P(int slot) { return part.p + (size_t)slot * kMaxRedBlocks; }
   int red_grid(size_t work) const { return grid_for(work, kBlock, kMaxRedBlocks); }
+  Sell sell() const { return Sell{so.p, scol.p, slen.p, sdiag.p}; }
 
   // ------------------------------------------------------------ set-up --
   void setup(int n_, int nt_, int ntri_, int nnod_, int nstr_, int nzone, int maxstr, const int *tetra,
@@ -147,6 +176,8 @@ struct Flow {
     CG_REQUIRE((long)3 * ntri * nstr == nt, "NT must equal 3*NTRI*NSTR (gen3d.f)");
     CG_REQUIRE(nzone >= 1 && maxstr >= nstr, "NZONE/MAXSTR");
     CG_REQUIRE(ordering == CATHY_GPU_ORDER_NATURAL || ordering == CATHY_GPU_ORDER_COLUMN3, "ORDERING");
+    CG_REQUIRE((long)n <= (long)kMaxRedBlocks * kBlock, "N above the per-launch row limit");
+    CG_REQUIRE(ordering != CATHY_GPU_ORDER_COLUMN3 || (nstr + 1) <= 64, "column ordering needs NSTR < 64");
     f32 = ordering == CATHY_GPU_ORDER_COLUMN3;
     CG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     CG_HIP(hipEventCreate(&ev0));
@@ -249,6 +280,7 @@ struct Flow {
     perm.assign(n, 0);
     iperm.assign(n, 0);
     sched.lv.clear();
+    sched.cols.clear();
     if (ordering == CATHY_GPU_ORDER_NATURAL) {
       for (int i = 0; i < n; ++i) perm[i] = iperm[i] = i;
       return;
@@ -267,7 +299,7 @@ struct Flow {
         iperm[dev] = (int)ref;
       }
     for (int k = 0; k < ncolr; ++k)
-      if (cnt[k]) sched.lv.push_back(LevelDesc{(int)base[k], cnt[k], cnt[k], nstr + 1, -1});
+      if (cnt[k]) sched.cols.push_back(ColDesc{(int)base[k], cnt[k], nstr + 1});
   }
 
   void build_pattern(const int *tetra) {
@@ -307,15 +339,59 @@ struct Flow {
         CG_REQUIRE(f != e && *f == r, "pattern not symmetric");
         htpos[p] = hrp[j] + (int)(f - b);
       }
-    rowptr.upload(hrp, st);
-    col.upload(hcol, st);
-    diagpos.upload(hdiag, st);
-    tpos.upload(htpos, st);
+    // SELL-64: slice s holds rows 64s..64s+63, entry j of row r at
+    // so[s] + 64 j + (r & 63); padding = own row id, zero value.
+    const int ns = (n + 63) / 64;
+    std::vector<int> hso(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+      int w = 0;
+      for (int r = 64 * s; r < std::min(n, 64 * s + 64); ++r) w = std::max(w, cnt[r]);
+      CG_REQUIRE((long)hso[s] + 64L * w < (1L << 31), "matrix too large for 32-bit slots");
+      hso[s + 1] = hso[s] + 64 * w;
+    }
+    nnz_sell = hso[ns];
+    auto spos = [&](int r, int j) { return hso[r >> 6] + 64 * j + (r & 63); };
+    std::vector<int> hsc(nnz_sell), hst(nnz_sell, -1), hlen(n), hsd(n);
+    for (int s = 0; s < ns; ++s)  // padding slots point at their own row
+      for (int q = hso[s]; q < hso[s + 1]; ++q) hsc[q] = std::min(n - 1, 64 * s + ((q - hso[s]) & 63));
+    for (int r = 0; r < n; ++r) {
+      hlen[r] = hrp[r + 1] - hrp[r];
+      hsd[r] = hdiag[r] - hrp[r];
+      for (int p = hrp[r]; p < hrp[r + 1]; ++p) {
+        const int j = p - hrp[r];
+        hsc[spos(r, j)] = hcol[p];
+        const int c = hcol[p];
+        hst[spos(r, j)] = spos(c, htpos[p] - hrp[c]);
+      }
+    }
+    so.upload(hso, st);
+    scol.upload(hsc, st);
+    slen.upload(hlen, st);
+    sdiag.upload(hsd, st);
+    stpos.upload(hst, st);
     if (ordering == CATHY_GPU_ORDER_NATURAL) natural_levels(n, hrp, hcol, sched, st);
+    // every level row must have its same-column neighbours where the column
+    // kernels expect them: predecessor = last lower slot, successor = first
+    // upper slot
+    if (ordering == CATHY_GPU_ORDER_COLUMN3)
+      for (const ColDesc &C : sched.cols)
+        for (int k = 0; k < C.len; ++k)
+          for (int t = 0; t < C.nc; ++t) {
+            const int r = C.base + t + k * C.nc;
+            if (k > 0) CG_REQUIRE(hsd[r] >= 1 && hsc[spos(r, hsd[r] - 1)] == r - C.nc, "column predecessor slot");
+            if (k + 1 < C.len)
+              CG_REQUIRE(hsd[r] + 1 < hlen[r] && hsc[spos(r, hsd[r] + 1)] == r + C.nc, "column successor slot");
+            for (int j = 0; j < hlen[r]; ++j) {  // no coupling inside a colour class but the chain
+              const int c = hsc[spos(r, j)];
+              if (c >= C.base && c < C.base + C.nc * C.len && c != r)
+                CG_REQUIRE(c == r - C.nc || c == r + C.nc, "colour class not independent");
+            }
+          }
     h_rowptr = std::move(hrp);
     h_col = std::move(hcol);
+    h_so = std::move(hso);
   }
-  std::vector<int> h_rowptr, h_col;
+  std::vector<int> h_rowptr, h_col, h_so;
 
   void build_elements(const int *tetra, const double *bi, const double *ci, const double *di,
                       const double *volu_, const int *ivol, const double *permx, const double *permy,
@@ -370,7 +446,8 @@ struct Flow {
       for (int l = 0; l < 4; ++l)
         lm[k * 4 + l] = lump ? (k == l ? 0.25 : 0.0) : (k == l ? 0.1 : 0.05);
     std::vector<double> hm4d(n, 0.0), hm4f;
-    if (!lump) hm4f.assign(nnz, 0.0);
+    if (!lump) hm4f.assign(nnz_sell, 0.0);
+    auto spos = [&](int r, int j) { return h_so[r >> 6] + 64 * j + (r & 63); };
     for (int r = 0; r < n; ++r)
       for (int i = icnt[r]; i < icnt[r + 1]; ++i) {
         const int e = hinc[i] >> 2, k = hinc[i] & 3;
@@ -380,7 +457,7 @@ struct Flow {
         } else {
           for (int l = 0; l < 4; ++l) {
             const int o = (hoff[i] >> (8 * l)) & 0xff;
-            hm4f[h_rowptr[r] + o] = hm4f[h_rowptr[r] + o] + ve * lm[k * 4 + l];
+            hm4f[spos(r, o)] = hm4f[spos(r, o)] + ve * lm[k * 4 + l];
           }
         }
       }
@@ -445,11 +522,18 @@ struct Flow {
       b->alloc(n);
     ckrwe.alloc(nt);
     et1e.alloc(nt);
-    A.alloc(nnz);
+    A.alloc(nnz_sell);
+    CG_HIP(hipMemsetAsync(A.p, 0, sizeof(double) * nnz_sell, st));
     isdir.alloc(n);
     ifatm.alloc(nnod); ifatmp.alloc(nnod);
     atmpot.alloc(nnod); atmact.alloc(nnod); atmold.alloc(nnod); qbot.alloc(nnod);
-    if (f32) F32.alloc(nnz); else F64.alloc(nnz);
+    if (f32) {
+      F32.alloc(nnz_sell);
+      CG_HIP(hipMemsetAsync(F32.p, 0, sizeof(float) * nnz_sell, st));
+    } else {
+      F64.alloc(nnz_sell);
+      CG_HIP(hipMemsetAsync(F64.p, 0, sizeof(double) * nnz_sell, st));
+    }
     part.alloc((size_t)kSlots * kMaxRedBlocks);
     ipart.alloc(kMaxRedBlocks);
     scal.alloc(1);
@@ -461,72 +545,94 @@ struct Flow {
   }
 
   // ------------------------------------------------------- kernels glue --
-  void precond_apply(const double *tin, double *out, const int *done) {
+  size_t col_smem(const ColDesc &C) const { return sizeof(double) * 2 * (size_t)C.len * kChains; }
+
+  template <typename FT>
+  void precond_apply_t(const FT *F, const double *tin, double *out, const int *done) {
     // z = (L L^T)^-1 t   (proddp.f)
-    for (size_t l = 0; l < sched.lv.size(); ++l) {
-      const LevelDesc &L = sched.lv[l];
-      const int g = grid_for(L.nchains, 64);
-      if (f32)
-        hipLaunchKernelGGL(k_sweep_fwd<float>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
-                           diagpos.p, F32.p, dinv.p, tin, out, done);
-      else
-        hipLaunchKernelGGL(k_sweep_fwd<double>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
-                           diagpos.p, F64.p, dinv.p, tin, out, done);
-    }
-    for (size_t l = sched.lv.size(); l-- > 0;) {
-      const LevelDesc &L = sched.lv[l];
-      const int g = grid_for(L.nchains, 64);
-      if (f32)
-        hipLaunchKernelGGL(k_sweep_bwd<float>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
-                           diagpos.p, F32.p, dinv.p, out, done);
-      else
-        hipLaunchKernelGGL(k_sweep_bwd<double>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
-                           diagpos.p, F64.p, dinv.p, out, done);
+    const Sell sl = sell();
+    if (ordering == CATHY_GPU_ORDER_COLUMN3) {
+      for (const ColDesc &C : sched.cols)
+        hipLaunchKernelGGL(k_fwd_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C, sl, F,
+                           dinv.p, tin, out, done);
+      for (size_t c = sched.cols.size(); c-- > 0;) {
+        const ColDesc &C = sched.cols[c];
+        hipLaunchKernelGGL(k_bwd_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C, sl, F,
+                           dinv.p, out, done);
+      }
+    } else {
+      for (const ListLevel &L : sched.lv)
+        hipLaunchKernelGGL(k_fwd_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p, sl, F,
+                           dinv.p, tin, out, done);
+      for (size_t l = sched.lv.size(); l-- > 0;) {
+        const ListLevel &L = sched.lv[l];
+        hipLaunchKernelGGL(k_bwd_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p, sl, F,
+                           dinv.p, out, done);
+      }
     }
   }
+  void precond_apply(const double *tin, double *out, const int *done) {
+    if (f32) precond_apply_t<float>(F32.p, tin, out, done);
+    else precond_apply_t<double>(F64.p, tin, out, done);
+  }
 
-  void factor() {
-    for (size_t l = 0; l < sched.lv.size(); ++l) {
-      const LevelDesc &L = sched.lv[l];
-      const int g = grid_for(L.nchains, 64);
-      if (f32)
-        hipLaunchKernelGGL(k_ic_factor<float>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
-                           diagpos.p, tpos.p, A.p, isdir.p, F32.p, dinv.p, flags.p + 1);
-      else
-        hipLaunchKernelGGL(k_ic_factor<double>, dim3(g), dim3(64), 0, st, L, sched.rows.p, rowptr.p, col.p,
-                           diagpos.p, tpos.p, A.p, isdir.p, F64.p, dinv.p, flags.p + 1);
+  template <typename FT>
+  void factor_t(FT *F) {
+    const Sell sl = sell();
+    if (ordering == CATHY_GPU_ORDER_COLUMN3) {
+      for (const ColDesc &C : sched.cols)
+        hipLaunchKernelGGL(k_ic_factor_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C,
+                           sl, stpos.p, A.p, isdir.p, F, dinv.p, flags.p + 1);
+    } else {
+      for (const ListLevel &L : sched.lv)
+        hipLaunchKernelGGL(k_ic_factor_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p,
+                           sl, stpos.p, A.p, isdir.p, F, dinv.p, flags.p + 1);
     }
+  }
+  void factor() {
+    if (f32) factor_t<float>(F32.p);
+    else factor_t<double>(F64.p);
   }
 
   // SYMSLV on the current A / TNOTI: x = pdiff.  Returns NITER; err out.
   int pcg(double tol, int imax, double *err_out, int expect) {
-    const int G = red_grid(n);
+    const int G = grid_for(n, kBlock);  // one thread per row (<= kMaxRedBlocks groups)
     factor();
     int *done = flags.p;
     precond_apply(tnoti.p, pdiff.p, nullptr);  // x0 = M^-1 b (symslv :4686)
-    hipLaunchKernelGGL(k_residual0, dim3(G), dim3(kBlock), 0, st, n, rowptr.p, col.p, A.p, pdiff.p,
-                       tnoti.p, isdir.p, r.p, P(0));
+    hipLaunchKernelGGL(k_residual0, dim3(G), dim3(kBlock), 0, st, n, sell(), A.p, pdiff.p, tnoti.p, isdir.p,
+                       r.p, P(0));
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kBlock), 0, st, G, P(0), scal.p, done, tol, imax);
     precond_apply(r.p, p.p, nullptr);  // p = M^-1 r
+    const int Gr = red_grid(n);  // grid-stride vector kernels
     int batch = std::max(1, std::min(expect, imax));
     int launched = 0;
     int h_done = 0;
     PcgScal hs{};
     while (true) {
       for (int b = 0; b < batch; ++b) {
-        hipLaunchKernelGGL(k_spmv_dot2, dim3(G), dim3(kBlock), 0, st, n, rowptr.p, col.p, A.p, p.p, r.p,
-                           Bv.p, P(1), P(2), done);
-        hipLaunchKernelGGL(k_pcg_alpha, dim3(G), dim3(kBlock), 0, st, n, G, P(1), P(2), Bv.p, p.p,
+        if (prof) CG_HIP(hipEventRecord(next_event(), st));
+        hipLaunchKernelGGL(k_spmv_dot2, dim3(G), dim3(kBlock), 0, st, n, sell(), A.p, p.p, r.p, Bv.p, P(1),
+                           P(2), done);
+        if (prof) CG_HIP(hipEventRecord(next_event(), st));
+        hipLaunchKernelGGL(k_pcg_alpha, dim3(Gr), dim3(kBlock), 0, st, n, G, P(1), P(2), Bv.p, p.p,
                            isdir.p, r.p, pdiff.p, P(3), scal.p, done);
         precond_apply(r.p, z.p, done);
-        hipLaunchKernelGGL(k_dot, dim3(G), dim3(kBlock), 0, st, n, Bv.p, z.p, P(4), done);
-        hipLaunchKernelGGL(k_pcg_beta, dim3(G), dim3(kBlock), 0, st, n, G, P(4), P(3), z.p, p.p, scal.p,
+        hipLaunchKernelGGL(k_dot, dim3(Gr), dim3(kBlock), 0, st, n, Bv.p, z.p, P(4), done);
+        hipLaunchKernelGGL(k_pcg_beta, dim3(Gr), dim3(kBlock), 0, st, n, Gr, P(4), P(3), z.p, p.p, scal.p,
                            done);
       }
       launched += batch;
       CG_HIP(hipMemcpyAsync(&h_done, done, sizeof(int), hipMemcpyDeviceToHost, st));
       CG_HIP(hipMemcpyAsync(&hs, scal.p, sizeof(PcgScal), hipMemcpyDeviceToHost, st));
       CG_HIP(hipStreamSynchronize(st));
+      if (prof) {
+        // launches after convergence exit at once; count only the live ones
+        const int executed = h_done ? hs.niter : hs.niter - 1;  // bodies run so far
+        const int live = std::max(0, std::min(batch, executed - (launched - batch)));
+        pev_used = std::min(pev_used, 2 * live);
+        collect_events();
+      }
       if (h_done) break;
       CG_REQUIRE(launched <= imax + 2, "PCG did not stop");
       batch = std::max(2, std::min(8, imax - hs.niter + 1));
@@ -545,7 +651,7 @@ struct Flow {
     hipLaunchKernelGGL(k_bc_rows, dim3(gn), dim3(kBlock), 0, st, n, row_surf.p, qnode.p, ifatm.p, atmact.p,
                        atmold.p, prm.tetaf, isdir.p, bcflux.p);
     AsmArgs a{};
-    a.n = n; a.rowptr = rowptr.p; a.col = col.p; a.diagpos = diagpos.p; a.inc_ptr = inc_ptr.p;
+    a.n = n; a.sl = sell(); a.inc_ptr = inc_ptr.p;
     a.inc = inc.p; a.incoff = incoff.p; a.S = S.p; a.gz = gz.p; a.volu = volu.p; a.ckrwe = ckrwe.p;
     a.et1e = et1e.p; a.m4d = m4d.p; a.m4full = m4full.p; a.lmass = lmass.p; a.ptnew = ptnew.p;
     a.pnew = pnew.p; a.ptimep = ptimep.p; a.swnew = swnew.p; a.swtimep = swtimep.p; a.et2 = et2.p;
@@ -576,8 +682,8 @@ struct Flow {
   // MASBAL (masbal.f:37-107) + NORMS (norms.f); result in h_res.
   void masbal_norms() {
     const int gs = grid_for(nnod);
-    hipLaunchKernelGGL(k_bkpic, dim3(gs), dim3(kBlock), 0, st, nnod, surf_row.p, rowptr.p, col.p, A.p,
-                       pdiff.p, xt5.p, ifatm.p, atmold.p, prm.tetaf, atmact.p);
+    hipLaunchKernelGGL(k_bkpic, dim3(gs), dim3(kBlock), 0, st, nnod, surf_row.p, sell(), A.p, pdiff.p, xt5.p,
+                       ifatm.p, atmold.p, prm.tetaf, atmact.p);
     const int G = red_grid(n), Gf = red_grid(nnod);
     NormPart np{P(5), P(6), P(7), P(8), P(9), P(10), ipart.p};
     hipLaunchKernelGGL(k_norms, dim3(G), dim3(kBlock), 0, st, n, refid.p, pnew.p, pold.p, tnoti.p, swnew.p,
@@ -1068,6 +1174,57 @@ int cathy_gpu_flow3d_set_pnew(int64_t *handle, const double *pnew) {
     for (int i = 0; i < f->n; ++i) h[f->perm[i]] = pnew[i];
     f->pnew.upload(h, f->st);
     CG_HIP(hipStreamSynchronize(f->st));
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_profile(int64_t *handle, const int *enable, double *spmv_ms, int64_t *launches) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    if (spmv_ms) *spmv_ms = f->spmv_ms;
+    if (launches) *launches = f->spmv_launches;
+    if (enable) {
+      f->prof = *enable != 0;
+      f->spmv_ms = 0.0;
+      f->spmv_launches = 0;
+      f->pev_used = 0;
+    }
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_spmv_bytes(int64_t *handle, double *bytes) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(bytes, "null argument");
+    // values (8 B) + column ids (4 B) per entry, row pointers, x read, y write
+    *bytes = 12.0 * f->nnz + 4.0 * (f->n + 1) + 16.0 * f->n;
+    return 0;
+  });
+}
+
+int cathy_gpu_calib_copy(const int64_t *nbytes, const int *reps, double *ms_per_launch) {
+  return guarded([&] {
+    CG_REQUIRE(nbytes && reps && ms_per_launch && *nbytes >= 8 && *reps >= 1, "bad argument");
+    const size_t n = (size_t)(*nbytes / 8);
+    DBuf<double> a(n), b(n);
+    hipStream_t s;
+    CG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    CG_HIP(hipEventCreate(&e0));
+    CG_HIP(hipEventCreate(&e1));
+    const int g = (int)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_fill, dim3(g), dim3(kBlock), 0, s, n, 1.0, a.p);
+    CG_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < *reps; ++i) hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBlock), 0, s, n, a.p, b.p);
+    CG_HIP(hipEventRecord(e1, s));
+    CG_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_launch = ms / *reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
     return 0;
   });
 }

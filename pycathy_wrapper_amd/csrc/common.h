@@ -17,7 +17,7 @@ constexpr int kMaxRow = 32;
 constexpr int kBlock = 256;
 // Upper bound on work-groups of reduction kernels: partial sums are
 // written per work-group and summed in a fixed order (deterministic).
-constexpr int kMaxRedBlocks = 1024;
+constexpr int kMaxRedBlocks = 8192;
 
 struct Error : std::runtime_error {
   int code;

@@ -1,4 +1,15 @@
 // flow_kernels.h -- declarations of the Picard/PCG kernels.
+//
+// Matrix layout: SELL-64.  Rows (device numbering) are grouped in slices of
+// 64; slice s stores its rows' entries column-major, entry j of row r at
+//   so[r >> 6] + 64*j + (r & 63),
+// so the 64 lanes of a wavefront reading entry j of 64 consecutive rows
+// touch one contiguous 512-byte (FP64) / 256-byte (FP32) segment.  Slice
+// width = longest row of the slice; padding slots hold the row's own id and
+// a zero value.  Each row's entries are sorted by column; len[r] is its
+// length and diag[r] the slot of its diagonal.  The IC(0) factor lives in
+// the same layout: lower slots L_rj, upper slots the transpose L_jr,
+// diagonal L_rr.
 #pragma once
 
 #include "kernels.h"
@@ -6,12 +17,26 @@
 namespace cg {
 
 constexpr int kAsmBlock = 128;  // rows per assembly work-group (LDS row buffers)
+constexpr int kChains = 64;     // chains per work-group in the column kernels
 
-// One level of a triangular-sweep / factorisation schedule: `nchains`
-// independent chains; chain t is the rows base + t + k*stride, k < len
-// (strided form) or the single row lvl_rows[rows_off + t] (list form).
-struct LevelDesc {
-  int base, nchains, stride, len, rows_off;
+struct Sell {
+  const int *so;    // slice offsets
+  const int *col;   // column id per slot
+  const int *len;   // row lengths
+  const int *diag;  // slot of the diagonal
+};
+
+__device__ __forceinline__ int sp(const int *so, int r, int j) { return so[r >> 6] + j * 64 + (r & 63); }
+
+// One colour class of the column ordering: chains t < nc, chain t = rows
+// base + t + k*nc for k < len (k = node plane, 0 = surface).
+struct ColDesc {
+  int base, nc, len;
+};
+
+// One level of the natural-order schedule: rows lvl_rows[off .. off+cnt).
+struct ListLevel {
+  int off, cnt;
 };
 
 struct PcgScal {
@@ -26,7 +51,8 @@ struct NormPart {
 
 struct AsmArgs {
   int n;
-  const int *rowptr, *col, *diagpos, *inc_ptr, *inc;
+  Sell sl;
+  const int *inc_ptr, *inc;
   const uint32_t *incoff;
   const double *S, *gz, *volu, *ckrwe, *et1e, *m4d, *m4full, *lmass;
   const double *ptnew, *pnew, *ptimep, *swnew, *swtimep, *et2;
@@ -46,21 +72,29 @@ __global__ void k_bc_rows(int n, const int *row_surf, const double *qnode, const
                           double *bcflux);
 template <bool LUMPED>
 __global__ void k_assemble(AsmArgs a);
+
 template <typename FT>
-__global__ void k_ic_factor(LevelDesc L, const int *lvl_rows, const int *rowptr, const int *col,
-                            const int *diagpos, const int *tpos, const double *A, const uint8_t *isdir, FT *F,
-                            double *dinv, int *nfall);
+__global__ void k_ic_factor_col3(ColDesc C, Sell sl, const int *tpos, const double *A, const uint8_t *isdir,
+                                 FT *F, double *dinv, int *nfall);
 template <typename FT>
-__global__ void k_sweep_fwd(LevelDesc L, const int *lvl_rows, const int *rowptr, const int *col,
-                            const int *diagpos, const FT *F, const double *dinv, const double *tin, double *y,
-                            const int *done);
+__global__ void k_fwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, const double *tin, double *y,
+                           const int *done);
 template <typename FT>
-__global__ void k_sweep_bwd(LevelDesc L, const int *lvl_rows, const int *rowptr, const int *col,
-                            const int *diagpos, const FT *F, const double *dinv, double *x, const int *done);
-__global__ void k_residual0(int n, const int *rowptr, const int *col, const double *A, const double *x,
-                            const double *b, const uint8_t *isdir, double *r, double *part);
-__global__ void k_spmv_dot2(int n, const int *rowptr, const int *col, const double *A, const double *p,
-                            const double *r, double *B, double *part_pr, double *part_pb, const int *done);
+__global__ void k_bwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, double *x, const int *done);
+template <typename FT>
+__global__ void k_ic_factor_list(ListLevel L, const int *rows, Sell sl, const int *tpos, const double *A,
+                                 const uint8_t *isdir, FT *F, double *dinv, int *nfall);
+template <typename FT>
+__global__ void k_fwd_list(ListLevel L, const int *rows, Sell sl, const FT *F, const double *dinv,
+                           const double *tin, double *y, const int *done);
+template <typename FT>
+__global__ void k_bwd_list(ListLevel L, const int *rows, Sell sl, const FT *F, const double *dinv, double *x,
+                           const int *done);
+
+__global__ void k_residual0(int n, Sell sl, const double *A, const double *x, const double *b,
+                            const uint8_t *isdir, double *r, double *part);
+__global__ void k_spmv_dot2(int n, Sell sl, const double *A, const double *p, const double *r, double *B,
+                            double *part_pr, double *part_pb, const int *done);
 __global__ void k_pcg_alpha(int n, int nparts, const double *part_pr, const double *part_pb, const double *B,
                             const double *p, const uint8_t *isdir, double *r, double *x, double *part_rr,
                             PcgScal *sc, const int *done);
@@ -70,9 +104,8 @@ __global__ void k_pcg_beta(int n, int nparts, const double *part_bz, const doubl
 __global__ void k_pcg_init(int nparts, const double *part, PcgScal *sc, int *done, double tol, int imax);
 __global__ void k_update_pnew(int n, const double *pdiff, const int *row_surf, const int *ifatm,
                               const double *pold, double *pnew);
-__global__ void k_bkpic(int nnod, const int *surf_row, const int *rowptr, const int *col, const double *A,
-                        const double *pdiff, const double *xt5, const int *ifatm, const double *atmold,
-                        double tetaf, double *atmact);
+__global__ void k_bkpic(int nnod, const int *surf_row, Sell sl, const double *A, const double *pdiff,
+                        const double *xt5, const int *ifatm, const double *atmold, double tetaf, double *atmact);
 __global__ void k_norms(int n, const int *refid, const double *pnew, const double *pold, const double *tnoti,
                         const double *swnew, const double *swtimep, const double *ptimep, const double *volnod,
                         const double *snodi, const double *pnodi, NormPart part);
@@ -92,7 +125,6 @@ __global__ void k_free_drainage(int nnod, const int *bot_row, const double *aren
 __global__ void k_store(int n, const double *sw, const double *volnod, const double *pnodi, double *part);
 __global__ void k_sum_final(int nb, const double *part, double *out);
 __global__ void k_copy(size_t n, const double *src, double *dst);
-__global__ void k_copy_int(int n, const int *src, int *dst);
 __global__ void k_fill(size_t n, double v, double *dst);
 
 }  // namespace cg

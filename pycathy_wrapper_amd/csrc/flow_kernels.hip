@@ -75,7 +75,7 @@ __global__ void k_bc_rows(int n, const int *__restrict__ row_surf,
   bcflux[r] = f;
 }
 
-// Upper-pair slot of (k,l), k <= l, in an element's 10 stiffness values.
+// Upper-pair slot of (k,l) in an element's 10 stiffness values.
 __device__ __forceinline__ int upair(int k, int l) {
   int a = k < l ? k : l, b = k < l ? l : k;
   return a * 4 - (a * (a - 1)) / 2 + (b - a);
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kAsmBlock) void k_assemble(AsmArgs a) {
   __shared__ double Ms[LUMPED ? 1 : kMaxRow * kAsmBlock];
   int r = blockIdx.x * kAsmBlock + threadIdx.x;
   if (r >= a.n) return;
-  const int beg = a.rowptr[r], len = a.rowptr[r + 1] - beg;
+  const int len = a.sl.len[r];
   double *K = Ks + threadIdx.x;
   double *M = LUMPED ? nullptr : Ms + threadIdx.x;
   for (int j = 0; j < len; ++j) {
@@ -129,20 +129,21 @@ __global__ __launch_bounds__(kAsmBlock) void k_assemble(AsmArgs a) {
     }
     grav = grav + kr * a.gz[(size_t)e * 4 + k];
   }
-  const int dg = a.diagpos[r] - beg;
+  const int dg = a.sl.diag[r];
   const double rdt = a.rdt;
   double t = 0.0;
-  for (int j = 0; j < len; ++j) t = t - K[j * kAsmBlock] * a.ptnew[a.col[beg + j]];
+  for (int j = 0; j < len; ++j) t = t - K[j * kAsmBlock] * a.ptnew[a.sl.col[sp(a.sl.so, r, j)]];
   double m4r;
   if (LUMPED) {
     m4r = a.m4d[r];
     t = t - (m2d * rdt) * (a.pnew[r] - a.ptimep[r]) - (m4r * rdt) * (a.swnew[r] - a.swtimep[r]);
   } else {
-    m4r = a.m4full[beg + dg];
+    m4r = a.m4full[sp(a.sl.so, r, dg)];
     for (int j = 0; j < len; ++j) {
-      const int c = a.col[beg + j];
+      const int q = sp(a.sl.so, r, j);
+      const int c = a.sl.col[q];
       t = t - (M[j * kAsmBlock] * rdt) * (a.pnew[c] - a.ptimep[c]) -
-          (a.m4full[beg + j] * rdt) * (a.swnew[c] - a.swtimep[c]);
+          (a.m4full[q] * rdt) * (a.swnew[c] - a.swtimep[c]);
     }
   }
   t = t - grav;
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(kAsmBlock) void k_assemble(AsmArgs a) {
       v = v + M[j * kAsmBlock] * rdt;
       if (j == dg) v = v + m5 * rdt;
     }
-    a.A[beg + j] = v;
+    a.A[sp(a.sl.so, r, j)] = v;
   }
 }
 template __global__ void k_assemble<true>(AsmArgs);
@@ -167,159 +168,358 @@ template __global__ void k_assemble<false>(AsmArgs);
 // ------------------------------------------------------------ IC(0) --
 // incldp.f restated as an up-looking IC(0) on the full pattern: for each
 // row r in elimination order, L_rj = (a_rj - sum_{m<j} L_rm L_jm) / L_jj for
-// its lower entries j (ascending), L_rr = sqrt(a_rr - sum L_rj^2).  A chain
-// is a run of rows one thread eliminates in order; chains of one level are
-// independent.  Dirichlet rows/columns are removed (identity, zero inverse
-// diagonal) -- what the 1e-9*RMAX diagonal penalty of bcpic.f does in exact
-// arithmetic.  The factor's transpose is mirrored into the upper half so
-// both sweeps read rows.
+// its lower entries j (ascending), L_rr = sqrt(a_rr - sum L_rj^2).
+// Dirichlet rows/columns are removed (identity, zero inverse diagonal) --
+// what the 1e-9*RMAX diagonal penalty of bcpic.f does in exact arithmetic.
+// The factor's transpose is mirrored into the upper slots so both sweeps
+// read rows.
+
+// Sparse dot of the lower parts of rows r and c over columns < limit
+// (both rows sorted): sum L_rm L_cm.  ra/rc = number of slots scanned.
 template <typename FT>
-__global__ void k_ic_factor(LevelDesc L, const int *__restrict__ lvl_rows,
-                            const int *__restrict__ rowptr, const int *__restrict__ col,
-                            const int *__restrict__ diagpos, const int *__restrict__ tpos,
-                            const double *__restrict__ A, const uint8_t *__restrict__ isdir,
-                            FT *__restrict__ F, double *__restrict__ dinv, int *nfall) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.nchains) return;
-  for (int k = 0; k < L.len; ++k) {
-    const int r = L.rows_off >= 0 ? lvl_rows[L.rows_off + t] : L.base + t + k * L.stride;
-    const int beg = rowptr[r], d = diagpos[r];
+__device__ __forceinline__ double row_dot(const Sell &sl, const FT *F, int r, int ra, int c, int rc) {
+  double v = 0.0;
+  int a = 0, b = 0;
+  while (a < ra && b < rc) {
+    const int pa = sp(sl.so, r, a), pb = sp(sl.so, c, b);
+    const int ca = sl.col[pa], cb = sl.col[pb];
+    if (ca < cb) {
+      ++a;
+    } else if (ca > cb) {
+      ++b;
+    } else {
+      v += (double)F[pa] * (double)F[pb];
+      ++a;
+      ++b;
+    }
+  }
+  return v;
+}
+
+// Column ordering, one colour class.  Each work-group owns kChains chains.
+// In this ordering a row's lower neighbours are (a) nodes of earlier
+// colours, final before the launch, and (b) its same-column predecessor
+// (plane k-1), whose entries common with the row are all of earlier
+// colours.  So: pass 1a computes every row's earlier-colour L entries in
+// parallel, pass 1b the predecessor's numerator, and pass 2 runs the
+// 2-term recurrence d_k = sqrt(s_k - (q_k / d_{k-1})^2) down each chain.
+template <typename FT>
+__global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, const int *__restrict__ tpos,
+                                                        const double *__restrict__ A,
+                                                        const uint8_t *__restrict__ isdir, FT *__restrict__ F,
+                                                        double *__restrict__ dinv, int *nfall) {
+  extern __shared__ double smem[];
+  double *Sv = smem;                      // a_rr - sum earlier L^2
+  double *Qv = smem + C.len * kChains;    // a_r,pred - earlier dot
+  const int t0 = blockIdx.x * kChains;
+  const int tot = C.len * kChains;
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int k = i / kChains, t = t0 + (i % kChains);
+    if (t >= C.nc) continue;
+    const int r = C.base + t + k * C.nc;
+    const int jd = sl.diag[r];
+    const int jp = k > 0 ? jd - 1 : jd;
     if (isdir[r]) {
-      for (int p = beg; p < d; ++p) {
-        F[p] = FT(0);
-        F[tpos[p]] = FT(0);
+      for (int j = 0; j < jp; ++j) {
+        const int pj = sp(sl.so, r, j);
+        F[pj] = FT(0);
+        F[tpos[pj]] = FT(0);
       }
-      F[d] = FT(1);
-      dinv[r] = 0.0;
+      Sv[i] = 1.0;
       continue;
     }
-    double sumsq = 0.0;
-    for (int p = beg; p < d; ++p) {
-      const int j = col[p];
-      double l = 0.0;
-      if (!isdir[j]) {
-        double s = A[p];
-        int q = beg, u = rowptr[j];
-        const int ue = diagpos[j];
-        while (q < p && u < ue) {
-          const int cq = col[q], cu = col[u];
-          if (cq < cu) ++q;
-          else if (cq > cu) ++u;
-          else {
-            s -= (double)F[q] * (double)F[u];
-            ++q;
-            ++u;
+    double s = A[sp(sl.so, r, jd)];
+    for (int j = 0; j < jp; ++j) {
+      const int pj = sp(sl.so, r, j);
+      const int c = sl.col[pj];
+      double L = 0.0;
+      if (!isdir[c]) L = (A[pj] - row_dot(sl, F, r, j, c, sl.diag[c])) * dinv[c];
+      F[pj] = FT(L);
+      F[tpos[pj]] = FT(L);
+      s -= L * L;
+    }
+    Sv[i] = s;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int k = i / kChains, t = t0 + (i % kChains);
+    if (t >= C.nc || k == 0) continue;
+    const int r = C.base + t + k * C.nc;
+    if (isdir[r]) continue;
+    const int jp = sl.diag[r] - 1;
+    const int pr = r - C.nc;
+    const int jpp = k > 1 ? sl.diag[pr] - 1 : sl.diag[pr];
+    Qv[i] = A[sp(sl.so, r, jp)] - row_dot(sl, F, r, jp, pr, jpp);
+  }
+  __syncthreads();
+  if (threadIdx.x < kChains) {
+    const int t = t0 + threadIdx.x;
+    if (t < C.nc) {
+      double dprev = 1.0;
+      bool pdir = true;
+      for (int k = 0; k < C.len; ++k) {
+        const int r = C.base + t + k * C.nc;
+        const int i = k * kChains + threadIdx.x;
+        const int jd = sl.diag[r];
+        if (isdir[r]) {
+          if (k > 0) {
+            const int pj = sp(sl.so, r, jd - 1);
+            F[pj] = FT(0);
+            F[tpos[pj]] = FT(0);
           }
+          F[sp(sl.so, r, jd)] = FT(1);
+          dinv[r] = 0.0;
+          pdir = true;
+          continue;
         }
-        l = s * dinv[j];
+        double l = 0.0;
+        if (k > 0) {
+          if (!pdir) l = Qv[i] / dprev;
+          const int pj = sp(sl.so, r, jd - 1);
+          F[pj] = FT(l);
+          F[tpos[pj]] = FT(l);
+        }
+        double v = Sv[i] - l * l;
+        if (!(v > 0.0)) {  // breakdown: keep the row's own diagonal
+          const double a = fabs(A[sp(sl.so, r, jd)]);
+          v = a > 0.0 ? a : 1.0;
+          atomicAdd(nfall, 1);
+        }
+        const double d = sqrt(v);
+        F[sp(sl.so, r, jd)] = FT(d);
+        dinv[r] = 1.0 / d;
+        dprev = d;
+        pdir = false;
       }
-      F[p] = FT(l);
-      F[tpos[p]] = FT(l);
-      sumsq += l * l;
     }
-    double av = A[d] - sumsq;
-    if (!(av > 0.0)) {  // breakdown: keep the row's own diagonal
-      av = fabs(A[d]) > 0.0 ? fabs(A[d]) : 1.0;
-      atomicAdd(nfall, 1);
-    }
-    const double lrr = sqrt(av);
-    F[d] = FT(lrr);
-    dinv[r] = 1.0 / lrr;
   }
 }
 
-// Forward sweep L y = t (proddp, solscal-extended.f:4175-4185): rows in chain order.
+// Forward sweep L y = t (proddp, solscal-extended.f:4175-4185) on one
+// colour: pass 1 gathers the earlier-colour terms of every row, pass 2
+// runs y_k = (t_k - L_k,k-1 y_k-1) / L_kk down each chain.
 template <typename FT>
-__global__ void k_sweep_fwd(LevelDesc L, const int *__restrict__ lvl_rows,
-                            const int *__restrict__ rowptr, const int *__restrict__ col,
-                            const int *__restrict__ diagpos, const FT *__restrict__ F,
-                            const double *__restrict__ dinv, const double *__restrict__ tin,
-                            double *__restrict__ y, const int *__restrict__ done) {
+__global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
+                                                  const double *__restrict__ dinv, const double *__restrict__ tin,
+                                                  double *__restrict__ y, const int *__restrict__ done) {
   if (done && *done) return;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.nchains) return;
-  for (int k = 0; k < L.len; ++k) {
-    const int r = L.rows_off >= 0 ? lvl_rows[L.rows_off + t] : L.base + t + k * L.stride;
+  extern __shared__ double smem[];
+  double *T = smem, *Lp = smem + C.len * kChains;
+  const int t0 = blockIdx.x * kChains;
+  const int tot = C.len * kChains;
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int k = i / kChains, t = t0 + (i % kChains);
+    if (t >= C.nc) continue;
+    const int r = C.base + t + k * C.nc;
+    const int jd = sl.diag[r];
+    const int jp = k > 0 ? jd - 1 : jd;
     double s = tin[r];
-    const int d = diagpos[r];
-    for (int p = rowptr[r]; p < d; ++p) s -= (double)F[p] * y[col[p]];
-    y[r] = s * dinv[r];
+    for (int j = 0; j < jp; ++j) {
+      const int pj = sp(sl.so, r, j);
+      s -= (double)F[pj] * y[sl.col[pj]];
+    }
+    T[i] = s;
+    Lp[i] = k > 0 ? (double)F[sp(sl.so, r, jp)] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x < kChains) {
+    const int t = t0 + threadIdx.x;
+    if (t < C.nc) {
+      double yp = 0.0;
+      for (int k = 0; k < C.len; ++k) {
+        const int r = C.base + t + k * C.nc;
+        const int i = k * kChains + threadIdx.x;
+        const double v = (T[i] - Lp[i] * yp) * dinv[r];
+        y[r] = v;
+        yp = v;
+      }
+    }
   }
 }
 
-// Backward sweep L^T x = y (proddp, solscal-extended.f:4186-4197), in place, chains and
-// levels in reverse.
+// Backward sweep L^T x = y (proddp, solscal-extended.f:4186-4197) on one
+// colour, in place: later colours are final; chains run from the base up.
 template <typename FT>
-__global__ void k_sweep_bwd(LevelDesc L, const int *__restrict__ lvl_rows,
-                            const int *__restrict__ rowptr, const int *__restrict__ col,
-                            const int *__restrict__ diagpos, const FT *__restrict__ F,
-                            const double *__restrict__ dinv, double *__restrict__ x,
-                            const int *__restrict__ done) {
+__global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
+                                                  const double *__restrict__ dinv, double *__restrict__ x,
+                                                  const int *__restrict__ done) {
   if (done && *done) return;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.nchains) return;
-  for (int k = L.len - 1; k >= 0; --k) {
-    const int r = L.rows_off >= 0 ? lvl_rows[L.rows_off + t] : L.base + t + k * L.stride;
+  extern __shared__ double smem[];
+  double *T = smem, *U = smem + C.len * kChains;
+  const int t0 = blockIdx.x * kChains;
+  const int tot = C.len * kChains;
+  const int kl = C.len - 1;
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int k = i / kChains, t = t0 + (i % kChains);
+    if (t >= C.nc) continue;
+    const int r = C.base + t + k * C.nc;
+    const int jd = sl.diag[r];
+    const int js = k < kl ? jd + 1 : jd;
+    const int len = sl.len[r];
     double s = x[r];
-    const int e = rowptr[r + 1];
-    for (int p = diagpos[r] + 1; p < e; ++p) s -= (double)F[p] * x[col[p]];
-    x[r] = s * dinv[r];
+    for (int j = js + 1; j < len; ++j) {
+      const int pj = sp(sl.so, r, j);
+      s -= (double)F[pj] * x[sl.col[pj]];
+    }
+    T[i] = s;
+    U[i] = k < kl ? (double)F[sp(sl.so, r, jd + 1)] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x < kChains) {
+    const int t = t0 + threadIdx.x;
+    if (t < C.nc) {
+      double xn = 0.0;
+      for (int k = kl; k >= 0; --k) {
+        const int r = C.base + t + k * C.nc;
+        const int i = k * kChains + threadIdx.x;
+        const double v = (T[i] - U[i] * xn) * dinv[r];
+        x[r] = v;
+        xn = v;
+      }
+    }
   }
 }
 
-template __global__ void k_ic_factor<float>(LevelDesc, const int *, const int *, const int *, const int *,
-                                            const int *, const double *, const uint8_t *, float *, double *, int *);
-template __global__ void k_ic_factor<double>(LevelDesc, const int *, const int *, const int *, const int *,
-                                             const int *, const double *, const uint8_t *, double *, double *, int *);
-template __global__ void k_sweep_fwd<float>(LevelDesc, const int *, const int *, const int *, const int *,
-                                            const float *, const double *, const double *, double *, const int *);
-template __global__ void k_sweep_fwd<double>(LevelDesc, const int *, const int *, const int *, const int *,
-                                             const double *, const double *, const double *, double *, const int *);
-template __global__ void k_sweep_bwd<float>(LevelDesc, const int *, const int *, const int *, const int *,
-                                            const float *, const double *, double *, const int *);
-template __global__ void k_sweep_bwd<double>(LevelDesc, const int *, const int *, const int *, const int *,
-                                             const double *, const double *, double *, const int *);
+// Natural order: one row per thread, levels of the reference's
+// elimination DAG launched in sequence (exact INCLDP/PRODDP order).
+template <typename FT>
+__global__ void k_ic_factor_list(ListLevel L, const int *__restrict__ rows, Sell sl,
+                                 const int *__restrict__ tpos, const double *__restrict__ A,
+                                 const uint8_t *__restrict__ isdir, FT *__restrict__ F,
+                                 double *__restrict__ dinv, int *nfall) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.cnt) return;
+  const int r = rows[L.off + t];
+  const int jd = sl.diag[r];
+  if (isdir[r]) {
+    for (int j = 0; j < jd; ++j) {
+      const int pj = sp(sl.so, r, j);
+      F[pj] = FT(0);
+      F[tpos[pj]] = FT(0);
+    }
+    F[sp(sl.so, r, jd)] = FT(1);
+    dinv[r] = 0.0;
+    return;
+  }
+  double s = A[sp(sl.so, r, jd)];
+  for (int j = 0; j < jd; ++j) {
+    const int pj = sp(sl.so, r, j);
+    const int c = sl.col[pj];
+    double Lv = 0.0;
+    if (!isdir[c]) Lv = (A[pj] - row_dot(sl, F, r, j, c, sl.diag[c])) * dinv[c];
+    F[pj] = FT(Lv);
+    F[tpos[pj]] = FT(Lv);
+    s -= Lv * Lv;
+  }
+  if (!(s > 0.0)) {
+    const double a = fabs(A[sp(sl.so, r, jd)]);
+    s = a > 0.0 ? a : 1.0;
+    atomicAdd(nfall, 1);
+  }
+  const double d = sqrt(s);
+  F[sp(sl.so, r, jd)] = FT(d);
+  dinv[r] = 1.0 / d;
+}
+
+template <typename FT>
+__global__ void k_fwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, const FT *__restrict__ F,
+                           const double *__restrict__ dinv, const double *__restrict__ tin,
+                           double *__restrict__ y, const int *__restrict__ done) {
+  if (done && *done) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.cnt) return;
+  const int r = rows[L.off + t];
+  const int jd = sl.diag[r];
+  double s = tin[r];
+  for (int j = 0; j < jd; ++j) {
+    const int pj = sp(sl.so, r, j);
+    s -= (double)F[pj] * y[sl.col[pj]];
+  }
+  y[r] = s * dinv[r];
+}
+
+template <typename FT>
+__global__ void k_bwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, const FT *__restrict__ F,
+                           const double *__restrict__ dinv, double *__restrict__ x,
+                           const int *__restrict__ done) {
+  if (done && *done) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.cnt) return;
+  const int r = rows[L.off + t];
+  const int jd = sl.diag[r], len = sl.len[r];
+  double s = x[r];
+  for (int j = jd + 1; j < len; ++j) {
+    const int pj = sp(sl.so, r, j);
+    s -= (double)F[pj] * x[sl.col[pj]];
+  }
+  x[r] = s * dinv[r];
+}
+
+#define CG_INST_FT(FT)                                                                                      \
+  template __global__ void k_ic_factor_col3<FT>(ColDesc, Sell, const int *, const double *, const uint8_t *, \
+                                                FT *, double *, int *);                                     \
+  template __global__ void k_fwd_col3<FT>(ColDesc, Sell, const FT *, const double *, const double *,        \
+                                          double *, const int *);                                          \
+  template __global__ void k_bwd_col3<FT>(ColDesc, Sell, const FT *, const double *, double *, const int *); \
+  template __global__ void k_ic_factor_list<FT>(ListLevel, const int *, Sell, const int *, const double *,  \
+                                                const uint8_t *, FT *, double *, int *);                    \
+  template __global__ void k_fwd_list<FT>(ListLevel, const int *, Sell, const FT *, const double *,         \
+                                          const double *, double *, const int *);                          \
+  template __global__ void k_bwd_list<FT>(ListLevel, const int *, Sell, const FT *, const double *, double *, \
+                                          const int *);
+CG_INST_FT(float)
+CG_INST_FT(double)
+#undef CG_INST_FT
 
 // --------------------------------------------------------------- PCG --
 // graddp.f:1275-1385 with the reference's recurrences:
 //   alpha = (p,r)/(p,Ap); r -= alpha Ap; x += alpha p; z = M^-1 r;
 //   beta = -(Ap,z)/(p,Ap); p = z + beta p; err = ||r_free|| / ||b_free||.
 // Dirichlet rows carry p = z = 0, so they drop out of every product.
+// One thread per row; a wavefront covers one SELL slice, so the loop count
+// (the slice width) is uniform and every load of A / col is coalesced.
+
+__device__ __forceinline__ double sell_row_dot(const Sell &sl, const double *__restrict__ A,
+                                               const double *__restrict__ x, int r) {
+  const int s = r >> 6, lane = r & 63;
+  const int o0 = sl.so[s], w = (sl.so[s + 1] - o0) >> 6;
+  double acc = 0.0;
+  int q = o0 + lane;
+  for (int j = 0; j < w; ++j, q += 64) acc += A[q] * x[sl.col[q]];
+  return acc;
+}
 
 // Partial sum of b_free^2 (XLUNG) and r = b - A x0.
-__global__ void k_residual0(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
-                            const double *__restrict__ A, const double *__restrict__ x,
+__global__ void k_residual0(int n, Sell sl, const double *__restrict__ A, const double *__restrict__ x,
                             const double *__restrict__ b, const uint8_t *__restrict__ isdir,
                             double *__restrict__ r, double *__restrict__ part) {
   __shared__ double sh[kBlock / 64 + 1];
   double acc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double s = b[i];
-    for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) s -= A[p] * x[col[p]];
-    r[i] = s;
-    if (!isdir[i]) acc += b[i] * b[i];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double bi = b[i];
+    r[i] = bi - sell_row_dot(sl, A, x, i);
+    if (!isdir[i]) acc = bi * bi;
   }
   double t = block_sum<kBlock>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 // B = A p with partial (p,r) and (p,B).
-__global__ void k_spmv_dot2(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
-                            const double *__restrict__ A, const double *__restrict__ p,
+__global__ void k_spmv_dot2(int n, Sell sl, const double *__restrict__ A, const double *__restrict__ p,
                             const double *__restrict__ r, double *__restrict__ B,
                             double *__restrict__ part_pr, double *__restrict__ part_pb,
                             const int *__restrict__ done) {
   if (*done) return;
   __shared__ double sh[kBlock / 64 + 1];
   double apr = 0.0, apb = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double s = 0.0;
-    const int e = rowptr[i + 1];
-    for (int q = rowptr[i]; q < e; ++q) s += A[q] * p[col[q]];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double s = sell_row_dot(sl, A, p, i);
     B[i] = s;
     const double pi = p[i];
-    apr += pi * r[i];
-    apb += pi * s;
+    apr = pi * r[i];
+    apb = pi * s;
   }
   double t1 = block_sum<kBlock>(apr, sh);
   double t2 = block_sum<kBlock>(apb, sh);
@@ -425,18 +625,21 @@ __global__ void k_update_pnew(int n, const double *__restrict__ pdiff,
 
 // bkpic.f:28-54 at atmospheric Dirichlet nodes: ATMACT = (A dpsi - XT5 -
 // (1-TETAF) ATMOLD) / TETAF with the true (restored) diagonal.
-__global__ void k_bkpic(int nnod, const int *__restrict__ surf_row, const int *__restrict__ rowptr,
-                        const int *__restrict__ col, const double *__restrict__ A,
+__global__ void k_bkpic(int nnod, const int *__restrict__ surf_row, Sell sl, const double *__restrict__ A,
                         const double *__restrict__ pdiff, const double *__restrict__ xt5,
-                        const int *__restrict__ ifatm, const double *__restrict__ atmold,
-                        double tetaf, double *__restrict__ atmact) {
+                        const int *__restrict__ ifatm, const double *__restrict__ atmold, double tetaf,
+                        double *__restrict__ atmact) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnod) return;
   const int fa = ifatm[s];
   if (!(fa == 1 || fa == 2)) return;
   const int r = surf_row[s];
   double v = 0.0;
-  for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) v += A[p] * pdiff[col[p]];
+  const int len = sl.len[r];
+  for (int j = 0; j < len; ++j) {
+    const int q = sp(sl.so, r, j);
+    v += A[q] * pdiff[sl.col[q]];
+  }
   v = v - xt5[r];
   atmact[s] = (v - (1.0 - tetaf) * atmold[s]) * (1.0 / tetaf);
 }
@@ -544,21 +747,53 @@ __global__ void k_flux(int nnod, const int *__restrict__ ifatm, const double *__
 __global__ void k_norms_final(int nb, int nbf, NormPart part, const double *__restrict__ part6,
                               double *__restrict__ res) {
   __shared__ double sh[kBlock / 64 + 2];
+  __shared__ double shv[kBlock / 64], shd[kBlock / 64], shf[kBlock / 64];
+  __shared__ int shi[kBlock / 64];
   const double pl2 = reduce_partials<kBlock>(part.pl2, nb, sh);
   const double fl2 = reduce_partials<kBlock>(part.fl2, nb, sh);
   const double dst = reduce_partials<kBlock>(part.dstore, nb, sh);
   double fx[6];
   for (int k = 0; k < 6; ++k) fx[k] = reduce_partials<kBlock>(part6 + k * kMaxRedBlocks, nbf, sh);
+  double bv = -1.0, bd = 0.0, bf = 0.0;
+  int bi = -1;
+  for (int k = threadIdx.x; k < nb; k += kBlock) {
+    const double v = part.pinf[k];
+    const int id = part.pidx[k];
+    if (v > bv || (v == bv && id > bi)) {
+      bv = v;
+      bi = id;
+      bd = part.pdif[k];
+    }
+    bf = fmax(bf, part.finf[k]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    bf = fmax(bf, __shfl_down(bf, o, 64));
+    const double ov = __shfl_down(bv, o, 64);
+    const int oi = __shfl_down(bi, o, 64);
+    const double od = __shfl_down(bd, o, 64);
+    if (ov > bv || (ov == bv && oi > bi)) {
+      bv = ov;
+      bi = oi;
+      bd = od;
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    shv[w] = bv;
+    shi[w] = bi;
+    shd[w] = bd;
+    shf[w] = bf;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double bv = -1.0, bd = 0.0, bf = 0.0;
-    int bi = -1;
-    for (int k = 0; k < nb; ++k) {
-      if (part.pinf[k] > bv || (part.pinf[k] == bv && part.pidx[k] > bi)) {
-        bv = part.pinf[k];
-        bi = part.pidx[k];
-        bd = part.pdif[k];
+    bv = -1.0; bd = 0.0; bf = 0.0; bi = -1;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      if (shv[k] > bv || (shv[k] == bv && shi[k] > bi)) {
+        bv = shv[k];
+        bi = shi[k];
+        bd = shd[k];
       }
-      bf = fmax(bf, part.finf[k]);
+      bf = fmax(bf, shf[k]);
     }
     res[0] = dst;
     res[1] = sqrt(pl2);
@@ -679,10 +914,6 @@ __global__ void k_sum_final(int nb, const double *__restrict__ part, double *__r
 
 __global__ void k_copy(size_t n, const double *__restrict__ src, double *__restrict__ dst) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
-}
-__global__ void k_copy_int(int n, const int *__restrict__ src, int *__restrict__ dst) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
 }
 __global__ void k_fill(size_t n, double v, double *__restrict__ dst) {

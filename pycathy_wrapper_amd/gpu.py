@@ -143,6 +143,20 @@ class FlowGPU:
         _check(lib().cathy_gpu_flow3d_counters(C.byref(self.h), _p(out)), "counters")
         return dict(picard=out[0], pcg=out[1], fallbacks=out[2], solve_ms=out[3], step_ms=out[4])
 
+    def profile(self, enable: bool | None = None) -> tuple[float, int]:
+        """SpMV kernel timing: returns (total ms, launches) accumulated since
+        the last enable/disable; ``enable`` switches timing on/off and resets."""
+        ms = C.c_double(0.0)
+        cnt = C.c_int64(0)
+        en = None if enable is None else _i(1 if enable else 0)
+        _check(lib().cathy_gpu_flow3d_profile(C.byref(self.h), en, C.byref(ms), C.byref(cnt)), "profile")
+        return ms.value, cnt.value
+
+    def spmv_bytes(self) -> float:
+        b = C.c_double(0.0)
+        _check(lib().cathy_gpu_flow3d_spmv_bytes(C.byref(self.h), C.byref(b)), "spmv_bytes")
+        return b.value
+
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             lib().cathy_gpu_flow3d_destroy(C.byref(self.h))
@@ -152,3 +166,10 @@ class FlowGPU:
             self.close()
         except Exception:
             pass
+
+
+def calib_copy(nbytes: int, reps: int = 20) -> float:
+    """Average ms of a copy kernel reading and writing ``nbytes``."""
+    ms = C.c_double(0.0)
+    _check(lib().cathy_gpu_calib_copy(C.byref(C.c_int64(int(nbytes))), _i(reps), C.byref(ms)), "calib_copy")
+    return ms.value
