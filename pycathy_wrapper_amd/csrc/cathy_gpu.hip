@@ -80,7 +80,15 @@ struct Flow {
 
   // mesh / pattern (device numbering)
   DBuf<int> so, scol, slen, sdiag, stpos;  // SELL-64 pattern
-  int nnz_sell = 0;
+  int nnz_sell = 0, maxrow = 0;
+  DBuf<int> plan_off, plan_ie;  // lumped assembly plan
+  DBuf<uint32_t> plan_io;
+  DBuf<double4> plan_s4;
+  DBuf<double> plan_w2, plan_gw;
+  bool use_plan = false;
+  DBuf<int> pair_ptr;  // IC(0) update pattern per SELL slot
+  DBuf<int2> pair_rc;
+  DBuf<double> dbeta;
   DBuf<int> inc_ptr, inc, refid, row_surf, surf_row, bot_row;
   DBuf<uint32_t> incoff;
   DBuf<int4> enode;
@@ -163,6 +171,7 @@ struct Flow {
   double *P(int slot) { return part.p + (size_t)slot * kMaxRedBlocks; }
   int red_grid(size_t work) const { return grid_for(work, kBlock, kMaxRedBlocks); }
   Sell sell() const { return Sell{so.p, scol.p, slen.p, sdiag.p}; }
+  PairList pairs() const { return PairList{pair_ptr.p, pair_rc.p}; }
 
   // ------------------------------------------------------------ set-up --
   void setup(int n_, int nt_, int ntri_, int nnod_, int nstr_, int nzone, int maxstr, const int *tetra,
@@ -364,6 +373,43 @@ struct Flow {
         hst[spos(r, j)] = spos(c, htpos[p] - hrp[c]);
       }
     }
+    // IC(0) update pattern: for each lower slot (r,j), c = col(r,j), the
+    // slot pairs (a in row r, b in row c) with col(r,a) == col(c,b) < c
+    maxrow = 0;
+    for (int r = 0; r < n; ++r) maxrow = std::max(maxrow, hlen[r]);
+    std::vector<int> hpp((size_t)nnz_sell + 1, 0);
+    std::vector<int2> hpr;
+    hpr.reserve((size_t)n * 16);
+    {
+      std::vector<std::vector<int2>> tmp;  // per slot, filled in slot order below
+      std::vector<int> cntq(nnz_sell, 0);
+      std::vector<std::pair<int, int2>> all;
+      all.reserve((size_t)n * 16);
+      for (int r = 0; r < n; ++r)
+        for (int j = 0; j < hsd[r]; ++j) {
+          const int c = hcol[hrp[r] + j];
+          int a = 0, b = 0;
+          while (a < j && b < hsd[c]) {
+            const int ca = hcol[hrp[r] + a], cb = hcol[hrp[c] + b];
+            if (ca < cb) ++a;
+            else if (ca > cb) ++b;
+            else {
+              all.push_back({spos(r, j), make_int2(spos(r, a), spos(c, b))});
+              ++a;
+              ++b;
+            }
+          }
+        }
+      std::stable_sort(all.begin(), all.end(),
+                       [](const std::pair<int, int2> &u, const std::pair<int, int2> &v) { return u.first < v.first; });
+      for (const auto &e : all) hpp[e.first + 1]++;
+      for (int q = 0; q < nnz_sell; ++q) hpp[q + 1] += hpp[q];
+      hpr.resize(all.size());
+      for (size_t m = 0; m < all.size(); ++m) hpr[m] = all[m].second;
+    }
+    pair_ptr.upload(hpp, st);
+    if (hpr.empty()) hpr.push_back(make_int2(0, 0));
+    pair_rc.upload(hpr, st);
     so.upload(hso, st);
     scol.upload(hsc, st);
     slen.upload(hlen, st);
@@ -461,6 +507,8 @@ struct Flow {
           }
         }
       }
+    use_plan = lump && maxrow <= 31;
+    if (use_plan) build_plan(hS, hgz, hvol, hinc, hoff, icnt);
     enode.upload(hen, st);
     S.upload(hS, st);
     gz.upload(hgz, st);
@@ -472,6 +520,54 @@ struct Flow {
     if (!lump) m4full.upload(hm4f, st);
     else m4full.alloc(1);
     lmass.upload(std::vector<double>(lm, lm + 16), st);
+    CG_HIP(hipStreamSynchronize(st));
+  }
+
+  // Lumped assembly plan (see AsmPlan): per row, one record per incident
+  // element; SELL weights of the storage (V/16 per node of each incident
+  // element) and gravity (gz/4).
+  void build_plan(const std::vector<double> &hS, const std::vector<double> &hgz, const std::vector<double> &hvol,
+                  const std::vector<int> &hinc, const std::vector<uint32_t> &hoff, const std::vector<int> &icnt) {
+    const int ns = (n + 63) / 64;
+    auto spos = [&](int r, int j) { return h_so[r >> 6] + 64 * j + (r & 63); };
+    auto up = [](int k, int l) {
+      int a = k < l ? k : l, b = k < l ? l : k;
+      return a * 4 - (a * (a - 1)) / 2 + (b - a);
+    };
+    std::vector<int> hoffp(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+      int w = 0;
+      for (int r = 64 * s; r < std::min(n, 64 * s + 64); ++r) w = std::max(w, icnt[r + 1] - icnt[r]);
+      CG_REQUIRE((long)hoffp[s] + 64L * w < (1L << 31), "assembly plan too large");
+      hoffp[s + 1] = hoffp[s] + 64 * w;
+    }
+    const size_t tot = hoffp[ns];
+    std::vector<int> hie(tot, -1);
+    std::vector<uint32_t> hio(tot, 0u);
+    std::vector<double4> hs4(tot, make_double4(0, 0, 0, 0));
+    std::vector<double> hw2(nnz_sell, 0.0), hgw(nnz_sell, 0.0);
+    for (int r = 0; r < n; ++r) {
+      const size_t b0 = hoffp[r >> 6] + (r & 63);
+      for (int i = icnt[r]; i < icnt[r + 1]; ++i) {
+        const int e = hinc[i] >> 2, k = hinc[i] & 3;
+        const size_t q = b0 + 64 * (size_t)(i - icnt[r]);
+        hie[q] = e;
+        hio[q] = hoff[i];
+        const double *se = &hS[(size_t)e * 10];
+        hs4[q] = make_double4(se[up(k, 0)], se[up(k, 1)], se[up(k, 2)], se[up(k, 3)]);
+        for (int l = 0; l < 4; ++l) {
+          const int j = (hoff[i] >> (8 * l)) & 0xff;
+          hw2[spos(r, j)] += hvol[e] * 0.25 * 0.25;
+          hgw[spos(r, j)] += hgz[(size_t)e * 4 + k] * 0.25;
+        }
+      }
+    }
+    plan_off.upload(hoffp, st);
+    plan_ie.upload(hie, st);
+    plan_io.upload(hio, st);
+    plan_s4.upload(hs4, st);
+    plan_w2.upload(hw2, st);
+    plan_gw.upload(hgw, st);
     CG_HIP(hipStreamSynchronize(st));
   }
 
@@ -537,6 +633,7 @@ struct Flow {
     part.alloc((size_t)kSlots * kMaxRedBlocks);
     ipart.alloc(kMaxRedBlocks);
     scal.alloc(1);
+    dbeta.alloc(1);
     flags.alloc(2);
     res.alloc(16);
     h_res.assign(16, 0.0);
@@ -546,6 +643,9 @@ struct Flow {
 
   // ------------------------------------------------------- kernels glue --
   size_t col_smem(const ColDesc &C) const { return sizeof(double) * 2 * (size_t)C.len * kChains; }
+  size_t fac_smem(const ColDesc &C) const {
+    return (sizeof(double) * 2 + sizeof(int) * 3) * (size_t)C.len * kChains;
+  }
 
   template <typename FT>
   void precond_apply_t(const FT *F, const double *tin, double *out, const int *done) {
@@ -581,12 +681,12 @@ struct Flow {
     const Sell sl = sell();
     if (ordering == CATHY_GPU_ORDER_COLUMN3) {
       for (const ColDesc &C : sched.cols)
-        hipLaunchKernelGGL(k_ic_factor_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C,
-                           sl, stpos.p, A.p, isdir.p, F, dinv.p, flags.p + 1);
+        hipLaunchKernelGGL(k_ic_factor_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), fac_smem(C), st, C,
+                           sl, pairs(), stpos.p, A.p, isdir.p, F, dinv.p, flags.p + 1);
     } else {
       for (const ListLevel &L : sched.lv)
         hipLaunchKernelGGL(k_ic_factor_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p,
-                           sl, stpos.p, A.p, isdir.p, F, dinv.p, flags.p + 1);
+                           sl, pairs(), stpos.p, A.p, isdir.p, F, dinv.p, flags.p + 1);
     }
   }
   void factor() {
@@ -596,7 +696,7 @@ struct Flow {
 
   // SYMSLV on the current A / TNOTI: x = pdiff.  Returns NITER; err out.
   int pcg(double tol, int imax, double *err_out, int expect) {
-    const int G = grid_for(n, kBlock);  // one thread per row (<= kMaxRedBlocks groups)
+    const int G = std::min(grid_for(n, kBlock), 1024);
     factor();
     int *done = flags.p;
     precond_apply(tnoti.p, pdiff.p, nullptr);  // x0 = M^-1 b (symslv :4686)
@@ -604,7 +704,8 @@ struct Flow {
                        r.p, P(0));
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kBlock), 0, st, G, P(0), scal.p, done, tol, imax);
     precond_apply(r.p, p.p, nullptr);  // p = M^-1 r
-    const int Gr = red_grid(n);  // grid-stride vector kernels
+    const int Gs = grid_for(n, kBlock);  // SpMV: one thread per row
+    const int Gr = std::min(grid_for(n, kBlock), 256);   // vector-kernel groups
     int batch = std::max(1, std::min(expect, imax));
     int launched = 0;
     int h_done = 0;
@@ -612,15 +713,14 @@ struct Flow {
     while (true) {
       for (int b = 0; b < batch; ++b) {
         if (prof) CG_HIP(hipEventRecord(next_event(), st));
-        hipLaunchKernelGGL(k_spmv_dot2, dim3(G), dim3(kBlock), 0, st, n, sell(), A.p, p.p, r.p, Bv.p, P(1),
+        hipLaunchKernelGGL(k_spmv_dot2, dim3(Gs), dim3(kBlock), 0, st, n, sell(), A.p, p.p, r.p, Bv.p, P(1),
                            P(2), done);
         if (prof) CG_HIP(hipEventRecord(next_event(), st));
-        hipLaunchKernelGGL(k_pcg_alpha, dim3(Gr), dim3(kBlock), 0, st, n, G, P(1), P(2), Bv.p, p.p,
-                           isdir.p, r.p, pdiff.p, P(3), scal.p, done);
+        hipLaunchKernelGGL(k_pcg_alpha, dim3(Gr), dim3(kBlock), 0, st, n, Gs, P(1), P(2), Bv.p, p.p, isdir.p,
+                           r.p, pdiff.p, P(3), scal.p, done);
         precond_apply(r.p, z.p, done);
         hipLaunchKernelGGL(k_dot, dim3(Gr), dim3(kBlock), 0, st, n, Bv.p, z.p, P(4), done);
-        hipLaunchKernelGGL(k_pcg_beta, dim3(Gr), dim3(kBlock), 0, st, n, Gr, P(4), P(3), z.p, p.p, scal.p,
-                           done);
+        hipLaunchKernelGGL(k_pcg_beta, dim3(Gr), dim3(kBlock), 0, st, n, Gr, P(4), P(3), z.p, p.p, scal.p, done);
       }
       launched += batch;
       CG_HIP(hipMemcpyAsync(&h_done, done, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -657,9 +757,18 @@ struct Flow {
     a.pnew = pnew.p; a.ptimep = ptimep.p; a.swnew = swnew.p; a.swtimep = swtimep.p; a.et2 = et2.p;
     a.isdir = isdir.p; a.bcflux = bcflux.p; a.tetaf = prm.tetaf; a.rdt = 1.0 / deltat; a.A = A.p;
     a.tnoti = tnoti.p; a.xt5 = xt5.p;
-    const int ga = grid_for(n, kAsmBlock);
-    if (lump) hipLaunchKernelGGL(k_assemble<true>, dim3(ga), dim3(kAsmBlock), 0, st, a);
-    else hipLaunchKernelGGL(k_assemble<false>, dim3(ga), dim3(kAsmBlock), 0, st, a);
+    const int ga = grid_for(n, kBlock);
+    if (use_plan) {
+      const AsmPlan pl{plan_off.p, plan_ie.p, plan_io.p, plan_s4.p, plan_w2.p, plan_gw.p};
+      if (maxrow <= 16) hipLaunchKernelGGL((k_assemble_plan<16>), dim3(ga), dim3(kBlock), 0, st, a, pl, ckrw.p, et1.p);
+      else hipLaunchKernelGGL((k_assemble_plan<32>), dim3(ga), dim3(kBlock), 0, st, a, pl, ckrw.p, et1.p);
+    } else if (maxrow <= 16) {
+      if (lump) hipLaunchKernelGGL((k_assemble<16, true>), dim3(ga), dim3(kBlock), 0, st, a);
+      else hipLaunchKernelGGL((k_assemble<16, false>), dim3(ga), dim3(kBlock), 0, st, a);
+    } else {
+      if (lump) hipLaunchKernelGGL((k_assemble<32, true>), dim3(ga), dim3(kBlock), 0, st, a);
+      else hipLaunchKernelGGL((k_assemble<32, false>), dim3(ga), dim3(kBlock), 0, st, a);
+    }
     CG_HIP(hipEventRecord(ev0, st));
     const int expect = last_pcg > 0 ? last_pcg : 8;
     int niter = pcg(prm.tolcg, prm.itmxcg, &err, expect);

@@ -17,7 +17,7 @@
 namespace cg {
 
 constexpr int kAsmBlock = 128;  // rows per assembly work-group (LDS row buffers)
-constexpr int kChains = 64;     // chains per work-group in the column kernels
+constexpr int kChains = 16;     // chains per work-group in the column kernels
 
 struct Sell {
   const int *so;    // slice offsets
@@ -70,20 +70,44 @@ __global__ void k_elem_avg(int nt, const int4 *enode, const double *ckrw, const 
 __global__ void k_bc_rows(int n, const int *row_surf, const double *qnode, const int *ifatm,
                           const double *atmact, const double *atmold, double tetaf, uint8_t *isdir,
                           double *bcflux);
-template <bool LUMPED>
+template <int MR, bool LUMPED>
 __global__ void k_assemble(AsmArgs a);
 
+// Precomputed lumped-mass assembly plan.  Per row, one record per incident
+// element (ascending element id), lane-interleaved per slice: record m of
+// row r at off[r>>6] + 64 m + (r&63) holds the element id (-1 = padding),
+// the row slots of its 4 nodes (4 bytes) and the element-matrix row
+// S_e(k, 0..3) as a double4.  w2 / gw (SELL layout) give the lumped
+// storage and gravity terms as weights of the nodal ET1 / kr of the row's
+// neighbours.
+struct AsmPlan {
+  const int *off;
+  const int *ie;
+  const uint32_t *io;
+  const double4 *s4;
+  const double *w2, *gw;
+};
+template <int MR>
+__global__ void k_assemble_plan(AsmArgs a, AsmPlan P, const double *ckrw_node, const double *et1_node);
+
+// Static IC(0) update pattern: for the lower slot q of row r (column c),
+// pp[q]..pp[q+1] index pairs (slot of row r, slot of row c) with equal
+// columns below c -- the terms of sum_m L_rm L_cm.
+struct PairList {
+  const int *pp;    // per SELL slot
+  const int2 *pr;   // (slot in row r, slot in row c)
+};
 template <typename FT>
-__global__ void k_ic_factor_col3(ColDesc C, Sell sl, const int *tpos, const double *A, const uint8_t *isdir,
-                                 FT *F, double *dinv, int *nfall);
+__global__ void k_ic_factor_col3(ColDesc C, Sell sl, PairList pl, const int *tpos, const double *A,
+                                 const uint8_t *isdir, FT *F, double *dinv, int *nfall);
 template <typename FT>
 __global__ void k_fwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, const double *tin, double *y,
                            const int *done);
 template <typename FT>
 __global__ void k_bwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, double *x, const int *done);
 template <typename FT>
-__global__ void k_ic_factor_list(ListLevel L, const int *rows, Sell sl, const int *tpos, const double *A,
-                                 const uint8_t *isdir, FT *F, double *dinv, int *nfall);
+__global__ void k_ic_factor_list(ListLevel L, const int *rows, Sell sl, PairList pl, const int *tpos,
+                                 const double *A, const uint8_t *isdir, FT *F, double *dinv, int *nfall);
 template <typename FT>
 __global__ void k_fwd_list(ListLevel L, const int *rows, Sell sl, const FT *F, const double *dinv,
                            const double *tin, double *y, const int *done);

@@ -84,25 +84,25 @@ __device__ __forceinline__ int upair(int k, int l) {
 // Row-gather assembly + RHS + CFMATP + RHSGRV + BCPIC right-hand side.
 // asspic.f:26-51 scattered element matrices; here each thread owns one row
 // and gathers the contributions of its incident elements (ascending
-// element id, the reference's accumulation order) into an LDS row buffer,
-// so the assembly is deterministic and atomic-free.
+// element id, the reference's accumulation order) into registers (MR =
+// compile-time row width, entries selected by slot), so the assembly is
+// deterministic and atomic-free.
 //   K      = sum_e kr_e * S_e                       (COEF1 before CFMATP)
 //   TNOTI  = -K PTNEW - M2/dt (PNEW-PTIMEP) - M4/dt (SWNEW-SWTIMEP) - grav
 //   A      = TETAF*K + (M2 + diag(M4_rr ET2_r))/dt  (cfmatp.f)
 // Dirichlet rows keep their true coefficients (BKPIC needs them) and get
 // TNOTI = 0; the preconditioner masks them.
-template <bool LUMPED>
-__global__ __launch_bounds__(kAsmBlock) void k_assemble(AsmArgs a) {
-  __shared__ double Ks[kMaxRow * kAsmBlock];
-  __shared__ double Ms[LUMPED ? 1 : kMaxRow * kAsmBlock];
-  int r = blockIdx.x * kAsmBlock + threadIdx.x;
+template <int MR, bool LUMPED>
+__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= a.n) return;
   const int len = a.sl.len[r];
-  double *K = Ks + threadIdx.x;
-  double *M = LUMPED ? nullptr : Ms + threadIdx.x;
-  for (int j = 0; j < len; ++j) {
-    K[j * kAsmBlock] = 0.0;
-    if (!LUMPED) M[j * kAsmBlock] = 0.0;
+  double K[MR];
+  double M[LUMPED ? 1 : MR];
+#pragma unroll
+  for (int j = 0; j < MR; ++j) {
+    K[j] = 0.0;
+    if (!LUMPED) M[j] = 0.0;
   }
   double m2d = 0.0, grav = 0.0;
   const int i1 = a.inc_ptr[r + 1];
@@ -112,38 +112,41 @@ __global__ __launch_bounds__(kAsmBlock) void k_assemble(AsmArgs a) {
     const uint32_t off = a.incoff[i];
     const double kr = a.ckrwe[e];
     const double *se = a.S + (size_t)e * 10;
+    const double ve1 = a.volu[e] * a.et1e[e];
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
       const int o = (off >> (8 * l)) & 0xff;
-      K[o * kAsmBlock] += kr * se[upair(k, l)];
-    }
-    const double ve1 = a.volu[e] * a.et1e[e];
-    if (LUMPED) {
-      m2d = m2d + ve1 * 0.25;
-    } else {
+      const double v = kr * se[upair(k, l)];
+      const double w = LUMPED ? 0.0 : ve1 * a.lmass[k * 4 + l];
 #pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        const int o = (off >> (8 * l)) & 0xff;
-        M[o * kAsmBlock] += ve1 * a.lmass[k * 4 + l];
+      for (int j = 0; j < MR; ++j) {
+        K[j] += (j == o) ? v : 0.0;
+        if (!LUMPED) M[j] += (j == o) ? w : 0.0;
       }
     }
+    if (LUMPED) m2d = m2d + ve1 * 0.25;
     grav = grav + kr * a.gz[(size_t)e * 4 + k];
   }
   const int dg = a.sl.diag[r];
+  const int base = a.sl.so[r >> 6] + (r & 63);
   const double rdt = a.rdt;
   double t = 0.0;
-  for (int j = 0; j < len; ++j) t = t - K[j * kAsmBlock] * a.ptnew[a.sl.col[sp(a.sl.so, r, j)]];
+#pragma unroll
+  for (int j = 0; j < MR; ++j)
+    if (j < len) t = t - K[j] * a.ptnew[a.sl.col[base + 64 * j]];
   double m4r;
   if (LUMPED) {
     m4r = a.m4d[r];
     t = t - (m2d * rdt) * (a.pnew[r] - a.ptimep[r]) - (m4r * rdt) * (a.swnew[r] - a.swtimep[r]);
   } else {
-    m4r = a.m4full[sp(a.sl.so, r, dg)];
-    for (int j = 0; j < len; ++j) {
-      const int q = sp(a.sl.so, r, j);
-      const int c = a.sl.col[q];
-      t = t - (M[j * kAsmBlock] * rdt) * (a.pnew[c] - a.ptimep[c]) -
-          (a.m4full[q] * rdt) * (a.swnew[c] - a.swtimep[c]);
+    m4r = a.m4full[base + 64 * dg];
+#pragma unroll
+    for (int j = 0; j < MR; ++j) {
+      if (j < len) {
+        const int q = base + 64 * j;
+        const int c = a.sl.col[q];
+        t = t - (M[j] * rdt) * (a.pnew[c] - a.ptimep[c]) - (a.m4full[q] * rdt) * (a.swnew[c] - a.swtimep[c]);
+      }
     }
   }
   t = t - grav;
@@ -151,47 +154,101 @@ __global__ __launch_bounds__(kAsmBlock) void k_assemble(AsmArgs a) {
   a.tnoti[r] = a.isdir[r] ? 0.0 : t + a.bcflux[r];
   const double tetaf = a.tetaf;
   const double m5 = m4r * a.et2[r];
-  for (int j = 0; j < len; ++j) {
-    double v = tetaf * K[j * kAsmBlock];
-    if (LUMPED) {
-      if (j == dg) v = v + m2d * rdt + m5 * rdt;
-    } else {
-      v = v + M[j * kAsmBlock] * rdt;
-      if (j == dg) v = v + m5 * rdt;
+#pragma unroll
+  for (int j = 0; j < MR; ++j) {
+    if (j < len) {
+      double v = tetaf * K[j];
+      if (LUMPED) {
+        if (j == dg) v = v + m2d * rdt + m5 * rdt;
+      } else {
+        v = v + M[j] * rdt;
+        if (j == dg) v = v + m5 * rdt;
+      }
+      a.A[base + 64 * j] = v;
     }
-    a.A[sp(a.sl.so, r, j)] = v;
   }
 }
-template __global__ void k_assemble<true>(AsmArgs);
-template __global__ void k_assemble<false>(AsmArgs);
+template __global__ void k_assemble<16, true>(AsmArgs);
+template __global__ void k_assemble<16, false>(AsmArgs);
+template __global__ void k_assemble<32, true>(AsmArgs);
+template __global__ void k_assemble<32, false>(AsmArgs);
+
+// Lumped assembly from the precomputed plan (same quantities as
+// k_assemble): plan records, A's slots and the weights are read coalesced;
+// the gathers are kr_e (once per incident element) and nodal values of
+// the row's neighbours.  Entries accumulate in registers by slot.
+template <int MR>
+__global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, const double *__restrict__ ckrw_node,
+                                                          const double *__restrict__ et1_node) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= a.n) return;
+  const int s = r >> 6, lane = r & 63;
+  const int base = a.sl.so[s] + lane;
+  const int w = (a.sl.so[s + 1] - a.sl.so[s]) >> 6;
+  double m2d = 0.0, grav = 0.0;
+  for (int j = 0; j < w; ++j) {
+    const int q = base + 64 * j;
+    const int c = a.sl.col[q];
+    m2d += P.w2[q] * et1_node[c];
+    grav += P.gw[q] * ckrw_node[c];
+  }
+  double K[MR];
+#pragma unroll
+  for (int j = 0; j < MR; ++j) K[j] = 0.0;
+  const int p0 = P.off[s] + lane;
+  const int cw = (P.off[s + 1] - P.off[s]) >> 6;
+  for (int m = 0; m < cw; ++m) {
+    const int q = p0 + 64 * m;
+    const int e = P.ie[q];
+    if (e < 0) continue;
+    const uint32_t off = P.io[q];
+    const double kr = a.ckrwe[e];
+    const double4 sv = P.s4[q];
+    const double v[4] = {kr * sv.x, kr * sv.y, kr * sv.z, kr * sv.w};
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int o = (off >> (8 * l)) & 0xff;
+#pragma unroll
+      for (int j = 0; j < MR; ++j) K[j] += (j == o) ? v[l] : 0.0;
+    }
+  }
+  const int len = a.sl.len[r], dg = a.sl.diag[r];
+  const double rdt = a.rdt, tetaf = a.tetaf;
+  const double m4r = a.m4d[r];
+  const double m5 = m4r * a.et2[r];
+  double t = 0.0;
+#pragma unroll
+  for (int j = 0; j < MR; ++j) {
+    if (j < len) {
+      const int q = base + 64 * j;
+      t = t - K[j] * a.ptnew[a.sl.col[q]];
+      a.A[q] = tetaf * K[j] + (j == dg ? m2d * rdt + m5 * rdt : 0.0);
+    }
+  }
+  t = t - (m2d * rdt) * (a.pnew[r] - a.ptimep[r]) - (m4r * rdt) * (a.swnew[r] - a.swtimep[r]);
+  t = t - grav;
+  a.xt5[r] = t;
+  a.tnoti[r] = a.isdir[r] ? 0.0 : t + a.bcflux[r];
+}
+template __global__ void k_assemble_plan<16>(AsmArgs, AsmPlan, const double *, const double *);
+template __global__ void k_assemble_plan<32>(AsmArgs, AsmPlan, const double *, const double *);
 
 // ------------------------------------------------------------ IC(0) --
 // incldp.f restated as an up-looking IC(0) on the full pattern: for each
 // row r in elimination order, L_rj = (a_rj - sum_{m<j} L_rm L_jm) / L_jj for
-// its lower entries j (ascending), L_rr = sqrt(a_rr - sum L_rj^2).
-// Dirichlet rows/columns are removed (identity, zero inverse diagonal) --
-// what the 1e-9*RMAX diagonal penalty of bcpic.f does in exact arithmetic.
-// The factor's transpose is mirrored into the upper slots so both sweeps
-// read rows.
-
-// Sparse dot of the lower parts of rows r and c over columns < limit
-// (both rows sorted): sum L_rm L_cm.  ra/rc = number of slots scanned.
+// its lower entries j (ascending), L_rr = sqrt(a_rr - sum L_rj^2).  The
+// terms of sum_{m<j} come from the static pair list.  Dirichlet rows and
+// columns are removed (identity, zero inverse diagonal) -- what the
+// 1e-9*RMAX diagonal penalty of bcpic.f does in exact arithmetic.  The
+// factor's transpose is mirrored into the upper slots so both sweeps read
+// rows.
 template <typename FT>
-__device__ __forceinline__ double row_dot(const Sell &sl, const FT *F, int r, int ra, int c, int rc) {
+__device__ __forceinline__ double pair_dot(const PairList &pl, const FT *__restrict__ F, int q) {
   double v = 0.0;
-  int a = 0, b = 0;
-  while (a < ra && b < rc) {
-    const int pa = sp(sl.so, r, a), pb = sp(sl.so, c, b);
-    const int ca = sl.col[pa], cb = sl.col[pb];
-    if (ca < cb) {
-      ++a;
-    } else if (ca > cb) {
-      ++b;
-    } else {
-      v += (double)F[pa] * (double)F[pb];
-      ++a;
-      ++b;
-    }
+  const int m1 = pl.pp[q + 1];
+  for (int m = pl.pp[q]; m < m1; ++m) {
+    const int2 pr = pl.pr[m];
+    v += (double)F[pr.x] * (double)F[pr.y];
   }
   return v;
 }
@@ -202,38 +259,48 @@ __device__ __forceinline__ double row_dot(const Sell &sl, const FT *F, int r, in
 // (plane k-1), whose entries common with the row are all of earlier
 // colours.  So: pass 1a computes every row's earlier-colour L entries in
 // parallel, pass 1b the predecessor's numerator, and pass 2 runs the
-// 2-term recurrence d_k = sqrt(s_k - (q_k / d_{k-1})^2) down each chain.
+// 2-term recurrence d_k = sqrt(s_k - (q_k / d_{k-1})^2) down each chain
+// from LDS only.
 template <typename FT>
-__global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, const int *__restrict__ tpos,
+__global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, PairList pl,
+                                                        const int *__restrict__ tpos,
                                                         const double *__restrict__ A,
                                                         const uint8_t *__restrict__ isdir, FT *__restrict__ F,
                                                         double *__restrict__ dinv, int *nfall) {
   extern __shared__ double smem[];
-  double *Sv = smem;                      // a_rr - sum earlier L^2
-  double *Qv = smem + C.len * kChains;    // a_r,pred - earlier dot
-  const int t0 = blockIdx.x * kChains;
   const int tot = C.len * kChains;
+  double *Sv = smem;                 // a_rr - sum earlier L^2
+  double *Qv = smem + tot;           // a_r,pred - earlier dot
+  int *Pd = (int *)(smem + 2 * tot); // slot of the diagonal (-1: Dirichlet)
+  int *Pl = Pd + tot;                // slot of the predecessor entry
+  int *Pt = Pl + tot;                // its transpose slot
+  const int t0 = blockIdx.x * kChains;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / kChains, t = t0 + (i % kChains);
     if (t >= C.nc) continue;
     const int r = C.base + t + k * C.nc;
     const int jd = sl.diag[r];
     const int jp = k > 0 ? jd - 1 : jd;
+    const int base = sl.so[r >> 6] + (r & 63);
+    Pl[i] = k > 0 ? base + 64 * jp : -1;
+    Pt[i] = k > 0 ? tpos[base + 64 * jp] : -1;
     if (isdir[r]) {
       for (int j = 0; j < jp; ++j) {
-        const int pj = sp(sl.so, r, j);
+        const int pj = base + 64 * j;
         F[pj] = FT(0);
         F[tpos[pj]] = FT(0);
       }
+      Pd[i] = -(base + 64 * jd) - 1;
       Sv[i] = 1.0;
       continue;
     }
-    double s = A[sp(sl.so, r, jd)];
+    Pd[i] = base + 64 * jd;
+    double s = A[base + 64 * jd];
     for (int j = 0; j < jp; ++j) {
-      const int pj = sp(sl.so, r, j);
+      const int pj = base + 64 * j;
       const int c = sl.col[pj];
       double L = 0.0;
-      if (!isdir[c]) L = (A[pj] - row_dot(sl, F, r, j, c, sl.diag[c])) * dinv[c];
+      if (!isdir[c]) L = (A[pj] - pair_dot(pl, F, pj)) * dinv[c];
       F[pj] = FT(L);
       F[tpos[pj]] = FT(L);
       s -= L * L;
@@ -244,12 +311,9 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, cons
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / kChains, t = t0 + (i % kChains);
     if (t >= C.nc || k == 0) continue;
-    const int r = C.base + t + k * C.nc;
-    if (isdir[r]) continue;
-    const int jp = sl.diag[r] - 1;
-    const int pr = r - C.nc;
-    const int jpp = k > 1 ? sl.diag[pr] - 1 : sl.diag[pr];
-    Qv[i] = A[sp(sl.so, r, jp)] - row_dot(sl, F, r, jp, pr, jpp);
+    if (Pd[i] < 0) continue;
+    const int q = Pl[i];
+    Qv[i] = A[q] - pair_dot(pl, F, q);
   }
   __syncthreads();
   if (threadIdx.x < kChains) {
@@ -258,16 +322,15 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, cons
       double dprev = 1.0;
       bool pdir = true;
       for (int k = 0; k < C.len; ++k) {
-        const int r = C.base + t + k * C.nc;
         const int i = k * kChains + threadIdx.x;
-        const int jd = sl.diag[r];
-        if (isdir[r]) {
+        const int r = C.base + t + k * C.nc;
+        const int pd = Pd[i];
+        if (pd < 0) {
           if (k > 0) {
-            const int pj = sp(sl.so, r, jd - 1);
-            F[pj] = FT(0);
-            F[tpos[pj]] = FT(0);
+            F[Pl[i]] = FT(0);
+            F[Pt[i]] = FT(0);
           }
-          F[sp(sl.so, r, jd)] = FT(1);
+          F[-pd - 1] = FT(1);
           dinv[r] = 0.0;
           pdir = true;
           continue;
@@ -275,18 +338,17 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, cons
         double l = 0.0;
         if (k > 0) {
           if (!pdir) l = Qv[i] / dprev;
-          const int pj = sp(sl.so, r, jd - 1);
-          F[pj] = FT(l);
-          F[tpos[pj]] = FT(l);
+          F[Pl[i]] = FT(l);
+          F[Pt[i]] = FT(l);
         }
         double v = Sv[i] - l * l;
         if (!(v > 0.0)) {  // breakdown: keep the row's own diagonal
-          const double a = fabs(A[sp(sl.so, r, jd)]);
+          const double a = fabs(A[pd]);
           v = a > 0.0 ? a : 1.0;
           atomicAdd(nfall, 1);
         }
         const double d = sqrt(v);
-        F[sp(sl.so, r, jd)] = FT(d);
+        F[pd] = FT(d);
         dinv[r] = 1.0 / d;
         dprev = d;
         pdir = false;
@@ -296,30 +358,32 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, cons
 }
 
 // Forward sweep L y = t (proddp, solscal-extended.f:4175-4185) on one
-// colour: pass 1 gathers the earlier-colour terms of every row, pass 2
-// runs y_k = (t_k - L_k,k-1 y_k-1) / L_kk down each chain.
+// colour: pass 1 gathers the earlier-colour terms of every row and scales
+// by 1/L_kk, pass 2 runs y_k = t'_k - l'_k y_k-1 down each chain in LDS.
 template <typename FT>
 __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
                                                   const double *__restrict__ dinv, const double *__restrict__ tin,
                                                   double *__restrict__ y, const int *__restrict__ done) {
   if (done && *done) return;
   extern __shared__ double smem[];
-  double *T = smem, *Lp = smem + C.len * kChains;
-  const int t0 = blockIdx.x * kChains;
   const int tot = C.len * kChains;
+  double *T = smem, *Lp = smem + tot;
+  const int t0 = blockIdx.x * kChains;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / kChains, t = t0 + (i % kChains);
     if (t >= C.nc) continue;
     const int r = C.base + t + k * C.nc;
     const int jd = sl.diag[r];
     const int jp = k > 0 ? jd - 1 : jd;
+    const int base = sl.so[r >> 6] + (r & 63);
     double s = tin[r];
     for (int j = 0; j < jp; ++j) {
-      const int pj = sp(sl.so, r, j);
+      const int pj = base + 64 * j;
       s -= (double)F[pj] * y[sl.col[pj]];
     }
-    T[i] = s;
-    Lp[i] = k > 0 ? (double)F[sp(sl.so, r, jp)] : 0.0;
+    const double di = dinv[r];
+    T[i] = s * di;
+    Lp[i] = k > 0 ? (double)F[base + 64 * jp] * di : 0.0;
   }
   __syncthreads();
   if (threadIdx.x < kChains) {
@@ -327,10 +391,9 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
     if (t < C.nc) {
       double yp = 0.0;
       for (int k = 0; k < C.len; ++k) {
-        const int r = C.base + t + k * C.nc;
         const int i = k * kChains + threadIdx.x;
-        const double v = (T[i] - Lp[i] * yp) * dinv[r];
-        y[r] = v;
+        const double v = T[i] - Lp[i] * yp;
+        y[C.base + t + k * C.nc] = v;
         yp = v;
       }
     }
@@ -345,9 +408,9 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
                                                   const int *__restrict__ done) {
   if (done && *done) return;
   extern __shared__ double smem[];
-  double *T = smem, *U = smem + C.len * kChains;
-  const int t0 = blockIdx.x * kChains;
   const int tot = C.len * kChains;
+  double *T = smem, *U = smem + tot;
+  const int t0 = blockIdx.x * kChains;
   const int kl = C.len - 1;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / kChains, t = t0 + (i % kChains);
@@ -356,13 +419,15 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
     const int jd = sl.diag[r];
     const int js = k < kl ? jd + 1 : jd;
     const int len = sl.len[r];
+    const int base = sl.so[r >> 6] + (r & 63);
     double s = x[r];
     for (int j = js + 1; j < len; ++j) {
-      const int pj = sp(sl.so, r, j);
+      const int pj = base + 64 * j;
       s -= (double)F[pj] * x[sl.col[pj]];
     }
-    T[i] = s;
-    U[i] = k < kl ? (double)F[sp(sl.so, r, jd + 1)] : 0.0;
+    const double di = dinv[r];
+    T[i] = s * di;
+    U[i] = k < kl ? (double)F[base + 64 * (jd + 1)] * di : 0.0;
   }
   __syncthreads();
   if (threadIdx.x < kChains) {
@@ -370,10 +435,9 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
     if (t < C.nc) {
       double xn = 0.0;
       for (int k = kl; k >= 0; --k) {
-        const int r = C.base + t + k * C.nc;
         const int i = k * kChains + threadIdx.x;
-        const double v = (T[i] - U[i] * xn) * dinv[r];
-        x[r] = v;
+        const double v = T[i] - U[i] * xn;
+        x[C.base + t + k * C.nc] = v;
         xn = v;
       }
     }
@@ -383,7 +447,7 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
 // Natural order: one row per thread, levels of the reference's
 // elimination DAG launched in sequence (exact INCLDP/PRODDP order).
 template <typename FT>
-__global__ void k_ic_factor_list(ListLevel L, const int *__restrict__ rows, Sell sl,
+__global__ void k_ic_factor_list(ListLevel L, const int *__restrict__ rows, Sell sl, PairList pl,
                                  const int *__restrict__ tpos, const double *__restrict__ A,
                                  const uint8_t *__restrict__ isdir, FT *__restrict__ F,
                                  double *__restrict__ dinv, int *nfall) {
@@ -391,33 +455,34 @@ __global__ void k_ic_factor_list(ListLevel L, const int *__restrict__ rows, Sell
   if (t >= L.cnt) return;
   const int r = rows[L.off + t];
   const int jd = sl.diag[r];
+  const int base = sl.so[r >> 6] + (r & 63);
   if (isdir[r]) {
     for (int j = 0; j < jd; ++j) {
-      const int pj = sp(sl.so, r, j);
+      const int pj = base + 64 * j;
       F[pj] = FT(0);
       F[tpos[pj]] = FT(0);
     }
-    F[sp(sl.so, r, jd)] = FT(1);
+    F[base + 64 * jd] = FT(1);
     dinv[r] = 0.0;
     return;
   }
-  double s = A[sp(sl.so, r, jd)];
+  double s = A[base + 64 * jd];
   for (int j = 0; j < jd; ++j) {
-    const int pj = sp(sl.so, r, j);
+    const int pj = base + 64 * j;
     const int c = sl.col[pj];
     double Lv = 0.0;
-    if (!isdir[c]) Lv = (A[pj] - row_dot(sl, F, r, j, c, sl.diag[c])) * dinv[c];
+    if (!isdir[c]) Lv = (A[pj] - pair_dot(pl, F, pj)) * dinv[c];
     F[pj] = FT(Lv);
     F[tpos[pj]] = FT(Lv);
     s -= Lv * Lv;
   }
   if (!(s > 0.0)) {
-    const double a = fabs(A[sp(sl.so, r, jd)]);
+    const double a = fabs(A[base + 64 * jd]);
     s = a > 0.0 ? a : 1.0;
     atomicAdd(nfall, 1);
   }
   const double d = sqrt(s);
-  F[sp(sl.so, r, jd)] = FT(d);
+  F[base + 64 * jd] = FT(d);
   dinv[r] = 1.0 / d;
 }
 
@@ -430,9 +495,10 @@ __global__ void k_fwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, c
   if (t >= L.cnt) return;
   const int r = rows[L.off + t];
   const int jd = sl.diag[r];
+  const int base = sl.so[r >> 6] + (r & 63);
   double s = tin[r];
   for (int j = 0; j < jd; ++j) {
-    const int pj = sp(sl.so, r, j);
+    const int pj = base + 64 * j;
     s -= (double)F[pj] * y[sl.col[pj]];
   }
   y[r] = s * dinv[r];
@@ -447,22 +513,23 @@ __global__ void k_bwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, c
   if (t >= L.cnt) return;
   const int r = rows[L.off + t];
   const int jd = sl.diag[r], len = sl.len[r];
+  const int base = sl.so[r >> 6] + (r & 63);
   double s = x[r];
   for (int j = jd + 1; j < len; ++j) {
-    const int pj = sp(sl.so, r, j);
+    const int pj = base + 64 * j;
     s -= (double)F[pj] * x[sl.col[pj]];
   }
   x[r] = s * dinv[r];
 }
 
 #define CG_INST_FT(FT)                                                                                      \
-  template __global__ void k_ic_factor_col3<FT>(ColDesc, Sell, const int *, const double *, const uint8_t *, \
-                                                FT *, double *, int *);                                     \
+  template __global__ void k_ic_factor_col3<FT>(ColDesc, Sell, PairList, const int *, const double *,      \
+                                                const uint8_t *, FT *, double *, int *);                    \
   template __global__ void k_fwd_col3<FT>(ColDesc, Sell, const FT *, const double *, const double *,        \
                                           double *, const int *);                                          \
   template __global__ void k_bwd_col3<FT>(ColDesc, Sell, const FT *, const double *, double *, const int *); \
-  template __global__ void k_ic_factor_list<FT>(ListLevel, const int *, Sell, const int *, const double *,  \
-                                                const uint8_t *, FT *, double *, int *);                    \
+  template __global__ void k_ic_factor_list<FT>(ListLevel, const int *, Sell, PairList, const int *,        \
+                                                const double *, const uint8_t *, FT *, double *, int *);    \
   template __global__ void k_fwd_list<FT>(ListLevel, const int *, Sell, const FT *, const double *,         \
                                           const double *, double *, const int *);                          \
   template __global__ void k_bwd_list<FT>(ListLevel, const int *, Sell, const FT *, const double *, double *, \
@@ -478,6 +545,8 @@ CG_INST_FT(double)
 // Dirichlet rows carry p = z = 0, so they drop out of every product.
 // One thread per row; a wavefront covers one SELL slice, so the loop count
 // (the slice width) is uniform and every load of A / col is coalesced.
+// Scalars are summed from per-work-group partials in a fixed order by
+// every group of the consuming kernel (deterministic, no atomics).
 
 __device__ __forceinline__ double sell_row_dot(const Sell &sl, const double *__restrict__ A,
                                                const double *__restrict__ x, int r) {
@@ -495,17 +564,16 @@ __global__ void k_residual0(int n, Sell sl, const double *__restrict__ A, const 
                             double *__restrict__ r, double *__restrict__ part) {
   __shared__ double sh[kBlock / 64 + 1];
   double acc = 0.0;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double bi = b[i];
     r[i] = bi - sell_row_dot(sl, A, x, i);
-    if (!isdir[i]) acc = bi * bi;
+    if (!isdir[i]) acc += bi * bi;
   }
   double t = block_sum<kBlock>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// B = A p with partial (p,r) and (p,B).
+// B = A p with partial (p,r) and (p,B); one thread per row.
 __global__ void k_spmv_dot2(int n, Sell sl, const double *__restrict__ A, const double *__restrict__ p,
                             const double *__restrict__ r, double *__restrict__ B,
                             double *__restrict__ part_pr, double *__restrict__ part_pb,
@@ -529,12 +597,13 @@ __global__ void k_spmv_dot2(int n, Sell sl, const double *__restrict__ A, const 
   }
 }
 
-// alpha from the partials; r -= alpha B; x += alpha p; partial r_free^2.
+// alpha from the SpMV partials (every group sums them in the same fixed
+// order); r -= alpha B; x += alpha p; partial r_free^2.
 __global__ void k_pcg_alpha(int n, int nparts, const double *__restrict__ part_pr,
                             const double *__restrict__ part_pb, const double *__restrict__ B,
                             const double *__restrict__ p, const uint8_t *__restrict__ isdir,
-                            double *__restrict__ r, double *__restrict__ x,
-                            double *__restrict__ part_rr, PcgScal *sc, const int *__restrict__ done) {
+                            double *__restrict__ r, double *__restrict__ x, double *__restrict__ part_rr,
+                            PcgScal *sc, const int *__restrict__ done) {
   if (*done) return;
   __shared__ double sh[kBlock / 64 + 2];
   const double pr = reduce_partials<kBlock>(part_pr, nparts, sh);
@@ -575,8 +644,7 @@ __global__ void k_pcg_beta(int n, int nparts, const double *__restrict__ part_bz
   __shared__ double sh[kBlock / 64 + 2];
   const double c = reduce_partials<kBlock>(part_bz, nparts, sh);
   const double rr = reduce_partials<kBlock>(part_rr, nparts, sh);
-  const double pb = sc->pb;
-  const double beta = -c / pb;
+  const double beta = -c / sc->pb;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     p[i] = z[i] + beta * p[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
