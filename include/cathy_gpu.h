@@ -46,13 +46,20 @@ const char *cathy_gpu_last_error(void);
 int cathy_gpu_device_info(char *name, int *name_len, int *cu_count);
 
 /* ------------------------------------------------------------------ *
- * SYMSLV (solscal-extended.f:4669): solve COEF1 * PNEW = TNOTI by IC(0)
- * preconditioned CG exactly as INCLDP + PRODDP + GRADDP + RESSYM do
- * (natural-order factor, x0 = M^-1 b, stop on ||r_free|| / ||b_free|| <=
- * TOLCG or NITER >= ITMXCG).  COEF1/TOPOL/JA are the reference's upper
- * CSR (diagonal first), NODDIR(1:NUMDIR) the Dirichlet rows whose
- * diagonal carries the 1e-9*RMAX penalty.  FP64 throughout.
- * Outputs: PNEW, NITER, ERR (GRADDP's last residual), RES (RESSYM).
+ * SYMSLV (solscal-extended.f:4669-4699): solve COEF1 * PNEW = TNOTI by
+ * IC(0)-preconditioned CG as INCLDP (:2142-2193) + PRODDP (:4169-4204) +
+ * GRADDP (:1260-1386) + RESSYM (:4389-4438) do: natural-order FP64
+ * factor with the breakdown fallback, x0 = M^-1 b, stop when
+ * ||r_free|| / ||b_free|| <= TOLCG or NITER >= ITMXCG.
+ * TOPOL(N+1) / JA(NTERM) / COEF1(NTERM): the reference's 1-based upper
+ * CSR with the diagonal first in each row (strpic.f).  NODDIR(1:NUMDIR):
+ * Dirichlet rows.  The reference's 1e-9*RMAX penalty on their diagonal
+ * decouples them to ~1e-90 relative; here they are masked out of the
+ * factor and the products, and PNEW = 0 on them.  FP64 throughout.
+ * Outputs: PNEW, NITER, ERR (GRADDP's last relative residual), RES
+ * (RESSYM's RMS residual: over ||b_free|| unless that is <= RMIN).
+ * The pattern set-up is kept per calling thread and reused while N,
+ * NTERM, TOPOL and JA are unchanged (the Picard loop's case).
  * Returns CATHY_GPU_LSFAIL when NITER >= ITMXCG.
  * ------------------------------------------------------------------ */
 int cathy_gpu_symslv(const int *n, const int *nterm, const int *numdir,
@@ -60,6 +67,9 @@ int cathy_gpu_symslv(const int *n, const int *nterm, const int *numdir,
                      const double *rmin, const int *noddir, const int *ja,
                      const int *topol, double *pnew, const double *tnoti,
                      const double *coef1, double *err, double *res);
+
+/* Frees the calling thread's SYMSLV set-up (no reference counterpart). */
+int cathy_gpu_symslv_release(void);
 
 /* ------------------------------------------------------------------ *
  * FLOW3D family.  Set-up takes the mesh and soil arrays the host builds

@@ -204,6 +204,92 @@ struct Flow {
     CG_HIP(hipStreamSynchronize(st));
   }
 
+  // SYMSLV set-up: the pattern of a 1-based upper CSR with the diagonal
+  // first in each row (strpic.f layout), natural order, FP64 factor.
+  // csr_map[q] = COEF1 index of SELL slot q (-1 = padding).
+  DBuf<int> csr_map;
+  int nterm = 0;
+  void setup_csr(int n_, int nterm_, const int *topol, const int *ja) {
+    n = n_;
+    nterm = nterm_;
+    nt = 0;
+    nnod = 0;
+    ordering = CATHY_GPU_ORDER_NATURAL;
+    f32 = false;
+    CG_REQUIRE(n > 0 && nterm > 0, "empty system");
+    CG_REQUIRE((long)n <= (long)kMaxRedBlocks * kBlock, "N above the per-launch row limit");
+    CG_REQUIRE(topol[0] == 1 && topol[n] - 1 == nterm, "TOPOL(1) must be 1 and TOPOL(N+1)-1 = NTERM");
+    for (int r = 0; r < n; ++r) {
+      CG_REQUIRE(topol[r + 1] > topol[r], "TOPOL not increasing");
+      CG_REQUIRE(ja[topol[r] - 1] == r + 1, "JA: the diagonal must come first in each row");
+      for (int q = topol[r]; q < topol[r + 1] - 1; ++q)
+        CG_REQUIRE(ja[q] > r + 1 && ja[q] <= n, "JA: upper-triangle column out of range");
+    }
+    CG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    CG_HIP(hipEventCreate(&ev0));
+    CG_HIP(hipEventCreate(&ev1));
+    perm.resize(n);
+    iperm.resize(n);
+    for (int i = 0; i < n; ++i) perm[i] = iperm[i] = i;
+    build_pattern(nullptr, topol, ja);
+    std::vector<int> hmap(nnz_sell, -1);
+    auto find = [&](int r, int c) {  // COEF1 index of (r, c), r <= c
+      for (int q = topol[r] - 1; q < topol[r + 1] - 1; ++q)
+        if (ja[q] - 1 == c) return q;
+      return -1;
+    };
+    for (int r = 0; r < n; ++r)
+      for (int k = h_rowptr[r]; k < h_rowptr[r + 1]; ++k) {
+        const int c = h_col[k];
+        const int q = c >= r ? find(r, c) : find(c, r);
+        CG_REQUIRE(q >= 0, "pattern entry missing from the upper CSR");
+        hmap[h_so[r >> 6] + 64 * (k - h_rowptr[r]) + (r & 63)] = q;
+      }
+    csr_map.upload(hmap, st);
+    alloc_solver();
+    coef.alloc(nterm);
+    CG_HIP(hipStreamSynchronize(st));
+  }
+  DBuf<double> coef;
+  std::vector<uint8_t> h_isdir;
+
+  // SYMSLV (solscal-extended.f:4669-4699): INCLDP + PRODDP + GRADDP +
+  // RESSYM on COEF1 x = TNOTI.  Rows NODDIR(1:NUMDIR) carry the
+  // 1e-9*RMAX penalty diagonal in the reference, which decouples them to
+  // within ~1e-90 relative; here they are masked out of the factor and
+  // the products, and return x = 0.
+  int symslv(int numdir, const int *noddir, int itmxcg, double tolcg, double rmin, const double *tnoti_h,
+             const double *coef1, double *pnew_h, int *niter_out, double *err_out, double *res_out) {
+    h_isdir.assign(n, 0);
+    for (int k = 0; k < numdir; ++k) {
+      CG_REQUIRE(noddir[k] >= 1 && noddir[k] <= n, "NODDIR entry out of range");
+      h_isdir[noddir[k] - 1] = 1;
+    }
+    isdir.upload(h_isdir.data(), n, st);
+    coef.upload(coef1, nterm, st);
+    tnoti.upload(tnoti_h, n, st);
+    hipLaunchKernelGGL(k_gather_coef, dim3(grid_for(nnz_sell)), dim3(kBlock), 0, st, nnz_sell, csr_map.p, coef.p,
+                       A.p);
+    double err = 0.0;
+    const int niter = pcg(tolcg, itmxcg, &err, last_pcg > 0 ? last_pcg : 8);
+    last_pcg = niter;
+    // RESSYM: sqrt(sum_free (b - A x)^2 / XLUNG), or / N when XLUNG <= RMIN
+    const int G = std::min(grid_for(n, kBlock), 1024);
+    hipLaunchKernelGGL(k_residual0, dim3(G), dim3(kBlock), 0, st, n, sell(), A.p, pdiff.p, tnoti.p, isdir.p,
+                       r.p, P(0));
+    hipLaunchKernelGGL(k_sumsq_free, dim3(G), dim3(kBlock), 0, st, n, r.p, isdir.p, P(1));
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, st, G, P(0), res.p);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, st, G, P(1), res.p + 1);
+    double hr[2];
+    res.download(hr, 2, st);
+    pdiff.download(pnew_h, n, st);
+    CG_HIP(hipStreamSynchronize(st));
+    *niter_out = niter;
+    *err_out = err;
+    *res_out = hr[0] <= rmin ? std::sqrt(hr[1] / n) : std::sqrt(hr[1] / hr[0]);
+    return niter >= itmxcg ? CATHY_GPU_LSFAIL : 0;
+  }
+
   // Surface 3-colouring by triangle propagation (a triangulated disc with
   // even interior degrees is 3-colourable and the colouring is forced);
   // greedy fallback otherwise.
@@ -311,8 +397,9 @@ struct Flow {
       if (cnt[k]) sched.cols.push_back(ColDesc{(int)base[k], cnt[k], nstr + 1});
   }
 
-  void build_pattern(const int *tetra) {
-    // full pattern in device numbering: sorted slots per row
+  // Full pattern in device numbering, from the tetrahedra (tetra != null)
+  // or from a 1-based upper CSR (topol, ja) in natural order.
+  void build_pattern(const int *tetra, const int *topol = nullptr, const int *ja = nullptr) {
     const int W = kMaxRow;
     std::vector<int> slot((size_t)n * W), cnt(n, 0);
     auto ins = [&](int r, int c) {
@@ -325,11 +412,22 @@ struct Flow {
       s[pos] = c;
       cnt[r] = m + 1;
     };
-    for (int e = 0; e < nt; ++e) {
-      int v[4];
-      for (int k = 0; k < 4; ++k) v[k] = perm[tetra[5 * e + k] - 1];
-      for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) ins(v[a], v[b]);
+    if (tetra) {
+      for (int e = 0; e < nt; ++e) {
+        int v[4];
+        for (int k = 0; k < 4; ++k) v[k] = perm[tetra[5 * e + k] - 1];
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 4; ++b) ins(v[a], v[b]);
+      }
+    } else {
+      for (int r = 0; r < n; ++r) {
+        ins(r, r);
+        for (int q = topol[r] - 1; q < topol[r + 1] - 1; ++q) {
+          const int c = ja[q] - 1;
+          ins(r, c);
+          ins(c, r);
+        }
+      }
     }
     std::vector<int> hrp(n + 1, 0);
     for (int r = 0; r < n; ++r) hrp[r + 1] = hrp[r] + cnt[r];
@@ -613,16 +711,23 @@ struct Flow {
   }
 
   void alloc_state() {
-    for (DBuf<double> *b : {&pnew, &pold, &ptnew, &ptold, &ptimep, &pdiff, &sw, &ckrw, &ckrwp, &et1, &et2,
-                            &swnew, &swtimep, &tnoti, &xt5, &bcflux, &qnode, &dinv, &r, &p, &Bv, &z})
+    for (DBuf<double> *b : {&pnew, &pold, &ptnew, &ptold, &ptimep, &sw, &ckrw, &ckrwp, &et1, &et2, &swnew,
+                            &swtimep, &xt5, &bcflux, &qnode})
       b->alloc(n);
     ckrwe.alloc(nt);
     et1e.alloc(nt);
+    ifatm.alloc(nnod); ifatmp.alloc(nnod);
+    atmpot.alloc(nnod); atmact.alloc(nnod); atmold.alloc(nnod); qbot.alloc(nnod);
+    alloc_solver();
+    CG_HIP(hipMemsetAsync(qnode.p, 0, sizeof(double) * n, st));
+  }
+  // what SYMSLV needs: matrix, factor, PCG vectors and reduction scratch
+  void alloc_solver() {
+    for (DBuf<double> *b : {&pdiff, &tnoti, &dinv, &r, &p, &Bv, &z}) b->alloc(n);
     A.alloc(nnz_sell);
     CG_HIP(hipMemsetAsync(A.p, 0, sizeof(double) * nnz_sell, st));
     isdir.alloc(n);
-    ifatm.alloc(nnod); ifatmp.alloc(nnod);
-    atmpot.alloc(nnod); atmact.alloc(nnod); atmold.alloc(nnod); qbot.alloc(nnod);
+    CG_HIP(hipMemsetAsync(isdir.p, 0, n, st));
     if (f32) {
       F32.alloc(nnz_sell);
       CG_HIP(hipMemsetAsync(F32.p, 0, sizeof(float) * nnz_sell, st));
@@ -637,7 +742,6 @@ struct Flow {
     flags.alloc(2);
     res.alloc(16);
     h_res.assign(16, 0.0);
-    CG_HIP(hipMemsetAsync(qnode.p, 0, sizeof(double) * n, st));
     CG_HIP(hipMemsetAsync(flags.p, 0, sizeof(int) * 2, st));
   }
 
@@ -1334,6 +1438,46 @@ int cathy_gpu_calib_copy(const int64_t *nbytes, const int *reps, double *ms_per_
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(s);
+    return 0;
+  });
+}
+
+// One SYMSLV set-up per calling thread, rebuilt when N, NTERM or the
+// pattern (TOPOL / JA checksum) changes.
+static uint64_t pattern_key(int n, int nterm, const int *topol, const int *ja) {
+  uint64_t h = 1469598103934665603ull ^ (uint64_t)n ^ ((uint64_t)nterm << 32);
+  for (int i = 0; i <= n; ++i) h = (h ^ (uint32_t)topol[i]) * 1099511628211ull;
+  for (int i = 0; i < nterm; ++i) h = (h ^ (uint32_t)ja[i]) * 1099511628211ull;
+  return h;
+}
+static thread_local std::unique_ptr<Flow> t_sym;
+static thread_local uint64_t t_sym_key = 0;
+
+int cathy_gpu_symslv(const int *n, const int *nterm, const int *numdir, int *niter, const int *itmxcg,
+                     const double *tolcg, const double *rmin, const int *noddir, const int *ja, const int *topol,
+                     double *pnew, const double *tnoti, const double *coef1, double *err, double *res) {
+  return guarded([&] {
+    CG_REQUIRE(n && nterm && numdir && niter && itmxcg && tolcg && rmin && ja && topol && pnew && tnoti && coef1 &&
+                   err && res,
+               "null argument");
+    CG_REQUIRE(*numdir >= 0 && (*numdir == 0 || noddir), "NUMDIR / NODDIR");
+    CG_REQUIRE(*itmxcg >= 1, "ITMXCG must be >= 1");
+    const uint64_t key = pattern_key(*n, *nterm, topol, ja);
+    if (!t_sym || key != t_sym_key) {
+      t_sym.reset();
+      auto f = std::make_unique<Flow>();
+      f->setup_csr(*n, *nterm, topol, ja);
+      t_sym = std::move(f);
+      t_sym_key = key;
+    }
+    return t_sym->symslv(*numdir, noddir, *itmxcg, *tolcg, *rmin, tnoti, coef1, pnew, niter, err, res);
+  });
+}
+
+int cathy_gpu_symslv_release(void) {
+  return guarded([&] {
+    t_sym.reset();
+    t_sym_key = 0;
     return 0;
   });
 }

@@ -148,6 +148,8 @@ __global__ void k_free_drainage(int nnod, const int *bot_row, const double *aren
                                 const double *kznod, double *qbot, double *qnode);
 __global__ void k_store(int n, const double *sw, const double *volnod, const double *pnodi, double *part);
 __global__ void k_sum_final(int nb, const double *part, double *out);
+__global__ void k_gather_coef(int nslots, const int *map, const double *coef, double *A);
+__global__ void k_sumsq_free(int n, const double *u, const uint8_t *isdir, double *part);
 __global__ void k_copy(size_t n, const double *src, double *dst);
 __global__ void k_fill(size_t n, double v, double *dst);
 

@@ -980,6 +980,27 @@ __global__ void k_sum_final(int nb, const double *__restrict__ part, double *__r
   if (threadIdx.x == 0) *out = v;
 }
 
+// A[q] = COEF1[map[q]] (upper CSR value of the slot's symmetric entry).
+__global__ void k_gather_coef(int nslots, const int *__restrict__ map, const double *__restrict__ coef,
+                              double *__restrict__ A) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nslots) {
+    const int m = map[q];
+    A[q] = m >= 0 ? coef[m] : 0.0;
+  }
+}
+
+// partial sums of u^2 over the free (non-Dirichlet) rows
+__global__ void k_sumsq_free(int n, const double *__restrict__ u, const uint8_t *__restrict__ isdir,
+                             double *__restrict__ part) {
+  __shared__ double sh[kBlock / 64 + 1];
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (!isdir[i]) acc += u[i] * u[i];
+  double t = block_sum<kBlock>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 __global__ void k_copy(size_t n, const double *__restrict__ src, double *__restrict__ dst) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];

@@ -168,6 +168,33 @@ class FlowGPU:
             pass
 
 
+def symslv(topol: np.ndarray, ja: np.ndarray, coef1: np.ndarray, tnoti: np.ndarray, noddir=None,
+           itmxcg: int = 500, tolcg: float = 1e-10, rmin: float = 1.7e-100):
+    """SYMSLV on the device (solscal-extended.f:4669): ``topol`` (N+1),
+    ``ja`` (NTERM) 1-based upper CSR with the diagonal first, ``coef1``
+    (NTERM), ``tnoti`` (N), ``noddir`` 1-based Dirichlet rows.  Returns
+    (pnew, niter, err, res, rc) with rc = 1 when NITER reached ITMXCG."""
+    topol = np.ascontiguousarray(topol, dtype=np.int32)
+    ja = np.ascontiguousarray(ja, dtype=np.int32)
+    coef1 = np.ascontiguousarray(coef1, dtype=np.float64)
+    tnoti = np.ascontiguousarray(tnoti, dtype=np.float64)
+    nd = np.ascontiguousarray(noddir if noddir is not None else np.zeros(0), dtype=np.int32)
+    n = len(topol) - 1
+    pnew = np.zeros(n)
+    niter = C.c_int(0)
+    err = C.c_double(0.0)
+    res = C.c_double(0.0)
+    rc = _check(lib().cathy_gpu_symslv(_i(n), _i(len(ja)), _i(len(nd)), C.byref(niter), _i(itmxcg),
+                                       C.byref(C.c_double(tolcg)), C.byref(C.c_double(rmin)),
+                                       _p(nd) if len(nd) else None, _p(ja), _p(topol), _p(pnew), _p(tnoti),
+                                       _p(coef1), C.byref(err), C.byref(res)), "cathy_gpu_symslv")
+    return pnew, niter.value, err.value, res.value, rc
+
+
+def symslv_release() -> None:
+    _check(lib().cathy_gpu_symslv_release(), "cathy_gpu_symslv_release")
+
+
 def calib_copy(nbytes: int, reps: int = 20) -> float:
     """Average ms of a copy kernel reading and writing ``nbytes``."""
     ms = C.c_double(0.0)

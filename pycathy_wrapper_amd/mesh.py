@@ -283,9 +283,10 @@ UPPER_PAIRS = [(k, l) for k in range(4) for l in range(k, 4)]
 
 
 def build_mesh(dem: np.ndarray, dx: float, dy: float, nstr: int, zratio, base: float,
-               zone: np.ndarray | None = None, ivert: int = 0) -> Mesh:
+               zone: np.ndarray | None = None, ivert: int = 0, west: float = 0.0, south: float = 0.0,
+               factor: float = 1.0) -> Mesh:
     nrow, ncol = dem.shape
-    nnod, ntri, x, y, z, tri = surface_from_dem(dem, dx, dy, zone)
+    nnod, ntri, x, y, z, tri = surface_from_dem(dem, dx, dy, zone, west, south, factor)
     X, Y, Z, tet = extrude(nnod, nstr, tri, x, y, z, zratio, base, ivert)
     are = area_nodes(nnod, tri, x, y)
     m = Mesh(nrow=nrow, ncol=ncol, nnod=nnod, ntri=ntri, nstr=nstr, x=X, y=Y, z=Z, triang=tri,

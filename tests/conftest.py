@@ -20,5 +20,11 @@ def has_ref() -> bool:
 
 @pytest.fixture(scope="session")
 def oracle():
+    """The C oracle (oracle/liboracle.so), built with gcc if absent."""
+    import subprocess
+    so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(so):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "liboracle"], check=True,
+                       stdout=subprocess.DEVNULL)
     import pyoracle
     return pyoracle.Oracle()

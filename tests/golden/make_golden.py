@@ -1,0 +1,144 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ (run in the development
+container, where the reference tree is mounted at /root/reference).
+
+1. meshes.npz -- mesh golden vectors held by the reference's stored runs:
+   for each example project, the DEM raster (prepro/dem), zone raster
+   (prepro/zone), the grid parameters (input/dem_parameters, the fields
+   datin.f:185-290 reads) and the mesh CATHY wrote (output/grid3d, written by
+   gen3d.f:91-101 before riord.f sorts the vertices: NNOD N NT, TETRA(5,NT),
+   X Y Z per node at 1PE13.6), plus NTERM from output/risul when printed.
+2. hillslope_small_ref.npz -- the hillslope_small run (21x21 nodes, 10
+   layers, 1 day) stepped by the reference's own FLOW3D (compiled from the
+   reference sources by oracle/Makefile into oracle/_ref/libcathy_ref.so and
+   driven by tests/refrun.py): per-step statistics of every step and psi/Sw
+   at selected steps.
+
+Only data goes into the fixtures; no reference source text.
+"""
+from __future__ import annotations
+
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+
+MESH_CASES = [
+    "examples/SSHydro/weill_exemple",
+    "examples/SSHydro/atmbc_spatially_from_weill_withnodata",
+    "examples/SSHydro/bc_noflow",
+    "examples/SSHydro/meshing_from_weill",
+    "examplesTmp/catchments/outlets",
+    "examplesTmp/Luca/my_first_project4",
+    "examples/DA/DA_with_non_uniform_ic",
+    "pyCATHY/tests/data/simu_test_DA/tensiometers_DA",
+]
+
+
+def read_raster(path: str, header: bool = False):
+    """GRASS-ASCII raster as rast_input_dem.f reads it (6 header lines)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    hdr = {}
+    for ln in lines[:6]:
+        k, v = ln.split(":", 1)
+        hdr[k.strip()] = float(v)
+    nrow, ncol = int(hdr["rows"]), int(hdr["cols"])
+    vals = np.array(" ".join(lines[6:]).split(), dtype=np.float64)
+    ras = vals[: nrow * ncol].reshape(nrow, ncol)
+    return (ras, hdr) if header else ras
+
+
+def read_dem_parameters(path: str) -> dict:
+    toks = []
+    for t in open(path).read().split():
+        try:
+            toks.append(float(t))
+        except ValueError:
+            break
+    dx, dy, factor, dostep, nzone, nstr, n1, ivert, isp, base = toks[:10]
+    nstr = int(nstr)
+    return dict(dx=dx, dy=dy, factor=factor, dostep=int(dostep), nzone=int(nzone), nstr=nstr, n1=int(n1),
+                ivert=int(ivert), isp=int(isp), base=base, zratio=np.array(toks[10:10 + nstr]))
+
+
+def read_grid3d(path: str):
+    with open(path) as f:
+        nnod, n, nt = (int(v) for v in f.readline().split())
+        tet = np.array([f.readline().split() for _ in range(nt)], dtype=np.int32)
+        xyz = np.array([f.readline().split() for _ in range(n)], dtype=np.float64)
+    return nnod, n, nt, tet, xyz
+
+
+def read_nterm(path: str) -> int:
+    if not os.path.exists(path):
+        return -1
+    with open(path, errors="replace") as f:
+        for ln in f:
+            m = re.search(r"NTERM \(# OF NONZERO TERMS IN SYS\. MAT\.\)\s*=\s*(\d+)", ln)
+            if m:
+                return int(m.group(1))
+    return -1
+
+
+def make_meshes(out: str) -> None:
+    arrs = {}
+    names = []
+    for i, case in enumerate(MESH_CASES):
+        d = os.path.join(REF, case)
+        prm = read_dem_parameters(os.path.join(d, "input/dem_parameters"))
+        dem, hdr = read_raster(os.path.join(d, "prepro/dem"), header=True)
+        zp = os.path.join(d, "prepro/zone")
+        zone = read_raster(zp).astype(np.int32) if os.path.exists(zp) else np.ones(dem.shape, np.int32)
+        nnod, n, nt, tet, xyz = read_grid3d(os.path.join(d, "output/grid3d"))
+        p = f"c{i}_"
+        arrs[p + "dem"] = dem
+        arrs[p + "zone"] = zone
+        arrs[p + "zratio"] = prm.pop("zratio")
+        arrs[p + "scalars"] = np.array([prm[k] for k in ("dx", "dy", "factor", "base")] + [hdr["west"], hdr["south"]])
+        arrs[p + "ints"] = np.array([prm[k] for k in ("dostep", "nzone", "nstr", "n1", "ivert", "isp")] +
+                                    [nnod, n, nt, read_nterm(os.path.join(d, "output/risul"))], dtype=np.int64)
+        arrs[p + "tetra"] = tet
+        arrs[p + "xyz"] = xyz
+        names.append(case)
+        print(f"{case}: nnod={nnod} n={n} nt={nt} nterm={arrs[p + 'ints'][-1]}")
+    arrs["cases"] = np.array(names)
+    np.savez_compressed(out, **arrs)
+
+
+def make_hillslope_small(out: str, steps_saved=(1, 2, 5, 10, 25, 50, 100, 150)) -> None:
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from pycathy_wrapper_amd import configs as CF
+    import pyoracle as P
+    import refrun as R
+    pr = CF.hillslope_small()
+    rr = R.RefFlowRun(P.RefLib(), pr.mesh, pr.soil, pr.psi0, float(pr.atm_vals[0]), pr.params.as_dict())
+    stats, psi, sw, saved = [], [], [], []
+    keys = None
+    k = 0
+    while True:
+        rc, s = rr.step()
+        k += 1
+        keys = keys or list(s)
+        stats.append([s[nm] for nm in keys])
+        if k in steps_saved or rc:
+            saved.append(k)
+            psi.append(rr.psi().copy())
+            sw.append(rr.sw().copy())
+        if rc:
+            break
+    np.savez_compressed(out, stat_names=np.array(keys), stats=np.array(stats), steps=np.array(saved),
+                        psi=np.array(psi), sw=np.array(sw), rc_last=rc)
+    print(f"hillslope_small: {k} steps, saved {saved}")
+
+
+if __name__ == "__main__":
+    make_meshes(os.path.join(HERE, "meshes.npz"))
+    make_hillslope_small(os.path.join(HERE, "hillslope_small_ref.npz"))

@@ -1,0 +1,182 @@
+"""The oracle against committed reference outputs and closed forms.
+
+tests/golden/hillslope_small_ref.npz is the hillslope_small day (21x21
+nodes, 10 layers, 185 time steps) as the reference's own FLOW3D computed it
+(tests/golden/make_golden.py).  These tests need only the oracle, so they
+pin it wherever the reference library cannot be built.  The closed-form
+tests check what the mathematics fixes: the van Genuchten-Mualem
+relations, IC(0) on a chordal pattern being the exact Cholesky factor, and
+PCG reaching the solution of the assembled system.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from pycathy_wrapper_amd import configs as CF
+import systems as SY
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hillslope_small_ref.npz")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(GOLD)
+
+
+def _oracle_run(oracle, pr):
+    import pyoracle as P
+    m = pr.mesh
+    return oracle.run_create(dict(tetra=m.tetra, x=m.x, y=m.y, z=m.z, arenod=m.arenod, nnod=m.nnod, nstr=m.nstr,
+                                  ntri=m.ntri), pr.soil, pr.psi0, pr.atm_times, pr.atm_vals, 1,
+                             P.OrcParams(**pr.params.as_dict()))
+
+
+def test_oracle_full_day_matches_reference_run(oracle, golden):
+    """Every statistic of all 185 steps and psi/Sw at the saved steps are
+    bit-identical to the reference run."""
+    pr = CF.hillslope_small()
+    names = [str(s) for s in golden["stat_names"]]
+    stats = golden["stats"]
+    saved = list(golden["steps"])
+    run = _oracle_run(oracle, pr)
+    try:
+        k = 0
+        while True:
+            rc, s = run.step()
+            k += 1
+            for j, nm in enumerate(names):
+                assert s[nm] == stats[k - 1, j], (k, nm, s[nm], stats[k - 1, j])
+            if k in saved:
+                i = saved.index(k)
+                assert np.array_equal(run.psi(), golden["psi"][i]), k
+                assert np.array_equal(run.sw(), golden["sw"][i]), k
+            if rc:
+                break
+        assert k == len(stats) and rc == int(golden["rc_last"])
+    finally:
+        run.close()
+
+
+# ------------------------------------------------------- closed forms --
+def _vg_tables(oracle, n, vgn, psat, rmc=0.05, poros=0.45):
+    """orc_chparm tables for n nodes of one soil."""
+    vg = np.zeros(11 * n)
+    pn = np.full(n, poros)
+    from pyoracle import dptr
+    # chparm reads VGN/VGRMC/VGPSAT from the vg array's own fields
+    vg[1 * n:2 * n] = vgn
+    vg[3 * n:4 * n] = rmc
+    vg[4 * n:5 * n] = psat
+    oracle.L.orc_chparm(n, dptr(pn), dptr(vg))
+    return vg
+
+
+def test_van_genuchten_closed_form(oracle):
+    """fvgse/fvgkr/fvgdse vs van Genuchten (1980) / Mualem (1976) written
+    from the published formulas, for psi < 0, and the saturated branch."""
+    vgn, psat = 1.46, 1.0  # psat = 1/alpha
+    vg = _vg_tables(oracle, 1, vgn, psat)
+    m = 1.0 - 1.0 / vgn
+    L = oracle.L
+    for psi in (-1e-3, -0.05, -0.7, -3.0, -40.0):
+        se = L.orc_fvgse(C.c_double(psi), vg.ctypes.data_as(C.c_void_p), 1, 0)
+        se_ref = (1.0 + (abs(psi) / psat) ** vgn) ** (-m)
+        assert se == pytest.approx(se_ref, rel=1e-13)
+        kr = L.orc_fvgkr(C.c_double(psi), C.c_double(se), vg.ctypes.data_as(C.c_void_p), 1, 0)
+        kr_ref = np.sqrt(se_ref) * (1.0 - (1.0 - se_ref ** (1.0 / m)) ** m) ** 2
+        assert kr == pytest.approx(kr_ref, rel=1e-10)
+        dse = L.orc_fvgdse(C.c_double(psi), vg.ctypes.data_as(C.c_void_p), 1, 0)
+        h = 1e-6 * abs(psi)
+        fd = ((1.0 + (abs(psi + h) / psat) ** vgn) ** (-m) - (1.0 + (abs(psi - h) / psat) ** vgn) ** (-m)) / (2 * h)
+        assert dse == pytest.approx(fd, rel=1e-6)
+    for psi in (0.0, 0.3):
+        assert L.orc_fvgse(C.c_double(psi), vg.ctypes.data_as(C.c_void_p), 1, 0) == 1.0
+        assert L.orc_fvgdse(C.c_double(psi), vg.ctypes.data_as(C.c_void_p), 1, 0) == 0.0
+    # kr is increasing in psi on the unsaturated branch
+    ps = -np.logspace(-3, 2, 40)
+    krs = [L.orc_fvgkr(C.c_double(p), C.c_double(L.orc_fvgse(C.c_double(p), vg.ctypes.data_as(C.c_void_p), 1, 0)),
+                       vg.ctypes.data_as(C.c_void_p), 1, 0) for p in ps]
+    assert np.all(np.diff(krs) < 0)
+
+
+def _tridiag_upper_csr(n, rng):
+    """SPD tridiagonal matrix in the reference's upper CSR (diagonal first)."""
+    off = -rng.uniform(0.1, 1.0, n - 1)
+    diag = np.abs(np.r_[off, 0]) + np.abs(np.r_[0, off]) + rng.uniform(0.5, 1.5, n)
+    topol = [0]; ja = []; val = []
+    for i in range(n):
+        ja.append(i); val.append(diag[i])
+        if i + 1 < n:
+            ja.append(i + 1); val.append(off[i])
+        topol.append(len(ja))
+    dense = np.diag(diag) + np.diag(off, 1) + np.diag(off, -1)
+    return np.array(topol, np.int32), np.array(ja, np.int32), np.array(val), dense
+
+
+def test_incldp_on_chordal_pattern_is_cholesky(oracle):
+    """A tridiagonal pattern has no fill, so IC(0) is the exact Cholesky
+    factor: INCLDP's upper factor equals numpy's L^T entry by entry."""
+    from pyoracle import dptr, iptr
+    rng = np.random.default_rng(5)
+    n = 50
+    topol, ja, val, dense = _tridiag_upper_csr(n, rng)
+    lr = np.zeros(len(val))
+    nf = oracle.L.orc_incldp(iptr(ja), iptr(topol), dptr(val), dptr(lr), n, len(val))
+    assert nf == 0
+    U = np.linalg.cholesky(dense).T
+    for i in range(n):
+        for q in range(topol[i], topol[i + 1]):
+            assert lr[q] == pytest.approx(U[i, ja[q]], rel=1e-12)
+
+
+def test_symslv_solves_assembled_system(oracle):
+    """PCG on the hillslope_small Picard system reaches the dense solution,
+    with and without penalised Dirichlet rows (whose unknowns go to 0)."""
+    pr = CF.hillslope_small()
+    s = SY.picard_system(oracle, pr, deltat=3600.0)
+    n = s["n"]
+    dense = np.zeros((n, n))
+    for i in range(n):
+        for q in range(s["topol"][i], s["topol"][i + 1]):
+            dense[i, s["ja"][q]] = s["coef1"][q]
+            dense[s["ja"][q], i] = s["coef1"][q]
+    x_ref = np.linalg.solve(dense, s["tnoti"])
+    x, it, err, res, _ = SY.oracle_symslv(oracle, s["topol"], s["ja"], s["coef1"], s["tnoti"], tolcg=1e-13)
+    assert err <= 1e-13
+    assert np.max(np.abs(x - x_ref)) <= 1e-8 * np.max(np.abs(x_ref))
+    rows = np.arange(0, pr.mesh.nnod, 3)
+    c1, tno, nd1 = SY.penalise(s, rows)
+    x2, it2, err2, res2, _ = SY.oracle_symslv(oracle, s["topol"], s["ja"], c1, tno, nd1 - 1, tolcg=1e-13)
+    free = np.setdiff1d(np.arange(n), rows)
+    sub = dense[np.ix_(free, free)]
+    x2_ref = np.linalg.solve(sub, s["tnoti"][free])
+    assert np.max(np.abs(x2[rows])) <= 1e-80
+    assert np.max(np.abs(x2[free] - x2_ref)) <= 1e-8 * np.max(np.abs(x2_ref))
+
+
+def test_assembly_identities(oracle):
+    """ASSPIC: stiffness rows sum to zero (constant psi is in the kernel
+    of K), the lumped storage matrix is diagonal with sum V*ET1, and the
+    porosity matrix sums to the domain volume times PEL."""
+    pr = CF.hillslope_small()
+    s = SY.picard_system(oracle, pr)
+    n, topol, ja = s["n"], s["topol"], s["ja"]
+    k1 = s["asm"]["c1"]
+    rows = np.zeros(n)
+    for i in range(n):
+        for q in range(topol[i], topol[i + 1]):
+            rows[i] += k1[q]
+            if q > topol[i]:
+                rows[ja[q]] += k1[q]
+    assert np.max(np.abs(rows)) <= 1e-12 * np.max(np.abs(k1))
+    c2 = s["asm"]["c2"]
+    offd = np.ones(len(ja), bool)
+    offd[topol[:-1]] = False
+    assert np.all(c2[offd] == 0.0)  # LUMP = 1
+    m = pr.mesh
+    assert c2.sum() == pytest.approx(np.sum(m.volu * s["et1e"]), rel=1e-12)
+    assert s["asm"]["c4"].sum() == pytest.approx(np.sum(m.volu * s["pele"]), rel=1e-12)
