@@ -12,14 +12,25 @@ column ordering).  A step = one accepted implicit time step of one member
 The W warm-up steps run on the device and the state is then reset to the
 initial condition, so the K timed steps are steps 1..K of the run; the CPU
 baseline (the C oracle, one core) times the first steps of the same run.
-The library call is synchronous, so the wall clock around the K calls
+Every library call is synchronous, so the wall clock around the K calls
 brackets all device work.
+
+The roofline entry is the PCG SpMV (k_spmv_dot2): its average kernel
+duration comes from the dispatch packets' own timestamps
+(hipExtLaunchKernelGGL start/stop events on the library's stream) over the
+timed region; the bytes are the algorithmic CSR SpMV bytes
+12*nnz + 4*(N+1) + 16*N.  ``traffic`` is the measured HBM bytes per launch
+from rocprofv3 FETCH_SIZE / WRITE_SIZE passes, read from
+profiles/k_spmv_dot2_hillslope_large_traffic.txt when that summary exists
+(made by --pmc-summary from two --pmc runs of this script with --calib-mb).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
 import sys
@@ -29,6 +40,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "ensemble member Picard steps/s at 1.2M nodes, learned vs plain; SpMV % HBM peak"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_spmv_dot2_hillslope_large_traffic.txt")
 
 
 def log(*a):
@@ -45,7 +58,7 @@ def build_problem(cells: int, nstr: int):
     return CF.hillslope_large(cells=cells, nstr=nstr, params=bench_params())
 
 
-def cpu_baseline(problem, seconds: float) -> dict:
+def cpu_baseline(problem, max_steps: int, seconds: float) -> dict:
     """The C oracle (one core) on the first steps of the same run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as P
@@ -62,29 +75,91 @@ def cpu_baseline(problem, seconds: float) -> dict:
         picard += s["ITER"]
         pcg += s["NITERT"]
         el = time.perf_counter() - t0
-        log(f"[cpu] step {steps} dt={s['DELTAT']:.4g} iter={s['ITER']:.0f} pcg={s['NITERT']:.0f} t={el:.1f}s")
-        if rc or el >= seconds:
+        log(f"[cpu] step {steps} dt={s['DELTAT']:.4g} iter={s['ITER']:.0f} pcg={s['NITERT']:.0f} "
+            f"back={s['BACK']:.0f} t={el:.1f}s")
+        if rc or steps >= max_steps or el >= seconds:
             break
     run.close()
-    return dict(value=steps / el, unit="member-steps/s", cores=1, kind="port",
-                sample=f"oracle/cathy_oracle.c, steps 1..{steps} of the same run ({picard:.0f} Picard, "
-                       f"{pcg:.0f} PCG iterations, natural-order IC(0)) in {el:.1f} s on one host core")
+    return dict(value=steps / el, unit="member-steps/s", cores=1, kind="port", steps=steps, seconds=el,
+                sample=f"oracle/cathy_oracle.c (C restatement of the reference FLOW3D, natural-order FP64 IC(0)), "
+                       f"steps 1..{steps} of the same run in {el:.1f} s on one host core; ITER/NITERT of the "
+                       f"accepted attempts: {picard:.0f} Picard, {pcg:.0f} PCG")
+
+
+def read_traffic():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    for ln in open(TRAFFIC_FILE):
+        if ":" in ln and not ln.lstrip().startswith("#"):
+            k, v = ln.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def pmc_summary(paths: list[str], kernel: str, copy_bytes: float) -> int:
+    """Average per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of ``kernel`` and of
+    the calibration copy kernel from rocprofv3 --pmc CSV output, and the HBM
+    bytes per launch each implies after scaling by copy_bytes / counter."""
+    acc = {}
+    for pat in paths:
+        for f in glob.glob(os.path.join(pat, "**", "*counter_collection.csv"), recursive=True) + \
+                 (glob.glob(pat) if pat.endswith(".csv") else []):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    name = row.get("Kernel_Name", "")
+                    key = "spmv" if kernel in name else ("copy" if "k_copy" in name else None)
+                    if key is None:
+                        continue
+                    cn = row.get("Counter_Name", "")
+                    v = float(row.get("Counter_Value", "nan"))
+                    acc.setdefault((key, cn), []).append(v)
+    out = {}
+    for (key, cn), vs in sorted(acc.items()):
+        # SpMV launches queued after PCG convergence exit at once and move
+        # (almost) nothing; only the live launches are averaged
+        live = [v for v in vs if v >= 0.05 * max(vs)]
+        out[(key, cn)] = sum(live) / len(live)
+        print(f"{key} {cn}: {len(live)} live of {len(vs)} dispatches, mean {out[(key, cn)]:.1f} KiB")
+    total = 0.0
+    ok = True
+    for cn in ("FETCH_SIZE", "WRITE_SIZE"):
+        if ("spmv", cn) not in out or ("copy", cn) not in out:
+            ok = False
+            continue
+        factor = copy_bytes / (out[("copy", cn)] * 1024.0)
+        b = out[("spmv", cn)] * 1024.0 * factor
+        print(f"{cn}_calibration_factor: {factor:.4f}")
+        print(f"{cn}_bytes_per_launch: {b:.0f}")
+        total += b
+    if ok:
+        print(f"traffic_bytes_per_launch: {total:.0f}")
+    return 0 if ok else 1
 
 
 def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__)
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=200)
     ap.add_argument("--nstr", type=int, default=30)
     ap.add_argument("--ordering", choices=["column3", "natural"], default="column3")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="0 skips the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="time steps of the CPU baseline (0 skips it)")
+    ap.add_argument("--cpu-seconds", type=float, default=60.0, help="stop the CPU baseline after this long")
+    ap.add_argument("--calib-mb", type=float, default=0.0,
+                    help="after the timed steps, run the copy kernel over this many MB (PMC calibration)")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DIR_OR_CSV",
+                    help="summarise rocprofv3 --pmc CSV output (FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        return pmc_summary(args.pmc_summary, "k_spmv_dot2", args.calib_mb * 1e6)
     if args.gpus != 1:
-        print(f"bench.py: this build runs one member per GPU on a single MI355X; --gpus {args.gpus} "
-              f"is not supported (ensemble members shard across GPUs without communication, see DESIGN.md)",
-              file=sys.stderr)
+        print(f"bench.py: one GPU only; --gpus {args.gpus} is not supported (members of an ensemble shard "
+              f"across GPUs without communication; no multi-GPU run is part of this build)", file=sys.stderr)
         return 2
 
     from pycathy_wrapper_amd import gpu
@@ -98,6 +173,9 @@ def main() -> int:
     t0 = time.perf_counter()
     g = gpu.FlowGPU(pr, order)
     log(f"[setup] device {name} ({cus} CUs), flow created in {time.perf_counter() - t0:.1f}s")
+    lay = g.layout()
+    log(f"[setup] full pattern nnz={lay['nnz']}, SELL-64 slots={lay['nnz_sell']} "
+        f"({lay['nnz_sell'] / lay['nnz'] - 1:.1%} padding), longest row {lay['maxrow']}")
 
     for w in range(args.warmup):
         rc, s = g.advance()
@@ -106,14 +184,17 @@ def main() -> int:
 
     g.profile(True)
     steps = picard = pcg = backs = 0
-    sim0 = 0.0
     sim1 = 0.0
+    step_wall = []
     t0 = time.perf_counter()
     for k in range(args.steps):
+        ts = time.perf_counter()
         rc, s = g.advance()
+        step_wall.append(time.perf_counter() - ts)
+        c = g.counters()
         steps += 1
-        picard += s["ITER"]
-        pcg += s["NITERT"]
+        picard += c["picard"]   # every Picard iteration, back-stepped attempts included
+        pcg += c["pcg"]
         backs += s["BACK"]
         sim1 = s["TIME"]
         if rc:
@@ -122,18 +203,26 @@ def main() -> int:
     spmv_ms, spmv_n = g.profile(None)
     g.profile(False)
     wall = t1 - t0
-    log(f"[timed] {steps} steps in {wall:.3f}s: {picard:.0f} Picard, {pcg:.0f} PCG, {backs:.0f} back-steps, "
-        f"sim {sim0:.1f}->{sim1:.1f}s")
+    log(f"[timed] {steps} steps in {wall:.3f}s: {picard:.0f} Picard, {pcg:.0f} PCG iterations (back-stepped "
+        f"attempts included), {backs:.0f} back-steps, sim 0 -> {sim1:.2f} s")
     spmv_bytes = g.spmv_bytes()
     avg_ms = spmv_ms / max(1, spmv_n)
     achieved = spmv_bytes / (avg_ms * 1e-3) / 1e9 if spmv_n else 0.0
     log(f"[roofline] k_spmv_dot2: {spmv_n} launches, {avg_ms * 1e3:.2f} us avg, {spmv_bytes / 1e6:.1f} MB "
         f"-> {achieved:.0f} GB/s")
+    if args.calib_mb > 0:
+        ms = gpu.calib_copy(int(args.calib_mb * 1e6), 5)
+        log(f"[calib] copy of {args.calib_mb:.0f} MB: {ms * 1e3:.1f} us/launch")
     cpu = None
-    if args.cpu_seconds > 0:
-        cpu = cpu_baseline(pr, args.cpu_seconds)
+    if args.cpu_steps > 0 and args.cpu_seconds > 0:
+        cpu = cpu_baseline(pr, args.cpu_steps, args.cpu_seconds)
+        k = min(cpu["steps"], len(step_wall))
+        gsec = sum(step_wall[:k])
+        cpu["device_seconds_same_steps"] = gsec
+        log(f"[cpu] steps 1..{k}: oracle {cpu['seconds']:.2f} s, device {gsec:.3f} s")
+    traffic = read_traffic()
     out = {
-        "metric": "ensemble member Picard steps/s at 1.2M nodes, learned vs plain; SpMV % HBM peak",
+        "metric": METRIC,
         "value": steps / wall,
         "unit": "member-steps/s",
         "n_gpus": 1,
@@ -146,16 +235,18 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"hillslope_large: {m.nrow}x{m.ncol} cells, {m.nstr} layers, N={m.n} nodes, "
-                        f"NT={m.nt} tets, plain Picard + IC(0)-PCG ({args.ordering}, FP32 factor), one member",
+            "workload": f"hillslope_large: {m.nrow}x{m.ncol} cells, {m.nstr} graded layers, N={m.n} nodes, "
+                        f"NT={m.nt} tets, steps 1..{steps} of the 2-day run, plain Picard + IC(0)-PCG "
+                        f"({args.ordering} ordering, FP32 factor, FP64 state/vectors), one member",
             "nodes": m.n,
             "tets": m.nt,
             "picard_per_s": picard / wall,
             "pcg_per_s": pcg / wall,
-            "sim_s_per_wall_s": (sim1 - sim0) / wall,
+            "sim_s_per_wall_s": sim1 / wall,
             "picard_iterations": picard,
             "pcg_iterations": pcg,
             "back_steps": backs,
+            "learned_guess": False,
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -165,7 +256,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "avg_us": avg_ms * 1e3,
             "launches": spmv_n,
             "bytes_per_launch": spmv_bytes,

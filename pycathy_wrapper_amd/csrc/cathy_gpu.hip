@@ -7,6 +7,7 @@
 //                       and the per-time-step host logic of cathy_main.f.
 //   extern "C" entries   the FLOW3D / SYMSLV drop-in boundary
 //                       (include/cathy_gpu.h).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -816,10 +817,15 @@ struct Flow {
     PcgScal hs{};
     while (true) {
       for (int b = 0; b < batch; ++b) {
-        if (prof) CG_HIP(hipEventRecord(next_event(), st));
-        hipLaunchKernelGGL(k_spmv_dot2, dim3(Gs), dim3(kBlock), 0, st, n, sell(), A.p, p.p, r.p, Bv.p, P(1),
-                           P(2), done);
-        if (prof) CG_HIP(hipEventRecord(next_event(), st));
+        if (prof) {
+          // the dispatch packet's own start/end timestamps (kernel time only)
+          hipEvent_t e0 = next_event(), e1 = next_event();
+          hipExtLaunchKernelGGL(k_spmv_dot2, dim3(Gs), dim3(kBlock), 0, st, e0, e1, 0, n, sell(), A.p, p.p, r.p,
+                                Bv.p, P(1), P(2), done);
+        } else {
+          hipLaunchKernelGGL(k_spmv_dot2, dim3(Gs), dim3(kBlock), 0, st, n, sell(), A.p, p.p, r.p, Bv.p, P(1),
+                             P(2), done);
+        }
         hipLaunchKernelGGL(k_pcg_alpha, dim3(Gr), dim3(kBlock), 0, st, n, Gs, P(1), P(2), Bv.p, p.p, isdir.p,
                            r.p, pdiff.p, P(3), scal.p, done);
         precond_apply(r.p, z.p, done);
@@ -1412,6 +1418,18 @@ int cathy_gpu_flow3d_spmv_bytes(int64_t *handle, double *bytes) {
     CG_REQUIRE(bytes, "null argument");
     // values (8 B) + column ids (4 B) per entry, row pointers, x read, y write
     *bytes = 12.0 * f->nnz + 4.0 * (f->n + 1) + 16.0 * f->n;
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_layout(int64_t *handle, int64_t *out4) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(out4, "null argument");
+    out4[0] = f->n;
+    out4[1] = f->nnz;
+    out4[2] = f->nnz_sell;
+    out4[3] = f->maxrow;
     return 0;
   });
 }

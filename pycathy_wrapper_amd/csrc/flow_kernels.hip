@@ -180,7 +180,7 @@ template __global__ void k_assemble<32, false>(AsmArgs);
 template <int MR>
 __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, const double *__restrict__ ckrw_node,
                                                           const double *__restrict__ et1_node) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
+  const int r = xcd_group() * kBlock + threadIdx.x;
   if (r >= a.n) return;
   const int s = r >> 6, lane = r & 63;
   const int base = a.sl.so[s] + lane;
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, Pair
   int *Pd = (int *)(smem + 2 * tot); // slot of the diagonal (-1: Dirichlet)
   int *Pl = Pd + tot;                // slot of the predecessor entry
   int *Pt = Pl + tot;                // its transpose slot
-  const int t0 = blockIdx.x * kChains;
+  const int t0 = xcd_group() * kChains;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / kChains, t = t0 + (i % kChains);
     if (t >= C.nc) continue;
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
   extern __shared__ double smem[];
   const int tot = C.len * kChains;
   double *T = smem, *Lp = smem + tot;
-  const int t0 = blockIdx.x * kChains;
+  const int t0 = xcd_group() * kChains;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / kChains, t = t0 + (i % kChains);
     if (t >= C.nc) continue;
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
   extern __shared__ double smem[];
   const int tot = C.len * kChains;
   double *T = smem, *U = smem + tot;
-  const int t0 = blockIdx.x * kChains;
+  const int t0 = xcd_group() * kChains;
   const int kl = C.len - 1;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / kChains, t = t0 + (i % kChains);
@@ -581,7 +581,7 @@ __global__ void k_spmv_dot2(int n, Sell sl, const double *__restrict__ A, const 
   if (*done) return;
   __shared__ double sh[kBlock / 64 + 1];
   double apr = 0.0, apb = 0.0;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_group() * blockDim.x + threadIdx.x;
   if (i < n) {
     const double s = sell_row_dot(sl, A, p, i);
     B[i] = s;

@@ -47,6 +47,20 @@ struct SoilSoA {
 };
 
 // ------------------------------------------------- block reductions --
+// MI355X dispatches the work-groups of a launch round-robin over its 8
+// XCDs (group g runs on XCD g % 8), each XCD with its own L2.  xcd_group
+// maps the physical group id to a logical one so that XCD x owns the
+// contiguous logical range [start(x), start(x) + count(x)): the rows an XCD
+// processes, and the neighbours they gather, then stay in one L2.  The map
+// is a bijection on [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_group() {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int x = b % kXcds, k = b / kXcds;
+  const int q = nb / kXcds, rem = nb % kXcds;
+  return x * q + (x < rem ? x : rem) + k;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

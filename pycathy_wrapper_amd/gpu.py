@@ -157,6 +157,11 @@ class FlowGPU:
         _check(lib().cathy_gpu_flow3d_spmv_bytes(C.byref(self.h), C.byref(b)), "spmv_bytes")
         return b.value
 
+    def layout(self) -> dict:
+        out = np.zeros(4, dtype=np.int64)
+        _check(lib().cathy_gpu_flow3d_layout(C.byref(self.h), _p(out)), "layout")
+        return dict(n=int(out[0]), nnz=int(out[1]), nnz_sell=int(out[2]), maxrow=int(out[3]))
+
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             lib().cathy_gpu_flow3d_destroy(C.byref(self.h))
