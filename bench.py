@@ -175,7 +175,8 @@ def main() -> int:
     log(f"[setup] device {name} ({cus} CUs), flow created in {time.perf_counter() - t0:.1f}s")
     lay = g.layout()
     log(f"[setup] full pattern nnz={lay['nnz']}, SELL-64 slots={lay['nnz_sell']} "
-        f"({lay['nnz_sell'] / lay['nnz'] - 1:.1%} padding), longest row {lay['maxrow']}")
+        f"({lay['nnz_sell'] / lay['nnz'] - 1:.1%} padding), longest row {lay['maxrow']}, coded columns "
+        f"{lay['coded']} (check mismatches {lay['code_mismatch']})")
 
     for w in range(args.warmup):
         rc, s = g.advance()

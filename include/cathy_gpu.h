@@ -148,9 +148,11 @@ int cathy_gpu_flow3d_profile(int64_t *handle, const int *enable,
                              double *spmv_ms, int64_t *launches);
 /* Algorithmic bytes of one SpMV launch: 12*nnz + 4*(N+1) + 16*N. */
 int cathy_gpu_flow3d_spmv_bytes(int64_t *handle, double *bytes);
-/* Matrix layout: out4 = N, nnz of the full pattern, SELL-64 slots
- * (padding included), longest row. */
-int cathy_gpu_flow3d_layout(int64_t *handle, int64_t *out4);
+/* Matrix layout: out6 = N, nnz of the full pattern, SELL-64 slots
+ * (padding included), longest row, 1 if column ids are decoded from the
+ * per-chain codes of the extruded mesh (else read per slot), and the
+ * number of slots whose code failed the set-up check (0). */
+int cathy_gpu_flow3d_layout(int64_t *handle, int64_t *out6);
 /* Copy kernel over NBYTES (read NBYTES, write NBYTES), REPS launches;
  * average ms per launch.  Calibrates HBM counters. */
 int cathy_gpu_calib_copy(const int64_t *nbytes, const int *reps,

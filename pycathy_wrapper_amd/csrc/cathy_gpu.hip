@@ -171,7 +171,114 @@ struct Flow {
   static constexpr int kSlots = 18;
   double *P(int slot) { return part.p + (size_t)slot * kMaxRedBlocks; }
   int red_grid(size_t work) const { return grid_for(work, kBlock, kMaxRedBlocks); }
-  Sell sell() const { return Sell{so.p, scol.p, slen.p, sdiag.p}; }
+  Sell sell() const {
+    Sell s{};
+    s.so = so.p;
+    s.col = scol.p;
+    s.len = slen.p;
+    s.diag = sdiag.p;
+    s.code = ccode.p;
+    s.clen = cclen.p;
+    s.cdiag = ccdiag.p;
+    s.coded = coded;
+    s.nchain = nnod;
+    s.width = cwidth;
+    s.nplane = nstr + 1;
+    s.ncolr = cncolr;
+    for (int c = 0; c < 4; ++c) {
+      s.cbase[c] = ccbase[c];
+      s.cnc[c] = ccnc[c];
+      s.cgoff[c] = ccgoff[c];
+    }
+    return s;
+  }
+
+  // Coded column source (see Sell in flow_kernels.h), derived from the
+  // explicit pattern and required to reproduce it row by row; if any row
+  // disagrees the explicit column ids stay in use.
+  DBuf<uint32_t> ccode;
+  DBuf<uint8_t> cclen, ccdiag;
+  int coded = 0, cwidth = 0, cncolr = 0;
+  int ccbase[4] = {0, 0, 0, 0}, ccnc[4] = {1, 1, 1, 1}, ccgoff[4] = {0, 0, 0, 0};
+  void build_codes() {
+    coded = 0;
+    if (ordering != CATHY_GPU_ORDER_COLUMN3) return;
+    const int ncl = (int)sched.cols.size();
+    if (ncl < 1 || ncl > 4 || nnod >= (1 << 27) || maxrow > 255) return;
+    for (int c = 0; c < ncl; ++c) {
+      ccbase[c] = sched.cols[c].base;
+      ccnc[c] = sched.cols[c].nc;
+      ccgoff[c] = sched.cols[c].g0;
+    }
+    const int nch = nnod, W = maxrow, np = nstr + 1;
+    const uint32_t unset = 0xffffffffu;
+    std::vector<uint32_t> code((size_t)3 * W * nch, unset);
+    std::vector<int> clen((size_t)3 * nch, -1), cdiag((size_t)3 * nch, -1);
+    auto decode = [&](int r, int &c, int &k, int &t) {
+      c = 0;
+      for (int i = 1; i < ncl; ++i)
+        if (r >= ccbase[i]) c = i;
+      const int off = r - ccbase[c];
+      k = off / ccnc[c];
+      t = off - k * ccnc[c];
+    };
+    for (int r = 0; r < n; ++r) {
+      int c, k, t;
+      decode(r, c, k, t);
+      const size_t g = (size_t)ccgoff[c] + t;
+      const int pt = k == 0 ? 0 : (k == np - 1 ? 2 : 1);
+      const int len = h_rowptr[r + 1] - h_rowptr[r];
+      int dg = -1;
+      for (int j = 0; j < len; ++j)
+        if (h_col[h_rowptr[r] + j] == r) dg = j;
+      int &L = clen[pt * nch + g], &D = cdiag[pt * nch + g];
+      if (L < 0) {
+        L = len;
+        D = dg;
+      } else if (L != len || D != dg) {
+        return;
+      }
+      for (int j = 0; j < W; ++j) {
+        uint32_t w;
+        if (j < len) {
+          int c2, k2, t2;
+          decode(h_col[h_rowptr[r] + j], c2, k2, t2);
+          const int dk = k2 - k;
+          if (dk < -1 || dk > 1) return;
+          w = (uint32_t)t2 | ((uint32_t)(dk + 1) << 27) | ((uint32_t)c2 << 29);
+        } else {
+          w = (uint32_t)t | (1u << 27) | ((uint32_t)c << 29);  // padding: the row itself
+        }
+        uint32_t &slot = code[((size_t)pt * W + j) * nch + g];
+        if (slot == unset) slot = w;
+        else if (slot != w) return;
+      }
+    }
+    std::vector<uint8_t> l8(clen.size()), d8(cdiag.size());
+    for (size_t i = 0; i < clen.size(); ++i) {
+      l8[i] = (uint8_t)std::max(0, clen[i]);
+      d8[i] = (uint8_t)std::max(0, cdiag[i]);
+    }
+    for (uint32_t &w : code)
+      if (w == unset) w = 1u << 27;  // plane types without rows (never read)
+    ccode.upload(code, st);
+    cclen.upload(l8, st);
+    ccdiag.upload(d8, st);
+    CG_HIP(hipStreamSynchronize(st));
+    cwidth = W;
+    cncolr = ncl;
+    coded = 1;
+    // the device decode must reproduce every explicit column id
+    DBuf<int> bad(1);
+    CG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_check_codes, dim3(grid_for(n)), dim3(kBlock), 0, st, n, sell(), bad.p);
+    int hb = 0;
+    bad.download(&hb, 1, st);
+    CG_HIP(hipStreamSynchronize(st));
+    if (hb != 0) coded = 0;
+    code_mismatch = hb;
+  }
+  int code_mismatch = 0;
   PairList pairs() const { return PairList{pair_ptr.p, pair_rc.p}; }
 
   // ------------------------------------------------------------ set-up --
@@ -394,8 +501,12 @@ struct Flow {
         perm[ref] = (int)dev;
         iperm[dev] = (int)ref;
       }
+    int g0 = 0;
     for (int k = 0; k < ncolr; ++k)
-      if (cnt[k]) sched.cols.push_back(ColDesc{(int)base[k], cnt[k], nstr + 1});
+      if (cnt[k]) {
+        sched.cols.push_back(ColDesc{(int)base[k], cnt[k], nstr + 1, g0});
+        g0 += cnt[k];
+      }
   }
 
   // Full pattern in device numbering, from the tetrahedra (tetra != null)
@@ -535,6 +646,7 @@ struct Flow {
     h_rowptr = std::move(hrp);
     h_col = std::move(hcol);
     h_so = std::move(hso);
+    if (tetra) build_codes();
   }
   std::vector<int> h_rowptr, h_col, h_so;
 
@@ -1422,14 +1534,16 @@ int cathy_gpu_flow3d_spmv_bytes(int64_t *handle, double *bytes) {
   });
 }
 
-int cathy_gpu_flow3d_layout(int64_t *handle, int64_t *out4) {
+int cathy_gpu_flow3d_layout(int64_t *handle, int64_t *out6) {
   return guarded([&] {
     Flow *f = lookup(handle);
-    CG_REQUIRE(out4, "null argument");
-    out4[0] = f->n;
-    out4[1] = f->nnz;
-    out4[2] = f->nnz_sell;
-    out4[3] = f->maxrow;
+    CG_REQUIRE(out6, "null argument");
+    out6[0] = f->n;
+    out6[1] = f->nnz;
+    out6[2] = f->nnz_sell;
+    out6[3] = f->maxrow;
+    out6[4] = f->coded;
+    out6[5] = f->code_mismatch;
     return 0;
   });
 }

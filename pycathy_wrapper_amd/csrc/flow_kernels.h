@@ -17,21 +17,85 @@
 namespace cg {
 
 constexpr int kAsmBlock = 128;  // rows per assembly work-group (LDS row buffers)
-constexpr int kChains = 16;     // chains per work-group in the column kernels
+constexpr int kChains = 8;      // chains per work-group in the column kernels (one row per thread)
+constexpr int kUnroll = 16;     // row slots handled by the unrolled gather loops
 
+// Column ids come from one of two sources.  Explicit: col[slot].  Coded
+// (extruded meshes in the column ordering): row r = base_c + k nc_c + t
+// (colour c, node plane k, rank t; chain g = goff_c + t) and every slot's
+// column is (c', k + dk, t'), where the code (c', dk, t') depends only on
+// the chain g and the plane type pt (0 surface, 1 interior, 2 base):
+// gen3d.f repeats the same prism split in every layer, so all interior
+// planes share one pattern.  code[(pt * width + j) * nchain + g] packs
+// t' | (dk + 1) << 27 | c' << 29; clen / cdiag give the row length and the
+// diagonal slot.  The set-up builds the codes from the explicit pattern
+// and checks that they reproduce it exactly (else it keeps explicit ids).
 struct Sell {
   const int *so;    // slice offsets
-  const int *col;   // column id per slot
-  const int *len;   // row lengths
-  const int *diag;  // slot of the diagonal
+  const int *col;   // column id per slot (explicit source)
+  const int *len;   // row lengths (explicit source)
+  const int *diag;  // slot of the diagonal (explicit source)
+  const uint32_t *code;
+  const uint8_t *clen, *cdiag;
+  int coded, nchain, width, nplane, ncolr;
+  int cbase[4], cnc[4], cgoff[4];
 };
 
 __device__ __forceinline__ int sp(const int *so, int r, int j) { return so[r >> 6] + j * 64 + (r & 63); }
 
+// Position of a row in the column structure (coded source only).
+struct RowPos {
+  int k, g, pt;
+};
+
+template <typename T>
+__device__ __forceinline__ T pick4(int i, T a0, T a1, T a2, T a3) {
+  return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
+}
+
+__device__ __forceinline__ RowPos row_pos(const Sell &sl, int r) {
+  RowPos p{0, 0, 0};
+  if (!sl.coded) return p;
+  int c = 0;
+#pragma unroll
+  for (int i = 1; i < 4; ++i)
+    if (i < sl.ncolr && r >= sl.cbase[i]) c = i;
+  const int b = pick4(c, sl.cbase[0], sl.cbase[1], sl.cbase[2], sl.cbase[3]);
+  const int nc = pick4(c, sl.cnc[0], sl.cnc[1], sl.cnc[2], sl.cnc[3]);
+  const int off = r - b;
+  p.k = off / nc;
+  p.g = pick4(c, sl.cgoff[0], sl.cgoff[1], sl.cgoff[2], sl.cgoff[3]) + (off - p.k * nc);
+  p.pt = p.k == 0 ? 0 : (p.k == sl.nplane - 1 ? 2 : 1);
+  return p;
+}
+
+__device__ __forceinline__ RowPos chain_pos(const Sell &sl, int g, int k) {
+  RowPos p;
+  p.k = k;
+  p.g = g;
+  p.pt = k == 0 ? 0 : (k == sl.nplane - 1 ? 2 : 1);
+  return p;
+}
+
+// column of slot j (SELL position q) of the row at p / r
+__device__ __forceinline__ int sell_col(const Sell &sl, const RowPos &p, int q, int j) {
+  if (!sl.coded) return sl.col[q];
+  const uint32_t w = sl.code[(size_t)(p.pt * sl.width + j) * sl.nchain + p.g];
+  const int c2 = (int)(w >> 29), dk = (int)((w >> 27) & 3u) - 1, t2 = (int)(w & 0x7ffffffu);
+  return pick4(c2, sl.cbase[0], sl.cbase[1], sl.cbase[2], sl.cbase[3]) +
+         (p.k + dk) * pick4(c2, sl.cnc[0], sl.cnc[1], sl.cnc[2], sl.cnc[3]) + t2;
+}
+__device__ __forceinline__ int sell_len(const Sell &sl, const RowPos &p, int r) {
+  return sl.coded ? (int)sl.clen[p.pt * sl.nchain + p.g] : sl.len[r];
+}
+__device__ __forceinline__ int sell_diag(const Sell &sl, const RowPos &p, int r) {
+  return sl.coded ? (int)sl.cdiag[p.pt * sl.nchain + p.g] : sl.diag[r];
+}
+
 // One colour class of the column ordering: chains t < nc, chain t = rows
 // base + t + k*nc for k < len (k = node plane, 0 = surface).
 struct ColDesc {
-  int base, nc, len;
+  int base, nc, len, g0;  // g0: chain id of t = 0
 };
 
 // One level of the natural-order schedule: rows lvl_rows[off .. off+cnt).
@@ -149,6 +213,7 @@ __global__ void k_free_drainage(int nnod, const int *bot_row, const double *aren
 __global__ void k_store(int n, const double *sw, const double *volnod, const double *pnodi, double *part);
 __global__ void k_sum_final(int nb, const double *part, double *out);
 __global__ void k_gather_coef(int nslots, const int *map, const double *coef, double *A);
+__global__ void k_check_codes(int n, Sell sl, int *bad);
 __global__ void k_sumsq_free(int n, const double *u, const uint8_t *isdir, double *part);
 __global__ void k_copy(size_t n, const double *src, double *dst);
 __global__ void k_fill(size_t n, double v, double *dst);

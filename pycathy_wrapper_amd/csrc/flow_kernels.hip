@@ -96,7 +96,8 @@ template <int MR, bool LUMPED>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= a.n) return;
-  const int len = a.sl.len[r];
+  const RowPos rp = row_pos(a.sl, r);
+  const int len = sell_len(a.sl, rp, r);
   double K[MR];
   double M[LUMPED ? 1 : MR];
 #pragma unroll
@@ -127,13 +128,13 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a) {
     if (LUMPED) m2d = m2d + ve1 * 0.25;
     grav = grav + kr * a.gz[(size_t)e * 4 + k];
   }
-  const int dg = a.sl.diag[r];
+  const int dg = sell_diag(a.sl, rp, r);
   const int base = a.sl.so[r >> 6] + (r & 63);
   const double rdt = a.rdt;
   double t = 0.0;
 #pragma unroll
   for (int j = 0; j < MR; ++j)
-    if (j < len) t = t - K[j] * a.ptnew[a.sl.col[base + 64 * j]];
+    if (j < len) t = t - K[j] * a.ptnew[sell_col(a.sl, rp, base + 64 * j, j)];
   double m4r;
   if (LUMPED) {
     m4r = a.m4d[r];
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a) {
     for (int j = 0; j < MR; ++j) {
       if (j < len) {
         const int q = base + 64 * j;
-        const int c = a.sl.col[q];
+        const int c = sell_col(a.sl, rp, q, j);
         t = t - (M[j] * rdt) * (a.pnew[c] - a.ptimep[c]) - (a.m4full[q] * rdt) * (a.swnew[c] - a.swtimep[c]);
       }
     }
@@ -182,13 +183,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
                                                           const double *__restrict__ et1_node) {
   const int r = xcd_group() * kBlock + threadIdx.x;
   if (r >= a.n) return;
+  const RowPos rp = row_pos(a.sl, r);
   const int s = r >> 6, lane = r & 63;
   const int base = a.sl.so[s] + lane;
   const int w = (a.sl.so[s + 1] - a.sl.so[s]) >> 6;
   double m2d = 0.0, grav = 0.0;
   for (int j = 0; j < w; ++j) {
     const int q = base + 64 * j;
-    const int c = a.sl.col[q];
+    const int c = sell_col(a.sl, rp, q, j);
     m2d += P.w2[q] * et1_node[c];
     grav += P.gw[q] * ckrw_node[c];
   }
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
       for (int j = 0; j < MR; ++j) K[j] += (j == o) ? v[l] : 0.0;
     }
   }
-  const int len = a.sl.len[r], dg = a.sl.diag[r];
+  const int len = sell_len(a.sl, rp, r), dg = sell_diag(a.sl, rp, r);
   const double rdt = a.rdt, tetaf = a.tetaf;
   const double m4r = a.m4d[r];
   const double m5 = m4r * a.et2[r];
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
   for (int j = 0; j < MR; ++j) {
     if (j < len) {
       const int q = base + 64 * j;
-      t = t - K[j] * a.ptnew[a.sl.col[q]];
+      t = t - K[j] * a.ptnew[sell_col(a.sl, rp, q, j)];
       a.A[q] = tetaf * K[j] + (j == dg ? m2d * rdt + m5 * rdt : 0.0);
     }
   }
@@ -279,7 +281,8 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, Pair
     const int k = i / kChains, t = t0 + (i % kChains);
     if (t >= C.nc) continue;
     const int r = C.base + t + k * C.nc;
-    const int jd = sl.diag[r];
+    const RowPos rp = chain_pos(sl, C.g0 + t, k);
+    const int jd = sell_diag(sl, rp, r);
     const int jp = k > 0 ? jd - 1 : jd;
     const int base = sl.so[r >> 6] + (r & 63);
     Pl[i] = k > 0 ? base + 64 * jp : -1;
@@ -298,7 +301,7 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, Pair
     double s = A[base + 64 * jd];
     for (int j = 0; j < jp; ++j) {
       const int pj = base + 64 * j;
-      const int c = sl.col[pj];
+      const int c = sell_col(sl, rp, pj, j);
       double L = 0.0;
       if (!isdir[c]) L = (A[pj] - pair_dot(pl, F, pj)) * dinv[c];
       F[pj] = FT(L);
@@ -373,13 +376,28 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
     const int k = i / kChains, t = t0 + (i % kChains);
     if (t >= C.nc) continue;
     const int r = C.base + t + k * C.nc;
-    const int jd = sl.diag[r];
+    const RowPos rp = chain_pos(sl, C.g0 + t, k);
+    const int jd = sell_diag(sl, rp, r);
     const int jp = k > 0 ? jd - 1 : jd;
     const int base = sl.so[r >> 6] + (r & 63);
     double s = tin[r];
-    for (int j = 0; j < jp; ++j) {
-      const int pj = base + 64 * j;
-      s -= (double)F[pj] * y[sl.col[pj]];
+    if (jp <= kUnroll) {
+      int c[kUnroll];
+      float f[kUnroll];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j)
+        if (j < jp) {
+          f[j] = (float)F[base + 64 * j];
+          c[j] = sell_col(sl, rp, base + 64 * j, j);
+        }
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j)
+        if (j < jp) s -= (double)f[j] * y[c[j]];
+    } else {
+      for (int j = 0; j < jp; ++j) {
+        const int pj = base + 64 * j;
+        s -= (double)F[pj] * y[sell_col(sl, rp, pj, j)];
+      }
     }
     const double di = dinv[r];
     T[i] = s * di;
@@ -416,14 +434,29 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
     const int k = i / kChains, t = t0 + (i % kChains);
     if (t >= C.nc) continue;
     const int r = C.base + t + k * C.nc;
-    const int jd = sl.diag[r];
+    const RowPos rp = chain_pos(sl, C.g0 + t, k);
+    const int jd = sell_diag(sl, rp, r);
     const int js = k < kl ? jd + 1 : jd;
-    const int len = sl.len[r];
+    const int len = sell_len(sl, rp, r);
     const int base = sl.so[r >> 6] + (r & 63);
     double s = x[r];
-    for (int j = js + 1; j < len; ++j) {
-      const int pj = base + 64 * j;
-      s -= (double)F[pj] * x[sl.col[pj]];
+    if (len <= kUnroll) {
+      int c[kUnroll];
+      float f[kUnroll];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j)
+        if (j > js && j < len) {
+          f[j] = (float)F[base + 64 * j];
+          c[j] = sell_col(sl, rp, base + 64 * j, j);
+        }
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j)
+        if (j > js && j < len) s -= (double)f[j] * x[c[j]];
+    } else {
+      for (int j = js + 1; j < len; ++j) {
+        const int pj = base + 64 * j;
+        s -= (double)F[pj] * x[sell_col(sl, rp, pj, j)];
+      }
     }
     const double di = dinv[r];
     T[i] = s * di;
@@ -552,9 +585,27 @@ __device__ __forceinline__ double sell_row_dot(const Sell &sl, const double *__r
                                                const double *__restrict__ x, int r) {
   const int s = r >> 6, lane = r & 63;
   const int o0 = sl.so[s], w = (sl.so[s + 1] - o0) >> 6;
+  const RowPos rp = row_pos(sl, r);
+  const int q0 = o0 + lane;
   double acc = 0.0;
-  int q = o0 + lane;
-  for (int j = 0; j < w; ++j, q += 64) acc += A[q] * x[sl.col[q]];
+  if (w <= kUnroll) {
+    // all column ids and values first, then the gathers: the loads of a
+    // row are independent and in flight together
+    int c[kUnroll];
+    double a[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j)
+      if (j < w) {
+        a[j] = A[q0 + 64 * j];
+        c[j] = sell_col(sl, rp, q0 + 64 * j, j);
+      }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j)
+      if (j < w) acc += a[j] * x[c[j]];
+  } else {
+    int q = q0;
+    for (int j = 0; j < w; ++j, q += 64) acc += A[q] * x[sell_col(sl, rp, q, j)];
+  }
   return acc;
 }
 
@@ -613,14 +664,26 @@ __global__ void k_pcg_alpha(int n, int nparts, const double *__restrict__ part_p
     sc->pb = pb;
     sc->alpha = alfa;
   }
-  double acc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int stride = gridDim.x * blockDim.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {  // four rows in flight per thread
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int iu = i + u * stride;
+      const double ri = r[iu] - alfa * B[iu];
+      r[iu] = ri;
+      x[iu] = x[iu] + alfa * p[iu];
+      if (!isdir[iu]) acc[u] += ri * ri;
+    }
+  }
+  for (; i < n; i += stride) {
     const double ri = r[i] - alfa * B[i];
     r[i] = ri;
     x[i] = x[i] + alfa * p[i];
-    if (!isdir[i]) acc += ri * ri;
+    if (!isdir[i]) acc[0] += ri * ri;
   }
-  double t = block_sum<kBlock>(acc, sh);
+  double t = block_sum<kBlock>((acc[0] + acc[1]) + (acc[2] + acc[3]), sh);
   if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
 }
 
@@ -629,10 +692,15 @@ __global__ void k_dot(int n, const double *__restrict__ u, const double *__restr
                       double *__restrict__ part, const int *__restrict__ done) {
   if (done && *done) return;
   __shared__ double sh[kBlock / 64 + 1];
-  double acc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    acc += u[i] * v[i];
-  double t = block_sum<kBlock>(acc, sh);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int stride = gridDim.x * blockDim.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) acc[w] += u[i + w * stride] * v[i + w * stride];
+  }
+  for (; i < n; i += stride) acc[0] += u[i] * v[i];
+  double t = block_sum<kBlock>((acc[0] + acc[1]) + (acc[2] + acc[3]), sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
@@ -645,8 +713,13 @@ __global__ void k_pcg_beta(int n, int nparts, const double *__restrict__ part_bz
   const double c = reduce_partials<kBlock>(part_bz, nparts, sh);
   const double rr = reduce_partials<kBlock>(part_rr, nparts, sh);
   const double beta = -c / sc->pb;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    p[i] = z[i] + beta * p[i];
+  const int stride = gridDim.x * blockDim.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p[i + u * stride] = z[i + u * stride] + beta * p[i + u * stride];
+  }
+  for (; i < n; i += stride) p[i] = z[i] + beta * p[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const double xl = sc->xlung;
     const double err = xl > 0.0 ? sqrt(rr / xl) : sqrt(rr / n);
@@ -702,11 +775,12 @@ __global__ void k_bkpic(int nnod, const int *__restrict__ surf_row, Sell sl, con
   const int fa = ifatm[s];
   if (!(fa == 1 || fa == 2)) return;
   const int r = surf_row[s];
+  const RowPos rp = row_pos(sl, r);
   double v = 0.0;
-  const int len = sl.len[r];
+  const int len = sell_len(sl, rp, r);
   for (int j = 0; j < len; ++j) {
     const int q = sp(sl.so, r, j);
-    v += A[q] * pdiff[sl.col[q]];
+    v += A[q] * pdiff[sell_col(sl, rp, q, j)];
   }
   v = v - xt5[r];
   atmact[s] = (v - (1.0 - tetaf) * atmold[s]) * (1.0 / tetaf);
@@ -978,6 +1052,22 @@ __global__ void k_sum_final(int nb, const double *__restrict__ part, double *__r
   __shared__ double sh[kBlock / 64 + 2];
   const double v = reduce_partials<kBlock>(part, nb, sh);
   if (threadIdx.x == 0) *out = v;
+}
+
+// Set-up check of the coded column source: every slot of every row must
+// decode to the explicit column id (and the length / diagonal match).
+__global__ void k_check_codes(int n, Sell sl, int *__restrict__ bad) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const RowPos rp = row_pos(sl, r);
+  const int s = r >> 6, w = (sl.so[s + 1] - sl.so[s]) >> 6;
+  int nb = 0;
+  if (sell_len(sl, rp, r) != sl.len[r] || sell_diag(sl, rp, r) != sl.diag[r]) ++nb;
+  for (int j = 0; j < w; ++j) {
+    const int q = sl.so[s] + 64 * j + (r & 63);
+    if (sell_col(sl, rp, q, j) != sl.col[q]) ++nb;
+  }
+  if (nb) atomicAdd(bad, nb);
 }
 
 // A[q] = COEF1[map[q]] (upper CSR value of the slot's symmetric entry).

@@ -85,9 +85,17 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 // threads).  Used by consumers of a previous kernel's partial sums.
 template <int BS>
 __device__ __forceinline__ double reduce_partials(const double *part, int cnt, double *sh) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < cnt; i += BS) v += part[i];
-  v = block_sum<BS>(v, sh);
+  // four independent accumulators keep four loads in flight
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * BS < cnt; i += 4 * BS) {
+    v0 += part[i];
+    v1 += part[i + BS];
+    v2 += part[i + 2 * BS];
+    v3 += part[i + 3 * BS];
+  }
+  for (; i < cnt; i += BS) v0 += part[i];
+  double v = block_sum<BS>((v0 + v1) + (v2 + v3), sh);
   if (threadIdx.x == 0) sh[BS / 64] = v;
   __syncthreads();
   double r = sh[BS / 64];

@@ -158,9 +158,10 @@ class FlowGPU:
         return b.value
 
     def layout(self) -> dict:
-        out = np.zeros(4, dtype=np.int64)
+        out = np.zeros(6, dtype=np.int64)
         _check(lib().cathy_gpu_flow3d_layout(C.byref(self.h), _p(out)), "layout")
-        return dict(n=int(out[0]), nnz=int(out[1]), nnz_sell=int(out[2]), maxrow=int(out[3]))
+        return dict(n=int(out[0]), nnz=int(out[1]), nnz_sell=int(out[2]), maxrow=int(out[3]),
+                    coded=int(out[4]), code_mismatch=int(out[5]))
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:

@@ -156,6 +156,10 @@ def test_flow_heterogeneous_matches_oracle(gpu, oracle, ordering):
     pr = CF.hillslope_large(cells=40, nstr=15, params=CF.hillslope_small_params(tmax=172800.0))
     order = gpu.ORDER_NATURAL if ordering == "natural" else gpu.ORDER_COLUMN3
     g = gpu.FlowGPU(pr, order)
+    lay = g.layout()
+    # the column ordering decodes column ids from the extruded mesh's
+    # per-chain codes (checked against the explicit ids at set-up)
+    assert lay["coded"] == (1 if ordering == "column3" else 0) and lay["code_mismatch"] == 0
     run = _oracle_run(oracle, pr)
     scale = 1e-6 * pr.mesh.volnod.sum()
     try:
