@@ -1,0 +1,126 @@
+! cathy_gpu_mod.f90 -- ISO_C_BINDING interface of libcathy_gpu.so for the
+! CATHY Fortran host.  Every C entry point takes its arguments by reference
+! (include/cathy_gpu.h), so the interfaces below pass the host's own
+! INTEGER*4 / REAL*8 arrays unchanged, with no copies and no VALUE.
+module cathy_gpu
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private :: c_strlen
+
+  integer(c_int), parameter :: CATHY_GPU_OK = 0
+  integer(c_int), parameter :: CATHY_GPU_LSFAIL = 1
+  integer(c_int), parameter :: CATHY_GPU_ORDER_NATURAL = 0
+  integer(c_int), parameter :: CATHY_GPU_ORDER_COLUMN3 = 1
+
+  interface
+    type(c_ptr) function cathy_gpu_last_error() bind(C, name="cathy_gpu_last_error")
+      import :: c_ptr
+    end function
+
+    integer(c_int) function cathy_gpu_symslv(n, nterm, numdir, niter, itmxcg, tolcg, rmin, noddir, ja, &
+                                              topol, pnew, tnoti, coef1, err, res) &
+        bind(C, name="cathy_gpu_symslv")
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: n, nterm, numdir, itmxcg
+      integer(c_int), intent(out) :: niter
+      real(c_double), intent(in) :: tolcg, rmin
+      integer(c_int), intent(in) :: noddir(*), ja(*), topol(*)
+      real(c_double), intent(out) :: pnew(*)
+      real(c_double), intent(in) :: tnoti(*), coef1(*)
+      real(c_double), intent(out) :: err, res
+    end function
+
+    integer(c_int) function cathy_gpu_symslv_release() bind(C, name="cathy_gpu_symslv_release")
+      import :: c_int
+    end function
+
+    integer(c_int) function cathy_gpu_flow3d_create(handle, n, nt, ntri, nnod, nstr, nzone, maxstr, tetra, &
+                                                    bi, ci, di, volu, ivol, volnod, arenod, permx, permy, &
+                                                    permz, snodi, pnodi, vgn, vgrmc, vgpsat, kznod, lump, &
+                                                    ordering) bind(C, name="cathy_gpu_flow3d_create")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(out) :: handle
+      integer(c_int), intent(in) :: n, nt, ntri, nnod, nstr, nzone, maxstr, lump, ordering
+      integer(c_int), intent(in) :: tetra(5, *), ivol(*)
+      real(c_double), intent(in) :: bi(4, *), ci(4, *), di(4, *), volu(*), volnod(*), arenod(*)
+      real(c_double), intent(in) :: permx(maxstr, *), permy(maxstr, *), permz(maxstr, *)
+      real(c_double), intent(in) :: snodi(*), pnodi(*), vgn(*), vgrmc(*), vgpsat(*), kznod(*)
+    end function
+
+    integer(c_int) function cathy_gpu_flow3d_destroy(handle) bind(C, name="cathy_gpu_flow3d_destroy")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), intent(inout) :: handle
+    end function
+
+    integer(c_int) function cathy_gpu_flow3d_set_params(handle, iprm, dprm) &
+        bind(C, name="cathy_gpu_flow3d_set_params")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      integer(c_int), intent(in) :: iprm(9)
+      real(c_double), intent(in) :: dprm(16)
+    end function
+
+    integer(c_int) function cathy_gpu_flow3d_set_forcing(handle, natm, atmtim, atmval) &
+        bind(C, name="cathy_gpu_flow3d_set_forcing")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      integer(c_int), intent(in) :: natm
+      real(c_double), intent(in) :: atmtim(*), atmval(*)
+    end function
+
+    integer(c_int) function cathy_gpu_flow3d_init(handle, ptimep) bind(C, name="cathy_gpu_flow3d_init")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      real(c_double), intent(in) :: ptimep(*)
+    end function
+
+    integer(c_int) function cathy_gpu_flow3d_advance(handle, stats) bind(C, name="cathy_gpu_flow3d_advance")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      real(c_double), intent(out) :: stats(32)
+    end function
+
+    ! PNEW, SW, CKRW, IFATM: pass c_loc(array) or c_null_ptr
+    integer(c_int) function cathy_gpu_flow3d_get_state(handle, pnew, sw, ckrw, ifatm) &
+        bind(C, name="cathy_gpu_flow3d_get_state")
+      import :: c_int, c_int64_t, c_ptr
+      integer(c_int64_t), intent(in) :: handle
+      type(c_ptr), value :: pnew, sw, ckrw, ifatm
+    end function
+
+    integer(c_int) function cathy_gpu_flow3d_set_pnew(handle, pnew) bind(C, name="cathy_gpu_flow3d_set_pnew")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      real(c_double), intent(in) :: pnew(*)
+    end function
+
+    integer(c_int) function cathy_gpu_flow3d_counters(handle, out6) bind(C, name="cathy_gpu_flow3d_counters")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      real(c_double), intent(out) :: out6(6)
+    end function
+
+    integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+      import :: c_size_t, c_ptr
+      type(c_ptr), value :: s
+    end function
+  end interface
+
+contains
+
+  ! Text of the library's last error on this thread.
+  function cathy_gpu_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: i, l
+    p = cathy_gpu_last_error()
+    l = int(c_strlen(p))
+    call c_f_pointer(p, chars, [l])
+    allocate(character(len=l) :: msg)
+    do i = 1, l
+      msg(i:i) = chars(i)
+    end do
+  end function
+
+end module cathy_gpu

@@ -1,0 +1,49 @@
+"""Child-process driver for test_gpu_dropin.py (TEST INFRASTRUCTURE).
+
+Runs the hillslope_small day through the reference's own FLOW3D and time
+loop (tests/refrun.py) from oracle/_ref/libcathy_ref_gpu.so, where SYMSLV is
+the Fortran shim of pycathy_wrapper_amd/fortran calling cathy_gpu_symslv.
+Writes per-step statistics and psi / Sw at the golden fixture's saved steps
+to the .npz named by argv[1].  Runs in its own process because a library
+error ends a Fortran SYMSLV with STOP.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+for p in (ROOT, os.path.join(ROOT, "oracle"), HERE):
+    sys.path.insert(0, p)
+
+from pycathy_wrapper_amd import configs as CF  # noqa: E402
+import pyoracle as P  # noqa: E402
+import refrun as R  # noqa: E402
+
+
+def main(out: str, steps_saved: list[int]) -> None:
+    lib = os.path.join(ROOT, "oracle", "_ref", "libcathy_ref_gpu.so")
+    pr = CF.hillslope_small()
+    rr = R.RefFlowRun(P.RefLib(lib), pr.mesh, pr.soil, pr.psi0, float(pr.atm_vals[0]), pr.params.as_dict())
+    stats, psi, sw, keys = [], [], [], None
+    k = 0
+    while True:
+        rc, s = rr.step()
+        k += 1
+        keys = keys or list(s)
+        stats.append([s[nm] for nm in keys])
+        if k in steps_saved:
+            psi.append(rr.psi())
+            sw.append(rr.sw())
+        if k % 25 == 0:
+            print(f"step {k} t={s['TIME']:.1f}", file=sys.stderr, flush=True)
+        if rc:
+            break
+    np.savez(out, stat_names=np.array(keys), stats=np.array(stats), psi=np.array(psi), sw=np.array(sw), rc=rc)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1], [int(v) for v in sys.argv[2].split(",")])
