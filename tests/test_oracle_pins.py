@@ -177,3 +177,99 @@ def test_whole_run_bit_identical(oracle, ref, problem):
                 break
     finally:
         run.close()
+
+
+# ---------------------------------------------------- mesh set-up pins --
+IP3 = np.array([1, 2, 3, 2, 3, 1, 3, 1, 2], dtype=np.int32)            # cathy_main.f:2490
+IP4 = np.array([1, 2, 3, 4, 2, 3, 4, 1, 3, 4, 1, 2, 4, 1, 2, 3], dtype=np.int32)  # cathy_main.f:2491
+
+
+def _tet1(m):
+    t = np.ascontiguousarray(m.tetra, dtype=np.int32).copy()
+    t[:, :4] += 1
+    return t
+
+
+def test_volbas_bit_identical(oracle, ref, problem):
+    """volbas.f:10-54 (volums.f, basis6.f): basis coefficients, volumes,
+    nodal volumes -- the host builder and the oracle against the reference."""
+    m = problem.mesh
+    n, nt = m.n, m.nt
+    ai = np.zeros((nt, 4)); bi = np.zeros((nt, 4)); ci = np.zeros((nt, 4)); di = np.zeros((nt, 4))
+    volnod = np.zeros(n); volu = np.zeros(nt); volur = np.zeros(nt); ivol = np.zeros(nt, dtype=np.int32)
+    x, y, z = (np.ascontiguousarray(a) for a in (m.x, m.y, m.z))
+    ref.L.volbas_(I(n), I(nt), iptr(_tet1(m)), iptr(IP4), dptr(ai), dptr(bi), dptr(ci), dptr(di), dptr(volnod),
+                  dptr(volu), dptr(volur), iptr(ivol), dptr(x), dptr(y), dptr(z))
+    for got, want in ((bi, m.bi), (ci, m.ci), (di, m.di), (volu, m.volu), (volnod, m.volnod), (ivol, m.ivol)):
+        assert np.array_equal(got, want)
+    oa = oracle.volbas(m.tetra, x, y, z)
+    assert np.array_equal(oa[0], ai) and np.array_equal(oa[6], volur)
+
+
+def test_strpic_tetpic_bit_identical(oracle, ref, problem):
+    """strpic.f:8-104 / tetpic.f:10-31: the symmetric pattern (diagonal
+    first) and the element-to-entry map, against the host builder and the
+    oracle."""
+    from pycathy_wrapper_amd import mesh as M
+    m = problem.mesh
+    n, nt, n1 = m.n, m.nt, 25
+    ja = np.zeros(n1 * n, dtype=np.int32)
+    topol = np.zeros(n + 1, dtype=np.int32)
+    nterm = C.c_int(0)
+    tet = _tet1(m)
+    # IMAX: sentinel above every node id for strpic.f's vertex selection sort
+    ref.L.strpic_(I(n), C.byref(nterm), iptr(tet), iptr(ja), iptr(topol), I(nt), I(n1), I(2 ** 30))
+    nterm = nterm.value
+    htop, hja = M.sym_pattern(m.tetra, n)
+    assert nterm == len(hja) == 35301
+    assert np.array_equal(topol - 1, htop) and np.array_equal(ja[:nterm] - 1, hja)
+    tetja = np.zeros((nt, 4, 4), dtype=np.int32)  # Fortran TETJA(4,4,NT): [e][j][i] here
+    ref.L.tetpic_(I(nt), iptr(tet), iptr(ja), iptr(topol), iptr(tetja))
+    tj = M.tetja_sym(m.tetra, htop, hja, n)
+    for idx, (k, l) in enumerate(M.UPPER_PAIRS):
+        assert np.array_equal(tetja[:, l, k] - 1, tj[:, idx])
+    ot, oj, _ = oracle.pattern(n, m.tetra)
+    assert np.array_equal(ot, htop) and np.array_equal(oj, hja)
+
+
+def test_area2d_bit_identical(oracle, ref, problem):
+    """area2d.f / areas.f: a third of each incident triangle's area."""
+    m = problem.mesh
+    tri = np.ascontiguousarray(m.triang, dtype=np.int32).copy()
+    tri[:, :3] += 1
+    are = np.zeros(m.nnod)
+    x, y = np.ascontiguousarray(m.x[:m.nnod]), np.ascontiguousarray(m.y[:m.nnod])
+    ref.L.area2d_(I(m.nnod), I(m.ntri), iptr(tri), iptr(IP3), dptr(are), dptr(x), dptr(y))
+    assert np.array_equal(are, m.arenod)
+    assert np.array_equal(oracle.area2d(m.nnod, m.triang, x, y), are)
+
+
+def test_strnew_bit_identical(oracle, ref, problem):
+    """strnew.f: the full pattern (all columns of a row, ascending) the
+    Newton path and the device's SELL layout are built from."""
+    from pycathy_wrapper_amd import mesh as M
+    m = problem.mesh
+    n, nt, n1 = m.n, m.nt, 40
+    ja = np.zeros(n1 * n, dtype=np.int32)
+    topol = np.zeros(n + 1, dtype=np.int32)
+    nterm = C.c_int(0)
+    ref.L.strnew_(I(n), C.byref(nterm), iptr(_tet1(m)), iptr(ja), iptr(topol), I(nt), I(n1), I(2 ** 30))
+    ft, fj = M.full_pattern(m.tetra, n)
+    assert nterm.value == len(fj) == 65751
+    assert np.array_equal(topol - 1, ft) and np.array_equal(ja[:nterm.value] - 1, fj)
+    ot, oj, _ = oracle.pattern(n, m.tetra, full=True)
+    assert np.array_equal(ot, ft) and np.array_equal(oj, fj)
+
+
+def test_icvdwt_bit_identical(oracle, ref, problem):
+    """icvdwt.f:8-31, the hydrostatic initial head (INDP=4)."""
+    from pycathy_wrapper_amd import configs as CFG
+    m = problem.mesh
+    p = np.zeros(m.n)
+    pond = np.zeros(m.nnod)
+    z = np.ascontiguousarray(m.z)
+    ref.L.icvdwt_(I(m.nnod), I(m.nstr), I(m.n), I(0), I(0), D(1.0), dptr(p), dptr(pond), dptr(z))
+    assert np.array_equal(CFG.icvdwt(m, 1.0), p)
+    q = np.zeros(m.n)
+    oracle.L.orc_icvdwt(m.nnod, m.nstr, C.c_double(1.0), dptr(q), dptr(z))
+    assert np.array_equal(q, p)
