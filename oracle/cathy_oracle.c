@@ -990,6 +990,471 @@ void orc_neumann_free(int nnod, const int *acontq, const double *arenod,
 /* ===================================================================== */
 /*                            whole run driver                           */
 /* ===================================================================== */
+/*              Newton linearisation (IOPT=2) and BiCGSTAB (a16)         */
+/* ===================================================================== */
+
+double orc_fvgdds(double psi, const double *vg, int n, int i) {
+  /* SRC/fvgdds.f: second derivative of Se. */
+  if (psi < -1.0e-14) {
+    double vgn = VGF(vg, n, VG_N, i);
+    double beta = pow(fabs(psi / VGF(vg, n, VG_PSAT, i)), vgn);
+    double b1 = beta + 1.0;
+    double b1r = 1.0 / b1;
+    return VGF(vg, n, VG_N1, i) * (beta / psi) * (1.0 / psi) *
+           ((1.0 + vgn * (beta - 1.0)) / pow(fabs(b1), VGF(vg, n, VG_M, i))) *
+           b1r * b1r;
+  }
+  return 0.0;
+}
+
+double orc_fvgdkr(double psi, const double *vg, int n, int i) {
+  /* SRC/fvgdkr.f: d kr / d psi. */
+  if (psi < -1.0e-14) {
+    double psat = VGF(vg, n, VG_PSAT, i), m = VGF(vg, n, VG_M, i);
+    double beta = pow(fabs(psi / psat), VGF(vg, n, VG_N, i));
+    double b1 = beta + 1.0;
+    double v1 = pow(fabs(b1), m) - pow(fabs(beta), m);
+    double v2 = psat / psi;
+    double v3 = VGF(vg, n, VG_N1, i) * beta * v2 *
+                pow(fabs(1.0 / b1), VGF(vg, n, VG_M52, i)) / psat;
+    double v4 = v2 * ((2.5 / b1) * beta - 2.0) -
+                0.5 * pow(fabs(b1), VGF(vg, n, VG_MM1, i));
+    return v3 * v1 * v4;
+  }
+  return 0.0;
+}
+
+void orc_chnew0(int n, const double *ptnew, const double *snodi,
+                const double *pnodi, const double *vg, double *sw,
+                double *ckrw, double *etai, double *dckrw, double *detai) {
+  /* SRC/chnew0.f (IVGHU=0 branch). */
+  for (int i = 0; i < n; ++i) {
+    double psi = ptnew[i];
+    double pnot = VGF(vg, n, VG_PNOT, i);
+    double se = orc_fvgse(psi, vg, n, i);
+    double dsedp = orc_fvgdse(psi, vg, n, i);
+    double dswdp = pnot * dsedp;
+    sw[i] = pnot * se + VGF(vg, n, VG_RMC, i) / pnodi[i];
+    etai[i] = sw[i] * snodi[i] + pnodi[i] * dswdp;
+    detai[i] = dswdp * snodi[i] + pnodi[i] * pnot * orc_fvgdds(psi, vg, n, i);
+    ckrw[i] = orc_fvgkr(psi, se, vg, n, i);
+    dckrw[i] = orc_fvgdkr(psi, vg, n, i);
+  }
+}
+
+void orc_assnew(int nt, int ntri, int nzone, const int *tetra,
+                const int *tetja, const double *lmass, double *coef1,
+                double *coef2, double *coef3, const double *permx,
+                const double *permy, const double *permz, const double *ckrwe,
+                const double *etae, const double *ptnew, const double *ptimep,
+                const double *pnew, const double *dckrw, const double *detai,
+                double tetaf, double deltat, const double *volu,
+                const double *volur, const int *ivol, const double *bi,
+                const double *ci, const double *di) {
+  /* SRC/assnew.f: stiffness (COEF1), storage (COEF2) and the Newton
+   * derivative terms (COEF3) on the full pattern; tetja[e*16+k*4+l] is the
+   * position of (TETRA(k), TETRA(l)) in row TETRA(k). */
+  double rdt = 1.0 / deltat;
+  for (int e = 0; e < nt; ++e) {
+    int istr = e / (3 * ntri);
+    int mt = tetra[(size_t)e * 5 + 4] - 1;
+    size_t ix = (size_t)istr * nzone + mt;
+    const int *t = tetra + (size_t)e * 5;
+    const int *tj = tetja + (size_t)e * 16;
+    const double *b = bi + (size_t)e * 4, *c = ci + (size_t)e * 4,
+                 *d = di + (size_t)e * 4;
+    double ve = volu[e] * etae[e];
+    double pv1 = permx[ix] * volur[e], pv2 = permy[ix] * volur[e],
+           pv3 = permz[ix] * volur[e];
+    double pvk1 = pv1 * ckrwe[e], pvk2 = pv2 * ckrwe[e], pvk3 = pv3 * ckrwe[e];
+    double tpi = tetaf * permz[ix] * (double)ivol[e];
+    double tvd = tetaf * volu[e] * rdt;
+    for (int k = 0; k < 4; ++k) {
+      int knod = t[k];
+      double pvb = pv1 * b[k], pvc = pv2 * c[k], pvd = pv3 * d[k];
+      double pvkb = pvk1 * b[k], pvkc = pvk2 * c[k], pvkd = pvk3 * d[k];
+      double sum = 0.0, sum1 = 0.0;
+      for (int m = 0; m < 4; ++m) {
+        int mnod = t[m];
+        sum = sum + (pvb * b[m] + pvc * c[m] + pvd * d[m]) * ptnew[mnod];
+        sum1 = sum1 + lmass[k * 4 + m] * (pnew[mnod] - ptimep[mnod]);
+      }
+      double tsumtd = tetaf * sum + tpi * d[k];
+      double sum1tv = sum1 * tvd;
+      for (int l = 0; l < 4; ++l) {
+        int lnod = t[l];
+        int ind = tj[k * 4 + l];
+        if (lnod >= knod) {
+          coef1[ind] = coef1[ind] + pvkb * b[l] + pvkc * c[l] + pvkd * d[l];
+          coef2[ind] = coef2[ind] + ve * lmass[k * 4 + l];
+        }
+        coef3[ind] = coef3[ind] + dckrw[lnod] * tsumtd + detai[lnod] * sum1tv;
+      }
+    }
+    for (int k = 0; k < 4; ++k)
+      for (int l = 0; l < 4; ++l)
+        if (t[l] < t[k]) {
+          coef1[tj[k * 4 + l]] = coef1[tj[l * 4 + k]];
+          coef2[tj[k * 4 + l]] = coef2[tj[l * 4 + k]];
+        }
+  }
+}
+
+void orc_rhsnew(int n, const int *ja, const int *topol, double deltat,
+                const double *ptnew, const double *pnew, const double *ptimep,
+                const double *coef1, const double *coef2, double *tnoti) {
+  /* SRC/rhsnew.f: row sums over the full pattern. */
+  double rdt = 1.0 / deltat;
+  for (int k = 0; k < n; ++k) {
+    tnoti[k] = 0.0;
+    for (int m = topol[k]; m < topol[k + 1]; ++m) {
+      int c = ja[m];
+      tnoti[k] = tnoti[k] - coef1[m] * ptnew[c] -
+                 coef2[m] * (pnew[c] - ptimep[c]) * rdt;
+    }
+  }
+}
+
+void orc_cfmatn(int nterm, double tetaf, double deltat, double *coef1,
+                const double *coef2, const double *coef3) {
+  /* SRC/cfmatn.f. */
+  double rdt = 1.0 / deltat;
+  for (int k = 0; k < nterm; ++k)
+    coef1[k] = tetaf * coef1[k] + coef2[k] * rdt + coef3[k];
+}
+
+static int diag_pos(const int *topol, const int *ja, int j) {
+  int ind = topol[j];
+  while (ja[ind] != j) ++ind;
+  return ind;
+}
+
+int orc_bcnew(int nq, const int *contq, const double *q, int nnod,
+              double *lhsatm, const int *ifatm, const double *atmact,
+              const double *atmold, const int *ja, const int *topol,
+              double *coef1, double *tnoti, double tetaf, double rmax,
+              int *noddir) {
+  /* SRC/bcnew.f with NP=0, NSF=0: penalty diagonals on the atmospheric
+   * Dirichlet nodes, Neumann and atmospheric fluxes. */
+  int numdir = 0;
+  for (int k = 0; k < nnod; ++k)
+    if (ifatm[k] == 1 || ifatm[k] == 2) {
+      noddir[numdir++] = k;
+      tnoti[k] = 0.0;
+      int ind = diag_pos(topol, ja, k);
+      lhsatm[k] = coef1[ind];
+      coef1[ind] = 1.0e-9 * rmax;
+    }
+  for (int k = 0; k < nq; ++k) tnoti[contq[k]] = tnoti[contq[k]] + q[k];
+  for (int k = 0; k < nnod; ++k)
+    if (ifatm[k] == 0)
+      tnoti[k] = tnoti[k] + (tetaf * atmact[k] + (1.0 - tetaf) * atmold[k]);
+  return numdir;
+}
+
+void orc_shlnew(const int *topol, const int *ja, double *coef1, double *pnew,
+                const double *pold, int nnod, const int *ifatm,
+                const double *lhsatm) {
+  /* SRC/shlnew.f with NP=0, NSF=0. */
+  for (int i = 0; i < nnod; ++i)
+    if (ifatm[i] == 1 || ifatm[i] == 2) {
+      pnew[i] = pold[i];
+      coef1[diag_pos(topol, ja, i)] = lhsatm[i];
+    }
+}
+
+void orc_bknew(int nnod, double tetaf, const int *topol, const int *ja,
+               const int *ifatm, const double *pdiff, double *atmact,
+               const double *atmold, const double *coef1, const double *xt5) {
+  /* SRC/bknew.f with NP=0, NSF=0: fluxes at the atmospheric Dirichlet
+   * nodes from the (restored) Newton matrix. */
+  double tetafr = 1.0 / tetaf;
+  for (int k = 0; k < nnod; ++k)
+    if (ifatm[k] == 1 || ifatm[k] == 2) {
+      double v = 0.0;
+      for (int m = topol[k]; m < topol[k + 1]; ++m) v = v + coef1[m] * pdiff[ja[m]];
+      v = v - xt5[k];
+      atmact[k] = (v - (1.0 - tetaf) * atmold[k]) * tetafr;
+    }
+}
+
+void orc_topia(int n, const int *topol, int *ia) {
+  /* SRC/topia.f: row index of every entry. */
+  for (int k = 0; k < n; ++k)
+    for (int m = topol[k]; m < topol[k + 1]; ++m) ia[m] = k;
+}
+
+int orc_inclu(const int *ia, const int *ja, const int *topol,
+              const double *coefr, double *coefc, double *lriga,
+              double *lcolon, int n, int nterm) {
+  /* SRC/solscal-extended.f:2194-2270 INCLU: Crout ILU(0) on the full
+   * (structurally symmetric, column-sorted) pattern.  On return LRIGA
+   * holds, row by row, L (lower slots and the diagonal) and the unit upper
+   * factor U (upper slots); LCOLON holds the transposes.  COEFC receives
+   * A^T in A's slots.  Row r is computed after its transposes were copied
+   * in from the rows above: L(c,r) = A(c,r) - sum_m L(c,m) U(m,r) into
+   * LCOLON(r,c) and U(r,c) = (A(r,c) - sum_m L(r,m) U(m,c)) / L(r,r) into
+   * LRIGA(r,c), sums over common columns m < r in ascending order.  A zero
+   * pivot takes the previous row's.  Returns the number of zero pivots. */
+  (void)n;
+  int nfall = 0;
+  for (int k = 0; k < nterm; ++k) {
+    int i = topol[ja[k]];
+    while (ja[i] < ia[k]) ++i;
+    coefc[i] = coefr[k];
+  }
+  int k = 0;
+  do { /* first row: column of L and row of U directly */
+    lcolon[k] = coefc[k];
+    lriga[k] = coefr[k] / coefr[0];
+    if (k == 0) lriga[0] = coefr[0];
+    ++k;
+  } while (k < nterm && ia[k] == ia[k - 1]);
+  while (k < nterm) {
+    /* transposes of the previous row's upper entries into later rows */
+    for (int i = k - 1; ja[i] > ia[i]; --i) {
+      int j = topol[ja[i]];
+      while (ja[j] < ia[i]) ++j;
+      lriga[j] = lcolon[i];
+      lcolon[j] = lriga[i];
+    }
+    while (ia[k] > ja[k]) ++k; /* to the diagonal of the new row */
+    int kk = k;
+    int r = ia[k];
+    for (;;) {
+      int j = topol[ja[k]];
+      int i = topol[r];
+      double a = 0.0, b = 0.0;
+      while (!(ja[i] >= r || ja[j] >= r)) {
+        if (ja[i] < ja[j]) {
+          ++i;
+        } else if (ja[i] == ja[j]) {
+          a = a + lcolon[i] * lriga[j];
+          if (i != j) b = b + lriga[i] * lcolon[j];
+          ++i;
+          ++j;
+          if (i > nterm - 1) break;
+        } else {
+          ++j;
+        }
+      }
+      lcolon[k] = coefc[k] - a;
+      if (k == kk && lcolon[k] == 0.0) {
+        int kkk = kk;
+        do --kkk; while (ia[kkk] != ja[kkk]);
+        lcolon[k] = lcolon[kkk];
+        ++nfall;
+      }
+      if (ia[k] < ja[k]) lriga[k] = (coefr[k] - b) / lriga[kk];
+      if (ia[k] == ja[k]) lriga[k] = lcolon[k];
+      ++k;
+      if (k >= nterm) return nfall;
+      if (ia[k] != ia[k - 1]) break;
+    }
+  }
+  return nfall;
+}
+
+void orc_prodl(double *z, const double *y, const int *ia, const int *ja,
+               const double *coef2, int nterm) {
+  /* SRC/solscal-extended.f:4205-4227 PRODL: L z = y. */
+  z[0] = y[0] / coef2[0];
+  double a = 0.0;
+  for (int k = 1; k < nterm; ++k) {
+    if (ja[k] < ia[k]) {
+      a = a + coef2[k] * z[ja[k]];
+    } else if (ja[k] == ia[k]) {
+      z[ia[k]] = (y[ia[k]] - a) / coef2[k];
+      a = 0.0;
+    }
+  }
+}
+
+void orc_produ(double *x, const double *h, int n, const int *ia,
+               const int *ja, const double *coef2, const int *topol) {
+  /* SRC/solscal-extended.f:4228-4250 PRODU: U x = h (unit diagonal). */
+  x[n - 1] = h[n - 1];
+  double a = 0.0;
+  for (int i = topol[n - 1] - 1; i >= 0; --i) {
+    if (ia[i] < ja[i]) {
+      a = a + coef2[i] * x[ja[i]];
+    } else if (ia[i] == ja[i]) {
+      x[ia[i]] = h[ia[i]] - a;
+      a = 0.0;
+    }
+  }
+}
+
+void orc_pru(double *y, const double *x, const int *ia, const int *ja,
+             const double *coef2, int nterm) {
+  /* SRC/solscal-extended.f:4251-4264 PRU: y = U x. */
+  for (int k = 0; k < nterm; ++k) {
+    if (ia[k] > ja[k]) continue;
+    if (ia[k] == ja[k]) {
+      y[ia[k]] = x[ia[k]];
+      continue;
+    }
+    y[ia[k]] = y[ia[k]] + coef2[k] * x[ja[k]];
+  }
+}
+
+void orc_prod1(double *x, const double *b, int n, const int *ia,
+               const int *ja, const double *coef2, int nterm,
+               const int *topol) {
+  /* SRC/solscal-extended.f:4107-4141 PROD1: x = (LU)^-1 b. */
+  orc_prodl(x, b, ia, ja, coef2, nterm);
+  double a = 0.0;
+  for (int i = topol[n - 1] - 1; i >= 0; --i) {
+    if (ia[i] < ja[i]) {
+      a = a + coef2[i] * x[ja[i]];
+    } else if (ia[i] == ja[i]) {
+      x[ia[i]] = x[ia[i]] - a;
+      a = 0.0;
+    }
+  }
+}
+
+static void aperbn(int n, const int *topol, const int *ja, const double *coef,
+                   const double *v, double *r) {
+  /* SRC/solscal-extended.f:97-107 APERBN: r = A v. */
+  for (int k = 0; k < n; ++k) {
+    r[k] = 0.0;
+    for (int i = topol[k]; i < topol[k + 1]; ++i) r[k] = r[k] + coef[i] * v[ja[i]];
+  }
+}
+
+static void prodbh(double *z, const double *h, int n, const int *ia,
+                   const int *ja, const double *coef1, const double *coef2,
+                   int nterm, const int *topol, double *x, double *y) {
+  /* SRC/solscal-extended.f:4142-4168 PRODBH: z = L^-1 A U^-1 h. */
+  orc_produ(x, h, n, ia, ja, coef2, topol);
+  for (int k = 0; k < n; ++k) {
+    z[k] = 0.0;
+    y[k] = 0.0;
+  }
+  for (int k = 0; k < nterm; ++k) y[ia[k]] = y[ia[k]] + coef1[k] * x[ja[k]];
+  orc_prodl(z, y, ia, ja, coef2, nterm);
+}
+
+double orc_resnsy(const int *topol, const int *ja, const double *coef1,
+                  const double *tnoti, const double *pnew, double *resv,
+                  int n, int numdir, const int *noddir, double rmin) {
+  /* SRC/solscal-extended.f:4332-4388 RESNSY. */
+  for (int k = 0; k < n; ++k) resv[k] = tnoti[k];
+  for (int k = 0; k < numdir; ++k) resv[noddir[k]] = 0.0;
+  double xlung = 0.0;
+  for (int k = 0; k < n; ++k) xlung = xlung + resv[k] * resv[k];
+  for (int k = 0; k < n; ++k)
+    for (int i = topol[k]; i < topol[k + 1]; ++i)
+      resv[k] = resv[k] - coef1[i] * pnew[ja[i]];
+  for (int k = 0; k < numdir; ++k) resv[noddir[k]] = 0.0;
+  double res = 0.0;
+  for (int k = 0; k < n; ++k) res = res + resv[k] * resv[k];
+  return xlung <= rmin ? sqrt(res / n) : sqrt(res / xlung);
+}
+
+int orc_gcstas(int n, double *poten, const double *tnoti, double *scr,
+               const int *topol, const int *ia, const int *ja,
+               const double *coef1, const double *coef2, int nterm, int imax,
+               double tol, int np, const int *contp, double *err_out) {
+  /* SRC/solscal-extended.f:1010-1128 GCSTAS: BiCGSTAB on L^-1 A U^-1 with
+   * y = U x, from x0 = poten; the relative error is that of the
+   * preconditioned residual with the Dirichlet entries zeroed, over
+   * ||b_free||.  scr: 10 n-vectors. */
+  double *pjm = scr, *vjm = scr + n, *p = scr + 2 * (size_t)n,
+         *v = scr + 3 * (size_t)n, *s = scr + 4 * (size_t)n,
+         *t = scr + 5 * (size_t)n, *r = scr + 6 * (size_t)n,
+         *rt = scr + 7 * (size_t)n, *scr1 = scr + 8 * (size_t)n,
+         *scr2 = scr + 9 * (size_t)n;
+  int niter = 1;
+  double xlung = 0.0, err = 0.0;
+  for (int i = 0; i < n; ++i) pjm[i] = tnoti[i];
+  for (int i = 0; i < np; ++i) pjm[contp[i]] = 0.0;
+  for (int i = 0; i < n; ++i) xlung = xlung + pjm[i] * pjm[i];
+  aperbn(n, topol, ja, coef1, poten, scr1);
+  for (int k = 0; k < n; ++k) scr1[k] = tnoti[k] - scr1[k];
+  orc_prodl(r, scr1, ia, ja, coef2, nterm);
+  for (int k = 0; k < n; ++k) rt[k] = r[k];
+  orc_pru(scr1, poten, ia, ja, coef2, nterm);
+  for (int k = 0; k < n; ++k) poten[k] = scr1[k];
+  for (int i = 0; i < n; ++i) {
+    pjm[i] = 0.0;
+    vjm[i] = 0.0;
+  }
+  double rojm1 = 1.0, alfjm1 = 1.0, etaj = 1.0;
+  for (;;) {
+    double roj = 0.0;
+    for (int i = 0; i < n; ++i) roj = roj + rt[i] * r[i];
+    double beta = (roj / rojm1) * (alfjm1 / etaj);
+    for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (pjm[i] - etaj * vjm[i]);
+    prodbh(v, p, n, ia, ja, coef1, coef2, nterm, topol, scr1, scr2);
+    double sigma = 0.0;
+    for (int i = 0; i < n; ++i) sigma = sigma + rt[i] * v[i];
+    double alfj = roj / sigma;
+    for (int i = 0; i < n; ++i) s[i] = r[i] - alfj * v[i];
+    prodbh(t, s, n, ia, ja, coef1, coef2, nterm, topol, scr1, scr2);
+    double aa = 0.0, bb = 0.0;
+    for (int i = 0; i < n; ++i) {
+      aa = aa + s[i] * t[i];
+      bb = bb + t[i] * t[i];
+    }
+    etaj = aa / bb;
+    for (int i = 0; i < n; ++i) {
+      poten[i] = poten[i] + alfj * p[i] + etaj * s[i];
+      r[i] = s[i] - etaj * t[i];
+    }
+    for (int i = 0; i < np; ++i) r[contp[i]] = 0.0;
+    aa = 0.0;
+    for (int i = 0; i < n; ++i) aa = aa + r[i] * r[i];
+    err = xlung > 0.0 ? sqrt(aa / xlung) : sqrt(aa / n);
+    if (niter <= imax && err > tol) {
+      niter = niter + 1;
+      rojm1 = roj;
+      alfjm1 = alfj;
+      for (int i = 0; i < n; ++i) {
+        pjm[i] = p[i];
+        vjm[i] = v[i];
+      }
+      continue;
+    }
+    break;
+  }
+  for (int i = 0; i < n; ++i) {
+    r[i] = poten[i];
+    poten[i] = 0.0;
+  }
+  orc_produ(poten, r, n, ia, ja, coef2, topol);
+  *err_out = err;
+  return niter;
+}
+
+int orc_nsyslv(int n, int nterm, int numdir, const int *noddir, int itmxcg,
+               double tolcg, double rmin, const int *ia, const int *ja,
+               const int *topol, const double *coef1, double *coef2,
+               double *coef3, double *scr, double *pnew, const double *tnoti,
+               double *err, double *res, double *resini) {
+  /* SRC/solscal-extended.f:3063-3174 NSYSLV, ISOLV=0: INCLU, x0 = (LU)^-1 b
+   * (PROD1), GCSTAS, RESNSY.  scr: max(nterm, 11 n) doubles (LCOLON, then
+   * the BiCGSTAB vectors).  coef3 is overwritten (A^T).  Returns NITER. */
+  double rhsn = 0.0;
+  for (int i = 0; i < n; ++i) rhsn = rhsn + tnoti[i] * tnoti[i];
+  if (rhsn == 0.0) {
+    for (int i = 0; i < n; ++i) pnew[i] = 0.0;
+    *err = 0.0;
+    *res = 0.0;
+    *resini = 0.0;
+    return 0;
+  }
+  orc_inclu(ia, ja, topol, coef1, coef3, coef2, scr, n, nterm);
+  orc_prod1(pnew, tnoti, n, ia, ja, coef2, nterm, topol);
+  *resini = orc_resnsy(topol, ja, coef1, tnoti, pnew, scr, n, numdir, noddir, rmin);
+  int niter = orc_gcstas(n, pnew, tnoti, scr, topol, ia, ja, coef1, coef2, nterm,
+                         itmxcg, tolcg, numdir, noddir, err);
+  *res = orc_resnsy(topol, ja, coef1, tnoti, pnew, scr, n, numdir, noddir, rmin);
+  return niter;
+}
+
+/* ===================================================================== */
 
 struct orc_run {
   orc_params p;
@@ -1004,6 +1469,9 @@ struct orc_run {
   double *sw, *ckrw, *ckrwp, *etai, *et1, *et2, *swnew, *swtimep;
   double *ckrwe, *etae, *et1e;
   double *coef1, *coef2, *coef4, *coef5, *tnoti, *xt5, *scr, *qtranie;
+  /* Newton (IOPT=2): full pattern row ids, derivative terms */
+  int *ia;
+  double *coef3, *dckrw, *detai;
   /* boundary conditions */
   int nq;
   int *contq;
@@ -1208,13 +1676,21 @@ orc_run *orc_run_create(int nnod, int nstr, int ntri, const int *tetra,
   orc_volbas(n, nt, r->tetra, r->x, r->y, r->z, r->ai, r->bi, r->ci, r->di,
              r->volnod, r->volu, r->volur, r->ivol);
   orc_locmas(r->lmass, prm->lump);
-  /* pattern: SRC/cathy_main.f:2599-2601 STRPIC + TETPIC */
-  int cap = 16 * n;
+  /* pattern: SRC/cathy_main.f:2599-2601 STRPIC + TETPIC for IOPT=1,
+   * STRNEW + TETNEW + TOPIA for IOPT=2 */
+  int cap = 32 * n;
   r->topol = IALLOC(n + 1);
   r->ja = IALLOC(cap);
-  r->nterm = orc_strpic(n, nt, r->tetra, r->topol, r->ja, cap);
   r->tetja = IALLOC(16 * (size_t)nt);
-  orc_tetpic(nt, r->tetra, r->topol, r->ja, r->tetja);
+  if (prm->iopt == 2) {
+    r->nterm = orc_strnew(n, nt, r->tetra, r->topol, r->ja, cap);
+    orc_tetnew(nt, r->tetra, r->topol, r->ja, r->tetja);
+    r->ia = IALLOC(r->nterm);
+    orc_topia(n, r->topol, r->ia);
+  } else {
+    r->nterm = orc_strpic(n, nt, r->tetra, r->topol, r->ja, cap);
+    orc_tetpic(nt, r->tetra, r->topol, r->ja, r->tetja);
+  }
   /* soil: SRC/inital.f -> TPNODI, CHPARM */
   r->pnodi = DALLOC(n); r->snodi = DALLOC(n);
   r->vg = DALLOC((size_t)VG_NFIELD * n); r->pel = DALLOC(nt);
@@ -1231,7 +1707,14 @@ orc_run *orc_run_create(int nnod, int nstr, int ntri, const int *tetra,
   r->ckrwe = DALLOC(nt); r->etae = DALLOC(nt); r->et1e = DALLOC(nt);
   r->coef1 = DALLOC(r->nterm); r->coef2 = DALLOC(r->nterm);
   r->coef4 = DALLOC(r->nterm); r->coef5 = DALLOC(r->nterm);
-  r->tnoti = DALLOC(n); r->xt5 = DALLOC(n); r->scr = DALLOC(5 * (size_t)n);
+  r->tnoti = DALLOC(n); r->xt5 = DALLOC(n);
+  r->scr = DALLOC(prm->iopt == 2 ? (r->nterm > 10 * n ? (size_t)r->nterm : 10 * (size_t)n)
+                                 : 5 * (size_t)n);
+  if (prm->iopt == 2) {
+    r->coef3 = DALLOC(r->nterm);
+    r->dckrw = DALLOC(n);
+    r->detai = DALLOC(n);
+  }
   r->qtranie = DALLOC(n);
   r->ifatm = IALLOC(nnod); r->ifatmp = IALLOC(nnod);
   r->noddir = IALLOC(nnod + 1);
@@ -1313,7 +1796,8 @@ void orc_run_free(orc_run *r) {
                   r->coef1, r->coef2, r->coef4, r->coef5, r->tnoti, r->xt5,
                   r->scr, r->qtranie, r->contq, r->q, r->ifatm, r->ifatmp,
                   r->noddir, r->atmpot, r->atmact, r->atmold, r->lhsatm,
-                  r->atm_times, r->atm_vals, r->atminp};
+                  r->atm_times, r->atm_vals, r->atminp, r->ia, r->coef3,
+                  r->dckrw, r->detai};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
   free(r);
 }
@@ -1366,13 +1850,71 @@ static void picard(orc_run *r, double *err_out) {
   if (err_out) *err_out = err;
 }
 
+/* One Newton iteration: SRC/newton.f (IVGHU=0, KSLOPE=0, ISOLV=0, NP=0,
+ * NSF=0). */
+static void newton(orc_run *r, double *err_out) {
+  int n = r->n, nt = r->nt;
+  orc_chnew0(n, r->ptnew, r->snodi, r->pnodi, r->vg, r->sw, r->ckrw, r->etai,
+             r->dckrw, r->detai);
+  orc_nodelt(nt, r->tetra, r->ckrw, r->ckrwe);
+  orc_nodelt(nt, r->tetra, r->etai, r->etae);
+  memset(r->coef1, 0, sizeof(double) * r->nterm);
+  memset(r->coef2, 0, sizeof(double) * r->nterm);
+  memset(r->coef3, 0, sizeof(double) * r->nterm);
+  orc_assnew(nt, r->ntri, r->nzone, r->tetra, r->tetja, r->lmass, r->coef1,
+             r->coef2, r->coef3, r->permx, r->permy, r->permz, r->ckrwe,
+             r->etae, r->ptnew, r->ptimep, r->pnew, r->dckrw, r->detai,
+             r->p.tetaf, r->deltat, r->volu, r->volur, r->ivol, r->bi, r->ci,
+             r->di);
+  orc_rhsnew(n, r->ja, r->topol, r->deltat, r->ptnew, r->pnew, r->ptimep,
+             r->coef1, r->coef2, r->tnoti);
+  orc_cfmatn(r->nterm, r->p.tetaf, r->deltat, r->coef1, r->coef2, r->coef3);
+  orc_rhsgrv(nt, r->ntri, r->nzone, r->tetra, r->tnoti, r->di, r->permz,
+             r->ivol, r->ckrwe);
+  memcpy(r->xt5, r->tnoti, sizeof(double) * n);
+  r->numdir = orc_bcnew(r->nq, r->contq, r->q, r->nnod, r->lhsatm, r->ifatm,
+                        r->atmact, r->atmold, r->ja, r->topol, r->coef1,
+                        r->tnoti, r->p.tetaf, 1.7e100, r->noddir);
+  double err = 0.0, res = 0.0, resini = 0.0;
+  int niter = orc_nsyslv(n, r->nterm, r->numdir, r->noddir, r->p.itmxcg,
+                         r->p.tolcg, 1.7e-100, r->ia, r->ja, r->topol,
+                         r->coef1, r->coef2, r->coef3, r->scr, r->pdiff,
+                         r->tnoti, &err, &res, &resini);
+  r->itlin += niter;
+  r->nitert += niter;
+  if (niter >= r->p.itmxcg) {
+    r->lsfail = 1;
+    r->klsfai += 1;
+  } else {
+    r->lsfail = 0;
+  }
+  for (int k = 0; k < n; ++k) r->pnew[k] = r->pnew[k] + r->pdiff[k];
+  orc_shlnew(r->topol, r->ja, r->coef1, r->pnew, r->pold, r->nnod, r->ifatm,
+             r->lhsatm);
+  if (err_out) *err_out = err;
+}
+
 /* MASBAL restated for IOPT=1, NP=0, NSF=0, no nudging
  * (SRC/masbal.f:37-107 with SRC/fluxmb.f). */
 static void masbal(orc_run *r) {
   int n = r->n, nnod = r->nnod;
-  orc_bkpic(n, 0, nnod, r->p.tetaf, r->topol, r->ja, NULL, r->ifatm,
-            r->pdiff, NULL, NULL, r->atmact, r->atmold, r->coef1, r->xt5,
-            r->scr);
+  if (r->p.iopt == 2) {
+    orc_bknew(nnod, r->p.tetaf, r->topol, r->ja, r->ifatm, r->pdiff,
+              r->atmact, r->atmold, r->coef1, r->xt5);
+    /* The reference's Newton branch never sets FLOW3D's SWNEW / SWTIMEP
+     * (SRC/flow3d.f:83 is a local of FLOW3D that only CHPIC0 writes), so
+     * its STORMB reads stale values; here they are Sw at PNEW and at
+     * PTIMEP, as the Picard branch has them. */
+    for (int i = 0; i < n; ++i) {
+      double pnot = VGF(r->vg, n, VG_PNOT, i), rmc = VGF(r->vg, n, VG_RMC, i);
+      r->swnew[i] = pnot * orc_fvgse(r->pnew[i], r->vg, n, i) + rmc / r->pnodi[i];
+      r->swtimep[i] = pnot * orc_fvgse(r->ptimep[i], r->vg, n, i) + rmc / r->pnodi[i];
+    }
+  } else {
+    orc_bkpic(n, 0, nnod, r->p.tetaf, r->topol, r->ja, NULL, r->ifatm,
+              r->pdiff, NULL, NULL, r->atmact, r->atmold, r->coef1, r->xt5,
+              r->scr);
+  }
   /* CHMASS(PTIMEP) into SCR and CHMASS(PNEW) into ETAI: only ETAI's new
    * value is used later (SRC/masbal.f:70-73). */
   orc_chmass(n, r->ptimep, r->snodi, r->pnodi, r->vg, r->scr);
@@ -1411,11 +1953,13 @@ static void masbal(orc_run *r) {
   r->anin = anin; r->anout = anout; r->nnin = nnin; r->nnout = nnout;
 }
 
-/* FLOW3D, Picard branch: SRC/flow3d.f:93-294 (NLRELX=0, NSF=0). */
+/* FLOW3D: SRC/flow3d.f:93-294 (NLRELX=0, NSF=0), Picard (IOPT=1) or
+ * Newton (IOPT=2) iterations. */
 static void flow3d(orc_run *r) {
   int n = r->n;
   for (;;) {
-    picard(r, NULL);
+    if (r->p.iopt == 2) newton(r, NULL);
+    else picard(r, NULL);
     masbal(r);
     /* CONVER: SRC/conver.f:24-48 */
     orc_norms(n, &r->pikmax, &r->pinf, &r->pl2, &r->finf, &r->fl2, r->pnew,

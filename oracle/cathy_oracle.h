@@ -221,6 +221,77 @@ void orc_icvdwt(int nnod, int nstr, double wtposition, double *ptimep,
 void orc_neumann_free(int nnod, const int *acontq, const double *arenod,
                       const double *ckrw, const double *kznod, double *q);
 
+/* ------------------------- Newton linearisation, IOPT=2 (a16) -- */
+/* SRC/fvgdds.f, SRC/fvgdkr.f. */
+double orc_fvgdds(double psi, const double *vg, int n, int i);
+double orc_fvgdkr(double psi, const double *vg, int n, int i);
+/* SRC/chnew0.f (IVGHU=0): Sw, kr, ETAI and their psi-derivatives. */
+void orc_chnew0(int n, const double *ptnew, const double *snodi,
+                const double *pnodi, const double *vg, double *sw,
+                double *ckrw, double *etai, double *dckrw, double *detai);
+/* SRC/assnew.f on the full pattern; tetja[e*16+k*4+l] = position of
+ * (TETRA(k), TETRA(l)).  Accumulates (caller zeroes coef1/2/3). */
+void orc_assnew(int nt, int ntri, int nzone, const int *tetra,
+                const int *tetja, const double *lmass, double *coef1,
+                double *coef2, double *coef3, const double *permx,
+                const double *permy, const double *permz, const double *ckrwe,
+                const double *etae, const double *ptnew, const double *ptimep,
+                const double *pnew, const double *dckrw, const double *detai,
+                double tetaf, double deltat, const double *volu,
+                const double *volur, const int *ivol, const double *bi,
+                const double *ci, const double *di);
+/* SRC/rhsnew.f, SRC/cfmatn.f. */
+void orc_rhsnew(int n, const int *ja, const int *topol, double deltat,
+                const double *ptnew, const double *pnew, const double *ptimep,
+                const double *coef1, const double *coef2, double *tnoti);
+void orc_cfmatn(int nterm, double tetaf, double deltat, double *coef1,
+                const double *coef2, const double *coef3);
+/* SRC/bcnew.f, SRC/shlnew.f, SRC/bknew.f (NP=0, NSF=0). */
+int orc_bcnew(int nq, const int *contq, const double *q, int nnod,
+              double *lhsatm, const int *ifatm, const double *atmact,
+              const double *atmold, const int *ja, const int *topol,
+              double *coef1, double *tnoti, double tetaf, double rmax,
+              int *noddir);
+void orc_shlnew(const int *topol, const int *ja, double *coef1, double *pnew,
+                const double *pold, int nnod, const int *ifatm,
+                const double *lhsatm);
+void orc_bknew(int nnod, double tetaf, const int *topol, const int *ja,
+               const int *ifatm, const double *pdiff, double *atmact,
+               const double *atmold, const double *coef1, const double *xt5);
+/* SRC/topia.f. */
+void orc_topia(int n, const int *topol, int *ia);
+/* SRC/solscal-extended.f:2194-2270 INCLU (Crout ILU(0)); returns the
+ * number of zero pivots replaced. */
+int orc_inclu(const int *ia, const int *ja, const int *topol,
+              const double *coefr, double *coefc, double *lriga,
+              double *lcolon, int n, int nterm);
+/* :4205 PRODL, :4228 PRODU, :4251 PRU, :4107 PROD1. */
+void orc_prodl(double *z, const double *y, const int *ia, const int *ja,
+               const double *coef2, int nterm);
+void orc_produ(double *x, const double *h, int n, const int *ia,
+               const int *ja, const double *coef2, const int *topol);
+void orc_pru(double *y, const double *x, const int *ia, const int *ja,
+             const double *coef2, int nterm);
+void orc_prod1(double *x, const double *b, int n, const int *ia,
+               const int *ja, const double *coef2, int nterm,
+               const int *topol);
+/* :4332 RESNSY. */
+double orc_resnsy(const int *topol, const int *ja, const double *coef1,
+                  const double *tnoti, const double *pnew, double *resv,
+                  int n, int numdir, const int *noddir, double rmin);
+/* :1010-1128 GCSTAS (scr: 10 n). */
+int orc_gcstas(int n, double *poten, const double *tnoti, double *scr,
+               const int *topol, const int *ia, const int *ja,
+               const double *coef1, const double *coef2, int nterm, int imax,
+               double tol, int np, const int *contp, double *err_out);
+/* :3063-3174 NSYSLV with ISOLV=0 (scr: max(nterm, 10 n)); coef2 receives
+ * the ILU factor, coef3 is overwritten.  Returns NITER. */
+int orc_nsyslv(int n, int nterm, int numdir, const int *noddir, int itmxcg,
+               double tolcg, double rmin, const int *ia, const int *ja,
+               const int *topol, const double *coef1, double *coef2,
+               double *coef3, double *scr, double *pnew, const double *tnoti,
+               double *err, double *res, double *resini);
+
 /* --------------------------------------------- whole run (a1-a18) -- */
 typedef struct orc_params {
   /* input/parm (SRC/datin.f:80-122) */

@@ -13,6 +13,10 @@ container, where the reference tree is mounted at /root/reference).
    reference sources by oracle/Makefile into oracle/_ref/libcathy_ref.so and
    driven by tests/refrun.py): per-step statistics of every step and psi/Sw
    at selected steps.
+3. hillslope_small_newton_ref.npz -- the first 60 steps of the same run
+   with the Newton linearisation (IOPT=2, ISOLV=0: ILU(0)-BiCGSTAB).  The
+   reference's Newton mass-balance terms are stale (its Newton branch never
+   sets FLOW3D's local SWNEW/SWTIMEP) and are stored as computed.
 
 Only data goes into the fixtures; no reference source text.
 """
@@ -139,6 +143,31 @@ def make_hillslope_small(out: str, steps_saved=(1, 2, 5, 10, 25, 50, 100, 150)) 
     print(f"hillslope_small: {k} steps, saved {saved}")
 
 
+def make_hillslope_small_newton(out: str, nsteps: int = 60, steps_saved=(1, 5, 10, 20, 40, 60)) -> None:
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from pycathy_wrapper_amd import configs as CF
+    import pyoracle as P
+    import refrun as R
+    pr = CF.hillslope_small(params=CF.hillslope_small_params(iopt=2))
+    rr = R.RefFlowRun(P.RefLib(), pr.mesh, pr.soil, pr.psi0, float(pr.atm_vals[0]), pr.params.as_dict())
+    stats, psi, sw, keys = [], [], [], None
+    for k in range(1, nsteps + 1):
+        rc, s = rr.step()
+        keys = keys or list(s)
+        stats.append([s[nm] for nm in keys])
+        if k in steps_saved:
+            psi.append(rr.psi().copy())
+            sw.append(rr.sw().copy())
+        if rc:
+            break
+    np.savez_compressed(out, stat_names=np.array(keys), stats=np.array(stats), steps=np.array(steps_saved),
+                        psi=np.array(psi), sw=np.array(sw))
+    print(f"hillslope_small newton: {len(stats)} steps")
+
+
 if __name__ == "__main__":
     make_meshes(os.path.join(HERE, "meshes.npz"))
     make_hillslope_small(os.path.join(HERE, "hillslope_small_ref.npz"))
+    make_hillslope_small_newton(os.path.join(HERE, "hillslope_small_newton_ref.npz"))

@@ -80,11 +80,18 @@ class RefFlowRun:
         self.mb[:] = 0.0
         ref.chord[0] = prm["pmin"]
         from pycathy_wrapper_amd import mesh as M
-        topol, ja = M.sym_pattern(mesh.tetra, n)
-        nterm = len(ja)
+        iopt = int(prm.get("iopt", 1))
         tj = np.full((nt, 4, 4), 0, dtype=np.int32)  # TETJA(4,4,NT) col-major -> [e][l][k]
-        for idx, (k, l) in enumerate(M.UPPER_PAIRS):
-            tj[:, l, k] = M.tetja_sym(mesh.tetra, topol, ja, n)[:, idx] + 1
+        if iopt == 2:   # STRNEW + TETNEW (all 16 pairs) + TOPIA
+            topol, ja = M.full_pattern(mesh.tetra, n)
+            for k in range(4):
+                for l in range(4):
+                    tj[:, l, k] = M.csr_lookup(topol, ja, mesh.tetra[:, k], mesh.tetra[:, l], n) + 1
+        else:           # STRPIC + TETPIC (upper pairs)
+            topol, ja = M.sym_pattern(mesh.tetra, n)
+            for idx, (k, l) in enumerate(M.UPPER_PAIRS):
+                tj[:, l, k] = M.tetja_sym(mesh.tetra, topol, ja, n)[:, idx] + 1
+        nterm = len(ja)
         a = {}
         I = lambda v: np.array([v], dtype=np.int32)
         D = lambda v: np.array([v], dtype=np.float64)
@@ -102,7 +109,10 @@ class RefFlowRun:
                 a[name] = np.zeros(big, dtype=np.int32)
             else:
                 a[name] = np.zeros(big, dtype=np.float64)
-        a["SCR"] = np.zeros(5 * n + 16)
+        a["SCR"] = np.zeros(max(5 * n, 11 * n + nterm) + 16)
+        if iopt == 2:
+            a["IA"][:nterm] = np.repeat(np.arange(n), np.diff(topol)) + 1
+            a["RNSYM"][0] = 1.7e100
         tet = np.ascontiguousarray(mesh.tetra, dtype=np.int32).copy()
         tet[:, :4] += 1
         a["TETRA"] = tet.reshape(-1)
@@ -112,9 +122,10 @@ class RefFlowRun:
         a["IVOL"] = mesh.ivol.astype(np.int32).copy()
         if prm["free_drainage"]:   # ACONTQ of BCONE with NNEU<0: the base nodes
             a["CONTQ"][:nnod] = mesh.nstr * nnod + np.arange(nnod) + 1
-        for k, v in dict(IOPT=1, IPRT1=0, NLRELX=0, L2NORM=prm["l2norm"], N=n, NT=nt, NTRI=mesh.ntri,
+        for k, v in dict(IOPT=iopt, IPRT1=0, NLRELX=0, L2NORM=prm["l2norm"], N=n, NT=nt, NTRI=mesh.ntri,
                          NTERM=nterm, NSTR=mesh.nstr, NP=0, NQ=nnod if prm["free_drainage"] else 0,
-                         HSPATM=1, ITER=1, ITUNS=prm["ituns"], ISOLV=2, ITMXCG=prm["itmxcg"], NNOD=nnod,
+                         HSPATM=1, ITER=1, ITUNS=prm["ituns"], ISOLV=0 if iopt == 2 else 2,
+                         ITMXCG=prm["itmxcg"], NNOD=nnod,
                          NLKP=0).items():
             a[k][0] = v
         for k, v in dict(TETAF=prm["tetaf"], DTMIN=prm["dtmin"], DTREDM=prm["dtredm"], DTREDS=prm["dtreds"],

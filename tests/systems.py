@@ -111,3 +111,82 @@ def oracle_symslv(orc, topol, ja, coef1, tnoti, noddir0=None, itmxcg=500, tolcg=
                           iptr(topol), dptr(x), dptr(np.ascontiguousarray(tnoti)), dptr(np.ascontiguousarray(coef1)),
                           dptr(cf2), dptr(scr), C.byref(err), C.byref(res))
     return x, it, err.value, res.value, cf2
+
+
+def newton_system(orc, pr, psi: np.ndarray | None = None, deltat: float = 10.0, tetaf: float = 1.0):
+    """One Newton system (newton.f: NEWUNS, ASSNEW, RHSNEW, CFMATN,
+    RHSGRV) at iterate ``psi`` from PTIMEP = psi0, on the full pattern."""
+    m, s = pr.mesh, pr.soil
+    n, nt = m.n, m.nt
+    ns = node_soil(orc, pr)
+    ptimep = np.ascontiguousarray(pr.psi0, dtype=np.float64)
+    if psi is None:
+        psi = ptimep.copy()
+        psi[::7] -= 0.7
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    out = {k: np.zeros(n) for k in ("sw", "ckrw", "etai", "dckrw", "detai")}
+    orc.L.orc_chnew0(n, dptr(psi), dptr(ns["snodi"]), dptr(ns["pnodi"]), dptr(ns["vg"]), dptr(out["sw"]),
+                     dptr(out["ckrw"]), dptr(out["etai"]), dptr(out["dckrw"]), dptr(out["detai"]))
+    tet = np.ascontiguousarray(m.tetra, dtype=np.int32)
+    ckrwe = np.zeros(nt); etae = np.zeros(nt)
+    orc.L.orc_nodelt(nt, iptr(tet), dptr(out["ckrw"]), dptr(ckrwe))
+    orc.L.orc_nodelt(nt, iptr(tet), dptr(out["etai"]), dptr(etae))
+    topol, ja = M.full_pattern(m.tetra, n)
+    topol = np.ascontiguousarray(topol, dtype=np.int32)
+    ja = np.ascontiguousarray(ja, dtype=np.int32)
+    nterm = len(ja)
+    ia = np.repeat(np.arange(n), np.diff(topol)).astype(np.int32)
+    tj = np.zeros((nt, 16), dtype=np.int32)
+    for k in range(4):
+        for l in range(4):
+            tj[:, k * 4 + l] = M.csr_lookup(topol, ja, m.tetra[:, k], m.tetra[:, l], n)
+    lm = np.zeros(16)
+    orc.L.orc_locmas(dptr(lm), int(pr.params.lump))
+    perm = [np.ascontiguousarray(s[k], dtype=np.float64) for k in ("permx", "permy", "permz")]
+    c1 = np.zeros(nterm); c2 = np.zeros(nterm); c3 = np.zeros(nterm)
+    volur = 1.0 / m.volu
+    bi, ci, di = (np.ascontiguousarray(a) for a in (m.bi, m.ci, m.di))
+    volu = np.ascontiguousarray(m.volu)
+    ivol = np.ascontiguousarray(m.ivol, dtype=np.int32)
+    pnew = psi  # PNEW = PTNEW for TETAF = 1
+    orc.L.orc_assnew(nt, m.ntri, s["nzone"], iptr(tet), iptr(tj), dptr(lm), dptr(c1), dptr(c2), dptr(c3),
+                     dptr(perm[0]), dptr(perm[1]), dptr(perm[2]), dptr(ckrwe), dptr(etae), dptr(psi), dptr(ptimep),
+                     dptr(pnew), dptr(out["dckrw"]), dptr(out["detai"]), C.c_double(tetaf), C.c_double(deltat),
+                     dptr(volu), dptr(volur), iptr(ivol), dptr(bi), dptr(ci), dptr(di))
+    asm = dict(c1=c1.copy(), c2=c2.copy(), c3=c3.copy())
+    tno = np.zeros(n)
+    orc.L.orc_rhsnew(n, iptr(ja), iptr(topol), C.c_double(deltat), dptr(psi), dptr(pnew), dptr(ptimep), dptr(c1),
+                     dptr(c2), dptr(tno))
+    rhs = tno.copy()
+    orc.L.orc_cfmatn(nterm, C.c_double(tetaf), C.c_double(deltat), dptr(c1), dptr(c2), dptr(c3))
+    orc.L.orc_rhsgrv(nt, m.ntri, s["nzone"], iptr(tet), dptr(tno), dptr(di), dptr(perm[2]), iptr(ivol),
+                     dptr(ckrwe))
+    return dict(n=n, nterm=nterm, topol=topol, ja=ja, ia=ia, tetja=tj, lmass=lm, psi=psi, ptimep=ptimep,
+                deltat=deltat, tetaf=tetaf, ckrwe=ckrwe, etae=etae, volur=volur, asm=asm, rhs_rhsnew=rhs,
+                coef1=c1, tnoti=tno, **out, **ns)
+
+
+def penalise_full(sysd: dict, rows: np.ndarray):
+    """BCNEW's Dirichlet treatment (bcnew.f) on the full pattern."""
+    c1 = sysd["coef1"].copy()
+    tno = sysd["tnoti"].copy()
+    rows = np.asarray(rows, dtype=np.int64)
+    for r in rows:
+        q = sysd["topol"][r] + int(np.nonzero(sysd["ja"][sysd["topol"][r]:sysd["topol"][r + 1]] == r)[0][0])
+        c1[q] = 1e-9 * RMAX
+    tno[rows] = 0.0
+    return c1, tno, (rows + 1).astype(np.int32)
+
+
+def oracle_nsyslv(orc, sysd, coef1, tnoti, noddir0=None, itmxcg=500, tolcg=1e-10, rmin=1.7e-100):
+    """orc_nsyslv (ISOLV=0); returns (x, niter, err, res, factor)."""
+    n, nterm = sysd["n"], sysd["nterm"]
+    nd = np.ascontiguousarray(noddir0 if noddir0 is not None else np.zeros(1), dtype=np.int32)
+    numdir = 0 if noddir0 is None else len(nd)
+    x = np.zeros(n); c2 = np.zeros(nterm); c3 = np.zeros(nterm); scr = np.zeros(max(nterm, 11 * n))
+    err = C.c_double(); res = C.c_double(); resini = C.c_double()
+    it = orc.L.orc_nsyslv(n, nterm, numdir, iptr(nd), itmxcg, C.c_double(tolcg), C.c_double(rmin),
+                          iptr(sysd["ia"]), iptr(sysd["ja"]), iptr(sysd["topol"]),
+                          dptr(np.ascontiguousarray(coef1)), dptr(c2), dptr(c3), dptr(scr), dptr(x),
+                          dptr(np.ascontiguousarray(tnoti)), C.byref(err), C.byref(res), C.byref(resini))
+    return x, it, err.value, res.value, c2

@@ -180,3 +180,25 @@ def test_assembly_identities(oracle):
     m = pr.mesh
     assert c2.sum() == pytest.approx(np.sum(m.volu * s["et1e"]), rel=1e-12)
     assert s["asm"]["c4"].sum() == pytest.approx(np.sum(m.volu * s["pele"]), rel=1e-12)
+
+
+def test_oracle_newton_matches_reference_run(oracle):
+    """The Newton linearisation (IOPT=2, ILU(0)-BiCGSTAB): 60 steps of the
+    oracle against the reference run (tests/golden/hillslope_small_newton_ref
+    .npz): dt, Newton and BiCGSTAB counts, back-steps and the boundary
+    volumes equal; psi/Sw bit-identical at the saved steps."""
+    g = np.load(os.path.join(os.path.dirname(GOLD), "hillslope_small_newton_ref.npz"))
+    pr = CF.hillslope_small(params=CF.hillslope_small_params(iopt=2))
+    names = [str(s) for s in g["stat_names"]]
+    saved = [int(v) for v in g["steps"]]
+    run = _oracle_run(oracle, pr)
+    try:
+        for k in range(1, len(g["stats"]) + 1):
+            rc, s = run.step()
+            for nm in ("DELTAT", "TIME", "ITER", "NITERT", "BACK", "VIN", "VOUT"):
+                assert s[nm] == g["stats"][k - 1, names.index(nm)], (k, nm)
+            if k in saved:
+                assert np.array_equal(run.psi(), g["psi"][saved.index(k)]), k
+                assert np.array_equal(run.sw(), g["sw"][saved.index(k)]), k
+    finally:
+        run.close()
