@@ -168,7 +168,8 @@ struct Flow {
   }
   // partial-sum slots: 0 XLUNG, 1 (p,r), 2 (p,Ap), 3 r_free^2, 4 (Ap,z),
   // 5-10 norms/DSTORE, 11-16 boundary fluxes, 17 storage
-  static constexpr int kSlots = 18;
+  // 0-4 and 18 also serve the BiCGSTAB of the Newton path (no PCG there)
+  static constexpr int kSlots = 19;
   double *P(int slot) { return part.p + (size_t)slot * kMaxRedBlocks; }
   int red_grid(size_t work) const { return grid_for(work, kBlock, kMaxRedBlocks); }
   Sell sell() const {
@@ -875,7 +876,7 @@ struct Flow {
       for (size_t c = sched.cols.size(); c-- > 0;) {
         const ColDesc &C = sched.cols[c];
         hipLaunchKernelGGL(k_bwd_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C, sl, F,
-                           dinv.p, out, done);
+                           dinv.p, out, out, done);
       }
     } else {
       for (const ListLevel &L : sched.lv)
@@ -884,7 +885,7 @@ struct Flow {
       for (size_t l = sched.lv.size(); l-- > 0;) {
         const ListLevel &L = sched.lv[l];
         hipLaunchKernelGGL(k_bwd_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p, sl, F,
-                           dinv.p, out, done);
+                           dinv.p, out, out, done);
       }
     }
   }
@@ -963,6 +964,171 @@ struct Flow {
     return hs.niter;
   }
 
+  // ---------------------------------------------- Newton (IOPT = 2) --
+  DBuf<double> etai_n, dckrw, detai, ones, b_rt, b_s, b_t, b_v, b_p, b_y, b_z1, b_z2;
+  DBuf<BicgScal> bscal;
+  bool newton_ready = false;
+  void alloc_newton() {
+    if (newton_ready) return;
+    CG_REQUIRE(use_plan, "the Newton path needs the lumped assembly (LUMP=1)");
+    for (DBuf<double> *b : {&etai_n, &dckrw, &detai, &ones, &b_rt, &b_s, &b_t, &b_v, &b_p, &b_y, &b_z1, &b_z2})
+      b->alloc(n);
+    bscal.alloc(1);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, st, (size_t)n, 1.0, ones.p);
+    CG_HIP(hipStreamSynchronize(st));
+    newton_ready = true;
+  }
+  // y = L^-1 t (forward sweep, L non-unit) and x = U^-1 y (unit U: the
+  // backward sweep with inverse diagonal 1).
+  template <typename FT>
+  void ilu_fwd_t(const FT *F, const double *tin, double *out, const int *done) {
+    const Sell sl = sell();
+    if (ordering == CATHY_GPU_ORDER_COLUMN3) {
+      for (const ColDesc &C : sched.cols)
+        hipLaunchKernelGGL(k_fwd_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C, sl, F,
+                           dinv.p, tin, out, done);
+    } else {
+      for (const ListLevel &L : sched.lv)
+        hipLaunchKernelGGL(k_fwd_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p, sl, F,
+                           dinv.p, tin, out, done);
+    }
+  }
+  template <typename FT>
+  void ilu_bwd_t(const FT *F, const double *xin, double *out, const int *done) {
+    const Sell sl = sell();
+    if (ordering == CATHY_GPU_ORDER_COLUMN3) {
+      for (size_t c = sched.cols.size(); c-- > 0;) {
+        const ColDesc &C = sched.cols[c];
+        hipLaunchKernelGGL(k_bwd_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C, sl, F,
+                           ones.p, xin, out, done);
+      }
+    } else {
+      for (size_t l = sched.lv.size(); l-- > 0;) {
+        const ListLevel &L = sched.lv[l];
+        hipLaunchKernelGGL(k_bwd_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p, sl, F,
+                           ones.p, xin, out, done);
+      }
+    }
+  }
+  void ilu_fwd(const double *tin, double *out, const int *done) {
+    if (f32) ilu_fwd_t<float>(F32.p, tin, out, done);
+    else ilu_fwd_t<double>(F64.p, tin, out, done);
+  }
+  void ilu_bwd(const double *xin, double *out, const int *done) {
+    if (f32) ilu_bwd_t<float>(F32.p, xin, out, done);
+    else ilu_bwd_t<double>(F64.p, xin, out, done);
+  }
+  template <typename FT>
+  void ilu_factor_t(FT *F) {
+    const Sell sl = sell();
+    if (ordering == CATHY_GPU_ORDER_COLUMN3) {
+      for (const ColDesc &C : sched.cols)
+        hipLaunchKernelGGL(k_ilu_factor_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), ilu_smem(C), st, C,
+                           sl, pairs(), stpos.p, A.p, isdir.p, F, dinv.p, flags.p + 1);
+    } else {
+      for (const ListLevel &L : sched.lv)
+        hipLaunchKernelGGL(k_ilu_factor_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p,
+                           sl, pairs(), stpos.p, A.p, isdir.p, F, dinv.p, flags.p + 1);
+    }
+  }
+  size_t ilu_smem(const ColDesc &C) const { return (sizeof(double) * 3 + sizeof(int) * 3) * (size_t)C.len * kChains; }
+  // y = A x, one launch
+  void spmv(const double *x, double *y, const int *done) {
+    hipLaunchKernelGGL(k_spmv, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, n, sell(), A.p, x, y, done);
+  }
+  // z = L^-1 A U^-1 h (prodbh)
+  void prodbh(const double *h, double *z, const int *done) {
+    ilu_bwd(h, b_z1.p, done);
+    spmv(b_z1.p, b_z2.p, done);
+    ilu_fwd(b_z2.p, z, done);
+  }
+
+  // NSYSLV with ISOLV=0 on the current A / TNOTI: x = pdiff.  Returns NITER.
+  int bicgstab(double tol, int imax, double *err_out, int expect) {
+    alloc_newton();
+    if (f32) ilu_factor_t<float>(F32.p);
+    else ilu_factor_t<double>(F64.p);
+    int *done = flags.p;
+    const int G = std::min(grid_for(n, kBlock), 1024);
+    const int Gr = std::min(grid_for(n, kBlock), 256);
+    // x0 = (LU)^-1 b (prod1); r = L^-1 (b - A x0); rt = r; y = U x0
+    ilu_fwd(tnoti.p, b_z2.p, nullptr);
+    ilu_bwd(b_z2.p, pdiff.p, nullptr);
+    hipLaunchKernelGGL(k_residual0, dim3(G), dim3(kBlock), 0, st, n, sell(), A.p, pdiff.p, tnoti.p, isdir.p,
+                       b_z1.p, P(0));
+    ilu_fwd(b_z1.p, r.p, nullptr);
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(kBlock), 0, st, (size_t)n, r.p, b_rt.p);
+    if (f32) hipLaunchKernelGGL(k_mul_upper<float>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, sell(), F32.p,
+                                pdiff.p, b_y.p);
+    else hipLaunchKernelGGL(k_mul_upper<double>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, sell(), F64.p,
+                            pdiff.p, b_y.p);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, st, (size_t)n, 0.0, b_p.p);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, st, (size_t)n, 0.0, b_v.p);
+    hipLaunchKernelGGL(k_dot, dim3(Gr), dim3(kBlock), 0, st, n, b_rt.p, r.p, P(1), nullptr);
+    hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(kBlock), 0, st, G, P(0), Gr, P(1), bscal.p, done, tol, imax);
+    int batch = std::max(1, std::min(expect, imax + 1));
+    int launched = 0, h_done = 0;
+    BicgScal hs{};
+    while (true) {
+      for (int b = 0; b < batch; ++b) {
+        hipLaunchKernelGGL(k_bicg_p, dim3(Gr), dim3(kBlock), 0, st, n, r.p, b_v.p, b_p.p, bscal.p, done);
+        prodbh(b_p.p, b_v.p, done);
+        hipLaunchKernelGGL(k_dot, dim3(Gr), dim3(kBlock), 0, st, n, b_rt.p, b_v.p, P(2), done);
+        hipLaunchKernelGGL(k_bicg_s, dim3(Gr), dim3(kBlock), 0, st, n, Gr, P(2), r.p, b_v.p, b_s.p, bscal.p, done);
+        prodbh(b_s.p, b_t.p, done);
+        hipLaunchKernelGGL(k_bicg_st, dim3(Gr), dim3(kBlock), 0, st, n, b_s.p, b_t.p, P(3), P(4), done);
+        hipLaunchKernelGGL(k_bicg_update, dim3(Gr), dim3(kBlock), 0, st, n, Gr, P(3), P(4), b_p.p, b_s.p, b_t.p,
+                           b_rt.p, isdir.p, b_y.p, r.p, P(18), P(1), bscal.p, done);
+        hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(kBlock), 0, st, n, Gr, P(18), P(1), bscal.p, done);
+      }
+      launched += batch;
+      CG_HIP(hipMemcpyAsync(&h_done, done, sizeof(int), hipMemcpyDeviceToHost, st));
+      CG_HIP(hipMemcpyAsync(&hs, bscal.p, sizeof(BicgScal), hipMemcpyDeviceToHost, st));
+      CG_HIP(hipStreamSynchronize(st));
+      if (h_done) break;
+      CG_REQUIRE(launched <= imax + 3, "BiCGSTAB did not stop");
+      batch = std::max(2, std::min(8, imax + 1 - hs.niter + 1));
+    }
+    // x = U^-1 y
+    ilu_bwd(b_y.p, pdiff.p, nullptr);
+    *err_out = hs.err;
+    return hs.niter;
+  }
+
+  // One Newton iteration (newton.f).
+  void newton_iteration(double &err) {
+    alloc_newton();
+    const int gn = grid_for(n), ge = grid_for(nt);
+    hipLaunchKernelGGL(k_constitutive_newton, dim3(gn), dim3(kBlock), 0, st, n, ptnew.p, soil(), sw.p, ckrw.p,
+                       etai_n.p, dckrw.p, detai.p);
+    hipLaunchKernelGGL(k_elem_avg, dim3(ge), dim3(kBlock), 0, st, nt, enode.p, ckrw.p, etai_n.p, ckrwe.p, et1e.p);
+    hipLaunchKernelGGL(k_bc_rows, dim3(gn), dim3(kBlock), 0, st, n, row_surf.p, qnode.p, ifatm.p, atmact.p,
+                       atmold.p, prm.tetaf, isdir.p, bcflux.p);
+    AsmArgs a{};
+    a.n = n; a.sl = sell(); a.ckrwe = ckrwe.p; a.ptnew = ptnew.p; a.pnew = pnew.p; a.ptimep = ptimep.p;
+    a.isdir = isdir.p; a.bcflux = bcflux.p; a.tetaf = prm.tetaf; a.rdt = 1.0 / deltat; a.A = A.p;
+    a.tnoti = tnoti.p; a.xt5 = xt5.p;
+    const AsmPlan pl{plan_off.p, plan_ie.p, plan_io.p, plan_s4.p, plan_w2.p, plan_gw.p};
+    const NewtonArgs nw{etai_n.p, ckrw.p, dckrw.p, detai.p, gz.p};
+    const int ga = grid_for(n, kBlock);
+    if (maxrow <= 16) hipLaunchKernelGGL((k_assemble_newton<16>), dim3(ga), dim3(kBlock), 0, st, a, pl, nw);
+    else hipLaunchKernelGGL((k_assemble_newton<32>), dim3(ga), dim3(kBlock), 0, st, a, pl, nw);
+    const int expect = last_pcg > 0 ? last_pcg : 8;
+    int niter = bicgstab(prm.tolcg, prm.itmxcg, &err, expect);
+    last_pcg = niter;
+    itlin += niter;
+    nitert += niter;
+    c_pcg += niter;
+    if (niter >= prm.itmxcg) {
+      lsfail = 1;
+      klsfai += 1;
+    } else {
+      lsfail = 0;
+    }
+    hipLaunchKernelGGL(k_update_pnew, dim3(gn), dim3(kBlock), 0, st, n, pdiff.p, row_surf.p, ifatm.p, pold.p,
+                       pnew.p);
+  }
+
   // ------------------------------------------------ Picard iteration --
   void picard_iteration(double &err) {
     const int gn = grid_for(n), ge = grid_for(nt);
@@ -1013,6 +1179,10 @@ struct Flow {
   // MASBAL (masbal.f:37-107) + NORMS (norms.f); result in h_res.
   void masbal_norms() {
     const int gs = grid_for(nnod);
+    if (prm.iopt == 2) {  // Sw at PNEW and PTIMEP for STORMB (the Picard branch has them from CHPIC0)
+      hipLaunchKernelGGL(k_sw_only, dim3(grid_for(n)), dim3(kBlock), 0, st, n, pnew.p, soil(), swnew.p);
+      hipLaunchKernelGGL(k_sw_only, dim3(grid_for(n)), dim3(kBlock), 0, st, n, ptimep.p, soil(), swtimep.p);
+    }
     hipLaunchKernelGGL(k_bkpic, dim3(gs), dim3(kBlock), 0, st, nnod, surf_row.p, sell(), A.p, pdiff.p, xt5.p,
                        ifatm.p, atmold.p, prm.tetaf, atmact.p);
     const int G = red_grid(n), Gf = red_grid(nnod);
@@ -1052,7 +1222,8 @@ struct Flow {
     const int gn = grid_for(n);
     while (true) {
       double err = 0.0;
-      picard_iteration(err);
+      if (prm.iopt == 2) newton_iteration(err);
+      else picard_iteration(err);
       c_picard += 1;
       masbal_norms();
       if ((prm.l2norm == 0 && pinf <= prm.tolswi) || (prm.l2norm != 0 && pl2 <= prm.tolswi)) switch_old();
@@ -1452,6 +1623,8 @@ int cathy_gpu_flow3d_set_params(int64_t *handle, const int *ip, const double *dp
   return guarded([&] {
     Flow *f = lookup(handle);
     CG_REQUIRE(ip && dp, "null argument");
+    CG_REQUIRE(ip[0] == 1 || ip[0] == 2, "IOPT must be 1 (Picard) or 2 (Newton)");
+    CG_REQUIRE(ip[0] == 1 || f->use_plan, "IOPT=2 on the device needs LUMP=1 and rows of at most 31 entries");
     Params &p = f->prm;
     p.iopt = ip[0]; p.l2norm = ip[1]; p.ituns = ip[2]; p.ituns1 = ip[3]; p.ituns2 = ip[4];
     p.itmxcg = ip[5]; p.ieto = ip[6]; p.free_drainage = ip[7]; p.hspatm = ip[8];
@@ -1459,7 +1632,6 @@ int cathy_gpu_flow3d_set_params(int64_t *handle, const int *ip, const double *dp
     p.deltat = dp[5]; p.dtmin = dp[6]; p.dtmax = dp[7]; p.tmax = dp[8]; p.dtmaga = dp[9];
     p.dtmagm = dp[10]; p.dtreds = dp[11]; p.dtredm = dp[12]; p.pmin = dp[13]; p.pondh_min = dp[14];
     p.scf = dp[15];
-    CG_REQUIRE(p.iopt == 1, "IOPT=2 (Newton) is not available in this build");
     CG_REQUIRE(p.tetaf > 0.0 && p.deltat > 0.0 && p.itmxcg >= 1 && p.ituns >= 1, "bad solver parameters");
     return 0;
   });

@@ -168,7 +168,8 @@ template <typename FT>
 __global__ void k_fwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, const double *tin, double *y,
                            const int *done);
 template <typename FT>
-__global__ void k_bwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, double *x, const int *done);
+__global__ void k_bwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, const double *xin, double *x,
+                           const int *done);
 template <typename FT>
 __global__ void k_ic_factor_list(ListLevel L, const int *rows, Sell sl, PairList pl, const int *tpos,
                                  const double *A, const uint8_t *isdir, FT *F, double *dinv, int *nfall);
@@ -176,8 +177,8 @@ template <typename FT>
 __global__ void k_fwd_list(ListLevel L, const int *rows, Sell sl, const FT *F, const double *dinv,
                            const double *tin, double *y, const int *done);
 template <typename FT>
-__global__ void k_bwd_list(ListLevel L, const int *rows, Sell sl, const FT *F, const double *dinv, double *x,
-                           const int *done);
+__global__ void k_bwd_list(ListLevel L, const int *rows, Sell sl, const FT *F, const double *dinv,
+                           const double *xin, double *x, const int *done);
 
 __global__ void k_residual0(int n, Sell sl, const double *A, const double *x, const double *b,
                             const uint8_t *isdir, double *r, double *part);
@@ -217,5 +218,40 @@ __global__ void k_check_codes(int n, Sell sl, int *bad);
 __global__ void k_sumsq_free(int n, const double *u, const uint8_t *isdir, double *part);
 __global__ void k_copy(size_t n, const double *src, double *dst);
 __global__ void k_fill(size_t n, double v, double *dst);
+
+// ------------------------------------------- Newton (newton_kernels.hip) --
+struct NewtonArgs {
+  const double *etai, *ckrw, *dckrw, *detai, *gz;
+};
+struct BicgScal {
+  double xlung, roj, rojm1, alfj, alfjm1, etaj, etaj_next, err, tol;
+  int niter, imax;
+};
+__global__ void k_constitutive_newton(int n, const double *ptnew, SoilSoA s, double *sw, double *ckrw,
+                                      double *etai, double *dckrw, double *detai);
+template <int MR>
+__global__ void k_assemble_newton(AsmArgs a, AsmPlan P, NewtonArgs nw);
+template <typename FT>
+__global__ void k_ilu_factor_col3(ColDesc C, Sell sl, PairList pl, const int *tpos, const double *A,
+                                  const uint8_t *isdir, FT *F, double *dinv, int *nfall);
+template <typename FT>
+__global__ void k_ilu_factor_list(ListLevel L, const int *rows, Sell sl, PairList pl, const int *tpos,
+                                  const double *A, const uint8_t *isdir, FT *F, double *dinv, int *nfall);
+template <typename FT>
+__global__ void k_mul_upper(int n, Sell sl, const FT *F, const double *x, double *y);
+__global__ void k_spmv(int n, Sell sl, const double *A, const double *x, double *y, const int *done);
+__global__ void k_bicg_init(int nxl, const double *part_xl, int nrr, const double *part_rr, BicgScal *sc,
+                            int *done, double tol, int imax);
+__global__ void k_bicg_p(int n, const double *r, const double *v, double *p, const BicgScal *sc, const int *done);
+__global__ void k_bicg_s(int n, int nparts, const double *part_sig, const double *r, const double *v, double *s,
+                         BicgScal *sc, const int *done);
+__global__ void k_bicg_st(int n, const double *s, const double *t, double *part_st, double *part_tt,
+                          const int *done);
+__global__ void k_bicg_update(int n, int nparts, const double *part_st, const double *part_tt, const double *p,
+                              const double *s, const double *t, const double *rt, const uint8_t *isdir, double *y,
+                              double *r, double *part_rr, double *part_rtr, BicgScal *sc, const int *done);
+__global__ void k_bicg_check(int n, int nparts, const double *part_rr, const double *part_rtr, BicgScal *sc,
+                             int *done);
+__global__ void k_sw_only(int n, const double *psi, SoilSoA s, double *sw);
 
 }  // namespace cg

@@ -422,7 +422,7 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
 // colour, in place: later colours are final; chains run from the base up.
 template <typename FT>
 __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
-                                                  const double *__restrict__ dinv, double *__restrict__ x,
+                                                  const double *__restrict__ dinv, const double *xin, double *x,
                                                   const int *__restrict__ done) {
   if (done && *done) return;
   extern __shared__ double smem[];
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
     const int js = k < kl ? jd + 1 : jd;
     const int len = sell_len(sl, rp, r);
     const int base = sl.so[r >> 6] + (r & 63);
-    double s = x[r];
+    double s = xin[r];
     if (len <= kUnroll) {
       int c[kUnroll];
       float f[kUnroll];
@@ -539,7 +539,7 @@ __global__ void k_fwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, c
 
 template <typename FT>
 __global__ void k_bwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, const FT *__restrict__ F,
-                           const double *__restrict__ dinv, double *__restrict__ x,
+                           const double *__restrict__ dinv, const double *xin, double *x,
                            const int *__restrict__ done) {
   if (done && *done) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -547,7 +547,7 @@ __global__ void k_bwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, c
   const int r = rows[L.off + t];
   const int jd = sl.diag[r], len = sl.len[r];
   const int base = sl.so[r >> 6] + (r & 63);
-  double s = x[r];
+  double s = xin[r];
   for (int j = jd + 1; j < len; ++j) {
     const int pj = base + 64 * j;
     s -= (double)F[pj] * x[sl.col[pj]];
@@ -560,13 +560,14 @@ __global__ void k_bwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, c
                                                 const uint8_t *, FT *, double *, int *);                    \
   template __global__ void k_fwd_col3<FT>(ColDesc, Sell, const FT *, const double *, const double *,        \
                                           double *, const int *);                                          \
-  template __global__ void k_bwd_col3<FT>(ColDesc, Sell, const FT *, const double *, double *, const int *); \
+  template __global__ void k_bwd_col3<FT>(ColDesc, Sell, const FT *, const double *, const double *, double *, \
+                                          const int *);                                                      \
   template __global__ void k_ic_factor_list<FT>(ListLevel, const int *, Sell, PairList, const int *,        \
                                                 const double *, const uint8_t *, FT *, double *, int *);    \
   template __global__ void k_fwd_list<FT>(ListLevel, const int *, Sell, const FT *, const double *,         \
                                           const double *, double *, const int *);                          \
-  template __global__ void k_bwd_list<FT>(ListLevel, const int *, Sell, const FT *, const double *, double *, \
-                                          const int *);
+  template __global__ void k_bwd_list<FT>(ListLevel, const int *, Sell, const FT *, const double *,          \
+                                          const double *, double *, const int *);
 CG_INST_FT(float)
 CG_INST_FT(double)
 #undef CG_INST_FT

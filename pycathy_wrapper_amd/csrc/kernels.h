@@ -41,6 +41,31 @@ __device__ __forceinline__ double dev_fvgdse(double psi, double psat, double vgn
   return 0.0;
 }
 
+// d2 Se / d psi2 (fvgdds.f) and the kr derivative term of fvgdkr.f, the
+// formulas the reference's Newton Jacobian uses (fvgdkr.f's is not the
+// exact derivative of fvgkr.f; it is reproduced as written).
+__device__ __forceinline__ double dev_fvgdds(double psi, double psat, double vgn, double vgn1, double vgm) {
+  if (psi < -1.0e-14) {
+    double beta = pow(fabs(psi / psat), vgn);
+    double b1 = beta + 1.0;
+    double b1r = 1.0 / b1;
+    return vgn1 * (beta / psi) * (1.0 / psi) * ((1.0 + vgn * (beta - 1.0)) / pow(fabs(b1), vgm)) * b1r * b1r;
+  }
+  return 0.0;
+}
+__device__ __forceinline__ double dev_fvgdkr(double psi, double psat, double vgn, double vgn1, double vgm) {
+  if (psi < -1.0e-14) {
+    double beta = pow(fabs(psi / psat), vgn);
+    double b1 = beta + 1.0;
+    double v1 = pow(fabs(b1), vgm) - pow(fabs(beta), vgm);
+    double v2 = psat / psi;
+    double v3 = vgn1 * beta * v2 * pow(fabs(1.0 / b1), 2.5 * vgm) / psat;
+    double v4 = v2 * ((2.5 / b1) * beta - 2.0) - 0.5 * pow(fabs(b1), vgm - 1.0);
+    return v3 * v1 * v4;
+  }
+  return 0.0;
+}
+
 // Node soil table, structure of arrays (device numbering).
 struct SoilSoA {
   const double *psat, *vgn, *vgm, *vgn1, *vgnr, *vgmr, *vgpsn, *pnot, *rmcp, *snodi, *pnodi;

@@ -1,0 +1,129 @@
+"""The Newton linearisation (IOPT=2: Jacobian with the derivative terms,
+ILU(0)-preconditioned BiCGSTAB) on the device, through the C-ABI.
+
+Against tests/golden/hillslope_small_newton_ref.npz (60 steps of the
+reference's own FLOW3D in Newton mode) and against the oracle (pinned to
+the reference in test_oracle_newton.py) on heterogeneous meshes.  With the
+natural ordering the factor is the reference's Crout ILU(0) in FP64 and the
+Newton and BiCGSTAB counts equal the reference's; the column ordering uses
+an FP32 factor in its own elimination order, so only its BiCGSTAB counts
+differ.  The reference's Newton mass balance is not a comparison target
+(its Newton branch never sets FLOW3D's SWNEW/SWTIMEP); the device's is
+compared with the oracle's.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from pycathy_wrapper_amd import configs as CF
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hillslope_small_newton_ref.npz")
+PSI_TOL = 1e-6
+MB_TOL = 1e-8
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from pycathy_wrapper_amd import gpu as G
+    G.device_info()
+    return G
+
+
+def _rel_inf(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("ordering", ["natural", "column3"])
+def test_newton_matches_reference_run(gpu, ordering):
+    g0 = np.load(GOLD)
+    names = [str(v) for v in g0["stat_names"]]
+    col = {nm: j for j, nm in enumerate(names)}
+    saved = [int(v) for v in g0["steps"]]
+    pr = CF.hillslope_small(params=CF.hillslope_small_params(iopt=2))
+    order = gpu.ORDER_NATURAL if ordering == "natural" else gpu.ORDER_COLUMN3
+    g = gpu.FlowGPU(pr, order)
+    worst = 0.0
+    try:
+        for k in range(1, len(g0["stats"]) + 1):
+            rc, s = g.advance()
+            ref = g0["stats"][k - 1]
+            assert s["DELTAT"] == ref[col["DELTAT"]] and s["TIME"] == ref[col["TIME"]], k
+            assert s["ITER"] == ref[col["ITER"]] and s["BACK"] == ref[col["BACK"]], (k, s["ITER"], ref[col["ITER"]])
+            if ordering == "natural":
+                assert s["NITERT"] == ref[col["NITERT"]], (k, s["NITERT"], ref[col["NITERT"]])
+            for nm in ("VIN", "VOUT"):
+                assert abs(s[nm] - ref[col[nm]]) <= MB_TOL * max(abs(ref[col[nm]]), 1e-12), (k, nm)
+            if k in saved:
+                e = _rel_inf(g.psi(), g0["psi"][saved.index(k)])
+                worst = max(worst, e)
+                assert e <= PSI_TOL, (k, e)
+            assert rc == 0
+    finally:
+        g.close()
+    say(f"newton {ordering}: {len(g0['stats'])} steps, worst psi rel {worst:.2e}")
+
+
+def _oracle_run(oracle, pr):
+    import pyoracle as P
+    m = pr.mesh
+    return oracle.run_create(dict(tetra=m.tetra, x=m.x, y=m.y, z=m.z, arenod=m.arenod, nnod=m.nnod, nstr=m.nstr,
+                                  ntri=m.ntri), pr.soil, pr.psi0, pr.atm_times, pr.atm_vals, 1,
+                             P.OrcParams(**pr.params.as_dict()))
+
+
+@pytest.mark.parametrize("ordering", ["natural", "column3"])
+def test_newton_heterogeneous_matches_oracle(gpu, oracle, ordering):
+    """41x41 nodes, 15 graded layers, per-cell soil: 6 Newton steps."""
+    pr = CF.hillslope_large(cells=40, nstr=15, params=CF.hillslope_small_params(tmax=172800.0, iopt=2))
+    order = gpu.ORDER_NATURAL if ordering == "natural" else gpu.ORDER_COLUMN3
+    g = gpu.FlowGPU(pr, order)
+    run = _oracle_run(oracle, pr)
+    scale = 1e-6 * pr.mesh.volnod.sum()
+    try:
+        for k in range(6):
+            rc_o, so = run.step()
+            rc_g, sg = g.advance()
+            say(f"{ordering} step {k + 1}: dt {sg['DELTAT']:.4g}/{so['DELTAT']:.4g} newton {sg['ITER']:.0f}/"
+                f"{so['ITER']:.0f} bicgstab {sg['NITERT']:.0f}/{so['NITERT']:.0f}")
+            assert rc_g == rc_o
+            assert sg["DELTAT"] == so["DELTAT"] and sg["ITER"] == so["ITER"] and sg["BACK"] == so["BACK"]
+            if ordering == "natural":
+                assert sg["NITERT"] == so["NITERT"]
+            assert _rel_inf(g.psi(), run.psi()) <= PSI_TOL
+            for key in ("DSTORE", "VIN", "VOUT", "ERRAS"):
+                assert abs(sg[key] - so[key]) <= MB_TOL * max(abs(so[key]), scale), (k, key, sg[key], so[key])
+            if rc_o:
+                break
+    finally:
+        g.close()
+        run.close()
+
+
+def test_newton_214k_nodes_first_step_matches_oracle(gpu, oracle):
+    """101x101 nodes, 20 layers (214,221 nodes), column ordering: the
+    first Newton time step against the oracle."""
+    pr = CF.hillslope_large(cells=100, nstr=20, params=CF.hillslope_small_params(tmax=172800.0, iopt=2))
+    say(f"n={pr.mesh.n}: oracle step 1")
+    run = _oracle_run(oracle, pr)
+    rc_o, so = run.step()
+    psi_o = run.psi()
+    run.close()
+    g = gpu.FlowGPU(pr, gpu.ORDER_COLUMN3)
+    try:
+        rc_g, sg = g.advance()
+        say(f"oracle dt {so['DELTAT']} newton {so['ITER']} bicgstab {so['NITERT']}; device dt {sg['DELTAT']} "
+            f"newton {sg['ITER']} bicgstab {sg['NITERT']}")
+        assert sg["DELTAT"] == so["DELTAT"] and sg["ITER"] == so["ITER"]
+        assert _rel_inf(g.psi(), psi_o) <= PSI_TOL
+    finally:
+        g.close()
