@@ -12,7 +12,8 @@
  * Reference interfaces replaced (paths relative to the vendored CATHY
  * source, pyCATHY/tests/weil_exemple/my_cathy_prj/src/):
  *   cathy_gpu_symslv        SYMSLV   solscal-extended.f:4669-4699
- *   cathy_gpu_flow3d_*      FLOW3D   flow3d.f:8-294 (Picard: picard.f:8)
+ *   cathy_gpu_flow3d_*      FLOW3D   flow3d.f:8-294 (Picard: picard.f:8;
+ *                           Newton: newton.f with NSYSLV ISOLV=0)
  *                           with the per-time-step host work of
  *                           cathy_main.f:2882-3860 (NEUMANN free drainage,
  *                           ATMNXT/ATMBAK, SWITCH_OLD, BKSTEP, CHVELO,
@@ -101,6 +102,7 @@ int cathy_gpu_flow3d_create(int64_t *handle, const int *n, const int *nt,
 int cathy_gpu_flow3d_destroy(int64_t *handle);
 
 /* input/parm and soil-file scalars (datin.f:80-122, :421).
+ * IOPT 1 = Picard + IC(0)-PCG, 2 = Newton + ILU(0)-BiCGSTAB (LUMP=1).
  * iprm: [0] IOPT [1] L2NORM [2] ITUNS [3] ITUNS1 [4] ITUNS2 [5] ITMXCG
  *       [6] IETO [7] free drainage at the base (NNEU<0) [8] HSPATM
  * dprm: [0] TETAF [1] TOLUNS [2] TOLSWI [3] ERNLMX [4] TOLCG [5] DELTAT
