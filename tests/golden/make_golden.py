@@ -13,6 +13,9 @@ container, where the reference tree is mounted at /root/reference).
    reference sources by oracle/Makefile into oracle/_ref/libcathy_ref.so and
    driven by tests/refrun.py): per-step statistics of every step and psi/Sw
    at selected steps.
+4. weil_outputs.npz -- text of the weil run's output files as the reference
+   wrote them (examples/SSHydro/weill_exemple/output/): the first two output
+   times of psi and sw (with sw's header line) and the whole mbeconv.
 3. hillslope_small_newton_ref.npz -- the first 60 steps of the same run
    with the Newton linearisation (IOPT=2, ISOLV=0: ILU(0)-BiCGSTAB).  The
    reference's Newton mass-balance terms are stale (its Newton branch never
@@ -167,7 +170,21 @@ def make_hillslope_small_newton(out: str, nsteps: int = 60, steps_saved=(1, 5, 1
     print(f"hillslope_small newton: {len(stats)} steps")
 
 
+def make_outputs(out: str, nblocks: int = 2) -> None:
+    d = os.path.join(REF, "examples/SSHydro/weill_exemple/output")
+    texts = {}
+    for name in ("psi", "sw"):
+        lines = open(os.path.join(d, name)).read().splitlines(keepends=True)
+        heads = [i for i, ln in enumerate(lines) if "NSTEP" in ln and "TIME" in ln]
+        end = heads[nblocks] if len(heads) > nblocks else len(lines)
+        texts[name] = "".join(lines[:end])
+    texts["mbeconv"] = open(os.path.join(d, "mbeconv")).read()
+    np.savez_compressed(out, **{k: np.array(v) for k, v in texts.items()})
+    print("weil outputs:", {k: len(v) for k, v in texts.items()})
+
+
 if __name__ == "__main__":
     make_meshes(os.path.join(HERE, "meshes.npz"))
     make_hillslope_small(os.path.join(HERE, "hillslope_small_ref.npz"))
     make_hillslope_small_newton(os.path.join(HERE, "hillslope_small_newton_ref.npz"))
+    make_outputs(os.path.join(HERE, "weil_outputs.npz"))
