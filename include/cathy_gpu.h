@@ -137,6 +137,19 @@ int cathy_gpu_flow3d_get_state(int64_t *handle, double *pnew, double *sw,
                                double *ckrw, int *ifatm);
 int cathy_gpu_flow3d_set_pnew(int64_t *handle, const double *pnew);
 
+/* VEL3D (vel3d.f) + VNOD3D (vnod3d.f), cathy_main.f:3243-3246: Darcy
+ * velocities UU/VV/WW(NT) per element and their nodal means UNOD/VNOD/
+ * WNOD(N) at the current PNEW and kr.  TETRA, PERMX/Y/Z(MAXSTR,NZONE),
+ * BI/CI/DI(4,NT), VOLUR(NT), IVOL(NT) are read on the first call only and
+ * kept on the device.  Any output pointer may be NULL. */
+int cathy_gpu_flow3d_vel3d(int64_t *handle, const int *tetra,
+                           const double *permx, const double *permy,
+                           const double *permz, const double *bi,
+                           const double *ci, const double *di,
+                           const double *volur, const int *ivol, double *uu,
+                           double *vv, double *ww, double *unod, double *vnod,
+                           double *wnod);
+
 /* Counters of the last advance: Picard its, PCG its, IC(0) breakdown
  * fallbacks, kernel time of the linear solver (ms), total device ms. */
 int cathy_gpu_flow3d_counters(int64_t *handle, double *out6);

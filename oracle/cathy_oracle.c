@@ -2094,3 +2094,52 @@ int orc_run_n(const orc_run *r) { return r->n; }
 int orc_run_nterm(const orc_run *r) { return r->nterm; }
 const int *orc_run_topol(const orc_run *r) { return r->topol; }
 const int *orc_run_ja(const orc_run *r) { return r->ja; }
+
+/* ===================================================================== */
+/*                            Darcy velocities                           */
+/* ===================================================================== */
+
+void orc_vel3d(int nt, int ntri, int nzone, const int *tetra, const double *pnew,
+               const double *permx, const double *permy, const double *permz,
+               double *uu, double *vv, double *ww, const double *bi,
+               const double *ci, const double *di, const double *ckrwe,
+               const double *volur, const int *ivol) {
+  /* SRC/vel3d.f:28-52: element velocities -kr K grad(psi + z) from the P1
+   * gradient; perm* are [nstr][nzone] here, PERMX(ISTR,MTYPE) there. */
+  for (int e = 0; e < nt; ++e) {
+    int istr = e / (3 * ntri);
+    int mt = tetra[(size_t)e * 5 + 4] - 1;
+    size_t ix = (size_t)istr * nzone + mt;
+    double bb = 0.0, cc = 0.0, dd = 0.0;
+    double xyz = -ckrwe[e] * volur[e] * (double)ivol[e];
+    for (int j = 0; j < 4; ++j) {
+      int i = tetra[(size_t)e * 5 + j];
+      bb = bb + pnew[i] * bi[(size_t)e * 4 + j];
+      cc = cc + pnew[i] * ci[(size_t)e * 4 + j];
+      dd = dd + pnew[i] * di[(size_t)e * 4 + j];
+    }
+    uu[e] = bb * xyz * permx[ix];
+    vv[e] = cc * xyz * permy[ix];
+    ww[e] = (dd * xyz - ckrwe[e]) * permz[ix];
+  }
+}
+
+void orc_vnod3d(int n, int nt, const int *tp, const int *tetra, const double *uu,
+                const double *vv, const double *ww, double *unod, double *vnod,
+                double *wnod) {
+  /* SRC/vnod3d.f:17-32: nodal means of the element velocities, summed in
+   * element order. */
+  for (int i = 0; i < n; ++i) unod[i] = vnod[i] = wnod[i] = 0.0;
+  for (int k = 0; k < nt; ++k)
+    for (int ii = 0; ii < 4; ++ii) {
+      int nd = tetra[(size_t)k * 5 + ii];
+      unod[nd] = unod[nd] + uu[k];
+      vnod[nd] = vnod[nd] + vv[k];
+      wnod[nd] = wnod[nd] + ww[k];
+    }
+  for (int i = 0; i < n; ++i) {
+    unod[i] = unod[i] / tp[i];
+    vnod[i] = vnod[i] / tp[i];
+    wnod[i] = wnod[i] / tp[i];
+  }
+}

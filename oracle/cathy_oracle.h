@@ -292,6 +292,18 @@ int orc_nsyslv(int n, int nterm, int numdir, const int *noddir, int itmxcg,
                double *coef3, double *scr, double *pnew, const double *tnoti,
                double *err, double *res, double *resini);
 
+/* ------------------------------------------------ Darcy velocities -- */
+/* SRC/vel3d.f: UU/VV/WW per element (perm* are [nstr][nzone]). */
+void orc_vel3d(int nt, int ntri, int nzone, const int *tetra, const double *pnew,
+               const double *permx, const double *permy, const double *permz,
+               double *uu, double *vv, double *ww, const double *bi,
+               const double *ci, const double *di, const double *ckrwe,
+               const double *volur, const int *ivol);
+/* SRC/vnod3d.f: nodal means over the tp[i] incident elements. */
+void orc_vnod3d(int n, int nt, const int *tp, const int *tetra, const double *uu,
+                const double *vv, const double *ww, double *unod, double *vnod,
+                double *wnod);
+
 /* --------------------------------------------- whole run (a1-a18) -- */
 typedef struct orc_params {
   /* input/parm (SRC/datin.f:80-122) */

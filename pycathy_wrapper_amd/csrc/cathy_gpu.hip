@@ -289,6 +289,8 @@ struct Flow {
              const double *permz, const double *snodi_, const double *pnodi_, const double *vgn_,
              const double *vgrmc_, const double *vgpsat_, const double *kznod_, int lump_, int ordering_) {
     n = n_; nt = nt_; ntri = ntri_; nnod = nnod_; nstr = nstr_; lump = lump_; ordering = ordering_;
+    nzone_ = nzone;
+    maxstr_ = maxstr;
     CG_REQUIRE(n > 0 && nt > 0 && nnod > 0 && nstr > 0, "empty mesh");
     CG_REQUIRE((long)nnod * (nstr + 1) == n, "N must equal NNOD*(NSTR+1) (gen3d.f numbering)");
     CG_REQUIRE((long)3 * ntri * nstr == nt, "NT must equal 3*NTRI*NSTR (gen3d.f)");
@@ -1517,6 +1519,55 @@ struct Flow {
     return deltat <= prm.dtmin ? 0 : 1;
   }
 
+  // VEL3D + VNOD3D on the current PNEW / CKRW (the step-end CHVELO
+  // values).  The per-element table (b, c, d, VOLUR, IVOL, Kx, Ky, Kz) is
+  // built from the caller's arrays on the first call and kept.
+  DBuf<double> vtab, e_uu, e_vv, e_ww, n_u, n_v, n_w;
+  void velocities(const int *tetra, const double *permx, const double *permy, const double *permz,
+                  const double *bi, const double *ci, const double *di, const double *volur, const int *ivol,
+                  double *uu, double *vv, double *ww, double *unod, double *vnod, double *wnod) {
+    if (!vtab.p) {
+      CG_REQUIRE(tetra && permx && permy && permz && bi && ci && di && volur && ivol,
+                 "the first velocity call needs TETRA, PERMX/Y/Z, BI/CI/DI, VOLUR and IVOL");
+      std::vector<double> h((size_t)nt * 17);
+      for (int e = 0; e < nt; ++e) {
+        const int istr = e / (3 * ntri), mt = tetra[5 * (size_t)e + 4] - 1;
+        CG_REQUIRE(mt >= 0 && mt < nzone_, "TETRA zone out of range");
+        const size_t ix = (size_t)mt * maxstr_ + istr;  // PERMX(MAXSTR, NZONE), column-major
+        double *t = &h[(size_t)e * 17];
+        for (int j = 0; j < 4; ++j) {
+          t[j] = bi[4 * (size_t)e + j];
+          t[4 + j] = ci[4 * (size_t)e + j];
+          t[8 + j] = di[4 * (size_t)e + j];
+        }
+        t[12] = volur[e];
+        t[13] = (double)ivol[e];
+        t[14] = permx[ix];
+        t[15] = permy[ix];
+        t[16] = permz[ix];
+      }
+      vtab.upload(h, st);
+      for (DBuf<double> *b : {&e_uu, &e_vv, &e_ww}) b->alloc(nt);
+      for (DBuf<double> *b : {&n_u, &n_v, &n_w}) b->alloc(n);
+    }
+    hipLaunchKernelGGL(k_vel3d, dim3(grid_for(nt)), dim3(kBlock), 0, st, nt, enode.p, pnew.p, ckrw.p, vtab.p,
+                       e_uu.p, e_vv.p, e_ww.p);
+    hipLaunchKernelGGL(k_vnod3d, dim3(grid_for(n)), dim3(kBlock), 0, st, n, inc_ptr.p, inc.p, e_uu.p, e_vv.p,
+                       e_ww.p, n_u.p, n_v.p, n_w.p);
+    if (uu) e_uu.download(uu, nt, st);
+    if (vv) e_vv.download(vv, nt, st);
+    if (ww) e_ww.download(ww, nt, st);
+    CG_HIP(hipStreamSynchronize(st));
+    std::vector<double> h(n);
+    for (auto pr : {std::make_pair(&n_u, unod), std::make_pair(&n_v, vnod), std::make_pair(&n_w, wnod)}) {
+      if (!pr.second) continue;
+      pr.first->download(h.data(), n, st);
+      CG_HIP(hipStreamSynchronize(st));
+      for (int i = 0; i < n; ++i) pr.second[i] = h[perm[i]];
+    }
+  }
+  int nzone_ = 0, maxstr_ = 0;
+
   void get_state(double *pn, double *swo, double *ck, int *fa) {
     std::vector<double> h(n);
     auto fetch = [&](const DBuf<double> &b, double *out) {
@@ -1665,6 +1716,18 @@ int cathy_gpu_flow3d_advance(int64_t *handle, double *stats) {
 int cathy_gpu_flow3d_get_state(int64_t *handle, double *pnew, double *sw, double *ckrw, int *ifatm) {
   return guarded([&] {
     lookup(handle)->get_state(pnew, sw, ckrw, ifatm);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_vel3d(int64_t *handle, const int *tetra, const double *permx, const double *permy,
+                           const double *permz, const double *bi, const double *ci, const double *di,
+                           const double *volur, const int *ivol, double *uu, double *vv, double *ww,
+                           double *unod, double *vnod, double *wnod) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(f->initialised, "flow3d_init not called");
+    f->velocities(tetra, permx, permy, permz, bi, ci, di, volur, ivol, uu, vv, ww, unod, vnod, wnod);
     return 0;
   });
 }

@@ -253,5 +253,9 @@ __global__ void k_bicg_update(int n, int nparts, const double *part_st, const do
 __global__ void k_bicg_check(int n, int nparts, const double *part_rr, const double *part_rtr, BicgScal *sc,
                              int *done);
 __global__ void k_sw_only(int n, const double *psi, SoilSoA s, double *sw);
+__global__ void k_vel3d(int nt, const int4 *enode, const double *pnew, const double *ckrw, const double *vt,
+                        double *uu, double *vv, double *ww);
+__global__ void k_vnod3d(int n, const int *inc_ptr, const int *inc, const double *uu, const double *vv,
+                         const double *ww, double *unod, double *vnod, double *wnod);
 
 }  // namespace cg

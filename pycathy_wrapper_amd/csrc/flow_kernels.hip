@@ -1101,4 +1101,54 @@ __global__ void k_fill(size_t n, double v, double *__restrict__ dst) {
   if (i < n) dst[i] = v;
 }
 
+// ----------------------------------------------------- velocities --
+// vel3d.f:28-52 per element: kr_e = mean nodal kr (nodelt.f), then
+//   UU = (sum psi_j b_j) (-kr_e VOLUR IVOL) Kx, VV likewise with c / Ky,
+//   WW = ((sum psi_j d_j) (-kr_e VOLUR IVOL) - kr_e) Kz.
+// vt: per element b[4], c[4], d[4], VOLUR, IVOL, Kx, Ky, Kz (stride 17).
+__global__ void k_vel3d(int nt, const int4 *__restrict__ enode, const double *__restrict__ pnew,
+                        const double *__restrict__ ckrw, const double *__restrict__ vt, double *__restrict__ uu,
+                        double *__restrict__ vv, double *__restrict__ ww) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+  const int4 v = enode[e];
+  const int nd[4] = {v.x, v.y, v.z, v.w};
+  const double *t = vt + (size_t)e * 17;
+  double kr = 0.0;
+  for (int j = 0; j < 4; ++j) kr = kr + ckrw[nd[j]];
+  kr = kr * 0.25;
+  double bb = 0.0, cc = 0.0, dd = 0.0;
+  const double xyz = -kr * t[12] * t[13];
+  for (int j = 0; j < 4; ++j) {
+    const double p = pnew[nd[j]];
+    bb = bb + p * t[j];
+    cc = cc + p * t[4 + j];
+    dd = dd + p * t[8 + j];
+  }
+  uu[e] = bb * xyz * t[14];
+  vv[e] = cc * xyz * t[15];
+  ww[e] = (dd * xyz - kr) * t[16];
+}
+
+// vnod3d.f:17-32: nodal means over the incident elements, summed in element order.
+__global__ void k_vnod3d(int n, const int *__restrict__ inc_ptr, const int *__restrict__ inc,
+                         const double *__restrict__ uu, const double *__restrict__ vv,
+                         const double *__restrict__ ww, double *__restrict__ unod, double *__restrict__ vnod,
+                         double *__restrict__ wnod) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double su = 0.0, sv = 0.0, sw = 0.0;
+  const int i0 = inc_ptr[r], i1 = inc_ptr[r + 1];
+  for (int i = i0; i < i1; ++i) {
+    const int e = inc[i] >> 2;
+    su = su + uu[e];
+    sv = sv + vv[e];
+    sw = sw + ww[e];
+  }
+  const double tp = (double)(i1 - i0);
+  unod[r] = su / tp;
+  vnod[r] = sv / tp;
+  wnod[r] = sw / tp;
+}
+
 }  // namespace cg

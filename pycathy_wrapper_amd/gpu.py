@@ -133,9 +133,34 @@ class FlowGPU:
         _check(lib().cathy_gpu_flow3d_get_state(C.byref(self.h), None, _p(out), None, None), "get_state")
         return out
 
+    def set_pnew(self, psi) -> None:
+        p = np.ascontiguousarray(psi, dtype=np.float64)
+        _check(lib().cathy_gpu_flow3d_set_pnew(C.byref(self.h), _p(p)), "set_pnew")
+
+    def ckrw(self) -> np.ndarray:
+        """Nodal kr; after advance() it is kr(PNEW) (chvelo.f)."""
+        out = np.zeros(self.n)
+        _check(lib().cathy_gpu_flow3d_get_state(C.byref(self.h), None, None, _p(out), None), "get_state")
+        return out
+
     def ifatm(self) -> np.ndarray:
         out = np.zeros(self.nnod, dtype=np.int32)
         _check(lib().cathy_gpu_flow3d_get_state(C.byref(self.h), None, None, None, _p(out)), "get_state")
+        return out
+
+    def velocity(self) -> dict:
+        """VEL3D + VNOD3D at the current state: element UU/VV/WW (NT) and
+        nodal UNOD/VNOD/WNOD (N, CATHY order)."""
+        k = self._keep
+        nt, n = len(k["volu"]), self.n
+        if "volur" not in k:
+            k["volur"] = np.ascontiguousarray(1.0 / k["volu"])
+        out = {name: np.zeros(nt) for name in ("uu", "vv", "ww")}
+        out.update({name: np.zeros(n) for name in ("unod", "vnod", "wnod")})
+        _check(lib().cathy_gpu_flow3d_vel3d(C.byref(self.h), _p(k["tet"]), _p(k["permx"]), _p(k["permy"]),
+                                            _p(k["permz"]), _p(k["bi"]), _p(k["ci"]), _p(k["di"]), _p(k["volur"]),
+                                            _p(k["ivol"]), *(_p(out[nm]) for nm in ("uu", "vv", "ww", "unod", "vnod",
+                                                                                   "wnod"))), "vel3d")
         return out
 
     def counters(self) -> dict:
