@@ -137,6 +137,67 @@ int cathy_gpu_flow3d_get_state(int64_t *handle, double *pnew, double *sw,
                                double *ckrw, int *ifatm);
 int cathy_gpu_flow3d_set_pnew(int64_t *handle, const double *pnew);
 
+/* ---------------------------------------------------------------------
+ * Learned Picard initial iterate (no counterpart in the reference).
+ *
+ * A per-column 1-D convolutional network (bf16 weights and activations,
+ * fp32 accumulation on MFMA) corrects the extrapolated increment of each
+ * soil column.  Per level it sees (standardised): psi_n, theta_n, log10 Ks,
+ * alpha, n, rainfall rate, log10 dt, depth below the surface and
+ * E = dt_n (psi_n - psi_{n-1}) / dt_{n-1}; each attempt after the first
+ * step starts its Picard loop from
+ *   psi_n + beta E + s_col * out * clamp(y, -10, 10),
+ *   s_col = rms_col(E) + floor
+ * instead of psi_n.  The Picard loop, its tolerance and its convergence
+ * test are unchanged.
+ *
+ * Network entries (host buffers, synchronous):
+ *   create   levels = NSTR+1 per column; *nparam receives 40385
+ *   set/get  params (fp32, per layer W[k][ci][co] then b[co]; layers
+ *            16->64, 64->64 x3, 64->1, kernel 3) and scale[35] = feature
+ *            means[16], standard deviations[16], out, floor (m), beta;
+ *            either pointer may be NULL
+ *   forward  x: bf16 bits [ncol][Lp][16] (Lp = levels rounded up to 16,
+ *            padded levels zero) -> y: fp32 [ncol][Lp]
+ *   gradient mean squared error against t [ncol][Lp] over live levels:
+ *            loss and its gradient [nparam]
+ *   load     a training set of nsamp columns (x as forward, t as gradient)
+ *   train    nsteps Adam steps (beta 0.9/0.999, eps 1e-8) on batches of
+ *            the loaded set; idx [nsteps][batch] sample numbers; loss[nsteps]
+ * ------------------------------------------------------------------- */
+int cathy_gpu_guess_create(int64_t *net, const int *levels, int *nparam);
+int cathy_gpu_guess_destroy(int64_t *net);
+int cathy_gpu_guess_set(int64_t *net, const float *params, const float *scale);
+int cathy_gpu_guess_get(int64_t *net, float *params, float *scale);
+int cathy_gpu_guess_forward(int64_t *net, const int *ncol, const uint16_t *x,
+                            float *y);
+int cathy_gpu_guess_gradient(int64_t *net, const int *ncol, const uint16_t *x,
+                             const float *t, float *loss, float *grad);
+int cathy_gpu_guess_load(int64_t *net, const int64_t *nsamp,
+                         const uint16_t *x, const float *t);
+int cathy_gpu_guess_train(int64_t *net, const int *nsteps, const int *batch,
+                          const int *idx, const float *lr, float *loss);
+
+/* Replay of step sizes (no counterpart in the reference; for comparing
+ * runs step by step): after init and before the first advance, dts[0] is
+ * step 1's DELTAT and dts[k] the first attempt's DELTAT of step k+1 in
+ * place of TIMUPD's rule (back-steps still reduce it; after nd entries the
+ * rule resumes). */
+int cathy_gpu_flow3d_set_dt_schedule(int64_t *handle, const int *nd,
+                                     const double *dts);
+
+/* Attach a network to a flow run (*net = 0 detaches; net NULL leaves it)
+ * and/or set the static column features stat[N][5] in CATHY node order:
+ * log10 Ks, alpha (1/VGPSAT), VGN, depth below the surface, porosity. */
+int cathy_gpu_flow3d_guess(int64_t *handle, const int64_t *net,
+                           const float *stat);
+/* Recording: with *on != 0 every attempt keeps its raw features and every
+ * accepted step its increment PNEW - PTIMEP; guess_sample returns those of
+ * the last accepted step: feat [NNOD][Lp][16], target [NNOD][Lp]. */
+int cathy_gpu_flow3d_guess_record(int64_t *handle, const int *on);
+int cathy_gpu_flow3d_guess_sample(int64_t *handle, float *feat,
+                                  float *target);
+
 /* VEL3D (vel3d.f) + VNOD3D (vnod3d.f), cathy_main.f:3243-3246: Darcy
  * velocities UU/VV/WW(NT) per element and their nodal means UNOD/VNOD/
  * WNOD(N) at the current PNEW and kr.  TETRA, PERMX/Y/Z(MAXSTR,NZONE),

@@ -124,6 +124,31 @@ def hillslope_small(seed: int = 1, params: Params | None = None, nstr: int = 10,
                    dict(cells=cells, dx=dx))
 
 
+def hillslope_small_member(seed: int, params: Params | None = None) -> Problem:
+    """One perturbed-parameter member of the small hillslope (same mesh,
+    initial state and settings): the soil stays uniform in space but its
+    parameters are drawn per member, as a data-assimilation ensemble draws
+    them: ln Ks ~ N(ln 1e-5, 0.5^2), alpha, n and porosity with 15%
+    relative Gaussian spread (clipped to n in [1.15, 2.5], porosity in
+    [0.25, 0.6]), and the rainfall rate 1e-7 m/s times a lognormal factor
+    (ln-sd 0.5).  Realisations for training and testing the learned
+    initial iterate.  (Spatially random fields on this mesh make the plain
+    solver's steps collapse to sub-second sizes, in the reference's
+    algorithm as here, so they are not used for training.)"""
+    base = hillslope_small(params=params)
+    rng = np.random.default_rng(20_000 + seed)
+    nstr = base.mesh.nstr
+    ks = sig4(np.exp(np.log(1e-5) + 0.5 * rng.standard_normal()))
+    alpha = max(0.3, 1.0 * (1.0 + 0.15 * rng.standard_normal()))
+    vgn = sig4(float(np.clip(1.46 * (1.0 + 0.15 * rng.standard_normal()), 1.15, 2.5)))
+    poros = sig4(float(np.clip(0.45 * (1.0 + 0.15 * rng.standard_normal()), 0.25, 0.6)))
+    rain = 1e-7 * float(np.exp(0.5 * rng.standard_normal()))
+    soil = homogeneous_soil(nstr, ks=float(ks), poros=float(poros), vgn=float(vgn), vgpsat=float(sig4(1.0 / alpha)))
+    return Problem("hillslope_small_member", base.mesh, soil, base.psi0, base.atm_times, np.array([rain]),
+                   base.params, seed, dict(base.meta, ks=float(ks), alpha=alpha, vgn=float(vgn),
+                                           poros=float(poros), rain=rain))
+
+
 def _spectral_dem(n: int, dx: float, relief: float, rng, beta: float = 3.0) -> np.ndarray:
     kx = np.fft.fftfreq(n, dx)
     k = np.sqrt(kx[:, None] ** 2 + kx[None, :] ** 2)

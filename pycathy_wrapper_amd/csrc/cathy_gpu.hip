@@ -24,6 +24,7 @@
 
 #include "cathy_gpu.h"
 #include "flow_kernels.h"
+#include "guess.h"
 
 namespace cg {
 
@@ -1420,6 +1421,8 @@ struct Flow {
     // SWTIMEP is a function of PTIMEP only
     hipLaunchKernelGGL(k_sw_kr, dim3(gn), dim3(kBlock), 0, st, n, ptimep.p, soil(), swtimep.p,
                        (double *)nullptr);
+    dtprev = 0.0;
+    if (pprev.p) CG_HIP(hipMemcpyAsync(pprev.p, ptimep.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     CG_HIP(hipStreamSynchronize(st));
     initialised = 1;
   }
@@ -1436,6 +1439,7 @@ struct Flow {
     nitert = 0;
     if (time > atmtim[1]) atmnxt(); else atmbak();
     switch_old();
+    guess_point();
     hipLaunchKernelGGL(k_start_iter, dim3(grid_for(n)), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
                        pold.p, ptnew.p, ptold.p);
   }
@@ -1449,6 +1453,7 @@ struct Flow {
     free_drainage(ckrw.p);
     atmnxt();
     switch_old();
+    guess_point();
     hipLaunchKernelGGL(k_start_iter, dim3(grid_for(n)), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
                        pold.p, ptnew.p, ptold.p);
     int backs = 0;
@@ -1481,6 +1486,11 @@ struct Flow {
     aninp = cur[4]; anoutp = cur[5]; nninp = cur[6]; nnoutp = cur[7];
     CG_HIP(hipMemcpyAsync(ifatmp.p, ifatm.p, sizeof(int) * nnod, hipMemcpyDeviceToDevice, st));
     CG_HIP(hipMemcpyAsync(atmold.p, atmact.p, sizeof(double) * nnod, hipMemcpyDeviceToDevice, st));
+    if (grecord) guess_target(st, nnod, gL, gLp, gcolrow.p, ptimep.p, pnew.p, gtarget.p);
+    if (pprev.p) {
+      CG_HIP(hipMemcpyAsync(pprev.p, ptimep.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      dtprev = deltat;
+    }
     CG_HIP(hipMemcpyAsync(ptimep.p, pnew.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     CG_HIP(hipMemcpyAsync(ckrwp.p, ckrw.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_sw_kr, dim3(grid_for(n)), dim3(kBlock), 0, st, n, ptimep.p, soil(), swtimep.p,
@@ -1499,8 +1509,31 @@ struct Flow {
   }
   double last_counters[6] = {0};
 
+  // Prescribed step sizes (replay of an earlier run's accepted DELTAT):
+  // entry k is the first attempt's dt of step k+2 (step 1's is set at
+  // once); back-steps still reduce it.  Past the end TIMUPD's rule resumes.
+  std::vector<double> dt_sched;
+  size_t dt_pos = 0;
+
+  void set_dt_schedule(const double *dts, int nd) {
+    CG_REQUIRE(initialised, "flow3d_init not called");
+    CG_REQUIRE(nstep == 1 && timep == 0.0, "a step-size schedule is set before the first step");
+    CG_REQUIRE(nd >= 1, "empty step-size schedule");
+    for (int i = 0; i < nd; ++i) CG_REQUIRE(dts[i] > 0.0, "step sizes must be > 0");
+    dt_sched.assign(dts, dts + nd);
+    deltat = dt_sched[0];
+    time = timep + deltat;
+    dt_pos = 1;
+  }
+
   int timupd() {  // timupd.f:21-40
     timep = time;
+    if (dt_pos < dt_sched.size()) {
+      deltat = dt_sched[dt_pos++];
+      if (time + deltat >= prm.tmax) { deltat = prm.tmax - time; time = prm.tmax; }
+      else time = time + deltat;
+      return deltat <= prm.dtmin ? 0 : 1;
+    }
     if (iter < prm.ituns1) {
       deltat = deltat * prm.dtmagm + prm.dtmaga;
       if (deltat > prm.dtmax) deltat = prm.dtmax;
@@ -1523,6 +1556,86 @@ struct Flow {
   // values).  The per-element table (b, c, d, VOLUR, IVOL, Kx, Ky, Kz) is
   // built from the caller's arrays on the first call and kept.
   DBuf<double> vtab, e_uu, e_vv, e_ww, n_u, n_v, n_w;
+
+  // Learned initial iterate.  Columns: surface node s, level l -> CATHY
+  // node l*NNOD + s.  At the start of every attempt (after SWITCH_OLD) the
+  // features of (PTIMEP, SWTIMEP, forcing, dt) are formed; with a network
+  // attached PNEW takes PTIMEP + the predicted increment on all but the
+  // Dirichlet surface nodes, and the Picard loop then runs unchanged.
+  std::shared_ptr<GuessNet> gnet;
+  bool grecord = false;
+  DBuf<double> pprev;   // PTIMEP of the previous accepted step
+  double dtprev = 0.0;  // its DELTAT (0: no previous step)
+  int gL = 0, gLp = 0;
+  DBuf<int> gcolrow;
+  DBuf<float> gstat, graw, gtarget, gy;
+  DBuf<unsigned short> gx;
+  long guess_calls = 0;
+
+  void set_guess(const int64_t *net, const float *stat) {
+    gL = nstr + 1;
+    gLp = (gL + 15) / 16 * 16;
+    if (stat) {
+      std::vector<int> cr((size_t)nnod * gL);
+      std::vector<float> sc((size_t)nnod * gL * 5);
+      for (int s = 0; s < nnod; ++s)
+        for (int l = 0; l < gL; ++l) {
+          const int node = l * nnod + s;
+          cr[(size_t)s * gL + l] = perm[node];
+          for (int c = 0; c < 5; ++c) sc[((size_t)s * gL + l) * 5 + c] = stat[(size_t)node * 5 + c];
+        }
+      for (int i = 0; i < nnod * gL; ++i)
+        CG_REQUIRE(sc[(size_t)i * 5 + 4] > 0.f, "column features: porosity must be > 0");
+      gcolrow.upload(cr, st);
+      gstat.upload(sc, st);
+      graw.alloc((size_t)nnod * gLp * kGuessIn);
+      gtarget.alloc((size_t)nnod * gLp);
+      gx.alloc((size_t)nnod * gLp * kGuessIn);
+      gy.alloc((size_t)nnod * gLp);
+      if (!pprev.p) {
+        pprev.alloc(n);
+        CG_HIP(hipMemcpyAsync(pprev.p, ptimep.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      }
+      CG_HIP(hipStreamSynchronize(st));
+    }
+    if (net) {
+      if (*net == 0) {
+        gnet.reset();
+      } else {
+        CG_REQUIRE(gstat.p, "column features not set");
+        auto g = guess_lookup(*net);
+        CG_REQUIRE(g->L == gL, "guess net levels differ from NSTR+1");
+        gnet = g;
+      }
+    }
+  }
+
+  void guess_point() {
+    if (!gnet && !grecord) return;
+    CG_REQUIRE(gstat.p, "column features not set");
+    GuessScale use{};
+    if (gnet) {
+      use = gnet->scale;
+    } else {
+      for (int c = 0; c < kGuessIn; ++c) { use.mu[c] = 0.f; use.sd[c] = 1.f; }
+    }
+    const double ratio = dtprev > 0.0 ? deltat / dtprev : 0.0;
+    guess_features(st, nnod, gL, gLp, gcolrow.p, ptimep.p, pprev.p, ratio, swtimep.p, gstat.p, atmpot.p, arenod.p,
+                   deltat, use, grecord ? graw.p : nullptr, gnet ? gx.p : nullptr);
+    if (!gnet || dtprev <= 0.0) return;  // the first step starts from PTIMEP
+    CG_HIP(hipStreamSynchronize(st));
+    gnet->forward(gx.p, nullptr, nnod, gy.p);
+    CG_HIP(hipStreamSynchronize(gnet->st));
+    guess_apply(st, nnod, gL, gLp, gcolrow.p, ifatm.p, ptimep.p, pprev.p, ratio, gy.p, use, pnew.p);
+    ++guess_calls;
+  }
+
+  void guess_sample(float *feat, float *target) {
+    CG_REQUIRE(grecord, "recording is off");
+    if (feat) graw.download(feat, (size_t)nnod * gLp * kGuessIn, st);
+    if (target) gtarget.download(target, (size_t)nnod * gLp, st);
+    CG_HIP(hipStreamSynchronize(st));
+  }
   void velocities(const int *tetra, const double *permx, const double *permy, const double *permz,
                   const double *bi, const double *ci, const double *di, const double *volur, const int *ivol,
                   double *uu, double *vv, double *ww, double *unod, double *vnod, double *wnod) {
@@ -1598,21 +1711,7 @@ static Flow *lookup(const int64_t *h) {
   return it->second.get();
 }
 
-template <typename F>
-static int guarded(F &&f) {
-  try {
-    return f();
-  } catch (const Error &e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    g_last_error = "host allocation failed";
-    return CATHY_GPU_ENOMEM;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return CATHY_GPU_EINVAL;
-  }
-}
+void set_last_error(const std::string &m) { g_last_error = m; }
 
 }  // namespace cg
 
@@ -1728,6 +1827,40 @@ int cathy_gpu_flow3d_vel3d(int64_t *handle, const int *tetra, const double *perm
     Flow *f = lookup(handle);
     CG_REQUIRE(f->initialised, "flow3d_init not called");
     f->velocities(tetra, permx, permy, permz, bi, ci, di, volur, ivol, uu, vv, ww, unod, vnod, wnod);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_set_dt_schedule(int64_t *handle, const int *nd, const double *dts) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(nd && dts, "null argument");
+    f->set_dt_schedule(dts, *nd);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_guess(int64_t *handle, const int64_t *net, const float *stat) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    f->set_guess(net, stat);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_guess_record(int64_t *handle, const int *on) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(on, "null argument");
+    CG_REQUIRE(!*on || f->gstat.p, "column features not set");
+    f->grecord = *on != 0;
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_guess_sample(int64_t *handle, float *feat, float *target) {
+  return guarded([&] {
+    lookup(handle)->guess_sample(feat, target);
     return 0;
   });
 }

@@ -81,6 +81,26 @@ struct DBuf {
   }
 };
 
+// Last error text of the calling thread (cathy_gpu_last_error).
+void set_last_error(const std::string &m);
+
+// Runs one C entry: exceptions become the entry's negative status code.
+template <typename F>
+int guarded(F &&f) {
+  try {
+    return f();
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return -3;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return -1;
+  }
+}
+
 inline int grid_for(size_t work, int block = kBlock, int cap = 1 << 30) {
   size_t g = (work + block - 1) / block;
   if (g < 1) g = 1;

@@ -133,6 +133,40 @@ class FlowGPU:
         _check(lib().cathy_gpu_flow3d_get_state(C.byref(self.h), None, _p(out), None, None), "get_state")
         return out
 
+    def guess(self, net=None, stat=None, detach: bool = False) -> None:
+        """Attach a learned-initial-iterate network (guess.GuessNet) and/or
+        set the static column features (guess.static_features)."""
+        st = None if stat is None else np.ascontiguousarray(stat, dtype=np.float32)
+        if st is not None:
+            self._keep["guess_stat"] = st
+        h = None
+        if net is not None:
+            h = C.byref(C.c_int64(net.h.value))
+            self._guess_net = net
+        elif detach:
+            h = C.byref(C.c_int64(0))
+            self._guess_net = None
+        _check(lib().cathy_gpu_flow3d_guess(C.byref(self.h), h, None if st is None else _p(st)), "flow3d_guess")
+
+    def set_dt_schedule(self, dts) -> None:
+        """Replay step sizes: dts[k] is the DELTAT of step k+1 (call after
+        construction, before the first advance)."""
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        _check(lib().cathy_gpu_flow3d_set_dt_schedule(C.byref(self.h), _i(d.size), _p(d)), "set_dt_schedule")
+
+    def guess_record(self, on: bool = True) -> None:
+        _check(lib().cathy_gpu_flow3d_guess_record(C.byref(self.h), _i(1 if on else 0)), "guess_record")
+
+    def guess_sample(self):
+        """Raw features [NNOD][Lp][16] and increments [NNOD][Lp] of the last
+        accepted step (recording on)."""
+        L = self.problem.mesh.nstr + 1
+        Lp = (L + 15) // 16 * 16
+        feat = np.zeros((self.nnod, Lp, 16), dtype=np.float32)
+        tgt = np.zeros((self.nnod, Lp), dtype=np.float32)
+        _check(lib().cathy_gpu_flow3d_guess_sample(C.byref(self.h), _p(feat), _p(tgt)), "guess_sample")
+        return feat, tgt
+
     def set_pnew(self, psi) -> None:
         p = np.ascontiguousarray(psi, dtype=np.float64)
         _check(lib().cathy_gpu_flow3d_set_pnew(C.byref(self.h), _p(p)), "set_pnew")
