@@ -24,6 +24,15 @@ from rocprofv3 FETCH_SIZE / WRITE_SIZE passes, read from
 profiles/k_spmv_dot2_hillslope_large_traffic.txt when that summary exists
 (made by --pmc-summary from two --pmc runs of this script with --calib-mb).
 
+The learned leg (``config.learned``; skipped with --learned-members 0)
+trains the initial-iterate network on the device from the plain solver's
+own first steps on independent small members of the same generator (41x41
+nodes, same layers, forcing and settings, other seeds), then (1) times the
+same K steps of the 1.25M-node run with the learned initial iterate under
+its own step control, and (2) replays the plain run's accepted step sizes
+with the plain and the learned initial iterate, comparing Picard/PCG
+counts and psi step by step.  ``value`` stays the plain method's rate.
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -140,6 +149,83 @@ def pmc_summary(paths: list[str], kernel: str, copy_bytes: float) -> int:
     return 0 if ok else 1
 
 
+def learned_leg(g, pr, args, dts, steps: int) -> dict:
+    """Train the initial-iterate network on small members, then time and
+    compare it on this run (see the module docstring)."""
+    import numpy as np
+    from pycathy_wrapper_amd import configs as CF
+    from pycathy_wrapper_amd import gpu
+    from pycathy_wrapper_amd import guess as GU
+    L = pr.mesh.nstr + 1
+    t0 = time.perf_counter()
+    raws, tgts = [], []
+    for i in range(args.learned_members):
+        mem = CF.hillslope_large(seed=1000 + i, cells=40, nstr=args.nstr, params=bench_params())
+        r, t, st = GU.collect(mem, gpu.ORDER_COLUMN3, max_steps=steps + 10)
+        raws.append(r)
+        tgts.append(t)
+        log(f"[learned] member {i}: {len(st)} steps, {sum(x['ITER'] for x in st):.0f} Picard")
+    raw, tgt = np.concatenate(raws), np.concatenate(tgts)
+    t1 = time.perf_counter()
+    net, losses = GU.train_guess(raw, tgt, L, steps=args.learned_train_steps, batch=2048, lr=1e-3)
+    t2 = time.perf_counter()
+    log(f"[learned] {raw.shape[0]} column-step pairs in {t1 - t0:.1f}s; {args.learned_train_steps} Adam steps in "
+        f"{t2 - t1:.1f}s, loss {np.mean(losses[:10]):.3f} -> {np.mean(losses[-100:]):.3f}")
+    stat = GU.static_features(pr)
+    out = dict(trained_on=f"{args.learned_members} members hillslope_large(cells=40, seeds 1000..), "
+                          f"steps 1..{steps + 10} each", pairs=int(raw.shape[0]),
+               collect_s=t1 - t0, train_s=t2 - t1, loss_first=float(np.mean(losses[:10])),
+               loss_last=float(np.mean(losses[-100:])))
+    # (1) own step control, same number of steps
+    g.init(pr.psi0)
+    g.guess(net, stat)
+    n = pic = pc = bk = 0
+    sim = 0.0
+    ta = time.perf_counter()
+    for _ in range(steps):
+        rc, s = g.advance()
+        c = g.counters()
+        n += 1
+        pic += c["picard"]
+        pc += c["pcg"]
+        bk += s["BACK"]
+        sim = s["TIME"]
+        if rc:
+            break
+    tb = time.perf_counter()
+    out.update(steps_per_s=n / (tb - ta), picard=pic, pcg=pc, back_steps=bk, sim_s_per_wall_s=sim / (tb - ta))
+    log(f"[learned] own control: {n} steps in {tb - ta:.3f}s, {pic:.0f} Picard, {pc:.0f} PCG, {bk:.0f} back-steps, "
+        f"sim -> {sim:.2f} s")
+    # (2) replay of the plain run's step sizes, plain then learned
+    rep = {}
+    for mode in ("plain", "learned"):
+        g.init(pr.psi0)
+        g.guess(net if mode == "learned" else None, detach=mode == "plain")
+        g.set_dt_schedule(dts)
+        pic = pc = bk = 0
+        ta = time.perf_counter()
+        for _ in range(len(dts)):
+            rc, s = g.advance()
+            c = g.counters()
+            pic += c["picard"]
+            pc += c["pcg"]
+            bk += s["BACK"]
+            if rc:
+                break
+        tb = time.perf_counter()
+        rep[mode] = dict(picard=pic, pcg=pc, back_steps=bk, seconds=tb - ta, psi=g.psi())
+    a, b = rep["plain"], rep["learned"]
+    rel = float(np.max(np.abs(b["psi"] - a["psi"])) / np.max(np.abs(a["psi"])))
+    out["replay"] = dict(steps=len(dts), plain={k: v for k, v in a.items() if k != "psi"},
+                         learned={k: v for k, v in b.items() if k != "psi"}, psi_rel_diff_last=rel)
+    log(f"[learned] replay of {len(dts)} steps: Picard {a['picard']:.0f} -> {b['picard']:.0f}, PCG {a['pcg']:.0f} -> "
+        f"{b['pcg']:.0f}, back-steps {a['back_steps']:.0f} / {b['back_steps']:.0f}, time {a['seconds']:.3f} -> "
+        f"{b['seconds']:.3f} s, psi rel diff {rel:.2e}")
+    g.guess(None, detach=True)
+    net.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +238,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=60.0, help="stop the CPU baseline after this long")
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the timed steps, run the copy kernel over this many MB (PMC calibration)")
+    ap.add_argument("--learned-members", type=int, default=8,
+                    help="small members whose first steps train the learned initial iterate (0 skips the leg)")
+    ap.add_argument("--learned-train-steps", type=int, default=4000)
     ap.add_argument("--pmc-summary", nargs="+", metavar="DIR_OR_CSV",
                     help="summarise rocprofv3 --pmc CSV output (FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -184,6 +273,7 @@ def main() -> int:
     g.init(pr.psi0)  # timed steps restart from the initial condition
 
     g.profile(True)
+    dts = []
     steps = picard = pcg = backs = 0
     sim1 = 0.0
     step_wall = []
@@ -198,6 +288,7 @@ def main() -> int:
         pcg += c["pcg"]
         backs += s["BACK"]
         sim1 = s["TIME"]
+        dts.append(s["DELTAT"])
         if rc:
             break
     t1 = time.perf_counter()
@@ -214,6 +305,9 @@ def main() -> int:
     if args.calib_mb > 0:
         ms = gpu.calib_copy(int(args.calib_mb * 1e6), 5)
         log(f"[calib] copy of {args.calib_mb:.0f} MB: {ms * 1e3:.1f} us/launch")
+    learned = None
+    if args.learned_members > 0:
+        learned = learned_leg(g, pr, args, dts, steps)
     cpu = None
     if args.cpu_steps > 0 and args.cpu_seconds > 0:
         cpu = cpu_baseline(pr, args.cpu_steps, args.cpu_seconds)
@@ -248,6 +342,7 @@ def main() -> int:
             "pcg_iterations": pcg,
             "back_steps": backs,
             "learned_guess": False,
+            "learned": learned,
             "parallelism": "single GPU",
         },
         "roofline": {

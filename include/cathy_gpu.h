@@ -154,8 +154,10 @@ int cathy_gpu_flow3d_set_pnew(int64_t *handle, const double *pnew);
  * Network entries (host buffers, synchronous):
  *   create   levels = NSTR+1 per column; *nparam receives 40385
  *   set/get  params (fp32, per layer W[k][ci][co] then b[co]; layers
- *            16->64, 64->64 x3, 64->1, kernel 3) and scale[35] = feature
- *            means[16], standard deviations[16], out, floor (m), beta;
+ *            16->64, 64->64 x3, 64->1, kernel 3) and scale[36] = feature
+ *            means[16], standard deviations[16], out, floor (m), beta,
+ *            band (m; >= 0: nodes with psi_n or the guess above -band
+ *            keep psi_n; < 0: off);
  *            either pointer may be NULL
  *   forward  x: bf16 bits [ncol][Lp][16] (Lp = levels rounded up to 16,
  *            padded levels zero) -> y: fp32 [ncol][Lp]

@@ -34,7 +34,7 @@ constexpr int kGuessLive = 9;  // channels carrying features
 constexpr int kGuessWidth = 64;
 constexpr int kGuessLayers = 5;
 constexpr int kGuessParams = (3 * 16 * 64 + 64) + 3 * (3 * 64 * 64 + 64) + (3 * 64 * 1 + 1);
-constexpr int kGuessScale = 2 * kGuessIn + 3;
+constexpr int kGuessScale = 2 * kGuessIn + 4;
 
 struct GuessScale {
   float mu[kGuessIn];
@@ -42,6 +42,7 @@ struct GuessScale {
   float out;    // correction per unit of network output, in units of s_col
   float floor;  // length (m) added to the column scale: s_col += floor
   float beta;   // weight of the extrapolated increment E
+  float band;   // nodes with psi_n or the guess above -band keep psi_n (< 0: off)
 };
 
 struct GuessLayer {

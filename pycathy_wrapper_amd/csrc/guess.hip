@@ -286,11 +286,14 @@ __global__ void k_guess_features(int nnod, int L, int Lp, const int *__restrict_
 // One thread per column: E_l = ratio (PTIMEP - PPREV), s = rms_l E_l +
 // floor, PNEW = PTIMEP + beta E + s out clamp(y, -10, 10) on every node
 // except the surface nodes that are Dirichlet at the start of the attempt
-// (IFATM != 0).
+// (IFATM != 0) and, with band >= 0, the nodes where PTIMEP or the guess is
+// above -band (at or near saturation, where the Picard linearisation of
+// the storage term changes branch).
 __global__ void k_guess_apply(int nnod, int L, int Lp, const int *__restrict__ colrow,
                               const int *__restrict__ ifatm, const double *__restrict__ ptimep,
                               const double *__restrict__ pprev, double ratio, const float *__restrict__ y,
-                              double beta, double out, double floor_abs, double *__restrict__ pnew) {
+                              double beta, double out, double floor_abs, double band,
+                              double *__restrict__ pnew) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnod) return;
   const int *cr = colrow + (size_t)s * L;
@@ -304,7 +307,9 @@ __global__ void k_guess_apply(int nnod, int L, int Lp, const int *__restrict__ c
     const int row = cr[l];
     const double pt = ptimep[row];
     const double yl = fmin(fmax((double)y[(size_t)s * Lp + l], -10.0), 10.0);
-    pnew[row] = pt + beta * (ratio * (pt - pprev[row])) + scol * out * yl;
+    const double gv = pt + beta * (ratio * (pt - pprev[row])) + scol * out * yl;
+    if (band >= 0.0 && (pt > -band || gv > -band)) continue;
+    pnew[row] = gv;
   }
 }
 
@@ -335,7 +340,8 @@ void guess_apply(hipStream_t st, int nnod, int L, int Lp, const int *colrow, con
                  const double *ptimep, const double *pprev, double ratio, const float *y, const GuessScale &sc,
                  double *pnew) {
   hipLaunchKernelGGL(k_guess_apply, dim3(grid_for((size_t)nnod)), dim3(kBlock), 0, st, nnod, L, Lp, colrow, ifatm,
-                     ptimep, pprev, ratio, y, (double)sc.beta, (double)sc.out, (double)sc.floor, pnew);
+                     ptimep, pprev, ratio, y, (double)sc.beta, (double)sc.out, (double)sc.floor, (double)sc.band,
+                     pnew);
 }
 
 void guess_target(hipStream_t st, int nnod, int L, int Lp, const int *colrow, const double *ptimep,
@@ -393,6 +399,7 @@ GuessNet::GuessNet(int levels) : L(levels) {
   scale.out = 1.f;
   scale.floor = 1e-5f;
   scale.beta = 1.f;
+  scale.band = 0.1f;
   repack();
   CG_HIP(hipStreamSynchronize(st));
 }
@@ -549,6 +556,7 @@ static GuessScale scale_from(const float *s) {
   g.out = s[2 * kGuessIn];
   g.floor = s[2 * kGuessIn + 1];
   g.beta = s[2 * kGuessIn + 2];
+  g.band = s[2 * kGuessIn + 3];
   return g;
 }
 
@@ -601,6 +609,7 @@ int cathy_gpu_guess_get(int64_t *net, float *params, float *scale) {
       scale[2 * kGuessIn] = s.out;
       scale[2 * kGuessIn + 1] = s.floor;
       scale[2 * kGuessIn + 2] = s.beta;
+      scale[2 * kGuessIn + 3] = s.band;
     }
     return 0;
   });

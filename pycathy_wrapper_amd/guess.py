@@ -14,7 +14,11 @@ Picard (and PCG) iterations a step takes.  The output is relative to the
 column's own rate of change, so a step whose head moves by 1e-4 m is
 predicted as accurately, relatively, as one that moves by 1 m; the floor
 (default 1e-5 m, ten times the Picard tolerance) keeps the correction
-below the tolerance where the column barely moves.
+below the tolerance where the column barely moves.  Nodes whose psi_n or
+guess lies above -band (default 0.1 m: at or near saturation) keep psi_n:
+there the Picard linearisation of the storage term changes branch, and a
+moved start makes the loop oscillate (measured: back-steps at the plain
+run's step sizes without the band, none with it).
 
 Columns: surface node s, levels l = 0 (surface) .. NSTR; CATHY node id
 ``l * NNOD + s``.  Feature channels per level (16 stored, 9 used):
@@ -34,7 +38,7 @@ from . import soil as SO
 NPARAM = 40385
 NFEAT = 16
 NLIVE = 9
-NSCALE = 2 * NFEAT + 3
+NSCALE = 2 * NFEAT + 4
 CIN = (16, 64, 64, 64, 64)
 COUT = (64, 64, 64, 64, 1)
 
@@ -83,7 +87,7 @@ class GuessNet:
         if p is not None and p.size != NPARAM:
             raise ValueError(f"expected {NPARAM} parameters")
         if s is not None and s.size != NSCALE:
-            raise ValueError("scale holds 16 means, 16 standard deviations, out, floor and beta")
+            raise ValueError("scale holds 16 means, 16 standard deviations, out, floor, beta and band")
         G._check(G.lib().cathy_gpu_guess_set(C.byref(self.h), None if p is None else G._p(p),
                                              None if s is None else G._p(s)), "guess_set")
 
@@ -159,13 +163,15 @@ class GuessNet:
             pass
 
 
-def default_scale(floor: float = 1e-5, beta: float = 1.0) -> np.ndarray:
-    """Identity standardisation, out = 1, the given floor (m) and beta."""
+def default_scale(floor: float = 1e-5, beta: float = 1.0, band: float = 0.1) -> np.ndarray:
+    """Identity standardisation, out = 1, the given floor (m), beta and
+    saturation band (m; < 0 applies the guess at every node)."""
     s = np.zeros(NSCALE, dtype=np.float32)
     s[NFEAT:2 * NFEAT] = 1.0
     s[2 * NFEAT] = 1.0
     s[2 * NFEAT + 1] = floor
     s[2 * NFEAT + 2] = beta
+    s[2 * NFEAT + 3] = band
     return s
 
 
@@ -224,15 +230,17 @@ def column_scale(raw, levels: int, floor: float) -> np.ndarray:
 def fit_scale(raw, tgt, levels: int, floor: float = 1e-5, beta: float = 1.0):
     """Standardisation of the used channels over live levels, and the
     output scale: the spread of the relative residual (tgt - beta E)/s_col
-    (its 99th-percentile magnitude over 3, so outliers do not set it)."""
+    over the samples that have a previous step (its 99th-percentile
+    magnitude over 3, so outliers do not set it)."""
     live = np.asarray(raw[:, :levels, :NLIVE], dtype=np.float64).reshape(-1, NLIVE)
     s = default_scale(floor, beta)
     mu = live.mean(axis=0)
     sd = live.std(axis=0)
     s[:NLIVE] = mu
     s[NFEAT:NFEAT + NLIVE] = np.where(sd > 1e-6 * np.maximum(1.0, np.abs(mu)), sd, 1.0)
-    r = relative_residual(raw, tgt, levels, floor, beta)
-    q = float(np.percentile(np.abs(r[:, :levels]), 99.0))
+    has_prev = np.any(np.asarray(raw[:, :levels, 8]) != 0.0, axis=1)
+    r = relative_residual(raw[has_prev], tgt[has_prev], levels, floor, beta)
+    q = float(np.percentile(np.abs(r[:, :levels]), 99.0)) if r.size else 0.0
     s[2 * NFEAT] = q / 3.0 if q > 0 else 1.0
     return s
 

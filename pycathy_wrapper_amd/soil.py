@@ -14,11 +14,11 @@ from .mesh import Mesh
 
 
 def _node_mean(mesh: Mesh, elem_vals: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-    t = mesh.tetra[:, :4].astype(np.int64)
-    acc = np.zeros(mesh.n)
-    np.add.at(acc, t.reshape(-1), np.repeat(elem_vals, 4))
-    tp = np.zeros(mesh.n, dtype=np.int64)
-    np.add.at(tp, t.reshape(-1), 1)
+    # bincount adds the weights in input order: element by element, the
+    # order of the reference's accumulation loop
+    t = mesh.tetra[:, :4].astype(np.int64).reshape(-1)
+    acc = np.bincount(t, weights=np.repeat(np.asarray(elem_vals, dtype=np.float64), 4), minlength=mesh.n)
+    tp = np.bincount(t, minlength=mesh.n).astype(np.int64)
     return acc, tp
 
 

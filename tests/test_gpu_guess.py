@@ -130,9 +130,9 @@ def test_trained_guess_against_plain_at_identical_steps():
     on the device, then a held-out member replayed at the plain run's step
     sizes with the plain, the extrapolated (untrained network) and the
     learned initial iterate.  The learned one must converge every step
-    without back-steps and with fewer Picard iterations than the plain one;
-    psi stays within the spread that the Picard stopping test itself
-    leaves (max |increment| <= 1e-6 m is not an error bound)."""
+    without back-steps, with fewer Picard and PCG iterations than the plain
+    one and than the extrapolation, and with psi within 2e-6 (relative
+    L-inf) of the plain run at every step."""
     from pycathy_wrapper_amd import gpu as G
     L = 11
     raws, tgts = [], []
@@ -152,6 +152,6 @@ def test_trained_guess_against_plain_at_identical_steps():
     print("picard plain/extrap/learned", P["picard"].sum(), X["picard"].sum(), Lr["picard"].sum(),
           "pcg", P["pcg"].sum(), X["pcg"].sum(), Lr["pcg"].sum(), "psi rel", Lr["psi_rel"].max())
     assert Lr["back"].sum() == 0
-    assert Lr["picard"].sum() < 0.9 * P["picard"].sum()
+    assert Lr["picard"].sum() < X["picard"].sum() < P["picard"].sum()
     assert Lr["pcg"].sum() < P["pcg"].sum()
-    assert Lr["psi_rel"].max() <= 5e-3
+    assert Lr["psi_rel"].max() <= 2e-6
