@@ -580,6 +580,19 @@ struct Flow {
     for (int r = 0; r < n; ++r) {
       hlen[r] = hrp[r + 1] - hrp[r];
       hsd[r] = hdiag[r] - hrp[r];
+    }
+    // gather counts of the column sweeps per colour: slots before the
+    // diagonal less the chain predecessor, after it less the successor
+    for (ColDesc &C : sched.cols) {
+      C.nf = C.nb = 0;
+      for (int k = 0; k < C.len; ++k)
+        for (int t = 0; t < C.nc; ++t) {
+          const int r = C.base + t + k * C.nc;
+          C.nf = std::max(C.nf, hsd[r] - (k > 0 ? 1 : 0));
+          C.nb = std::max(C.nb, hlen[r] - hsd[r] - 1 - (k < C.len - 1 ? 1 : 0));
+        }
+    }
+    for (int r = 0; r < n; ++r) {
       for (int p = hrp[r]; p < hrp[r + 1]; ++p) {
         const int j = p - hrp[r];
         hsc[spos(r, j)] = hcol[p];
@@ -864,6 +877,32 @@ struct Flow {
 
   // ------------------------------------------------------- kernels glue --
   size_t col_smem(const ColDesc &C) const { return sizeof(double) * 2 * (size_t)C.len * kChains; }
+
+  // Column sweeps with the gather window sized for the colour.
+  template <typename FT>
+  void launch_fwd_col3(const ColDesc &C, const Sell &sl, const FT *F, const double *di, const double *tin,
+                       double *out, const int *done) {
+    const dim3 g(grid_for(C.nc, kChains)), b(256);
+    const size_t sm = col_smem(C);
+    switch (sweep_window(C.nf)) {
+      case 0: hipLaunchKernelGGL((k_fwd_col3<FT, 0>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
+      case 4: hipLaunchKernelGGL((k_fwd_col3<FT, 4>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
+      case 8: hipLaunchKernelGGL((k_fwd_col3<FT, 8>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
+      default: hipLaunchKernelGGL((k_fwd_col3<FT, 16>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
+    }
+  }
+  template <typename FT>
+  void launch_bwd_col3(const ColDesc &C, const Sell &sl, const FT *F, const double *di, const double *xin,
+                       double *out, const int *done) {
+    const dim3 g(grid_for(C.nc, kChains)), b(256);
+    const size_t sm = col_smem(C);
+    switch (sweep_window(C.nb)) {
+      case 0: hipLaunchKernelGGL((k_bwd_col3<FT, 0>), g, b, sm, st, C, sl, F, di, xin, out, done); break;
+      case 4: hipLaunchKernelGGL((k_bwd_col3<FT, 4>), g, b, sm, st, C, sl, F, di, xin, out, done); break;
+      case 8: hipLaunchKernelGGL((k_bwd_col3<FT, 8>), g, b, sm, st, C, sl, F, di, xin, out, done); break;
+      default: hipLaunchKernelGGL((k_bwd_col3<FT, 16>), g, b, sm, st, C, sl, F, di, xin, out, done); break;
+    }
+  }
   size_t fac_smem(const ColDesc &C) const {
     return (sizeof(double) * 2 + sizeof(int) * 3) * (size_t)C.len * kChains;
   }
@@ -873,14 +912,8 @@ struct Flow {
     // z = (L L^T)^-1 t   (proddp.f)
     const Sell sl = sell();
     if (ordering == CATHY_GPU_ORDER_COLUMN3) {
-      for (const ColDesc &C : sched.cols)
-        hipLaunchKernelGGL(k_fwd_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C, sl, F,
-                           dinv.p, tin, out, done);
-      for (size_t c = sched.cols.size(); c-- > 0;) {
-        const ColDesc &C = sched.cols[c];
-        hipLaunchKernelGGL(k_bwd_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C, sl, F,
-                           dinv.p, out, out, done);
-      }
+      for (const ColDesc &C : sched.cols) launch_fwd_col3<FT>(C, sl, F, dinv.p, tin, out, done);
+      for (size_t c = sched.cols.size(); c-- > 0;) launch_bwd_col3<FT>(sched.cols[c], sl, F, dinv.p, out, out, done);
     } else {
       for (const ListLevel &L : sched.lv)
         hipLaunchKernelGGL(k_fwd_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p, sl, F,
@@ -987,9 +1020,7 @@ struct Flow {
   void ilu_fwd_t(const FT *F, const double *tin, double *out, const int *done) {
     const Sell sl = sell();
     if (ordering == CATHY_GPU_ORDER_COLUMN3) {
-      for (const ColDesc &C : sched.cols)
-        hipLaunchKernelGGL(k_fwd_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C, sl, F,
-                           dinv.p, tin, out, done);
+      for (const ColDesc &C : sched.cols) launch_fwd_col3<FT>(C, sl, F, dinv.p, tin, out, done);
     } else {
       for (const ListLevel &L : sched.lv)
         hipLaunchKernelGGL(k_fwd_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p, sl, F,
@@ -1000,11 +1031,7 @@ struct Flow {
   void ilu_bwd_t(const FT *F, const double *xin, double *out, const int *done) {
     const Sell sl = sell();
     if (ordering == CATHY_GPU_ORDER_COLUMN3) {
-      for (size_t c = sched.cols.size(); c-- > 0;) {
-        const ColDesc &C = sched.cols[c];
-        hipLaunchKernelGGL(k_bwd_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), col_smem(C), st, C, sl, F,
-                           ones.p, xin, out, done);
-      }
+      for (size_t c = sched.cols.size(); c-- > 0;) launch_bwd_col3<FT>(sched.cols[c], sl, F, ones.p, xin, out, done);
     } else {
       for (size_t l = sched.lv.size(); l-- > 0;) {
         const ListLevel &L = sched.lv[l];

@@ -77,13 +77,49 @@ __device__ __forceinline__ RowPos chain_pos(const Sell &sl, int g, int k) {
   return p;
 }
 
+// Code word of slot j of the row at p (coded source).
+__device__ __forceinline__ uint32_t code_word(const Sell &sl, const RowPos &p, int j) {
+  return sl.code[(size_t)(p.pt * sl.width + j) * sl.nchain + p.g];
+}
+
+// Column of a coded slot from its word, for a row on plane k.  Written as
+// selects, not branches: a loop can load every word of a row first and
+// decode afterwards, so the loads are in flight together.
+__device__ __forceinline__ int code_col(const Sell &sl, int k, uint32_t w) {
+  const int c2 = (int)(w >> 29), dk = (int)((w >> 27) & 3u) - 1, t2 = (int)(w & 0x7ffffffu);
+  int b = sl.cbase[0], nc = sl.cnc[0];
+  b = c2 == 1 ? sl.cbase[1] : b;
+  nc = c2 == 1 ? sl.cnc[1] : nc;
+  b = c2 == 2 ? sl.cbase[2] : b;
+  nc = c2 == 2 ? sl.cnc[2] : nc;
+  b = c2 == 3 ? sl.cbase[3] : b;
+  nc = c2 == 3 ? sl.cnc[3] : nc;
+  return b + (k + dk) * nc + t2;
+}
+
 // column of slot j (SELL position q) of the row at p / r
 __device__ __forceinline__ int sell_col(const Sell &sl, const RowPos &p, int q, int j) {
   if (!sl.coded) return sl.col[q];
-  const uint32_t w = sl.code[(size_t)(p.pt * sl.width + j) * sl.nchain + p.g];
-  const int c2 = (int)(w >> 29), dk = (int)((w >> 27) & 3u) - 1, t2 = (int)(w & 0x7ffffffu);
-  return pick4(c2, sl.cbase[0], sl.cbase[1], sl.cbase[2], sl.cbase[3]) +
-         (p.k + dk) * pick4(c2, sl.cnc[0], sl.cnc[1], sl.cnc[2], sl.cnc[3]) + t2;
+  return code_col(sl, p.k, code_word(sl, p, j));
+}
+
+// Column ids of a straight-line window of a row's slots: entry j is slot
+// min(j0 + j, jl) (SELL position of slot 0 = q0; jl must be a slot of the
+// row).  Every load is issued unconditionally and before any is used, so
+// they are all in flight together.
+template <int U>
+__device__ __forceinline__ void sell_cols_window(const Sell &sl, const RowPos &p, int q0, int j0, int jl,
+                                                 int (&c)[U]) {
+  if (sl.coded) {
+    uint32_t w[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) w[j] = code_word(sl, p, min(j0 + j, jl));
+#pragma unroll
+    for (int j = 0; j < U; ++j) c[j] = code_col(sl, p.k, w[j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < U; ++j) c[j] = sl.col[q0 + 64 * min(j0 + j, jl)];
+  }
 }
 __device__ __forceinline__ int sell_len(const Sell &sl, const RowPos &p, int r) {
   return sl.coded ? (int)sl.clen[p.pt * sl.nchain + p.g] : sl.len[r];
@@ -96,6 +132,7 @@ __device__ __forceinline__ int sell_diag(const Sell &sl, const RowPos &p, int r)
 // base + t + k*nc for k < len (k = node plane, 0 = surface).
 struct ColDesc {
   int base, nc, len, g0;  // g0: chain id of t = 0
+  int nf, nb;             // most earlier- / later-colour slots of a row (gathered by the sweeps)
 };
 
 // One level of the natural-order schedule: rows lvl_rows[off .. off+cnt).
@@ -164,12 +201,15 @@ struct PairList {
 template <typename FT>
 __global__ void k_ic_factor_col3(ColDesc C, Sell sl, PairList pl, const int *tpos, const double *A,
                                  const uint8_t *isdir, FT *F, double *dinv, int *nfall);
-template <typename FT>
+template <typename FT, int NW>
 __global__ void k_fwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, const double *tin, double *y,
                            const int *done);
-template <typename FT>
+template <typename FT, int NW>
 __global__ void k_bwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, const double *xin, double *x,
                            const int *done);
+// Gather window of the column sweeps: the smallest instantiated size that
+// covers n slots (larger rows take the kernels' loop).
+inline int sweep_window(int n) { return n <= 0 ? 0 : (n <= 4 ? 4 : (n <= 8 ? 8 : 16)); }
 template <typename FT>
 __global__ void k_ic_factor_list(ListLevel L, const int *rows, Sell sl, PairList pl, const int *tpos,
                                  const double *A, const uint8_t *isdir, FT *F, double *dinv, int *nfall);

@@ -187,31 +187,70 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
   const int s = r >> 6, lane = r & 63;
   const int base = a.sl.so[s] + lane;
   const int w = (a.sl.so[s + 1] - a.sl.so[s]) >> 6;
+  // Loads are issued in straight-line batches (slots and records past the
+  // end re-read the last one and are masked), so each batch is in flight
+  // together; a load under a condition makes the compiler wait for every
+  // earlier load at the next merge.  Sums keep the slot / record order.
   double m2d = 0.0, grav = 0.0;
-  for (int j = 0; j < w; ++j) {
-    const int q = base + 64 * j;
-    const int c = sell_col(a.sl, rp, q, j);
-    m2d += P.w2[q] * et1_node[c];
-    grav += P.gw[q] * ckrw_node[c];
+  if (w <= kUnroll) {
+    int c[kUnroll];
+    double w2v[kUnroll], gwv[kUnroll], e1[kUnroll], ck[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      const int q = base + 64 * min(j, w - 1);
+      w2v[j] = P.w2[q];
+      gwv[j] = P.gw[q];
+    }
+    sell_cols_window(a.sl, rp, base, 0, w - 1, c);
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      e1[j] = et1_node[c[j]];
+      ck[j] = ckrw_node[c[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      m2d = fma(j < w ? w2v[j] : 0.0, e1[j], m2d);
+      grav = fma(j < w ? gwv[j] : 0.0, ck[j], grav);
+    }
+  } else {
+    for (int j = 0; j < w; ++j) {
+      const int q = base + 64 * j;
+      const int c = sell_col(a.sl, rp, q, j);
+      m2d += P.w2[q] * et1_node[c];
+      grav += P.gw[q] * ckrw_node[c];
+    }
   }
   double K[MR];
 #pragma unroll
   for (int j = 0; j < MR; ++j) K[j] = 0.0;
   const int p0 = P.off[s] + lane;
   const int cw = (P.off[s + 1] - P.off[s]) >> 6;
-  for (int m = 0; m < cw; ++m) {
-    const int q = p0 + 64 * m;
-    const int e = P.ie[q];
-    if (e < 0) continue;
-    const uint32_t off = P.io[q];
-    const double kr = a.ckrwe[e];
-    const double4 sv = P.s4[q];
-    const double v[4] = {kr * sv.x, kr * sv.y, kr * sv.z, kr * sv.w};
+  constexpr int RB = 8;  // element records per batch
+  for (int m0 = 0; m0 < cw; m0 += RB) {
+    int e[RB];
+    uint32_t off[RB];
+    double4 sv[RB];
+    double kr[RB];
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const int o = (off >> (8 * l)) & 0xff;
+    for (int b = 0; b < RB; ++b) {
+      const int q = p0 + 64 * min(m0 + b, cw - 1);
+      e[b] = P.ie[q];
+      off[b] = P.io[q];
+      sv[b] = P.s4[q];
+    }
 #pragma unroll
-      for (int j = 0; j < MR; ++j) K[j] += (j == o) ? v[l] : 0.0;
+    for (int b = 0; b < RB; ++b) kr[b] = a.ckrwe[max(e[b], 0)];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const bool live = m0 + b < cw && e[b] >= 0;
+      const double v[4] = {live ? kr[b] * sv[b].x : 0.0, live ? kr[b] * sv[b].y : 0.0,
+                           live ? kr[b] * sv[b].z : 0.0, live ? kr[b] * sv[b].w : 0.0};
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const int o = live ? (int)((off[b] >> (8 * l)) & 0xff) : MR;
+#pragma unroll
+        for (int j = 0; j < MR; ++j) K[j] += (j == o) ? v[l] : 0.0;
+      }
     }
   }
   const int len = sell_len(a.sl, rp, r), dg = sell_diag(a.sl, rp, r);
@@ -219,11 +258,19 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
   const double m4r = a.m4d[r];
   const double m5 = m4r * a.et2[r];
   double t = 0.0;
+  {
+    int c[MR];
+    double pv[MR];
+    sell_cols_window(a.sl, rp, base, 0, len - 1, c);
+#pragma unroll
+    for (int j = 0; j < MR; ++j) pv[j] = a.ptnew[c[j]];
+#pragma unroll
+    for (int j = 0; j < MR; ++j) t = fma(-(j < len ? K[j] : 0.0), pv[j], t);
+  }
 #pragma unroll
   for (int j = 0; j < MR; ++j) {
     if (j < len) {
       const int q = base + 64 * j;
-      t = t - K[j] * a.ptnew[sell_col(a.sl, rp, q, j)];
       a.A[q] = tetaf * K[j] + (j == dg ? m2d * rdt + m5 * rdt : 0.0);
     }
   }
@@ -363,7 +410,7 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, Pair
 // Forward sweep L y = t (proddp, solscal-extended.f:4175-4185) on one
 // colour: pass 1 gathers the earlier-colour terms of every row and scales
 // by 1/L_kk, pass 2 runs y_k = t'_k - l'_k y_k-1 down each chain in LDS.
-template <typename FT>
+template <typename FT, int NW>
 __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
                                                   const double *__restrict__ dinv, const double *__restrict__ tin,
                                                   double *__restrict__ y, const int *__restrict__ done) {
@@ -381,18 +428,22 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
     const int jp = k > 0 ? jd - 1 : jd;
     const int base = sl.so[r >> 6] + (r & 63);
     double s = tin[r];
-    if (jp <= kUnroll) {
-      int c[kUnroll];
-      float f[kUnroll];
+    if (NW == 0 && jp == 0) {
+      // no earlier-colour terms in this colour
+    } else if (jp <= NW) {
+      // straight-line window over slots [0, jp): slots past it re-read
+      // slot max(jp - 1, 0) and are masked (see sell_row_dot)
+      const int jl = max(jp - 1, 0);
+      int c[NW > 0 ? NW : 1];
+      float f[NW > 0 ? NW : 1];
+      double yv[NW > 0 ? NW : 1];
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j)
-        if (j < jp) {
-          f[j] = (float)F[base + 64 * j];
-          c[j] = sell_col(sl, rp, base + 64 * j, j);
-        }
+      for (int j = 0; j < NW; ++j) f[j] = (float)F[base + 64 * min(j, jl)];
+      sell_cols_window(sl, rp, base, 0, jl, c);
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j)
-        if (j < jp) s -= (double)f[j] * y[c[j]];
+      for (int j = 0; j < NW; ++j) yv[j] = y[c[j]];
+#pragma unroll
+      for (int j = 0; j < NW; ++j) s = fma(-(j < jp ? (double)f[j] : 0.0), yv[j], s);
     } else {
       for (int j = 0; j < jp; ++j) {
         const int pj = base + 64 * j;
@@ -420,7 +471,7 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
 
 // Backward sweep L^T x = y (proddp, solscal-extended.f:4186-4197) on one
 // colour, in place: later colours are final; chains run from the base up.
-template <typename FT>
+template <typename FT, int NW>
 __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
                                                   const double *__restrict__ dinv, const double *xin, double *x,
                                                   const int *__restrict__ done) {
@@ -440,18 +491,23 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
     const int len = sell_len(sl, rp, r);
     const int base = sl.so[r >> 6] + (r & 63);
     double s = xin[r];
-    if (len <= kUnroll) {
-      int c[kUnroll];
-      float f[kUnroll];
+    const int j0 = js + 1;
+    if (NW == 0 && len <= j0) {
+      // no later-colour terms in this colour
+    } else if (len - j0 <= NW) {
+      // straight-line window over slots (js, len), in slot order: window
+      // entry j is slot min(js + 1 + j, len - 1), masked past len
+      const int jl = len - 1;
+      int c[NW > 0 ? NW : 1];
+      float f[NW > 0 ? NW : 1];
+      double xv[NW > 0 ? NW : 1];
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j)
-        if (j > js && j < len) {
-          f[j] = (float)F[base + 64 * j];
-          c[j] = sell_col(sl, rp, base + 64 * j, j);
-        }
+      for (int j = 0; j < NW; ++j) f[j] = (float)F[base + 64 * min(j0 + j, jl)];
+      sell_cols_window(sl, rp, base, j0, jl, c);
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j)
-        if (j > js && j < len) s -= (double)f[j] * x[c[j]];
+      for (int j = 0; j < NW; ++j) xv[j] = x[c[j]];
+#pragma unroll
+      for (int j = 0; j < NW; ++j) s = fma(-(j0 + j < len ? (double)f[j] : 0.0), xv[j], s);
     } else {
       for (int j = js + 1; j < len; ++j) {
         const int pj = base + 64 * j;
@@ -555,13 +611,18 @@ __global__ void k_bwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, c
   x[r] = s * dinv[r];
 }
 
+#define CG_INST_SWEEP(FT, U)                                                                              \
+  template __global__ void k_fwd_col3<FT, U>(ColDesc, Sell, const FT *, const double *, const double *,      \
+                                             double *, const int *);                                        \
+  template __global__ void k_bwd_col3<FT, U>(ColDesc, Sell, const FT *, const double *, const double *,      \
+                                             double *, const int *);
 #define CG_INST_FT(FT)                                                                                      \
   template __global__ void k_ic_factor_col3<FT>(ColDesc, Sell, PairList, const int *, const double *,      \
                                                 const uint8_t *, FT *, double *, int *);                    \
-  template __global__ void k_fwd_col3<FT>(ColDesc, Sell, const FT *, const double *, const double *,        \
-                                          double *, const int *);                                          \
-  template __global__ void k_bwd_col3<FT>(ColDesc, Sell, const FT *, const double *, const double *, double *, \
-                                          const int *);                                                      \
+  CG_INST_SWEEP(FT, 0)                                                                                      \
+  CG_INST_SWEEP(FT, 4)                                                                                      \
+  CG_INST_SWEEP(FT, 8)                                                                                      \
+  CG_INST_SWEEP(FT, 16)                                                                                     \
   template __global__ void k_ic_factor_list<FT>(ListLevel, const int *, Sell, PairList, const int *,        \
                                                 const double *, const uint8_t *, FT *, double *, int *);    \
   template __global__ void k_fwd_list<FT>(ListLevel, const int *, Sell, const FT *, const double *,         \
@@ -571,6 +632,7 @@ __global__ void k_bwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, c
 CG_INST_FT(float)
 CG_INST_FT(double)
 #undef CG_INST_FT
+#undef CG_INST_SWEEP
 
 // --------------------------------------------------------------- PCG --
 // graddp.f:1275-1385 with the reference's recurrences:
@@ -590,19 +652,29 @@ __device__ __forceinline__ double sell_row_dot(const Sell &sl, const double *__r
   const int q0 = o0 + lane;
   double acc = 0.0;
   if (w <= kUnroll) {
-    // all column ids and values first, then the gathers: the loads of a
-    // row are independent and in flight together
+    // Straight-line loads: every slot of the unrolled window is loaded
+    // (slots past the row's end re-read its last slot) and masked when
+    // summed, so values, column codes and then the gathers are each in
+    // flight together -- a load under a condition makes the compiler wait
+    // for every earlier load at the next merge.
     int c[kUnroll];
-    double a[kUnroll];
+    double a[kUnroll], xv[kUnroll];
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j)
-      if (j < w) {
-        a[j] = A[q0 + 64 * j];
-        c[j] = sell_col(sl, rp, q0 + 64 * j, j);
-      }
+    for (int j = 0; j < kUnroll; ++j) a[j] = A[q0 + 64 * min(j, w - 1)];
+    if (sl.coded) {
+      uint32_t cw[kUnroll];
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j)
-      if (j < w) acc += a[j] * x[c[j]];
+      for (int j = 0; j < kUnroll; ++j) cw[j] = code_word(sl, rp, min(j, w - 1));
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) c[j] = code_col(sl, rp.k, cw[j]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) c[j] = sl.col[q0 + 64 * min(j, w - 1)];
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) xv[j] = x[c[j]];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) acc = fma(j < w ? a[j] : 0.0, xv[j], acc);
   } else {
     int q = q0;
     for (int j = 0; j < w; ++j, q += 64) acc += A[q] * x[sell_col(sl, rp, q, j)];
