@@ -1,19 +1,25 @@
 #!/usr/bin/env python3
 """Throughput of the CATHY subsurface Picard step on one MI355X.
 
-Workload: the 1.25M-node heterogeneous hillslope (201x201-node DEM, 2 m
-cells, 0-20 m spectral relief, 30 graded layers to 10 m, lognormal Ks per
-(layer, cell), +-20% vG alpha/n, 6 h of 5e-7 m/s rain then drainage, free
-drainage at the base), FP64 Picard with IC(0)-PCG (FP32 factor, 3-colour
-column ordering).  A step = one accepted implicit time step of one member
-(FLOW3D returning converged, back-steps included) -- the metric's
-"member Picard step".
+Workload: an ensemble of M members (default 2) of the 1.25M-node
+heterogeneous hillslope generator (201x201-node DEM, 2 m cells, 0-20 m
+spectral relief, 30 graded layers to 10 m, lognormal Ks per (layer, cell),
++-20% vG alpha/n, 6 h of 5e-7 m/s rain then drainage, free drainage at the
+base; member i uses seed 2+i, member 0 is the configuration's own run),
+FP64 Picard with IC(0)-PCG (FP32 factor, 3-colour column ordering).  A step
+= one accepted implicit time step of one member (FLOW3D returning
+converged, back-steps included) -- the metric's "member Picard step".
+``value`` = member-steps/s of the whole ensemble: each member runs steps
+1..K on its own HIP stream, driven by its own host thread, so the members'
+kernels overlap on the GPU.
 
-The W warm-up steps run on the device and the state is then reset to the
-initial condition, so the K timed steps are steps 1..K of the run; the CPU
-baseline (the C oracle, one core) times the first steps of the same run.
-Every library call is synchronous, so the wall clock around the K calls
-brackets all device work.
+Member 0 first runs the same K steps alone (``config.single_member``):
+that pass carries the SpMV roofline, so no other stream shares the GPU
+while the kernel is timed.  The W warm-up steps run on the device and each
+state is then reset to the initial condition; the CPU baseline (the C
+oracle, one core) times the first steps of member 0's run.  Every library
+call is synchronous, so the wall clock around the calls brackets all
+device work.
 
 The roofline entry is the PCG SpMV (k_spmv_dot2): its average kernel
 duration comes from the dispatch packets' own timestamps
@@ -62,9 +68,45 @@ def bench_params():
     return CF.hillslope_small_params(tmax=172800.0)
 
 
-def build_problem(cells: int, nstr: int):
+def build_problem(cells: int, nstr: int, seed: int = 2):
     from pycathy_wrapper_amd import configs as CF
-    return CF.hillslope_large(cells=cells, nstr=nstr, params=bench_params())
+    return CF.hillslope_large(seed=seed, cells=cells, nstr=nstr, params=bench_params())
+
+
+def ensemble_run(members, steps: int) -> dict:
+    """Steps 1..K of every member at once, one host thread and HIP stream
+    per member (the library releases nothing between calls; ctypes drops
+    the interpreter lock during each call)."""
+    import threading
+    res = [None] * len(members)
+
+    def run(j):
+        pr, g = members[j]
+        g.init(pr.psi0)
+        n = pic = pc = bk = 0
+        sim = 0.0
+        for _ in range(steps):
+            rc, s = g.advance()
+            c = g.counters()
+            n += 1
+            pic += c["picard"]
+            pc += c["pcg"]
+            bk += s["BACK"]
+            sim = s["TIME"]
+            if rc:
+                break
+        res[j] = dict(steps=n, picard=pic, pcg=pc, back_steps=bk, sim_s=sim)
+
+    threads = [threading.Thread(target=run, args=(j,)) for j in range(len(members))]
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t0
+    if any(r is None for r in res):
+        raise RuntimeError("an ensemble member failed")
+    return dict(wall=wall, members=res)
 
 
 def cpu_baseline(problem, max_steps: int, seconds: float) -> dict:
@@ -238,6 +280,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=60.0, help="stop the CPU baseline after this long")
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the timed steps, run the copy kernel over this many MB (PMC calibration)")
+    ap.add_argument("--members", type=int, default=2, help="ensemble members run concurrently (streams)")
     ap.add_argument("--learned-members", type=int, default=8,
                     help="small members whose first steps train the learned initial iterate (0 skips the leg)")
     ap.add_argument("--learned-train-steps", type=int, default=4000)
@@ -271,6 +314,16 @@ def main() -> int:
         rc, s = g.advance()
         log(f"[warmup] step {w + 1} dt={s['DELTAT']:.4g} iter={s['ITER']:.0f} pcg={s['NITERT']:.0f}")
     g.init(pr.psi0)  # timed steps restart from the initial condition
+    others = []
+    for i in range(1, args.members):
+        t0 = time.perf_counter()
+        pi = build_problem(args.cells, args.nstr, seed=2 + i)
+        gi = gpu.FlowGPU(pi, order)
+        for w in range(args.warmup):
+            gi.advance()
+        gi.init(pi.psi0)
+        others.append((pi, gi))
+        log(f"[setup] member {i} (seed {2 + i}) built, warmed up and reset in {time.perf_counter() - t0:.1f}s")
 
     g.profile(True)
     dts = []
@@ -305,6 +358,12 @@ def main() -> int:
     if args.calib_mb > 0:
         ms = gpu.calib_copy(int(args.calib_mb * 1e6), 5)
         log(f"[calib] copy of {args.calib_mb:.0f} MB: {ms * 1e3:.1f} us/launch")
+    ens = ensemble_run([(pr, g)] + others, steps)
+    ens_steps = sum(r["steps"] for r in ens["members"])
+    log(f"[ensemble] {len(ens['members'])} members: {ens_steps} member-steps in {ens['wall']:.3f}s -> "
+        f"{ens_steps / ens['wall']:.2f} member-steps/s (member 0 alone: {steps / wall:.2f} steps/s)")
+    for _, gi in others:
+        gi.close()
     learned = None
     if args.learned_members > 0:
         learned = learned_leg(g, pr, args, dts, steps)
@@ -318,29 +377,33 @@ def main() -> int:
     traffic = read_traffic()
     out = {
         "metric": METRIC,
-        "value": steps / wall,
+        "value": ens_steps / ens["wall"],
         "unit": "member-steps/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / steps,
+        "ms_per_step": 1e3 * ens["wall"] / ens_steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"hillslope_large: {m.nrow}x{m.ncol} cells, {m.nstr} graded layers, N={m.n} nodes, "
-                        f"NT={m.nt} tets, steps 1..{steps} of the 2-day run, plain Picard + IC(0)-PCG "
-                        f"({args.ordering} ordering, FP32 factor, FP64 state/vectors), one member",
+            "workload": f"hillslope_large ensemble: {args.members} members (seeds 2..{args.members + 1}), each "
+                        f"{m.nrow}x{m.ncol} cells, {m.nstr} graded layers, N={m.n} nodes, NT={m.nt} tets, steps "
+                        f"1..{steps} of the 2-day run, plain Picard + IC(0)-PCG ({args.ordering} ordering, FP32 "
+                        f"factor, FP64 state/vectors), one HIP stream per member",
+            "members": args.members,
             "nodes": m.n,
             "tets": m.nt,
-            "picard_per_s": picard / wall,
-            "pcg_per_s": pcg / wall,
-            "sim_s_per_wall_s": sim1 / wall,
-            "picard_iterations": picard,
-            "pcg_iterations": pcg,
-            "back_steps": backs,
+            "member_steps": ens_steps,
+            "picard_per_s": sum(r["picard"] for r in ens["members"]) / ens["wall"],
+            "pcg_per_s": sum(r["pcg"] for r in ens["members"]) / ens["wall"],
+            "ensemble": ens["members"],
+            "single_member": {"steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
+                              "picard_per_s": picard / wall, "pcg_per_s": pcg / wall,
+                              "sim_s_per_wall_s": sim1 / wall, "picard_iterations": picard,
+                              "pcg_iterations": pcg, "back_steps": backs},
             "learned_guess": False,
             "learned": learned,
             "parallelism": "single GPU",

@@ -888,6 +888,7 @@ struct Flow {
       case 0: hipLaunchKernelGGL((k_fwd_col3<FT, 0>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
       case 4: hipLaunchKernelGGL((k_fwd_col3<FT, 4>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
       case 8: hipLaunchKernelGGL((k_fwd_col3<FT, 8>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
+      case 12: hipLaunchKernelGGL((k_fwd_col3<FT, 12>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
       default: hipLaunchKernelGGL((k_fwd_col3<FT, 16>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
     }
   }
@@ -900,6 +901,7 @@ struct Flow {
       case 0: hipLaunchKernelGGL((k_bwd_col3<FT, 0>), g, b, sm, st, C, sl, F, di, xin, out, done); break;
       case 4: hipLaunchKernelGGL((k_bwd_col3<FT, 4>), g, b, sm, st, C, sl, F, di, xin, out, done); break;
       case 8: hipLaunchKernelGGL((k_bwd_col3<FT, 8>), g, b, sm, st, C, sl, F, di, xin, out, done); break;
+      case 12: hipLaunchKernelGGL((k_bwd_col3<FT, 12>), g, b, sm, st, C, sl, F, di, xin, out, done); break;
       default: hipLaunchKernelGGL((k_bwd_col3<FT, 16>), g, b, sm, st, C, sl, F, di, xin, out, done); break;
     }
   }

@@ -209,7 +209,7 @@ __global__ void k_bwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, 
                            const int *done);
 // Gather window of the column sweeps: the smallest instantiated size that
 // covers n slots (larger rows take the kernels' loop).
-inline int sweep_window(int n) { return n <= 0 ? 0 : (n <= 4 ? 4 : (n <= 8 ? 8 : 16)); }
+inline int sweep_window(int n) { return n <= 0 ? 0 : (n <= 4 ? 4 : (n <= 8 ? 8 : (n <= 12 ? 12 : 16))); }
 template <typename FT>
 __global__ void k_ic_factor_list(ListLevel L, const int *rows, Sell sl, PairList pl, const int *tpos,
                                  const double *A, const uint8_t *isdir, FT *F, double *dinv, int *nfall);

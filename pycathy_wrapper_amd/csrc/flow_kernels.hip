@@ -622,6 +622,7 @@ __global__ void k_bwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, c
   CG_INST_SWEEP(FT, 0)                                                                                      \
   CG_INST_SWEEP(FT, 4)                                                                                      \
   CG_INST_SWEEP(FT, 8)                                                                                      \
+  CG_INST_SWEEP(FT, 12)                                                                                     \
   CG_INST_SWEEP(FT, 16)                                                                                     \
   template __global__ void k_ic_factor_list<FT>(ListLevel, const int *, Sell, PairList, const int *,        \
                                                 const double *, const uint8_t *, FT *, double *, int *);    \
