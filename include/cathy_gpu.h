@@ -200,6 +200,22 @@ int cathy_gpu_flow3d_guess_record(int64_t *handle, const int *on);
 int cathy_gpu_flow3d_guess_sample(int64_t *handle, float *feat,
                                   float *target);
 
+/* EnKF analysis of the member states in place (pyCATHY DA/enkf.py:16-224,
+ * enkf_analysis; state inflation as in enkf_analysis_localized_with_
+ * inflation, enkf.py:316-318; no localisation).  handles: nens flow
+ * handles on one mesh and ordering (2 <= nens <= 64); w: the nens x nens
+ * weight W(k,j) at w[j*nens + k] such that P_xo COV^-1 D = X' W, i.e.
+ * W = S^T COV^-1 D / (nens-1) from the observation-space quantities (the
+ * Sakov form replaces COV^-1 D by D / diag(R)); *inflation: 1 = none.
+ * PNEW of every member becomes x_j + sum_k (x_k - xbar) W(k,j) (then
+ * inflated about the analysis mean), and each run continues from it as
+ * from an accepted step (cathy_gpu_flow3d_adopt_state). */
+int cathy_gpu_enkf_update(const int *nens, int64_t *handles, const double *w,
+                          const double *inflation);
+/* Continue a run from its current PNEW (after set_pnew or an analysis):
+ * PTIMEP = PNEW, Sw and kr at it, storage baseline reset. */
+int cathy_gpu_flow3d_adopt_state(int64_t *handle);
+
 /* VEL3D (vel3d.f) + VNOD3D (vnod3d.f), cathy_main.f:3243-3246: Darcy
  * velocities UU/VV/WW(NT) per element and their nodal means UNOD/VNOD/
  * WNOD(N) at the current PNEW and kr.  TETRA, PERMX/Y/Z(MAXSTR,NZONE),

@@ -1710,6 +1710,23 @@ struct Flow {
   }
   int nzone_ = 0, maxstr_ = 0;
 
+  // After an analysis rewrote PNEW: the run continues from it as from an
+  // accepted step (PTIMEP = PNEW, Sw and kr at it, the storage baseline
+  // of the mass balance reset; the previous-step history of the learned
+  // start is dropped).
+  void adopt_state() {
+    CG_REQUIRE(initialised, "flow3d_init not called");
+    CG_HIP(hipMemcpyAsync(ptimep.p, pnew.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_sw_kr, dim3(grid_for(n)), dim3(kBlock), 0, st, n, pnew.p, soil(), sw.p, ckrw.p);
+    CG_HIP(hipMemcpyAsync(ckrwp.p, ckrw.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_sw_kr, dim3(grid_for(n)), dim3(kBlock), 0, st, n, ptimep.p, soil(), swtimep.p,
+                       (double *)nullptr);
+    store1 = device_sum_store();
+    store2 = store1;
+    dtprev = 0.0;
+    CG_HIP(hipStreamSynchronize(st));
+  }
+
   void get_state(double *pn, double *swo, double *ck, int *fa) {
     std::vector<double> h(n);
     auto fetch = [&](const DBuf<double> &b, double *out) {
@@ -1890,6 +1907,54 @@ int cathy_gpu_flow3d_guess_record(int64_t *handle, const int *on) {
 int cathy_gpu_flow3d_guess_sample(int64_t *handle, float *feat, float *target) {
   return guarded([&] {
     lookup(handle)->guess_sample(feat, target);
+    return 0;
+  });
+}
+
+int cathy_gpu_enkf_update(const int *nens, int64_t *handles, const double *w, const double *inflation) {
+  return guarded([&] {
+    CG_REQUIRE(nens && handles && w && inflation, "null argument");
+    const int m = *nens;
+    CG_REQUIRE(m >= 2 && m <= kEnkfMax, "ensemble size out of range (2..64)");
+    std::vector<Flow *> fl(m);
+    for (int j = 0; j < m; ++j) fl[j] = lookup(&handles[j]);
+    Flow *f0 = fl[0];
+    int dev0 = -1;
+    CG_HIP(hipGetDevice(&dev0));
+    for (int j = 0; j < m; ++j) {
+      CG_REQUIRE(fl[j]->initialised, "flow3d_init not called on every member");
+      for (int k = 0; k < j; ++k) CG_REQUIRE(fl[k] != fl[j], "a member appears twice");
+      CG_REQUIRE(fl[j]->n == f0->n && fl[j]->perm == f0->perm,
+                 "members differ in mesh or node ordering; their states do not align");
+    }
+    for (int j = 0; j < m * m; ++j) CG_REQUIRE(std::isfinite(w[j]), "non-finite analysis weight");
+    // every member's queued work ends before its state is read
+    for (int j = 0; j < m; ++j) CG_HIP(hipStreamSynchronize(fl[j]->st));
+    std::vector<const double *> hx(m);
+    std::vector<double *> hxa(m);
+    for (int j = 0; j < m; ++j) {
+      hx[j] = fl[j]->pnew.p;
+      hxa[j] = fl[j]->pnew.p;  // in place: each thread reads its row of every member before writing
+    }
+    hipStream_t s = f0->st;
+    DBuf<const double *> dx(m);
+    DBuf<double *> dxa(m);
+    DBuf<double> dw((size_t)m * m);
+    dx.upload(hx.data(), m, s);
+    dxa.upload(hxa.data(), m, s);
+    dw.upload(w, (size_t)m * m, s);
+    hipLaunchKernelGGL(k_enkf_update, dim3(grid_for(f0->n)), dim3(kBlock), 0, s, f0->n, m, dx.p, dw.p, *inflation,
+                       dxa.p);
+    CG_HIP(hipGetLastError());
+    CG_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < m; ++j) fl[j]->adopt_state();
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_adopt_state(int64_t *handle) {
+  return guarded([&] {
+    lookup(handle)->adopt_state();
     return 0;
   });
 }

@@ -293,6 +293,9 @@ __global__ void k_bicg_update(int n, int nparts, const double *part_st, const do
 __global__ void k_bicg_check(int n, int nparts, const double *part_rr, const double *part_rtr, BicgScal *sc,
                              int *done);
 __global__ void k_sw_only(int n, const double *psi, SoilSoA s, double *sw);
+constexpr int kEnkfMax = 64;  // members per analysis launch
+__global__ void k_enkf_update(int n, int nens, const double *const *X, const double *W, double alpha,
+                              double *const *Xa);
 __global__ void k_vel3d(int nt, const int4 *enode, const double *pnew, const double *ckrw, const double *vt,
                         double *uu, double *vv, double *ww);
 __global__ void k_vnod3d(int n, const int *inc_ptr, const int *inc, const double *uu, const double *vv,

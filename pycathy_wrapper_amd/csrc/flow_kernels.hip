@@ -1175,6 +1175,36 @@ __global__ void k_fill(size_t n, double v, double *__restrict__ dst) {
 }
 
 // ----------------------------------------------------- velocities --
+// ------------------------------------------------------------ EnKF --
+// Ensemble analysis on member states held in place (enkf.py:16-224,
+// enkf_analysis): with the N x N weight W = S^T COV^-1 D / (N-1) formed
+// from the observation-space quantities, P_xo COV^-1 D = X' W, so per row
+//   xa_j = x_j + sum_k (x_k - xbar) W(k, j),
+// then, for inflation alpha != 1 (enkf.py:316-318), xa = mean(xa) +
+// alpha (xa - mean(xa)).  One thread per row; the member values of the
+// row live in registers (N <= kEnkfMax).
+__global__ void k_enkf_update(int n, int nens, const double *const *__restrict__ X, const double *__restrict__ W,
+                              double alpha, double *const *__restrict__ Xa) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x[kEnkfMax], xa[kEnkfMax];
+  double mean = 0.0;
+  for (int k = 0; k < nens; ++k) {
+    x[k] = X[k][i];
+    mean += x[k];
+  }
+  mean /= nens;
+  double meana = 0.0;
+  for (int j = 0; j < nens; ++j) {
+    double s = x[j];
+    for (int k = 0; k < nens; ++k) s = fma(x[k] - mean, W[(size_t)j * nens + k], s);
+    xa[j] = s;
+    meana += s;
+  }
+  meana /= nens;
+  for (int j = 0; j < nens; ++j) Xa[j][i] = alpha == 1.0 ? xa[j] : meana + alpha * (xa[j] - meana);
+}
+
 // vel3d.f:28-52 per element: kr_e = mean nodal kr (nodelt.f), then
 //   UU = (sum psi_j b_j) (-kr_e VOLUR IVOL) Kx, VV likewise with c / Ky,
 //   WW = ((sum psi_j d_j) (-kr_e VOLUR IVOL) - kr_e) Kz.
