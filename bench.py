@@ -268,6 +268,50 @@ def learned_leg(g, pr, args, dts, steps: int) -> dict:
     return out
 
 
+def mixed_ensemble(args) -> int:
+    """Steps 1..K of every member of the mixed-size ensemble at once (one
+    host thread and HIP stream per member); prints the JSON line with
+    member-steps/s of the whole ensemble."""
+    from pycathy_wrapper_amd import configs as CF
+    from pycathy_wrapper_amd import gpu
+    counts = {sz: max(1, int(round(c * args.mixed))) for sz, c in (("S", 32), ("M", 24), ("L", 8))}
+    members, sizes = [], []
+    t0 = time.perf_counter()
+    seed = 100
+    for sz, c in counts.items():
+        for _ in range(c):
+            pr = CF.ensemble_member(sz, seed, params=CF.hillslope_small_params(tmax=86400.0))
+            members.append((pr, gpu.FlowGPU(pr, gpu.ORDER_COLUMN3)))
+            sizes.append(sz)
+            seed += 1
+        log(f"[setup] {c} members of size {sz} ({members[-1][0].mesh.n} nodes) after {time.perf_counter() - t0:.1f}s")
+    for pr, g in members:  # one warm-up step each, then back to the initial state
+        for _ in range(args.warmup):
+            g.advance()
+    ens = ensemble_run(members, args.steps)
+    total = sum(r["steps"] for r in ens["members"])
+    per = {sz: dict(members=counts[sz], nodes=int(next(pr.mesh.n for (pr, _), s2 in zip(members, sizes) if s2 == sz)),
+                    steps=sum(r["steps"] for r, s2 in zip(ens["members"], sizes) if s2 == sz),
+                    picard=sum(r["picard"] for r, s2 in zip(ens["members"], sizes) if s2 == sz),
+                    pcg=sum(r["pcg"] for r, s2 in zip(ens["members"], sizes) if s2 == sz)) for sz in counts}
+    log(f"[mixed] {len(members)} members: {total} member-steps in {ens['wall']:.3f}s -> "
+        f"{total / ens['wall']:.2f} member-steps/s; {per}")
+    out = {
+        "metric": METRIC, "value": total / ens["wall"], "unit": "member-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * ens["wall"] / total,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "mixed-size ensemble: " + ", ".join(f"{c} x {sz}" for sz, c in counts.items()) +
+                               f" members (ensemble_member, seeds 100..), steps 1..{args.steps} each, one HIP "
+                               f"stream per member, plain Picard + IC(0)-PCG (column3, FP32 factor)",
+                   "per_size": per, "learned_guess": False, "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    for _, g in members:
+        g.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +325,9 @@ def main() -> int:
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the timed steps, run the copy kernel over this many MB (PMC calibration)")
     ap.add_argument("--members", type=int, default=2, help="ensemble members run concurrently (streams)")
+    ap.add_argument("--mixed", type=float, default=0.0,
+                    help="instead: the mixed-size ensemble (32 S + 24 M + 8 L members times this factor, "
+                         "41x41x15 / 101x101x20 / 201x201x30 nodes), all members concurrently")
     ap.add_argument("--learned-members", type=int, default=8,
                     help="small members whose first steps train the learned initial iterate (0 skips the leg)")
     ap.add_argument("--learned-train-steps", type=int, default=4000)
@@ -295,6 +342,9 @@ def main() -> int:
         return 2
 
     from pycathy_wrapper_amd import gpu
+
+    if args.mixed > 0:
+        return mixed_ensemble(args)
 
     t0 = time.perf_counter()
     pr = build_problem(args.cells, args.nstr)

@@ -149,6 +149,44 @@ def hillslope_small_member(seed: int, params: Params | None = None) -> Problem:
                                            poros=float(poros), rain=rain))
 
 
+ENSEMBLE_SIZES = {"S": (40, 15), "M": (100, 20), "L": (200, 30)}  # cells per side, layers
+
+
+def gamma_rain(hours: int = 24, mean: float = 2e-7, shape: float = 0.5, seed: int = 7):
+    """Hourly rainfall intensities drawn from a gamma law (shared by every
+    member of an ensemble): ATMTIM / ATMVAL with one record per hour."""
+    rng = np.random.default_rng(seed)
+    vals = rng.gamma(shape, mean / shape, hours)
+    times = 3600.0 * np.arange(hours + 1)
+    return times, np.r_[vals, vals[-1]]
+
+
+def ensemble_member(size: str, seed: int, params: Params | None = None) -> Problem:
+    """One member of the mixed-size ensemble: the hillslope_large generator
+    (spectral DEM, 2 m cells, graded layers to 10 m, lognormal Ks field
+    with ln-sd 1 and 20 m correlation, +-20% alpha / n per cell) at size
+    S, M or L (40x40x15, 100x100x20, 200x200x30 cells), with the member's
+    own field and its own prior draws: ln Ks mean ~ N(ln 1e-5, 0.5^2),
+    alpha, n and porosity scaled by 1 + 0.15 N(0, 1) (n >= 1.1, porosity in
+    [0.25, 0.6]); the shared gamma hourly rainfall over 24 h."""
+    cells, nstr = ENSEMBLE_SIZES[size]
+    base = hillslope_large(seed=seed, cells=cells, nstr=nstr, params=params or Params(tmax=86400.0))
+    rng = np.random.default_rng(30_000 + seed)
+    s = dict(base.soil)
+    f_ks = float(np.exp(0.5 * rng.standard_normal()))
+    f_al = 1.0 + 0.15 * rng.standard_normal()
+    f_n = 1.0 + 0.15 * rng.standard_normal()
+    f_po = 1.0 + 0.15 * rng.standard_normal()
+    ks = sig4(np.asarray(s["permx"]) * f_ks)
+    s["permx"], s["permy"], s["permz"] = ks, ks.copy(), ks.copy()
+    s["vgpsat"] = sig4(np.asarray(s["vgpsat"]) / max(0.3, f_al))
+    s["vgn"] = sig4(np.maximum(1.1, np.asarray(s["vgn"]) * f_n))
+    s["poros"] = sig4(np.clip(np.asarray(s["poros"]) * f_po, 0.25, 0.6))
+    t, v = gamma_rain()
+    return Problem(f"ensemble_member_{size}", base.mesh, s, base.psi0, t, v, base.params, seed,
+                   dict(base.meta, size=size))
+
+
 def _spectral_dem(n: int, dx: float, relief: float, rng, beta: float = 3.0) -> np.ndarray:
     kx = np.fft.fftfreq(n, dx)
     k = np.sqrt(kx[:, None] ** 2 + kx[None, :] ** 2)
