@@ -84,18 +84,22 @@ def ensemble_run(members, steps: int) -> dict:
         pr, g = members[j]
         g.init(pr.psi0)
         n = pic = pc = bk = 0
-        sim = 0.0
+        sim = mbe = 0.0
+        lat = []
         for _ in range(steps):
+            ts = time.perf_counter()
             rc, s = g.advance()
+            lat.append(time.perf_counter() - ts)
             c = g.counters()
             n += 1
             pic += c["picard"]
             pc += c["pcg"]
             bk += s["BACK"]
             sim = s["TIME"]
+            mbe = max(mbe, abs(s["ERREL"]))
             if rc:
                 break
-        res[j] = dict(steps=n, picard=pic, pcg=pc, back_steps=bk, sim_s=sim)
+        res[j] = dict(steps=n, picard=pic, pcg=pc, back_steps=bk, sim_s=sim, max_rel_mbe_pct=mbe, step_s=lat)
 
     threads = [threading.Thread(target=run, args=(j,)) for j in range(len(members))]
     t0 = time.perf_counter()
@@ -274,15 +278,25 @@ def mixed_ensemble(args) -> int:
     member-steps/s of the whole ensemble."""
     from pycathy_wrapper_amd import configs as CF
     from pycathy_wrapper_amd import gpu
-    counts = {sz: max(1, int(round(c * args.mixed))) for sz, c in (("S", 32), ("M", 24), ("L", 8))}
-    members, sizes = [], []
+    import numpy as np
+    stress = args.stress > 0
+    f = args.stress if stress else args.mixed
+    base = (("S", 64), ("M", 48), ("L", 16)) if stress else (("S", 32), ("M", 24), ("L", 8))
+    counts = {sz: max(1, int(round(c * f))) for sz, c in base}
+    members, sizes, iopts = [], [], []
     t0 = time.perf_counter()
     seed = 100
     for sz, c in counts.items():
-        for _ in range(c):
-            pr = CF.ensemble_member(sz, seed, params=CF.hillslope_small_params(tmax=86400.0))
+        for i in range(c):
+            iopt = 2 if (stress and i % 2 == 0) else 1
+            pr = CF.ensemble_member(sz, seed, params=CF.hillslope_small_params(tmax=86400.0, iopt=iopt))
+            if stress:  # dry start, 2 h storm, then nothing
+                pr.psi0 = np.full(pr.mesh.n, -5.0)
+                pr.atm_times = np.array([0.0, 7200.0, 7201.0])
+                pr.atm_vals = np.array([2e-6, 2e-6, 0.0])
             members.append((pr, gpu.FlowGPU(pr, gpu.ORDER_COLUMN3)))
             sizes.append(sz)
+            iopts.append(iopt)
             seed += 1
         log(f"[setup] {c} members of size {sz} ({members[-1][0].mesh.n} nodes) after {time.perf_counter() - t0:.1f}s")
     for pr, g in members:  # one warm-up step each, then back to the initial state
@@ -294,16 +308,30 @@ def mixed_ensemble(args) -> int:
                     steps=sum(r["steps"] for r, s2 in zip(ens["members"], sizes) if s2 == sz),
                     picard=sum(r["picard"] for r, s2 in zip(ens["members"], sizes) if s2 == sz),
                     pcg=sum(r["pcg"] for r, s2 in zip(ens["members"], sizes) if s2 == sz)) for sz in counts}
-    log(f"[mixed] {len(members)} members: {total} member-steps in {ens['wall']:.3f}s -> "
-        f"{total / ens['wall']:.2f} member-steps/s; {per}")
+    lat = np.concatenate([r["step_s"] for r in ens["members"]])
+    by_iopt = {("newton" if io == 2 else "picard"): dict(
+        members=sum(1 for x in iopts if x == io),
+        steps=sum(r["steps"] for r, x in zip(ens["members"], iopts) if x == io),
+        max_rel_mbe_pct=max((r["max_rel_mbe_pct"] for r, x in zip(ens["members"], iopts) if x == io),
+                            default=0.0)) for io in sorted(set(iopts))}
+    lat_ms = dict(p50=1e3 * float(np.percentile(lat, 50)), p99=1e3 * float(np.percentile(lat, 99)))
+    for r in ens["members"]:
+        del r["step_s"]
+    log(f"[{'stress' if stress else 'mixed'}] {len(members)} members: {total} member-steps in {ens['wall']:.3f}s -> "
+        f"{total / ens['wall']:.2f} member-steps/s; step latency p50 {lat_ms['p50']:.1f} ms, p99 {lat_ms['p99']:.1f} "
+        f"ms; {per}; {by_iopt}")
     out = {
         "metric": METRIC, "value": total / ens["wall"], "unit": "member-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * ens["wall"] / total,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "mixed-size ensemble: " + ", ".join(f"{c} x {sz}" for sz, c in counts.items()) +
+        "config": {"workload": ("stress" if stress else "mixed-size") + " ensemble: " +
+                               ", ".join(f"{c} x {sz}" for sz, c in counts.items()) +
                                f" members (ensemble_member, seeds 100..), steps 1..{args.steps} each, one HIP "
-                               f"stream per member, plain Picard + IC(0)-PCG (column3, FP32 factor)",
-                   "per_size": per, "learned_guess": False, "parallelism": "single GPU"},
+                               f"stream per member, " + ("dry start psi = -5 m, 2 h of 2e-6 m/s rain, Newton "
+                               "(ILU-BiCGSTAB) for every other member, Picard + IC(0)-PCG for the rest"
+                               if stress else "plain Picard + IC(0)-PCG") + " (column3, FP32 factor)",
+                   "per_size": per, "by_linearisation": by_iopt, "step_latency_ms": lat_ms,
+                   "learned_guess": False, "parallelism": "single GPU"},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -325,6 +353,9 @@ def main() -> int:
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the timed steps, run the copy kernel over this many MB (PMC calibration)")
     ap.add_argument("--members", type=int, default=2, help="ensemble members run concurrently (streams)")
+    ap.add_argument("--stress", type=float, default=0.0,
+                    help="instead: the stress ensemble (64 S + 48 M + 16 L members times this factor; dry start "
+                         "psi = -5 m, 2 h of 2e-6 m/s rain; Newton for half the members, Picard for the rest)")
     ap.add_argument("--mixed", type=float, default=0.0,
                     help="instead: the mixed-size ensemble (32 S + 24 M + 8 L members times this factor, "
                          "41x41x15 / 101x101x20 / 201x201x30 nodes), all members concurrently")
@@ -343,7 +374,7 @@ def main() -> int:
 
     from pycathy_wrapper_amd import gpu
 
-    if args.mixed > 0:
+    if args.mixed > 0 or args.stress > 0:
         return mixed_ensemble(args)
 
     t0 = time.perf_counter()
