@@ -1166,8 +1166,11 @@ struct Flow {
     const int gn = grid_for(n), ge = grid_for(nt);
     hipLaunchKernelGGL(k_constitutive, dim3(gn), dim3(kBlock), 0, st, n, ptnew.p, pnew.p, soil(), sw.p,
                        ckrw.p, et1.p, et2.p, swnew.p);
-    hipLaunchKernelGGL(k_elem_avg, dim3(ge), dim3(kBlock), 0, st, nt, enode.p, ckrw.p, et1.p, ckrwe.p,
-                       et1e.p);
+    // the element means feed only the assembly; the plan kernel forms kr_e
+    // from the row's own nodal kr when every row fits its 16-slot window
+    if (!(use_plan && maxrow <= kUnroll))
+      hipLaunchKernelGGL(k_elem_avg, dim3(ge), dim3(kBlock), 0, st, nt, enode.p, ckrw.p, et1.p, ckrwe.p,
+                         et1e.p);
     hipLaunchKernelGGL(k_bc_rows, dim3(gn), dim3(kBlock), 0, st, n, row_surf.p, qnode.p, ifatm.p, atmact.p,
                        atmold.p, prm.tetaf, isdir.p, bcflux.p);
     AsmArgs a{};

@@ -191,6 +191,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
   // end re-read the last one and are masked), so each batch is in flight
   // together; a load under a condition makes the compiler wait for every
   // earlier load at the next merge.  Sums keep the slot / record order.
+  // nodal kr at the row's slots, for the element means of the records:
+  // cks[j][thread] (element kr = NODELT over its 4 nodes, all of which are
+  // slots of this row)
+  __shared__ double cks[kUnroll * kBlock];
+  const bool local_kr = w <= kUnroll;
   double m2d = 0.0, grav = 0.0;
   if (w <= kUnroll) {
     int c[kUnroll];
@@ -211,6 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
     for (int j = 0; j < kUnroll; ++j) {
       m2d = fma(j < w ? w2v[j] : 0.0, e1[j], m2d);
       grav = fma(j < w ? gwv[j] : 0.0, ck[j], grav);
+      cks[j * kBlock + threadIdx.x] = ck[j];
     }
   } else {
     for (int j = 0; j < w; ++j) {
@@ -238,8 +244,21 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
       off[b] = P.io[q];
       sv[b] = P.s4[q];
     }
+    if (local_kr) {
+      // nodelt.f order: (((k1 + k2) + k3) + k4) / 4, nodes in element order
 #pragma unroll
-    for (int b = 0; b < RB; ++b) kr[b] = a.ckrwe[max(e[b], 0)];
+      for (int b = 0; b < RB; ++b) {
+        const uint32_t o = off[b];
+        const double s1 = cks[(o & 15u) * kBlock + threadIdx.x];
+        const double s2 = cks[((o >> 8) & 15u) * kBlock + threadIdx.x];
+        const double s3 = cks[((o >> 16) & 15u) * kBlock + threadIdx.x];
+        const double s4 = cks[((o >> 24) & 15u) * kBlock + threadIdx.x];
+        kr[b] = (((s1 + s2) + s3) + s4) * 0.25;
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < RB; ++b) kr[b] = a.ckrwe[max(e[b], 0)];
+    }
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
       const bool live = m0 + b < cw && e[b] >= 0;
