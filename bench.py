@@ -440,6 +440,8 @@ def main() -> int:
         ms = gpu.calib_copy(int(args.calib_mb * 1e6), 5)
         log(f"[calib] copy of {args.calib_mb:.0f} MB: {ms * 1e3:.1f} us/launch")
     ens = ensemble_run([(pr, g)] + others, steps)
+    for r in ens["members"]:
+        r.pop("step_s")
     ens_steps = sum(r["steps"] for r in ens["members"])
     log(f"[ensemble] {len(ens['members'])} members: {ens_steps} member-steps in {ens['wall']:.3f}s -> "
         f"{ens_steps / ens['wall']:.2f} member-steps/s (member 0 alone: {steps / wall:.2f} steps/s)")

@@ -961,7 +961,7 @@ struct Flow {
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kBlock), 0, st, G, P(0), scal.p, done, tol, imax);
     precond_apply(r.p, p.p, nullptr);  // p = M^-1 r
     const int Gs = grid_for(n, kBlock);  // SpMV: one thread per row
-    const int Gr = std::min(grid_for(n, kBlock), 256);   // vector-kernel groups
+    const int Gr = std::min(grid_for(n, kBlock), 1024);  // vector-kernel groups
     int batch = std::max(1, std::min(expect, imax));
     int launched = 0;
     int h_done = 0;
@@ -1220,7 +1220,8 @@ struct Flow {
     }
     hipLaunchKernelGGL(k_bkpic, dim3(gs), dim3(kBlock), 0, st, nnod, surf_row.p, sell(), A.p, pdiff.p, xt5.p,
                        ifatm.p, atmold.p, prm.tetaf, atmact.p);
-    const int G = red_grid(n), Gf = red_grid(nnod);
+    // at most 1024 groups: their partials are summed by one group below
+    const int G = std::min(red_grid(n), 1024), Gf = red_grid(nnod);
     NormPart np{P(5), P(6), P(7), P(8), P(9), P(10), ipart.p};
     hipLaunchKernelGGL(k_norms, dim3(G), dim3(kBlock), 0, st, n, refid.p, pnew.p, pold.p, tnoti.p, swnew.p,
                        swtimep.p, ptimep.p, volnod.p, snodi.p, pnodi.p, np);
