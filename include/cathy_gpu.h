@@ -166,6 +166,8 @@ int cathy_gpu_flow3d_set_pnew(int64_t *handle, const double *pnew);
  *   load     a training set of nsamp columns (x as forward, t as gradient)
  *   train    nsteps Adam steps (beta 0.9/0.999, eps 1e-8) on batches of
  *            the loaded set; idx [nsteps][batch] sample numbers; loss[nsteps]
+ * A network may be attached to several flow handles, also ones advanced
+ * from different threads: its calls and evaluations are serialised.
  * ------------------------------------------------------------------- */
 int cathy_gpu_guess_create(int64_t *net, const int *levels, int *nparam);
 int cathy_gpu_guess_destroy(int64_t *net);

@@ -1657,8 +1657,11 @@ struct Flow {
                    deltat, use, grecord ? graw.p : nullptr, gnet ? gx.p : nullptr);
     if (!gnet || dtprev <= 0.0) return;  // the first step starts from PTIMEP
     CG_HIP(hipStreamSynchronize(st));
-    gnet->forward(gx.p, nullptr, nnod, gy.p);
-    CG_HIP(hipStreamSynchronize(gnet->st));
+    {
+      std::lock_guard<std::mutex> lk(gnet->mu);
+      gnet->forward(gx.p, nullptr, nnod, gy.p);
+      CG_HIP(hipStreamSynchronize(gnet->st));
+    }
     guess_apply(st, nnod, gL, gLp, gcolrow.p, ifatm.p, ptimep.p, pprev.p, ratio, gy.p, use, pnew.p);
     ++guess_calls;
   }

@@ -22,6 +22,7 @@
 
 #include <array>
 #include <memory>
+#include <mutex>
 
 #include "common.h"
 
@@ -68,6 +69,9 @@ struct GuessNet {
   int cap = 0, nsplit_cap = 0;
   int64_t nsamp = 0;
   long adam_t = 0;
+  // serialises users of the work buffers (flows on several threads may
+  // share one network)
+  std::mutex mu;
 
   explicit GuessNet(int levels);
   ~GuessNet();

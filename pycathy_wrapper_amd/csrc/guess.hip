@@ -588,6 +588,7 @@ int cathy_gpu_guess_set(int64_t *net, const float *params, const float *scale) {
   return guarded([&] {
     CG_REQUIRE(net, "null argument");
     auto g = guess_lookup(*net);
+    std::lock_guard<std::mutex> lk(g->mu);
     GuessScale s;
     if (scale) s = scale_from(scale);
     g->set_params(params, scale ? &s : nullptr);
@@ -599,6 +600,7 @@ int cathy_gpu_guess_get(int64_t *net, float *params, float *scale) {
   return guarded([&] {
     CG_REQUIRE(net, "null argument");
     auto g = guess_lookup(*net);
+    std::lock_guard<std::mutex> lk(g->mu);
     GuessScale s;
     g->get_params(params, &s);
     if (scale) {
@@ -619,6 +621,7 @@ int cathy_gpu_guess_forward(int64_t *net, const int *ncol, const uint16_t *x, fl
   return guarded([&] {
     CG_REQUIRE(net && ncol && x && y && *ncol > 0, "bad argument");
     auto g = guess_lookup(*net);
+    std::lock_guard<std::mutex> lk(g->mu);
     g->ensure(*ncol);
     const size_t pos = (size_t)*ncol * g->Lp;
     g->xs.upload(x, pos * kGuessIn, g->st);
@@ -634,6 +637,7 @@ int cathy_gpu_guess_gradient(int64_t *net, const int *ncol, const uint16_t *x, c
   return guarded([&] {
     CG_REQUIRE(net && ncol && x && t && *ncol > 0, "bad argument");
     auto g = guess_lookup(*net);
+    std::lock_guard<std::mutex> lk(g->mu);
     g->ensure(*ncol);
     const size_t pos = (size_t)*ncol * g->Lp;
     g->xs.upload(x, pos * kGuessIn, g->st);
@@ -650,6 +654,7 @@ int cathy_gpu_guess_load(int64_t *net, const int64_t *nsamp, const uint16_t *x, 
   return guarded([&] {
     CG_REQUIRE(net && nsamp && x && t && *nsamp > 0, "bad argument");
     auto g = guess_lookup(*net);
+    std::lock_guard<std::mutex> lk(g->mu);
     const size_t pos = (size_t)*nsamp * g->Lp;
     g->X.alloc(pos * kGuessIn);
     g->T.alloc(pos);
@@ -666,6 +671,7 @@ int cathy_gpu_guess_train(int64_t *net, const int *nsteps, const int *batch, con
   return guarded([&] {
     CG_REQUIRE(net && nsteps && batch && idx && lr && *nsteps > 0 && *batch > 0, "bad argument");
     auto g = guess_lookup(*net);
+    std::lock_guard<std::mutex> lk(g->mu);
     CG_REQUIRE(g->nsamp > 0, "no training set loaded");
     const size_t nidx = (size_t)*nsteps * *batch;
     for (size_t i = 0; i < nidx; ++i)
