@@ -1,7 +1,7 @@
 """CPU restatement of the learned-initial-iterate network (test infrastructure).
 
-Only tests/, the package's smoke check and bench.py's CPU baseline use this
-module; the product path runs the HIP kernels (csrc/guess.hip).
+Only tests/ use this module; the product path runs the HIP kernels
+(pycathy_wrapper_amd/csrc/guess.hip).
 
 The per-column 1-D convolutional network: layers 16->64, 64->64 (x3) with
 ReLU, then 64->1, all with kernel 3 and zero padding outside the column's
