@@ -878,18 +878,19 @@ struct Flow {
   // ------------------------------------------------------- kernels glue --
   size_t col_smem(const ColDesc &C) const { return sizeof(double) * 2 * (size_t)C.len * kChains; }
 
-  // Column sweeps with the gather window sized for the colour.
-  template <typename FT>
+  // Column sweeps with the gather window sized for the colour.  LAST
+  // fuses the colour's backward sweep (it needs C.nb == 0).
+  template <typename FT, bool LAST = false>
   void launch_fwd_col3(const ColDesc &C, const Sell &sl, const FT *F, const double *di, const double *tin,
                        double *out, const int *done) {
     const dim3 g(grid_for(C.nc, kChains)), b(256);
-    const size_t sm = col_smem(C);
+    const size_t sm = (LAST ? 2 : 1) * col_smem(C);
     switch (sweep_window(C.nf)) {
-      case 0: hipLaunchKernelGGL((k_fwd_col3<FT, 0>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
-      case 4: hipLaunchKernelGGL((k_fwd_col3<FT, 4>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
-      case 8: hipLaunchKernelGGL((k_fwd_col3<FT, 8>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
-      case 12: hipLaunchKernelGGL((k_fwd_col3<FT, 12>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
-      default: hipLaunchKernelGGL((k_fwd_col3<FT, 16>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
+      case 0: hipLaunchKernelGGL((k_fwd_col3<FT, 0, LAST>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
+      case 4: hipLaunchKernelGGL((k_fwd_col3<FT, 4, LAST>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
+      case 8: hipLaunchKernelGGL((k_fwd_col3<FT, 8, LAST>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
+      case 12: hipLaunchKernelGGL((k_fwd_col3<FT, 12, LAST>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
+      default: hipLaunchKernelGGL((k_fwd_col3<FT, 16, LAST>), g, b, sm, st, C, sl, F, di, tin, out, done); break;
     }
   }
   template <typename FT>
@@ -914,8 +915,16 @@ struct Flow {
     // z = (L L^T)^-1 t   (proddp.f)
     const Sell sl = sell();
     if (ordering == CATHY_GPU_ORDER_COLUMN3) {
-      for (const ColDesc &C : sched.cols) launch_fwd_col3<FT>(C, sl, F, dinv.p, tin, out, done);
-      for (size_t c = sched.cols.size(); c-- > 0;) launch_bwd_col3<FT>(sched.cols[c], sl, F, dinv.p, out, out, done);
+      // the last colour couples to no later colour: its two sweeps share
+      // one launch
+      const size_t nc = sched.cols.size();
+      const bool fuse = nc > 0 && sched.cols[nc - 1].nb == 0;
+      for (size_t c = 0; c < nc; ++c) {
+        if (fuse && c + 1 == nc) launch_fwd_col3<FT, true>(sched.cols[c], sl, F, dinv.p, tin, out, done);
+        else launch_fwd_col3<FT>(sched.cols[c], sl, F, dinv.p, tin, out, done);
+      }
+      for (size_t c = fuse ? nc - 1 : nc; c-- > 0;)
+        launch_bwd_col3<FT>(sched.cols[c], sl, F, dinv.p, out, out, done);
     } else {
       for (const ListLevel &L : sched.lv)
         hipLaunchKernelGGL(k_fwd_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p, sl, F,

@@ -201,7 +201,9 @@ struct PairList {
 template <typename FT>
 __global__ void k_ic_factor_col3(ColDesc C, Sell sl, PairList pl, const int *tpos, const double *A,
                                  const uint8_t *isdir, FT *F, double *dinv, int *nfall);
-template <typename FT, int NW>
+// LAST: the colour has no later-colour neighbours, so its backward sweep
+// runs in the same launch and y holds the final x for its rows.
+template <typename FT, int NW, bool LAST>
 __global__ void k_fwd_col3(ColDesc C, Sell sl, const FT *F, const double *dinv, const double *tin, double *y,
                            const int *done);
 template <typename FT, int NW>

@@ -429,7 +429,10 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, Pair
 // Forward sweep L y = t (proddp, solscal-extended.f:4175-4185) on one
 // colour: pass 1 gathers the earlier-colour terms of every row and scales
 // by 1/L_kk, pass 2 runs y_k = t'_k - l'_k y_k-1 down each chain in LDS.
-template <typename FT, int NW>
+// LAST (the final colour, whose rows couple to no later colour): pass 2
+// then runs the backward sweep L^T x = y of the same chains from LDS
+// (x_k = y_k/L_kk - u'_k x_k+1, as k_bwd_col3 forms it) and stores x.
+template <typename FT, int NW, bool LAST>
 __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
                                                   const double *__restrict__ dinv, const double *__restrict__ tin,
                                                   double *__restrict__ y, const int *__restrict__ done) {
@@ -437,7 +440,9 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
   extern __shared__ double smem[];
   const int tot = C.len * kChains;
   double *T = smem, *Lp = smem + tot;
+  double *U = smem + 2 * tot, *D = smem + 3 * tot;  // LAST only
   const int t0 = xcd_group() * kChains;
+  const int kl = C.len - 1;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / kChains, t = t0 + (i % kChains);
     if (t >= C.nc) continue;
@@ -472,6 +477,10 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
     const double di = dinv[r];
     T[i] = s * di;
     Lp[i] = k > 0 ? (double)F[base + 64 * jp] * di : 0.0;
+    if (LAST) {
+      U[i] = k < kl ? (double)F[base + 64 * (jd + 1)] * di : 0.0;
+      D[i] = di;
+    }
   }
   __syncthreads();
   if (threadIdx.x < kChains) {
@@ -481,8 +490,18 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
       for (int k = 0; k < C.len; ++k) {
         const int i = k * kChains + threadIdx.x;
         const double v = T[i] - Lp[i] * yp;
-        y[C.base + t + k * C.nc] = v;
+        if (LAST) T[i] = v * D[i];
+        else y[C.base + t + k * C.nc] = v;
         yp = v;
+      }
+      if (LAST) {
+        double xn = 0.0;
+        for (int k = kl; k >= 0; --k) {
+          const int i = k * kChains + threadIdx.x;
+          const double v = T[i] - U[i] * xn;
+          y[C.base + t + k * C.nc] = v;
+          xn = v;
+        }
       }
     }
   }
@@ -631,8 +650,10 @@ __global__ void k_bwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, c
 }
 
 #define CG_INST_SWEEP(FT, U)                                                                              \
-  template __global__ void k_fwd_col3<FT, U>(ColDesc, Sell, const FT *, const double *, const double *,      \
-                                             double *, const int *);                                        \
+  template __global__ void k_fwd_col3<FT, U, false>(ColDesc, Sell, const FT *, const double *,              \
+                                                    const double *, double *, const int *);                 \
+  template __global__ void k_fwd_col3<FT, U, true>(ColDesc, Sell, const FT *, const double *,               \
+                                                   const double *, double *, const int *);                  \
   template __global__ void k_bwd_col3<FT, U>(ColDesc, Sell, const FT *, const double *, const double *,      \
                                              double *, const int *);
 #define CG_INST_FT(FT)                                                                                      \
