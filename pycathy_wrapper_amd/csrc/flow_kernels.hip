@@ -426,6 +426,35 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, Pair
   }
 }
 
+// The chain recurrence v_k = b_k - a_k v_(k-1) (DOWN: k ascending) or
+// v_k = b_k - a_k v_(k+1) (up), v = 0 outside the chain, for all chains of
+// a work-group at once: element i = k*kChains + c of the LDS arrays b, a,
+// one element per thread (len*kChains <= blockDim.x).  A Hillis-Steele
+// scan of the affine maps: after the round of stride s, (b_k, a_k) is the
+// composition of the maps k .. k-2s+1, so after ceil(log2 len) rounds
+// b_k = v_k.  Composition: b_k - a_k (b_j - a_j v) = (b_k - a_k b_j) -
+// (-a_k a_j) v.
+template <bool DOWN>
+__device__ __forceinline__ void chain_scan(double *b, double *a, int len) {
+  const int i = threadIdx.x, k = i / kChains;
+  for (int s = 1; s < len; s <<= 1) {
+    const bool act = k < len && (DOWN ? k >= s : k + s < len);
+    double nb = 0.0, na = 0.0;
+    if (act) {
+      const int j = DOWN ? i - s * kChains : i + s * kChains;
+      const double ai = a[i];
+      nb = fma(-ai, b[j], b[i]);
+      na = -ai * a[j];
+    }
+    __syncthreads();
+    if (act) {
+      b[i] = nb;
+      a[i] = na;
+    }
+    __syncthreads();
+  }
+}
+
 // Forward sweep L y = t (proddp, solscal-extended.f:4175-4185) on one
 // colour: pass 1 gathers the earlier-colour terms of every row and scales
 // by 1/L_kk, pass 2 runs y_k = t'_k - l'_k y_k-1 down each chain in LDS.
@@ -483,6 +512,19 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
     }
   }
   __syncthreads();
+  if (tot <= (int)blockDim.x) {
+    // chains of at most blockDim.x / kChains levels: log-depth scans
+    const int i = threadIdx.x, k = i / kChains, t = t0 + (i % kChains);
+    const bool store = k < C.len && t < C.nc;
+    chain_scan<true>(T, Lp, C.len);
+    if (LAST) {
+      if (k < C.len) T[i] *= D[i];
+      __syncthreads();
+      chain_scan<false>(T, U, C.len);
+    }
+    if (store) y[C.base + t + k * C.nc] = T[i];
+    return;
+  }
   if (threadIdx.x < kChains) {
     const int t = t0 + threadIdx.x;
     if (t < C.nc) {
@@ -557,6 +599,12 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
     U[i] = k < kl ? (double)F[base + 64 * (jd + 1)] * di : 0.0;
   }
   __syncthreads();
+  if (tot <= (int)blockDim.x) {
+    const int i = threadIdx.x, k = i / kChains, t = t0 + (i % kChains);
+    chain_scan<false>(T, U, C.len);
+    if (k < C.len && t < C.nc) x[C.base + t + k * C.nc] = T[i];
+    return;
+  }
   if (threadIdx.x < kChains) {
     const int t = t0 + threadIdx.x;
     if (t < C.nc) {
@@ -744,7 +792,7 @@ __global__ void k_spmv_dot2(int n, Sell sl, const double *__restrict__ A, const 
                             double *__restrict__ part_pr, double *__restrict__ part_pb,
                             const int *__restrict__ done) {
   if (*done) return;
-  __shared__ double sh[kBlock / 64 + 1];
+  __shared__ double sh[2 * (kBlock / 64)];
   double apr = 0.0, apb = 0.0;
   const int i = xcd_group() * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -754,11 +802,10 @@ __global__ void k_spmv_dot2(int n, Sell sl, const double *__restrict__ A, const 
     apr = pi * r[i];
     apb = pi * s;
   }
-  double t1 = block_sum<kBlock>(apr, sh);
-  double t2 = block_sum<kBlock>(apb, sh);
+  const double2 t = block_sum2<kBlock>(apr, apb, sh);
   if (threadIdx.x == 0) {
-    part_pr[blockIdx.x] = t1;
-    part_pb[blockIdx.x] = t2;
+    part_pr[blockIdx.x] = t.x;
+    part_pb[blockIdx.x] = t.y;
   }
 }
 
@@ -770,9 +817,9 @@ __global__ void k_pcg_alpha(int n, int nparts, const double *__restrict__ part_p
                             double *__restrict__ r, double *__restrict__ x, double *__restrict__ part_rr,
                             PcgScal *sc, const int *__restrict__ done) {
   if (*done) return;
-  __shared__ double sh[kBlock / 64 + 2];
-  const double pr = reduce_partials<kBlock>(part_pr, nparts, sh);
-  const double pb = reduce_partials<kBlock>(part_pb, nparts, sh);
+  __shared__ double sh[2 * (kBlock / 64)];
+  const double2 prb = reduce_partials2<kBlock>(part_pr, part_pb, nparts, sh);
+  const double pr = prb.x, pb = prb.y;
   const double alfa = pr / pb;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->pb = pb;
@@ -823,9 +870,9 @@ __global__ void k_pcg_beta(int n, int nparts, const double *__restrict__ part_bz
                            const double *__restrict__ part_rr, const double *__restrict__ z,
                            double *__restrict__ p, PcgScal *sc, int *done) {
   if (*done) return;
-  __shared__ double sh[kBlock / 64 + 2];
-  const double c = reduce_partials<kBlock>(part_bz, nparts, sh);
-  const double rr = reduce_partials<kBlock>(part_rr, nparts, sh);
+  __shared__ double sh[2 * (kBlock / 64)];
+  const double2 crr = reduce_partials2<kBlock>(part_bz, part_rr, nparts, sh);
+  const double c = crr.x, rr = crr.y;
   const double beta = -c / sc->pb;
   const int stride = gridDim.x * blockDim.x;
   int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -128,4 +128,75 @@ __device__ __forceinline__ double reduce_partials(const double *part, int cnt, d
   return r;
 }
 
+// Deterministic block sums of two values: the totals in thread 0 (sh:
+// 2*BS/64 doubles).
+template <int BS>
+__device__ __forceinline__ double2 block_sum2(double a, double b, double *sh) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[2 * w] = a;
+    sh[2 * w + 1] = b;
+  }
+  __syncthreads();
+  double2 t = make_double2(0.0, 0.0);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < BS / 64; ++k) {
+      t.x += sh[2 * k];
+      t.y += sh[2 * k + 1];
+    }
+  __syncthreads();
+  return t;
+}
+
+// Sums of two partial arrays of `cnt` entries in a fixed order, in every
+// thread (sh: 2*BS/64 doubles).  Eight strided loads of each array are in
+// flight per round; the tail round re-reads the last entry for strides
+// past cnt and masks it (cnt >= 1).
+template <int BS>
+__device__ __forceinline__ double2 reduce_partials2(const double *pa, const double *pb, int cnt, double *sh) {
+  double a[8], b[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) a[u] = b[u] = 0.0;
+  int i = threadIdx.x;
+  for (; i + 7 * BS < cnt; i += 8 * BS) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] += pa[i + u * BS];
+      b[u] += pb[i + u * BS];
+    }
+  }
+  double ta[8], tb[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int j = min(i + u * BS, cnt - 1);
+    ta[u] = pa[j];
+    tb[u] = pb[j];
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const bool in = i + u * BS < cnt;
+    a[u] += in ? ta[u] : 0.0;
+    b[u] += in ? tb[u] : 0.0;
+  }
+  double va = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  double vb = ((b[0] + b[1]) + (b[2] + b[3])) + ((b[4] + b[5]) + (b[6] + b[7]));
+  va = wave_sum(va);
+  vb = wave_sum(vb);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[2 * w] = va;
+    sh[2 * w + 1] = vb;
+  }
+  __syncthreads();
+  double2 t = make_double2(0.0, 0.0);
+  for (int k = 0; k < BS / 64; ++k) {
+    t.x += sh[2 * k];
+    t.y += sh[2 * k + 1];
+  }
+  __syncthreads();
+  return t;
+}
+
 }  // namespace cg
