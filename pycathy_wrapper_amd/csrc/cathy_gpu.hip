@@ -90,6 +90,9 @@ struct Flow {
   bool use_plan = false;
   DBuf<int> pair_ptr;  // IC(0) update pattern per SELL slot
   DBuf<int2> pair_rc;
+  DBuf<uint32_t> ftab;  // IC(0) chain tables (column ordering), see FacTab
+  int ftab_nch = 0;
+  bool ftab_ok = false;
   DBuf<double> dbeta;
   DBuf<int> inc_ptr, inc, refid, row_surf, surf_row, bot_row;
   DBuf<uint32_t> incoff;
@@ -513,6 +516,77 @@ struct Flow {
       }
   }
 
+  // IC(0) chain tables (FacTab): the bytes of each (class, chain) come from
+  // the class's first row; every row is then checked against its own
+  // pattern, and any mismatch leaves the pair-list factor in use.
+  void build_factor_tables(const std::vector<int> &hrp, const std::vector<int> &hcol, const std::vector<int> &hsd,
+                           const std::vector<int> &htpos) {
+    ftab_ok = false;
+    if (ordering != CATHY_GPU_ORDER_COLUMN3 || sched.cols.empty()) return;
+    int nch = 0;
+    for (const ColDesc &C : sched.cols) nch = std::max(nch, C.g0 + C.nc);
+    constexpr int NB = kFacWords * 4;
+    auto pos_in = [&](int row, int col) -> int {
+      const int *b = &hcol[hrp[row]], *e = &hcol[hrp[row + 1]];
+      const int *f = std::lower_bound(b, e, col);
+      return (f != e && *f == col) ? (int)(f - b) : -1;
+    };
+    // the bytes row r (plane k of a colour with nc chains) needs
+    auto row_bytes = [&](int r, int k, int nc, uint8_t *out) -> bool {
+      std::fill(out, out + NB, (uint8_t)0xFF);
+      const int jd = hsd[r], jp = k > 0 ? jd - 1 : jd;
+      if (jd > 16) return false;
+      for (int j = 1; j < jp; ++j) {
+        const int c = hcol[hrp[r] + j];
+        for (int a = 0; a < j; ++a) {
+          const int b = pos_in(c, hcol[hrp[r] + a]);
+          if (b >= 0) out[j * (j - 1) / 2 + a] = (uint8_t)b;
+        }
+      }
+      if (k > 0) {
+        const int pred = r - nc;
+        if (hcol[hrp[r] + jd - 1] != pred) return false;
+        for (int a = 0; a < jp; ++a) {
+          const int b = pos_in(pred, hcol[hrp[r] + a]);
+          if (b >= 0) out[120 + a] = (uint8_t)b;
+        }
+      }
+      for (int j = 0; j < jd; ++j) out[136 + j] = (uint8_t)(htpos[hrp[r] + j] - hrp[hcol[hrp[r] + j]]);
+      return true;
+    };
+    std::vector<uint8_t> tb((size_t)5 * nch * NB, 0xFF), tmp(NB);
+    for (const ColDesc &C : sched.cols) {
+      if (C.len * kChains > 256) return;  // the table kernel holds one row per thread
+      for (int cls = 0; cls < 5; ++cls) {
+        int krep = -1;
+        for (int k = 0; k < C.len && krep < 0; ++k)
+          if (fac_class(k, C.len) == cls) krep = k;
+        if (krep < 0) continue;
+        for (int t = 0; t < C.nc; ++t)
+          if (!row_bytes(C.base + t + krep * C.nc, krep, C.nc, &tb[((size_t)cls * nch + C.g0 + t) * NB])) return;
+      }
+    }
+    for (const ColDesc &C : sched.cols)
+      for (int k = 0; k < C.len; ++k) {
+        const int cls = fac_class(k, C.len);
+        for (int t = 0; t < C.nc; ++t) {
+          if (!row_bytes(C.base + t + k * C.nc, k, C.nc, tmp.data())) return;
+          if (std::memcmp(tmp.data(), &tb[((size_t)cls * nch + C.g0 + t) * NB], NB) != 0) return;
+        }
+      }
+    std::vector<uint32_t> w((size_t)5 * kFacWords * nch);
+    for (int cls = 0; cls < 5; ++cls)
+      for (int g = 0; g < nch; ++g) {
+        const uint8_t *b = &tb[((size_t)cls * nch + g) * NB];
+        for (int q = 0; q < kFacWords; ++q)
+          w[((size_t)cls * kFacWords + q) * nch + g] = (uint32_t)b[4 * q] | ((uint32_t)b[4 * q + 1] << 8) |
+                                                        ((uint32_t)b[4 * q + 2] << 16) | ((uint32_t)b[4 * q + 3] << 24);
+      }
+    ftab.upload(w, st);
+    ftab_nch = nch;
+    ftab_ok = true;
+  }
+
   // Full pattern in device numbering, from the tetrahedra (tetra != null)
   // or from a 1-based upper CSR (topol, ja) in natural order.
   void build_pattern(const int *tetra, const int *topol = nullptr, const int *ja = nullptr) {
@@ -637,6 +711,7 @@ struct Flow {
     pair_ptr.upload(hpp, st);
     if (hpr.empty()) hpr.push_back(make_int2(0, 0));
     pair_rc.upload(hpr, st);
+    build_factor_tables(hrp, hcol, hsd, htpos);
     so.upload(hso, st);
     scol.upload(hsc, st);
     slen.upload(hlen, st);
@@ -945,9 +1020,23 @@ struct Flow {
   void factor_t(FT *F) {
     const Sell sl = sell();
     if (ordering == CATHY_GPU_ORDER_COLUMN3) {
-      for (const ColDesc &C : sched.cols)
-        hipLaunchKernelGGL(k_ic_factor_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), fac_smem(C), st, C,
-                           sl, pairs(), stpos.p, A.p, isdir.p, F, dinv.p, flags.p + 1);
+      const FacTab ft{ftab.p, ftab_nch};
+      for (const ColDesc &C : sched.cols) {
+        const dim3 g(grid_for(C.nc, kChains)), b(256);
+        const size_t sm = fac_smem(C);
+        const int w = sweep_window(C.nf);
+        if (ftab_ok && w <= 12) {
+          switch (w) {
+            case 0: hipLaunchKernelGGL((k_ic_factor_col3t<FT, 0>), g, b, sm, st, C, sl, ft, A.p, isdir.p, F, dinv.p, flags.p + 1); break;
+            case 4: hipLaunchKernelGGL((k_ic_factor_col3t<FT, 4>), g, b, sm, st, C, sl, ft, A.p, isdir.p, F, dinv.p, flags.p + 1); break;
+            case 8: hipLaunchKernelGGL((k_ic_factor_col3t<FT, 8>), g, b, sm, st, C, sl, ft, A.p, isdir.p, F, dinv.p, flags.p + 1); break;
+            default: hipLaunchKernelGGL((k_ic_factor_col3t<FT, 12>), g, b, sm, st, C, sl, ft, A.p, isdir.p, F, dinv.p, flags.p + 1); break;
+          }
+        } else {
+          hipLaunchKernelGGL(k_ic_factor_col3<FT>, g, b, sm, st, C, sl, pairs(), stpos.p, A.p, isdir.p, F, dinv.p,
+                             flags.p + 1);
+        }
+      }
     } else {
       for (const ListLevel &L : sched.lv)
         hipLaunchKernelGGL(k_ic_factor_list<FT>, dim3(grid_for(L.cnt, 64)), dim3(64), 0, st, L, sched.rows.p,

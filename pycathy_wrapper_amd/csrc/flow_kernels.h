@@ -201,6 +201,35 @@ struct PairList {
 template <typename FT>
 __global__ void k_ic_factor_col3(ColDesc C, Sell sl, PairList pl, const int *tpos, const double *A,
                                  const uint8_t *isdir, FT *F, double *dinv, int *nfall);
+
+// IC(0) tables of an extruded mesh in the column ordering.  A row's
+// coupling pattern depends only on its chain g and its class: plane 0,
+// plane 1, planes 2..len-3, plane len-2, plane len-1 (the first and last
+// two planes see the surface / base rows, whose slot layouts differ).  Per
+// (class, g), kFacWords 32-bit words; word w at w[(cls*kFacWords + w)*nchain + g]:
+//   bytes [0, 120):   for lower slot j (1 <= j < 16) and own slot a < j,
+//                     byte j(j-1)/2 + a = slot b of row c = col(r, j) with
+//                     col(c, b) = col(r, a), or 0xFF: the update pairs
+//                     L_ra L_cb of L_rj;
+//   bytes [120, 136): byte 120 + a: the same for the chain predecessor;
+//   bytes [136, 152): byte 136 + j: slot of r in row col(r, j), j < 16 (the
+//                     transposed position of L_rj).
+// The set-up checks every row against the explicit pattern.
+constexpr int kFacWords = 40;
+struct FacTab {
+  const uint32_t *w;
+  int nchain;
+};
+__host__ __device__ inline int fac_class(int k, int len) {
+  return k == 0 ? 0 : (k == len - 1 ? 4 : (k == 1 ? 1 : (k == len - 2 ? 3 : 2)));
+}
+// Column ordering, one colour class, from the tables: every gather of a
+// row is issued before any is used and the row's own entries stay in
+// registers.  Needs len*kChains <= 256 and at most NW earlier-colour slots
+// per row.
+template <typename FT, int NW>
+__global__ void k_ic_factor_col3t(ColDesc C, Sell sl, FacTab ft, const double *A, const uint8_t *isdir, FT *F,
+                                  double *dinv, int *nfall);
 // LAST: the colour has no later-colour neighbours, so its backward sweep
 // runs in the same launch and y holds the final x for its rows.
 template <typename FT, int NW, bool LAST>

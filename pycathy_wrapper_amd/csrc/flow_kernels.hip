@@ -321,6 +321,68 @@ __device__ __forceinline__ double pair_dot(const PairList &pl, const FT *__restr
   return v;
 }
 
+// Pass 2 of the column factor (k_ic_factor_col3, k_ic_factor_col3t): the
+// chain recurrence d_k = sqrt(s_k - (q_k / d_(k-1))^2) in its squared form
+// e_k = s_k - q_k^2 / e_(k-1), one division per level, by one thread per
+// chain (e_k overwrites Sv); then, for all rows at once, d_k = sqrt(e_k),
+// 1/d_k and the predecessor entry l_k = q_k / d_(k-1).  A Dirichlet row is
+// an identity row and restarts its chain (its successor's l is 0); a
+// non-positive e_k (breakdown) is replaced by |a_kk| (or 1), as incldp.f
+// keeps the row's own diagonal.
+template <typename FT>
+__device__ __forceinline__ void factor_chains(const ColDesc &C, int t0, double *Sv, const double *Qv,
+                                              const int *Pd, const int *Pl, const int *Pt,
+                                              const double *__restrict__ A, FT *__restrict__ F,
+                                              double *__restrict__ dinv, int *nfall) {
+  const int tot = C.len * kChains;
+  if (threadIdx.x < kChains && t0 + threadIdx.x < C.nc) {
+    double e = 1.0;
+    bool pdir = true;
+    for (int k = 0; k < C.len; ++k) {
+      const int i = k * kChains + threadIdx.x;
+      const int pd = Pd[i];
+      if (pd < 0) {
+        pdir = true;
+        continue;
+      }
+      const double qq = (k > 0 && !pdir) ? Qv[i] * Qv[i] : 0.0;
+      double v = Sv[i] - qq / e;
+      if (!(v > 0.0)) {
+        const double a = fabs(A[pd]);
+        v = a > 0.0 ? a : 1.0;
+        atomicAdd(nfall, 1);
+      }
+      Sv[i] = v;
+      e = v;
+      pdir = false;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int k = i / kChains, t = t0 + (i % kChains);
+    if (t >= C.nc) continue;
+    const int r = C.base + t + k * C.nc;
+    const int pd = Pd[i];
+    if (pd < 0) {
+      if (k > 0) {
+        F[Pl[i]] = FT(0);
+        F[Pt[i]] = FT(0);
+      }
+      F[-pd - 1] = FT(1);
+      dinv[r] = 0.0;
+      continue;
+    }
+    const double d = sqrt(Sv[i]);
+    F[pd] = FT(d);
+    dinv[r] = 1.0 / d;
+    if (k > 0) {
+      const double l = Pd[i - kChains] >= 0 ? Qv[i] / sqrt(Sv[i - kChains]) : 0.0;
+      F[Pl[i]] = FT(l);
+      F[Pt[i]] = FT(l);
+    }
+  }
+}
+
 // Column ordering, one colour class.  Each work-group owns kChains chains.
 // In this ordering a row's lower neighbours are (a) nodes of earlier
 // colours, final before the launch, and (b) its same-column predecessor
@@ -385,45 +447,141 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, Pair
     Qv[i] = A[q] - pair_dot(pl, F, q);
   }
   __syncthreads();
-  if (threadIdx.x < kChains) {
-    const int t = t0 + threadIdx.x;
-    if (t < C.nc) {
-      double dprev = 1.0;
-      bool pdir = true;
-      for (int k = 0; k < C.len; ++k) {
-        const int i = k * kChains + threadIdx.x;
-        const int r = C.base + t + k * C.nc;
-        const int pd = Pd[i];
-        if (pd < 0) {
-          if (k > 0) {
-            F[Pl[i]] = FT(0);
-            F[Pt[i]] = FT(0);
-          }
-          F[-pd - 1] = FT(1);
-          dinv[r] = 0.0;
-          pdir = true;
-          continue;
-        }
-        double l = 0.0;
-        if (k > 0) {
-          if (!pdir) l = Qv[i] / dprev;
-          F[Pl[i]] = FT(l);
-          F[Pt[i]] = FT(l);
-        }
-        double v = Sv[i] - l * l;
-        if (!(v > 0.0)) {  // breakdown: keep the row's own diagonal
-          const double a = fabs(A[pd]);
-          v = a > 0.0 ? a : 1.0;
-          atomicAdd(nfall, 1);
-        }
-        const double d = sqrt(v);
-        F[pd] = FT(d);
-        dinv[r] = 1.0 / d;
-        dprev = d;
-        pdir = false;
+  factor_chains(C, t0, Sv, Qv, Pd, Pl, Pt, A, F, dinv, nfall);
+}
+
+// Byte i of a little-endian word array (i a compile-time constant after
+// unrolling).
+__device__ __forceinline__ int tab_byte(const uint32_t *w, int i) { return (int)((w[i >> 2] >> (8 * (i & 3))) & 0xFFu); }
+
+// k_ic_factor_col3's three passes, with pass 1 (earlier-colour entries)
+// and pass 1b (predecessor numerator) from the chain tables.  Each lower
+// entry L_rj = (a_rj - sum_a L_ra L_cb) / L_cc sums its pairs in ascending
+// a, as the pair lists do, with the row's own L_ra (rounded to FT, as
+// stored) held in registers.
+template <typename FT, int NW>
+__global__ __launch_bounds__(256) void k_ic_factor_col3t(ColDesc C, Sell sl, FacTab ft,
+                                                         const double *__restrict__ A,
+                                                         const uint8_t *__restrict__ isdir, FT *__restrict__ F,
+                                                         double *__restrict__ dinv, int *nfall) {
+  constexpr int NWc = NW > 0 ? NW : 1;
+  constexpr int NP = NW * (NW - 1) / 2;  // (j, a) pairs, a < j < NW
+  constexpr int NPW = NP > 0 ? (NP + 3) / 4 : 1;
+  extern __shared__ double smem[];
+  const int tot = C.len * kChains;
+  double *Sv = smem;                 // a_rr - sum earlier L^2
+  double *Qv = smem + tot;           // a_r,pred - earlier dot
+  int *Pd = (int *)(smem + 2 * tot); // slot of the diagonal (-(slot)-1: Dirichlet)
+  int *Pl = Pd + tot;                // slot of the predecessor entry
+  int *Pt = Pl + tot;                // its transpose slot
+  const int t0 = xcd_group() * kChains;
+  const int i = threadIdx.x, k = i / kChains, t = t0 + (i % kChains);
+  const bool live = k < C.len && t < C.nc;
+  double Lr[NWc];
+  uint32_t wpre[4];
+  int r = 0, base = 0, jd = 0, jp = 0;
+  if (live) {
+    r = C.base + t + k * C.nc;
+    const int g = C.g0 + t;
+    const RowPos rp = chain_pos(sl, g, k);
+    jd = sell_diag(sl, rp, r);
+    jp = k > 0 ? jd - 1 : jd;
+    base = sl.so[r >> 6] + (r & 63);
+    const uint32_t *tw = ft.w + (size_t)fac_class(k, C.len) * kFacWords * ft.nchain + g;
+    uint32_t wp[NPW], wt[4];
+#pragma unroll
+    for (int w = 0; w < NPW; ++w) wp[w] = tw[(size_t)w * ft.nchain];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      wpre[w] = tw[(size_t)(30 + w) * ft.nchain];
+      wt[w] = tw[(size_t)(34 + w) * ft.nchain];
+    }
+    const bool rdir = isdir[r] != 0;
+    const int jl = max(jp - 1, 0);
+    int c[NWc];
+    double av[NWc], dc[NWc];
+    int so_c[NWc];
+    uint8_t ic[NWc];
+    FT G[NP > 0 ? NP : 1];
+    if (NW > 0) {
+      sell_cols_window(sl, rp, base, 0, jl, c);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        av[j] = A[base + 64 * min(j, jl)];
+        dc[j] = dinv[c[j]];
+        ic[j] = isdir[c[j]];
+        so_c[j] = sl.so[c[j] >> 6] + (c[j] & 63);
       }
+#pragma unroll
+      for (int j = 1; j < NW; ++j)
+#pragma unroll
+        for (int a = 0; a < j; ++a) {
+          const int q = j * (j - 1) / 2 + a, b = tab_byte(wp, q);
+          G[q] = F[so_c[j] + 64 * (b == 0xFF ? 0 : b)];
+        }
+    }
+    if (rdir) {
+#pragma unroll
+      for (int j = 0; j < NW; ++j)
+        if (j < jp) {
+          F[base + 64 * j] = FT(0);
+          F[so_c[j] + 64 * tab_byte(wt, j)] = FT(0);
+        }
+#pragma unroll
+      for (int j = 0; j < NWc; ++j) Lr[j] = 0.0;
+      Pd[i] = -(base + 64 * jd) - 1;
+      Sv[i] = 1.0;
+    } else {
+      double s = A[base + 64 * jd];
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < j; ++a) {
+          const int q = j * (j - 1) / 2 + a;
+          v = fma(tab_byte(wp, q) != 0xFF ? Lr[a] : 0.0, (double)G[q], v);
+        }
+        const double L = (j < jp && !ic[j]) ? (av[j] - v) * dc[j] : 0.0;
+        const FT Lf = FT(L);
+        Lr[j] = (double)Lf;
+        if (j < jp) {
+          F[base + 64 * j] = Lf;
+          F[so_c[j] + 64 * tab_byte(wt, j)] = Lf;
+        }
+        s -= L * L;
+      }
+      Sv[i] = s;
+      Pd[i] = base + 64 * jd;
+    }
+    if (k > 0) {
+      const int pred = r - C.nc;
+      Pl[i] = base + 64 * (jd - 1);
+      Pt[i] = sl.so[pred >> 6] + (pred & 63) + 64 * tab_byte(wt, jd - 1);
+    } else {
+      Pl[i] = -1;
+      Pt[i] = -1;
     }
   }
+  __syncthreads();
+  // the predecessor's numerator: its own earlier-colour entries are final
+  // after the barrier
+  if (live && k > 0 && Pd[i] >= 0) {
+    const int pred = r - C.nc;
+    const int sp0 = sl.so[pred >> 6] + (pred & 63);
+    FT G2[NWc];
+#pragma unroll
+    for (int a = 0; a < NW; ++a) {
+      const int b = tab_byte(wpre, a);
+      G2[a] = F[sp0 + 64 * (b == 0xFF ? 0 : b)];
+    }
+    const int q = Pl[i];
+    double v = 0.0;
+#pragma unroll
+    for (int a = 0; a < NW; ++a) v = fma((a < jp && tab_byte(wpre, a) != 0xFF) ? Lr[a] : 0.0, (double)G2[a], v);
+    Qv[i] = A[q] - v;
+  }
+  __syncthreads();
+  factor_chains(C, t0, Sv, Qv, Pd, Pl, Pt, A, F, dinv, nfall);
 }
 
 // The chain recurrence v_k = b_k - a_k v_(k-1) (DOWN: k ascending) or
@@ -704,9 +862,16 @@ __global__ void k_bwd_list(ListLevel L, const int *__restrict__ rows, Sell sl, c
                                                    const double *, double *, const int *);                  \
   template __global__ void k_bwd_col3<FT, U>(ColDesc, Sell, const FT *, const double *, const double *,      \
                                              double *, const int *);
+#define CG_INST_FTAB(FT, W)                                                                               \
+  template __global__ void k_ic_factor_col3t<FT, W>(ColDesc, Sell, FacTab, const double *, const uint8_t *, \
+                                                    FT *, double *, int *);
 #define CG_INST_FT(FT)                                                                                      \
   template __global__ void k_ic_factor_col3<FT>(ColDesc, Sell, PairList, const int *, const double *,      \
                                                 const uint8_t *, FT *, double *, int *);                    \
+  CG_INST_FTAB(FT, 0)                                                                                       \
+  CG_INST_FTAB(FT, 4)                                                                                       \
+  CG_INST_FTAB(FT, 8)                                                                                       \
+  CG_INST_FTAB(FT, 12)                                                                                      \
   CG_INST_SWEEP(FT, 0)                                                                                      \
   CG_INST_SWEEP(FT, 4)                                                                                      \
   CG_INST_SWEEP(FT, 8)                                                                                      \

@@ -182,6 +182,34 @@ def test_flow_heterogeneous_matches_oracle(gpu, oracle, ordering):
         run.close()
 
 
+def test_flow_deep_columns_match_oracle(gpu, oracle):
+    """Columns of 36 nodes (35 layers): more levels than the column kernels'
+    one-row-per-thread forms hold (32), so the factor takes the pair-list
+    kernel and the sweeps their serial chain loop.  6 steps of a
+    heterogeneous 13x13-node hillslope against the oracle."""
+    pr = CF.hillslope_large(cells=12, nstr=35, params=CF.hillslope_small_params(tmax=172800.0))
+    assert pr.mesh.nstr + 1 > 32
+    g = gpu.FlowGPU(pr, gpu.ORDER_COLUMN3)
+    run = _oracle_run(oracle, pr)
+    scale = 1e-6 * pr.mesh.volnod.sum()
+    try:
+        for k in range(6):
+            rc_o, so = run.step()
+            rc_g, sg = g.advance()
+            say(f"deep step {k + 1}: dt {sg['DELTAT']:.4g}/{so['DELTAT']:.4g} iter {sg['ITER']:.0f}/"
+                f"{so['ITER']:.0f} pcg {sg['NITERT']:.0f}/{so['NITERT']:.0f}")
+            assert rc_g == rc_o
+            assert sg["DELTAT"] == so["DELTAT"] and sg["ITER"] == so["ITER"] and sg["BACK"] == so["BACK"]
+            assert _rel_inf(g.psi(), run.psi()) <= PSI_TOL
+            for key in ("DSTORE", "VIN", "VOUT", "ERRAS"):
+                assert _mb_close(sg[key], so[key], scale), (k, key, sg[key], so[key])
+            if rc_o:
+                break
+    finally:
+        g.close()
+        run.close()
+
+
 @pytest.mark.parametrize("ordering", ["column3", "natural"])
 def test_flow_full_size_first_step_matches_oracle(gpu, oracle, ordering):
     """The 1.25M-node workload (201x201 nodes, 30 layers, 7.2M tets): the
