@@ -846,7 +846,7 @@ struct Flow {
     }
     const size_t tot = hoffp[ns];
     std::vector<int> hie(tot, -1);
-    std::vector<uint32_t> hio(tot, 0u);
+    std::vector<uint32_t> hio(tot, 0xFFFFFFFFu);  // padding: no slot byte is 0xFF
     std::vector<double4> hs4(tot, make_double4(0, 0, 0, 0));
     std::vector<double> hw2(nnz_sell, 0.0), hgw(nnz_sell, 0.0);
     for (int r = 0; r < n; ++r) {

@@ -177,8 +177,8 @@ __global__ void k_assemble(AsmArgs a);
 // Precomputed lumped-mass assembly plan.  Per row, one record per incident
 // element (ascending element id), lane-interleaved per slice: record m of
 // row r at off[r>>6] + 64 m + (r&63) holds the element id (-1 = padding),
-// the row slots of its 4 nodes (4 bytes) and the element-matrix row
-// S_e(k, 0..3) as a double4.  w2 / gw (SELL layout) give the lumped
+// the row slots of its 4 nodes (4 bytes; 0xFFFFFFFF = padding, as no slot
+// byte is 0xFF) and the element-matrix row S_e(k, 0..3) as a double4.  w2 / gw (SELL layout) give the lumped
 // storage and gravity terms as weights of the nodal ET1 / kr of the row's
 // neighbours.
 struct AsmPlan {

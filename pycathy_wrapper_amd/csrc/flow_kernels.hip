@@ -226,24 +226,28 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
       grav += P.gw[q] * ckrw_node[c];
     }
   }
-  double K[MR];
+  // element contributions accumulate by slot in the thread's own LDS
+  // column (slot MR absorbs masked records): one read-modify-write each,
+  // in record order per slot, where a register array indexed by the
+  // record's slot would take a compare and select per slot
+  __shared__ double kacc[(MR + 1) * kBlock];
 #pragma unroll
-  for (int j = 0; j < MR; ++j) K[j] = 0.0;
+  for (int j = 0; j <= MR; ++j) kacc[j * kBlock + threadIdx.x] = 0.0;
   const int p0 = P.off[s] + lane;
   const int cw = (P.off[s + 1] - P.off[s]) >> 6;
   constexpr int RB = 8;  // element records per batch
   for (int m0 = 0; m0 < cw; m0 += RB) {
-    int e[RB];
     uint32_t off[RB];
     double4 sv[RB];
     double kr[RB];
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
       const int q = p0 + 64 * min(m0 + b, cw - 1);
-      e[b] = P.ie[q];
       off[b] = P.io[q];
       sv[b] = P.s4[q];
     }
+    // the element ids are read only where kr_e comes from the element
+    // means; a padding record's slot word marks it
     if (local_kr) {
       // nodelt.f order: (((k1 + k2) + k3) + k4) / 4, nodes in element order
 #pragma unroll
@@ -256,22 +260,27 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
         kr[b] = (((s1 + s2) + s3) + s4) * 0.25;
       }
     } else {
+      int e[RB];
+#pragma unroll
+      for (int b = 0; b < RB; ++b) e[b] = P.ie[p0 + 64 * min(m0 + b, cw - 1)];
 #pragma unroll
       for (int b = 0; b < RB; ++b) kr[b] = a.ckrwe[max(e[b], 0)];
     }
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
-      const bool live = m0 + b < cw && e[b] >= 0;
+      const bool live = m0 + b < cw && off[b] != 0xFFFFFFFFu;
       const double v[4] = {live ? kr[b] * sv[b].x : 0.0, live ? kr[b] * sv[b].y : 0.0,
                            live ? kr[b] * sv[b].z : 0.0, live ? kr[b] * sv[b].w : 0.0};
 #pragma unroll
       for (int l = 0; l < 4; ++l) {
         const int o = live ? (int)((off[b] >> (8 * l)) & 0xff) : MR;
-#pragma unroll
-        for (int j = 0; j < MR; ++j) K[j] += (j == o) ? v[l] : 0.0;
+        kacc[o * kBlock + threadIdx.x] += v[l];
       }
     }
   }
+  double K[MR];
+#pragma unroll
+  for (int j = 0; j < MR; ++j) K[j] = kacc[j * kBlock + threadIdx.x];
   const int len = sell_len(a.sl, rp, r), dg = sell_diag(a.sl, rp, r);
   const double rdt = a.rdt, tetaf = a.tetaf;
   const double m4r = a.m4d[r];
