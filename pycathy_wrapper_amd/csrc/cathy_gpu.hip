@@ -1094,7 +1094,9 @@ struct Flow {
       }
       if (h_done) break;
       CG_REQUIRE(launched <= imax + 2, "PCG did not stop");
-      batch = std::max(2, std::min(8, imax - hs.niter + 1));
+      // later batches of two: counts fall over a step's Picard iterations, so
+      // larger follow-up batches mostly add launches that exit at once
+      batch = std::max(1, std::min(2, imax - hs.niter + 1));
     }
     *err_out = hs.err;
     return hs.niter;
