@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the CATHY subsurface Picard step on one MI355X.
 
-Workload: an ensemble of M members (default 2) of the 1.25M-node
+Workload: an ensemble of M members (default 3) of the 1.25M-node
 heterogeneous hillslope generator (201x201-node DEM, 2 m cells, 0-20 m
 spectral relief, 30 graded layers to 10 m, lognormal Ks per (layer, cell),
 +-20% vG alpha/n, 6 h of 5e-7 m/s rain then drainage, free drainage at the
@@ -352,7 +352,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=60.0, help="stop the CPU baseline after this long")
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the timed steps, run the copy kernel over this many MB (PMC calibration)")
-    ap.add_argument("--members", type=int, default=2, help="ensemble members run concurrently (streams)")
+    ap.add_argument("--members", type=int, default=3, help="ensemble members run concurrently (streams)")
     ap.add_argument("--stress", type=float, default=0.0,
                     help="instead: the stress ensemble (64 S + 48 M + 16 L members times this factor; dry start "
                          "psi = -5 m, 2 h of 2e-6 m/s rain; Newton for half the members, Picard for the rest)")
