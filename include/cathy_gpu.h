@@ -231,6 +231,17 @@ int cathy_gpu_flow3d_vel3d(int64_t *handle, const int *tetra,
                            double *vv, double *ww, double *unod, double *vnod,
                            double *wnod);
 
+/* PCG SpMV variant.  No counterpart in the reference (its products read
+ * the upper CSR once, solscal-extended.f:1326-1335).  ON != 0: each row
+ * reads its diagonal and upper entries and takes the lower ones from the
+ * transposed slots (the assembled matrix is bitwise symmetric), so HBM
+ * streams about one triangle; results are bitwise the same.  Faster for one
+ * member alone on the GPU, slower when several members share it; default 0.
+ * ACTIVE (may be NULL) receives 1 if the variant applies to this mesh
+ * (column ordering, per-chain tables valid, rows of at most 16 entries). */
+int cathy_gpu_flow3d_set_spmv_transposed(int64_t *handle, const int *on,
+                                         int *active);
+
 /* Counters of the last advance: Picard its, PCG its, IC(0) breakdown
  * fallbacks, kernel time of the linear solver (ms), total device ms. */
 int cathy_gpu_flow3d_counters(int64_t *handle, double *out6);

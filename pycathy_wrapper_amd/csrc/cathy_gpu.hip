@@ -77,6 +77,11 @@ struct Params {
 struct Flow {
   int n = 0, nt = 0, ntri = 0, nnod = 0, nstr = 0, nnz = 0, ordering = 0, lump = 1;
   bool f32 = true;  // IC(0) factor precision
+  // PCG SpMV reads the lower triangle through the transpose (column
+  // ordering with valid chain tables): one member alone runs faster, since
+  // the smaller HBM stream leaves the last-level cache to the sweeps; several
+  // members sharing the GPU run slower, so it is off unless asked for.
+  bool spmv_transposed = false;
   hipStream_t st = nullptr;
   std::vector<int> perm, iperm;  // CATHY id -> device id, device -> CATHY
 
@@ -1060,6 +1065,9 @@ struct Flow {
     precond_apply(r.p, p.p, nullptr);  // p = M^-1 r
     const int Gs = grid_for(n, kBlock);  // SpMV: one thread per row
     const int Gr = std::min(grid_for(n, kBlock), 1024);  // vector-kernel groups
+    // the SpMV reads the lower triangle through the transpose where the chain
+    // tables hold every row's transposed slots (column ordering, coded rows)
+    const FacTab fts{(spmv_transposed && ftab_ok && sell().coded && maxrow <= kUnroll) ? ftab.p : nullptr, ftab_nch};
     int batch = std::max(1, std::min(expect, imax));
     int launched = 0;
     int h_done = 0;
@@ -1069,11 +1077,11 @@ struct Flow {
         if (prof) {
           // the dispatch packet's own start/end timestamps (kernel time only)
           hipEvent_t e0 = next_event(), e1 = next_event();
-          hipExtLaunchKernelGGL(k_spmv_dot2, dim3(Gs), dim3(kBlock), 0, st, e0, e1, 0, n, sell(), A.p, p.p, r.p,
-                                Bv.p, P(1), P(2), done);
+          hipExtLaunchKernelGGL(k_spmv_dot2, dim3(Gs), dim3(kBlock), 0, st, e0, e1, 0, n, sell(), fts, A.p, p.p,
+                                r.p, Bv.p, P(1), P(2), done);
         } else {
-          hipLaunchKernelGGL(k_spmv_dot2, dim3(Gs), dim3(kBlock), 0, st, n, sell(), A.p, p.p, r.p, Bv.p, P(1),
-                             P(2), done);
+          hipLaunchKernelGGL(k_spmv_dot2, dim3(Gs), dim3(kBlock), 0, st, n, sell(), fts, A.p, p.p, r.p, Bv.p,
+                             P(1), P(2), done);
         }
         hipLaunchKernelGGL(k_pcg_alpha, dim3(Gr), dim3(kBlock), 0, st, n, Gs, P(1), P(2), Bv.p, p.p, isdir.p,
                            r.p, pdiff.p, P(3), scal.p, done);
@@ -1980,6 +1988,16 @@ int cathy_gpu_flow3d_vel3d(int64_t *handle, const int *tetra, const double *perm
     Flow *f = lookup(handle);
     CG_REQUIRE(f->initialised, "flow3d_init not called");
     f->velocities(tetra, permx, permy, permz, bi, ci, di, volur, ivol, uu, vv, ww, unod, vnod, wnod);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_set_spmv_transposed(int64_t *handle, const int *on, int *active) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(on, "null argument");
+    f->spmv_transposed = *on != 0;
+    if (active) *active = (f->spmv_transposed && f->ftab_ok && f->sell().coded && f->maxrow <= kUnroll) ? 1 : 0;
     return 0;
   });
 }

@@ -253,8 +253,12 @@ __global__ void k_bwd_list(ListLevel L, const int *rows, Sell sl, const FT *F, c
 
 __global__ void k_residual0(int n, Sell sl, const double *A, const double *x, const double *b,
                             const uint8_t *isdir, double *r, double *part);
-__global__ void k_spmv_dot2(int n, Sell sl, const double *A, const double *p, const double *r, double *B,
-                            double *part_pr, double *part_pb, const int *done);
+// ft.w != null (column ordering, chain tables valid): each row reads its
+// diagonal and upper slots and takes a lower entry a_rj = a_jr from row j's
+// upper slot (the assembled matrix is bitwise symmetric), so HBM serves
+// about one triangle and the last-level cache the other.
+__global__ void k_spmv_dot2(int n, Sell sl, FacTab ft, const double *A, const double *p, const double *r,
+                            double *B, double *part_pr, double *part_pb, const int *done);
 __global__ void k_pcg_alpha(int n, int nparts, const double *part_pr, const double *part_pb, const double *B,
                             const double *p, const uint8_t *isdir, double *r, double *x, double *part_rr,
                             PcgScal *sc, const int *done);

@@ -945,6 +945,41 @@ __device__ __forceinline__ double sell_row_dot(const Sell &sl, const double *__r
   return acc;
 }
 
+// sell_row_dot reading the strict lower triangle through the transpose:
+// slot j < diag takes a_jr from row col(r, j) at the slot the chain table
+// names (byte 136 + j); the other slots read the row's own values.  The
+// products and their order are sell_row_dot's (coded rows of at most
+// kUnroll slots; the caller checks).
+__device__ __forceinline__ double sell_row_dot_sym(const Sell &sl, const FacTab &ft, const double *__restrict__ A,
+                                                   const double *__restrict__ x, int r) {
+  const int s = r >> 6, lane = r & 63;
+  const int o0 = sl.so[s], w = (sl.so[s + 1] - o0) >> 6;
+  const RowPos rp = row_pos(sl, r);
+  const int q0 = o0 + lane;
+  const int jd = sell_diag(sl, rp, r);
+  const uint32_t *tw = ft.w + (size_t)fac_class(rp.k, sl.nplane) * kFacWords * ft.nchain + rp.g;
+  uint32_t wt[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) wt[u] = tw[(size_t)(34 + u) * ft.nchain];
+  uint32_t cw[kUnroll];
+#pragma unroll
+  for (int j = 0; j < kUnroll; ++j) cw[j] = code_word(sl, rp, min(j, w - 1));
+  int c[kUnroll], sc[kUnroll];
+#pragma unroll
+  for (int j = 0; j < kUnroll; ++j) c[j] = code_col(sl, rp.k, cw[j]);
+#pragma unroll
+  for (int j = 0; j < kUnroll; ++j) sc[j] = sl.so[c[j] >> 6] + (c[j] & 63);
+  double a[kUnroll], xv[kUnroll];
+#pragma unroll
+  for (int j = 0; j < kUnroll; ++j) xv[j] = x[c[j]];
+#pragma unroll
+  for (int j = 0; j < kUnroll; ++j) a[j] = A[j < jd ? sc[j] + 64 * tab_byte(wt, j) : q0 + 64 * min(j, w - 1)];
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < kUnroll; ++j) acc = fma(j < w ? a[j] : 0.0, xv[j], acc);
+  return acc;
+}
+
 // Partial sum of b_free^2 (XLUNG) and r = b - A x0.
 __global__ void k_residual0(int n, Sell sl, const double *__restrict__ A, const double *__restrict__ x,
                             const double *__restrict__ b, const uint8_t *__restrict__ isdir,
@@ -961,7 +996,7 @@ __global__ void k_residual0(int n, Sell sl, const double *__restrict__ A, const 
 }
 
 // B = A p with partial (p,r) and (p,B); one thread per row.
-__global__ void k_spmv_dot2(int n, Sell sl, const double *__restrict__ A, const double *__restrict__ p,
+__global__ void k_spmv_dot2(int n, Sell sl, FacTab ft, const double *__restrict__ A, const double *__restrict__ p,
                             const double *__restrict__ r, double *__restrict__ B,
                             double *__restrict__ part_pr, double *__restrict__ part_pb,
                             const int *__restrict__ done) {
@@ -970,7 +1005,7 @@ __global__ void k_spmv_dot2(int n, Sell sl, const double *__restrict__ A, const 
   double apr = 0.0, apb = 0.0;
   const int i = xcd_group() * blockDim.x + threadIdx.x;
   if (i < n) {
-    const double s = sell_row_dot(sl, A, p, i);
+    const double s = ft.w ? sell_row_dot_sym(sl, ft, A, p, i) : sell_row_dot(sl, A, p, i);
     B[i] = s;
     const double pi = p[i];
     apr = pi * r[i];

@@ -154,6 +154,15 @@ class FlowGPU:
         d = np.ascontiguousarray(dts, dtype=np.float64)
         _check(lib().cathy_gpu_flow3d_set_dt_schedule(C.byref(self.h), _i(d.size), _p(d)), "set_dt_schedule")
 
+    def spmv_transposed(self, on: bool = True) -> bool:
+        """PCG SpMV reading the lower triangle through the transpose (bitwise
+        the same results; faster for one member alone, slower when members
+        share the GPU).  Returns whether it applies to this mesh."""
+        act = C.c_int(0)
+        _check(lib().cathy_gpu_flow3d_set_spmv_transposed(C.byref(self.h), _i(1 if on else 0), C.byref(act)),
+               "set_spmv_transposed")
+        return bool(act.value)
+
     def guess_record(self, on: bool = True) -> None:
         _check(lib().cathy_gpu_flow3d_guess_record(C.byref(self.h), _i(1 if on else 0)), "guess_record")
 

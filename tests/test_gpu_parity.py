@@ -210,6 +210,25 @@ def test_flow_deep_columns_match_oracle(gpu, oracle):
         run.close()
 
 
+def test_spmv_transposed_matches_default(gpu):
+    """The PCG SpMV that takes the lower triangle from the transposed slots
+    gives the same bits as the default one (the assembled matrix is bitwise
+    symmetric): 6 steps of the heterogeneous 41x41x15 hillslope."""
+    pr = CF.hillslope_large(cells=40, nstr=15, params=CF.hillslope_small_params(tmax=172800.0))
+    runs = []
+    for transposed in (False, True):
+        g = gpu.FlowGPU(pr, gpu.ORDER_COLUMN3)
+        try:
+            assert g.spmv_transposed(transposed) == transposed
+            st = [g.advance()[1] for _ in range(6)]
+            runs.append((g.psi(), st))
+            say(f"transposed {transposed}: pcg {[s['NITERT'] for s in st]}")
+        finally:
+            g.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1] == runs[1][1]
+
+
 @pytest.mark.parametrize("ordering", ["column3", "natural"])
 def test_flow_full_size_first_step_matches_oracle(gpu, oracle, ordering):
     """The 1.25M-node workload (201x201 nodes, 30 layers, 7.2M tets): the
