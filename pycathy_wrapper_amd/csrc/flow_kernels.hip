@@ -397,9 +397,8 @@ __device__ __forceinline__ void factor_chains(const ColDesc &C, int t0, double *
 // colours, final before the launch, and (b) its same-column predecessor
 // (plane k-1), whose entries common with the row are all of earlier
 // colours.  So: pass 1a computes every row's earlier-colour L entries in
-// parallel, pass 1b the predecessor's numerator, and pass 2 runs the
-// 2-term recurrence d_k = sqrt(s_k - (q_k / d_{k-1})^2) down each chain
-// from LDS only.
+// parallel, pass 1b the predecessor's numerator, and pass 2
+// (factor_chains) the chain recurrence from LDS.
 template <typename FT>
 __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, PairList pl,
                                                         const int *__restrict__ tpos,
