@@ -524,8 +524,10 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3t(ColDesc C, Sell sl, Fac
       for (int j = 1; j < NW; ++j)
 #pragma unroll
         for (int a = 0; a < j; ++a) {
+          // absent pairs read F[0]: a wave whose rows all lack the pair
+          // issues one broadcast load (the value is multiplied by 0)
           const int q = j * (j - 1) / 2 + a, b = tab_byte(wp, q);
-          G[q] = F[so_c[j] + 64 * (b == 0xFF ? 0 : b)];
+          G[q] = F[b == 0xFF ? 0 : so_c[j] + 64 * b];
         }
     }
     if (rdir) {
@@ -580,7 +582,7 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3t(ColDesc C, Sell sl, Fac
 #pragma unroll
     for (int a = 0; a < NW; ++a) {
       const int b = tab_byte(wpre, a);
-      G2[a] = F[sp0 + 64 * (b == 0xFF ? 0 : b)];
+      G2[a] = F[b == 0xFF ? 0 : sp0 + 64 * b];
     }
     const int q = Pl[i];
     double v = 0.0;
