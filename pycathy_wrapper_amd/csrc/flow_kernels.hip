@@ -658,11 +658,13 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
       int c[NW > 0 ? NW : 1];
       float f[NW > 0 ? NW : 1];
       double yv[NW > 0 ? NW : 1];
+      // slots past the window read index 0 (a broadcast where the whole
+      // wave is past it) and are masked
 #pragma unroll
-      for (int j = 0; j < NW; ++j) f[j] = (float)F[base + 64 * min(j, jl)];
+      for (int j = 0; j < NW; ++j) f[j] = (float)F[j < jp ? base + 64 * j : 0];
       sell_cols_window(sl, rp, base, 0, jl, c);
 #pragma unroll
-      for (int j = 0; j < NW; ++j) yv[j] = y[c[j]];
+      for (int j = 0; j < NW; ++j) yv[j] = y[j < jp ? c[j] : 0];
 #pragma unroll
       for (int j = 0; j < NW; ++j) s = fma(-(j < jp ? (double)f[j] : 0.0), yv[j], s);
     } else {
@@ -749,11 +751,13 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
       int c[NW > 0 ? NW : 1];
       float f[NW > 0 ? NW : 1];
       double xv[NW > 0 ? NW : 1];
+      // slots past the row read index 0 (a broadcast where the whole wave
+      // is past it) and are masked
 #pragma unroll
-      for (int j = 0; j < NW; ++j) f[j] = (float)F[base + 64 * min(j0 + j, jl)];
+      for (int j = 0; j < NW; ++j) f[j] = (float)F[j0 + j < len ? base + 64 * (j0 + j) : 0];
       sell_cols_window(sl, rp, base, j0, jl, c);
 #pragma unroll
-      for (int j = 0; j < NW; ++j) xv[j] = x[c[j]];
+      for (int j = 0; j < NW; ++j) xv[j] = x[j0 + j < len ? c[j] : 0];
 #pragma unroll
       for (int j = 0; j < NW; ++j) s = fma(-(j0 + j < len ? (double)f[j] : 0.0), xv[j], s);
     } else {
