@@ -201,16 +201,16 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
     int c[kUnroll];
     double w2v[kUnroll], gwv[kUnroll], e1[kUnroll], ck[kUnroll];
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-      const int q = base + 64 * min(j, w - 1);
+    for (int j = 0; j < kUnroll; ++j) {  // slots past the slice width: one broadcast each
+      const int q = j < w ? base + 64 * j : 0;
       w2v[j] = P.w2[q];
       gwv[j] = P.gw[q];
     }
     sell_cols_window(a.sl, rp, base, 0, w - 1, c);
 #pragma unroll
     for (int j = 0; j < kUnroll; ++j) {
-      e1[j] = et1_node[c[j]];
-      ck[j] = ckrw_node[c[j]];
+      e1[j] = et1_node[j < w ? c[j] : 0];
+      ck[j] = ckrw_node[j < w ? c[j] : 0];
     }
 #pragma unroll
     for (int j = 0; j < kUnroll; ++j) {
@@ -241,8 +241,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
     double4 sv[RB];
     double kr[RB];
 #pragma unroll
-    for (int b = 0; b < RB; ++b) {
-      const int q = p0 + 64 * min(m0 + b, cw - 1);
+    for (int b = 0; b < RB; ++b) {  // records past the slice's count: one broadcast each
+      const int q = m0 + b < cw ? p0 + 64 * (m0 + b) : 0;
       off[b] = P.io[q];
       sv[b] = P.s4[q];
     }
@@ -927,8 +927,10 @@ __device__ __forceinline__ double sell_row_dot(const Sell &sl, const double *__r
     // for every earlier load at the next merge.
     int c[kUnroll];
     double a[kUnroll], xv[kUnroll];
+    // slots past the slice width (uniform over the wave) read index 0: one
+    // broadcast each
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) a[j] = A[q0 + 64 * min(j, w - 1)];
+    for (int j = 0; j < kUnroll; ++j) a[j] = A[j < w ? q0 + 64 * j : 0];
     if (sl.coded) {
       uint32_t cw[kUnroll];
 #pragma unroll
@@ -940,7 +942,7 @@ __device__ __forceinline__ double sell_row_dot(const Sell &sl, const double *__r
       for (int j = 0; j < kUnroll; ++j) c[j] = sl.col[q0 + 64 * min(j, w - 1)];
     }
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) xv[j] = x[c[j]];
+    for (int j = 0; j < kUnroll; ++j) xv[j] = x[j < w ? c[j] : 0];
 #pragma unroll
     for (int j = 0; j < kUnroll; ++j) acc = fma(j < w ? a[j] : 0.0, xv[j], acc);
   } else {
@@ -976,9 +978,9 @@ __device__ __forceinline__ double sell_row_dot_sym(const Sell &sl, const FacTab 
   for (int j = 0; j < kUnroll; ++j) sc[j] = sl.so[c[j] >> 6] + (c[j] & 63);
   double a[kUnroll], xv[kUnroll];
 #pragma unroll
-  for (int j = 0; j < kUnroll; ++j) xv[j] = x[c[j]];
+  for (int j = 0; j < kUnroll; ++j) xv[j] = x[j < w ? c[j] : 0];
 #pragma unroll
-  for (int j = 0; j < kUnroll; ++j) a[j] = A[j < jd ? sc[j] + 64 * tab_byte(wt, j) : q0 + 64 * min(j, w - 1)];
+  for (int j = 0; j < kUnroll; ++j) a[j] = A[j < jd ? sc[j] + 64 * tab_byte(wt, j) : (j < w ? q0 + 64 * j : 0)];
   double acc = 0.0;
 #pragma unroll
   for (int j = 0; j < kUnroll; ++j) acc = fma(j < w ? a[j] : 0.0, xv[j], acc);
