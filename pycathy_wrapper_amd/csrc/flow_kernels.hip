@@ -514,11 +514,12 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3t(ColDesc C, Sell sl, Fac
     if (NW > 0) {
       sell_cols_window(sl, rp, base, 0, jl, c);
 #pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        av[j] = A[base + 64 * min(j, jl)];
-        dc[j] = dinv[c[j]];
-        ic[j] = isdir[c[j]];
-        so_c[j] = sl.so[c[j] >> 6] + (c[j] & 63);
+      for (int j = 0; j < NW; ++j) {  // slots past jp: index 0 (broadcasts), unused
+        const int cj = j < jp ? c[j] : 0;
+        av[j] = A[j < jp ? base + 64 * j : 0];
+        dc[j] = dinv[cj];
+        ic[j] = isdir[cj];
+        so_c[j] = sl.so[cj >> 6] + (cj & 63);
       }
 #pragma unroll
       for (int j = 1; j < NW; ++j)
