@@ -353,6 +353,8 @@ def main() -> int:
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the timed steps, run the copy kernel over this many MB (PMC calibration)")
     ap.add_argument("--members", type=int, default=3, help="ensemble members run concurrently (streams)")
+    ap.add_argument("--fp64-factor", type=int, default=1,
+                    help="also run member 0's steps with an FP64 IC(0) factor (config.fp64_factor; 0 skips)")
     ap.add_argument("--stress", type=float, default=0.0,
                     help="instead: the stress ensemble (64 S + 48 M + 16 L members times this factor; dry start "
                          "psi = -5 m, 2 h of 2e-6 m/s rain; Newton for half the members, Picard for the rest)")
@@ -447,6 +449,30 @@ def main() -> int:
         f"{ens_steps / ens['wall']:.2f} member-steps/s (member 0 alone: {steps / wall:.2f} steps/s)")
     for _, gi in others:
         gi.close()
+    fp64 = None
+    if args.fp64_factor and args.ordering == "column3":
+        # the same steps of member 0 with its IC(0) factor in FP64 (the
+        # column ordering's factor is FP32 otherwise)
+        import numpy as np
+        psi32 = g.psi()
+        g.factor_fp64(True)
+        g.init(pr.psi0)
+        n64 = pic64 = pcg64 = 0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            rc, s = g.advance()
+            c = g.counters()
+            n64 += 1
+            pic64 += c["picard"]
+            pcg64 += c["pcg"]
+            if rc:
+                break
+        w64 = time.perf_counter() - t0
+        rel = float(np.max(np.abs(g.psi() - psi32)) / np.max(np.abs(psi32)))
+        g.factor_fp64(False)
+        fp64 = dict(steps=n64, steps_per_s=n64 / w64, picard=pic64, pcg=pcg64, psi_rel_diff_vs_fp32=rel)
+        log(f"[fp64 factor] {n64} steps in {w64:.3f}s -> {n64 / w64:.2f} steps/s: {pic64:.0f} Picard, {pcg64:.0f} "
+            f"PCG iterations; psi rel diff vs the FP32 factor {rel:.2e}")
     learned = None
     if args.learned_members > 0:
         learned = learned_leg(g, pr, args, dts, steps)
@@ -489,6 +515,7 @@ def main() -> int:
                               "pcg_iterations": pcg, "back_steps": backs},
             "learned_guess": False,
             "learned": learned,
+            "fp64_factor": fp64,
             "parallelism": "single GPU",
         },
         "roofline": {

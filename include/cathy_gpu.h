@@ -231,6 +231,13 @@ int cathy_gpu_flow3d_vel3d(int64_t *handle, const int *tetra,
                            double *vv, double *ww, double *unod, double *vnod,
                            double *wnod);
 
+/* IC(0) factor precision.  The reference factors in REAL*8 (INCLDP,
+ * solscal-extended.f:2142-2193); the natural ordering does so always.  In
+ * the column ordering the factor is FP32 unless ON != 0 selects FP64 (same
+ * elimination order, twice the factor's bytes).  Call between steps; the
+ * next solve refactors. */
+int cathy_gpu_flow3d_set_factor_fp64(int64_t *handle, const int *on);
+
 /* PCG SpMV variant.  No counterpart in the reference (its products read
  * the upper CSR once, solscal-extended.f:1326-1335).  ON != 0: each row
  * reads its diagonal and upper entries and takes the lower ones from the

@@ -955,6 +955,25 @@ struct Flow {
     CG_HIP(hipMemsetAsync(flags.p, 0, sizeof(int) * 2, st));
   }
 
+  // IC(0) factor precision of the column ordering (FP32 by default; the
+  // natural ordering keeps the reference's FP64).  Reallocates the factor.
+  void set_factor_fp64(bool on) {
+    CG_REQUIRE(on || ordering == CATHY_GPU_ORDER_COLUMN3, "the natural ordering keeps the FP64 factor");
+    if (on == !f32) return;
+    CG_HIP(hipStreamSynchronize(st));
+    if (on) {
+      F64.alloc(nnz_sell);
+      CG_HIP(hipMemsetAsync(F64.p, 0, sizeof(double) * nnz_sell, st));
+      F32.release();
+    } else {
+      F32.alloc(nnz_sell);
+      CG_HIP(hipMemsetAsync(F32.p, 0, sizeof(float) * nnz_sell, st));
+      F64.release();
+    }
+    f32 = !on;
+    CG_HIP(hipStreamSynchronize(st));
+  }
+
   // ------------------------------------------------------- kernels glue --
   size_t col_smem(const ColDesc &C) const { return sizeof(double) * 2 * (size_t)C.len * kChains; }
 
@@ -1998,6 +2017,15 @@ int cathy_gpu_flow3d_set_spmv_transposed(int64_t *handle, const int *on, int *ac
     CG_REQUIRE(on, "null argument");
     f->spmv_transposed = *on != 0;
     if (active) *active = (f->spmv_transposed && f->ftab_ok && f->sell().coded && f->maxrow <= kUnroll) ? 1 : 0;
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_set_factor_fp64(int64_t *handle, const int *on) {
+  return guarded([&] {
+    Flow *f = lookup(handle);
+    CG_REQUIRE(on, "null argument");
+    f->set_factor_fp64(*on != 0);
     return 0;
   });
 }

@@ -154,6 +154,11 @@ class FlowGPU:
         d = np.ascontiguousarray(dts, dtype=np.float64)
         _check(lib().cathy_gpu_flow3d_set_dt_schedule(C.byref(self.h), _i(d.size), _p(d)), "set_dt_schedule")
 
+    def factor_fp64(self, on: bool = True) -> None:
+        """FP64 IC(0) factor in the column ordering (FP32 by default; the
+        natural ordering is always FP64)."""
+        _check(lib().cathy_gpu_flow3d_set_factor_fp64(C.byref(self.h), _i(1 if on else 0)), "set_factor_fp64")
+
     def spmv_transposed(self, on: bool = True) -> bool:
         """PCG SpMV reading the lower triangle through the transpose (bitwise
         the same results; faster for one member alone, slower when members

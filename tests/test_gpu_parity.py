@@ -149,17 +149,20 @@ def _oracle_run(oracle, pr):
                              P.OrcParams(**pr.params.as_dict()))
 
 
-@pytest.mark.parametrize("ordering", ["natural", "column3"])
+@pytest.mark.parametrize("ordering", ["natural", "column3", "column3_fp64"])
 def test_flow_heterogeneous_matches_oracle(gpu, oracle, ordering):
     """41x41 nodes, 15 graded layers, per-cell lognormal Ks and +-20% vG
-    parameters, rain then drainage: 8 steps against the oracle."""
+    parameters, rain then drainage: 8 steps against the oracle (column3_fp64:
+    the column ordering with its IC(0) factor in FP64)."""
     pr = CF.hillslope_large(cells=40, nstr=15, params=CF.hillslope_small_params(tmax=172800.0))
     order = gpu.ORDER_NATURAL if ordering == "natural" else gpu.ORDER_COLUMN3
     g = gpu.FlowGPU(pr, order)
+    if ordering == "column3_fp64":
+        g.factor_fp64(True)
     lay = g.layout()
     # the column ordering decodes column ids from the extruded mesh's
     # per-chain codes (checked against the explicit ids at set-up)
-    assert lay["coded"] == (1 if ordering == "column3" else 0) and lay["code_mismatch"] == 0
+    assert lay["coded"] == (0 if ordering == "natural" else 1) and lay["code_mismatch"] == 0
     run = _oracle_run(oracle, pr)
     scale = 1e-6 * pr.mesh.volnod.sum()
     try:
