@@ -4,11 +4,18 @@ The CATHY main program cannot be linked in this image (it needs LAPACK for
 the transport code), but FLOW3D and everything it calls are in
 oracle/_ref/libcathy_ref.so.  This module drives them through the
 subsurface-only time loop of cathy_main.f:2882-3860 (ISIMGR<=1, IOPT=1,
-homogeneous atmospheric forcing, free drainage at the base, no seepage faces,
+atmospheric forcing (below), free drainage at the base, no seepage faces,
 no nudging, no ET), calling the reference's FLOW3D, NEUMANN, SWITCH_OLD,
-WEIGHT, CHVELO, STORCAL, TIMUPD and ICVDWT routines directly.  The only
-host logic restated here is what reads input files (ATMONE/ATMNXT, the
-atmbc reader; BKSTEP's BC re-reads), which has no file to read.
+WEIGHT, CHVELO, STORCAL, TIMUPD and ICVDWT routines directly.
+
+Atmospheric forcing comes one of two ways:
+- ``atmbc`` names an atmbc file (``write_atmbc``).  The reference's own
+  ATMONE and ATMNXT then read and interpolate it (the file is opened on
+  their input unit by REFATM_OPEN, oracle/refatm_io.f), and back-steps run
+  the reference's own BKSTEP, which calls ATMNXT or ATMBAK itself.
+- otherwise a single constant homogeneous record ``rain`` at t = 0, for
+  which ATMONE/ATMNXT are restated below (their file reads reduce to that
+  one record).
 
 The mesh is given to the reference as the host builder makes it; the reference
 itself is sized by its CATHY.H (NMAX=7056, MAXSTR=15, MAXZON=1,
@@ -68,7 +75,7 @@ FLOW3D_ARGS = [
 class RefFlowRun:
     """Reference-FLOW3D driven run.  Parameters follow orc_params."""
 
-    def __init__(self, ref: RefLib, mesh, soil, psi0, rain, prm: dict):
+    def __init__(self, ref: RefLib, mesh, soil, psi0, rain, prm: dict, atmbc: str | None = None):
         assert soil["nzone"] == 1, "reference CATHY.H has MAXZON=1"
         assert mesh.nstr <= REF_MAXSTR and mesh.n <= REF_NMAX
         self.ref = ref
@@ -155,6 +162,7 @@ class RefFlowRun:
         self.a = a
         self.mesh = mesh
         self.rain = rain
+        self.atmbc = atmbc
         self.nterm = nterm
         self._argv = [a[name].ctypes.data_as(C.c_void_p) for name, _ in FLOW3D_ARGS]
         self._init_state(prm)
@@ -205,7 +213,15 @@ class RefFlowRun:
         a["ITER"][0] = 1
         a["PNEW"][:n] = a["PTIMEP"][:n]
         self.nstep = 1
-        # ATMONE restated for a single constant homogeneous record at t=0
+        if self.atmbc is not None:
+            self._atmone()
+        else:
+            self._atmone_constant(prm)
+        self._init_tail()
+
+    def _atmone_constant(self, prm):
+        """ATMONE restated for a single constant homogeneous record at t=0."""
+        a, nnod = self.a, self.nnod
         ar = a["ARENOD"][:nnod]
         a["IFATM"][:nnod] = 0; a["IFATMP"][:nnod] = 0
         a["ATMOLD"][:nnod] = self.rain * ar  # atmtim(3) <= 0
@@ -222,6 +238,45 @@ class RefFlowRun:
             a["ATMACT"][i] = a["ATMPOT"][i] if a["IFATM"][i] == 0 else 0.0
             if a["IFATMP"][i] in (1, 2):
                 a["ATMOLD"][i] = 0.0
+
+    def _atm_common(self):
+        """Arguments shared by the reference's ATMONE and ATMNXT: NP, NQ,
+        CONTP, CONTQ, NSF = 0, NSFNUM, NSFNOD."""
+        a = self.a
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        self._nsf = np.zeros(4, dtype=np.int32)
+        return [ip(a["NP"]), ip(a["NQ"]), ip(a["CONTP"]), ip(a["CONTQ"]), C.byref(C.c_int(0)), ip(self._nsf),
+                ip(self._nsf)]
+
+    def _atmone(self):
+        """The reference's ATMONE on the atmbc file (atmone.f:9-163)."""
+        a, L = self.a, self.ref.L
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        np.ctypeslib.as_array((C.c_double * 1).in_dll(L, "feddes_"))[0] = self.p["scf"]  # SCF of /FEDDES/
+        codes = np.frombuffer(os.fsencode(self.atmbc), dtype=np.uint8).astype(np.int32)
+        ierr = C.c_int(0)
+        L.refatm_open_(ip(codes), C.byref(C.c_int(len(codes))), C.byref(ierr))
+        if ierr.value != 0:
+            raise OSError(f"REFATM_OPEN({self.atmbc}) failed: IOSTAT {ierr.value}")
+        self.ieto = np.zeros(1, dtype=np.int32)
+        L.atmone_(C.byref(C.c_int(self.nnod)), ip(a["HSPATM"]), ip(a["HTIATM"]), ip(self.ieto), dp(a["TIME"]),
+                  dp(a["DELTAT"]), dp(a["PONDH_MIN"]), ip(a["IFATM"]), ip(a["IFATMP"]), dp(a["ARENOD"]),
+                  dp(a["ATMPOT"]), dp(a["ATMACT"]), dp(a["ATMOLD"]), dp(a["ATMTIM"]), dp(a["ATMINP"]),
+                  dp(a["PNEW"]), dp(a["PTIMEP"]), *self._atm_common())
+
+    def _atmnxt(self):
+        """The reference's ATMNXT (atmnxt.f:9-104)."""
+        a = self.a
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        self.ref.L.atmnxt_(C.byref(C.c_int(self.nnod)), ip(a["HSPATM"]), ip(a["HTIATM"]), ip(self.ieto),
+                           dp(a["TIME"]), ip(a["IFATM"]), dp(a["ARENOD"]), dp(a["ATMPOT"]), dp(a["ATMACT"]),
+                           dp(a["ATMTIM"]), dp(a["ATMINP"]), dp(a["DELTAT"]), *self._atm_common(),
+                           C.byref(C.c_double(self.p["scf"])))
+
+    def _init_tail(self):
+        a, n, nnod = self.a, self.n, self.nnod
         self._copy("POLD", "PNEW", n)
         self._weight()
         self._copy("PTOLD", "PTNEW", n)
@@ -312,8 +367,44 @@ class RefFlowRun:
     def _flow3d(self):
         self.ref.L.flow3d_(*self._argv)
 
+    def _bkstep_ref(self):
+        """The reference's BKSTEP (bkstep.f:4-167) with constant
+        non-atmospheric BCs: PTIM(1) = PTIM(2) and QTIM(1) = QTIM(2) lie
+        beyond the run, so BKSTEP takes BCBAK, which returns at once
+        (bcbak.f: BCTIM(1) >= BCTIM(2)); no seepage faces (NSF = SFV(1) = 0),
+        no surface routing (SURF false).  It chooses ATMNXT or ATMBAK itself."""
+        a, n, nnod = self.a, self.n, self.nnod
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        i32 = lambda k=4: np.zeros(k, dtype=np.int32)
+        never = np.full(4, 1e300)
+        ints = {k: i32() for k in ("NDIR", "NDIRC", "NP", "NNEU", "NNEUC", "NQ", "HTIDIR", "HTINEU", "KBACKT",
+                                   "KBACK", "NSF", "NUMRES", "NCELNL", "SURF", "PONDING", "PONDP", "TRANSP",
+                                   "ANP", "SFV")}
+        ints["ANQ"] = np.array([a["NQ"][0], 0, 0, 0], dtype=np.int32)
+        nsf_arr = i32(64)
+        self.ref.L.bkstep_(
+            ip(a["IPRT1"]), C.byref(C.c_int(n)), C.byref(C.c_int(self.mesh.nstr)), ip(ints["NDIR"]),
+            ip(ints["NDIRC"]), ip(ints["NP"]), ip(ints["NNEU"]), ip(ints["NNEUC"]), ip(ints["NQ"]),
+            ip(a["HSPATM"]), ip(a["HTIATM"]), ip(self.ieto), ip(ints["HTIDIR"]), ip(ints["HTINEU"]),
+            ip(a["ITER"]), ip(a["NITERT"]), ip(ints["KBACKT"]), ip(ints["KBACK"]), C.byref(C.c_int(nnod)),
+            ip(ints["NSF"]), ip(ints["NUMRES"]), ip(ints["NCELNL"]), ip(a["CONTP"]), ip(a["CONTQ"]),
+            ip(a["IFATM"]), ip(a["IFATMP"]), ip(nsf_arr), ip(nsf_arr), ip(nsf_arr), ip(nsf_arr),
+            ip(ints["SURF"]), ip(ints["PONDING"]), ip(ints["PONDP"]), ip(a["DTGMIN"]),
+            dp(a["TIME"]), dp(a["TIMEP"]), dp(a["DELTAT"]), dp(a["DTMIN"]), dp(a["DTREDM"]), dp(a["DTREDS"]),
+            dp(a["TETAF"]), dp(a["OVFLNOD"]), dp(np.zeros(nnod + 1)), dp(a["PRESC"]), dp(never), dp(a["PINP"]),
+            dp(a["Q"]), dp(never), dp(a["QINP"]), dp(a["ATMTIM"]), dp(a["ATMINP"]), dp(a["ARENOD"]),
+            dp(a["POLD"]), dp(a["PTOLD"]), dp(a["PNEW"]), dp(a["PTNEW"]), dp(a["PTIMEP"]),
+            dp(a["ATMACT"]), dp(a["ATMPOT"]), dp(a["ATMOLD"]), dp(a["QPNEW"]), dp(a["QPOLD"]),
+            dp(a["SFQ"]), dp(a["SFQP"]), ip(ints["ANP"]), ip(ints["ANQ"]), ip(a["CONTP"]), ip(a["CONTQ"]),
+            ip(nsf_arr), ip(ints["SFV"]), ip(nsf_arr), ip(nsf_arr), dp(np.zeros(2)), ip(ints["TRANSP"]))
+        assert ints["NSF"][0] == 0 and ints["KBACK"][0] == 1
+
     def _bkstep(self):
-        """bkstep.f:58-147 restated (constant BCs; constant forcing)."""
+        """bkstep.f:58-147 restated (constant non-atmospheric BCs); the
+        reference's own BKSTEP where the forcing comes from a file."""
+        if self.atmbc is not None:
+            return self._bkstep_ref()
         a, n, nnod, p = self.a, self.n, self.nnod, self.p
         self._copy("PNEW", "PTIMEP", n)
         a["IFATM"][:nnod] = a["IFATMP"][:nnod]
@@ -328,7 +419,7 @@ class RefFlowRun:
         a["TIME"][0] = a["TIME"][0] + dt
         a["ITER"][0] = 1
         a["NITERT"][0] = 0
-        # constant forcing: ATMNXT/ATMBAK leave ATMPOT unchanged
+        # a constant record: ATMNXT/ATMBAK leave ATMPOT unchanged
         self._switch_old()
         self._copy("POLD", "PNEW", n)
         self._weight()
@@ -340,10 +431,12 @@ class RefFlowRun:
             return self.finished, {}
         a, n, nnod, p = self.a, self.n, self.nnod, self.p
         self._neumann(a["CKRW"])
-        # ATMNXT: constant record -> ATMACT = ATMPOT for IFATM = 0 nodes
-        for i in range(nnod):
-            if a["IFATM"][i] == 0:
-                a["ATMACT"][i] = a["ATMPOT"][i]
+        if self.atmbc is not None:
+            self._atmnxt()
+        else:   # ATMNXT on a constant record: ATMACT = ATMPOT for IFATM = 0 nodes
+            for i in range(nnod):
+                if a["IFATM"][i] == 0:
+                    a["ATMACT"][i] = a["ATMPOT"][i]
         self._switch_old()
         self._copy("POLD", "PNEW", n)
         self._weight()
@@ -399,3 +492,18 @@ class RefFlowRun:
 
     def sw(self):
         return self.a["SW"][:self.n].copy()
+
+
+def write_atmbc(path, times, vals, hspatm: int, ieto: int = 0) -> None:
+    """An atmbc file in the layout ATMONE/ATMNXT read (atmone.f:41-58):
+    "HSPATM IETO", then per record its time and its values (one value for
+    HSPATM != 0, NNOD values for HSPATM = 0), written with 17 significant
+    digits so the reads return the given doubles."""
+    times = np.asarray(times, dtype=np.float64)
+    vals = np.asarray(vals, dtype=np.float64).reshape(len(times), -1)
+    with open(path, "w") as f:
+        f.write(f"{hspatm} {ieto}\n")
+        for t, row in zip(times, vals):
+            f.write(f"{t:.17g}\n")
+            for i in range(0, len(row), 6):
+                f.write(" ".join(f"{v:.17g}" for v in row[i:i + 6]) + "\n")

@@ -179,6 +179,77 @@ def test_whole_run_bit_identical(oracle, ref, problem):
         run.close()
 
 
+def _atm_series(nnod, hspatm, seed=3):
+    """Rain and evaporation records at uneven times over the first 215 s
+    (the run's steps shrink to under a second near t = 200 s as the water
+    table falls): per-node rates of both signs (HSPATM = 0) or one rate
+    (HSPATM = 1).  The last record falls inside the run, so the reader meets
+    END (HTIATM = 1) as well."""
+    rng = np.random.default_rng(seed)
+    times = np.array([0.0, 40.0, 95.0, 150.0, 190.0, 215.0])
+    base = np.array([2e-7, 6e-7, -3e-7, 9e-7, -1e-7, 4e-7])
+    if hspatm:
+        return times, base
+    return times, (base[:, None] + 5e-7 * rng.uniform(-1.0, 1.0, (len(times), nnod))).ravel()
+
+
+@pytest.mark.parametrize("hspatm,ieto", [(0, 0), (1, 1)])
+def test_whole_run_atmbc_file_bit_identical(oracle, ref, problem, tmp_path, hspatm, ieto):
+    """ATMONE / ATMNXT / ATMBAK and BKSTEP pinned: the reference's own
+    readers take the forcing from an atmbc file, and back-steps run the
+    reference's BKSTEP (tests/refrun.py, oracle/refatm_io.f); the oracle
+    takes the same records as arrays.  Spatially variable
+    piecewise-linear records (HSPATM = 0, IETO = 0) and homogeneous step
+    records (HSPATM = 1, IETO = 1), with rain, evaporation, infiltration
+    switching (PMIN just above the initial surface head, so evaporating
+    nodes start head-limited) and back-steps; trajectories and IFATM
+    bit-identical for 70 steps."""
+    import dataclasses
+    import pyoracle as P
+    import refrun as R
+    m = problem.mesh
+    prm = dataclasses.replace(problem.params, ieto=ieto, pmin=-0.995, deltat=30.0).as_dict()
+    times, vals = _atm_series(m.nnod, hspatm)
+    path = str(tmp_path / "atmbc")
+    R.write_atmbc(path, times, vals, hspatm, ieto)
+    rr = R.RefFlowRun(ref, m, problem.soil, problem.psi0, 0.0, prm, atmbc=path)
+    bak_calls = []      # back-steps on which BKSTEP took ATMBAK (TIME <= ATMTIM(2), bkstep.f:115-122)
+    ref_bkstep = rr._bkstep_ref
+
+    def counted_bkstep():
+        atmtim = rr.a["ATMTIM"][:3].copy()
+        ref_bkstep()
+        if rr.a["TIME"][0] <= atmtim[1] and atmtim[0] < atmtim[1]:
+            bak_calls.append(rr.a["TIME"][0])
+    rr._bkstep_ref = counted_bkstep
+    run = oracle.run_create(dict(tetra=m.tetra, x=m.x, y=m.y, z=m.z, arenod=m.arenod, nnod=m.nnod, nstr=m.nstr,
+                                 ntri=m.ntri), problem.soil, problem.psi0, times, vals, hspatm, P.OrcParams(**prm))
+    backs = 0
+    switched = set()
+    try:
+        for k in range(70):
+            rc1, s1 = rr.step()
+            rc2, s2 = run.step()
+            assert rc1 == rc2
+            for key in ("DELTAT", "TIME", "ITER", "NITERT", "BACK", "DSTORE", "VIN", "VOUT", "ERRAS", "ERREL",
+                        "STORE1", "STORE2", "CDSTOR"):
+                assert s1[key] == s2[key], (k, key, s1[key], s2[key])
+            assert np.array_equal(rr.psi(), run.psi()), k
+            assert np.array_equal(rr.sw(), run.sw()), k
+            ifatm = rr.a["IFATM"][:m.nnod].copy()
+            assert np.array_equal(ifatm, run.ifatm()), k
+            switched.update(np.unique(ifatm).tolist())
+            backs += s1["BACK"]
+            if rc1:
+                break
+        assert rr.a["TIME"][0] > times[-1]          # the series ran out inside the run
+        assert rr.a["HTIATM"][0] == 1
+    finally:
+        run.close()
+    assert backs > 0 and bak_calls, "no back-step into ATMBAK with two distinct record times"
+    assert {0, 1} <= switched, switched
+
+
 # ---------------------------------------------------- mesh set-up pins --
 IP3 = np.array([1, 2, 3, 2, 3, 1, 3, 1, 2], dtype=np.int32)            # cathy_main.f:2490
 IP4 = np.array([1, 2, 3, 4, 2, 3, 4, 1, 3, 4, 1, 2, 4, 1, 2, 3], dtype=np.int32)  # cathy_main.f:2491
