@@ -120,7 +120,7 @@ def cpu_baseline(problem, max_steps: int, seconds: float) -> dict:
     m = problem.mesh
     orc = P.Oracle()
     run = orc.run_create(dict(tetra=m.tetra, x=m.x, y=m.y, z=m.z, arenod=m.arenod, nnod=m.nnod, nstr=m.nstr,
-                              ntri=m.ntri), problem.soil, problem.psi0, problem.atm_times, problem.atm_vals, 1,
+                              ntri=m.ntri), problem.soil, problem.psi0, problem.atm_times, problem.atm_vals, problem.hspatm,
                          P.OrcParams(**problem.params.as_dict()))
     steps = picard = pcg = 0
     t0 = time.perf_counter()

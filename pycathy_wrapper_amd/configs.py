@@ -71,10 +71,11 @@ class Problem:
     soil: dict            # permx/permy/permz/elstor/poros/vgn/vgrmc/vgpsat: (nstr, nzone); nzone
     psi0: np.ndarray
     atm_times: np.ndarray
-    atm_vals: np.ndarray  # (n_atm,) homogeneous (HSPATM=1)
+    atm_vals: np.ndarray  # (n_atm,) if hspatm else (n_atm * nnod,), record-major
     params: Params
     seed: int = 0
     meta: dict = field(default_factory=dict)
+    hspatm: int = 1       # HSPATM of the atmbc file: 1 homogeneous, 0 one value per surface node
 
 
 def sig4(a: np.ndarray) -> np.ndarray:

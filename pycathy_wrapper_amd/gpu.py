@@ -102,8 +102,11 @@ class FlowGPU:
         self.set_forcing(problem.atm_times, problem.atm_vals)
         self.init(problem.psi0)
 
-    def set_params(self, p):
-        ip = np.array([p.iopt, p.l2norm, p.ituns, p.ituns1, p.ituns2, p.itmxcg, p.ieto, p.free_drainage, 1],
+    def set_params(self, p, hspatm: int | None = None):
+        """IPRM/DPRM of cathy_gpu_flow3d_set_params; HSPATM defaults to the
+        problem's (and fixes the record width set_forcing expects)."""
+        hsp = int(self.problem.hspatm if hspatm is None else hspatm)
+        ip = np.array([p.iopt, p.l2norm, p.ituns, p.ituns1, p.ituns2, p.itmxcg, p.ieto, p.free_drainage, hsp],
                       dtype=np.int32)
         dp = np.array([p.tetaf, p.toluns, p.tolswi, p.ernlmx, p.tolcg, p.deltat, p.dtmin, p.dtmax, p.tmax,
                        p.dtmaga, p.dtmagm, p.dtreds, p.dtredm, p.pmin, p.pondh_min, p.scf], dtype=np.float64)
@@ -112,6 +115,9 @@ class FlowGPU:
     def set_forcing(self, times, vals):
         t = np.ascontiguousarray(times, dtype=np.float64)
         v = np.ascontiguousarray(vals, dtype=np.float64)
+        width = 1 if self.problem.hspatm else self.nnod
+        if v.size != len(t) * width:
+            raise ValueError(f"set_forcing: {v.size} values for {len(t)} records of width {width}")
         _check(lib().cathy_gpu_flow3d_set_forcing(C.byref(self.h), _i(len(t)), _p(t), _p(v)), "set_forcing")
 
     def init(self, psi0):

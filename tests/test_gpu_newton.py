@@ -77,7 +77,7 @@ def _oracle_run(oracle, pr):
     import pyoracle as P
     m = pr.mesh
     return oracle.run_create(dict(tetra=m.tetra, x=m.x, y=m.y, z=m.z, arenod=m.arenod, nnod=m.nnod, nstr=m.nstr,
-                                  ntri=m.ntri), pr.soil, pr.psi0, pr.atm_times, pr.atm_vals, 1,
+                                  ntri=m.ntri), pr.soil, pr.psi0, pr.atm_times, pr.atm_vals, pr.hspatm,
                              P.OrcParams(**pr.params.as_dict()))
 
 

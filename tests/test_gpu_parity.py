@@ -145,7 +145,7 @@ def _oracle_run(oracle, pr):
     import pyoracle as P
     m = pr.mesh
     return oracle.run_create(dict(tetra=m.tetra, x=m.x, y=m.y, z=m.z, arenod=m.arenod, nnod=m.nnod, nstr=m.nstr,
-                                  ntri=m.ntri), pr.soil, pr.psi0, pr.atm_times, pr.atm_vals, 1,
+                                  ntri=m.ntri), pr.soil, pr.psi0, pr.atm_times, pr.atm_vals, pr.hspatm,
                              P.OrcParams(**pr.params.as_dict()))
 
 
@@ -180,6 +180,55 @@ def test_flow_heterogeneous_matches_oracle(gpu, oracle, ordering):
                 assert _mb_close(sg[key], so[key], scale), (k, key, sg[key], so[key])
             if rc_o:
                 break
+    finally:
+        g.close()
+        run.close()
+
+
+def spatial_forcing_problem(ieto: int = 0):
+    """The heterogeneous 41x41-node hillslope under per-node atmospheric
+    records (HSPATM = 0): rain and evaporation of both signs at each record,
+    piecewise linear in time (IETO = 0) or held (IETO = 1), at record times
+    that the first steps cross (the steps last 0.02 to 0.05 s).
+    PMIN lies just above the initial surface head, so evaporating nodes start
+    head-limited (IFATM = 1) and switch as the records change sign."""
+    import dataclasses
+    pr = CF.hillslope_large(cells=40, nstr=15, params=CF.hillslope_small_params(
+        tmax=172800.0, deltat=0.1, pmin=-0.995, ieto=ieto))
+    rng = np.random.default_rng(11)
+    times = np.array([0.0, 0.04, 0.09, 0.14, 0.2, 5.0])
+    base = np.array([2e-7, 6e-7, -3e-7, 9e-7, -1e-7, 4e-7])
+    vals = (base[:, None] + 5e-7 * rng.uniform(-1.0, 1.0, (len(times), pr.mesh.nnod))).ravel()
+    return dataclasses.replace(pr, atm_times=times, atm_vals=vals, hspatm=0)
+
+
+@pytest.mark.parametrize("ordering,ieto", [("column3", 0), ("natural", 1)])
+def test_flow_spatial_forcing_matches_oracle(gpu, oracle, ordering, ieto):
+    """Per-node atmbc records (HSPATM = 0) through ATMONE/ATMNXT on the
+    device: 8 steps against the oracle, whose readers are pinned to the
+    reference's own on an atmbc file.  IFATM must agree node for node."""
+    pr = spatial_forcing_problem(ieto)
+    g = gpu.FlowGPU(pr, gpu.ORDER_NATURAL if ordering == "natural" else gpu.ORDER_COLUMN3)
+    run = _oracle_run(oracle, pr)
+    scale = 1e-6 * pr.mesh.volnod.sum()
+    seen = set()
+    try:
+        for k in range(8):
+            rc_o, so = run.step()
+            rc_g, sg = g.advance()
+            say(f"spatial forcing {ordering} step {k + 1}: t {sg['TIME']:.4g}/{so['TIME']:.4g} iter "
+                f"{sg['ITER']:.0f}/{so['ITER']:.0f} VIN {sg['VIN']:.6e}/{so['VIN']:.6e}")
+            assert rc_g == rc_o
+            assert sg["DELTAT"] == so["DELTAT"] and sg["ITER"] == so["ITER"] and sg["BACK"] == so["BACK"]
+            assert _rel_inf(g.psi(), run.psi()) <= PSI_TOL
+            for key in ("DSTORE", "VIN", "VOUT", "ERRAS"):
+                assert _mb_close(sg[key], so[key], scale), (k, key, sg[key], so[key])
+            ifatm = g.ifatm()
+            assert np.array_equal(ifatm, run.ifatm()), k
+            seen.update(np.unique(ifatm).tolist())
+            if rc_o:
+                break
+        assert so["TIME"] > 0.2 and {0, 1} <= seen, (so["TIME"], seen)
     finally:
         g.close()
         run.close()

@@ -167,7 +167,7 @@ def test_newton_whole_run_bit_identical(oracle, ref, problem):
     m = problem.mesh
     rr = R.RefFlowRun(ref, m, problem.soil, problem.psi0, float(problem.atm_vals[0]), problem.params.as_dict())
     run = oracle.run_create(dict(tetra=m.tetra, x=m.x, y=m.y, z=m.z, arenod=m.arenod, nnod=m.nnod, nstr=m.nstr,
-                                 ntri=m.ntri), problem.soil, problem.psi0, problem.atm_times, problem.atm_vals, 1,
+                                 ntri=m.ntri), problem.soil, problem.psi0, problem.atm_times, problem.atm_vals, problem.hspatm,
                             P.OrcParams(**problem.params.as_dict()))
     try:
         for k in range(40):
