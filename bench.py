@@ -340,6 +340,51 @@ def mixed_ensemble(args) -> int:
     return 0
 
 
+def route_bench(members: int, steps: int, warmup: int) -> int:
+    """Surface routing substeps (ROUTE + ALTEZZE, one launch for all
+    members) on the synthetic 201x201-cell network, through the C-ABI with
+    host buffers (each call copies the state in and out).  The oracle times
+    one member's substep on one core beside it."""
+    import numpy as np
+    from pycathy_wrapper_amd import routing as RT
+    net = RT.synthetic_network(201, 201, dx=10.0, seed=4)
+    rng = np.random.default_rng(21)
+    shp = (members, net.ngrid)
+    area = net.dx * net.dy
+    st = dict(surface_water=rng.uniform(0.0, 2e-6, shp) * area, q_in_kk=rng.uniform(0.0, 0.05, shp),
+              q_out_kk1=rng.uniform(0.0, 0.04, shp), q_out_kk2=rng.uniform(0.0, 0.02, shp),
+              volume_kk=rng.uniform(0.0, 0.01, shp) * area, q_in_kkp1=np.zeros(shp), q_out_kkp1_1=np.zeros(shp),
+              q_out_kkp1_2=np.zeros(shp), volume_kkp1=np.zeros(shp), h_water_kkp1=np.zeros(shp))
+    g = RT.RouteGPU(net)
+    log(f"[route] {net.ngrid} cells, {g.nlevel} levels, {members} members")
+    for _ in range(warmup):
+        g.step(15.0, st)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st["q_in_kkp1"][...] = 0.0
+        g.step(15.0, st)
+    dt = time.perf_counter() - t0
+    g.close()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as P
+    orc = P.Oracle()
+    one = {k: np.ascontiguousarray(v[0]) for k, v in st.items()}
+    nrep, t1 = 0, time.perf_counter()
+    while nrep < 3 or time.perf_counter() - t1 < 2.0:
+        one["q_in_kkp1"][...] = 0.0
+        orc.route(net, 15.0, one)
+        nrep += 1
+    cpu = (time.perf_counter() - t1) / nrep
+    print(json.dumps(dict(metric="routing member-substeps/s (ROUTE + ALTEZZE, 201x201 cells)",
+                          value=members * steps / dt, unit="member-substeps/s", n_gpus=1, steps=steps,
+                          warmup=warmup, ms_per_step=1e3 * dt / steps, higher_is_better=True, dtype="f64",
+                          data="synthetic", config=dict(workload="route_201x201", members=members,
+                                                        levels=g.nlevel, host_buffers=True),
+                          cpu_baseline=dict(value=1.0 / cpu, unit="member-substeps/s", cores=1, kind="port",
+                                            sample=f"{nrep} substeps of member 0"))), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,6 +409,9 @@ def main() -> int:
     ap.add_argument("--learned-members", type=int, default=8,
                     help="small members whose first steps train the learned initial iterate (0 skips the leg)")
     ap.add_argument("--learned-train-steps", type=int, default=4000)
+    ap.add_argument("--route-only", type=int, default=0, metavar="MEMBERS",
+                    help="time the surface routing substep (ROUTE + ALTEZZE) of MEMBERS members on a "
+                         "201x201-cell network instead, print its own JSON line and exit")
     ap.add_argument("--pmc-summary", nargs="+", metavar="DIR_OR_CSV",
                     help="summarise rocprofv3 --pmc CSV output (FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -376,6 +424,8 @@ def main() -> int:
 
     from pycathy_wrapper_amd import gpu
 
+    if args.route_only > 0:
+        return route_bench(args.route_only, args.steps, args.warmup)
     if args.mixed > 0 or args.stress > 0:
         return mixed_ensemble(args)
 

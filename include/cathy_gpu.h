@@ -254,6 +254,51 @@ int cathy_gpu_flow3d_set_spmv_transposed(int64_t *handle, const int *on,
 int cathy_gpu_flow3d_counters(int64_t *handle, double *out6);
 
 /* ------------------------------------------------------------------ *
+ * Surface routing substep (ISIMGR = 2): ROUTE + ALTEZZE.
+ * Replaces, per routing substep of SURF_ROUTE / cathy_main.f:3060-3068:
+ *   CALL ROUTE(NROW,NCELL,TIPO_R,RESERVR,CELLS_R,N_HA,TIME,DELTAT)
+ *     (route.f:4-5; MC mc.f:4-5)
+ *   CALL ALTEZZE(NCELL,TIPO_R,DELTA_X,DELTA_Y,Q_IN_KK_SN,...,ELEVATION)
+ *     (altezze.f:4-9)
+ * for basins without reservoirs (every TIPO_R = 0).  The reference reads
+ * the network from COMMON /RIVERNETWORK/ and the flows from /SURFWATER/;
+ * here they are arguments.
+ * ------------------------------------------------------------------ */
+/* The drainage network: QOI_SN(NCELL) (1-based I_BASIN = (I-1)*NROW + J,
+ * routing order), and the DTM_* grids of RIVERNETWORK.H, element (I, J)
+ * at (I-1) + (J-1)*LDC (LDC = COLMAX >= NCOL): P_OUTFLOW_1/2, W_1/2,
+ * LOCAL_SLOPE_1/2, EPL_1/2, KSS1_SF_1/2, WS1_SF_1/2, B1_SF, Y1_SF, NRC.
+ * Every routed branch (W /= 0; the last cell's cardinal branch excepted)
+ * must drain to a grid cell listed later in QOI_SN, else -1. */
+int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol,
+                           const int *ldc, const int *ncell, const int *qoi_sn,
+                           const int *p_outflow_1, const int *p_outflow_2,
+                           const double *w_1, const double *w_2,
+                           const double *local_slope_1, const double *local_slope_2,
+                           const double *epl_1, const double *epl_2,
+                           const double *kss1_sf_1, const double *kss1_sf_2,
+                           const double *ws1_sf_1, const double *ws1_sf_2,
+                           const double *b1_sf, const double *y1_sf, const double *nrc);
+/* info(4): levels of the network, most feeders of one cell, NCELL,
+ * NROW*NCOL. */
+int cathy_gpu_route_info(int64_t *handle, int *info);
+/* One substep for NMEMBER members sharing the network; every array is
+ * (NROW*NCOL, NMEMBER) indexed by I_BASIN (SURFWATER.H names).
+ * In: SURFACE_WATER_SN, Q_IN_KK_SN, Q_OUT_KK_SN_1/2, VOLUME_KK_SN.
+ * In/out: Q_IN_KKP1_SN (receives the upstream outflows), Q_OUT_KKP1_SN_1/2
+ * (a branch with W = 0 keeps its value).  Out: VOLUME_KKP1_SN,
+ * H_WATER_KKP1_SN.  cu_ak(2, NMEMBER): CU_MAX (out), AK_MAX (in/out: kept
+ * when no branch is routed).  Synchronous. */
+int cathy_gpu_route_step(int64_t *handle, const int *nmember, const double *deltat,
+                         const double *delta_x, const double *delta_y,
+                         const double *surface_water_sn, const double *q_in_kk_sn,
+                         const double *q_out_kk_sn_1, const double *q_out_kk_sn_2,
+                         const double *volume_kk_sn, double *q_in_kkp1_sn,
+                         double *q_out_kkp1_sn_1, double *q_out_kkp1_sn_2,
+                         double *volume_kkp1_sn, double *h_water_kkp1_sn, double *cu_ak);
+int cathy_gpu_route_destroy(int64_t *handle);
+
+/* ------------------------------------------------------------------ *
  * Instrumentation (no counterpart in the reference; used by bench.py).
  * ------------------------------------------------------------------ */
 /* Enable/disable HIP-event timing of the PCG SpMV kernel (k_spmv_dot2) on

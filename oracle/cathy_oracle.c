@@ -2143,3 +2143,99 @@ void orc_vnod3d(int n, int nt, const int *tp, const int *tetra, const double *uu
     wnod[i] = wnod[i] / tp[i];
   }
 }
+
+/* ---------------------------------------------------------------- routing */
+
+/* MC (SRC/mc.f:8-63): Muskingum-Cunge coefficients of one branch and its
+ * outflow at the new time level. */
+void orc_mc(double local_slope, double epl, double ks, double w, double b1, double y1, double deltat,
+            double q_in_kk, double q_in_kkp1, double q_out_kk, double q_overland, double *cu, double *ak,
+            double *q_out_kkp1) {
+  double qc = 1.0 / 3.0 * (q_in_kk + q_in_kkp1 + q_out_kk);
+  const double qcmin = 1.0e-05;
+  if (qc <= qcmin) qc = qcmin;
+  const double beta = atan(local_slope);
+  const double g = (1.0 - y1 + 2.0 / 3.0 * b1);
+  const double ck = 5.0 / (3.0 * g) * pow(ks, 3.0 / 5.0) * pow(w, -2.0 / 5.0) * pow(sin(beta), 3.0 / 1.0e1) *
+                    pow(qc, 1.0 - 3.0 * g / 5.0);
+  *ak = ck / epl;
+  *cu = ck * deltat / epl;
+  double dh = pow(qc, 1.0 - b1) / (2 * g * w * tan(beta));
+  if (dh < (1.0 - *cu)) dh = 1.0 - *cu;
+  const double x = 0.50 - dh / (ck * epl);
+  const double den = 2.0 * (1.0 - x) + *cu;
+  const double c1 = (*cu - 2.0 * x) / den;
+  const double c2 = (*cu + 2.0 * x) / den;
+  const double c3 = (2.0 * (1.0 - x) - *cu) / den;
+  const double c4 = (2.0 * ck * deltat) / den;
+  *q_out_kkp1 = c1 * q_in_kkp1 + c2 * q_in_kk + c3 * q_out_kk + c4 * q_overland;
+}
+
+/* ROUTE (SRC/route.f:4-234) for TIPO_R = 0: cells in QOI_SN order, each
+ * routing the cardinal (W_1) and diagonal (W_2) shares of its inflow and
+ * of the surface water over its cell, then adding each outflow to the
+ * downstream cell's Q_IN_KKP1 (the last cell's cardinal outflow leaves). */
+void orc_route(const orc_network *net, double deltat, const double *surface_water, const double *q_in_kk,
+               const double *q_out_kk1, const double *q_out_kk2, double *q_in_kkp1, double *q_out_kkp1_1,
+               double *q_out_kkp1_2, double *cu_ak) {
+  const int nrow = net->nrow;
+  double cu_max = 0.0;
+  for (int q = 0; q < net->ncell; ++q) {
+    const int ib = net->qoi[q];           /* 1-based I_BASIN */
+    const int jr = ib % nrow;
+    const int j = jr != 0 ? jr : nrow;
+    const int i = jr != 0 ? (ib - j) / nrow + 1 : ib / nrow;
+    const size_t g = (size_t)(i - 1) + (size_t)(j - 1) * net->ldc;
+    const double nrc = net->nrc[g];
+    const double sw = surface_water[ib - 1] / nrc;
+    for (int br = 0; br < 2; ++br) {
+      const double w = br ? net->w2[g] : net->w1[g];
+      if (w == 0) continue;
+      const double epl = br ? net->epl2[g] : net->epl1[g];
+      const double cr = 1.0 / epl;
+      const double q_overland = sw * w * cr;
+      const int pout = br ? net->pout2[g] : net->pout1[g];
+      /* NINT((P_OUTFLOW-5)/3.): a default-real quotient, rounded half away from zero */
+      const int iii = (int)lroundf((float)(pout - 5) / 3.0f);
+      const int jjj = pout - 5 - 3 * iii;
+      const int ib_cv = (i + iii - 1) * nrow + (j + jjj);
+      const double qin_kk = q_in_kk[ib - 1] * w / nrc;
+      const double qout_kk = (br ? q_out_kk2[ib - 1] : q_out_kk1[ib - 1]) / nrc;
+      const double qin_kkp1 = q_in_kkp1[ib - 1] * w / nrc;
+      double cu, ak, qout;
+      orc_mc(br ? net->slope2[g] : net->slope1[g], epl, br ? net->kss2[g] : net->kss1[g],
+             br ? net->ws2[g] : net->ws1[g], net->b1[g], net->y1[g], deltat, qin_kk, qin_kkp1, qout_kk,
+             q_overland, &cu, &ak, &qout);
+      if (qout < 0.0) qout = 0.0;
+      double *out = br ? q_out_kkp1_2 : q_out_kkp1_1;
+      out[ib - 1] = qout * nrc;
+      if (cu >= cu_max) {
+        cu_max = cu;
+        cu_ak[1] = ak;
+      }
+      if (br == 1 || q != net->ncell - 1) q_in_kkp1[ib_cv - 1] = q_in_kkp1[ib_cv - 1] + out[ib - 1];
+    }
+  }
+  cu_ak[0] = cu_max;
+}
+
+/* ALTEZZE (SRC/altezze.f:4-60), TIPO_R = 0: cell volume and water depth at
+ * the new time level. */
+void orc_altezze(int ncell, double dx, double dy, double deltat, const double *q_in_kk,
+                 const double *q_in_kkp1, const double *q_out_kk1, const double *q_out_kk2,
+                 const double *q_out_kkp1_1, const double *q_out_kkp1_2, const double *volume_kk,
+                 const double *surface_water, double *volume_kkp1, double *h_water_kkp1) {
+  for (int c = 0; c < ncell; ++c) {
+    const double dv = (q_in_kk[c] + q_in_kkp1[c]) / 2 * deltat + surface_water[c] * deltat -
+                      (q_out_kk1[c] + q_out_kk2[c]) / 2 * deltat -
+                      (q_out_kkp1_1[c] + q_out_kkp1_2[c]) / 2 * deltat;
+    const double v = volume_kk[c] + dv;
+    volume_kkp1[c] = v;
+    if (v >= 0.0) {
+      h_water_kkp1[c] = v / (dx * dy);
+    } else {
+      volume_kkp1[c] = 0.0;
+      h_water_kkp1[c] = 0.0;
+    }
+  }
+}

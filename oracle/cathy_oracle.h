@@ -359,6 +359,33 @@ int orc_run_nterm(const orc_run *r);
 const int *orc_run_topol(const orc_run *r);
 const int *orc_run_ja(const orc_run *r);
 
+/* ---- surface routing (ISIMGR = 2): one ROUTE + ALTEZZE substep ----------
+ * Grids of the drainage network are the reference's DTM_*(COLMAX, ROWMAX)
+ * arrays, element (I, J) at (I-1) + (J-1)*ldc, I = 1..ncol, J = 1..nrow;
+ * cell arrays are indexed by I_BASIN - 1 = (I-1)*nrow + J - 1.  qoi holds
+ * the 1-based I_BASIN of the ncell cells in routing order (QOI_SN).  Only
+ * cells without reservoirs (TIPO_R = 0) are restated; the network is taken
+ * as given.  Q_OUT_KKP1_SN_1/2 of a branch with zero weight keep their input
+ * values, as in the reference; q_in_kkp1 enters with the values the caller
+ * holds and receives the upstream outflows.  cu_ak[0] = CU_MAX (reset to 0),
+ * cu_ak[1] = AK_MAX (kept if no branch is routed). */
+typedef struct {
+  int nrow, ncol, ldc, ncell;
+  const int *qoi, *pout1, *pout2;
+  const double *w1, *w2, *slope1, *slope2, *epl1, *epl2, *kss1, *kss2, *ws1, *ws2, *b1, *y1, *nrc;
+} orc_network;
+
+void orc_mc(double local_slope, double epl, double ks, double w, double b1, double y1, double deltat,
+            double q_in_kk, double q_in_kkp1, double q_out_kk, double q_overland, double *cu, double *ak,
+            double *q_out_kkp1);
+void orc_route(const orc_network *net, double deltat, const double *surface_water, const double *q_in_kk,
+               const double *q_out_kk1, const double *q_out_kk2, double *q_in_kkp1, double *q_out_kkp1_1,
+               double *q_out_kkp1_2, double *cu_ak);
+void orc_altezze(int ncell, double dx, double dy, double deltat, const double *q_in_kk,
+                 const double *q_in_kkp1, const double *q_out_kk1, const double *q_out_kk2,
+                 const double *q_out_kkp1_1, const double *q_out_kkp1_2, const double *volume_kk,
+                 const double *surface_water, double *volume_kkp1, double *h_water_kkp1);
+
 #ifdef __cplusplus
 }
 #endif

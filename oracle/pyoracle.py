@@ -29,6 +29,8 @@ VG_NFIELD = len(VG_FIELDS)
 REF_NMAX = 441 * 16
 REF_MAXSTR = 15
 REF_MAXTRM = 52111
+REF_ROWMAX = REF_COLMAX = 20      # CATHY.H of the reference build: MAXCEL = 400, MAXRES = 1
+REF_MAXCEL = REF_ROWMAX * REF_COLMAX
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -55,6 +57,15 @@ class OrcParams(C.Structure):
         ("pmin", C.c_double), ("pondh_min", C.c_double), ("scf", C.c_double),
         ("ieto", C.c_int), ("free_drainage", C.c_int),
     ]
+
+
+NETWORK_REAL = ("w1", "w2", "slope1", "slope2", "epl1", "epl2", "kss1", "kss2", "ws1", "ws2", "b1", "y1", "nrc")
+
+
+class OrcNetwork(C.Structure):
+    _fields_ = [("nrow", C.c_int), ("ncol", C.c_int), ("ldc", C.c_int), ("ncell", C.c_int),
+                ("qoi", C.POINTER(C.c_int)), ("pout1", C.POINTER(C.c_int)), ("pout2", C.POINTER(C.c_int))] + \
+               [(k, C.POINTER(C.c_double)) for k in NETWORK_REAL]
 
 
 class Oracle:
@@ -140,6 +151,24 @@ class Oracle:
         return topol, ja[:nterm].copy(), tetja.reshape(-1, 4, 4)
 
     # --- whole run ----------------------------------------------------------
+    def route(self, net, deltat: float, st: dict) -> tuple[float, float]:
+        """orc_route + orc_altezze on one member's state dict (the keys of
+        pycathy_wrapper_amd.routing.RouteGPU.step); arrays updated in place.
+        Returns (CU_MAX, AK_MAX)."""
+        keep = [np.ascontiguousarray(a) for a in
+                (net.qoi.astype(np.int32), net.pout1.astype(np.int32), net.pout2.astype(np.int32))]
+        reals = [np.ascontiguousarray(getattr(net, k), dtype=np.float64) for k in NETWORK_REAL]
+        on = OrcNetwork(net.nrow, net.ncol, net.ncol, net.ncell, *[iptr(a) for a in keep], *[dptr(a) for a in reals])
+        cu_ak = np.array([0.0, float(st.get("ak_max", 0.0))])
+        self.L.orc_route(C.byref(on), C.c_double(deltat), *[dptr(st[k]) for k in
+                         ("surface_water", "q_in_kk", "q_out_kk1", "q_out_kk2", "q_in_kkp1", "q_out_kkp1_1",
+                          "q_out_kkp1_2")], dptr(cu_ak))
+        self.L.orc_altezze(C.c_int(net.nrow * net.ncol), C.c_double(net.dx), C.c_double(net.dy), C.c_double(deltat),
+                           *[dptr(st[k]) for k in ("q_in_kk", "q_in_kkp1", "q_out_kk1", "q_out_kk2",
+                                                   "q_out_kkp1_1", "q_out_kkp1_2", "volume_kk", "surface_water",
+                                                   "volume_kkp1", "h_water_kkp1")])
+        return float(cu_ak[0]), float(cu_ak[1])
+
     def run_create(self, mesh, soil, psi0, atm_times, atm_vals, hspatm, params: OrcParams):
         keep = dict(
             tetra=np.ascontiguousarray(mesh["tetra"], dtype=np.int32),

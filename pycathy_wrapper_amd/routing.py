@@ -1,0 +1,172 @@
+"""Surface routing substep (ISIMGR = 2): ROUTE + ALTEZZE on the device.
+
+The reference routes the surface water of every DEM cell down its drainage
+network once per routing substep (route.f:4-234, mc.f:8-63): cells are
+visited from the highest to the lowest (QOI_SN), each splitting its inflow
+and its share of surface water between a cardinal (W_1) and a diagonal
+(W_2) downstream cell with Muskingum-Cunge coefficients, and each outflow is
+added to the downstream cell's inflow Q_IN_KKP1_SN.  ALTEZZE
+(altezze.f:4-60) then updates every cell's water volume and depth.
+
+``Network`` holds the reference's RIVERNETWORK arrays for one DEM:
+DTM_*(I, J) at row J-1, column I-1 of an (nrow, ncol) array (so the flat
+index is (I-1) + (J-1)*ncol, the reference's column-major layout with
+COLMAX = ncol) and QOI_SN as 1-based I_BASIN = (I-1)*nrow + J.
+``RouteGPU`` runs the substep for a batch of ensemble members that share the
+network (cathy_gpu_route_* in include/cathy_gpu.h).  Only cells without
+reservoirs (TIPO_R = 0) are supported; the library rejects networks whose
+downstream cells do not come later in QOI_SN, or whose targets leave the
+grid.
+
+``synthetic_network`` builds a pit-free network on a tilted, perturbed
+plane.  It stands in for the DEM preprocessing (not part of this package):
+its cells drain toward cell (1, 1), which comes last.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from .gpu import CathyGpuError, _check, _i, _p, lib  # noqa: F401  (CathyGpuError re-exported)
+
+DTM_REAL = ("w1", "w2", "slope1", "slope2", "epl1", "epl2", "kss1", "kss2", "ws1", "ws2", "b1", "y1", "nrc")
+
+# D8 direction codes of DTM_P_OUTFLOW_*: code = 5 + 3*dI + dJ (route.f:95-98)
+CARDINAL = ((-1, 0), (1, 0), (0, -1), (0, 1))
+DIAGONAL = ((-1, -1), (-1, 1), (1, -1), (1, 1))
+
+
+@dataclass
+class Network:
+    nrow: int
+    ncol: int
+    qoi: np.ndarray        # (ncell,) int32, 1-based I_BASIN in routing order
+    pout1: np.ndarray      # (nrow, ncol) int32, DTM_P_OUTFLOW_1
+    pout2: np.ndarray      # (nrow, ncol) int32, DTM_P_OUTFLOW_2
+    w1: np.ndarray         # (nrow, ncol) float64, DTM_W_1 (likewise below)
+    w2: np.ndarray
+    slope1: np.ndarray
+    slope2: np.ndarray
+    epl1: np.ndarray
+    epl2: np.ndarray
+    kss1: np.ndarray
+    kss2: np.ndarray
+    ws1: np.ndarray
+    ws2: np.ndarray
+    b1: np.ndarray
+    y1: np.ndarray
+    nrc: np.ndarray
+    dx: float = 1.0
+    dy: float = 1.0
+
+    @property
+    def ncell(self) -> int:
+        return int(len(self.qoi))
+
+    @property
+    def ngrid(self) -> int:
+        return self.nrow * self.ncol
+
+
+def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0) -> Network:
+    """A drainage network on z = s*((I-1) + (J-1))*dx + noise (|noise| <
+    s*dx/4, so every cell but (1, 1) has a lower cardinal neighbour and no
+    two cells tie).  Each cell sends its water down its steepest cardinal
+    and steepest diagonal descents in proportion to their slopes; cells on
+    the low edges have no lower diagonal and send all of it cardinally.
+    Roughness, width and channel-shape parameters vary by cell."""
+    rng = np.random.default_rng(seed)
+    s = 0.02
+    I = np.arange(1, ncol + 1)[None, :]
+    J = np.arange(1, nrow + 1)[:, None]
+    z = s * ((I - 1) + (J - 1)) * dx + rng.uniform(-0.24, 0.24, (nrow, ncol)) * s * dx
+    ib = (I - 1) * nrow + J                                   # 1-based I_BASIN
+    pout1 = np.full((nrow, ncol), 5, dtype=np.int32)
+    pout2 = np.full((nrow, ncol), 5, dtype=np.int32)
+    sl1 = np.zeros((nrow, ncol))
+    sl2 = np.zeros((nrow, ncol))
+    for jj in range(nrow):
+        for ii in range(ncol):
+            for dirs, dist, po, sl in ((CARDINAL, dx, pout1, sl1), (DIAGONAL, dx * np.sqrt(2.0), pout2, sl2)):
+                best = 0.0
+                for di, dj in dirs:
+                    i2, j2 = ii + di, jj + dj
+                    if 0 <= i2 < ncol and 0 <= j2 < nrow:
+                        drop = (z[jj, ii] - z[j2, i2]) / dist
+                        if drop > best:
+                            best = drop
+                            po[jj, ii] = 5 + 3 * di + dj
+                sl[jj, ii] = best
+    w1 = np.where(sl1 + sl2 > 0, sl1 / np.maximum(sl1 + sl2, 1e-300), 0.0)
+    w2 = np.where(sl1 + sl2 > 0, sl2 / np.maximum(sl1 + sl2, 1e-300), 0.0)
+    w1[0, 0], w2[0, 0], pout1[0, 0], sl1[0, 0] = 1.0, 0.0, 2, s   # the outlet leaves through I = 0
+    order = np.argsort(-z.ravel(order="C"), kind="stable")
+    qoi = ib.ravel(order="C")[order].astype(np.int32)
+    f = lambda lo, hi: rng.uniform(lo, hi, (nrow, ncol))
+    return Network(nrow=nrow, ncol=ncol, qoi=qoi, pout1=pout1, pout2=pout2, w1=w1, w2=w2,
+                   slope1=np.maximum(sl1, 1e-3), slope2=np.maximum(sl2, 1e-3),
+                   epl1=np.full((nrow, ncol), dx), epl2=np.full((nrow, ncol), dx * np.sqrt(2.0)),
+                   kss1=f(5.0, 40.0), kss2=f(5.0, 40.0), ws1=f(0.5, 5.0), ws2=f(0.5, 5.0),
+                   b1=f(0.0, 0.5), y1=f(0.0, 0.3), nrc=np.round(f(1.0, 3.0)), dx=dx, dy=dx)
+
+
+STATE_IN = ("surface_water", "q_in_kk", "q_out_kk1", "q_out_kk2", "volume_kk")
+STATE_INOUT = ("q_in_kkp1", "q_out_kkp1_1", "q_out_kkp1_2")
+STATE_OUT = ("volume_kkp1", "h_water_kkp1")
+
+
+class RouteGPU:
+    """One drainage network on the device; ``step`` runs ROUTE + ALTEZZE
+    for ``nmember`` members at once (arrays [member][I_BASIN-1])."""
+
+    def __init__(self, net: Network):
+        self.net = net
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        self._keep = dict(qoi=i32(net.qoi), pout1=i32(net.pout1), pout2=i32(net.pout2),
+                          **{k: f64(getattr(net, k)) for k in DTM_REAL})
+        k = self._keep
+        h = C.c_int64(0)
+        _check(lib().cathy_gpu_route_create(
+            C.byref(h), _i(net.nrow), _i(net.ncol), _i(net.ncol), _i(net.ncell), _p(k["qoi"]), _p(k["pout1"]),
+            _p(k["pout2"]), *[_p(k[n]) for n in DTM_REAL]), "cathy_gpu_route_create")
+        self.h = h
+        info = np.zeros(4, dtype=np.int32)
+        _check(lib().cathy_gpu_route_info(C.byref(self.h), _p(info)), "cathy_gpu_route_info")
+        self.nlevel, self.maxpred = int(info[0]), int(info[1])
+
+    def step(self, deltat: float, state: dict) -> tuple[np.ndarray, np.ndarray]:
+        """Advance ``state`` (dict of float64 arrays, (nmember, nrow*ncol) or
+        (nrow*ncol,), indexed by I_BASIN - 1) in place: q_in_kkp1 / q_out_kkp1_* are read and written,
+        volume_kkp1 / h_water_kkp1 written.  Returns CU_MAX and AK_MAX per
+        member (AK_MAX enters as state["ak_max"] if present, else 0)."""
+        ncell = self.net.ngrid
+        arrs = {}
+        for k in STATE_IN + STATE_INOUT + STATE_OUT:
+            a = state[k]
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.size % ncell:
+                raise ValueError(f"{k}: float64, C-contiguous, a multiple of {ncell} values")
+            arrs[k] = a
+        nm = arrs["q_in_kk"].size // ncell
+        if any(a.size != nm * ncell for a in arrs.values()):
+            raise ValueError("every state array needs nmember * ncell values")
+        cu_ak = np.zeros((nm, 2))
+        cu_ak[:, 1] = np.asarray(state.get("ak_max", 0.0), dtype=np.float64)
+        _check(lib().cathy_gpu_route_step(
+            C.byref(self.h), _i(nm), C.byref(C.c_double(deltat)), C.byref(C.c_double(self.net.dx)),
+            C.byref(C.c_double(self.net.dy)), *[_p(arrs[k]) for k in STATE_IN + STATE_INOUT + STATE_OUT],
+            _p(cu_ak)), "cathy_gpu_route_step")
+        return cu_ak[:, 0].copy(), cu_ak[:, 1].copy()
+
+    def close(self):
+        if self.h.value:
+            lib().cathy_gpu_route_destroy(C.byref(self.h))
+            self.h = C.c_int64(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,194 @@
+"""The routing oracle (orc_route / orc_mc / orc_altezze) pinned against the
+reference's own ROUTE, MC and ALTEZZE (oracle/_ref, flang), which read the
+drainage network and the surface-water state from their COMMON blocks
+/RIVERNETWORK/ and /SURFWATER/ (RIVERNETWORK.H, SURFWATER.H): the test
+writes the same synthetic network and states there and requires
+bit-identical outflows, inflows, volumes, depths, CU_MAX and AK_MAX over
+several substeps.  Also checked without the reference: mass bookkeeping of
+the substep (every routed outflow but the outlet's arrives downstream) and
+Muskingum-Cunge special cases.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import has_ref
+from pycathy_wrapper_amd import routing as RT
+from pyoracle import REF_COLMAX, REF_MAXCEL, REF_ROWMAX, dptr, iptr
+
+NSCAL_NET = 7          # NORTH, SOUTH, EAST, WEST, FACTOR, DELTA_X, DELTA_Y
+NSCAL_SW = 5           # AK_MAX, AK_MAX_SAV, AK_MAX_P, CUTRGT, CU_MAX
+SW_ARRAYS = ("ELEVATION", "ELEVATION_WITH_LAKES", "CELLCOARSE", "H_WATER_KKP1_SN", "SURFACE_WATER_SN",
+             "Q_IN_KK_SN", "Q_IN_KK_SN_SAV", "Q_IN_KK_SN_P", "Q_IN_KKP1_SN", "Q_OUT_KK_SN_1",
+             "Q_OUT_KK_SN_1_SAV", "Q_OUT_KK_SN_2", "Q_OUT_KK_SN_2_SAV", "Q_OUT_KK_SN_1_P", "Q_OUT_KK_SN_2_P",
+             "Q_OUT_KKP1_SN_1", "Q_OUT_KKP1_SN_2", "VOLUME_KK_SN", "VOLUME_KK_SN_SAV", "VOLUME_KK_SN_P",
+             "VOLUME_KKP1_SN")       # /SURFWATER/ order after the scalars
+
+
+def initial_state(net, nmember: int = 1, seed: int = 5) -> dict:
+    """Surface water over each cell and flows of a previous substep."""
+    rng = np.random.default_rng(seed)
+    shp = (nmember, net.ncell) if nmember > 1 else (net.ncell,)
+    area = net.dx * net.dy
+    st = dict(surface_water=rng.uniform(0.0, 2e-6, shp) * area, q_in_kk=rng.uniform(0.0, 0.05, shp),
+              q_out_kk1=rng.uniform(0.0, 0.04, shp), q_out_kk2=rng.uniform(0.0, 0.02, shp),
+              volume_kk=rng.uniform(0.0, 0.01, shp) * area, q_in_kkp1=np.zeros(shp),
+              q_out_kkp1_1=np.zeros(shp), q_out_kkp1_2=np.zeros(shp), volume_kkp1=np.zeros(shp),
+              h_water_kkp1=np.zeros(shp))
+    return st
+
+
+def advance(st: dict) -> None:
+    """Substep bookkeeping of the caller: new levels become old, inflows
+    restart from zero."""
+    st["q_in_kk"][...] = st["q_in_kkp1"]
+    st["q_out_kk1"][...] = st["q_out_kkp1_1"]
+    st["q_out_kk2"][...] = st["q_out_kkp1_2"]
+    st["volume_kk"][...] = st["volume_kkp1"]
+    st["q_in_kkp1"][...] = 0.0
+
+
+class RefRouting:
+    """The reference's ROUTE + ALTEZZE over its COMMON blocks."""
+
+    def __init__(self, ref, net):
+        assert (net.nrow, net.ncol) == (REF_ROWMAX, REF_COLMAX) and net.ncell == REF_MAXCEL
+        L = ref.L
+        self.L, self.net = L, net
+        ng = REF_COLMAX * REF_ROWMAX
+        nd = NSCAL_NET + len(RT.DTM_REAL) * ng
+        base = C.addressof(C.c_char.in_dll(L, "rivernetwork_"))
+        self.rn_d = np.ctypeslib.as_array((C.c_double * nd).from_address(base))
+        self.rn_i = np.ctypeslib.as_array((C.c_int * (2 * ng + REF_MAXCEL)).from_address(base + 8 * nd))
+        sw_base = C.addressof(C.c_char.in_dll(L, "surfwater_"))
+        self.sw = np.ctypeslib.as_array((C.c_double * (NSCAL_SW + len(SW_ARRAYS) * REF_MAXCEL)).from_address(sw_base))
+        self.rn_d[:] = 0.0
+        self.rn_i[:] = 0
+        for k, name in enumerate(RT.DTM_REAL):          # DTM_*(I, J): flat (I-1) + (J-1)*COLMAX
+            self.rn_d[NSCAL_NET + k * ng: NSCAL_NET + (k + 1) * ng] = getattr(net, name).ravel()
+        self.rn_i[:ng] = net.pout1.ravel()
+        self.rn_i[ng:2 * ng] = net.pout2.ravel()
+        self.rn_i[2 * ng:] = net.qoi
+        self.tipo = np.zeros(REF_MAXCEL, dtype=np.int32)
+        self.one = np.ones(4, dtype=np.int32)
+
+    def arr(self, name):
+        k = SW_ARRAYS.index(name)
+        return self.sw[NSCAL_SW + k * REF_MAXCEL: NSCAL_SW + (k + 1) * REF_MAXCEL]
+
+    def step(self, deltat, st, ak_max=0.0):
+        for name, key in (("SURFACE_WATER_SN", "surface_water"), ("Q_IN_KK_SN", "q_in_kk"),
+                          ("Q_OUT_KK_SN_1", "q_out_kk1"), ("Q_OUT_KK_SN_2", "q_out_kk2"),
+                          ("Q_IN_KKP1_SN", "q_in_kkp1"), ("Q_OUT_KKP1_SN_1", "q_out_kkp1_1"),
+                          ("Q_OUT_KKP1_SN_2", "q_out_kkp1_2"), ("VOLUME_KK_SN", "volume_kk")):
+            self.arr(name)[:] = st[key]
+        self.sw[0] = ak_max
+        I = lambda v: C.byref(C.c_int(v))
+        D = lambda v: C.byref(C.c_double(v))
+        self.L.route_(I(self.net.nrow), I(self.net.ncell), iptr(self.tipo), iptr(self.one), iptr(self.one),
+                      iptr(self.one), D(0.0), D(deltat))
+        out = {k: np.zeros(REF_MAXCEL) for k in ("v", "h")}
+        hpool = np.zeros(4)
+        q = {n: np.ascontiguousarray(self.arr(n)) for n in ("Q_IN_KK_SN", "Q_IN_KKP1_SN", "Q_OUT_KK_SN_1",
+                                                             "Q_OUT_KK_SN_2", "Q_OUT_KKP1_SN_1",
+                                                             "Q_OUT_KKP1_SN_2", "VOLUME_KK_SN",
+                                                             "SURFACE_WATER_SN", "ELEVATION")}
+        self.L.altezze_(I(self.net.ncell), iptr(self.tipo), D(self.net.dx), D(self.net.dy), dptr(q["Q_IN_KK_SN"]),
+                        dptr(q["Q_IN_KKP1_SN"]), dptr(q["Q_OUT_KK_SN_1"]), dptr(q["Q_OUT_KK_SN_2"]),
+                        dptr(q["Q_OUT_KKP1_SN_1"]), dptr(q["Q_OUT_KKP1_SN_2"]), dptr(q["VOLUME_KK_SN"]),
+                        dptr(out["v"]), dptr(q["SURFACE_WATER_SN"]), dptr(out["h"]), D(deltat), dptr(hpool),
+                        iptr(self.one), dptr(q["ELEVATION"]))
+        return dict(q_in_kkp1=q["Q_IN_KKP1_SN"], q_out_kkp1_1=q["Q_OUT_KKP1_SN_1"],
+                    q_out_kkp1_2=q["Q_OUT_KKP1_SN_2"], volume_kkp1=out["v"], h_water_kkp1=out["h"],
+                    cu_max=self.sw[4], ak_max=self.sw[0])
+
+
+@pytest.fixture(scope="module")
+def ref():
+    import pyoracle
+    return pyoracle.RefLib()
+
+
+@pytest.fixture(scope="module")
+def net():
+    return RT.synthetic_network(REF_ROWMAX, REF_COLMAX, dx=10.0, seed=2)
+
+
+@pytest.mark.skipif(not has_ref(), reason="oracle/_ref not built (needs the reference sources)")
+def test_route_altezze_bit_identical(oracle, ref, net):
+    """Four substeps (dt = 30 s): the reference's ROUTE + ALTEZZE and the
+    oracle give bit-identical outflows, inflows, volumes, depths, CU_MAX and
+    AK_MAX.  Both branches are routed in most cells, and the last cell's
+    cardinal outflow leaves the grid."""
+    rr = RefRouting(ref, net)
+    st = initial_state(net)
+    assert (net.w2.ravel() > 0).sum() > net.ncell // 2
+    ak = 0.0
+    for sub in range(4):
+        ref_out = rr.step(30.0, st, ak)
+        cu, ak_o = oracle.route(net, 30.0, st | {"ak_max": ak})
+        for k in ("q_in_kkp1", "q_out_kkp1_1", "q_out_kkp1_2", "volume_kkp1", "h_water_kkp1"):
+            assert np.array_equal(st[k], ref_out[k]), (sub, k)
+        assert cu == ref_out["cu_max"] and ak_o == ref_out["ak_max"], sub
+        assert cu > 0.0
+        ak = ak_o
+        advance(st)
+    assert st["q_in_kk"].max() > 0.0
+
+
+def test_route_mass_bookkeeping(oracle, net):
+    """Every routed outflow arrives at its downstream cell: the inflows the
+    substep adds sum to all outflows but the outlet's cardinal one, and each
+    cell's new volume is old volume + dt * (mean in - mean out + surface
+    water) where that is non-negative."""
+    st = initial_state(net, seed=9)
+    oracle.route(net, 20.0, st)
+    out1 = st["q_out_kkp1_1"].copy()
+    outlet = net.qoi[-1] - 1
+    added = st["q_in_kkp1"].sum()
+    sent = out1.sum() - out1[outlet] + st["q_out_kkp1_2"].sum()
+    assert abs(added - sent) <= 1e-12 * sent
+    dv = 20.0 * ((st["q_in_kk"] + st["q_in_kkp1"]) / 2 + st["surface_water"]
+                 - (st["q_out_kk1"] + st["q_out_kk2"]) / 2 - (st["q_out_kkp1_1"] + st["q_out_kkp1_2"]) / 2)
+    v = st["volume_kk"] + dv
+    pos = v >= 0
+    assert np.allclose(st["volume_kkp1"][pos], v[pos], rtol=1e-12, atol=0)
+    assert np.all(st["volume_kkp1"][~pos] == 0.0) and np.all(st["h_water_kkp1"] >= 0.0)
+    assert np.allclose(st["h_water_kkp1"], st["volume_kkp1"] / (net.dx * net.dy), rtol=1e-15, atol=0)
+
+
+def test_mc_special_cases(oracle):
+    """MC (mc.f): with no water anywhere the outflow is zero; the four
+    coefficients sum to 1 + c4 * (overland / dt-scaled) -- a steady inflow q
+    with no overland input passes unchanged (c1 + c2 + c3 = 1)."""
+    cu, ak, q = C.c_double(), C.c_double(), C.c_double()
+    args = (0.05, 10.0, 20.0, 2.0, 0.3, 0.1, 30.0)
+    oracle.L.orc_mc(*[C.c_double(v) for v in args], C.c_double(0.0), C.c_double(0.0), C.c_double(0.0),
+                    C.c_double(0.0), C.byref(cu), C.byref(ak), C.byref(q))
+    assert q.value == 0.0 and cu.value > 0.0 and ak.value == pytest.approx(cu.value / 30.0, rel=1e-14)
+    for qs in (1e-3, 0.2, 3.0):
+        oracle.L.orc_mc(*[C.c_double(v) for v in args], C.c_double(qs), C.c_double(qs), C.c_double(qs),
+                        C.c_double(0.0), C.byref(cu), C.byref(ak), C.byref(q))
+        assert q.value == pytest.approx(qs, rel=1e-13)
+
+
+def test_network_is_routable(net):
+    """The synthetic network: every routed branch's downstream cell is on
+    the grid and later in QOI_SN; the outlet is last."""
+    pos = np.empty(net.ncell, dtype=np.int64)
+    pos[net.qoi - 1] = np.arange(net.ncell)
+    for p, ib in enumerate(net.qoi):
+        j = (ib - 1) % net.nrow
+        i = (ib - 1) // net.nrow
+        for w, po in ((net.w1, net.pout1), (net.w2, net.pout2)):
+            if w[j, i] == 0 or (po is net.pout1 and p == net.ncell - 1):
+                continue
+            d = int(po[j, i]) - 5
+            di = int(np.round(d / 3.0))
+            dj = d - 3 * di
+            assert 0 <= i + di < net.ncol and 0 <= j + dj < net.nrow
+            assert pos[(i + di) * net.nrow + (j + dj)] > p
+    assert net.qoi[-1] == 1
