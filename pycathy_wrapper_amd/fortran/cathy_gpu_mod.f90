@@ -100,6 +100,40 @@ module cathy_gpu
       real(c_double), intent(out) :: out6(6)
     end function
 
+    ! surface routing substep (ROUTE + ALTEZZE), route.f:4-234, altezze.f:4-60
+    integer(c_int) function cathy_gpu_route_create(handle, nrow, ncol, ldc, ncell, qoi_sn, p_outflow_1, &
+                                                   p_outflow_2, w_1, w_2, local_slope_1, local_slope_2, &
+                                                   epl_1, epl_2, kss1_sf_1, kss1_sf_2, ws1_sf_1, ws1_sf_2, &
+                                                   b1_sf, y1_sf, nrc) bind(C, name="cathy_gpu_route_create")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(out) :: handle
+      integer(c_int), intent(in) :: nrow, ncol, ldc, ncell, qoi_sn(*), p_outflow_1(*), p_outflow_2(*)
+      real(c_double), intent(in) :: w_1(*), w_2(*), local_slope_1(*), local_slope_2(*), epl_1(*), epl_2(*)
+      real(c_double), intent(in) :: kss1_sf_1(*), kss1_sf_2(*), ws1_sf_1(*), ws1_sf_2(*), b1_sf(*)
+      real(c_double), intent(in) :: y1_sf(*), nrc(*)
+    end function
+
+    integer(c_int) function cathy_gpu_route_step(handle, nmember, deltat, delta_x, delta_y, &
+                                                 surface_water_sn, q_in_kk_sn, q_out_kk_sn_1, q_out_kk_sn_2, &
+                                                 volume_kk_sn, q_in_kkp1_sn, q_out_kkp1_sn_1, &
+                                                 q_out_kkp1_sn_2, volume_kkp1_sn, h_water_kkp1_sn, cu_ak) &
+        bind(C, name="cathy_gpu_route_step")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      integer(c_int), intent(in) :: nmember
+      real(c_double), intent(in) :: deltat, delta_x, delta_y
+      real(c_double), intent(in) :: surface_water_sn(*), q_in_kk_sn(*), q_out_kk_sn_1(*), q_out_kk_sn_2(*)
+      real(c_double), intent(in) :: volume_kk_sn(*)
+      real(c_double), intent(inout) :: q_in_kkp1_sn(*), q_out_kkp1_sn_1(*), q_out_kkp1_sn_2(*)
+      real(c_double), intent(out) :: volume_kkp1_sn(*), h_water_kkp1_sn(*)
+      real(c_double), intent(inout) :: cu_ak(2, *)
+    end function
+
+    integer(c_int) function cathy_gpu_route_destroy(handle) bind(C, name="cathy_gpu_route_destroy")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), intent(inout) :: handle
+    end function
+
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s

@@ -1,0 +1,64 @@
+C  route_gpu.f -- ROUTE on the device, a drop-in for route.f in a CATHY
+C  build (compiled with that build's CATHY.H, RIVERNETWORK.H, SURFWATER.H).
+C
+C  Same name and dummy arguments as ROUTE (route.f:4-5).  Like ROUTE it
+C  reads the drainage network from COMMON /RIVERNETWORK/ and the flows from
+C  /SURFWATER/, and it returns Q_IN_KKP1_SN, Q_OUT_KKP1_SN_1/2, CU_MAX and
+C  AK_MAX there; cathy_gpu_route_step also fills VOLUME_KKP1_SN and
+C  H_WATER_KKP1_SN, which the caller's ALTEZZE then recomputes from the same
+C  flows.  The device network is set up at the first call: the network of a
+C  run does not change.  The grid is NROW x (NCELL/NROW) cells.  Basins
+C  with reservoirs (any routed TIPO_R /= 0) stop, as does any error of the
+C  library (its text is printed first).
+      SUBROUTINE ROUTE(NROW,NCELL,TIPO_R,RESERVR,
+     1     CELLS_R,N_HA,TIME,DELTAT)
+      USE CATHY_GPU
+      IMPLICIT NONE
+      INCLUDE 'CATHY.H'
+      INCLUDE 'RIVERNETWORK.H'
+      INCLUDE 'SURFWATER.H'
+      INTEGER NCELL,NROW,I,NCOL,IRC
+      INTEGER TIPO_R(MAXCEL)
+      INTEGER CELLS_R(MAXRES),N_HA(MAXRES)
+      INTEGER RESERVR(MAXCEL)
+      REAL*8 TIME,DELTAT
+      REAL*8 CUAK(2)
+      INTEGER(C_INT64_T) HROUTE
+      SAVE HROUTE
+      DATA HROUTE /0/
+      IF (HROUTE .EQ. 0) THEN
+         IF (NROW .LT. 1 .OR. MOD(NCELL,NROW) .NE. 0) THEN
+            WRITE(6,*) 'ROUTE (GPU): NCELL is not a multiple of NROW'
+            STOP
+         END IF
+         NCOL=NCELL/NROW
+         DO I=1,NCELL
+            IF (TIPO_R(QOI_SN(I)) .NE. 0) THEN
+               WRITE(6,*) 'ROUTE (GPU): reservoirs are not supported'
+               STOP
+            END IF
+         END DO
+         IRC=CATHY_GPU_ROUTE_CREATE(HROUTE,NROW,NCOL,COLMAX,NCELL,
+     1        QOI_SN,DTM_P_OUTFLOW_1,DTM_P_OUTFLOW_2,DTM_W_1,DTM_W_2,
+     2        DTM_LOCAL_SLOPE_1,DTM_LOCAL_SLOPE_2,DTM_EPL_1,DTM_EPL_2,
+     3        DTM_KSS1_SF_1,DTM_KSS1_SF_2,DTM_WS1_SF_1,DTM_WS1_SF_2,
+     4        DTM_B1_SF,DTM_Y1_SF,DTM_NRC)
+         IF (IRC .LT. 0) THEN
+            WRITE(6,*) 'ROUTE (GPU): ', TRIM(CATHY_GPU_ERROR_MESSAGE())
+            STOP
+         END IF
+      END IF
+      CUAK(1)=0.0D0
+      CUAK(2)=AK_MAX
+      IRC=CATHY_GPU_ROUTE_STEP(HROUTE,1,DELTAT,DELTA_X,DELTA_Y,
+     1     SURFACE_WATER_SN,Q_IN_KK_SN,Q_OUT_KK_SN_1,Q_OUT_KK_SN_2,
+     2     VOLUME_KK_SN,Q_IN_KKP1_SN,Q_OUT_KKP1_SN_1,Q_OUT_KKP1_SN_2,
+     3     VOLUME_KKP1_SN,H_WATER_KKP1_SN,CUAK)
+      IF (IRC .LT. 0) THEN
+         WRITE(6,*) 'ROUTE (GPU): ', TRIM(CATHY_GPU_ERROR_MESSAGE())
+         STOP
+      END IF
+      CU_MAX=CUAK(1)
+      AK_MAX=CUAK(2)
+      RETURN
+      END
