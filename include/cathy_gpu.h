@@ -18,6 +18,9 @@
  *                           cathy_main.f:2882-3860 (NEUMANN free drainage,
  *                           ATMNXT/ATMBAK, SWITCH_OLD, BKSTEP, CHVELO,
  *                           STORCAL, TIMUPD) available as one call.
+ *   cathy_gpu_route_*       ROUTE    route.f:4-234 (MC mc.f, SERBATOIO
+ *                           serbatoio.f, CALCOLI calcoli.f) followed by
+ *                           ALTEZZE altezze.f:4-107, per routing substep
  */
 #ifndef CATHY_GPU_H
 #define CATHY_GPU_H
@@ -260,7 +263,8 @@ int cathy_gpu_flow3d_counters(int64_t *handle, double *out6);
  *     (route.f:4-5; MC mc.f:4-5)
  *   CALL ALTEZZE(NCELL,TIPO_R,DELTA_X,DELTA_Y,Q_IN_KK_SN,...,ELEVATION)
  *     (altezze.f:4-9)
- * for basins without reservoirs (every TIPO_R = 0).  The reference reads
+ * with SERBATOIO / CALCOLI for reservoirs (serbatoio.f:6-7, calcoli.f:4-6)
+ * once cathy_gpu_route_set_reservoirs is called.  The reference reads
  * the network from COMMON /RIVERNETWORK/ and the flows from /SURFWATER/;
  * here they are arguments.
  * ------------------------------------------------------------------ */
@@ -279,6 +283,18 @@ int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol,
                            const double *kss1_sf_1, const double *kss1_sf_2,
                            const double *ws1_sf_1, const double *ws1_sf_2,
                            const double *b1_sf, const double *y1_sf, const double *nrc);
+/* Reservoirs (optional; without this call every TIPO_R is taken as 0):
+ * TIPO_R, RESERVR and ELEVATION (NROW*NCOL, by I_BASIN), NUMRES <= MAXRES,
+ * N_HA(NUMRES) in 2..30, HRES / ARES (MAXRES, 30), H_SOGLIA, H_FONDO
+ * (NUMRES) as in SURFWATER.H and cathy_main.f:2312-2313.  Refused (-1) where
+ * the reference stops: a routing cell with TIPO_R >= 1000 (route.f:138-141).
+ * A reservoir's new level is the last of its outlet branches in QOI_SN order,
+ * as the reference's sequential writes leave it. */
+int cathy_gpu_route_set_reservoirs(int64_t *handle, const int *tipo_r, const int *reservr,
+                                   const int *numres, const int *maxres, const int *n_ha,
+                                   const double *hres, const double *ares,
+                                   const double *h_soglia, const double *h_fondo,
+                                   const double *elevation);
 /* info(4): levels of the network, most feeders of one cell, NCELL,
  * NROW*NCOL. */
 int cathy_gpu_route_info(int64_t *handle, int *info);
@@ -286,16 +302,20 @@ int cathy_gpu_route_info(int64_t *handle, int *info);
  * (NROW*NCOL, NMEMBER) indexed by I_BASIN (SURFWATER.H names).
  * In: SURFACE_WATER_SN, Q_IN_KK_SN, Q_OUT_KK_SN_1/2, VOLUME_KK_SN.
  * In/out: Q_IN_KKP1_SN (receives the upstream outflows), Q_OUT_KKP1_SN_1/2
- * (a branch with W = 0 keeps its value).  Out: VOLUME_KKP1_SN,
- * H_WATER_KKP1_SN.  cu_ak(2, NMEMBER): CU_MAX (out), AK_MAX (in/out: kept
- * when no branch is routed).  Synchronous. */
+ * (a branch with W = 0 keeps its value), VOLUME_KKP1_SN (reservoir and lake
+ * cells keep theirs).  Out: H_WATER_KKP1_SN.  cu_ak(2, NMEMBER): CU_MAX
+ * (out), AK_MAX (in/out: kept when no branch is routed).  With reservoirs:
+ * H_POOL_KK_VEC (in) and H_POOL_KKP1_VEC (in/out), (NUMRES, NMEMBER); else
+ * both may be NULL.  Returns -1 where CALCOLI stops (a pool level at or
+ * below its table).  Synchronous. */
 int cathy_gpu_route_step(int64_t *handle, const int *nmember, const double *deltat,
                          const double *delta_x, const double *delta_y,
                          const double *surface_water_sn, const double *q_in_kk_sn,
                          const double *q_out_kk_sn_1, const double *q_out_kk_sn_2,
                          const double *volume_kk_sn, double *q_in_kkp1_sn,
                          double *q_out_kkp1_sn_1, double *q_out_kkp1_sn_2,
-                         double *volume_kkp1_sn, double *h_water_kkp1_sn, double *cu_ak);
+                         double *volume_kkp1_sn, double *h_water_kkp1_sn, double *cu_ak,
+                         const double *h_pool_kk_vec, double *h_pool_kkp1_vec);
 int cathy_gpu_route_destroy(int64_t *handle);
 
 /* ------------------------------------------------------------------ *

@@ -2171,13 +2171,85 @@ void orc_mc(double local_slope, double epl, double ks, double w, double b1, doub
   *q_out_kkp1 = c1 * q_in_kkp1 + c2 * q_in_kk + c3 * q_out_kk + c4 * q_overland;
 }
 
-/* ROUTE (SRC/route.f:4-234) for TIPO_R = 0: cells in QOI_SN order, each
- * routing the cardinal (W_1) and diagonal (W_2) shares of its inflow and
- * of the surface water over its cell, then adding each outflow to the
+/* CALCOLI (SRC/calcoli.f:4-76): the level's rate of change dd at time t of
+ * the substep.  Returns -1 where the reference stops (level at or below
+ * the table's first height). */
+static int calcoli(double hh, double *dd, double delta_t, int n_ha, const double *ah_r, const double *ha_r,
+                   double q_in_kk, double t, double h_sfioro, double q_in_kkp1, double surface_water,
+                   double h_fondo_rr) {
+  double q_inflow_p = q_in_kk + (q_in_kkp1 - q_in_kk) / delta_t * t;
+  q_inflow_p = q_inflow_p + surface_water;
+  if (hh == h_fondo_rr && q_inflow_p < 0.0) {
+    *dd = 0.0;
+    return 0;
+  }
+  double ah_r_p = 0.0;
+  int found = 0;
+  for (int j = 0; j < n_ha - 1; ++j) {
+    if (ha_r[j] < hh && ha_r[j + 1] >= hh) {
+      ah_r_p = ah_r[j] + (ah_r[j + 1] - ah_r[j]) / (ha_r[j + 1] - ha_r[j]) * (hh - ha_r[j]);
+      found = 1;
+      break;
+    }
+  }
+  if (!found) {
+    if (hh > ha_r[n_ha - 1]) ah_r_p = ah_r[n_ha - 1];
+    else return -1;
+  }
+  double qh_r_p;
+  if (hh <= h_sfioro) qh_r_p = 0.0;
+  else qh_r_p = q_inflow_p >= 0.0 ? q_inflow_p : 0.0;
+  *dd = 1 / ah_r_p * (q_inflow_p - qh_r_p);
+  return 0;
+}
+
+int orc_serbatoio(const orc_network *net, int r, double q_in_kk, double q_in_kkp1, double h_pool_kk,
+                  double delta_t, double surface_water, double *q_out_kkp1, double *h_pool_kkp1) {
+  const int n_ha = net->n_ha[r];
+  double ha_r[30], ah_r[30];
+  for (int i = 0; i < n_ha && i < 30; ++i) {
+    ha_r[i] = net->hres[r + (size_t)i * net->maxres];
+    ah_r[i] = net->ares[r + (size_t)i * net->maxres];
+  }
+  const double h_sfioro = net->h_soglia[r], h_fondo = net->h_fondo[r];
+  const double t_iniz = 0.0, t_fin = delta_t;
+  double dd, hh = h_pool_kk;
+  if (calcoli(hh, &dd, delta_t, n_ha, ah_r, ha_r, q_in_kk, t_iniz, h_sfioro, q_in_kkp1, surface_water, h_fondo))
+    return -1;
+  const double dh1 = delta_t * dd;
+  hh = h_pool_kk + 0.50 * dh1;
+  if (hh < h_fondo) hh = h_fondo;
+  if (calcoli(hh, &dd, delta_t, n_ha, ah_r, ha_r, q_in_kk, t_iniz + 0.50 * delta_t, h_sfioro, q_in_kkp1,
+              surface_water, h_fondo))
+    return -1;
+  const double dh2 = delta_t * dd;
+  hh = h_pool_kk + 0.50 * dh2;
+  if (hh < h_fondo) hh = h_fondo;
+  if (calcoli(hh, &dd, delta_t, n_ha, ah_r, ha_r, q_in_kk, t_iniz + 0.50 * delta_t, h_sfioro, q_in_kkp1,
+              surface_water, h_fondo))
+    return -1;
+  const double dh3 = delta_t * dd;
+  hh = h_pool_kk + 0.50 * dh3; /* the reference's fourth stage: half of dh3 */
+  if (hh < h_fondo) hh = h_fondo;
+  if (calcoli(hh, &dd, delta_t, n_ha, ah_r, ha_r, q_in_kk, t_fin, h_sfioro, q_in_kkp1, surface_water, h_fondo))
+    return -1;
+  const double dh4 = delta_t * dd;
+  double h = h_pool_kk + dh1 / 6 + dh2 / 3 + dh3 / 3 + dh4 / 6;
+  if (h < h_fondo) h = h_fondo;
+  *h_pool_kkp1 = h;
+  if (h <= h_sfioro) *q_out_kkp1 = 0.0;
+  else *q_out_kkp1 = q_in_kkp1 + surface_water >= 0.0 ? q_in_kkp1 + surface_water : 0.0;
+  return 0;
+}
+
+/* ROUTE (SRC/route.f:4-234): cells in QOI_SN order, each routing the
+ * cardinal (W_1) and diagonal (W_2) shares of its inflow and of the surface
+ * water over its cell -- by Muskingum-Cunge (TIPO_R = 0) or through its
+ * reservoir (0 < TIPO_R < 1000) -- then adding each outflow to the
  * downstream cell's Q_IN_KKP1 (the last cell's cardinal outflow leaves). */
-void orc_route(const orc_network *net, double deltat, const double *surface_water, const double *q_in_kk,
-               const double *q_out_kk1, const double *q_out_kk2, double *q_in_kkp1, double *q_out_kkp1_1,
-               double *q_out_kkp1_2, double *cu_ak) {
+int orc_route(const orc_network *net, double deltat, const double *surface_water, const double *q_in_kk,
+              const double *q_out_kk1, const double *q_out_kk2, double *q_in_kkp1, double *q_out_kkp1_1,
+              double *q_out_kkp1_2, double *cu_ak, const double *h_pool_kk, double *h_pool_kkp1) {
   const int nrow = net->nrow;
   double cu_max = 0.0;
   for (int q = 0; q < net->ncell; ++q) {
@@ -2188,6 +2260,7 @@ void orc_route(const orc_network *net, double deltat, const double *surface_wate
     const size_t g = (size_t)(i - 1) + (size_t)(j - 1) * net->ldc;
     const double nrc = net->nrc[g];
     const double sw = surface_water[ib - 1] / nrc;
+    const int tipo = net->tipo_r ? net->tipo_r[ib - 1] : 0;
     for (int br = 0; br < 2; ++br) {
       const double w = br ? net->w2[g] : net->w1[g];
       if (w == 0) continue;
@@ -2202,40 +2275,63 @@ void orc_route(const orc_network *net, double deltat, const double *surface_wate
       const double qin_kk = q_in_kk[ib - 1] * w / nrc;
       const double qout_kk = (br ? q_out_kk2[ib - 1] : q_out_kk1[ib - 1]) / nrc;
       const double qin_kkp1 = q_in_kkp1[ib - 1] * w / nrc;
-      double cu, ak, qout;
-      orc_mc(br ? net->slope2[g] : net->slope1[g], epl, br ? net->kss2[g] : net->kss1[g],
-             br ? net->ws2[g] : net->ws1[g], net->b1[g], net->y1[g], deltat, qin_kk, qin_kkp1, qout_kk,
-             q_overland, &cu, &ak, &qout);
-      if (qout < 0.0) qout = 0.0;
       double *out = br ? q_out_kkp1_2 : q_out_kkp1_1;
-      out[ib - 1] = qout * nrc;
-      if (cu >= cu_max) {
-        cu_max = cu;
-        cu_ak[1] = ak;
+      if (tipo >= 1000) return -1; /* 'INCOMPRENSIBILE CASE!' and STOP */
+      if (tipo != 0) {
+        const int r = net->reservr[ib - 1] - 1;
+        double qout, h;
+        if (orc_serbatoio(net, r, qin_kk, qin_kkp1, h_pool_kk[r], deltat, sw, &qout, &h)) return -1;
+        if (qout < 0.0) qout = 0.0;
+        h_pool_kkp1[r] = h;
+        out[ib - 1] = qout;
+      } else {
+        double cu, ak, qout;
+        orc_mc(br ? net->slope2[g] : net->slope1[g], epl, br ? net->kss2[g] : net->kss1[g],
+               br ? net->ws2[g] : net->ws1[g], net->b1[g], net->y1[g], deltat, qin_kk, qin_kkp1, qout_kk,
+               q_overland, &cu, &ak, &qout);
+        if (qout < 0.0) qout = 0.0;
+        out[ib - 1] = qout * nrc;
+        if (cu >= cu_max) {
+          cu_max = cu;
+          cu_ak[1] = ak;
+        }
       }
       if (br == 1 || q != net->ncell - 1) q_in_kkp1[ib_cv - 1] = q_in_kkp1[ib_cv - 1] + out[ib - 1];
     }
   }
   cu_ak[0] = cu_max;
+  return 0;
 }
 
-/* ALTEZZE (SRC/altezze.f:4-60), TIPO_R = 0: cell volume and water depth at
- * the new time level. */
-void orc_altezze(int ncell, double dx, double dy, double deltat, const double *q_in_kk,
+/* ALTEZZE (SRC/altezze.f:4-107): cell volume and water depth at the new
+ * time level; reservoir cells (0 < TIPO_R < 1000) and lake cells
+ * (TIPO_R >= 1000) take the depth of their reservoir's pool. */
+void orc_altezze(const orc_network *net, int ncell, double dx, double dy, double deltat, const double *q_in_kk,
                  const double *q_in_kkp1, const double *q_out_kk1, const double *q_out_kk2,
                  const double *q_out_kkp1_1, const double *q_out_kkp1_2, const double *volume_kk,
-                 const double *surface_water, double *volume_kkp1, double *h_water_kkp1) {
+                 const double *surface_water, double *volume_kkp1, double *h_water_kkp1,
+                 const double *h_pool_kkp1) {
   for (int c = 0; c < ncell; ++c) {
-    const double dv = (q_in_kk[c] + q_in_kkp1[c]) / 2 * deltat + surface_water[c] * deltat -
-                      (q_out_kk1[c] + q_out_kk2[c]) / 2 * deltat -
-                      (q_out_kkp1_1[c] + q_out_kkp1_2[c]) / 2 * deltat;
-    const double v = volume_kk[c] + dv;
-    volume_kkp1[c] = v;
-    if (v >= 0.0) {
-      h_water_kkp1[c] = v / (dx * dy);
+    const int tipo = net->tipo_r ? net->tipo_r[c] : 0;
+    if (tipo == 0) {
+      const double dv = (q_in_kk[c] + q_in_kkp1[c]) / 2 * deltat + surface_water[c] * deltat -
+                        (q_out_kk1[c] + q_out_kk2[c]) / 2 * deltat -
+                        (q_out_kkp1_1[c] + q_out_kkp1_2[c]) / 2 * deltat;
+      const double v = volume_kk[c] + dv;
+      volume_kkp1[c] = v;
+      if (v >= 0.0) {
+        h_water_kkp1[c] = v / (dx * dy);
+      } else {
+        volume_kkp1[c] = 0.0;
+        h_water_kkp1[c] = 0.0;
+      }
+    } else if (tipo < 1000) {
+      h_water_kkp1[c] = h_pool_kkp1[net->reservr[c] - 1] - net->elevation[c];
+      if (h_water_kkp1[c] < 0) h_water_kkp1[c] = 0.0;
     } else {
-      volume_kkp1[c] = 0.0;
-      h_water_kkp1[c] = 0.0;
+      const int aa = net->reservr[tipo - 1000 - 1];
+      const double h = h_pool_kkp1[aa - 1] - net->elevation[c];
+      h_water_kkp1[c] = h > 0.0 ? h : 0.0;
     }
   }
 }

@@ -373,18 +373,37 @@ typedef struct {
   int nrow, ncol, ldc, ncell;
   const int *qoi, *pout1, *pout2;
   const double *w1, *w2, *slope1, *slope2, *epl1, *epl2, *kss1, *kss2, *ws1, *ws2, *b1, *y1, *nrc;
+  /* reservoirs (NULL tipo_r: none).  tipo_r / reservr / elevation per cell
+   * (I_BASIN - 1): TIPO_R, RESERVR (1-based reservoir), ELEVATION.  Per
+   * reservoir r (0-based): n_ha[r] table entries hres/ares[r + i*maxres]
+   * (HRES(MAXRES,30), ARES), h_soglia[r], h_fondo[r]. */
+  const int *tipo_r, *reservr, *n_ha;
+  int maxres;
+  const double *hres, *ares, *h_soglia, *h_fondo, *elevation;
 } orc_network;
+
+/* SERBATOIO (SRC/serbatoio.f:6-119) with CALCOLI (SRC/calcoli.f): RK4
+ * level-pool step of reservoir r.  Returns 0, or -1 where CALCOLI stops
+ * (level below its height table). */
+int orc_serbatoio(const orc_network *net, int r, double q_in_kk, double q_in_kkp1, double h_pool_kk,
+                  double delta_t, double surface_water, double *q_out_kkp1, double *h_pool_kkp1);
 
 void orc_mc(double local_slope, double epl, double ks, double w, double b1, double y1, double deltat,
             double q_in_kk, double q_in_kkp1, double q_out_kk, double q_overland, double *cu, double *ak,
             double *q_out_kkp1);
-void orc_route(const orc_network *net, double deltat, const double *surface_water, const double *q_in_kk,
-               const double *q_out_kk1, const double *q_out_kk2, double *q_in_kkp1, double *q_out_kkp1_1,
-               double *q_out_kkp1_2, double *cu_ak);
-void orc_altezze(int ncell, double dx, double dy, double deltat, const double *q_in_kk,
+/* Returns 0, or -1 where the reference stops (a reservoir level outside its
+ * table, or a routed cell with TIPO_R >= 1000).  h_pool_kk / h_pool_kkp1:
+ * H_POOL_KK_VEC / H_POOL_KKP1_VEC (only with reservoirs). */
+int orc_route(const orc_network *net, double deltat, const double *surface_water, const double *q_in_kk,
+              const double *q_out_kk1, const double *q_out_kk2, double *q_in_kkp1, double *q_out_kkp1_1,
+              double *q_out_kkp1_2, double *cu_ak, const double *h_pool_kk, double *h_pool_kkp1);
+/* ALTEZZE over ncell cells; with reservoirs (net->tipo_r) the reservoir and
+ * lake cells take their depth from H_POOL_KKP1_VEC and keep volume_kkp1. */
+void orc_altezze(const orc_network *net, int ncell, double dx, double dy, double deltat, const double *q_in_kk,
                  const double *q_in_kkp1, const double *q_out_kk1, const double *q_out_kk2,
                  const double *q_out_kkp1_1, const double *q_out_kkp1_2, const double *volume_kk,
-                 const double *surface_water, double *volume_kkp1, double *h_water_kkp1);
+                 const double *surface_water, double *volume_kkp1, double *h_water_kkp1,
+                 const double *h_pool_kkp1);
 
 #ifdef __cplusplus
 }

@@ -65,7 +65,10 @@ NETWORK_REAL = ("w1", "w2", "slope1", "slope2", "epl1", "epl2", "kss1", "kss2", 
 class OrcNetwork(C.Structure):
     _fields_ = [("nrow", C.c_int), ("ncol", C.c_int), ("ldc", C.c_int), ("ncell", C.c_int),
                 ("qoi", C.POINTER(C.c_int)), ("pout1", C.POINTER(C.c_int)), ("pout2", C.POINTER(C.c_int))] + \
-               [(k, C.POINTER(C.c_double)) for k in NETWORK_REAL]
+               [(k, C.POINTER(C.c_double)) for k in NETWORK_REAL] + \
+               [("tipo_r", C.POINTER(C.c_int)), ("reservr", C.POINTER(C.c_int)), ("n_ha", C.POINTER(C.c_int)),
+                ("maxres", C.c_int)] + \
+               [(k, C.POINTER(C.c_double)) for k in ("hres", "ares", "h_soglia", "h_fondo", "elevation")]
 
 
 class Oracle:
@@ -153,20 +156,39 @@ class Oracle:
     # --- whole run ----------------------------------------------------------
     def route(self, net, deltat: float, st: dict) -> tuple[float, float]:
         """orc_route + orc_altezze on one member's state dict (the keys of
-        pycathy_wrapper_amd.routing.RouteGPU.step); arrays updated in place.
-        Returns (CU_MAX, AK_MAX)."""
+        pycathy_wrapper_amd.routing.RouteGPU.step, plus h_pool_kk /
+        h_pool_kkp1 with reservoirs); arrays updated in place.  Returns
+        (CU_MAX, AK_MAX); raises where the reference would stop."""
         keep = [np.ascontiguousarray(a) for a in
                 (net.qoi.astype(np.int32), net.pout1.astype(np.int32), net.pout2.astype(np.int32))]
         reals = [np.ascontiguousarray(getattr(net, k), dtype=np.float64) for k in NETWORK_REAL]
         on = OrcNetwork(net.nrow, net.ncol, net.ncol, net.ncell, *[iptr(a) for a in keep], *[dptr(a) for a in reals])
+        res = net.reservoirs
+        hp0 = hp1 = None
+        if res is not None:
+            rk = {k: np.ascontiguousarray(res[k], dtype=np.int32) for k in ("tipo_r", "reservr", "n_ha")}
+            rd = {k: np.ascontiguousarray(res[k], dtype=np.float64)
+                  for k in ("hres", "ares", "h_soglia", "h_fondo", "elevation")}
+            on.tipo_r, on.reservr, on.n_ha = iptr(rk["tipo_r"]), iptr(rk["reservr"]), iptr(rk["n_ha"])
+            on.maxres = int(res["maxres"])
+            on.hres, on.ares, on.h_soglia, on.h_fondo, on.elevation = (
+                dptr(rd[k]) for k in ("hres", "ares", "h_soglia", "h_fondo", "elevation"))
+            hp0, hp1 = st["h_pool_kk"], st["h_pool_kkp1"]
         cu_ak = np.array([0.0, float(st.get("ak_max", 0.0))])
-        self.L.orc_route(C.byref(on), C.c_double(deltat), *[dptr(st[k]) for k in
-                         ("surface_water", "q_in_kk", "q_out_kk1", "q_out_kk2", "q_in_kkp1", "q_out_kkp1_1",
-                          "q_out_kkp1_2")], dptr(cu_ak))
-        self.L.orc_altezze(C.c_int(net.nrow * net.ncol), C.c_double(net.dx), C.c_double(net.dy), C.c_double(deltat),
+        self.L.orc_route.restype = C.c_int
+        rc = self.L.orc_route(C.byref(on), C.c_double(deltat), *[dptr(st[k]) for k in
+                              ("surface_water", "q_in_kk", "q_out_kk1", "q_out_kk2", "q_in_kkp1", "q_out_kkp1_1",
+                               "q_out_kkp1_2")], dptr(cu_ak), dptr(hp0) if hp0 is not None else None,
+                              dptr(hp1) if hp1 is not None else None)
+        if rc != 0:
+            raise RuntimeError("orc_route: the reference stops here (reservoir level outside its table, "
+                               "or a routed lake cell)")
+        self.L.orc_altezze(C.byref(on), C.c_int(net.nrow * net.ncol), C.c_double(net.dx), C.c_double(net.dy),
+                           C.c_double(deltat),
                            *[dptr(st[k]) for k in ("q_in_kk", "q_in_kkp1", "q_out_kk1", "q_out_kk2",
                                                    "q_out_kkp1_1", "q_out_kkp1_2", "volume_kk", "surface_water",
-                                                   "volume_kkp1", "h_water_kkp1")])
+                                                   "volume_kkp1", "h_water_kkp1")],
+                           dptr(hp1) if hp1 is not None else None)
         return float(cu_ak[0]), float(cu_ak[1])
 
     def run_create(self, mesh, soil, psi0, atm_times, atm_vals, hspatm, params: OrcParams):

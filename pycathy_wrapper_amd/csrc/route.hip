@@ -41,6 +41,18 @@ constexpr int kRouteBlock = 512;
 // 4 diagonal neighbours' W_2 branches.
 constexpr int kMaxFeed = 8;
 
+// Reservoirs (SERBATOIO / CALCOLI); res == nullptr: none.
+struct ResDev {
+  int maxres;               // leading dimension of the height-area tables
+  const int *n_ha;          // [numres] table entries
+  const double *hres, *ares;  // HRES / ARES (MAXRES, 30) column-major
+  const double *h_soglia, *h_fondo;
+  const int *rid;           // [ncell] reservoir of order index k, -1 if none
+  const int *rflag;         // [ncell] bit br: branch br goes through the reservoir; bit 2+br: it sets the pool level
+  const int *alt;           // [ngrid] ALTEZZE: -1 channel cell, 2r outlet cell of reservoir r, 2r+1 lake cell
+  const double *elevation;  // [ngrid]
+};
+
 struct RouteDev {
   int ncell;                // routed cells
   int ngrid;                // cells of the grid (nrow * ncol): state arrays
@@ -51,6 +63,8 @@ struct RouteDev {
   const int *pred_ptr;      // [ncell + 1]
   const int *pred;          // feeders: grid index * 2 + branch (0: W_1, 1: W_2)
   const double *prm;        // [kNParam][ncell]
+  int numres;
+  ResDev res;
 };
 
 // The flow-independent factors of MC for one branch (mc.f:40-49), in the
@@ -92,6 +106,71 @@ __device__ __forceinline__ double mc_outflow(double ck0, double eq, double eb, d
   return c1 * q_in_kkp1 + c2 * q_in_kk + c3 * q_out_kk + c4 * q_overland;
 }
 
+// CALCOLI (calcoli.f:4-76): rate of change of the pool level; false where
+// the reference stops (level at or below the table's first height).
+__device__ bool calcoli(double hh, double &dd, double delta_t, int n_ha, const double *ah_r, const double *ha_r,
+                        int ld, double q_in_kk, double t, double h_sfioro, double q_in_kkp1, double surface_water,
+                        double h_fondo_rr) {
+  double q_inflow_p = q_in_kk + (q_in_kkp1 - q_in_kk) / delta_t * t;
+  q_inflow_p = q_inflow_p + surface_water;
+  if (hh == h_fondo_rr && q_inflow_p < 0.0) {
+    dd = 0.0;
+    return true;
+  }
+  double ah_r_p = 0.0;
+  bool found = false;
+  for (int j = 0; j < n_ha - 1 && !found; ++j) {
+    const double h0 = ha_r[j * ld], h1 = ha_r[(j + 1) * ld];
+    if (h0 < hh && h1 >= hh) {
+      ah_r_p = ah_r[j * ld] + (ah_r[(j + 1) * ld] - ah_r[j * ld]) / (h1 - h0) * (hh - h0);
+      found = true;
+    }
+  }
+  if (!found) {
+    if (hh > ha_r[(n_ha - 1) * ld]) ah_r_p = ah_r[(n_ha - 1) * ld];
+    else return false;
+  }
+  const double qh_r_p = hh <= h_sfioro ? 0.0 : (q_inflow_p >= 0.0 ? q_inflow_p : 0.0);
+  dd = 1 / ah_r_p * (q_inflow_p - qh_r_p);
+  return true;
+}
+
+// SERBATOIO (serbatoio.f:6-119): RK4 level-pool step of reservoir r, whose
+// fourth stage takes half of the third increment as the reference does.
+__device__ bool serbatoio(const ResDev &R, int r, double q_in_kk, double q_in_kkp1, double h_pool_kk,
+                          double delta_t, double surface_water, double &q_out_kkp1, double &h_pool_kkp1) {
+  const int n_ha = R.n_ha[r], ld = R.maxres;
+  const double *ha_r = R.hres + r, *ah_r = R.ares + r;
+  const double h_sfioro = R.h_soglia[r], h_fondo = R.h_fondo[r];
+  double dd, hh = h_pool_kk;
+  if (!calcoli(hh, dd, delta_t, n_ha, ah_r, ha_r, ld, q_in_kk, 0.0, h_sfioro, q_in_kkp1, surface_water, h_fondo))
+    return false;
+  const double dh1 = delta_t * dd;
+  hh = h_pool_kk + 0.50 * dh1;
+  if (hh < h_fondo) hh = h_fondo;
+  if (!calcoli(hh, dd, delta_t, n_ha, ah_r, ha_r, ld, q_in_kk, 0.0 + 0.50 * delta_t, h_sfioro, q_in_kkp1,
+               surface_water, h_fondo))
+    return false;
+  const double dh2 = delta_t * dd;
+  hh = h_pool_kk + 0.50 * dh2;
+  if (hh < h_fondo) hh = h_fondo;
+  if (!calcoli(hh, dd, delta_t, n_ha, ah_r, ha_r, ld, q_in_kk, 0.0 + 0.50 * delta_t, h_sfioro, q_in_kkp1,
+               surface_water, h_fondo))
+    return false;
+  const double dh3 = delta_t * dd;
+  hh = h_pool_kk + 0.50 * dh3;
+  if (hh < h_fondo) hh = h_fondo;
+  if (!calcoli(hh, dd, delta_t, n_ha, ah_r, ha_r, ld, q_in_kk, delta_t, h_sfioro, q_in_kkp1, surface_water,
+               h_fondo))
+    return false;
+  const double dh4 = delta_t * dd;
+  double h = h_pool_kk + dh1 / 6 + dh2 / 3 + dh3 / 3 + dh4 / 6;
+  if (h < h_fondo) h = h_fondo;
+  h_pool_kkp1 = h;
+  q_out_kkp1 = h <= h_sfioro ? 0.0 : (q_in_kkp1 + surface_water >= 0.0 ? q_in_kkp1 + surface_water : 0.0);
+  return true;
+}
+
 struct CuBest {
   double cu, ak;
   int key;  // -1: no branch routed (AK_MAX unchanged)
@@ -106,12 +185,18 @@ __global__ __launch_bounds__(kRouteBlock) void k_route(
     const double *__restrict__ qin_kk_all, const double *__restrict__ qout_kk1_all,
     const double *__restrict__ qout_kk2_all, const double *__restrict__ vol_kk_all, double *qin_kkp1_all,
     double *qout1_all, double *qout2_all, double *__restrict__ vol_kkp1_all, double *__restrict__ h_all,
-    double *__restrict__ cu_ak) {
+    double *__restrict__ cu_ak, const double *__restrict__ hpool_kk_all, double *__restrict__ hpool_kkp1_all,
+    int *__restrict__ err) {
   const size_t off = (size_t)blockIdx.x * d.ngrid;
   const double *sw = sw_all + off, *qin_kk = qin_kk_all + off;
   const double *qout_kk[2] = {qout_kk1_all + off, qout_kk2_all + off};
   double *qin = qin_kkp1_all + off;
   double *qout[2] = {qout1_all + off, qout2_all + off};
+  const double *hpool_kk = d.numres ? hpool_kk_all + (size_t)blockIdx.x * d.numres : nullptr;
+  double *hpool = d.numres ? hpool_kkp1_all + (size_t)blockIdx.x * d.numres : nullptr;
+  __shared__ int s_err;
+  if (threadIdx.x == 0) s_err = 0;
+  __syncthreads();
   CuBest best{0.0, 0.0, -1};
   for (int lv = 0; lv < d.nlev; ++lv) {
     const int kend = d.lev_ptr[lv + 1];
@@ -135,9 +220,22 @@ __global__ __launch_bounds__(kRouteBlock) void k_route(
       const int n = d.ncell;
       const double nrc = P[kNrc * n + k];
       const double swc = sw[c] / nrc;
+      const int rflag = d.numres ? d.res.rflag[k] : 0;
       for (int br = 0; br < 2; ++br) {
         const double w = P[(kW1 + br) * n + k];
         if (w == 0) continue;
+        if (rflag & (1 << br)) {  // route.f:101-126: through the reservoir, no NRC scaling of the outflow
+          const int r = d.res.rid[k];
+          double q, h;
+          if (!serbatoio(d.res, r, qin_kk[c] * w / nrc, q_in * w / nrc, hpool_kk[r], deltat, swc, q, h)) {
+            s_err = 1;
+            q = h = 0.0;
+          }
+          if (q < 0.0) q = 0.0;
+          if (rflag & (4 << br)) hpool[r] = h;
+          qout[br][c] = q;
+          continue;
+        }
         const double epl = P[(kEpl1 + br) * n + k];
         const double cr = 1.0 / epl;
         const double q_overland = swc * w * cr;
@@ -156,8 +254,14 @@ __global__ __launch_bounds__(kRouteBlock) void k_route(
     }
     __syncthreads();
   }
-  // ALTEZZE (altezze.f:33-58) over every grid cell
+  // ALTEZZE (altezze.f:33-105) over every grid cell
   for (int c = threadIdx.x; c < d.ngrid; c += blockDim.x) {
+    const int alt = d.numres ? d.res.alt[c] : -1;
+    if (alt >= 0) {  // reservoir outlet (depth clamped below at 0) or lake cell (positive depths only)
+      const double hw = hpool[alt >> 1] - d.res.elevation[c];
+      h_all[off + c] = (alt & 1) ? (hw > 0.0 ? hw : 0.0) : (hw < 0 ? 0.0 : hw);
+      continue;
+    }
     const double dv = (qin_kk[c] + qin[c]) / 2 * deltat + sw[c] * deltat -
                       (qout_kk[0][c] + qout_kk[1][c]) / 2 * deltat - (qout[0][c] + qout[1][c]) / 2 * deltat;
     const double v = vol_kk_all[off + c] + dv;
@@ -185,6 +289,7 @@ __global__ __launch_bounds__(kRouteBlock) void k_route(
       if (s_key[w] >= 0 && cu_beats(s_cu[w], s_key[w], b)) b = CuBest{s_cu[w], s_ak[w], s_key[w]};
     cu_ak[2 * blockIdx.x] = b.key >= 0 ? b.cu : 0.0;
     if (b.key >= 0) cu_ak[2 * blockIdx.x + 1] = b.ak;
+    err[blockIdx.x] = s_err;
   }
 }
 
@@ -193,16 +298,26 @@ struct Route {
   hipStream_t st = nullptr;
   DBuf<int> lev_ptr, cell, key, pred_ptr, pred;
   DBuf<double> prm;
-  // per-step state [member][ngrid]: 5 inputs, 3 in/out, 2 outputs; cu_ak
-  DBuf<double> state, cu_ak;
+  // per-step state [member][ngrid]: 5 inputs, 4 in/out (volume_kkp1 keeps
+  // reservoir cells), 1 output; cu_ak; pool levels [member][numres] x 2
+  DBuf<double> state, cu_ak, hpool;
+  DBuf<int> err;
   int cap_members = 0;
+  // reservoirs
+  int numres = 0, maxres = 0;
+  std::vector<int> pos_of_k, cellh;  // QOI_SN position and grid index of order index k
+  std::vector<double> wgrid[2];      // W_1 / W_2 by grid index
+  DBuf<int> r_nha, r_rid, r_rflag, r_alt;
+  DBuf<double> r_hres, r_ares, r_soglia, r_fondo, r_elev;
 
   ~Route() {
     if (st) (void)hipStreamDestroy(st);
   }
 
   RouteDev dev() const {
-    return RouteDev{ncell, ngrid, nlev, lev_ptr.p, cell.p, key.p, pred_ptr.p, pred.p, prm.p};
+    return RouteDev{ncell, ngrid, nlev, lev_ptr.p, cell.p, key.p, pred_ptr.p, pred.p, prm.p, numres,
+                    ResDev{maxres, r_nha.p, r_hres.p, r_ares.p, r_soglia.p, r_fondo.p, r_rid.p, r_rflag.p,
+                           r_alt.p, r_elev.p}};
   }
 };
 
@@ -327,6 +442,15 @@ int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol, co
     r->ngrid = ngrid;
     r->nlev = nlev;
     r->maxpred = maxpred;
+    r->pos_of_k.resize(n);
+    for (int k = 0; k < n; ++k) r->pos_of_k[k] = key[k] / 2;
+    r->cellh = cell;
+    for (int br = 0; br < 2; ++br) {
+      r->wgrid[br].assign(ngrid, 0.0);
+      for (int jj = 1; jj <= nr; ++jj)
+        for (int ii = 1; ii <= nc; ++ii)
+          r->wgrid[br][(size_t)(ii - 1) * nr + jj - 1] = grids[gW1 + br][(size_t)(ii - 1) + (size_t)(jj - 1) * ld];
+    }
     CG_HIP(hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking));
     r->lev_ptr.upload(lev_ptr, r->st);
     r->cell.upload(cell, r->st);
@@ -339,6 +463,75 @@ int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol, co
     std::lock_guard<std::mutex> lk(g_route_mu);
     *handle = g_route_next++;
     g_routes[*handle] = std::move(r);
+    return 0;
+  });
+}
+
+int cathy_gpu_route_set_reservoirs(int64_t *handle, const int *tipo_r, const int *reservr, const int *numres,
+                                   const int *maxres, const int *n_ha, const double *hres, const double *ares,
+                                   const double *h_soglia, const double *h_fondo, const double *elevation) {
+  return guarded([&] {
+    CG_REQUIRE(tipo_r && reservr && numres && maxres && n_ha && hres && ares && h_soglia && h_fondo && elevation,
+               "null argument");
+    Route *r = route_lookup(handle);
+    const int nres = *numres, ld = *maxres, ng = r->ngrid, n = r->ncell;
+    CG_REQUIRE(nres >= 1 && ld >= nres, "NUMRES must lie in 1..MAXRES");
+    for (int q = 0; q < nres; ++q) CG_REQUIRE(n_ha[q] >= 2 && n_ha[q] <= 30, "N_HA must lie in 2..30");
+    auto res_of = [&](int c) {  // reservoir (0-based) of grid cell c
+      const int v = reservr[c] - 1;
+      if (v < 0 || v >= nres) throw Error(-1, "cell " + std::to_string(c + 1) + ": RESERVR outside 1..NUMRES");
+      return v;
+    };
+    std::vector<int> rid(n, -1), rflag(n, 0), alt(ng, -1);
+    std::vector<int> last_key(nres, -1), last_k(nres, -1), last_br(nres, 0);
+    for (int k = 0; k < n; ++k) {
+      const int c = r->cellh[k], t = tipo_r[c];
+      if (t == 0) continue;
+      const bool routes = r->wgrid[0][c] != 0 || r->wgrid[1][c] != 0;
+      if (t >= 1000) {
+        if (routes)
+          throw Error(-1, "cell " + std::to_string(c + 1) + " has TIPO_R >= 1000 and routes water (route.f stops)");
+        continue;
+      }
+      const int q = res_of(c);
+      rid[k] = q;
+      for (int br = 0; br < 2; ++br) {
+        if (r->wgrid[br][c] == 0) continue;
+        rflag[k] |= 1 << br;
+        const int key = 2 * r->pos_of_k[k] + br;  // the last branch in visiting order sets the level
+        if (key > last_key[q]) {
+          last_key[q] = key;
+          last_k[q] = k;
+          last_br[q] = br;
+        }
+      }
+    }
+    for (int q = 0; q < nres; ++q)
+      if (last_k[q] >= 0) rflag[last_k[q]] |= 4 << last_br[q];
+    for (int c = 0; c < ng; ++c) {
+      const int t = tipo_r[c];
+      if (t == 0) continue;
+      if (t < 1000) {
+        alt[c] = 2 * res_of(c);
+      } else {
+        const int c2 = t - 1000;
+        CG_REQUIRE(c2 >= 1 && c2 <= ng, "TIPO_R - 1000 outside the grid");
+        alt[c] = 2 * res_of(c2 - 1) + 1;
+      }
+    }
+    r->numres = nres;
+    r->maxres = ld;
+    r->r_nha.upload(std::vector<int>(n_ha, n_ha + nres), r->st);
+    r->r_hres.upload(std::vector<double>(hres, hres + (size_t)ld * 30), r->st);
+    r->r_ares.upload(std::vector<double>(ares, ares + (size_t)ld * 30), r->st);
+    r->r_soglia.upload(std::vector<double>(h_soglia, h_soglia + nres), r->st);
+    r->r_fondo.upload(std::vector<double>(h_fondo, h_fondo + nres), r->st);
+    r->r_elev.upload(std::vector<double>(elevation, elevation + ng), r->st);
+    r->r_rid.upload(rid, r->st);
+    r->r_rflag.upload(rflag, r->st);
+    r->r_alt.upload(alt, r->st);
+    CG_HIP(hipStreamSynchronize(r->st));
+    r->cap_members = 0;  // pool buffers follow NUMRES
     return 0;
   });
 }
@@ -359,36 +552,54 @@ int cathy_gpu_route_step(int64_t *handle, const int *nmember, const double *delt
                          const double *delta_y, const double *surface_water_sn, const double *q_in_kk_sn,
                          const double *q_out_kk_sn_1, const double *q_out_kk_sn_2, const double *volume_kk_sn,
                          double *q_in_kkp1_sn, double *q_out_kkp1_sn_1, double *q_out_kkp1_sn_2,
-                         double *volume_kkp1_sn, double *h_water_kkp1_sn, double *cu_ak) {
+                         double *volume_kkp1_sn, double *h_water_kkp1_sn, double *cu_ak,
+                         const double *h_pool_kk_vec, double *h_pool_kkp1_vec) {
   return guarded([&] {
     CG_REQUIRE(nmember && deltat && delta_x && delta_y && surface_water_sn && q_in_kk_sn && q_out_kk_sn_1 &&
                    q_out_kk_sn_2 && volume_kk_sn && q_in_kkp1_sn && q_out_kkp1_sn_1 && q_out_kkp1_sn_2 &&
                    volume_kkp1_sn && h_water_kkp1_sn && cu_ak,
                "null argument");
     Route *r = route_lookup(handle);
+    CG_REQUIRE(!r->numres || (h_pool_kk_vec && h_pool_kkp1_vec), "reservoirs need H_POOL_KK_VEC / H_POOL_KKP1_VEC");
     const int m = *nmember;
     CG_REQUIRE(m >= 1 && m <= 65535, "NMEMBER must lie in 1..65535");
     const size_t len = (size_t)m * r->ngrid;
+    const size_t lres = (size_t)m * r->numres;
     if (m > r->cap_members) {
       r->state.alloc(10 * (size_t)m * r->ngrid);
       r->cu_ak.alloc(2 * (size_t)m);
+      r->err.alloc(m);
+      r->hpool.alloc(2 * std::max<size_t>(lres, 1));
       r->cap_members = m;
     }
     double *S = r->state.p;
-    const double *in[8] = {surface_water_sn, q_in_kk_sn,    q_out_kk_sn_1,  q_out_kk_sn_2,
-                           volume_kk_sn,     q_in_kkp1_sn,  q_out_kkp1_sn_1, q_out_kkp1_sn_2};
-    for (int a = 0; a < 8; ++a)
+    const double *in[9] = {surface_water_sn, q_in_kk_sn,      q_out_kk_sn_1,   q_out_kk_sn_2, volume_kk_sn,
+                           q_in_kkp1_sn,     q_out_kkp1_sn_1, q_out_kkp1_sn_2, volume_kkp1_sn};
+    for (int a = 0; a < 9; ++a)
       CG_HIP(hipMemcpyAsync(S + a * len, in[a], len * sizeof(double), hipMemcpyHostToDevice, r->st));
     CG_HIP(hipMemcpyAsync(r->cu_ak.p, cu_ak, 2 * (size_t)m * sizeof(double), hipMemcpyHostToDevice, r->st));
+    if (lres) {
+      CG_HIP(hipMemcpyAsync(r->hpool.p, h_pool_kk_vec, lres * sizeof(double), hipMemcpyHostToDevice, r->st));
+      CG_HIP(hipMemcpyAsync(r->hpool.p + lres, h_pool_kkp1_vec, lres * sizeof(double), hipMemcpyHostToDevice,
+                            r->st));
+    }
     hipLaunchKernelGGL(k_route, dim3(m), dim3(kRouteBlock), 0, r->st, r->dev(), *deltat, *delta_x, *delta_y,
                        S, S + len, S + 2 * len, S + 3 * len, S + 4 * len, S + 5 * len, S + 6 * len, S + 7 * len,
-                       S + 8 * len, S + 9 * len, r->cu_ak.p);
+                       S + 8 * len, S + 9 * len, r->cu_ak.p, r->hpool.p, r->hpool.p + lres, r->err.p);
     CG_HIP(hipGetLastError());
     double *out[5] = {q_in_kkp1_sn, q_out_kkp1_sn_1, q_out_kkp1_sn_2, volume_kkp1_sn, h_water_kkp1_sn};
     for (int a = 0; a < 5; ++a)
       CG_HIP(hipMemcpyAsync(out[a], S + (5 + a) * len, len * sizeof(double), hipMemcpyDeviceToHost, r->st));
     CG_HIP(hipMemcpyAsync(cu_ak, r->cu_ak.p, 2 * (size_t)m * sizeof(double), hipMemcpyDeviceToHost, r->st));
+    if (lres)
+      CG_HIP(hipMemcpyAsync(h_pool_kkp1_vec, r->hpool.p + lres, lres * sizeof(double), hipMemcpyDeviceToHost,
+                            r->st));
+    std::vector<int> herr(m);
+    CG_HIP(hipMemcpyAsync(herr.data(), r->err.p, m * sizeof(int), hipMemcpyDeviceToHost, r->st));
     CG_HIP(hipStreamSynchronize(r->st));
+    for (int j = 0; j < m; ++j)
+      CG_REQUIRE(!herr[j], "member " + std::to_string(j) +
+                               ": a reservoir level left its height table (CALCOLI stops there)");
     return 0;
   });
 }

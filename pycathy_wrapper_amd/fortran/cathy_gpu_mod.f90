@@ -116,7 +116,8 @@ module cathy_gpu
     integer(c_int) function cathy_gpu_route_step(handle, nmember, deltat, delta_x, delta_y, &
                                                  surface_water_sn, q_in_kk_sn, q_out_kk_sn_1, q_out_kk_sn_2, &
                                                  volume_kk_sn, q_in_kkp1_sn, q_out_kkp1_sn_1, &
-                                                 q_out_kkp1_sn_2, volume_kkp1_sn, h_water_kkp1_sn, cu_ak) &
+                                                 q_out_kkp1_sn_2, volume_kkp1_sn, h_water_kkp1_sn, cu_ak, &
+                                                 h_pool_kk_vec, h_pool_kkp1_vec) &
         bind(C, name="cathy_gpu_route_step")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), intent(in) :: handle
@@ -125,8 +126,20 @@ module cathy_gpu
       real(c_double), intent(in) :: surface_water_sn(*), q_in_kk_sn(*), q_out_kk_sn_1(*), q_out_kk_sn_2(*)
       real(c_double), intent(in) :: volume_kk_sn(*)
       real(c_double), intent(inout) :: q_in_kkp1_sn(*), q_out_kkp1_sn_1(*), q_out_kkp1_sn_2(*)
-      real(c_double), intent(out) :: volume_kkp1_sn(*), h_water_kkp1_sn(*)
+      real(c_double), intent(inout) :: volume_kkp1_sn(*)
+      real(c_double), intent(out) :: h_water_kkp1_sn(*)
       real(c_double), intent(inout) :: cu_ak(2, *)
+      real(c_double), intent(in) :: h_pool_kk_vec(*)
+      real(c_double), intent(inout) :: h_pool_kkp1_vec(*)
+    end function
+
+    integer(c_int) function cathy_gpu_route_set_reservoirs(handle, tipo_r, reservr, numres, maxres, n_ha, &
+                                                           hres, ares, h_soglia, h_fondo, elevation) &
+        bind(C, name="cathy_gpu_route_set_reservoirs")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      integer(c_int), intent(in) :: tipo_r(*), reservr(*), numres, maxres, n_ha(*)
+      real(c_double), intent(in) :: hres(maxres, *), ares(maxres, *), h_soglia(*), h_fondo(*), elevation(*)
     end function
 
     integer(c_int) function cathy_gpu_route_destroy(handle) bind(C, name="cathy_gpu_route_destroy")

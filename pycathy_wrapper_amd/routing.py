@@ -60,6 +60,10 @@ class Network:
     nrc: np.ndarray
     dx: float = 1.0
     dy: float = 1.0
+    # reservoirs, or None: tipo_r, reservr, elevation (nrow*ncol, by I_BASIN - 1:
+    # TIPO_R, RESERVR, ELEVATION), n_ha (numres), hres / ares ((30, maxres):
+    # HRES(MAXRES,30) column-major), h_soglia, h_fondo (numres), maxres
+    reservoirs: dict | None = None
 
     @property
     def ncell(self) -> int:
@@ -70,13 +74,19 @@ class Network:
         return self.nrow * self.ncol
 
 
-def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0) -> Network:
+def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0, reservoir: bool = False) -> Network:
     """A drainage network on z = s*((I-1) + (J-1))*dx + noise (|noise| <
     s*dx/4, so every cell but (1, 1) has a lower cardinal neighbour and no
     two cells tie).  Each cell sends its water down its steepest cardinal
     and steepest diagonal descents in proportion to their slopes; cells on
     the low edges have no lower diagonal and send all of it cardinally.
-    Roughness, width and channel-shape parameters vary by cell."""
+    Roughness, width and channel-shape parameters vary by cell.
+
+    With ``reservoir``: one reservoir (RESERVR = 1) whose outlet is the cell
+    (I, J) = (ncol/2, nrow/2) (TIPO_R = 1) and whose lake covers the two
+    cells above it (TIPO_R = 1000 + the outlet's I_BASIN, no routing of
+    their own); a 5-entry height-area table around the outlet's elevation,
+    the spill height 0.2 m above it and the bottom 1 m below."""
     rng = np.random.default_rng(seed)
     s = 0.02
     I = np.arange(1, ncol + 1)[None, :]
@@ -105,11 +115,29 @@ def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0) -> 
     order = np.argsort(-z.ravel(order="C"), kind="stable")
     qoi = ib.ravel(order="C")[order].astype(np.int32)
     f = lambda lo, hi: rng.uniform(lo, hi, (nrow, ncol))
+    res = None
+    if reservoir:
+        io, jo = ncol // 2, nrow // 2                        # 1-based outlet (I, J)
+        ib_o = (io - 1) * nrow + jo
+        tipo = np.zeros(nrow * ncol, dtype=np.int32)
+        reservr = np.zeros(nrow * ncol, dtype=np.int32)
+        tipo[ib_o - 1], reservr[ib_o - 1] = 1, 1
+        for i, j in ((io + 1, jo), (io, jo + 1)):
+            tipo[(i - 1) * nrow + j - 1] = 1000 + ib_o
+            reservr[(i - 1) * nrow + j - 1] = 1
+            w1[j - 1, i - 1] = w2[j - 1, i - 1] = 0.0
+        zo = z[jo - 1, io - 1]
+        elev = np.empty(nrow * ncol)
+        elev[(ib.ravel(order="C") - 1)] = z.ravel(order="C")
+        res = dict(tipo_r=tipo, reservr=reservr, n_ha=np.array([5], dtype=np.int32), maxres=1,
+                   hres=(zo + np.array([-2.0, -0.5, 0.5, 1.5, 3.0])).reshape(5, 1),
+                   ares=np.array([5.0, 10.0, 20.0, 30.0, 40.0]).reshape(5, 1),
+                   h_soglia=np.array([zo + 0.2]), h_fondo=np.array([zo - 1.0]), elevation=elev)
     return Network(nrow=nrow, ncol=ncol, qoi=qoi, pout1=pout1, pout2=pout2, w1=w1, w2=w2,
                    slope1=np.maximum(sl1, 1e-3), slope2=np.maximum(sl2, 1e-3),
                    epl1=np.full((nrow, ncol), dx), epl2=np.full((nrow, ncol), dx * np.sqrt(2.0)),
                    kss1=f(5.0, 40.0), kss2=f(5.0, 40.0), ws1=f(0.5, 5.0), ws2=f(0.5, 5.0),
-                   b1=f(0.0, 0.5), y1=f(0.0, 0.3), nrc=np.round(f(1.0, 3.0)), dx=dx, dy=dx)
+                   b1=f(0.0, 0.5), y1=f(0.0, 0.3), nrc=np.round(f(1.0, 3.0)), dx=dx, dy=dx, reservoirs=res)
 
 
 STATE_IN = ("surface_water", "q_in_kk", "q_out_kk1", "q_out_kk2", "volume_kk")
@@ -133,15 +161,28 @@ class RouteGPU:
             C.byref(h), _i(net.nrow), _i(net.ncol), _i(net.ncol), _i(net.ncell), _p(k["qoi"]), _p(k["pout1"]),
             _p(k["pout2"]), *[_p(k[n]) for n in DTM_REAL]), "cathy_gpu_route_create")
         self.h = h
+        self.numres = 0
+        if net.reservoirs is not None:
+            res = net.reservoirs
+            self._keep.update(tipo_r=i32(res["tipo_r"]), reservr=i32(res["reservr"]), n_ha=i32(res["n_ha"]),
+                              hres=f64(res["hres"]), ares=f64(res["ares"]), h_soglia=f64(res["h_soglia"]),
+                              h_fondo=f64(res["h_fondo"]), elevation=f64(res["elevation"]))
+            self.numres = len(res["h_soglia"])
+            _check(lib().cathy_gpu_route_set_reservoirs(
+                C.byref(self.h), _p(k["tipo_r"]), _p(k["reservr"]), _i(self.numres), _i(res["maxres"]),
+                _p(k["n_ha"]), _p(k["hres"]), _p(k["ares"]), _p(k["h_soglia"]), _p(k["h_fondo"]),
+                _p(k["elevation"])), "cathy_gpu_route_set_reservoirs")
         info = np.zeros(4, dtype=np.int32)
         _check(lib().cathy_gpu_route_info(C.byref(self.h), _p(info)), "cathy_gpu_route_info")
         self.nlevel, self.maxpred = int(info[0]), int(info[1])
 
     def step(self, deltat: float, state: dict) -> tuple[np.ndarray, np.ndarray]:
         """Advance ``state`` (dict of float64 arrays, (nmember, nrow*ncol) or
-        (nrow*ncol,), indexed by I_BASIN - 1) in place: q_in_kkp1 / q_out_kkp1_* are read and written,
-        volume_kkp1 / h_water_kkp1 written.  Returns CU_MAX and AK_MAX per
-        member (AK_MAX enters as state["ak_max"] if present, else 0)."""
+        (nrow*ncol,), indexed by I_BASIN - 1) in place: q_in_kkp1 / q_out_kkp1_* /
+        volume_kkp1 are read and written, h_water_kkp1 written; with
+        reservoirs also h_pool_kk (read) and h_pool_kkp1 (read and written),
+        (nmember, numres).  Returns CU_MAX and AK_MAX per member (AK_MAX
+        enters as state["ak_max"] if present, else 0)."""
         ncell = self.net.ngrid
         arrs = {}
         for k in STATE_IN + STATE_INOUT + STATE_OUT:
@@ -154,10 +195,17 @@ class RouteGPU:
             raise ValueError("every state array needs nmember * ncell values")
         cu_ak = np.zeros((nm, 2))
         cu_ak[:, 1] = np.asarray(state.get("ak_max", 0.0), dtype=np.float64)
+        hp = [None, None]
+        if self.numres:
+            for i, k in enumerate(("h_pool_kk", "h_pool_kkp1")):
+                a = state[k]
+                if a.dtype != np.float64 or not a.flags.c_contiguous or a.size != nm * self.numres:
+                    raise ValueError(f"{k}: float64, C-contiguous, nmember * numres values")
+                hp[i] = _p(a)
         _check(lib().cathy_gpu_route_step(
             C.byref(self.h), _i(nm), C.byref(C.c_double(deltat)), C.byref(C.c_double(self.net.dx)),
             C.byref(C.c_double(self.net.dy)), *[_p(arrs[k]) for k in STATE_IN + STATE_INOUT + STATE_OUT],
-            _p(cu_ak)), "cathy_gpu_route_step")
+            _p(cu_ak), hp[0], hp[1]), "cathy_gpu_route_step")
         return cu_ak[:, 0].copy(), cu_ak[:, 1].copy()
 
     def close(self):

@@ -69,21 +69,22 @@ def test_reference_route_call_runs_on_device():
     reference's library (linked ahead of route.o): the caller writes the
     network and flows into COMMON /RIVERNETWORK/ and /SURFWATER/ and calls
     ROUTE with the reference's arguments, then the reference's own ALTEZZE.
-    Four chained substeps against the oracle (bit-identical to the
-    reference's ROUTE and ALTEZZE), to 1e-12."""
+    The network holds a reservoir (SERBATOIO on the device) beside the
+    channel cells.  Six chained substeps against the oracle (bit-identical
+    to the reference's ROUTE and ALTEZZE), to 1e-12, the pool level too."""
     if not os.path.exists(REF_GPU):
         pytest.skip("oracle/_ref/libcathy_ref_gpu.so not built (needs the reference sources)")
     import pyoracle as P
     from pycathy_wrapper_amd import routing as RT
     from test_oracle_route import RefRouting, advance, initial_state
     from pyoracle import REF_COLMAX, REF_ROWMAX
-    net = RT.synthetic_network(REF_ROWMAX, REF_COLMAX, dx=10.0, seed=2)
+    net = RT.synthetic_network(REF_ROWMAX, REF_COLMAX, dx=10.0, seed=2, reservoir=True)
     rr = RefRouting(P.RefLib(REF_GPU), net)
     orc = P.Oracle()
     st = initial_state(net)
     so = {k: v.copy() for k, v in st.items()}
     ak_r = ak_o = 0.0
-    for sub in range(4):
+    for sub in range(6):
         out = rr.step(30.0, st, ak_r)
         cu_o, ak_o = orc.route(net, 30.0, so | {"ak_max": ak_o})
         for k in ("q_in_kkp1", "q_out_kkp1_1", "q_out_kkp1_2", "volume_kkp1", "h_water_kkp1"):
@@ -91,6 +92,8 @@ def test_reference_route_call_runs_on_device():
             assert e <= 1e-12, (sub, k, e)
             st[k][...] = out[k]
         assert out["cu_max"] == pytest.approx(cu_o, rel=1e-12) and out["ak_max"] == pytest.approx(ak_o, rel=1e-12)
+        assert out["h_pool_kkp1"][0] == pytest.approx(so["h_pool_kkp1"][0], rel=1e-12, abs=0), sub
+        st["h_pool_kkp1"][...] = out["h_pool_kkp1"]
         ak_r = float(out["ak_max"])
         advance(st)
         advance(so)

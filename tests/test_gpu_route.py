@@ -39,10 +39,12 @@ def _compare(net, dev_st, orc_states, cu_d, ak_d, cu_o, ak_o, tag):
         assert cu_d[m] == pytest.approx(cu_o[m], rel=TOL) and ak_d[m] == pytest.approx(ak_o[m], rel=TOL), (tag, m)
 
 
-def test_route_substeps_match_oracle():
-    """The 20x20 network of the reference pin, one member, four substeps
-    (each substep's new levels feed the next)."""
-    net = RT.synthetic_network(20, 20, dx=10.0, seed=2)
+@pytest.mark.parametrize("reservoir", [False, True], ids=["channels", "reservoir"])
+def test_route_substeps_match_oracle(reservoir):
+    """The 20x20 network of the reference pin, one member, six substeps
+    (each substep's new levels feed the next); with a reservoir whose pool
+    rises through its spill height."""
+    net = RT.synthetic_network(20, 20, dx=10.0, seed=2, reservoir=reservoir)
     import pyoracle as P
     orc = P.Oracle()
     g = RT.RouteGPU(net)
@@ -51,10 +53,12 @@ def test_route_substeps_match_oracle():
     so = {k: v.copy() for k, v in sd.items()}
     ak_d = ak_o = 0.0
     try:
-        for sub in range(4):
+        for sub in range(6):
             cu_d, akd = g.step(30.0, sd | {"ak_max": ak_d})
             cu_o, ako = orc.route(net, 30.0, so | {"ak_max": ak_o})
             _compare(net, sd, [so], cu_d, akd, [cu_o], [ako], sub)
+            if reservoir:
+                assert sd["h_pool_kkp1"][0] == pytest.approx(so["h_pool_kkp1"][0], rel=TOL, abs=0), sub
             ak_d, ak_o = float(akd[0]), ako
             advance(sd)
             advance(so)
@@ -62,11 +66,12 @@ def test_route_substeps_match_oracle():
         g.close()
 
 
-def test_route_large_ensemble_matches_oracle():
+@pytest.mark.parametrize("reservoir", [False, True], ids=["channels", "reservoir"])
+def test_route_large_ensemble_matches_oracle(reservoir):
     """A 201x201-cell network (40,401 cells) with four members of different
     surface water and flows in one launch, against the oracle member by
-    member."""
-    net = RT.synthetic_network(201, 201, dx=10.0, seed=4)
+    member (with a reservoir: also its pool level)."""
+    net = RT.synthetic_network(201, 201, dx=10.0, seed=4, reservoir=reservoir)
     import pyoracle as P
     orc = P.Oracle()
     g = RT.RouteGPU(net)
@@ -78,6 +83,9 @@ def test_route_large_ensemble_matches_oracle():
         cu_d, ak_d = g.step(15.0, sd)
         res = [orc.route(net, 15.0, st) for st in orc_states]
         _compare(net, sd, orc_states, cu_d, ak_d, [r[0] for r in res], [r[1] for r in res], "large")
+        if reservoir:
+            for m, st in enumerate(orc_states):
+                assert sd["h_pool_kkp1"][m, 0] == pytest.approx(st["h_pool_kkp1"][0], rel=TOL, abs=0), m
         assert len(set(np.round(cu_d, 12))) == nm
     finally:
         g.close()
@@ -89,4 +97,17 @@ def test_route_rejects_unroutable_network():
     net = RT.synthetic_network(6, 5, dx=10.0, seed=1)
     bad = RT.Network(**{**net.__dict__, "qoi": net.qoi[::-1].copy()})
     with pytest.raises(RT.CathyGpuError, match="drains"):
+        RT.RouteGPU(bad)
+
+
+def test_route_rejects_routing_lake_cell():
+    """A lake cell (TIPO_R >= 1000) that routes water is refused, where the
+    reference stops (route.f: 'INCOMPRENSIBILE CASE')."""
+    net = RT.synthetic_network(20, 20, dx=10.0, seed=2, reservoir=True)
+    lake = int(np.flatnonzero(net.reservoirs["tipo_r"] >= 1000)[0])
+    j, i = lake % net.nrow, lake // net.nrow
+    w1 = net.w1.copy()
+    w1[j, i] = 1.0
+    bad = RT.Network(**{**net.__dict__, "w1": w1})
+    with pytest.raises(RT.CathyGpuError, match="TIPO_R >= 1000"):
         RT.RouteGPU(bad)

@@ -38,6 +38,11 @@ def initial_state(net, nmember: int = 1, seed: int = 5) -> dict:
               volume_kk=rng.uniform(0.0, 0.01, shp) * area, q_in_kkp1=np.zeros(shp),
               q_out_kkp1_1=np.zeros(shp), q_out_kkp1_2=np.zeros(shp), volume_kkp1=np.zeros(shp),
               h_water_kkp1=np.zeros(shp))
+    if net.reservoirs is not None:     # pool 2 cm below its spill height
+        nres = len(net.reservoirs["h_soglia"])
+        hp = np.broadcast_to(net.reservoirs["h_soglia"] - 0.02, (nmember, nres) if nmember > 1 else (nres,))
+        st["h_pool_kk"] = np.array(hp)
+        st["h_pool_kkp1"] = st["h_pool_kk"].copy()
     return st
 
 
@@ -49,6 +54,8 @@ def advance(st: dict) -> None:
     st["q_out_kk2"][...] = st["q_out_kkp1_2"]
     st["volume_kk"][...] = st["volume_kkp1"]
     st["q_in_kkp1"][...] = 0.0
+    if "h_pool_kk" in st:
+        st["h_pool_kk"][...] = st["h_pool_kkp1"]
 
 
 class RefRouting:
@@ -64,7 +71,8 @@ class RefRouting:
         self.rn_d = np.ctypeslib.as_array((C.c_double * nd).from_address(base))
         self.rn_i = np.ctypeslib.as_array((C.c_int * (2 * ng + REF_MAXCEL)).from_address(base + 8 * nd))
         sw_base = C.addressof(C.c_char.in_dll(L, "surfwater_"))
-        self.sw = np.ctypeslib.as_array((C.c_double * (NSCAL_SW + len(SW_ARRAYS) * REF_MAXCEL)).from_address(sw_base))
+        nsw = NSCAL_SW + len(SW_ARRAYS) * REF_MAXCEL + 66      # + the MAXRES = 1 reservoir block
+        self.sw = np.ctypeslib.as_array((C.c_double * nsw).from_address(sw_base))
         self.rn_d[:] = 0.0
         self.rn_i[:] = 0
         for k, name in enumerate(RT.DTM_REAL):          # DTM_*(I, J): flat (I-1) + (J-1)*COLMAX
@@ -73,7 +81,29 @@ class RefRouting:
         self.rn_i[ng:2 * ng] = net.pout2.ravel()
         self.rn_i[2 * ng:] = net.qoi
         self.tipo = np.zeros(REF_MAXCEL, dtype=np.int32)
+        self.reservr = np.ones(REF_MAXCEL, dtype=np.int32)
+        self.n_ha = np.ones(4, dtype=np.int32)
         self.one = np.ones(4, dtype=np.int32)
+        res = net.reservoirs
+        if res is not None:                 # /SURFWATER/ reservoir block, MAXRES = 1
+            assert res["maxres"] == 1
+            self.tipo[:] = res["tipo_r"]
+            self.reservr[:] = res["reservr"]
+            self.n_ha[0] = res["n_ha"][0]
+            rb = self.res_block()
+            rb[:] = 0.0
+            rb[4:4 + 30] = 0.0
+            rb[4:4 + res["hres"].shape[0]] = res["hres"][:, 0]
+            rb[34:34 + res["ares"].shape[0]] = res["ares"][:, 0]
+            rb[64] = res["h_soglia"][0]
+            rb[65] = res["h_fondo"][0]
+            self.arr("ELEVATION")[:] = res["elevation"]
+
+    def res_block(self):
+        """H_POOL_KK_VEC, _SAV, _P, H_POOL_KKP1_VEC, HRES(1,30), ARES(1,30),
+        H_SOGLIA, H_FONDO (MAXRES = 1)."""
+        o = NSCAL_SW + len(SW_ARRAYS) * REF_MAXCEL
+        return self.sw[o:o + 66]
 
     def arr(self, name):
         k = SW_ARRAYS.index(name)
@@ -86,12 +116,15 @@ class RefRouting:
                           ("Q_OUT_KKP1_SN_2", "q_out_kkp1_2"), ("VOLUME_KK_SN", "volume_kk")):
             self.arr(name)[:] = st[key]
         self.sw[0] = ak_max
+        if self.net.reservoirs is not None:
+            self.res_block()[0] = st["h_pool_kk"][0]
+            self.res_block()[3] = st["h_pool_kkp1"][0]
         I = lambda v: C.byref(C.c_int(v))
         D = lambda v: C.byref(C.c_double(v))
-        self.L.route_(I(self.net.nrow), I(self.net.ncell), iptr(self.tipo), iptr(self.one), iptr(self.one),
-                      iptr(self.one), D(0.0), D(deltat))
-        out = {k: np.zeros(REF_MAXCEL) for k in ("v", "h")}
-        hpool = np.zeros(4)
+        self.L.route_(I(self.net.nrow), I(self.net.ncell), iptr(self.tipo), iptr(self.reservr), iptr(self.one),
+                      iptr(self.n_ha), D(0.0), D(deltat))
+        out = dict(v=st["volume_kkp1"].copy(), h=np.zeros(REF_MAXCEL))   # ALTEZZE keeps reservoir cells' volume
+        hpool = np.ascontiguousarray(self.res_block()[3:4]) if self.net.reservoirs is not None else np.zeros(4)
         q = {n: np.ascontiguousarray(self.arr(n)) for n in ("Q_IN_KK_SN", "Q_IN_KKP1_SN", "Q_OUT_KK_SN_1",
                                                              "Q_OUT_KK_SN_2", "Q_OUT_KKP1_SN_1",
                                                              "Q_OUT_KKP1_SN_2", "VOLUME_KK_SN",
@@ -100,10 +133,10 @@ class RefRouting:
                         dptr(q["Q_IN_KKP1_SN"]), dptr(q["Q_OUT_KK_SN_1"]), dptr(q["Q_OUT_KK_SN_2"]),
                         dptr(q["Q_OUT_KKP1_SN_1"]), dptr(q["Q_OUT_KKP1_SN_2"]), dptr(q["VOLUME_KK_SN"]),
                         dptr(out["v"]), dptr(q["SURFACE_WATER_SN"]), dptr(out["h"]), D(deltat), dptr(hpool),
-                        iptr(self.one), dptr(q["ELEVATION"]))
+                        iptr(self.reservr), dptr(q["ELEVATION"]))
         return dict(q_in_kkp1=q["Q_IN_KKP1_SN"], q_out_kkp1_1=q["Q_OUT_KKP1_SN_1"],
                     q_out_kkp1_2=q["Q_OUT_KKP1_SN_2"], volume_kkp1=out["v"], h_water_kkp1=out["h"],
-                    cu_max=self.sw[4], ak_max=self.sw[0])
+                    cu_max=self.sw[4], ak_max=self.sw[0], h_pool_kkp1=hpool.copy())
 
 
 @pytest.fixture(scope="module")
@@ -118,25 +151,36 @@ def net():
 
 
 @pytest.mark.skipif(not has_ref(), reason="oracle/_ref not built (needs the reference sources)")
-def test_route_altezze_bit_identical(oracle, ref, net):
-    """Four substeps (dt = 30 s): the reference's ROUTE + ALTEZZE and the
+@pytest.mark.parametrize("reservoir", [False, True], ids=["channels", "reservoir"])
+def test_route_altezze_bit_identical(oracle, ref, reservoir):
+    """Six substeps (dt = 30 s): the reference's ROUTE + ALTEZZE and the
     oracle give bit-identical outflows, inflows, volumes, depths, CU_MAX and
     AK_MAX.  Both branches are routed in most cells, and the last cell's
-    cardinal outflow leaves the grid."""
+    cardinal outflow leaves the grid.  With a reservoir (SERBATOIO, CALCOLI):
+    its pool level too, which rises through the spill height, so the outlet
+    passes from holding its inflow to releasing it; lake cells take the pool's
+    depth."""
+    net = RT.synthetic_network(REF_ROWMAX, REF_COLMAX, dx=10.0, seed=2, reservoir=reservoir)
     rr = RefRouting(ref, net)
     st = initial_state(net)
     assert (net.w2.ravel() > 0).sum() > net.ncell // 2
     ak = 0.0
-    for sub in range(4):
+    spills = []
+    for sub in range(6):
         ref_out = rr.step(30.0, st, ak)
         cu, ak_o = oracle.route(net, 30.0, st | {"ak_max": ak})
         for k in ("q_in_kkp1", "q_out_kkp1_1", "q_out_kkp1_2", "volume_kkp1", "h_water_kkp1"):
             assert np.array_equal(st[k], ref_out[k]), (sub, k)
         assert cu == ref_out["cu_max"] and ak_o == ref_out["ak_max"], sub
         assert cu > 0.0
+        if reservoir:
+            assert np.array_equal(st["h_pool_kkp1"], ref_out["h_pool_kkp1"]), sub
+            spills.append(st["h_pool_kkp1"][0] > net.reservoirs["h_soglia"][0])
         ak = ak_o
         advance(st)
     assert st["q_in_kk"].max() > 0.0
+    if reservoir:
+        assert not spills[0] and spills[-1], spills
 
 
 def test_route_mass_bookkeeping(oracle, net):
