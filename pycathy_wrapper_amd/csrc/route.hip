@@ -60,8 +60,8 @@ struct RouteDev {
   const int *lev_ptr;       // [nlev + 1] level k holds order indices lev_ptr[k]..lev_ptr[k+1]-1
   const int *cell;          // [ncell] grid index (I_BASIN - 1) of order index k
   const int *key;           // [ncell] 2 * QOI_SN position
-  const int *pred_ptr;      // [ncell + 1]
-  const int *pred;          // feeders: grid index * 2 + branch (0: W_1, 1: W_2)
+  const int *pred;          // [ncell][kMaxFeed] feeders in order: grid index * 2 + branch (0: W_1,
+                            // 1: W_2), then -1
   const double *prm;        // [kNParam][ncell]
   int numres;
   ResDev res;
@@ -198,24 +198,26 @@ __global__ __launch_bounds__(kRouteBlock) void k_route(
   if (threadIdx.x == 0) s_err = 0;
   __syncthreads();
   CuBest best{0.0, 0.0, -1};
+  int kbeg = d.lev_ptr[0], kend = d.lev_ptr[1];
   for (int lv = 0; lv < d.nlev; ++lv) {
-    const int kend = d.lev_ptr[lv + 1];
-    for (int k = d.lev_ptr[lv] + (int)threadIdx.x; k < kend; k += blockDim.x) {
+    const int knext = lv + 2 <= d.nlev ? d.lev_ptr[lv + 2] : 0;  // consumed after the barrier
+    for (int k = kbeg + (int)threadIdx.x; k < kend; k += blockDim.x) {
       const int c = d.cell[k];
-      // all feeder indices, then all their outflows, as two batches of
-      // loads (slots past the cell's feeders re-read its first entry, or
-      // entry 0); the sum then runs over the feeders in order
-      const int e0 = d.pred_ptr[k], ne = d.pred_ptr[k + 1] - e0;
+      // the feeder slots, then all their outflows, as two batches of loads
+      // (empty slots read entry 0); the sum runs over the feeders in order
       int src[kMaxFeed];
 #pragma unroll
-      for (int u = 0; u < kMaxFeed; ++u) src[u] = d.pred[ne > 0 ? e0 + min(u, ne - 1) : 0];
+      for (int u = 0; u < kMaxFeed; ++u) src[u] = d.pred[(size_t)k * kMaxFeed + u];
       double vin[kMaxFeed];
 #pragma unroll
-      for (int u = 0; u < kMaxFeed; ++u) vin[u] = qout[src[u] & 1][src[u] >> 1];
+      for (int u = 0; u < kMaxFeed; ++u) {
+        const int sidx = src[u] >= 0 ? src[u] : 0;
+        vin[u] = qout[sidx & 1][sidx >> 1];
+      }
       double q_in = qin[c];
 #pragma unroll
       for (int u = 0; u < kMaxFeed; ++u)
-        if (u < ne) q_in = q_in + vin[u];
+        if (src[u] >= 0) q_in = q_in + vin[u];
       const double *P = d.prm;
       const int n = d.ncell;
       const double nrc = P[kNrc * n + k];
@@ -253,6 +255,8 @@ __global__ __launch_bounds__(kRouteBlock) void k_route(
       qin[c] = q_in;
     }
     __syncthreads();
+    kbeg = kend;
+    kend = knext;
   }
   // ALTEZZE (altezze.f:33-105) over every grid cell
   for (int c = threadIdx.x; c < d.ngrid; c += blockDim.x) {
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(kRouteBlock) void k_route(
 struct Route {
   int nrow = 0, ncol = 0, ncell = 0, ngrid = 0, nlev = 0, maxpred = 0;
   hipStream_t st = nullptr;
-  DBuf<int> lev_ptr, cell, key, pred_ptr, pred;
+  DBuf<int> lev_ptr, cell, key, pred;
   DBuf<double> prm;
   // per-step state [member][ngrid]: 5 inputs, 4 in/out (volume_kkp1 keeps
   // reservoir cells), 1 output; cu_ak; pool levels [member][numres] x 2
@@ -315,7 +319,7 @@ struct Route {
   }
 
   RouteDev dev() const {
-    return RouteDev{ncell, ngrid, nlev, lev_ptr.p, cell.p, key.p, pred_ptr.p, pred.p, prm.p, numres,
+    return RouteDev{ncell, ngrid, nlev, lev_ptr.p, cell.p, key.p, pred.p, prm.p, numres,
                     ResDev{maxres, r_nha.p, r_hres.p, r_ares.p, r_soglia.p, r_fondo.p, r_rid.p, r_rflag.p,
                            r_alt.p, r_elev.p}};
   }
@@ -403,7 +407,7 @@ int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol, co
     for (int l = 0; l < nlev; ++l) lev_ptr[l + 1] += lev_ptr[l];
     std::vector<int> slot(lev_ptr.begin(), lev_ptr.end() - 1), of(n);
     for (int p = 0; p < n; ++p) of[p] = slot[level[p]]++;
-    std::vector<int> cell(n), key(n), pred_ptr(n + 1, 0), pred;
+    std::vector<int> cell(n), key(n), pred((size_t)n * kMaxFeed, -1);
     std::vector<double> prm((size_t)kNParam * n);
     int maxpred = 0;
     std::vector<int> by_k(n);
@@ -430,9 +434,8 @@ int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol, co
         put(kEb1 + br, c.eb);
         put(kDh1 + br, c.dh0);
       }
-      pred.insert(pred.end(), feeders[p].begin(), feeders[p].end());
-      pred_ptr[k + 1] = (int)pred.size();
       maxpred = std::max(maxpred, (int)feeders[p].size());
+      for (size_t u = 0; u < feeders[p].size() && u < (size_t)kMaxFeed; ++u) pred[(size_t)k * kMaxFeed + u] = feeders[p][u];
     }
     CG_REQUIRE(maxpred <= kMaxFeed, "a cell has more than 8 feeders");
     auto r = std::make_unique<Route>();
@@ -455,8 +458,6 @@ int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol, co
     r->lev_ptr.upload(lev_ptr, r->st);
     r->cell.upload(cell, r->st);
     r->key.upload(key, r->st);
-    r->pred_ptr.upload(pred_ptr, r->st);
-    if (pred.empty()) pred.push_back(0);
     r->pred.upload(pred, r->st);
     r->prm.upload(prm, r->st);
     CG_HIP(hipStreamSynchronize(r->st));
