@@ -74,3 +74,22 @@ def test_product_never_imports_the_oracle():
         for f in files:
             txt = open(os.path.join(dirpath, f), errors="replace").read()
             assert "cathy_oracle" not in txt and "orc_" not in txt, f
+
+
+def test_route_network_checks_run_before_the_device():
+    """cathy_gpu_route_create checks the drainage network on the host before
+    any HIP call, so a network the reference could not route (a cell draining
+    to one before it in QOI_SN, or off the grid) is refused with a message
+    even without a GPU."""
+    from pycathy_wrapper_amd import routing as RT
+    net = RT.synthetic_network(6, 5, dx=10.0, seed=1)
+    q = net.qoi.copy()
+    q[[0, -2]] = q[[-2, 0]]
+    with pytest.raises(RT.CathyGpuError, match="not after it in QOI_SN"):
+        RT.RouteGPU(RT.Network(**{**net.__dict__, "qoi": q}))
+    with pytest.raises(RT.CathyGpuError, match="drains off the grid"):
+        RT.RouteGPU(RT.Network(**{**net.__dict__, "qoi": net.qoi[::-1].copy()}))
+    dup = net.qoi.copy()
+    dup[1] = dup[0]
+    with pytest.raises(RT.CathyGpuError, match="twice"):
+        RT.RouteGPU(RT.Network(**{**net.__dict__, "qoi": dup}))
