@@ -307,7 +307,8 @@ int cathy_gpu_route_info(int64_t *handle, int *info);
  * (out), AK_MAX (in/out: kept when no branch is routed).  With reservoirs:
  * H_POOL_KK_VEC (in) and H_POOL_KKP1_VEC (in/out), (NUMRES, NMEMBER); else
  * both may be NULL.  Returns -1 where CALCOLI stops (a pool level at or
- * below its table).  Synchronous. */
+ * below its table).  Synchronous; calls on one handle from several threads
+ * run one after another. */
 int cathy_gpu_route_step(int64_t *handle, const int *nmember, const double *deltat,
                          const double *delta_x, const double *delta_y,
                          const double *surface_water_sn, const double *q_in_kk_sn,

@@ -299,6 +299,7 @@ __global__ __launch_bounds__(kRouteBlock) void k_route(
 
 struct Route {
   int nrow = 0, ncol = 0, ncell = 0, ngrid = 0, nlev = 0, maxpred = 0;
+  std::mutex mu;  // steps and reservoir set-up on one handle share its device buffers
   hipStream_t st = nullptr;
   DBuf<int> lev_ptr, cell, key, pred;
   DBuf<double> prm;
@@ -475,6 +476,7 @@ int cathy_gpu_route_set_reservoirs(int64_t *handle, const int *tipo_r, const int
     CG_REQUIRE(tipo_r && reservr && numres && maxres && n_ha && hres && ares && h_soglia && h_fondo && elevation,
                "null argument");
     Route *r = route_lookup(handle);
+    std::lock_guard<std::mutex> lk(r->mu);
     const int nres = *numres, ld = *maxres, ng = r->ngrid, n = r->ncell;
     CG_REQUIRE(nres >= 1 && ld >= nres, "NUMRES must lie in 1..MAXRES");
     for (int q = 0; q < nres; ++q) CG_REQUIRE(n_ha[q] >= 2 && n_ha[q] <= 30, "N_HA must lie in 2..30");
@@ -561,6 +563,7 @@ int cathy_gpu_route_step(int64_t *handle, const int *nmember, const double *delt
                    volume_kkp1_sn && h_water_kkp1_sn && cu_ak,
                "null argument");
     Route *r = route_lookup(handle);
+    std::lock_guard<std::mutex> lk(r->mu);
     CG_REQUIRE(!r->numres || (h_pool_kk_vec && h_pool_kkp1_vec), "reservoirs need H_POOL_KK_VEC / H_POOL_KKP1_VEC");
     const int m = *nmember;
     CG_REQUIRE(m >= 1 && m <= 65535, "NMEMBER must lie in 1..65535");
