@@ -116,7 +116,10 @@ int cathy_gpu_flow3d_set_params(int64_t *handle, const int *iprm,
 
 /* Atmospheric series (the atmbc file read by ATMONE/ATMNXT): NATM
  * records at ATMTIM(NATM), values ATMVAL(NATM) if HSPATM=1 else
- * ATMVAL(NNOD,NATM) (m/s, +ve rain). */
+ * ATMVAL(NNOD,NATM) (m/s, +ve rain).  The record width is the one of the
+ * HSPATM in force at this call, so set_params comes first; after a
+ * set_params that changes HSPATM, init and advance refuse the series until
+ * it is set again. */
 int cathy_gpu_flow3d_set_forcing(int64_t *handle, const int *natm,
                                  const double *atmtim, const double *atmval);
 
@@ -295,6 +298,14 @@ int cathy_gpu_route_set_reservoirs(int64_t *handle, const int *tipo_r, const int
                                    const double *hres, const double *ares,
                                    const double *h_soglia, const double *h_fondo,
                                    const double *elevation);
+/* ALTEZZE's cell-type array (its second argument, altezze.f:11,33-105):
+ * CELTYPE(NROW*NCOL) by I_BASIN; 0 takes the volume balance, 1..999 the
+ * pool depth of reservoir NUM_R(c) = RESERVR(c), >= 1000 the lake depth of
+ * the reservoir of cell CELTYPE-1000.  Default, and what the reference's
+ * time loop passes (cathy_main.f:3062, surf_flowtra.f:108, where CELTYPE is
+ * never assigned): 0 everywhere.  Non-zero types need set_reservoirs first
+ * (set_reservoirs clears them). */
+int cathy_gpu_route_set_celtype(int64_t *handle, const int *celtype);
 /* info(4): levels of the network, most feeders of one cell, NCELL,
  * NROW*NCOL. */
 int cathy_gpu_route_info(int64_t *handle, int *info);
@@ -302,8 +313,8 @@ int cathy_gpu_route_info(int64_t *handle, int *info);
  * (NROW*NCOL, NMEMBER) indexed by I_BASIN (SURFWATER.H names).
  * In: SURFACE_WATER_SN, Q_IN_KK_SN, Q_OUT_KK_SN_1/2, VOLUME_KK_SN.
  * In/out: Q_IN_KKP1_SN (receives the upstream outflows), Q_OUT_KKP1_SN_1/2
- * (a branch with W = 0 keeps its value), VOLUME_KKP1_SN (reservoir and lake
- * cells keep theirs).  Out: H_WATER_KKP1_SN.  cu_ak(2, NMEMBER): CU_MAX
+ * (a branch with W = 0 keeps its value), VOLUME_KKP1_SN (cells of non-zero
+ * ALTEZZE type keep theirs).  Out: H_WATER_KKP1_SN.  cu_ak(2, NMEMBER): CU_MAX
  * (out), AK_MAX (in/out: kept when no branch is routed).  With reservoirs:
  * H_POOL_KK_VEC (in) and H_POOL_KKP1_VEC (in/out), (NUMRES, NMEMBER); else
  * both may be NULL.  Returns -1 where CALCOLI stops (a pool level at or

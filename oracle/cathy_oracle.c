@@ -2306,13 +2306,13 @@ int orc_route(const orc_network *net, double deltat, const double *surface_water
 /* ALTEZZE (SRC/altezze.f:4-107): cell volume and water depth at the new
  * time level; reservoir cells (0 < TIPO_R < 1000) and lake cells
  * (TIPO_R >= 1000) take the depth of their reservoir's pool. */
-void orc_altezze(const orc_network *net, int ncell, double dx, double dy, double deltat, const double *q_in_kk,
-                 const double *q_in_kkp1, const double *q_out_kk1, const double *q_out_kk2,
+void orc_altezze(const orc_network *net, const int *celtype, int ncell, double dx, double dy, double deltat,
+                 const double *q_in_kk, const double *q_in_kkp1, const double *q_out_kk1, const double *q_out_kk2,
                  const double *q_out_kkp1_1, const double *q_out_kkp1_2, const double *volume_kk,
                  const double *surface_water, double *volume_kkp1, double *h_water_kkp1,
                  const double *h_pool_kkp1) {
   for (int c = 0; c < ncell; ++c) {
-    const int tipo = net->tipo_r ? net->tipo_r[c] : 0;
+    const int tipo = celtype ? celtype[c] : 0;
     if (tipo == 0) {
       const double dv = (q_in_kk[c] + q_in_kkp1[c]) / 2 * deltat + surface_water[c] * deltat -
                         (q_out_kk1[c] + q_out_kk2[c]) / 2 * deltat -

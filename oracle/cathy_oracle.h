@@ -397,10 +397,13 @@ void orc_mc(double local_slope, double epl, double ks, double w, double b1, doub
 int orc_route(const orc_network *net, double deltat, const double *surface_water, const double *q_in_kk,
               const double *q_out_kk1, const double *q_out_kk2, double *q_in_kkp1, double *q_out_kkp1_1,
               double *q_out_kkp1_2, double *cu_ak, const double *h_pool_kk, double *h_pool_kkp1);
-/* ALTEZZE over ncell cells; with reservoirs (net->tipo_r) the reservoir and
- * lake cells take their depth from H_POOL_KKP1_VEC and keep volume_kkp1. */
-void orc_altezze(const orc_network *net, int ncell, double dx, double dy, double deltat, const double *q_in_kk,
-                 const double *q_in_kkp1, const double *q_out_kk1, const double *q_out_kk2,
+/* ALTEZZE (SRC/altezze.f:11-107) over ncell cells with its type array
+ * celtype (NULL: 0 everywhere, what the reference's time loop passes,
+ * cathy_main.f:3062): cells of type 1..999 take the depth of reservoir
+ * NUM_R(c) = net->reservr[c] from H_POOL_KKP1_VEC, type >= 1000 the lake
+ * depth of the reservoir of cell type-1000; both keep volume_kkp1. */
+void orc_altezze(const orc_network *net, const int *celtype, int ncell, double dx, double dy, double deltat,
+                 const double *q_in_kk, const double *q_in_kkp1, const double *q_out_kk1, const double *q_out_kk2,
                  const double *q_out_kkp1_1, const double *q_out_kkp1_2, const double *volume_kk,
                  const double *surface_water, double *volume_kkp1, double *h_water_kkp1,
                  const double *h_pool_kkp1);

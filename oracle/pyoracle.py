@@ -154,10 +154,12 @@ class Oracle:
         return topol, ja[:nterm].copy(), tetja.reshape(-1, 4, 4)
 
     # --- whole run ----------------------------------------------------------
-    def route(self, net, deltat: float, st: dict) -> tuple[float, float]:
+    def route(self, net, deltat: float, st: dict, celtype=None) -> tuple[float, float]:
         """orc_route + orc_altezze on one member's state dict (the keys of
         pycathy_wrapper_amd.routing.RouteGPU.step, plus h_pool_kk /
-        h_pool_kkp1 with reservoirs); arrays updated in place.  Returns
+        h_pool_kkp1 with reservoirs); arrays updated in place.  ``celtype``:
+        ALTEZZE's type array (nrow*ncol, by I_BASIN - 1); None is 0
+        everywhere, as the reference's time loop passes it.  Returns
         (CU_MAX, AK_MAX); raises where the reference would stop."""
         keep = [np.ascontiguousarray(a) for a in
                 (net.qoi.astype(np.int32), net.pout1.astype(np.int32), net.pout2.astype(np.int32))]
@@ -183,7 +185,9 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("orc_route: the reference stops here (reservoir level outside its table, "
                                "or a routed lake cell)")
-        self.L.orc_altezze(C.byref(on), C.c_int(net.nrow * net.ncol), C.c_double(net.dx), C.c_double(net.dy),
+        ct = None if celtype is None else np.ascontiguousarray(celtype, dtype=np.int32)
+        self.L.orc_altezze(C.byref(on), iptr(ct) if ct is not None else None, C.c_int(net.nrow * net.ncol),
+                           C.c_double(net.dx), C.c_double(net.dy),
                            C.c_double(deltat),
                            *[dptr(st[k]) for k in ("q_in_kk", "q_in_kkp1", "q_out_kk1", "q_out_kk2",
                                                    "q_out_kkp1_1", "q_out_kkp1_2", "volume_kk", "surface_water",

@@ -125,6 +125,7 @@ struct Flow {
   // host time / balance state (cathy_main.f)
   Params prm;
   std::vector<double> atm_t, atm_v;  // series
+  int atm_w = 0;                     // values per record of the series (1 or NNOD, from HSPATM when set)
   int atm_next = 0, htiatm = 0;
   double atmtim[3] = {0, 0, 0};
   std::vector<double> atminp;  // 3 x (1 or nnod) rates
@@ -1489,6 +1490,8 @@ struct Flow {
   // inital.f (INIT1, ATMONE, MBINIT) + cathy_main.f:2663-2704.
   void init(const double *ptimep_ref) {
     CG_REQUIRE(!atm_t.empty(), "forcing not set");
+    CG_REQUIRE(atm_w == arow() && atm_v.size() == atm_t.size() * (size_t)arow(),
+               "forcing records do not match HSPATM (set the parameters before the forcing)");
     std::vector<double> h(n);
     for (int i = 0; i < n; ++i) h[perm[i]] = ptimep_ref[i];
     ptimep.upload(h, st);
@@ -1607,6 +1610,8 @@ struct Flow {
   // One accepted step of cathy_main.f:2882-3830.
   int advance(double *stv) {
     CG_REQUIRE(initialised, "flow3d_init not called");
+    CG_REQUIRE(atm_w == arow() && atm_v.size() == atm_t.size() * (size_t)arow(),
+               "forcing records do not match HSPATM");
     if (finished) return finished;
     const auto t0 = std::chrono::steady_clock::now();
     const double pcg0 = c_pcg, pic0 = c_picard;
@@ -1879,16 +1884,18 @@ struct Flow {
 };
 
 // ----------------------------------------------------------- handles --
+// Handles own their Flow jointly with the calls in flight: a destroy from
+// another thread removes the handle, and the Flow ends with the last call.
 static std::mutex g_mu;
-static std::map<int64_t, std::unique_ptr<Flow>> g_flows;
+static std::map<int64_t, std::shared_ptr<Flow>> g_flows;
 static int64_t g_next = 1;
 
-static Flow *lookup(const int64_t *h) {
+static std::shared_ptr<Flow> lookup(const int64_t *h) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!h) throw Error(CATHY_GPU_EHANDLE, "null handle");
   auto it = g_flows.find(*h);
   if (it == g_flows.end()) throw Error(CATHY_GPU_EHANDLE, "unknown handle");
-  return it->second.get();
+  return it->second;
 }
 
 void set_last_error(const std::string &m) { g_last_error = m; }
@@ -1951,7 +1958,7 @@ int cathy_gpu_flow3d_destroy(int64_t *handle) {
 
 int cathy_gpu_flow3d_set_params(int64_t *handle, const int *ip, const double *dp) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(ip && dp, "null argument");
     CG_REQUIRE(ip[0] == 1 || ip[0] == 2, "IOPT must be 1 (Picard) or 2 (Newton)");
     CG_REQUIRE(ip[0] == 1 || f->use_plan, "IOPT=2 on the device needs LUMP=1 and rows of at most 31 entries");
@@ -1970,18 +1977,19 @@ int cathy_gpu_flow3d_set_params(int64_t *handle, const int *ip, const double *dp
 int cathy_gpu_flow3d_set_forcing(int64_t *handle, const int *natm, const double *atmtim,
                                  const double *atmval) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(natm && *natm >= 1 && atmtim && atmval, "forcing needs at least one record");
-    const int w = f->prm.hspatm ? 1 : f->nnod;
+    const int w = f->arow();  // one value per record (HSPATM=1) or one per surface node
     f->atm_t.assign(atmtim, atmtim + *natm);
     f->atm_v.assign(atmval, atmval + (size_t)(*natm) * w);
+    f->atm_w = w;
     return 0;
   });
 }
 
 int cathy_gpu_flow3d_init(int64_t *handle, const double *ptimep) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(ptimep, "null argument");
     f->init(ptimep);
     return 0;
@@ -2004,7 +2012,7 @@ int cathy_gpu_flow3d_vel3d(int64_t *handle, const int *tetra, const double *perm
                            const double *volur, const int *ivol, double *uu, double *vv, double *ww,
                            double *unod, double *vnod, double *wnod) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(f->initialised, "flow3d_init not called");
     f->velocities(tetra, permx, permy, permz, bi, ci, di, volur, ivol, uu, vv, ww, unod, vnod, wnod);
     return 0;
@@ -2013,7 +2021,7 @@ int cathy_gpu_flow3d_vel3d(int64_t *handle, const int *tetra, const double *perm
 
 int cathy_gpu_flow3d_set_spmv_transposed(int64_t *handle, const int *on, int *active) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(on, "null argument");
     f->spmv_transposed = *on != 0;
     if (active) *active = (f->spmv_transposed && f->ftab_ok && f->sell().coded && f->maxrow <= kUnroll) ? 1 : 0;
@@ -2023,7 +2031,7 @@ int cathy_gpu_flow3d_set_spmv_transposed(int64_t *handle, const int *on, int *ac
 
 int cathy_gpu_flow3d_set_factor_fp64(int64_t *handle, const int *on) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(on, "null argument");
     f->set_factor_fp64(*on != 0);
     return 0;
@@ -2032,7 +2040,7 @@ int cathy_gpu_flow3d_set_factor_fp64(int64_t *handle, const int *on) {
 
 int cathy_gpu_flow3d_set_dt_schedule(int64_t *handle, const int *nd, const double *dts) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(nd && dts, "null argument");
     f->set_dt_schedule(dts, *nd);
     return 0;
@@ -2041,7 +2049,7 @@ int cathy_gpu_flow3d_set_dt_schedule(int64_t *handle, const int *nd, const doubl
 
 int cathy_gpu_flow3d_guess(int64_t *handle, const int64_t *net, const float *stat) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     f->set_guess(net, stat);
     return 0;
   });
@@ -2049,7 +2057,7 @@ int cathy_gpu_flow3d_guess(int64_t *handle, const int64_t *net, const float *sta
 
 int cathy_gpu_flow3d_guess_record(int64_t *handle, const int *on) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(on, "null argument");
     CG_REQUIRE(!*on || f->gstat.p, "column features not set");
     f->grecord = *on != 0;
@@ -2069,9 +2077,9 @@ int cathy_gpu_enkf_update(const int *nens, int64_t *handles, const double *w, co
     CG_REQUIRE(nens && handles && w && inflation, "null argument");
     const int m = *nens;
     CG_REQUIRE(m >= 2 && m <= kEnkfMax, "ensemble size out of range (2..64)");
-    std::vector<Flow *> fl(m);
+    std::vector<std::shared_ptr<Flow>> fl(m);
     for (int j = 0; j < m; ++j) fl[j] = lookup(&handles[j]);
-    Flow *f0 = fl[0];
+    Flow *f0 = fl[0].get();
     int dev0 = -1;
     CG_HIP(hipGetDevice(&dev0));
     for (int j = 0; j < m; ++j) {
@@ -2114,7 +2122,7 @@ int cathy_gpu_flow3d_adopt_state(int64_t *handle) {
 
 int cathy_gpu_flow3d_set_pnew(int64_t *handle, const double *pnew) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(pnew, "null argument");
     std::vector<double> h(f->n);
     for (int i = 0; i < f->n; ++i) h[f->perm[i]] = pnew[i];
@@ -2126,7 +2134,7 @@ int cathy_gpu_flow3d_set_pnew(int64_t *handle, const double *pnew) {
 
 int cathy_gpu_flow3d_profile(int64_t *handle, const int *enable, double *spmv_ms, int64_t *launches) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     if (spmv_ms) *spmv_ms = f->spmv_ms;
     if (launches) *launches = f->spmv_launches;
     if (enable) {
@@ -2141,7 +2149,7 @@ int cathy_gpu_flow3d_profile(int64_t *handle, const int *enable, double *spmv_ms
 
 int cathy_gpu_flow3d_spmv_bytes(int64_t *handle, double *bytes) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(bytes, "null argument");
     // values (8 B) + column ids (4 B) per entry, row pointers, x read, y write
     *bytes = 12.0 * f->nnz + 4.0 * (f->n + 1) + 16.0 * f->n;
@@ -2151,7 +2159,7 @@ int cathy_gpu_flow3d_spmv_bytes(int64_t *handle, double *bytes) {
 
 int cathy_gpu_flow3d_layout(int64_t *handle, int64_t *out6) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(out6, "null argument");
     out6[0] = f->n;
     out6[1] = f->nnz;
@@ -2231,7 +2239,7 @@ int cathy_gpu_symslv_release(void) {
 
 int cathy_gpu_flow3d_counters(int64_t *handle, double *out6) {
   return guarded([&] {
-    Flow *f = lookup(handle);
+    const std::shared_ptr<Flow> f = lookup(handle);
     CG_REQUIRE(out6, "null argument");
     for (int i = 0; i < 6; ++i) out6[i] = f->last_counters[i];
     return 0;

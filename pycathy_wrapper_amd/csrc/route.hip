@@ -13,7 +13,9 @@
 // outflows in that order (the reference's summation order, so the inflow
 // is bitwise the reference's for the same outflows), routes both branches
 // and stores them; a barrier separates the levels.  ALTEZZE follows over
-// every grid cell.  CU_MAX / AK_MAX keep the reference's rule (the last
+// every grid cell, with its own cell-type array (the reference's time loop
+// passes CELTYPE, which is zero everywhere, so by default every cell takes
+// the volume balance).  CU_MAX / AK_MAX keep the reference's rule (the last
 // branch in visiting order whose CU is >= the running maximum, starting
 // from 0) as a (CU, position) maximum.
 
@@ -49,7 +51,8 @@ struct ResDev {
   const double *h_soglia, *h_fondo;
   const int *rid;           // [ncell] reservoir of order index k, -1 if none
   const int *rflag;         // [ncell] bit br: branch br goes through the reservoir; bit 2+br: it sets the pool level
-  const int *alt;           // [ngrid] ALTEZZE: -1 channel cell, 2r outlet cell of reservoir r, 2r+1 lake cell
+  const int *alt;           // [ngrid] ALTEZZE type: -1 volume balance, 2r pool depth of reservoir r
+                            // (type 1..999), 2r+1 lake depth (type >= 1000); null: all -1
   const double *elevation;  // [ngrid]
 };
 
@@ -258,9 +261,10 @@ __global__ __launch_bounds__(kRouteBlock) void k_route(
     kbeg = kend;
     kend = knext;
   }
-  // ALTEZZE (altezze.f:33-105) over every grid cell
+  // ALTEZZE (altezze.f:33-105) over every grid cell; reservoir and lake
+  // depths only where the caller's type array asks for them
   for (int c = threadIdx.x; c < d.ngrid; c += blockDim.x) {
-    const int alt = d.numres ? d.res.alt[c] : -1;
+    const int alt = d.res.alt ? d.res.alt[c] : -1;
     if (alt >= 0) {  // reservoir outlet (depth clamped below at 0) or lake cell (positive depths only)
       const double hw = hpool[alt >> 1] - d.res.elevation[c];
       h_all[off + c] = (alt & 1) ? (hw > 0.0 ? hw : 0.0) : (hw < 0 ? 0.0 : hw);
@@ -312,6 +316,8 @@ struct Route {
   int numres = 0, maxres = 0;
   std::vector<int> pos_of_k, cellh;  // QOI_SN position and grid index of order index k
   std::vector<double> wgrid[2];      // W_1 / W_2 by grid index
+  std::vector<int> reservr_h;        // RESERVR by grid index (NUM_R of ALTEZZE)
+  bool typed = false;                // ALTEZZE types set (some cell not 0)
   DBuf<int> r_nha, r_rid, r_rflag, r_alt;
   DBuf<double> r_hres, r_ares, r_soglia, r_fondo, r_elev;
 
@@ -322,20 +328,22 @@ struct Route {
   RouteDev dev() const {
     return RouteDev{ncell, ngrid, nlev, lev_ptr.p, cell.p, key.p, pred.p, prm.p, numres,
                     ResDev{maxres, r_nha.p, r_hres.p, r_ares.p, r_soglia.p, r_fondo.p, r_rid.p, r_rflag.p,
-                           r_alt.p, r_elev.p}};
+                           typed ? r_alt.p : nullptr, r_elev.p}};
   }
 };
 
+// A call in flight holds its Route: a destroy from another thread removes
+// the handle, and the Route (its buffers and mutex) ends with the last call.
 std::mutex g_route_mu;
-std::map<int64_t, std::unique_ptr<Route>> g_routes;
+std::map<int64_t, std::shared_ptr<Route>> g_routes;
 int64_t g_route_next = 1;
 
-Route *route_lookup(const int64_t *h) {
+std::shared_ptr<Route> route_lookup(const int64_t *h) {
   std::lock_guard<std::mutex> lk(g_route_mu);
   if (!h) throw Error(CATHY_GPU_EHANDLE, "null route handle");
   auto it = g_routes.find(*h);
   if (it == g_routes.end()) throw Error(CATHY_GPU_EHANDLE, "unknown route handle");
-  return it->second.get();
+  return it->second;
 }
 
 // Cell (I, J) of I_BASIN (route.f:43-51).
@@ -475,7 +483,7 @@ int cathy_gpu_route_set_reservoirs(int64_t *handle, const int *tipo_r, const int
   return guarded([&] {
     CG_REQUIRE(tipo_r && reservr && numres && maxres && n_ha && hres && ares && h_soglia && h_fondo && elevation,
                "null argument");
-    Route *r = route_lookup(handle);
+    const std::shared_ptr<Route> r = route_lookup(handle);
     std::lock_guard<std::mutex> lk(r->mu);
     const int nres = *numres, ld = *maxres, ng = r->ngrid, n = r->ncell;
     CG_REQUIRE(nres >= 1 && ld >= nres, "NUMRES must lie in 1..MAXRES");
@@ -485,7 +493,7 @@ int cathy_gpu_route_set_reservoirs(int64_t *handle, const int *tipo_r, const int
       if (v < 0 || v >= nres) throw Error(-1, "cell " + std::to_string(c + 1) + ": RESERVR outside 1..NUMRES");
       return v;
     };
-    std::vector<int> rid(n, -1), rflag(n, 0), alt(ng, -1);
+    std::vector<int> rid(n, -1), rflag(n, 0);
     std::vector<int> last_key(nres, -1), last_k(nres, -1), last_br(nres, 0);
     for (int k = 0; k < n; ++k) {
       const int c = r->cellh[k], t = tipo_r[c];
@@ -511,18 +519,18 @@ int cathy_gpu_route_set_reservoirs(int64_t *handle, const int *tipo_r, const int
     }
     for (int q = 0; q < nres; ++q)
       if (last_k[q] >= 0) rflag[last_k[q]] |= 4 << last_br[q];
-    for (int c = 0; c < ng; ++c) {
+    for (int c = 0; c < ng; ++c) {  // lake cells name a grid cell of their reservoir
       const int t = tipo_r[c];
-      if (t == 0) continue;
-      if (t < 1000) {
-        alt[c] = 2 * res_of(c);
-      } else {
-        const int c2 = t - 1000;
-        CG_REQUIRE(c2 >= 1 && c2 <= ng, "TIPO_R - 1000 outside the grid");
-        alt[c] = 2 * res_of(c2 - 1) + 1;
+      if (t >= 1000) {
+        CG_REQUIRE(t - 1000 >= 1 && t - 1000 <= ng, "TIPO_R - 1000 outside the grid");
+        (void)res_of(t - 1000 - 1);
+      } else if (t != 0) {
+        (void)res_of(c);
       }
     }
     r->numres = nres;
+    r->reservr_h.assign(reservr, reservr + ng);
+    r->typed = false;  // ALTEZZE types refer to these reservoirs: set them again
     r->maxres = ld;
     r->r_nha.upload(std::vector<int>(n_ha, n_ha + nres), r->st);
     r->r_hres.upload(std::vector<double>(hres, hres + (size_t)ld * 30), r->st);
@@ -532,9 +540,42 @@ int cathy_gpu_route_set_reservoirs(int64_t *handle, const int *tipo_r, const int
     r->r_elev.upload(std::vector<double>(elevation, elevation + ng), r->st);
     r->r_rid.upload(rid, r->st);
     r->r_rflag.upload(rflag, r->st);
-    r->r_alt.upload(alt, r->st);
     CG_HIP(hipStreamSynchronize(r->st));
     r->cap_members = 0;  // pool buffers follow NUMRES
+    return 0;
+  });
+}
+
+int cathy_gpu_route_set_celtype(int64_t *handle, const int *celtype) {
+  return guarded([&] {
+    CG_REQUIRE(celtype, "null argument");
+    const std::shared_ptr<Route> r = route_lookup(handle);
+    std::lock_guard<std::mutex> lk(r->mu);
+    const int ng = r->ngrid;
+    std::vector<int> alt(ng, -1);
+    bool any = false;
+    auto res_of = [&](int c) {  // NUM_R(c) - 1 (altezze.f:78, :89)
+      CG_REQUIRE(r->numres > 0, "ALTEZZE types other than 0 need the reservoirs (set_reservoirs) first");
+      const int v = r->reservr_h[c] - 1;
+      if (v < 0 || v >= r->numres) throw Error(-1, "cell " + std::to_string(c + 1) + ": RESERVR outside 1..NUMRES");
+      return v;
+    };
+    for (int c = 0; c < ng; ++c) {
+      const int t = celtype[c];
+      if (t == 0) continue;
+      any = true;
+      if (t < 1000) {
+        alt[c] = 2 * res_of(c);
+      } else {
+        CG_REQUIRE(t - 1000 >= 1 && t - 1000 <= ng, "ALTEZZE type - 1000 outside the grid");
+        alt[c] = 2 * res_of(t - 1000 - 1) + 1;
+      }
+    }
+    if (any) {
+      r->r_alt.upload(alt, r->st);
+      CG_HIP(hipStreamSynchronize(r->st));
+    }
+    r->typed = any;
     return 0;
   });
 }
@@ -542,7 +583,7 @@ int cathy_gpu_route_set_reservoirs(int64_t *handle, const int *tipo_r, const int
 int cathy_gpu_route_info(int64_t *handle, int *info) {
   return guarded([&] {
     CG_REQUIRE(info, "null argument");
-    const Route *r = route_lookup(handle);
+    const std::shared_ptr<const Route> r = route_lookup(handle);
     info[0] = r->nlev;
     info[1] = r->maxpred;
     info[2] = r->ncell;
@@ -562,7 +603,7 @@ int cathy_gpu_route_step(int64_t *handle, const int *nmember, const double *delt
                    q_out_kk_sn_2 && volume_kk_sn && q_in_kkp1_sn && q_out_kkp1_sn_1 && q_out_kkp1_sn_2 &&
                    volume_kkp1_sn && h_water_kkp1_sn && cu_ak,
                "null argument");
-    Route *r = route_lookup(handle);
+    const std::shared_ptr<Route> r = route_lookup(handle);
     std::lock_guard<std::mutex> lk(r->mu);
     CG_REQUIRE(!r->numres || (h_pool_kk_vec && h_pool_kkp1_vec), "reservoirs need H_POOL_KK_VEC / H_POOL_KKP1_VEC");
     const int m = *nmember;
@@ -610,7 +651,7 @@ int cathy_gpu_route_step(int64_t *handle, const int *nmember, const double *delt
 
 int cathy_gpu_route_destroy(int64_t *handle) {
   return guarded([&] {
-    std::unique_ptr<Route> r;
+    std::shared_ptr<Route> r;
     {
       std::lock_guard<std::mutex> lk(g_route_mu);
       if (!handle) throw Error(CATHY_GPU_EHANDLE, "null route handle");

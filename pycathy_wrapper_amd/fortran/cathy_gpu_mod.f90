@@ -142,6 +142,12 @@ module cathy_gpu
       real(c_double), intent(in) :: hres(maxres, *), ares(maxres, *), h_soglia(*), h_fondo(*), elevation(*)
     end function
 
+    integer(c_int) function cathy_gpu_route_set_celtype(handle, celtype) bind(C, name="cathy_gpu_route_set_celtype")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), intent(in) :: handle
+      integer(c_int), intent(in) :: celtype(*)
+    end function
+
     integer(c_int) function cathy_gpu_route_destroy(handle) bind(C, name="cathy_gpu_route_destroy")
       import :: c_int, c_int64_t
       integer(c_int64_t), intent(inout) :: handle

@@ -7,8 +7,13 @@ C  /SURFWATER/, and it returns Q_IN_KKP1_SN, Q_OUT_KKP1_SN_1/2, CU_MAX and
 C  AK_MAX there; cathy_gpu_route_step also fills VOLUME_KKP1_SN and
 C  H_WATER_KKP1_SN, which the caller's ALTEZZE then recomputes from the same
 C  flows.  The device network is set up at the first call: the network of a
-C  run does not change.  The grid is NROW x (NCELL/NROW) cells.  Reservoirs
-C  (TIPO_R /= 0) go to the library with their tables at that first call;
+C  run does not change.  NCELL counts the catchment's cells (ZONE /= 0,
+C  index_dem.f), which may be fewer than the grid's, while QOI_SN names
+C  cells as I_BASIN = (I-1)*NROW + J with I up to NCOL <= COLMAX; the
+C  device grid is therefore NROW x COLMAX cells, the extent of the DTM_*
+C  arrays (COLMAX, ROWMAX), and cells outside the catchment are simply
+C  never routed.  Reservoirs (TIPO_R /= 0 anywhere on that grid) go to the
+C  library with their tables at that first call;
 C  H_POOL_KKP1_VEC is returned as SERBATOIO leaves it.  Any error of the
 C  library stops, as the reference does where it cannot go on (its text is
 C  printed first).
@@ -29,13 +34,13 @@ C  printed first).
       SAVE HROUTE
       DATA HROUTE /0/
       IF (HROUTE .EQ. 0) THEN
-         IF (NROW .LT. 1 .OR. MOD(NCELL,NROW) .NE. 0) THEN
-            WRITE(6,*) 'ROUTE (GPU): NCELL is not a multiple of NROW'
+         IF (NROW .LT. 1 .OR. NROW .GT. ROWMAX) THEN
+            WRITE(6,*) 'ROUTE (GPU): NROW outside 1..ROWMAX'
             STOP
          END IF
-         NCOL=NCELL/NROW
+         NCOL=COLMAX
          NRES=0
-         DO I=1,NCELL
+         DO I=1,NROW*NCOL
             IF (TIPO_R(I) .NE. 0) NRES=MAX(NRES,RESERVR(I))
          END DO
          IRC=CATHY_GPU_ROUTE_CREATE(HROUTE,NROW,NCOL,COLMAX,NCELL,

@@ -13,10 +13,12 @@ DTM_*(I, J) at row J-1, column I-1 of an (nrow, ncol) array (so the flat
 index is (I-1) + (J-1)*ncol, the reference's column-major layout with
 COLMAX = ncol) and QOI_SN as 1-based I_BASIN = (I-1)*nrow + J.
 ``RouteGPU`` runs the substep for a batch of ensemble members that share the
-network (cathy_gpu_route_* in include/cathy_gpu.h).  Only cells without
-reservoirs (TIPO_R = 0) are supported; the library rejects networks whose
-downstream cells do not come later in QOI_SN, or whose targets leave the
-grid.
+network (cathy_gpu_route_* in include/cathy_gpu.h), reservoirs included
+(SERBATOIO / CALCOLI).  ALTEZZE's cell types are its own argument: the
+reference's time loop passes CELTYPE, zero everywhere (cathy_main.f:3062),
+so every cell takes the volume balance unless ``celtype`` says otherwise.
+The library rejects networks whose downstream cells do not come later in
+QOI_SN, or whose targets leave the grid.
 
 ``synthetic_network`` builds a pit-free network on a tilted, perturbed
 plane.  It stands in for the DEM preprocessing (not part of this package):
@@ -74,7 +76,8 @@ class Network:
         return self.nrow * self.ncol
 
 
-def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0, reservoir: bool = False) -> Network:
+def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0, reservoir: bool = False,
+                      masked: float = 0.0) -> Network:
     """A drainage network on z = s*((I-1) + (J-1))*dx + noise (|noise| <
     s*dx/4, so every cell but (1, 1) has a lower cardinal neighbour and no
     two cells tie).  Each cell sends its water down its steepest cardinal
@@ -86,7 +89,12 @@ def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0, res
     (I, J) = (ncol/2, nrow/2) (TIPO_R = 1) and whose lake covers the two
     cells above it (TIPO_R = 1000 + the outlet's I_BASIN, no routing of
     their own); a 5-entry height-area table around the outlet's elevation,
-    the spill height 0.2 m above it and the bottom 1 m below."""
+    the spill height 0.2 m above it and the bottom 1 m below.
+
+    With ``masked`` > 0: that fraction of the grid, the highest cells, lies
+    outside the catchment (ZONE = 0 in the reference's DEM): those cells are
+    left out of QOI_SN and route nothing, so NCELL < NROW*NCOL.  No cell
+    drains into them, since every drainage path descends."""
     rng = np.random.default_rng(seed)
     s = 0.02
     I = np.arange(1, ncol + 1)[None, :]
@@ -113,6 +121,12 @@ def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0, res
     w2 = np.where(sl1 + sl2 > 0, sl2 / np.maximum(sl1 + sl2, 1e-300), 0.0)
     w1[0, 0], w2[0, 0], pout1[0, 0], sl1[0, 0] = 1.0, 0.0, 2, s   # the outlet leaves through I = 0
     order = np.argsort(-z.ravel(order="C"), kind="stable")
+    nmask = int(round(masked * nrow * ncol))
+    if nmask:
+        out = order[:nmask]                                  # the highest cells
+        w1.ravel(order="C")[out] = 0.0
+        w2.ravel(order="C")[out] = 0.0
+        order = order[nmask:]
     qoi = ib.ravel(order="C")[order].astype(np.int32)
     f = lambda lo, hi: rng.uniform(lo, hi, (nrow, ncol))
     res = None
@@ -147,9 +161,11 @@ STATE_OUT = ("volume_kkp1", "h_water_kkp1")
 
 class RouteGPU:
     """One drainage network on the device; ``step`` runs ROUTE + ALTEZZE
-    for ``nmember`` members at once (arrays [member][I_BASIN-1])."""
+    for ``nmember`` members at once (arrays [member][I_BASIN-1]).
+    ``celtype``: ALTEZZE's type array (nrow*ncol, by I_BASIN - 1), None for
+    0 everywhere (the reference's call)."""
 
-    def __init__(self, net: Network):
+    def __init__(self, net: Network, celtype=None):
         self.net = net
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
@@ -172,6 +188,10 @@ class RouteGPU:
                 C.byref(self.h), _p(k["tipo_r"]), _p(k["reservr"]), _i(self.numres), _i(res["maxres"]),
                 _p(k["n_ha"]), _p(k["hres"]), _p(k["ares"]), _p(k["h_soglia"]), _p(k["h_fondo"]),
                 _p(k["elevation"])), "cathy_gpu_route_set_reservoirs")
+        if celtype is not None:
+            self._keep["celtype"] = i32(celtype)
+            _check(lib().cathy_gpu_route_set_celtype(C.byref(self.h), _p(self._keep["celtype"])),
+                   "cathy_gpu_route_set_celtype")
         info = np.zeros(4, dtype=np.int32)
         _check(lib().cathy_gpu_route_info(C.byref(self.h), _p(info)), "cathy_gpu_route_info")
         self.nlevel, self.maxpred = int(info[0]), int(info[1])

@@ -39,15 +39,20 @@ def _compare(net, dev_st, orc_states, cu_d, ak_d, cu_o, ak_o, tag):
         assert cu_d[m] == pytest.approx(cu_o[m], rel=TOL) and ak_d[m] == pytest.approx(ak_o[m], rel=TOL), (tag, m)
 
 
-@pytest.mark.parametrize("reservoir", [False, True], ids=["channels", "reservoir"])
-def test_route_substeps_match_oracle(reservoir):
+@pytest.mark.parametrize("reservoir,typed,masked", [(False, False, 0.0), (True, False, 0.0), (True, True, 0.0),
+                                                    (False, False, 0.3)],
+                         ids=["channels", "reservoir", "reservoir-typed-altezze", "masked-catchment"])
+def test_route_substeps_match_oracle(reservoir, typed, masked):
     """The 20x20 network of the reference pin, one member, six substeps
     (each substep's new levels feed the next); with a reservoir whose pool
-    rises through its spill height."""
-    net = RT.synthetic_network(20, 20, dx=10.0, seed=2, reservoir=reservoir)
+    rises through its spill height.  ALTEZZE takes the reference's all-zero
+    cell types, or TIPO_R ("typed"); "masked" leaves 30% of the grid outside
+    the catchment (NCELL < NROW*NCOL)."""
+    net = RT.synthetic_network(20, 20, dx=10.0, seed=2, reservoir=reservoir, masked=masked)
     import pyoracle as P
     orc = P.Oracle()
-    g = RT.RouteGPU(net)
+    celtype = net.reservoirs["tipo_r"] if typed else None
+    g = RT.RouteGPU(net, celtype=celtype)
     say(f"route 20x20: {g.nlevel} levels, at most {g.maxpred} feeders per cell")
     sd = initial_state(net)
     so = {k: v.copy() for k, v in sd.items()}
@@ -55,7 +60,7 @@ def test_route_substeps_match_oracle(reservoir):
     try:
         for sub in range(6):
             cu_d, akd = g.step(30.0, sd | {"ak_max": ak_d})
-            cu_o, ako = orc.route(net, 30.0, so | {"ak_max": ak_o})
+            cu_o, ako = orc.route(net, 30.0, so | {"ak_max": ak_o}, celtype=celtype)
             _compare(net, sd, [so], cu_d, akd, [cu_o], [ako], sub)
             if reservoir:
                 assert sd["h_pool_kkp1"][0] == pytest.approx(so["h_pool_kkp1"][0], rel=TOL, abs=0), sub

@@ -29,9 +29,10 @@ SW_ARRAYS = ("ELEVATION", "ELEVATION_WITH_LAKES", "CELLCOARSE", "H_WATER_KKP1_SN
 
 
 def initial_state(net, nmember: int = 1, seed: int = 5) -> dict:
-    """Surface water over each cell and flows of a previous substep."""
+    """Surface water over each grid cell and flows of a previous substep
+    (arrays over the whole grid, by I_BASIN - 1, like the SURFWATER arrays)."""
     rng = np.random.default_rng(seed)
-    shp = (nmember, net.ncell) if nmember > 1 else (net.ncell,)
+    shp = (nmember, net.ngrid) if nmember > 1 else (net.ngrid,)
     area = net.dx * net.dy
     st = dict(surface_water=rng.uniform(0.0, 2e-6, shp) * area, q_in_kk=rng.uniform(0.0, 0.05, shp),
               q_out_kk1=rng.uniform(0.0, 0.04, shp), q_out_kk2=rng.uniform(0.0, 0.02, shp),
@@ -62,7 +63,7 @@ class RefRouting:
     """The reference's ROUTE + ALTEZZE over its COMMON blocks."""
 
     def __init__(self, ref, net):
-        assert (net.nrow, net.ncol) == (REF_ROWMAX, REF_COLMAX) and net.ncell == REF_MAXCEL
+        assert (net.nrow, net.ncol) == (REF_ROWMAX, REF_COLMAX) and net.ncell <= REF_MAXCEL
         L = ref.L
         self.L, self.net = L, net
         ng = REF_COLMAX * REF_ROWMAX
@@ -79,7 +80,7 @@ class RefRouting:
             self.rn_d[NSCAL_NET + k * ng: NSCAL_NET + (k + 1) * ng] = getattr(net, name).ravel()
         self.rn_i[:ng] = net.pout1.ravel()
         self.rn_i[ng:2 * ng] = net.pout2.ravel()
-        self.rn_i[2 * ng:] = net.qoi
+        self.rn_i[2 * ng:2 * ng + net.ncell] = net.qoi     # QOI_SN(1:NCELL); the rest unused
         self.tipo = np.zeros(REF_MAXCEL, dtype=np.int32)
         self.reservr = np.ones(REF_MAXCEL, dtype=np.int32)
         self.n_ha = np.ones(4, dtype=np.int32)
@@ -109,7 +110,10 @@ class RefRouting:
         k = SW_ARRAYS.index(name)
         return self.sw[NSCAL_SW + k * REF_MAXCEL: NSCAL_SW + (k + 1) * REF_MAXCEL]
 
-    def step(self, deltat, st, ak_max=0.0):
+    def step(self, deltat, st, ak_max=0.0, celtype=None):
+        """ROUTE, then ALTEZZE with the type array ``celtype`` (None: zero
+        everywhere, the CELTYPE the reference's time loop passes)."""
+        ct = np.zeros(REF_MAXCEL, dtype=np.int32) if celtype is None else np.ascontiguousarray(celtype, np.int32)
         for name, key in (("SURFACE_WATER_SN", "surface_water"), ("Q_IN_KK_SN", "q_in_kk"),
                           ("Q_OUT_KK_SN_1", "q_out_kk1"), ("Q_OUT_KK_SN_2", "q_out_kk2"),
                           ("Q_IN_KKP1_SN", "q_in_kkp1"), ("Q_OUT_KKP1_SN_1", "q_out_kkp1_1"),
@@ -129,7 +133,7 @@ class RefRouting:
                                                              "Q_OUT_KK_SN_2", "Q_OUT_KKP1_SN_1",
                                                              "Q_OUT_KKP1_SN_2", "VOLUME_KK_SN",
                                                              "SURFACE_WATER_SN", "ELEVATION")}
-        self.L.altezze_(I(self.net.ncell), iptr(self.tipo), D(self.net.dx), D(self.net.dy), dptr(q["Q_IN_KK_SN"]),
+        self.L.altezze_(I(self.net.ncell), iptr(ct), D(self.net.dx), D(self.net.dy), dptr(q["Q_IN_KK_SN"]),
                         dptr(q["Q_IN_KKP1_SN"]), dptr(q["Q_OUT_KK_SN_1"]), dptr(q["Q_OUT_KK_SN_2"]),
                         dptr(q["Q_OUT_KKP1_SN_1"]), dptr(q["Q_OUT_KKP1_SN_2"]), dptr(q["VOLUME_KK_SN"]),
                         dptr(out["v"]), dptr(q["SURFACE_WATER_SN"]), dptr(out["h"]), D(deltat), dptr(hpool),
@@ -151,24 +155,31 @@ def net():
 
 
 @pytest.mark.skipif(not has_ref(), reason="oracle/_ref not built (needs the reference sources)")
-@pytest.mark.parametrize("reservoir", [False, True], ids=["channels", "reservoir"])
-def test_route_altezze_bit_identical(oracle, ref, reservoir):
+@pytest.mark.parametrize("reservoir,typed,masked", [(False, False, 0.0), (True, False, 0.0), (True, True, 0.0),
+                                                    (False, False, 0.3)],
+                         ids=["channels", "reservoir", "reservoir-typed-altezze", "masked-catchment"])
+def test_route_altezze_bit_identical(oracle, ref, reservoir, typed, masked):
     """Six substeps (dt = 30 s): the reference's ROUTE + ALTEZZE and the
     oracle give bit-identical outflows, inflows, volumes, depths, CU_MAX and
     AK_MAX.  Both branches are routed in most cells, and the last cell's
-    cardinal outflow leaves the grid.  With a reservoir (SERBATOIO, CALCOLI):
-    its pool level too, which rises through the spill height, so the outlet
-    passes from holding its inflow to releasing it; lake cells take the pool's
-    depth."""
-    net = RT.synthetic_network(REF_ROWMAX, REF_COLMAX, dx=10.0, seed=2, reservoir=reservoir)
+    cardinal outflow leaves the grid.  ALTEZZE gets the all-zero CELTYPE the
+    reference's time loop passes (cathy_main.f:3062), so every cell takes the
+    volume balance; "typed" passes TIPO_R as its type array instead, so the
+    reservoir outlet and lake cells take the pool's depth.  With a reservoir
+    (SERBATOIO, CALCOLI): its pool level too, which rises through the spill
+    height, so the outlet passes from holding its inflow to releasing it.
+    "masked": 30% of the grid lies outside the catchment, NCELL < NROW*NCOL."""
+    net = RT.synthetic_network(REF_ROWMAX, REF_COLMAX, dx=10.0, seed=2, reservoir=reservoir, masked=masked)
     rr = RefRouting(ref, net)
     st = initial_state(net)
     assert (net.w2.ravel() > 0).sum() > net.ncell // 2
+    assert (net.ncell < net.ngrid) == (masked > 0)
+    celtype = net.reservoirs["tipo_r"] if typed else None
     ak = 0.0
     spills = []
     for sub in range(6):
-        ref_out = rr.step(30.0, st, ak)
-        cu, ak_o = oracle.route(net, 30.0, st | {"ak_max": ak})
+        ref_out = rr.step(30.0, st, ak, celtype=celtype)
+        cu, ak_o = oracle.route(net, 30.0, st | {"ak_max": ak}, celtype=celtype)
         for k in ("q_in_kkp1", "q_out_kkp1_1", "q_out_kkp1_2", "volume_kkp1", "h_water_kkp1"):
             assert np.array_equal(st[k], ref_out[k]), (sub, k)
         assert cu == ref_out["cu_max"] and ak_o == ref_out["ak_max"], sub
