@@ -9,9 +9,12 @@ base; member i uses seed 2+i, member 0 is the configuration's own run),
 FP64 Picard with IC(0)-PCG (FP32 factor, 3-colour column ordering).  A step
 = one accepted implicit time step of one member (FLOW3D returning
 converged, back-steps included) -- the metric's "member Picard step".
-``value`` = member-steps/s of the whole ensemble: each member runs steps
-1..K on its own HIP stream, driven by its own host thread, so the members'
-kernels overlap on the GPU.
+``value`` = member-steps/s of the whole ensemble: the members run steps
+1..K in lockstep (cathy_gpu_ensemble_*): each round every member still
+iterating takes one Picard iteration, and each kernel of it is one launch
+over all members, with a PCG convergence mask per member.  The same steps
+with one HIP stream and host thread per member are reported beside it
+(``config.streams``).
 
 Member 0 first runs the same K steps alone (``config.single_member``):
 that pass carries the SpMV roofline, so no other stream shares the GPU
@@ -111,6 +114,81 @@ def ensemble_run(members, steps: int) -> dict:
     if any(r is None for r in res):
         raise RuntimeError("an ensemble member failed")
     return dict(wall=wall, members=res)
+
+
+def lockstep_run(members, steps: int) -> dict:
+    """Steps 1..K of every member in lockstep (cathy_gpu_ensemble_*): each
+    round every member still iterating takes one Picard iteration, one
+    launch per kernel for all of them; members leave the lockstep when they
+    have taken K steps."""
+    from pycathy_wrapper_amd import gpu
+    for pr, g in members:
+        g.init(pr.psi0)
+    e = gpu.EnsembleGPU([g for _, g in members])
+    t0 = time.perf_counter()
+    rc, taken, stats = e.advance(steps)
+    wall = time.perf_counter() - t0
+    info = e.info()
+    res = []
+    for j in range(len(members)):
+        st = stats[j]
+        res.append(dict(steps=int(taken[j]), picard=sum(s["PICARD"] for s in st), pcg=sum(s["PCG"] for s in st),
+                        back_steps=sum(s["BACK"] for s in st), sim_s=st[-1]["TIME"] if st else 0.0,
+                        max_rel_mbe_pct=max((abs(s["ERREL"]) for s in st), default=0.0),
+                        step_s=[v / 1e3 for v in e.wall_ms[j]]))
+    e.close()
+    return dict(wall=wall, members=res, info=info)
+
+
+def make_groups(members, group_rows: int) -> list[list[int]]:
+    """Lockstep groups: members of equal node count together; with
+    group_rows < 0 as many per group as the autotuner's table gives for the
+    size (pycathy_wrapper_amd/tuning.json), else each group filled up to
+    group_rows rows (at least one member).  Returns member indices per
+    group."""
+    if group_rows < 0:
+        from pycathy_wrapper_amd import autotune
+        return autotune.groups([pr.mesh.n for pr, _ in members])
+    order = sorted(range(len(members)), key=lambda j: members[j][0].mesh.n)
+    groups, cur, rows, size = [], [], 0, None
+    for j in order:
+        n = members[j][0].mesh.n
+        if cur and (n != size or rows + n > group_rows):
+            groups.append(cur)
+            cur, rows = [], 0
+        cur.append(j)
+        rows += n
+        size = n
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def grouped_run(members, steps: int, group_rows: int) -> dict:
+    """Steps 1..K of every member: lockstep groups (make_groups), each group
+    on its own HIP stream and host thread, the groups concurrently."""
+    import threading
+    groups = make_groups(members, group_rows)
+    out = [None] * len(groups)
+
+    def run(i):
+        out[i] = lockstep_run([members[j] for j in groups[i]], steps)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(len(groups))]
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t0
+    if any(o is None for o in out):
+        raise RuntimeError("a lockstep group failed")
+    res = [None] * len(members)
+    for g, o in zip(groups, out):
+        for j, r in zip(g, o["members"]):
+            res[j] = r
+    return dict(wall=wall, members=res, groups=[len(g) for g in groups],
+                rounds=sum(o["info"]["rounds"] for o in out))
 
 
 def cpu_baseline(problem, max_steps: int, seconds: float) -> dict:
@@ -294,7 +372,11 @@ def mixed_ensemble(args) -> int:
                 pr.psi0 = np.full(pr.mesh.n, -5.0)
                 pr.atm_times = np.array([0.0, 7200.0, 7201.0])
                 pr.atm_vals = np.array([2e-6, 2e-6, 0.0])
-            members.append((pr, gpu.FlowGPU(pr, gpu.ORDER_COLUMN3)))
+            fl = gpu.FlowGPU(pr, gpu.ORDER_COLUMN3)
+            if args.group_rows < 0:
+                from pycathy_wrapper_amd import autotune
+                autotune.apply(fl, autotune.entry_for(pr.mesh.n))
+            members.append((pr, fl))
             sizes.append(sz)
             iopts.append(iopt)
             seed += 1
@@ -302,7 +384,46 @@ def mixed_ensemble(args) -> int:
     for pr, g in members:  # one warm-up step each, then back to the initial state
         for _ in range(args.warmup):
             g.advance()
-    ens = ensemble_run(members, args.steps)
+    # Picard members in lockstep groups (one launch per kernel for a
+    # group's members, groups on streams of their own); Newton members,
+    # which the lockstep does not take, on a stream each beside them
+    pic = [j for j, io in enumerate(iopts) if io == 1]
+    newt = [j for j, io in enumerate(iopts) if io == 2]
+    if args.group_rows == 0:   # every member on its own stream
+        newt, pic = newt + pic, []
+    import threading
+    out_lock, out_streams = {}, {}
+
+    def run_lock():
+        out_lock["r"] = grouped_run([members[j] for j in pic], args.steps, args.group_rows)
+
+    def run_streams():
+        out_streams["r"] = ensemble_run([members[j] for j in newt], args.steps)
+
+    threads = [threading.Thread(target=run_lock)] if pic else []
+    if newt:
+        threads.append(threading.Thread(target=run_streams))
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t0
+    res = [None] * len(members)
+    if pic:
+        for j, r in zip(pic, out_lock["r"]["members"]):
+            res[j] = r
+    if newt:
+        for j, r in zip(newt, out_streams["r"]["members"]):
+            res[j] = r
+    ens = dict(wall=wall, members=res, groups=out_lock["r"]["groups"] if pic else [])
+    streams = None
+    if pic:  # the same steps with one stream and host thread per member
+        sr = ensemble_run(members, args.steps)
+        st_steps = sum(r["steps"] for r in sr["members"])
+        streams = dict(member_steps_per_s=st_steps / sr["wall"], wall_s=sr["wall"], member_steps=st_steps)
+        log(f"[streams] {len(members)} members, one stream each: {st_steps} member-steps in {sr['wall']:.3f}s -> "
+            f"{st_steps / sr['wall']:.2f} member-steps/s")
     total = sum(r["steps"] for r in ens["members"])
     per = {sz: dict(members=counts[sz], nodes=int(next(pr.mesh.n for (pr, _), s2 in zip(members, sizes) if s2 == sz)),
                     steps=sum(r["steps"] for r, s2 in zip(ens["members"], sizes) if s2 == sz),
@@ -326,11 +447,14 @@ def mixed_ensemble(args) -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": ("stress" if stress else "mixed-size") + " ensemble: " +
                                ", ".join(f"{c} x {sz}" for sz, c in counts.items()) +
-                               f" members (ensemble_member, seeds 100..), steps 1..{args.steps} each, one HIP "
-                               f"stream per member, " + ("dry start psi = -5 m, 2 h of 2e-6 m/s rain, Newton "
+                               f" members (ensemble_member, seeds 100..), steps 1..{args.steps} each, Picard "
+                               f"members in lockstep (one launch per kernel), Newton members on a HIP stream each, "
+                               + ("dry start psi = -5 m, 2 h of 2e-6 m/s rain, Newton "
                                "(ILU-BiCGSTAB) for every other member, Picard + IC(0)-PCG for the rest"
                                if stress else "plain Picard + IC(0)-PCG") + " (column3, FP32 factor)",
                    "per_size": per, "by_linearisation": by_iopt, "step_latency_ms": lat_ms,
+                   "lockstep_group_rows": args.group_rows, "lockstep_groups": ens["groups"],
+                   "streams": streams,
                    "learned_guess": False, "parallelism": "single GPU"},
         "roofline": None, "cpu_baseline": None,
     }
@@ -397,7 +521,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=60.0, help="stop the CPU baseline after this long")
     ap.add_argument("--calib-mb", type=float, default=0.0,
                     help="after the timed steps, run the copy kernel over this many MB (PMC calibration)")
-    ap.add_argument("--members", type=int, default=3, help="ensemble members run concurrently (streams)")
+    ap.add_argument("--members", type=int, default=3, help="ensemble members")
+    ap.add_argument("--group-rows", type=int, default=-1,
+                    help="lockstep groups of members up to this many rows each, one stream per group (0: one "
+                         "stream per member, no lockstep; -1: the group sizes, PCG batches and SpMV variant of the "
+                         "autotuner's table pycathy_wrapper_amd/tuning.json)")
     ap.add_argument("--fp64-factor", type=int, default=1,
                     help="also run member 0's steps with an FP64 IC(0) factor (config.fp64_factor; 0 skips)")
     ap.add_argument("--stress", type=float, default=0.0,
@@ -438,6 +566,11 @@ def main() -> int:
     t0 = time.perf_counter()
     g = gpu.FlowGPU(pr, order)
     log(f"[setup] device {name} ({cus} CUs), flow created in {time.perf_counter() - t0:.1f}s")
+    from pycathy_wrapper_amd import autotune
+    tuned = autotune.entry_for(m.n) if args.group_rows < 0 else None
+    if tuned is not None:
+        autotune.apply(g, tuned)
+        log(f"[setup] tuned launch shapes for {m.n} nodes: {dict((k, tuned[k]) for k in autotune.DEFAULT_ENTRY)}")
     lay = g.layout()
     log(f"[setup] full pattern nnz={lay['nnz']}, SELL-64 slots={lay['nnz_sell']} "
         f"({lay['nnz_sell'] / lay['nnz'] - 1:.1%} padding), longest row {lay['maxrow']}, coded columns "
@@ -452,6 +585,8 @@ def main() -> int:
         t0 = time.perf_counter()
         pi = build_problem(args.cells, args.nstr, seed=2 + i)
         gi = gpu.FlowGPU(pi, order)
+        if tuned is not None:
+            autotune.apply(gi, tuned)
         for w in range(args.warmup):
             gi.advance()
         gi.init(pi.psi0)
@@ -491,12 +626,26 @@ def main() -> int:
     if args.calib_mb > 0:
         ms = gpu.calib_copy(int(args.calib_mb * 1e6), 5)
         log(f"[calib] copy of {args.calib_mb:.0f} MB: {ms * 1e3:.1f} us/launch")
-    ens = ensemble_run([(pr, g)] + others, steps)
+    allm = [(pr, g)] + others
+    ens = grouped_run(allm, steps, args.group_rows)
     for r in ens["members"]:
         r.pop("step_s")
     ens_steps = sum(r["steps"] for r in ens["members"])
-    log(f"[ensemble] {len(ens['members'])} members: {ens_steps} member-steps in {ens['wall']:.3f}s -> "
-        f"{ens_steps / ens['wall']:.2f} member-steps/s (member 0 alone: {steps / wall:.2f} steps/s)")
+    log(f"[grouped] {len(ens['members'])} members in lockstep groups {ens['groups']}: {ens_steps} member-steps "
+        f"in {ens['wall']:.3f}s -> {ens_steps / ens['wall']:.2f} member-steps/s "
+        f"(member 0 alone: {steps / wall:.2f} steps/s)")
+    one = lockstep_run(allm, steps)
+    one_steps = sum(r["steps"] for r in one["members"])
+    one_rate = one_steps / one["wall"]
+    log(f"[lockstep] all {len(allm)} members in one group: {one_steps} member-steps in {one['wall']:.3f}s -> "
+        f"{one_rate:.2f} member-steps/s; {one['info']}")
+    streams = ensemble_run([(pr, g)] + others, steps)
+    for r in streams["members"]:
+        r.pop("step_s")
+    st_steps = sum(r["steps"] for r in streams["members"])
+    streams_rate = st_steps / streams["wall"]
+    log(f"[streams] {len(streams['members'])} members: {st_steps} member-steps in {streams['wall']:.3f}s -> "
+        f"{streams_rate:.2f} member-steps/s")
     for _, gi in others:
         gi.close()
     fp64 = None
@@ -551,7 +700,17 @@ def main() -> int:
             "workload": f"hillslope_large ensemble: {args.members} members (seeds 2..{args.members + 1}), each "
                         f"{m.nrow}x{m.ncol} cells, {m.nstr} graded layers, N={m.n} nodes, NT={m.nt} tets, steps "
                         f"1..{steps} of the 2-day run, plain Picard + IC(0)-PCG ({args.ordering} ordering, FP32 "
-                        f"factor, FP64 state/vectors), one HIP stream per member",
+                        f"factor, FP64 state/vectors), members in lockstep groups of "
+                        f"{ens['groups']} members ("
+                        + ("autotuned" if args.group_rows < 0 else f"up to {args.group_rows} rows each")
+                        + "; one launch per kernel for a group's members, per-member PCG convergence masks; "
+                        "groups on streams of their own)",
+            "lockstep_group_rows": args.group_rows,
+            "lockstep_groups": ens["groups"],
+            "tuned": None if tuned is None else {k: tuned[k] for k in autotune.DEFAULT_ENTRY},
+            "one_lockstep_group": {"member_steps_per_s": one_rate, "wall_s": one["wall"], **one["info"]},
+            "streams": {"member_steps_per_s": streams_rate, "wall_s": streams["wall"],
+                        "note": "the same members on one HIP stream and host thread each"},
             "members": args.members,
             "nodes": m.n,
             "tets": m.nt,

@@ -133,7 +133,9 @@ int cathy_gpu_flow3d_init(int64_t *handle, const double *ptimep);
  *  [6] STORE1 [7] STORE2 [8] DSTORE [9] CDSTOR [10] VIN [11] CVIN
  *  [12] VOUT [13] CVOUT [14] VIN+VOUT [15] VTOT [16] ERRAS [17] ERREL
  *  [18] CERRAS [19] CAERAS [20] PINF [21] ITRTOT [22] ITLIN [23] KLSFAI
- *  [24] PL2 [25] FINF [26] FL2 [27] IKMAX (1-based) [28..31] reserved
+ *  [24] PL2 [25] FINF [26] FL2 [27] IKMAX (1-based) [28] wall-clock ms
+ *  of the step (host clock) [29] Picard and [30] PCG iterations of the
+ *  step, back-stepped attempts included [31] reserved
  * Returns 0 while the run continues, 1 at TMAX, 2 on NOBACK, <0 on error. */
 int cathy_gpu_flow3d_advance(int64_t *handle, double *stats);
 
@@ -255,6 +257,15 @@ int cathy_gpu_flow3d_set_factor_fp64(int64_t *handle, const int *on);
 int cathy_gpu_flow3d_set_spmv_transposed(int64_t *handle, const int *on,
                                          int *active);
 
+/* PCG launch batches (no counterpart in the reference, whose PCG loop is
+ * a host loop, solscal-extended.f:1325-1377).  The device enqueues PCG
+ * iterations in batches and polls its convergence flag between them; the
+ * first batch covers FIRST_PCT % of the previous solve's iterations (8 before
+ * any), later ones FOLLOWUP iterations each.  Launches past convergence exit
+ * at once, so the results do not depend on these; defaults 100 and 2. */
+int cathy_gpu_flow3d_set_pcg_batches(int64_t *handle, const int *first_pct,
+                                     const int *followup);
+
 /* Counters of the last advance: Picard its, PCG its, IC(0) breakdown
  * fallbacks, kernel time of the linear solver (ms), total device ms. */
 int cathy_gpu_flow3d_counters(int64_t *handle, double *out6);
@@ -329,6 +340,35 @@ int cathy_gpu_route_step(int64_t *handle, const int *nmember, const double *delt
                          double *volume_kkp1_sn, double *h_water_kkp1_sn, double *cu_ak,
                          const double *h_pool_kk_vec, double *h_pool_kkp1_vec);
 int cathy_gpu_route_destroy(int64_t *handle);
+
+/* ------------------------------------------------------------------ *
+ * Lockstep ensemble.  Replaces the per-member ./cathy processes of
+ * pyCATHY's data-assimilation forecast (pyCATHY/DA/cathy_DA.py:1373-1390:
+ * multiprocessing.Pool(cpu_count()-5).map over subprocess_run_multi,
+ * cathy_tools.py:76-98) by NMEMBER flow handles advanced together on one
+ * GPU.  Each round, every member still iterating takes one Picard
+ * iteration, and every kernel of it is ONE launch over all members; the PCG
+ * keeps a convergence mask per member (a converged member's groups exit at
+ * once), and each member decides on its own when to iterate, back-step or
+ * end its step, so members of uneven mesh size and iteration counts share
+ * the launches without waiting for each other's steps.  Each member's
+ * results are those of cathy_gpu_flow3d_advance alone, bit for bit.
+ * Members: flow handles after flow3d_init, in the column ordering, with
+ * IOPT = 1, LUMP = 1 and one IC(0) precision.  While an ensemble advances,
+ * its members must not be used through other calls.
+ * ------------------------------------------------------------------ */
+int cathy_gpu_ensemble_create(int64_t *ens, const int *nmember,
+                              const int64_t *members);
+/* Advance every member by NSTEPS accepted steps (or to the end of its run).
+ * RC(NMEMBER): 0 after NSTEPS steps, 1 at TMAX, 2 on NOBACK; TAKEN(NMEMBER):
+ * steps taken; STATS(32, NSTEPS, NMEMBER) (may be NULL): each step's
+ * statistics as cathy_gpu_flow3d_advance returns them. */
+int cathy_gpu_ensemble_advance(int64_t *ens, const int *nsteps, int *rc,
+                               int *taken, double *stats);
+/* out4: members, Picard rounds, PCG polls and PCG launch batches of the
+ * last advance. */
+int cathy_gpu_ensemble_info(int64_t *ens, int64_t *out4);
+int cathy_gpu_ensemble_destroy(int64_t *ens);
 
 /* ------------------------------------------------------------------ *
  * Instrumentation (no counterpart in the reference; used by bench.py).

@@ -294,6 +294,66 @@ __global__ void k_sumsq_free(int n, const double *u, const uint8_t *isdir, doubl
 __global__ void k_copy(size_t n, const double *src, double *dst);
 __global__ void k_fill(size_t n, double v, double *dst);
 
+// ------------------------------------------ batched ensemble launches --
+// Everything one member's Picard iteration and IC(0)-PCG read or write,
+// held in device memory per member (an array over the ensemble).  A batched
+// kernel finds its member from the launch's group table and runs the
+// single-member body on that member's arrays, so each member computes what
+// its own launches would, bit for bit.
+struct MemberDev {
+  int n, nnod, nq;             // rows, surface nodes, base nodes with free drainage (0: none)
+  int g_rows, g_red, g_flux;   // groups of the member's own row / vector / flux launches
+  double tetaf, rdt, tol, pmin;
+  int imax, ncol;
+  Sell sl;
+  SoilSoA soil;
+  AsmArgs a;
+  AsmPlan P;
+  FacTab ft, fts;              // chain tables; fts: the SpMV's transposed reads (w null: off)
+  ColDesc col[4];
+  float *F32;
+  double *F64;
+  double *pnew, *pold, *ptnew, *ptold, *ptimep, *pdiff, *sw, *ckrw, *et1, *et2, *swnew, *swtimep;
+  double *tnoti, *xt5, *bcflux, *qnode, *qbot, *atmact, *atmold, *atmpot;
+  double *A, *dinv, *r, *p, *Bv, *z, *part, *res;
+  uint8_t *isdir;
+  int *ifatm, *flags, *ipart;
+  const int *row_surf, *surf_row, *refid;
+  const double *volnod, *snodi, *pnodi;
+  PcgScal *scal;
+};
+// vectors a batched sweep reads / writes
+enum SweepVec { kVecTnoti = 0, kVecPdiff, kVecR, kVecP, kVecZ };
+
+// Batched launches: `start` (nmem + 1 group offsets, see batch_member)
+// belongs to the launch shape of the kernel; `act` (per member, bits
+// 1 switch_old, 2 next iteration) selects members for the last two.
+__global__ void kb_constitutive(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_bc_rows(const int *start, int nmem, const MemberDev *md);
+template <int MR>
+__global__ void kb_assemble_plan(const int *start, int nmem, const MemberDev *md);
+template <typename FT, int NW>
+__global__ void kb_ic_factor_col3t(const int *start, int nmem, const MemberDev *md, int c);
+template <typename FT, int NW, bool LAST>
+__global__ void kb_fwd_col3(const int *start, int nmem, const MemberDev *md, int c, int vin, int vout,
+                            int use_done);
+template <typename FT, int NW>
+__global__ void kb_bwd_col3(const int *start, int nmem, const MemberDev *md, int c, int vio, int use_done);
+__global__ void kb_residual0(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_pcg_init(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_spmv_dot2(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_pcg_alpha(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_dot(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_pcg_beta(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_poll(const int *start, int nmem, const MemberDev *md, double *out);
+__global__ void kb_update_pnew(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_bkpic(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_norms(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_flux(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_norms_final(const int *start, int nmem, const MemberDev *md);
+__global__ void kb_switch_old(const int *start, int nmem, const MemberDev *md, const int *act);
+__global__ void kb_next_iter(const int *start, int nmem, const MemberDev *md, const int *act);
+
 // ------------------------------------------- Newton (newton_kernels.hip) --
 struct NewtonArgs {
   const double *etai, *ckrw, *dckrw, *detai, *gz;

@@ -7,12 +7,12 @@ namespace cg {
 // ----------------------------------------------- constitutive (a2) --
 // chpic0.f:23-36: Sw, kr, ET1 = Sw*Ss, ET2 = VGPNOT*dSe/dpsi at PTNEW, and
 // SWNEW at PNEW.
-__global__ void k_constitutive(int n, const double *__restrict__ ptnew,
+__device__ __forceinline__ void constitutive_body(Blk bk, int n, const double *__restrict__ ptnew,
                                const double *__restrict__ pnew, SoilSoA s,
                                double *__restrict__ sw, double *__restrict__ ckrw,
                                double *__restrict__ et1, double *__restrict__ et2,
                                double *__restrict__ swnew) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = bk.b * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double psat = s.psat[i], vgn = s.vgn[i], vgm = s.vgm[i];
   double pnot = s.pnot[i], rmcp = s.rmcp[i];
@@ -25,17 +25,28 @@ __global__ void k_constitutive(int n, const double *__restrict__ ptnew,
   et2[i] = pnot * dev_fvgdse(psi, psat, vgn, s.vgn1[i], s.vgpsn[i], s.vgnr[i]);
   ckrw[i] = dev_fvgkr(psi, se, s.vgmr[i], vgm);
 }
+__global__ void k_constitutive(int n, const double *__restrict__ ptnew,
+                               const double *__restrict__ pnew, SoilSoA s,
+                               double *__restrict__ sw, double *__restrict__ ckrw,
+                               double *__restrict__ et1, double *__restrict__ et2,
+                               double *__restrict__ swnew) {
+  constitutive_body(self_blk(), n, ptnew, pnew, s, sw, ckrw, et1, et2, swnew);
+}
 
 // Sw at one head vector (SWTIMEP of chpic0.f:24-27, Sw of chvelo.f) and,
 // optionally, kr (chvelo.f).
-__global__ void k_sw_kr(int n, const double *__restrict__ psi_in, SoilSoA s,
+__device__ __forceinline__ void sw_kr_body(Blk bk, int n, const double *__restrict__ psi_in, SoilSoA s,
                         double *__restrict__ sw, double *__restrict__ ckrw) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = bk.b * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double psi = psi_in[i];
   double se = dev_fvgse(psi, s.psat[i], s.vgn[i], s.vgm[i]);
   sw[i] = s.pnot[i] * se + s.rmcp[i];
   if (ckrw) ckrw[i] = dev_fvgkr(psi, se, s.vgmr[i], s.vgm[i]);
+}
+__global__ void k_sw_kr(int n, const double *__restrict__ psi_in, SoilSoA s,
+                        double *__restrict__ sw, double *__restrict__ ckrw) {
+  sw_kr_body(self_blk(), n, psi_in, s, sw, ckrw);
 }
 
 // nodelt.f: element means of kr and ET1.
@@ -56,12 +67,12 @@ __global__ void k_elem_avg(int nt, const int4 *__restrict__ enode,
 // bcpic.f:34-87 classification of each row: Dirichlet (atmospheric nodes
 // with IFATM = 1 or 2) or the flux it receives (Neumann Q, atmospheric
 // TETAF*ATMACT + (1-TETAF)*ATMOLD for IFATM = 0).
-__global__ void k_bc_rows(int n, const int *__restrict__ row_surf,
+__device__ __forceinline__ void bc_rows_body(Blk bk, int n, const int *__restrict__ row_surf,
                           const double *__restrict__ qnode, const int *__restrict__ ifatm,
                           const double *__restrict__ atmact, const double *__restrict__ atmold,
                           double tetaf, uint8_t *__restrict__ isdir,
                           double *__restrict__ bcflux) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = bk.b * blockDim.x + threadIdx.x;
   if (r >= n) return;
   uint8_t d = 0;
   double f = qnode[r];
@@ -73,6 +84,13 @@ __global__ void k_bc_rows(int n, const int *__restrict__ row_surf,
   }
   isdir[r] = d;
   bcflux[r] = f;
+}
+__global__ void k_bc_rows(int n, const int *__restrict__ row_surf,
+                          const double *__restrict__ qnode, const int *__restrict__ ifatm,
+                          const double *__restrict__ atmact, const double *__restrict__ atmold,
+                          double tetaf, uint8_t *__restrict__ isdir,
+                          double *__restrict__ bcflux) {
+  bc_rows_body(self_blk(), n, row_surf, qnode, ifatm, atmact, atmold, tetaf, isdir, bcflux);
 }
 
 // Upper-pair slot of (k,l) in an element's 10 stiffness values.
@@ -179,9 +197,9 @@ template __global__ void k_assemble<32, false>(AsmArgs);
 // the gathers are kr_e (once per incident element) and nodal values of
 // the row's neighbours.  Entries accumulate in registers by slot.
 template <int MR>
-__global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, const double *__restrict__ ckrw_node,
+__device__ __forceinline__ void assemble_plan_body(Blk bk, AsmArgs a, AsmPlan P, const double *__restrict__ ckrw_node,
                                                           const double *__restrict__ et1_node) {
-  const int r = xcd_group() * kBlock + threadIdx.x;
+  const int r = bk.b * kBlock + threadIdx.x;
   if (r >= a.n) return;
   const RowPos rp = row_pos(a.sl, r);
   const int s = r >> 6, lane = r & 63;
@@ -306,6 +324,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, 
   t = t - grav;
   a.xt5[r] = t;
   a.tnoti[r] = a.isdir[r] ? 0.0 : t + a.bcflux[r];
+}
+template <int MR>
+__global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, const double *__restrict__ ckrw_node,
+                                                          const double *__restrict__ et1_node) {
+  assemble_plan_body<MR>(self_xcd_blk(), a, P, ckrw_node, et1_node);
 }
 template __global__ void k_assemble_plan<16>(AsmArgs, AsmPlan, const double *, const double *);
 template __global__ void k_assemble_plan<32>(AsmArgs, AsmPlan, const double *, const double *);
@@ -468,21 +491,20 @@ __device__ __forceinline__ int tab_byte(const uint32_t *w, int i) { return (int)
 // a, as the pair lists do, with the row's own L_ra (rounded to FT, as
 // stored) held in registers.
 template <typename FT, int NW>
-__global__ __launch_bounds__(256) void k_ic_factor_col3t(ColDesc C, Sell sl, FacTab ft,
+__device__ __forceinline__ void ic_factor_col3t_body(Blk bk, double *smem, ColDesc C, Sell sl, FacTab ft,
                                                          const double *__restrict__ A,
                                                          const uint8_t *__restrict__ isdir, FT *__restrict__ F,
                                                          double *__restrict__ dinv, int *nfall) {
   constexpr int NWc = NW > 0 ? NW : 1;
   constexpr int NP = NW * (NW - 1) / 2;  // (j, a) pairs, a < j < NW
   constexpr int NPW = NP > 0 ? (NP + 3) / 4 : 1;
-  extern __shared__ double smem[];
-  const int tot = C.len * kChains;
+    const int tot = C.len * kChains;
   double *Sv = smem;                 // a_rr - sum earlier L^2
   double *Qv = smem + tot;           // a_r,pred - earlier dot
   int *Pd = (int *)(smem + 2 * tot); // slot of the diagonal (-(slot)-1: Dirichlet)
   int *Pl = Pd + tot;                // slot of the predecessor entry
   int *Pt = Pl + tot;                // its transpose slot
-  const int t0 = xcd_group() * kChains;
+  const int t0 = bk.b * kChains;
   const int i = threadIdx.x, k = i / kChains, t = t0 + (i % kChains);
   const bool live = k < C.len && t < C.nc;
   double Lr[NWc];
@@ -594,6 +616,14 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3t(ColDesc C, Sell sl, Fac
   __syncthreads();
   factor_chains(C, t0, Sv, Qv, Pd, Pl, Pt, A, F, dinv, nfall);
 }
+template <typename FT, int NW>
+__global__ __launch_bounds__(256) void k_ic_factor_col3t(ColDesc C, Sell sl, FacTab ft,
+                                                         const double *__restrict__ A,
+                                                         const uint8_t *__restrict__ isdir, FT *__restrict__ F,
+                                                         double *__restrict__ dinv, int *nfall) {
+  extern __shared__ double smem[];
+  ic_factor_col3t_body<FT, NW>(self_xcd_blk(), smem, C, sl, ft, A, isdir, F, dinv, nfall);
+}
 
 // The chain recurrence v_k = b_k - a_k v_(k-1) (DOWN: k ascending) or
 // v_k = b_k - a_k v_(k+1) (up), v = 0 outside the chain, for all chains of
@@ -631,15 +661,14 @@ __device__ __forceinline__ void chain_scan(double *b, double *a, int len) {
 // then runs the backward sweep L^T x = y of the same chains from LDS
 // (x_k = y_k/L_kk - u'_k x_k+1, as k_bwd_col3 forms it) and stores x.
 template <typename FT, int NW, bool LAST>
-__global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
+__device__ __forceinline__ void fwd_col3_body(Blk bk, double *smem, ColDesc C, Sell sl, const FT *__restrict__ F,
                                                   const double *__restrict__ dinv, const double *__restrict__ tin,
                                                   double *__restrict__ y, const int *__restrict__ done) {
   if (done && *done) return;
-  extern __shared__ double smem[];
-  const int tot = C.len * kChains;
+    const int tot = C.len * kChains;
   double *T = smem, *Lp = smem + tot;
   double *U = smem + 2 * tot, *D = smem + 3 * tot;  // LAST only
-  const int t0 = xcd_group() * kChains;
+  const int t0 = bk.b * kChains;
   const int kl = C.len - 1;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / kChains, t = t0 + (i % kChains);
@@ -719,18 +748,24 @@ __global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *
     }
   }
 }
+template <typename FT, int NW, bool LAST>
+__global__ __launch_bounds__(256) void k_fwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
+                                                  const double *__restrict__ dinv, const double *__restrict__ tin,
+                                                  double *__restrict__ y, const int *__restrict__ done) {
+  extern __shared__ double smem[];
+  fwd_col3_body<FT, NW, LAST>(self_xcd_blk(), smem, C, sl, F, dinv, tin, y, done);
+}
 
 // Backward sweep L^T x = y (proddp, solscal-extended.f:4186-4197) on one
 // colour, in place: later colours are final; chains run from the base up.
 template <typename FT, int NW>
-__global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
+__device__ __forceinline__ void bwd_col3_body(Blk bk, double *smem, ColDesc C, Sell sl, const FT *__restrict__ F,
                                                   const double *__restrict__ dinv, const double *xin, double *x,
                                                   const int *__restrict__ done) {
   if (done && *done) return;
-  extern __shared__ double smem[];
-  const int tot = C.len * kChains;
+    const int tot = C.len * kChains;
   double *T = smem, *U = smem + tot;
-  const int t0 = xcd_group() * kChains;
+  const int t0 = bk.b * kChains;
   const int kl = C.len - 1;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / kChains, t = t0 + (i % kChains);
@@ -790,6 +825,13 @@ __global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *
       }
     }
   }
+}
+template <typename FT, int NW>
+__global__ __launch_bounds__(256) void k_bwd_col3(ColDesc C, Sell sl, const FT *__restrict__ F,
+                                                  const double *__restrict__ dinv, const double *xin, double *x,
+                                                  const int *__restrict__ done) {
+  extern __shared__ double smem[];
+  bwd_col3_body<FT, NW>(self_xcd_blk(), smem, C, sl, F, dinv, xin, x, done);
 }
 
 // Natural order: one row per thread, levels of the reference's
@@ -989,29 +1031,34 @@ __device__ __forceinline__ double sell_row_dot_sym(const Sell &sl, const FacTab 
 }
 
 // Partial sum of b_free^2 (XLUNG) and r = b - A x0.
-__global__ void k_residual0(int n, Sell sl, const double *__restrict__ A, const double *__restrict__ x,
+__device__ __forceinline__ void residual0_body(Blk bk, int n, Sell sl, const double *__restrict__ A, const double *__restrict__ x,
                             const double *__restrict__ b, const uint8_t *__restrict__ isdir,
                             double *__restrict__ r, double *__restrict__ part) {
   __shared__ double sh[kBlock / 64 + 1];
   double acc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (int i = bk.b * blockDim.x + threadIdx.x; i < n; i += bk.nb * blockDim.x) {
     const double bi = b[i];
     r[i] = bi - sell_row_dot(sl, A, x, i);
     if (!isdir[i]) acc += bi * bi;
   }
   double t = block_sum<kBlock>(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  if (threadIdx.x == 0) part[bk.b] = t;
+}
+__global__ void k_residual0(int n, Sell sl, const double *__restrict__ A, const double *__restrict__ x,
+                            const double *__restrict__ b, const uint8_t *__restrict__ isdir,
+                            double *__restrict__ r, double *__restrict__ part) {
+  residual0_body(self_blk(), n, sl, A, x, b, isdir, r, part);
 }
 
 // B = A p with partial (p,r) and (p,B); one thread per row.
-__global__ void k_spmv_dot2(int n, Sell sl, FacTab ft, const double *__restrict__ A, const double *__restrict__ p,
+__device__ __forceinline__ void spmv_dot2_body(Blk bk, int n, Sell sl, FacTab ft, const double *__restrict__ A, const double *__restrict__ p,
                             const double *__restrict__ r, double *__restrict__ B,
                             double *__restrict__ part_pr, double *__restrict__ part_pb,
                             const int *__restrict__ done) {
   if (*done) return;
   __shared__ double sh[2 * (kBlock / 64)];
   double apr = 0.0, apb = 0.0;
-  const int i = xcd_group() * blockDim.x + threadIdx.x;
+  const int i = bk.b * blockDim.x + threadIdx.x;
   if (i < n) {
     const double s = ft.w ? sell_row_dot_sym(sl, ft, A, p, i) : sell_row_dot(sl, A, p, i);
     B[i] = s;
@@ -1021,14 +1068,20 @@ __global__ void k_spmv_dot2(int n, Sell sl, FacTab ft, const double *__restrict_
   }
   const double2 t = block_sum2<kBlock>(apr, apb, sh);
   if (threadIdx.x == 0) {
-    part_pr[blockIdx.x] = t.x;
-    part_pb[blockIdx.x] = t.y;
+    part_pr[bk.b] = t.x;
+    part_pb[bk.b] = t.y;
   }
+}
+__global__ void k_spmv_dot2(int n, Sell sl, FacTab ft, const double *__restrict__ A, const double *__restrict__ p,
+                            const double *__restrict__ r, double *__restrict__ B,
+                            double *__restrict__ part_pr, double *__restrict__ part_pb,
+                            const int *__restrict__ done) {
+  spmv_dot2_body(self_xcd_blk(), n, sl, ft, A, p, r, B, part_pr, part_pb, done);
 }
 
 // alpha from the SpMV partials (every group sums them in the same fixed
 // order); r -= alpha B; x += alpha p; partial r_free^2.
-__global__ void k_pcg_alpha(int n, int nparts, const double *__restrict__ part_pr,
+__device__ __forceinline__ void pcg_alpha_body(Blk bk, int n, int nparts, const double *__restrict__ part_pr,
                             const double *__restrict__ part_pb, const double *__restrict__ B,
                             const double *__restrict__ p, const uint8_t *__restrict__ isdir,
                             double *__restrict__ r, double *__restrict__ x, double *__restrict__ part_rr,
@@ -1038,13 +1091,13 @@ __global__ void k_pcg_alpha(int n, int nparts, const double *__restrict__ part_p
   const double2 prb = reduce_partials2<kBlock>(part_pr, part_pb, nparts, sh);
   const double pr = prb.x, pb = prb.y;
   const double alfa = pr / pb;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bk.b == 0 && threadIdx.x == 0) {
     sc->pb = pb;
     sc->alpha = alfa;
   }
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int stride = gridDim.x * blockDim.x;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = bk.nb * blockDim.x;
+  int i = bk.b * blockDim.x + threadIdx.x;
   for (; i + 3 * stride < n; i += 4 * stride) {  // four rows in flight per thread
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -1062,28 +1115,39 @@ __global__ void k_pcg_alpha(int n, int nparts, const double *__restrict__ part_p
     if (!isdir[i]) acc[0] += ri * ri;
   }
   double t = block_sum<kBlock>((acc[0] + acc[1]) + (acc[2] + acc[3]), sh);
-  if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
+  if (threadIdx.x == 0) part_rr[bk.b] = t;
+}
+__global__ void k_pcg_alpha(int n, int nparts, const double *__restrict__ part_pr,
+                            const double *__restrict__ part_pb, const double *__restrict__ B,
+                            const double *__restrict__ p, const uint8_t *__restrict__ isdir,
+                            double *__restrict__ r, double *__restrict__ x, double *__restrict__ part_rr,
+                            PcgScal *sc, const int *__restrict__ done) {
+  pcg_alpha_body(self_blk(), n, nparts, part_pr, part_pb, B, p, isdir, r, x, part_rr, sc, done);
 }
 
 // partial (B, z)
-__global__ void k_dot(int n, const double *__restrict__ u, const double *__restrict__ v,
+__device__ __forceinline__ void dot_body(Blk bk, int n, const double *__restrict__ u, const double *__restrict__ v,
                       double *__restrict__ part, const int *__restrict__ done) {
   if (done && *done) return;
   __shared__ double sh[kBlock / 64 + 1];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int stride = gridDim.x * blockDim.x;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = bk.nb * blockDim.x;
+  int i = bk.b * blockDim.x + threadIdx.x;
   for (; i + 3 * stride < n; i += 4 * stride) {
 #pragma unroll
     for (int w = 0; w < 4; ++w) acc[w] += u[i + w * stride] * v[i + w * stride];
   }
   for (; i < n; i += stride) acc[0] += u[i] * v[i];
   double t = block_sum<kBlock>((acc[0] + acc[1]) + (acc[2] + acc[3]), sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  if (threadIdx.x == 0) part[bk.b] = t;
+}
+__global__ void k_dot(int n, const double *__restrict__ u, const double *__restrict__ v,
+                      double *__restrict__ part, const int *__restrict__ done) {
+  dot_body(self_blk(), n, u, v, part, done);
 }
 
 // beta, p = z + beta p, err and the stopping test (graddp.f:1348-1377).
-__global__ void k_pcg_beta(int n, int nparts, const double *__restrict__ part_bz,
+__device__ __forceinline__ void pcg_beta_body(Blk bk, int n, int nparts, const double *__restrict__ part_bz,
                            const double *__restrict__ part_rr, const double *__restrict__ z,
                            double *__restrict__ p, PcgScal *sc, int *done) {
   if (*done) return;
@@ -1091,14 +1155,14 @@ __global__ void k_pcg_beta(int n, int nparts, const double *__restrict__ part_bz
   const double2 crr = reduce_partials2<kBlock>(part_bz, part_rr, nparts, sh);
   const double c = crr.x, rr = crr.y;
   const double beta = -c / sc->pb;
-  const int stride = gridDim.x * blockDim.x;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = bk.nb * blockDim.x;
+  int i = bk.b * blockDim.x + threadIdx.x;
   for (; i + 3 * stride < n; i += 4 * stride) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) p[i + u * stride] = z[i + u * stride] + beta * p[i + u * stride];
   }
   for (; i < n; i += stride) p[i] = z[i] + beta * p[i];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bk.b == 0 && threadIdx.x == 0) {
     const double xl = sc->xlung;
     const double err = xl > 0.0 ? sqrt(rr / xl) : sqrt(rr / n);
     sc->err = err;
@@ -1109,9 +1173,14 @@ __global__ void k_pcg_beta(int n, int nparts, const double *__restrict__ part_bz
     }
   }
 }
+__global__ void k_pcg_beta(int n, int nparts, const double *__restrict__ part_bz,
+                           const double *__restrict__ part_rr, const double *__restrict__ z,
+                           double *__restrict__ p, PcgScal *sc, int *done) {
+  pcg_beta_body(self_blk(), n, nparts, part_bz, part_rr, z, p, sc, done);
+}
 
 // XLUNG from k_residual0's partials; reset the iteration state.
-__global__ void k_pcg_init(int nparts, const double *__restrict__ part, PcgScal *sc, int *done,
+__device__ __forceinline__ void pcg_init_body(Blk bk, int nparts, const double *__restrict__ part, PcgScal *sc, int *done,
                            double tol, int imax) {
   __shared__ double sh[kBlock / 64 + 2];
   const double xl = reduce_partials<kBlock>(part, nparts, sh);
@@ -1124,14 +1193,18 @@ __global__ void k_pcg_init(int nparts, const double *__restrict__ part, PcgScal 
     *done = 0;
   }
 }
+__global__ void k_pcg_init(int nparts, const double *__restrict__ part, PcgScal *sc, int *done,
+                           double tol, int imax) {
+  pcg_init_body(self_blk(), nparts, part, sc, done, tol, imax);
+}
 
 // ------------------------------------------------ Picard bookkeeping --
 // picard.f:185-200: PNEW += PDIFF, then SHLPIC (atmospheric Dirichlet
 // nodes keep POLD).
-__global__ void k_update_pnew(int n, const double *__restrict__ pdiff,
+__device__ __forceinline__ void update_pnew_body(Blk bk, int n, const double *__restrict__ pdiff,
                               const int *__restrict__ row_surf, const int *__restrict__ ifatm,
                               const double *__restrict__ pold, double *__restrict__ pnew) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = bk.b * blockDim.x + threadIdx.x;
   if (r >= n) return;
   double v = pnew[r] + pdiff[r];
   const int s = row_surf[r];
@@ -1141,14 +1214,19 @@ __global__ void k_update_pnew(int n, const double *__restrict__ pdiff,
   }
   pnew[r] = v;
 }
+__global__ void k_update_pnew(int n, const double *__restrict__ pdiff,
+                              const int *__restrict__ row_surf, const int *__restrict__ ifatm,
+                              const double *__restrict__ pold, double *__restrict__ pnew) {
+  update_pnew_body(self_blk(), n, pdiff, row_surf, ifatm, pold, pnew);
+}
 
 // bkpic.f:28-54 at atmospheric Dirichlet nodes: ATMACT = (A dpsi - XT5 -
 // (1-TETAF) ATMOLD) / TETAF with the true (restored) diagonal.
-__global__ void k_bkpic(int nnod, const int *__restrict__ surf_row, Sell sl, const double *__restrict__ A,
+__device__ __forceinline__ void bkpic_body(Blk bk, int nnod, const int *__restrict__ surf_row, Sell sl, const double *__restrict__ A,
                         const double *__restrict__ pdiff, const double *__restrict__ xt5,
                         const int *__restrict__ ifatm, const double *__restrict__ atmold, double tetaf,
                         double *__restrict__ atmact) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = bk.b * blockDim.x + threadIdx.x;
   if (s >= nnod) return;
   const int fa = ifatm[s];
   if (!(fa == 1 || fa == 2)) return;
@@ -1163,10 +1241,16 @@ __global__ void k_bkpic(int nnod, const int *__restrict__ surf_row, Sell sl, con
   v = v - xt5[r];
   atmact[s] = (v - (1.0 - tetaf) * atmold[s]) * (1.0 / tetaf);
 }
+__global__ void k_bkpic(int nnod, const int *__restrict__ surf_row, Sell sl, const double *__restrict__ A,
+                        const double *__restrict__ pdiff, const double *__restrict__ xt5,
+                        const int *__restrict__ ifatm, const double *__restrict__ atmold, double tetaf,
+                        double *__restrict__ atmact) {
+  bkpic_body(self_blk(), nnod, surf_row, sl, A, pdiff, xt5, ifatm, atmold, tetaf, atmact);
+}
 
 // norms.f (PINF with its last-index tie-break on CATHY ids, PL2, FINF,
 // FL2) and stormb.f's DSTORE, as per-block partials.
-__global__ void k_norms(int n, const int *__restrict__ refid, const double *__restrict__ pnew,
+__device__ __forceinline__ void norms_body(Blk bk, int n, const int *__restrict__ refid, const double *__restrict__ pnew,
                         const double *__restrict__ pold, const double *__restrict__ tnoti,
                         const double *__restrict__ swnew, const double *__restrict__ swtimep,
                         const double *__restrict__ ptimep, const double *__restrict__ volnod,
@@ -1177,7 +1261,7 @@ __global__ void k_norms(int n, const int *__restrict__ refid, const double *__re
   __shared__ int shi[kBlock / 64];
   double pl2 = 0.0, fl2 = 0.0, dst = 0.0, finf = 0.0, pinf = -1.0, pdif = 0.0;
   int pidx = -1;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (int i = bk.b * blockDim.x + threadIdx.x; i < n; i += bk.nb * blockDim.x) {
     const double diff = pnew[i] - pold[i];
     const double a = fabs(diff);
     pl2 += diff * diff;
@@ -1227,22 +1311,30 @@ __global__ void k_norms(int n, const int *__restrict__ refid, const double *__re
       }
       bf = fmax(bf, sh[k]);
     }
-    part.pl2[blockIdx.x] = t1;
-    part.fl2[blockIdx.x] = t2;
-    part.dstore[blockIdx.x] = t3;
-    part.finf[blockIdx.x] = bf;
-    part.pinf[blockIdx.x] = bv;
-    part.pidx[blockIdx.x] = bi;
-    part.pdif[blockIdx.x] = bd;
+    part.pl2[bk.b] = t1;
+    part.fl2[bk.b] = t2;
+    part.dstore[bk.b] = t3;
+    part.finf[bk.b] = bf;
+    part.pinf[bk.b] = bv;
+    part.pidx[bk.b] = bi;
+    part.pdif[bk.b] = bd;
   }
+}
+__global__ void k_norms(int n, const int *__restrict__ refid, const double *__restrict__ pnew,
+                        const double *__restrict__ pold, const double *__restrict__ tnoti,
+                        const double *__restrict__ swnew, const double *__restrict__ swtimep,
+                        const double *__restrict__ ptimep, const double *__restrict__ volnod,
+                        const double *__restrict__ snodi, const double *__restrict__ pnodi,
+                        NormPart part) {
+  norms_body(self_blk(), n, refid, pnew, pold, tnoti, swnew, swtimep, ptimep, volnod, snodi, pnodi, part);
 }
 
 // fluxmb.f: atmospheric and Neumann in/out sums on the surface / base.
-__global__ void k_flux(int nnod, const int *__restrict__ ifatm, const double *__restrict__ atmact,
+__device__ __forceinline__ void flux_body(Blk bk, int nnod, const int *__restrict__ ifatm, const double *__restrict__ atmact,
                        const double *__restrict__ qbot, int nq, double *__restrict__ part6) {
   __shared__ double sh[kBlock / 64 + 1];
   double v[6] = {0, 0, 0, 0, 0, 0};  // adin adout anin anout nnin nnout
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nnod; s += gridDim.x * blockDim.x) {
+  for (int s = bk.b * blockDim.x + threadIdx.x; s < nnod; s += bk.nb * blockDim.x) {
     const int fa = ifatm[s];
     const double a = atmact[s];
     if (fa == 1 || fa == 2) {
@@ -1257,14 +1349,18 @@ __global__ void k_flux(int nnod, const int *__restrict__ ifatm, const double *__
   }
   for (int k = 0; k < 6; ++k) {
     double t = block_sum<kBlock>(v[k], sh);
-    if (threadIdx.x == 0) part6[k * kMaxRedBlocks + blockIdx.x] = t;
+    if (threadIdx.x == 0) part6[k * kMaxRedBlocks + bk.b] = t;
   }
+}
+__global__ void k_flux(int nnod, const int *__restrict__ ifatm, const double *__restrict__ atmact,
+                       const double *__restrict__ qbot, int nq, double *__restrict__ part6) {
+  flux_body(self_blk(), nnod, ifatm, atmact, qbot, nq, part6);
 }
 
 // One work-group folds the partials of k_norms and k_flux into res[16]:
 // 0 DSTORE 1 PL2 2 FL2 3 FINF 4 PINF 5 PIKMAX 6 IKMAX 7 ADIN 8 ADOUT
 // 9 ANIN 10 ANOUT 11 NNIN 12 NNOUT
-__global__ void k_norms_final(int nb, int nbf, NormPart part, const double *__restrict__ part6,
+__device__ __forceinline__ void norms_final_body(Blk bk, int nb, int nbf, NormPart part, const double *__restrict__ part6,
                               double *__restrict__ res) {
   __shared__ double sh[kBlock / 64 + 2];
   __shared__ double shv[kBlock / 64], shd[kBlock / 64], shf[kBlock / 64];
@@ -1325,12 +1421,16 @@ __global__ void k_norms_final(int nb, int nbf, NormPart part, const double *__re
     for (int k = 0; k < 6; ++k) res[7 + k] = fx[k];
   }
 }
+__global__ void k_norms_final(int nb, int nbf, NormPart part, const double *__restrict__ part6,
+                              double *__restrict__ res) {
+  norms_final_body(self_blk(), nb, nbf, part, part6, res);
+}
 
 // switch_old.f:22-52 on the surface nodes.
-__global__ void k_switch_old(int nnod, const int *__restrict__ surf_row, int *__restrict__ ifatm,
+__device__ __forceinline__ void switch_old_body(Blk bk, int nnod, const int *__restrict__ surf_row, int *__restrict__ ifatm,
                              double *__restrict__ atmact, const double *__restrict__ atmpot,
                              double *__restrict__ pnew, double pmin) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = bk.b * blockDim.x + threadIdx.x;
   if (i >= nnod) return;
   int fa = ifatm[i];
   if (fa == -1) return;
@@ -1360,19 +1460,29 @@ __global__ void k_switch_old(int nnod, const int *__restrict__ surf_row, int *__
     pnew[r] = pmin;
   }
 }
+__global__ void k_switch_old(int nnod, const int *__restrict__ surf_row, int *__restrict__ ifatm,
+                             double *__restrict__ atmact, const double *__restrict__ atmpot,
+                             double *__restrict__ pnew, double pmin) {
+  switch_old_body(self_blk(), nnod, surf_row, ifatm, atmact, atmpot, pnew, pmin);
+}
 
 // flow3d.f:235-240 / cathy_main.f:2976-2978: POLD = PNEW, PTOLD = PTNEW,
 // PTNEW = TETAF*PNEW + (1-TETAF)*PTIMEP (weight.f).
-__global__ void k_next_iter(int n, double tetaf, const double *__restrict__ pnew,
+__device__ __forceinline__ void next_iter_body(Blk bk, int n, double tetaf, const double *__restrict__ pnew,
                             const double *__restrict__ ptimep, double *__restrict__ pold,
                             double *__restrict__ ptnew, double *__restrict__ ptold) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = bk.b * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double pn = pnew[i];
   pold[i] = pn;
   ptold[i] = ptnew[i];
   const double w = tetaf * pn + (1.0 - tetaf) * ptimep[i];
   ptnew[i] = w;
+}
+__global__ void k_next_iter(int n, double tetaf, const double *__restrict__ pnew,
+                            const double *__restrict__ ptimep, double *__restrict__ pold,
+                            double *__restrict__ ptnew, double *__restrict__ ptold) {
+  next_iter_body(self_blk(), n, tetaf, pnew, ptimep, pold, ptnew, ptold);
 }
 
 // Start of a step: POLD = PNEW, PTNEW = weight, PTOLD = PTNEW.
@@ -1416,20 +1526,27 @@ __global__ void k_free_drainage(int nnod, const int *__restrict__ bot_row,
 }
 
 // storcal.f partial: sum Sw * VOLNOD * PNODI.
-__global__ void k_store(int n, const double *__restrict__ sw, const double *__restrict__ volnod,
+__device__ __forceinline__ void store_body(Blk bk, int n, const double *__restrict__ sw, const double *__restrict__ volnod,
                         const double *__restrict__ pnodi, double *__restrict__ part) {
   __shared__ double sh[kBlock / 64 + 1];
   double acc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+  for (int i = bk.b * blockDim.x + threadIdx.x; i < n; i += bk.nb * blockDim.x)
     acc += sw[i] * volnod[i] * pnodi[i];
   double t = block_sum<kBlock>(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  if (threadIdx.x == 0) part[bk.b] = t;
+}
+__global__ void k_store(int n, const double *__restrict__ sw, const double *__restrict__ volnod,
+                        const double *__restrict__ pnodi, double *__restrict__ part) {
+  store_body(self_blk(), n, sw, volnod, pnodi, part);
 }
 
-__global__ void k_sum_final(int nb, const double *__restrict__ part, double *__restrict__ out) {
+__device__ __forceinline__ void sum_final_body(Blk bk, int nb, const double *__restrict__ part, double *__restrict__ out) {
   __shared__ double sh[kBlock / 64 + 2];
   const double v = reduce_partials<kBlock>(part, nb, sh);
   if (threadIdx.x == 0) *out = v;
+}
+__global__ void k_sum_final(int nb, const double *__restrict__ part, double *__restrict__ out) {
+  sum_final_body(self_blk(), nb, part, out);
 }
 
 // Set-up check of the coded column source: every slot of every row must
@@ -1557,5 +1674,156 @@ __global__ void k_vnod3d(int n, const int *__restrict__ inc_ptr, const int *__re
   vnod[r] = sv / tp;
   wnod[r] = sw / tp;
 }
+
+
+// ------------------------------------------- batched ensemble launches --
+// One grid over the members of an ensemble: each member's groups run the
+// single-member body on its own arrays (MemberDev), with the group numbers
+// of its own launch.
+#define CG_BATCH_HEAD                                      \
+  const int g_ = xcd_group();                              \
+  const int m_ = batch_member(start, nmem, g_);            \
+  const MemberDev &d = md[m_];                             \
+  const Blk bk{g_ - start[m_], start[m_ + 1] - start[m_]};
+
+__device__ __forceinline__ double *slot_part(const MemberDev &d, int k) { return d.part + (size_t)k * kMaxRedBlocks; }
+template <typename FT>
+__device__ __forceinline__ FT *fac_of(const MemberDev &d);
+template <>
+__device__ __forceinline__ float *fac_of<float>(const MemberDev &d) { return d.F32; }
+template <>
+__device__ __forceinline__ double *fac_of<double>(const MemberDev &d) { return d.F64; }
+__device__ __forceinline__ double *vec_of(const MemberDev &d, int v) {
+  return v == kVecTnoti ? d.tnoti : (v == kVecPdiff ? d.pdiff : (v == kVecR ? d.r : (v == kVecP ? d.p : d.z)));
+}
+
+__global__ void kb_constitutive(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  constitutive_body(bk, d.n, d.ptnew, d.pnew, d.soil, d.sw, d.ckrw, d.et1, d.et2, d.swnew);
+}
+__global__ void kb_bc_rows(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  bc_rows_body(bk, d.n, d.row_surf, d.qnode, d.ifatm, d.atmact, d.atmold, d.tetaf, d.isdir, d.bcflux);
+}
+template <int MR>
+__global__ __launch_bounds__(kBlock) void kb_assemble_plan(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  assemble_plan_body<MR>(bk, d.a, d.P, d.ckrw, d.et1);
+}
+template __global__ void kb_assemble_plan<16>(const int *, int, const MemberDev *);
+template __global__ void kb_assemble_plan<32>(const int *, int, const MemberDev *);
+
+template <typename FT, int NW>
+__global__ __launch_bounds__(256) void kb_ic_factor_col3t(const int *start, int nmem, const MemberDev *md, int c) {
+  extern __shared__ double smem[];
+  CG_BATCH_HEAD
+  ic_factor_col3t_body<FT, NW>(bk, smem, d.col[c], d.sl, d.ft, d.A, d.isdir, fac_of<FT>(d), d.dinv, d.flags + 1);
+}
+template <typename FT, int NW, bool LAST>
+__global__ __launch_bounds__(256) void kb_fwd_col3(const int *start, int nmem, const MemberDev *md, int c, int vin,
+                                                   int vout, int use_done) {
+  extern __shared__ double smem[];
+  CG_BATCH_HEAD
+  fwd_col3_body<FT, NW, LAST>(bk, smem, d.col[c], d.sl, fac_of<FT>(d), d.dinv, vec_of(d, vin), vec_of(d, vout),
+                              use_done ? d.flags : nullptr);
+}
+template <typename FT, int NW>
+__global__ __launch_bounds__(256) void kb_bwd_col3(const int *start, int nmem, const MemberDev *md, int c, int vio,
+                                                   int use_done) {
+  extern __shared__ double smem[];
+  CG_BATCH_HEAD
+  double *x = vec_of(d, vio);
+  bwd_col3_body<FT, NW>(bk, smem, d.col[c], d.sl, fac_of<FT>(d), d.dinv, x, x, use_done ? d.flags : nullptr);
+}
+#define CG_INST_BATCH_SWEEP(FT, U)                                                                         \
+  template __global__ void kb_fwd_col3<FT, U, false>(const int *, int, const MemberDev *, int, int, int, int); \
+  template __global__ void kb_fwd_col3<FT, U, true>(const int *, int, const MemberDev *, int, int, int, int);  \
+  template __global__ void kb_bwd_col3<FT, U>(const int *, int, const MemberDev *, int, int, int);
+#define CG_INST_BATCH_FT(FT)                                                                   \
+  template __global__ void kb_ic_factor_col3t<FT, 0>(const int *, int, const MemberDev *, int);  \
+  template __global__ void kb_ic_factor_col3t<FT, 4>(const int *, int, const MemberDev *, int);  \
+  template __global__ void kb_ic_factor_col3t<FT, 8>(const int *, int, const MemberDev *, int);  \
+  template __global__ void kb_ic_factor_col3t<FT, 12>(const int *, int, const MemberDev *, int); \
+  CG_INST_BATCH_SWEEP(FT, 0)                                                                   \
+  CG_INST_BATCH_SWEEP(FT, 4)                                                                   \
+  CG_INST_BATCH_SWEEP(FT, 8)                                                                   \
+  CG_INST_BATCH_SWEEP(FT, 12)                                                                  \
+  CG_INST_BATCH_SWEEP(FT, 16)
+CG_INST_BATCH_FT(float)
+CG_INST_BATCH_FT(double)
+#undef CG_INST_BATCH_FT
+#undef CG_INST_BATCH_SWEEP
+
+__global__ void kb_residual0(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  residual0_body(bk, d.n, d.sl, d.A, d.pdiff, d.tnoti, d.isdir, d.r, slot_part(d, 0));
+}
+__global__ void kb_pcg_init(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  pcg_init_body(bk, d.g_red, slot_part(d, 0), d.scal, d.flags, d.tol, d.imax);
+}
+__global__ void kb_spmv_dot2(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  spmv_dot2_body(bk, d.n, d.sl, d.fts, d.A, d.p, d.r, d.Bv, slot_part(d, 1), slot_part(d, 2), d.flags);
+}
+__global__ void kb_pcg_alpha(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  pcg_alpha_body(bk, d.n, d.g_rows, slot_part(d, 1), slot_part(d, 2), d.Bv, d.p, d.isdir, d.r, d.pdiff,
+                 slot_part(d, 3), d.scal, d.flags);
+}
+__global__ void kb_dot(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  dot_body(bk, d.n, d.Bv, d.z, slot_part(d, 4), d.flags);
+}
+__global__ void kb_pcg_beta(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  pcg_beta_body(bk, d.n, d.g_red, slot_part(d, 4), slot_part(d, 3), d.z, d.p, d.scal, d.flags);
+}
+// out[3m..3m+2] = done flag, iterations, last relative residual of member m
+__global__ void kb_poll(const int *start, int nmem, const MemberDev *md, double *out) {
+  CG_BATCH_HEAD
+  (void)bk;
+  if (threadIdx.x == 0) {
+    out[3 * m_] = (double)d.flags[0];
+    out[3 * m_ + 1] = (double)d.scal->niter;
+    out[3 * m_ + 2] = d.scal->err;
+  }
+}
+__global__ void kb_update_pnew(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  update_pnew_body(bk, d.n, d.pdiff, d.row_surf, d.ifatm, d.pold, d.pnew);
+}
+__global__ void kb_bkpic(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  bkpic_body(bk, d.nnod, d.surf_row, d.sl, d.A, d.pdiff, d.xt5, d.ifatm, d.atmold, d.tetaf, d.atmact);
+}
+__device__ __forceinline__ NormPart norm_part_of(const MemberDev &d) {
+  return NormPart{slot_part(d, 5), slot_part(d, 6), slot_part(d, 7), slot_part(d, 8), slot_part(d, 9),
+                  slot_part(d, 10), d.ipart};
+}
+__global__ void kb_norms(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  norms_body(bk, d.n, d.refid, d.pnew, d.pold, d.tnoti, d.swnew, d.swtimep, d.ptimep, d.volnod, d.snodi, d.pnodi,
+             norm_part_of(d));
+}
+__global__ void kb_flux(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  flux_body(bk, d.nnod, d.ifatm, d.atmact, d.qbot, d.nq, slot_part(d, 11));
+}
+__global__ void kb_norms_final(const int *start, int nmem, const MemberDev *md) {
+  CG_BATCH_HEAD
+  norms_final_body(bk, d.g_red, d.g_flux, norm_part_of(d), slot_part(d, 11), d.res);
+}
+__global__ void kb_switch_old(const int *start, int nmem, const MemberDev *md, const int *act) {
+  CG_BATCH_HEAD
+  if (!(act[m_] & 1)) return;
+  switch_old_body(bk, d.nnod, d.surf_row, d.ifatm, d.atmact, d.atmpot, d.pnew, d.pmin);
+}
+__global__ void kb_next_iter(const int *start, int nmem, const MemberDev *md, const int *act) {
+  CG_BATCH_HEAD
+  if (!(act[m_] & 2)) return;
+  next_iter_body(bk, d.n, d.tetaf, d.pnew, d.ptimep, d.pold, d.ptnew, d.ptold);
+}
+#undef CG_BATCH_HEAD
 
 }  // namespace cg

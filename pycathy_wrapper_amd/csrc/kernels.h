@@ -86,6 +86,31 @@ __device__ __forceinline__ int xcd_group() {
   return x * q + (x < rem ? x : rem) + k;
 }
 
+// Logical work-group of a launch.  A single-member kernel takes its own
+// group (b = blockIdx.x or xcd_group(), nb = gridDim.x); a batched kernel
+// runs every member of an ensemble in one grid and hands each member's
+// share of groups to the same body as groups 0..nb-1 of that member, so a
+// member's rows, partial sums and their order are those of its own launch.
+struct Blk {
+  int b, nb;
+};
+__device__ __forceinline__ Blk self_blk() { return Blk{(int)blockIdx.x, (int)gridDim.x}; }
+__device__ __forceinline__ Blk self_xcd_blk() { return Blk{xcd_group(), (int)gridDim.x}; }
+
+// Batched launches: member m owns the groups [start[m], start[m+1]) of the
+// grid (after the XCD remap, so each XCD serves contiguous runs of every
+// member); returns the member of logical group g (start[0] = 0, g <
+// start[nmem]; members without groups are skipped).
+__device__ __forceinline__ int batch_member(const int *__restrict__ start, int nmem, int g) {
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

@@ -148,6 +148,29 @@ module cathy_gpu
       integer(c_int), intent(in) :: celtype(*)
     end function
 
+    ! lockstep ensemble of flow handles (one launch per kernel for all members)
+    integer(c_int) function cathy_gpu_ensemble_create(ens, nmember, members) &
+        bind(C, name="cathy_gpu_ensemble_create")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), intent(out) :: ens
+      integer(c_int), intent(in) :: nmember
+      integer(c_int64_t), intent(in) :: members(*)
+    end function
+
+    integer(c_int) function cathy_gpu_ensemble_advance(ens, nsteps, rc, taken, stats) &
+        bind(C, name="cathy_gpu_ensemble_advance")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: ens
+      integer(c_int), intent(in) :: nsteps
+      integer(c_int), intent(out) :: rc(*), taken(*)
+      real(c_double), intent(out) :: stats(32, nsteps, *)
+    end function
+
+    integer(c_int) function cathy_gpu_ensemble_destroy(ens) bind(C, name="cathy_gpu_ensemble_destroy")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), intent(inout) :: ens
+    end function
+
     integer(c_int) function cathy_gpu_route_destroy(handle) bind(C, name="cathy_gpu_route_destroy")
       import :: c_int, c_int64_t
       integer(c_int64_t), intent(inout) :: handle

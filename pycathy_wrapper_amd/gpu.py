@@ -25,7 +25,13 @@ ORDER_COLUMN3 = 1
 
 STAT_NAMES = ["NSTEP", "DELTAT", "TIME", "ITER", "NITERT", "BACK", "STORE1", "STORE2", "DSTORE", "CDSTOR",
               "VIN", "CVIN", "VOUT", "CVOUT", "VINOUT", "VTOT", "ERRAS", "ERREL", "CERRAS", "CAERAS", "PINF",
-              "ITRTOT", "ITLIN", "KLSFAI", "PL2", "FINF", "FL2", "IKMAX"]
+              "ITRTOT", "ITLIN", "KLSFAI", "PL2", "FINF", "FL2", "IKMAX", "PICARD", "PCG"]
+# stats(32) slots of STAT_NAMES; slot 28 (the step's wall-clock ms) is kept apart
+_STAT_SLOTS = list(range(28)) + [29, 30]
+
+
+def _stat_dict(st) -> dict:
+    return {nm: float(st[i]) for nm, i in zip(STAT_NAMES, _STAT_SLOTS)}
 
 _lib = None
 
@@ -127,7 +133,8 @@ class FlowGPU:
     def advance(self):
         st = np.zeros(32)
         rc = _check(lib().cathy_gpu_flow3d_advance(C.byref(self.h), _p(st)), "flow3d_advance")
-        return rc, dict(zip(STAT_NAMES, st.tolist()))
+        self.last_wall_ms = float(st[28])
+        return rc, _stat_dict(st)
 
     def psi(self) -> np.ndarray:
         out = np.zeros(self.n)
@@ -173,6 +180,11 @@ class FlowGPU:
         _check(lib().cathy_gpu_flow3d_set_spmv_transposed(C.byref(self.h), _i(1 if on else 0), C.byref(act)),
                "set_spmv_transposed")
         return bool(act.value)
+
+    def pcg_batches(self, first_pct: int = 100, followup: int = 2) -> None:
+        """PCG launch batches between convergence polls (results unchanged)."""
+        _check(lib().cathy_gpu_flow3d_set_pcg_batches(C.byref(self.h), _i(first_pct), _i(followup)),
+               "set_pcg_batches")
 
     def guess_record(self, on: bool = True) -> None:
         _check(lib().cathy_gpu_flow3d_guess_record(C.byref(self.h), _i(1 if on else 0)), "guess_record")
@@ -245,6 +257,50 @@ class FlowGPU:
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             lib().cathy_gpu_flow3d_destroy(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class EnsembleGPU:
+    """Flow runs advanced in lockstep (cathy_gpu_ensemble_*): one launch per
+    kernel for all members, a convergence mask per member.  ``members``:
+    FlowGPU objects (column ordering, Picard) that the ensemble drives while
+    ``advance`` runs."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        h = np.array([m.h.value for m in self.members], dtype=np.int64)
+        e = C.c_int64(0)
+        _check(lib().cathy_gpu_ensemble_create(C.byref(e), _i(len(h)), _p(h)), "cathy_gpu_ensemble_create")
+        self.h = e
+
+    def advance(self, nsteps: int = 1):
+        """Every member takes ``nsteps`` accepted steps (or ends its run).
+        Returns (rc[m], taken[m], stats) with stats[m][s] the statistics dict
+        of member m's step s (s < taken[m]); ``self.wall_ms[m][s]`` is that
+        step's wall-clock ms, from its start to its end."""
+        nm = len(self.members)
+        rc = np.zeros(nm, dtype=np.int32)
+        taken = np.zeros(nm, dtype=np.int32)
+        st = np.zeros((nm, nsteps, 32))
+        _check(lib().cathy_gpu_ensemble_advance(C.byref(self.h), _i(nsteps), _p(rc), _p(taken), _p(st)),
+               "cathy_gpu_ensemble_advance")
+        stats = [[_stat_dict(st[m, s]) for s in range(int(taken[m]))] for m in range(nm)]
+        self.wall_ms = [st[m, :int(taken[m]), 28].copy() for m in range(nm)]
+        return rc, taken, stats
+
+    def info(self) -> dict:
+        out = np.zeros(4, dtype=np.int64)
+        _check(lib().cathy_gpu_ensemble_info(C.byref(self.h), _p(out)), "ensemble_info")
+        return dict(members=int(out[0]), rounds=int(out[1]), polls=int(out[2]), pcg_batches=int(out[3]))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().cathy_gpu_ensemble_destroy(C.byref(self.h))
 
     def __del__(self):
         try:

@@ -281,29 +281,49 @@ def test_spmv_transposed_matches_default(gpu):
     assert runs[0][1] == runs[1][1]
 
 
-@pytest.mark.parametrize("ordering", ["column3", "natural"])
-def test_flow_full_size_first_step_matches_oracle(gpu, oracle, ordering):
-    """The 1.25M-node workload (201x201 nodes, 30 layers, 7.2M tets): the
-    first time step against the oracle at full size."""
+FULL_STEPS = 2
+
+
+@pytest.fixture(scope="module")
+def full_size_oracle(oracle):
+    """The oracle's first FULL_STEPS steps of the 1.25M-node run (step 2 is
+    taken after three back-steps: dt 10 -> 1.25 s), shared by both
+    orderings."""
     pr = CF.hillslope_large(params=CF.hillslope_small_params(tmax=172800.0))
-    say(f"full size n={pr.mesh.n}: oracle step 1")
     run = _oracle_run(oracle, pr)
-    rc_o, so = run.step()
-    psi_o = run.psi()
-    run.close()
-    say(f"oracle: dt {so['DELTAT']} iter {so['ITER']} pcg {so['NITERT']}; {ordering} on the device")
+    out = []
+    try:
+        for k in range(FULL_STEPS):
+            say(f"full size n={pr.mesh.n}: oracle step {k + 1}")
+            rc_o, so = run.step()
+            out.append((rc_o, so, run.psi()))
+            say(f"oracle step {k + 1}: dt {so['DELTAT']} iter {so['ITER']} back {so['BACK']} pcg {so['NITERT']}")
+    finally:
+        run.close()
+    return pr, out
+
+
+@pytest.mark.parametrize("ordering", ["column3", "natural"])
+def test_flow_full_size_first_steps_match_oracle(gpu, full_size_oracle, ordering):
+    """The 1.25M-node workload (201x201 nodes, 30 layers, 7.2M tets): its
+    first two time steps against the oracle at full size, the back-stepped
+    second step included."""
+    pr, ref = full_size_oracle
     order = gpu.ORDER_NATURAL if ordering == "natural" else gpu.ORDER_COLUMN3
     g = gpu.FlowGPU(pr, order)
+    scale = 1e-6 * pr.mesh.volnod.sum()
     try:
-        rc_g, sg = g.advance()
-        say(f"device: dt {sg['DELTAT']} iter {sg['ITER']} pcg {sg['NITERT']}")
-        assert sg["DELTAT"] == so["DELTAT"] and sg["ITER"] == so["ITER"]
-        if ordering == "natural":
-            assert sg["NITERT"] == so["NITERT"]
-        assert _rel_inf(g.psi(), psi_o) <= PSI_TOL
-        scale = 1e-6 * pr.mesh.volnod.sum()
-        for key in ("DSTORE", "VIN", "VOUT", "ERRAS"):
-            assert _mb_close(sg[key], so[key], scale), (key, sg[key], so[key])
+        for k, (rc_o, so, psi_o) in enumerate(ref):
+            rc_g, sg = g.advance()
+            say(f"{ordering} step {k + 1}: dt {sg['DELTAT']} iter {sg['ITER']} back {sg['BACK']} pcg {sg['NITERT']}")
+            assert rc_g == rc_o
+            assert sg["DELTAT"] == so["DELTAT"] and sg["ITER"] == so["ITER"] and sg["BACK"] == so["BACK"]
+            if ordering == "natural":
+                assert sg["NITERT"] == so["NITERT"]
+            assert _rel_inf(g.psi(), psi_o) <= PSI_TOL
+            for key in ("DSTORE", "VIN", "VOUT", "ERRAS"):
+                assert _mb_close(sg[key], so[key], scale), (k, key, sg[key], so[key])
+        assert sum(so["BACK"] for _, so, _ in ref) >= 1
     finally:
         g.close()
 
