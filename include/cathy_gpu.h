@@ -222,6 +222,16 @@ int cathy_gpu_flow3d_guess_sample(int64_t *handle, float *feat,
  * from an accepted step (cathy_gpu_flow3d_adopt_state). */
 int cathy_gpu_enkf_update(const int *nens, int64_t *handles, const double *w,
                           const double *inflation);
+/* The same analysis without allocating device memory.  No counterpart in
+ * the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  W and
+ * the member state pointer tables are staged in one fixed device array per
+ * GPU (kScratchBytes = 33,792 bytes: 64 x 64 FP64 weights + 2 x 64
+ * pointers), held until the process ends; NENS above 64 is refused.  Calls
+ * that overlap on one GPU share that array and race: serialise them.
+ * Results are those of cathy_gpu_enkf_update (the same kernel). */
+int cathy_gpu_enkf_update_noalloc(const int *nens, int64_t *handles,
+                                  const double *w, const double *inflation);
 /* Continue a run from its current PNEW (after set_pnew or an analysis):
  * PTIMEP = PNEW, Sw and kr at it, storage baseline reset. */
 int cathy_gpu_flow3d_adopt_state(int64_t *handle);

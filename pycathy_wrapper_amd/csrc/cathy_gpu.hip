@@ -32,6 +32,68 @@ std::shared_ptr<Flow> lookup(const int64_t *h) {
 
 void set_last_error(const std::string &m) { g_last_error = m; }
 
+// Device staging of the allocation-free EnKF analysis: W (nens x nens FP64)
+// then the members' read and write state pointers, for nens <= kEnkfMax.
+// One array per GPU, held until the process ends; every call on a GPU uses
+// its GPU's copy, so calls that overlap on one GPU race.
+constexpr size_t kScratchW = sizeof(double) * kEnkfMax * kEnkfMax;
+constexpr size_t kScratchTab = sizeof(double *) * kEnkfMax;
+constexpr size_t kScratchBytes = kScratchW + 2 * kScratchTab;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the current GPU, looked up once per GPU
+// with that GPU current (the host-side name is only a stand-in).
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch[kMaxDevices] = {};
+static unsigned char *scratch_ptr() {
+  int dev = -1;
+  CG_HIP(hipGetDevice(&dev));
+  CG_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    CG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    CG_REQUIRE(ptr != nullptr, "no device address for the analysis staging array");
+    g_scratch[dev] = static_cast<unsigned char *>(ptr);
+  }
+  return g_scratch[dev];
+}
+
+// X_a = X + X' W on the members' PNEW in place (then inflation about the
+// analysis mean), with W and the state pointer tables staged in device
+// memory the caller provides (dw: m*m doubles, dx / dxa: m pointers each);
+// every member then continues from its analysis state.
+static void enkf_apply(int m, int64_t *handles, const double *w, double inflation, double *dw,
+                       const double **dx, double **dxa) {
+  std::array<std::shared_ptr<Flow>, kEnkfMax> fl;
+  for (int j = 0; j < m; ++j) fl[j] = lookup(&handles[j]);
+  Flow *f0 = fl[0].get();
+  for (int j = 0; j < m; ++j) {
+    CG_REQUIRE(fl[j]->initialised, "flow3d_init not called on every member");
+    for (int k = 0; k < j; ++k) CG_REQUIRE(fl[k] != fl[j], "a member appears twice");
+    CG_REQUIRE(fl[j]->n == f0->n && fl[j]->perm == f0->perm,
+               "members differ in mesh or node ordering; their states do not align");
+  }
+  for (int j = 0; j < m * m; ++j) CG_REQUIRE(std::isfinite(w[j]), "non-finite analysis weight");
+  // every member's queued work ends before its state is read
+  for (int j = 0; j < m; ++j) CG_HIP(hipStreamSynchronize(fl[j]->st));
+  std::array<const double *, kEnkfMax> hx;
+  std::array<double *, kEnkfMax> hxa;
+  for (int j = 0; j < m; ++j) {
+    hx[j] = fl[j]->pnew.p;
+    hxa[j] = fl[j]->pnew.p;  // in place: each thread reads its row of every member before writing
+  }
+  hipStream_t s = f0->st;
+  CG_HIP(hipMemcpyAsync(dx, hx.data(), sizeof(double *) * m, hipMemcpyHostToDevice, s));
+  CG_HIP(hipMemcpyAsync(dxa, hxa.data(), sizeof(double *) * m, hipMemcpyHostToDevice, s));
+  CG_HIP(hipMemcpyAsync(dw, w, sizeof(double) * m * m, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_enkf_update, dim3(grid_for(f0->n)), dim3(kBlock), 0, s, f0->n, m, dx, dw, inflation, dxa);
+  CG_HIP(hipGetLastError());
+  CG_HIP(hipStreamSynchronize(s));
+  for (int j = 0; j < m; ++j) fl[j]->adopt_state();
+}
+
 }  // namespace cg
 
 using namespace cg;
@@ -209,36 +271,25 @@ int cathy_gpu_enkf_update(const int *nens, int64_t *handles, const double *w, co
     CG_REQUIRE(nens && handles && w && inflation, "null argument");
     const int m = *nens;
     CG_REQUIRE(m >= 2 && m <= kEnkfMax, "ensemble size out of range (2..64)");
-    std::vector<std::shared_ptr<Flow>> fl(m);
-    for (int j = 0; j < m; ++j) fl[j] = lookup(&handles[j]);
-    Flow *f0 = fl[0].get();
-    for (int j = 0; j < m; ++j) {
-      CG_REQUIRE(fl[j]->initialised, "flow3d_init not called on every member");
-      for (int k = 0; k < j; ++k) CG_REQUIRE(fl[k] != fl[j], "a member appears twice");
-      CG_REQUIRE(fl[j]->n == f0->n && fl[j]->perm == f0->perm,
-                 "members differ in mesh or node ordering; their states do not align");
-    }
-    for (int j = 0; j < m * m; ++j) CG_REQUIRE(std::isfinite(w[j]), "non-finite analysis weight");
-    // every member's queued work ends before its state is read
-    for (int j = 0; j < m; ++j) CG_HIP(hipStreamSynchronize(fl[j]->st));
-    std::vector<const double *> hx(m);
-    std::vector<double *> hxa(m);
-    for (int j = 0; j < m; ++j) {
-      hx[j] = fl[j]->pnew.p;
-      hxa[j] = fl[j]->pnew.p;  // in place: each thread reads its row of every member before writing
-    }
-    hipStream_t s = f0->st;
     DBuf<const double *> dx(m);
     DBuf<double *> dxa(m);
     DBuf<double> dw((size_t)m * m);
-    dx.upload(hx.data(), m, s);
-    dxa.upload(hxa.data(), m, s);
-    dw.upload(w, (size_t)m * m, s);
-    hipLaunchKernelGGL(k_enkf_update, dim3(grid_for(f0->n)), dim3(kBlock), 0, s, f0->n, m, dx.p, dw.p, *inflation,
-                       dxa.p);
-    CG_HIP(hipGetLastError());
-    CG_HIP(hipStreamSynchronize(s));
-    for (int j = 0; j < m; ++j) fl[j]->adopt_state();
+    enkf_apply(m, handles, w, *inflation, dw.p, dx.p, dxa.p);
+    return 0;
+  });
+}
+
+int cathy_gpu_enkf_update_noalloc(const int *nens, int64_t *handles, const double *w, const double *inflation) {
+  return guarded([&] {
+    CG_REQUIRE(nens && handles && w && inflation, "null argument");
+    const int m = *nens;
+    CG_REQUIRE(m >= 2 && m <= kEnkfMax,
+               "ensemble size out of range (2..64): the allocation-free analysis stages at most 64 members");
+    unsigned char *base = scratch_ptr();
+    double *dw = reinterpret_cast<double *>(base);
+    const double **dx = reinterpret_cast<const double **>(base + kScratchW);
+    double **dxa = reinterpret_cast<double **>(base + kScratchW + kScratchTab);
+    enkf_apply(m, handles, w, *inflation, dw, dx, dxa);
     return 0;
   });
 }

@@ -66,14 +66,18 @@ def observe(flows, nodes, kind: str = "psi", porosity=None) -> np.ndarray:
     return np.stack(cols, axis=1)
 
 
-def enkf_update(flows, W, inflation: float = 1.0) -> None:
+def enkf_update(flows, W, inflation: float = 1.0, noalloc: bool = False) -> None:
     """X_a = X + X' W on the members' PNEW in place (then inflated about the
-    analysis mean); every run continues from its analysis state."""
+    analysis mean); every run continues from its analysis state.
+    ``noalloc``: the entry that stages W in the library's fixed device array
+    instead of allocating (at most 64 members; calls on one GPU must not
+    overlap)."""
     N = len(flows)
     W = np.asarray(W, dtype=np.float64)
     if W.shape != (N, N):
         raise ValueError("W must be N x N")
     h = np.array([f.h.value for f in flows], dtype=np.int64)
     w = np.ascontiguousarray(W.T)  # W(k, j) at w[j*N + k]
-    G._check(G.lib().cathy_gpu_enkf_update(G._i(N), G._p(h), G._p(w), C.byref(C.c_double(inflation))),
-             "enkf_update")
+    entry = G.lib().cathy_gpu_enkf_update_noalloc if noalloc else G.lib().cathy_gpu_enkf_update
+    G._check(entry(G._i(N), G._p(h), G._p(w), C.byref(C.c_double(inflation))),
+             "enkf_update_noalloc" if noalloc else "enkf_update")
