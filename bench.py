@@ -116,7 +116,7 @@ def ensemble_run(members, steps: int) -> dict:
     return dict(wall=wall, members=res)
 
 
-def lockstep_run(members, steps: int) -> dict:
+def lockstep_run(members, steps: int, graphs: bool | None = None) -> dict:
     """Steps 1..K of every member in lockstep (cathy_gpu_ensemble_*): each
     round every member still iterating takes one Picard iteration, one
     launch per kernel for all of them; members leave the lockstep when they
@@ -125,6 +125,10 @@ def lockstep_run(members, steps: int) -> dict:
     for pr, g in members:
         g.init(pr.psi0)
     e = gpu.EnsembleGPU([g for _, g in members])
+    if graphs is None:  # the autotuner's choice for the members' size
+        from pycathy_wrapper_amd import autotune
+        graphs = bool(autotune.entry_for(members[0][0].mesh.n)["graphs"])
+    e.graphs(graphs)
     t0 = time.perf_counter()
     rc, taken, stats = e.advance(steps)
     wall = time.perf_counter() - t0

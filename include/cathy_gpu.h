@@ -375,9 +375,15 @@ int cathy_gpu_ensemble_create(int64_t *ens, const int *nmember,
  * statistics as cathy_gpu_flow3d_advance returns them. */
 int cathy_gpu_ensemble_advance(int64_t *ens, const int *nsteps, int *rc,
                                int *taken, double *stats);
-/* out4: members, Picard rounds, PCG polls and PCG launch batches of the
- * last advance. */
-int cathy_gpu_ensemble_info(int64_t *ens, int64_t *out4);
+/* HIP graphs: with ON != 0 each round's fixed launch sequences (the
+ * Picard iteration up to the PCG loop, a follow-up batch of PCG
+ * iterations, the post-solve balance) are captured once and replayed until
+ * the set of members still iterating changes.  Results are the same bits;
+ * default 0. */
+int cathy_gpu_ensemble_set_graphs(int64_t *ens, const int *on);
+/* out5: members, Picard rounds, PCG polls, PCG launch batches and graph
+ * captures of the last advance. */
+int cathy_gpu_ensemble_info(int64_t *ens, int64_t *out5);
 int cathy_gpu_ensemble_destroy(int64_t *ens);
 
 /* ------------------------------------------------------------------ *

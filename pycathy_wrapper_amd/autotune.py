@@ -12,7 +12,8 @@ size and the ensemble is tuned here, by timing steps on the device:
 * the PCG launch batches between convergence polls (first batch as a
   percentage of the previous solve's iterations, follow-up batch size;
   cathy_gpu_flow3d_set_pcg_batches);
-* the transposed-read SpMV (cathy_gpu_flow3d_set_spmv_transposed).
+* the transposed-read SpMV (cathy_gpu_flow3d_set_spmv_transposed);
+* HIP graphs for the lockstep rounds (cathy_gpu_ensemble_set_graphs).
 
 ``python -m pycathy_wrapper_amd.autotune`` runs the sweep on the GPU and
 writes the table pycathy_wrapper_amd/tuning.json, one entry per mesh size
@@ -32,7 +33,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-DEFAULT_ENTRY = dict(members_per_group=1, pcg_first_pct=100, pcg_followup=2, spmv_transposed=0)
+DEFAULT_ENTRY = dict(members_per_group=1, pcg_first_pct=100, pcg_followup=2, spmv_transposed=0, graphs=0)
 
 
 def load(path: str = TABLE) -> dict:
@@ -78,13 +79,15 @@ def apply(flow, entry: dict) -> None:
 
 
 # ------------------------------------------------------------------ sweep --
-def _timed(problems, flows, grp: list[list[int]], steps: int) -> float:
+def _timed(problems, flows, grp: list[list[int]], steps: int, graphs: bool = False) -> float:
     """member-steps/s of ``steps`` steps of every member from its initial
     state, lockstep groups ``grp`` concurrently (one host thread each)."""
     from . import gpu
     for pr, f in zip(problems, flows):
         f.init(pr.psi0)
     ens = [gpu.EnsembleGPU([flows[j] for j in g]) for g in grp]
+    for e in ens:
+        e.graphs(graphs)
     done = [0] * len(ens)
 
     def run(i):
@@ -113,15 +116,15 @@ def sweep_class(size: str, count: int, steps: int, mpg_candidates, log=print) ->
     nodes = problems[0].mesh.n
     trials = []
 
-    def trial(mpg, first, follow, transposed):
+    def trial(mpg, first, follow, transposed, graphs=0):
         for f in flows:
             apply(f, dict(pcg_first_pct=first, pcg_followup=follow, spmv_transposed=transposed))
         grp = [list(range(i, min(i + mpg, count))) for i in range(0, count, mpg)]
-        rate = _timed(problems, flows, grp, steps)
+        rate = _timed(problems, flows, grp, steps, bool(graphs))
         trials.append(dict(members_per_group=mpg, pcg_first_pct=first, pcg_followup=follow,
-                           spmv_transposed=transposed, member_steps_per_s=rate))
+                           spmv_transposed=transposed, graphs=graphs, member_steps_per_s=rate))
         log(f"[autotune] {size} ({nodes} nodes, {count} members): {mpg} per group, PCG batches {first}% + "
-            f"{follow}, transposed SpMV {transposed}: {rate:.2f} member-steps/s")
+            f"{follow}, transposed SpMV {transposed}, graphs {graphs}: {rate:.2f} member-steps/s")
         return rate
 
     _timed(problems, flows, [[j] for j in range(count)], 1)  # warm-up
@@ -142,10 +145,12 @@ def sweep_class(size: str, count: int, steps: int, mpg_candidates, log=print) ->
              t["pcg_followup"] == best["pcg_followup"] and t["pcg_first_pct"] == best["pcg_first_pct"] and
              not t["spmv_transposed"])
     best["spmv_transposed"] = 1 if r1 > r0 else 0
+    rg = trial(mpg, best["pcg_first_pct"], best["pcg_followup"], best["spmv_transposed"], 1)
+    best["graphs"] = 1 if rg > max(r0, r1) else 0
     for f in flows:
         f.close()
     return dict(size=size, nodes=nodes, layers=problems[0].mesh.nstr, members=count, steps=steps,
-                member_steps_per_s=max(r0, r1), **best, trials=trials)
+                member_steps_per_s=max(r0, r1, rg), **best, trials=trials)
 
 
 def main(argv=None) -> int:
@@ -168,7 +173,8 @@ def main(argv=None) -> int:
         json.dump(table, f, indent=1)
     log(f"[autotune] table written to {args.out}")
     print(json.dumps({c["size"]: {k: c[k] for k in ("nodes", "members_per_group", "pcg_first_pct", "pcg_followup",
-                                                      "spmv_transposed", "member_steps_per_s")} for c in classes}))
+                                                      "spmv_transposed", "graphs", "member_steps_per_s")}
+                      for c in classes}))
     return 0
 
 

@@ -16,9 +16,12 @@
 // next step), so members of uneven size and uneven iteration counts share
 // the launches without waiting for each other's steps.
 //
-// The work of one round that does not depend on host decisions -- the
-// Picard iteration up to the PCG polls, the PCG iterations, and the
-// post-solve balance -- can be captured as HIP graphs and replayed.
+// The work of one round between host decisions -- the Picard iteration up
+// to the PCG loop, a batch of PCG iterations, the post-solve balance -- is a
+// fixed launch sequence whose per-round values live in device tables, so it
+// can be captured as HIP graphs once and replayed every round
+// (cathy_gpu_ensemble_set_graphs) until the set of members taking part
+// changes.
 #include "flow.h"
 
 namespace cg {
@@ -82,10 +85,58 @@ struct Ensemble {
   Pinned<int> hact;
   std::vector<char> active, tab_active;
   // statistics of the last advance
-  long rounds = 0, polls = 0, pcg_batches = 0;
+  long rounds = 0, polls = 0, pcg_batches = 0, graph_captures = 0;
+  // HIP graphs of a round's fixed launch sequences: pre-solve (Picard
+  // assembly, factor, PCG start), one PCG batch of `g_follow` iterations,
+  // post-solve (update, MASBAL / NORMS).  Their launches read every
+  // per-round value from the device tables, so they stay valid until the
+  // set of members taking part changes (the group counts of the grids).
+  bool use_graphs = false;
+  hipGraphExec_t g_pre = nullptr, g_iter = nullptr, g_post = nullptr;
+  int g_follow = 0;
 
   ~Ensemble() {
+    drop_graphs();
     if (st) (void)hipStreamDestroy(st);
+  }
+  void drop_graphs() {
+    for (hipGraphExec_t *g : {&g_pre, &g_iter, &g_post})
+      if (*g) {
+        (void)hipGraphExecDestroy(*g);
+        *g = nullptr;
+      }
+  }
+  // capture what `body` launches on the stream as an executable graph (no
+  // memset or copy nodes: only kernel launches)
+  template <typename F>
+  hipGraphExec_t capture(F &&body) {
+    hipGraph_t graph = nullptr;
+    CG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      body();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    CG_HIP(hipStreamEndCapture(st, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    CG_HIP(e);
+    ++graph_captures;
+    return exec;
+  }
+  // run `body`'s launches directly, or through the graph *g (captured on
+  // first use)
+  template <typename F>
+  void run(hipGraphExec_t *g, F &&body) {
+    if (!use_graphs) {
+      body();
+      return;
+    }
+    if (!*g) *g = capture(body);
+    CG_HIP(hipGraphLaunch(*g, st));
   }
 
   const int *tab(int sh) const { return dtab.p + (size_t)sh * (nm + 1); }
@@ -146,6 +197,7 @@ struct Ensemble {
     }
     CG_HIP(hipMemcpyAsync(dtab.p, htab.p, sizeof(int) * kNShape * (nm + 1), hipMemcpyHostToDevice, st));
     tab_active = active;
+    drop_graphs();  // the grids change with the members taking part
   }
 
   MemberDev member_dev(int j) const {
@@ -249,12 +301,37 @@ struct Ensemble {
 
   // SYMSLV of every active member on its current A / TNOTI (Flow::pcg):
   // the same launches and batch policy, one launch per kernel for all.
-  void pcg(std::vector<int> &niter) {
+  // one PCG iteration for every member still iterating (graddp.f:1325-1377)
+  void pcg_iteration() {
+    launch(kb_spmv_dot2, kShRows, 0);
+    launch(kb_pcg_alpha, kShRed, 0);
+    precond(kVecR, kVecZ, 1);
+    launch(kb_dot, kShRed, 0);
+    launch(kb_pcg_beta, kShRed, 0);
+  }
+  // the Picard iteration up to the PCG loop: CHPIC0, BCPIC rows, the fused
+  // assembly (picard.f:74-165), IC(0), x0 = M^-1 b, r, p = M^-1 r
+  void pre_solve() {
+    launch(kb_constitutive, kShRows, 0);
+    launch(kb_bc_rows, kShRows, 0);
+    launch(kb_assemble_plan<16>, kShRows, 0);
     factor();
-    precond(kVecTnoti, kVecPdiff, 0);  // x0 = M^-1 b
+    precond(kVecTnoti, kVecPdiff, 0);
     launch(kb_residual0, kShRed, 0);
     launch(kb_pcg_init, kShOne, 0);
-    precond(kVecR, kVecP, 0);  // p = M^-1 r
+    precond(kVecR, kVecP, 0);
+  }
+  // PNEW update, then MASBAL + NORMS of every member into one array
+  void post_solve() {
+    launch(kb_update_pnew, kShRows, 0);
+    launch(kb_bkpic, kShNnod, 0);
+    launch(kb_norms, kShRed, 0);
+    launch(kb_flux, kShFlux, 0);
+    launch(kb_norms_final, kShOne, 0);
+  }
+
+  void pcg(std::vector<int> &niter) {
+    run(&g_pre, [&] { pre_solve(); });
     int batch = 1, imax = 1, followup = 1;
     for (int j = 0; j < nm; ++j)
       if (active[j]) {
@@ -263,14 +340,23 @@ struct Ensemble {
         imax = std::max(imax, f.prm.itmxcg);
         followup = std::max(followup, f.pcg_followup);
       }
+    if (use_graphs && g_iter && g_follow != followup) {
+      (void)hipGraphExecDestroy(g_iter);
+      g_iter = nullptr;
+    }
+    g_follow = followup;
     int launched = 0;
     while (true) {
-      for (int b = 0; b < batch; ++b) {
-        launch(kb_spmv_dot2, kShRows, 0);
-        launch(kb_pcg_alpha, kShRed, 0);
-        precond(kVecR, kVecZ, 1);
-        launch(kb_dot, kShRed, 0);
-        launch(kb_pcg_beta, kShRed, 0);
+      if (use_graphs) {
+        // whole follow-up batches (iterations past convergence exit at once)
+        const int reps = (batch + followup - 1) / followup;
+        for (int k = 0; k < reps; ++k)
+          run(&g_iter, [&] {
+            for (int b = 0; b < followup; ++b) pcg_iteration();
+          });
+        batch = reps * followup;
+      } else {
+        for (int b = 0; b < batch; ++b) pcg_iteration();
       }
       launched += batch;
       ++pcg_batches;
@@ -286,7 +372,7 @@ struct Ensemble {
           least = std::min(least, (int)hpoll.p[3 * j + 1]);
         }
       if (all) break;
-      CG_REQUIRE(launched <= imax + 2, "PCG did not stop");
+      CG_REQUIRE(launched <= imax + followup + 2, "PCG did not stop");
       batch = std::max(1, std::min(followup, imax - least + 1));
     }
     for (int j = 0; j < nm; ++j) niter[j] = active[j] ? (int)hpoll.p[3 * j + 1] : 0;
@@ -316,7 +402,7 @@ struct Ensemble {
         for (int j = 0; j < e->nm; ++j) e->mem[j]->st = own[j];
       }
     } restore{this, own};
-    rounds = polls = pcg_batches = 0;
+    rounds = polls = pcg_batches = graph_captures = 0;
     std::vector<int> niter(nm, 0), cont(nm, 0);
     for (int j = 0; j < nm; ++j) {
       rc[j] = 0;
@@ -337,19 +423,12 @@ struct Ensemble {
       for (int j = 0; j < nm; ++j)
         if (active[j]) hargs.p[j] = member_dev(j);
       CG_HIP(hipMemcpyAsync(dargs.p, hargs.p, sizeof(MemberDev) * nm, hipMemcpyHostToDevice, st));
-      // PICARD (picard.f:74-165): CHPIC0, BCPIC rows, the fused assembly
-      launch(kb_constitutive, kShRows, 0);
-      launch(kb_bc_rows, kShRows, 0);
-      launch(kb_assemble_plan<16>, kShRows, 0);
+      // PICARD (picard.f:74-200) for every member, one read-back of all
+      // their MASBAL / NORMS values
       pcg(niter);
       for (int j = 0; j < nm; ++j)
         if (active[j]) mem[j]->after_solve(niter[j]);
-      launch(kb_update_pnew, kShRows, 0);
-      // MASBAL + NORMS, one read-back for every member
-      launch(kb_bkpic, kShNnod, 0);
-      launch(kb_norms, kShRed, 0);
-      launch(kb_flux, kShFlux, 0);
-      launch(kb_norms_final, kShOne, 0);
+      run(&g_post, [&] { post_solve(); });
       CG_HIP(hipMemcpyAsync(hres.p, dres.p, sizeof(double) * 16 * nm, hipMemcpyDeviceToHost, st));
       CG_HIP(hipStreamSynchronize(st));
       for (int j = 0; j < nm; ++j) {
@@ -426,14 +505,26 @@ int cathy_gpu_ensemble_advance(int64_t *ens, const int *nsteps, int *rc, int *ta
   });
 }
 
-int cathy_gpu_ensemble_info(int64_t *ens, int64_t *out4) {
+int cathy_gpu_ensemble_set_graphs(int64_t *ens, const int *on) {
   return guarded([&] {
-    CG_REQUIRE(out4, "null argument");
+    CG_REQUIRE(on, "null argument");
     const std::shared_ptr<Ensemble> e = ens_lookup(ens);
-    out4[0] = e->nm;
-    out4[1] = e->rounds;
-    out4[2] = e->polls;
-    out4[3] = e->pcg_batches;
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->use_graphs = *on != 0;
+    e->drop_graphs();
+    return 0;
+  });
+}
+
+int cathy_gpu_ensemble_info(int64_t *ens, int64_t *out5) {
+  return guarded([&] {
+    CG_REQUIRE(out5, "null argument");
+    const std::shared_ptr<Ensemble> e = ens_lookup(ens);
+    out5[0] = e->nm;
+    out5[1] = e->rounds;
+    out5[2] = e->polls;
+    out5[3] = e->pcg_batches;
+    out5[4] = e->graph_captures;
     return 0;
   });
 }

@@ -166,6 +166,12 @@ module cathy_gpu
       real(c_double), intent(out) :: stats(32, nsteps, *)
     end function
 
+    integer(c_int) function cathy_gpu_ensemble_set_graphs(ens, on) bind(C, name="cathy_gpu_ensemble_set_graphs")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), intent(in) :: ens
+      integer(c_int), intent(in) :: on
+    end function
+
     integer(c_int) function cathy_gpu_ensemble_destroy(ens) bind(C, name="cathy_gpu_ensemble_destroy")
       import :: c_int, c_int64_t
       integer(c_int64_t), intent(inout) :: ens

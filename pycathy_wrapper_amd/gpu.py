@@ -293,10 +293,16 @@ class EnsembleGPU:
         self.wall_ms = [st[m, :int(taken[m]), 28].copy() for m in range(nm)]
         return rc, taken, stats
 
+    def graphs(self, on: bool = True) -> None:
+        """Capture each round's fixed launch sequences as HIP graphs and
+        replay them (the same results)."""
+        _check(lib().cathy_gpu_ensemble_set_graphs(C.byref(self.h), _i(1 if on else 0)), "ensemble_set_graphs")
+
     def info(self) -> dict:
-        out = np.zeros(4, dtype=np.int64)
+        out = np.zeros(5, dtype=np.int64)
         _check(lib().cathy_gpu_ensemble_info(C.byref(self.h), _p(out)), "ensemble_info")
-        return dict(members=int(out[0]), rounds=int(out[1]), polls=int(out[2]), pcg_batches=int(out[3]))
+        return dict(members=int(out[0]), rounds=int(out[1]), polls=int(out[2]), pcg_batches=int(out[3]),
+                    graph_captures=int(out[4]))
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:

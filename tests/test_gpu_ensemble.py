@@ -176,3 +176,35 @@ def test_stress_dry_start_matches_oracle(oracle, iopt):
         g.close()
         run.close()
     say(f"IOPT={iopt}: largest |ERREL| {worst:.4g}% over 12 steps")
+
+
+def test_lockstep_graphs_match_direct_launches():
+    """The lockstep rounds replayed from HIP graphs (each round's fixed
+    launch sequences captured once) against the same rounds launched
+    directly: every step's statistics and the final psi of every member
+    bit for bit.  The graphs are replayed hundreds of times with the
+    members' own per-step work and synchronisations between replays, and
+    captured again when a member leaves (its run ends at TMAX = 60 s)."""
+    from pycathy_wrapper_amd import gpu as G
+    probs = [_member("S", 120), _member("S", 121), _member("S", 122), _member("M", 140),
+             CF.hillslope_small_member(1, params=CF.hillslope_small_params(tmax=60.0))]
+    out = []
+    for graphs in (False, True):
+        flows = [G.FlowGPU(p, G.ORDER_COLUMN3) for p in probs]
+        ens = G.EnsembleGPU(flows)
+        ens.graphs(graphs)
+        try:
+            rc, taken, stats = ens.advance(6)
+            info = ens.info()
+            say(f"graphs {graphs}: rc {rc.tolist()} taken {taken.tolist()} {info}")
+            out.append((rc.tolist(), taken.tolist(), stats, [f.psi() for f in flows], info))
+        finally:
+            ens.close()
+            for f in flows:
+                f.close()
+    (rc0, tk0, st0, ps0, i0), (rc1, tk1, st1, ps1, i1) = out
+    assert i0["graph_captures"] == 0 and i1["graph_captures"] >= 6 and i1["rounds"] > 20
+    assert rc0 == rc1 and tk0 == tk1 and rc0[4] == 1
+    for j in range(len(probs)):
+        assert st0[j] == st1[j], j
+        assert np.array_equal(ps0[j], ps1[j]), j
