@@ -83,9 +83,11 @@ def ensemble_run(members, steps: int) -> dict:
     import threading
     res = [None] * len(members)
 
+    for pr, g in members:  # every member from its initial state, set before the clock starts
+        g.init(pr.psi0)
+
     def run(j):
         pr, g = members[j]
-        g.init(pr.psi0)
         n = pic = pc = bk = 0
         sim = mbe = 0.0
         lat = []
@@ -116,11 +118,9 @@ def ensemble_run(members, steps: int) -> dict:
     return dict(wall=wall, members=res)
 
 
-def lockstep_run(members, steps: int, graphs: bool | None = None) -> dict:
-    """Steps 1..K of every member in lockstep (cathy_gpu_ensemble_*): each
-    round every member still iterating takes one Picard iteration, one
-    launch per kernel for all of them; members leave the lockstep when they
-    have taken K steps."""
+def lockstep_prepare(members, graphs: bool | None = None):
+    """An ensemble handle over the members, each reset to its initial state
+    (set-up outside any timed region)."""
     from pycathy_wrapper_amd import gpu
     for pr, g in members:
         g.init(pr.psi0)
@@ -129,6 +129,16 @@ def lockstep_run(members, steps: int, graphs: bool | None = None) -> dict:
         from pycathy_wrapper_amd import autotune
         graphs = bool(autotune.entry_for(members[0][0].mesh.n)["graphs"])
     e.graphs(graphs)
+    return e
+
+
+def lockstep_run(members, steps: int, graphs: bool | None = None, e=None) -> dict:
+    """Steps 1..K of every member in lockstep (cathy_gpu_ensemble_*): each
+    round every member still iterating takes one Picard iteration, one
+    launch per kernel for all of them; members leave the lockstep when they
+    have taken K steps."""
+    if e is None:
+        e = lockstep_prepare(members, graphs)
     t0 = time.perf_counter()
     rc, taken, stats = e.advance(steps)
     wall = time.perf_counter() - t0
@@ -174,9 +184,36 @@ def grouped_run(members, steps: int, group_rows: int) -> dict:
     import threading
     groups = make_groups(members, group_rows)
     out = [None] * len(groups)
+    # a group of one member is that member's own run on its own stream
+    ens = [lockstep_prepare([members[j] for j in g]) if len(g) > 1 else None for g in groups]
+    for g, e in zip(groups, ens):
+        if e is None:
+            pr, fl = members[g[0]]
+            fl.init(pr.psi0)
+
+    def solo(j):
+        pr, g = members[j]
+        n = pic = pc = bk = 0
+        sim = mbe = 0.0
+        lat = []
+        for _ in range(steps):
+            rc, s = g.advance()
+            lat.append(g.last_wall_ms / 1e3)
+            n += 1
+            pic += s["PICARD"]
+            pc += s["PCG"]
+            bk += s["BACK"]
+            sim = s["TIME"]
+            mbe = max(mbe, abs(s["ERREL"]))
+            if rc:
+                break
+        return dict(steps=n, picard=pic, pcg=pc, back_steps=bk, sim_s=sim, max_rel_mbe_pct=mbe, step_s=lat)
 
     def run(i):
-        out[i] = lockstep_run([members[j] for j in groups[i]], steps)
+        if ens[i] is None:
+            out[i] = dict(members=[solo(groups[i][0])], info=dict(rounds=0))
+        else:
+            out[i] = lockstep_run([members[j] for j in groups[i]], steps, e=ens[i])
 
     threads = [threading.Thread(target=run, args=(i,)) for i in range(len(groups))]
     t0 = time.perf_counter()
@@ -193,6 +230,77 @@ def grouped_run(members, steps: int, group_rows: int) -> dict:
             res[j] = r
     return dict(wall=wall, members=res, groups=[len(g) for g in groups],
                 rounds=sum(o["info"]["rounds"] for o in out))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def reference_baseline(args, members: int, max_steps: int, seconds: float) -> dict | None:
+    """The reference's own FLOW3D (Fortran, flang -O2, sized by pyCATHY's
+    update_cathyH rules) on the host, one process per member as pyCATHY's
+    DA forecast runs ./cathy (oracle/refbench.py): first member 0 alone
+    (the baseline value), then min(members, cores - 5) members at once, the
+    forecast pool of pyCATHY/DA/cathy_DA.py:83,1389."""
+    import subprocess
+    script = os.path.join(ROOT, "oracle", "refbench.py")
+    base = [sys.executable, "-u", script, "--cells", str(args.cells), "--nstr", str(args.nstr), "--steps",
+            str(max_steps), "--seconds", str(seconds)]
+
+    def run(seeds):
+        # every process builds its member first and reports ready; the timed
+        # steps of all start together
+        procs = [subprocess.Popen(base + ["--seed", str(s), "--wait"], stdin=subprocess.PIPE,
+                                  stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True) for s in seeds]
+        for p in procs:
+            p.stdout.readline()
+        for p in procs:
+            p.stdin.write("go\n")
+            p.stdin.flush()
+        out = []
+        for p in procs:
+            o, _ = p.communicate(timeout=900)
+            line = [l for l in o.splitlines() if l.startswith("{")]
+            out.append(json.loads(line[-1]) if line else dict(error=f"no output (rc {p.returncode})"))
+        return out
+
+    log("[cpu] the reference FLOW3D, member 0 alone (set-up untimed)")
+    one = run([2])[0]
+    if "error" in one:
+        log(f"[cpu] reference: {one['error']}")
+        return None
+    log(f"[cpu] reference: {one['steps']} step(s) in {one['seconds']:.2f} s ({one['picard']} Picard, "
+        f"{one['pcg']} PCG)")
+    share = min(16, os.cpu_count() or 1)  # this process's share of the host
+    nproc = max(1, min(members, share - 5))
+    log(f"[cpu] the reference, {nproc} member processes at once")
+    pool = run([2 + i for i in range(nproc)])
+    ok = [r for r in pool if "error" not in r]
+    pool_rate = None
+    if ok:
+        span = max(r["t_end"] for r in ok) - min(r["t_start"] for r in ok)
+        pool_rate = sum(r["steps"] for r in ok) / span
+        log(f"[cpu] reference pool: {sum(r['steps'] for r in ok)} member-steps in {span:.2f} s -> "
+            f"{pool_rate:.3f} member-steps/s")
+    return dict(value=one["steps"] / one["seconds"], unit="member-steps/s", cores=1, kind="reference",
+                steps=one["steps"], seconds=one["seconds"], cpu_model=cpu_model(),
+                sample=f"the reference's own FLOW3D and subsurface time loop (Fortran sources built with flang -O2 "
+                       f"-fno-automatic against a CATHY.H sized by pyCATHY's update_cathyH rules, "
+                       f"oracle/_ref/large; natural-order FP64 IC(0)-PCG): steps 1..{one['steps']} of member 0 "
+                       f"in {one['seconds']:.2f} s on one host core ({one['picard']} Picard, {one['pcg']} PCG); "
+                       f"set-up untimed",
+                process_pool=dict(processes=nproc, member_steps_per_s=pool_rate,
+                                  members=[dict((k, r.get(k)) for k in ("seed", "steps", "seconds", "picard", "pcg"))
+                                           for r in pool],
+                                  note="one process per member, min(members, cores - 5) at a time, as "
+                                       "pyCATHY's DA forecast pool"))
 
 
 def cpu_baseline(problem, max_steps: int, seconds: float) -> dict:
@@ -681,11 +789,16 @@ def main() -> int:
         learned = learned_leg(g, pr, args, dts, steps)
     cpu = None
     if args.cpu_steps > 0 and args.cpu_seconds > 0:
-        cpu = cpu_baseline(pr, args.cpu_steps, args.cpu_seconds)
+        port = cpu_baseline(pr, args.cpu_steps, args.cpu_seconds)
+        cpu = reference_baseline(args, args.members, args.cpu_steps, args.cpu_seconds)
+        if cpu is None:  # no reference build for this size: the port is the baseline
+            cpu = port
+        else:
+            cpu["port"] = {k: port[k] for k in ("value", "steps", "seconds", "sample")}
         k = min(cpu["steps"], len(step_wall))
         gsec = sum(step_wall[:k])
         cpu["device_seconds_same_steps"] = gsec
-        log(f"[cpu] steps 1..{k}: oracle {cpu['seconds']:.2f} s, device {gsec:.3f} s")
+        log(f"[cpu] steps 1..{k}: {cpu['kind']} {cpu['seconds']:.2f} s, device {gsec:.3f} s")
     traffic = read_traffic()
     out = {
         "metric": METRIC,

@@ -21,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libcathy_ref.so")
+REF_LARGE_SO = os.path.join(HERE, "_ref", "large", "libcathy_ref_large.so")
 
 VG_FIELDS = ["PNOT", "N", "M", "RMC", "PSAT", "N1", "M52", "NR", "MM1", "MR", "PSN"]
 VG_NFIELD = len(VG_FIELDS)
@@ -258,21 +259,41 @@ class RefLib:
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing: run make -C oracle ref")
         self.L = C.CDLL(path)
-        self.vg = np.ctypeslib.as_array((C.c_double * (VG_NFIELD * REF_NMAX)).in_dll(self.L, "vg_"))
+        # array bounds of the build: the weil CATHY.H, or the sized header's
+        # sizes.json beside the library (oracle/mkcathyh.py)
+        sz = os.path.join(os.path.dirname(os.path.abspath(path)), "sizes.json")
+        if os.path.exists(sz):
+            import json
+            with open(sz) as f:
+                d = json.load(f)
+            self.nmax, self.maxstr, self.maxzon, self.maxtrm = d["nmax"], d["maxstr"], d["maxzon"], d["maxtrm"]
+        else:
+            self.nmax, self.maxstr, self.maxzon, self.maxtrm = REF_NMAX, REF_MAXSTR, 1, REF_MAXTRM
+        self.vg = np.ctypeslib.as_array((C.c_double * (VG_NFIELD * self.nmax)).in_dll(self.L, "vg_"))
         # /CHORD/PMIN,... : PMIN is the first REAL*8 of the block
         self.chord = np.ctypeslib.as_array((C.c_double * 1).in_dll(self.L, "chord_"))
 
     def set_vg(self, vg_orc: np.ndarray, n: int):
         """Copy an oracle-layout vg (VG_NFIELD*n) into the NMAX-strided /VG/."""
-        assert n <= REF_NMAX
+        assert n <= self.nmax
         for f in range(VG_NFIELD):
-            self.vg[f * REF_NMAX: f * REF_NMAX + n] = vg_orc[f * n:(f + 1) * n]
+            self.vg[f * self.nmax: f * self.nmax + n] = vg_orc[f * n:(f + 1) * n]
 
     def get_vg(self, n: int) -> np.ndarray:
         out = np.zeros(VG_NFIELD * n)
         for f in range(VG_NFIELD):
-            out[f * n:(f + 1) * n] = self.vg[f * REF_NMAX: f * REF_NMAX + n]
+            out[f * n:(f + 1) * n] = self.vg[f * self.nmax: f * self.nmax + n]
         return out
+
+    def zone_table(self, tab) -> np.ndarray:
+        """A (layer, zone) soil table as the reference's (MAXSTR, MAXZON)
+        column-major array (PERMX, POROS, VGNCELL, ...)."""
+        t = np.asarray(tab, dtype=np.float64)
+        nstr, nz = t.shape
+        assert nstr <= self.maxstr and nz <= self.maxzon, (t.shape, self.maxstr, self.maxzon)
+        out = np.zeros((max(nz, 1), self.maxstr))
+        out[:nz, :nstr] = t.T
+        return out.reshape(-1)
 
     def __getattr__(self, name):
         return getattr(self.L, name.lower() + "_")

@@ -1,4 +1,5 @@
-"""Time loop around the reference's own FLOW3D (TEST INFRASTRUCTURE ONLY).
+"""Time loop around the reference's own FLOW3D (TEST INFRASTRUCTURE ONLY:
+the checker of tests/ and the CPU baseline of bench.py).
 
 The CATHY main program cannot be linked in this image (it needs LAPACK for
 the transport code), but FLOW3D and everything it calls are in
@@ -18,8 +19,10 @@ Atmospheric forcing comes one of two ways:
   one record).
 
 The mesh is given to the reference as the host builder makes it; the reference
-itself is sized by its CATHY.H (NMAX=7056, MAXSTR=15, MAXZON=1,
-MAXTRM=52111, NTRMAX=800), so runs here are config-1 sized.
+itself is sized by its CATHY.H: the weil build (oracle/_ref/libcathy_ref.so:
+NMAX=7056, MAXSTR=15, MAXZON=1, MAXTRM=52111, NTRMAX=800) takes config-1
+sized runs, the sized build (oracle/_ref/large, oracle/mkcathyh.py) the
+201x201-node, 30-layer, 40,000-zone workload and anything smaller.
 """
 from __future__ import annotations
 
@@ -28,7 +31,7 @@ import os
 
 import numpy as np
 
-from pyoracle import REF_MAXSTR, REF_NMAX, RefLib
+from pyoracle import RefLib
 
 MB_FIELDS = ("NUDIN NUDOUT NUDINP NUDOUTP VNUDIN VNUDOUT VNUDTOT OVFLOW REFLOW SFFLW SFFLWP VSFFLW APOT AACT AACTP "
              "AACTAV ADIN ADOUT ADINP ADOUTP NDIN NDOUT NDINP NDOUTP ANIN ANOUT ANINP ANOUTP NNIN NNOUT NNINP NNOUTP "
@@ -76,8 +79,8 @@ class RefFlowRun:
     """Reference-FLOW3D driven run.  Parameters follow orc_params."""
 
     def __init__(self, ref: RefLib, mesh, soil, psi0, rain, prm: dict, atmbc: str | None = None):
-        assert soil["nzone"] == 1, "reference CATHY.H has MAXZON=1"
-        assert mesh.nstr <= REF_MAXSTR and mesh.n <= REF_NMAX
+        assert soil["nzone"] <= ref.maxzon, "more zones than the reference build's MAXZON"
+        assert mesh.nstr <= ref.maxstr and mesh.n <= ref.nmax, "mesh larger than the reference build"
         self.ref = ref
         self.p = dict(prm)
         n, nt, nnod = mesh.n, mesh.nt, mesh.nnod
@@ -96,8 +99,9 @@ class RefFlowRun:
                     tj[:, l, k] = M.csr_lookup(topol, ja, mesh.tetra[:, k], mesh.tetra[:, l], n) + 1
         else:           # STRPIC + TETPIC (upper pairs)
             topol, ja = M.sym_pattern(mesh.tetra, n)
+            tsym = M.tetja_sym(mesh.tetra, topol, ja, n)
             for idx, (k, l) in enumerate(M.UPPER_PAIRS):
-                tj[:, l, k] = M.tetja_sym(mesh.tetra, topol, ja, n)[:, idx] + 1
+                tj[:, l, k] = tsym[:, idx] + 1
         nterm = len(ja)
         a = {}
         I = lambda v: np.array([v], dtype=np.int32)
@@ -139,11 +143,8 @@ class RefFlowRun:
                          RMAX=1.7e100, RMIN=1.7e-100, TOLUNS=prm["toluns"], TOLSWI=prm["tolswi"],
                          TOLCG=prm["tolcg"], ERNLMX=prm["ernlmx"], PONDH_MIN=prm["pondh_min"]).items():
             a[k][0] = v
-        perm = np.zeros(REF_MAXSTR)
         for key, src in (("PERMX", "permx"), ("PERMY", "permy"), ("PERMZ", "permz")):
-            perm = np.zeros(REF_MAXSTR)
-            perm[:mesh.nstr] = soil[src][:, 0]
-            a[key] = perm.copy()
+            a[key] = ref.zone_table(soil[src])     # PERMX(MAXSTR, MAXZON)
         a["BI"] = np.ascontiguousarray(mesh.bi).reshape(-1).copy()
         a["CI"] = np.ascontiguousarray(mesh.ci).reshape(-1).copy()
         a["DI"] = np.ascontiguousarray(mesh.di).reshape(-1).copy()
@@ -174,11 +175,9 @@ class RefFlowRun:
         dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
         ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
         big = len(a["PNODI"])
-        tabs = {}
-        for k in ("poros", "elstor", "vgn", "vgrmc", "vgpsat"):
-            t = np.zeros(REF_MAXSTR); t[:mesh.nstr] = soil[k][:, 0]; tabs[k] = t
+        tabs = {k: self.ref.zone_table(soil[k]) for k in ("poros", "elstor", "vgn", "vgrmc", "vgpsat")}
         zeros = {k: np.zeros(big) for k in ("PORE", "INDE", "DEF", "SEL", "LEL", "KEL", "LAMBDANODI", "KDNODI")}
-        lam = np.zeros(REF_MAXSTR); kd = np.zeros(REF_MAXSTR)
+        lam = np.zeros(len(tabs["poros"])); kd = np.zeros(len(tabs["poros"]))
         self.ref.vg[:] = 0.0
         a["TP"][:] = 0
         L.tpnodi_(C.byref(C.c_int(n)), C.byref(C.c_int(nt)), C.byref(C.c_int(mesh.ntri)), ip(a["TETRA"]),
@@ -188,9 +187,9 @@ class RefFlowRun:
                   dp(tabs["vgrmc"]), dp(tabs["vgpsat"]), dp(zeros["LEL"]), dp(lam), dp(zeros["LAMBDANODI"]),
                   dp(zeros["KEL"]), dp(kd), dp(zeros["KDNODI"]))
         L.chparm_(C.byref(C.c_int(n)), dp(a["SNODI"]), dp(a["PNODI"]), C.byref(C.c_int(0)))
-        kz = np.zeros(nt)
         lay = mesh.layer_of_element()
-        kz[:] = soil["permz"][lay, 0]
+        zone = np.asarray(mesh.tetra)[:, 4] - 1   # TETRA(5, e): the element's zone, 1-based
+        kz = np.ascontiguousarray(np.asarray(soil["permz"])[lay, zone], dtype=np.float64)
         self.kznod = np.zeros(big)
         L.eltnod_(C.byref(C.c_int(n)), C.byref(C.c_int(nt)), ip(a["TP"]), ip(a["TETRA"]), dp(kz), dp(self.kznod))
 

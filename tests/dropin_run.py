@@ -1,7 +1,7 @@
 """Child-process driver for test_gpu_dropin.py (TEST INFRASTRUCTURE).
 
 Runs the hillslope_small day through the reference's own FLOW3D and time
-loop (tests/refrun.py) from oracle/_ref/libcathy_ref_gpu.so, where SYMSLV is
+loop (oracle/refrun.py) from oracle/_ref/libcathy_ref_gpu.so, where SYMSLV is
 the Fortran shim of pycathy_wrapper_amd/fortran calling cathy_gpu_symslv.
 Writes per-step statistics and psi / Sw at the golden fixture's saved steps
 to the .npz named by argv[1].  Runs in its own process because a library

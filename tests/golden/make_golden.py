@@ -11,7 +11,7 @@ container, where the reference tree is mounted at /root/reference).
 2. hillslope_small_ref.npz -- the hillslope_small run (21x21 nodes, 10
    layers, 1 day) stepped by the reference's own FLOW3D (compiled from the
    reference sources by oracle/Makefile into oracle/_ref/libcathy_ref.so and
-   driven by tests/refrun.py): per-step statistics of every step and psi/Sw
+   driven by oracle/refrun.py): per-step statistics of every step and psi/Sw
    at selected steps.
 4. weil_outputs.npz -- text of the weil run's output files as the reference
    wrote them (examples/SSHydro/weill_exemple/output/): the first two output

@@ -158,7 +158,7 @@ def test_nsyslv_bit_identical(oracle, ref, problem, system, dirichlet):
 
 def test_newton_whole_run_bit_identical(oracle, ref, problem):
     """40 time steps of the oracle's Newton time loop against the reference
-    FLOW3D in Newton mode (tests/refrun.py): identical dt, Newton and
+    FLOW3D in Newton mode (oracle/refrun.py): identical dt, Newton and
     BiCGSTAB counts, back-steps, psi and Sw.  The reference's own DSTORE is
     not compared: its Newton branch never sets FLOW3D's SWNEW / SWTIMEP
     (flow3d.f:83), so its mass balance reads stale values (NaN here)."""

@@ -155,7 +155,7 @@ def test_symslv_bit_identical(oracle, ref, problem, system, dirichlet):
 
 def test_whole_run_bit_identical(oracle, ref, problem):
     """The oracle's time loop vs the reference's FLOW3D driven through the
-    same loop (tests/refrun.py): 40 steps, identical trajectories."""
+    same loop (oracle/refrun.py): 40 steps, identical trajectories."""
     import pyoracle as P
     import refrun as R
     m = problem.mesh
@@ -197,7 +197,7 @@ def _atm_series(nnod, hspatm, seed=3):
 def test_whole_run_atmbc_file_bit_identical(oracle, ref, problem, tmp_path, hspatm, ieto):
     """ATMONE / ATMNXT / ATMBAK and BKSTEP pinned: the reference's own
     readers take the forcing from an atmbc file, and back-steps run the
-    reference's BKSTEP (tests/refrun.py, oracle/refatm_io.f); the oracle
+    reference's BKSTEP (oracle/refrun.py, oracle/refatm_io.f); the oracle
     takes the same records as arrays.  Spatially variable
     piecewise-linear records (HSPATM = 0, IETO = 0) and homogeneous step
     records (HSPATM = 1, IETO = 1), with rain, evaporation, infiltration
