@@ -276,6 +276,66 @@ int cathy_gpu_flow3d_set_spmv_transposed(int64_t *handle, const int *on,
 int cathy_gpu_flow3d_set_pcg_batches(int64_t *handle, const int *first_pct,
                                      const int *followup);
 
+/* Non-atmospheric, non-seepage-face boundary series; the run starts over
+ * at the next init.
+ * KIND 0: fixed-head (Dirichlet) nodes with their heads PRESC (m), the
+ * nansfdirbc series; KIND 1: Neumann nodes with their fluxes Q (m^3/s,
+ * > 0 into the domain), the nansfneubc series.  Record k (NREC of them,
+ * TIMES not decreasing, the first at 0) lists COUNTS(k) 1-based NODES and
+ * their VALUES, expanded as RDNDBC/READBC expand a record (surface nodes
+ * replicated down the layers); for KIND 1 under free drainage only the
+ * entries after the NNOD base nodes.  Replaces, per time step:
+ *   CALL BCONE / BCNXT / BCBAK('...',IIN8|IIN9,...,NP|NQ,...,PTIM|QTIM,
+ *        PINP|QINP,CONTP|CONTQ,PRESC|Q,ANP|ANQ,ACONTP|ACONTQ)
+ *     (bcone.f:1-4, bcnxt.f:5-8, bcbak.f:4-5; cathy_main.f:2892-2903,
+ *      bkstep.f:73-88) and NEUMANN's listed part (neumann.f:25-31),
+ * and feeds BCPIC (bcpic.f:33-39 fixed heads, :71-74 Neumann fluxes),
+ * SHLPIC (shlpic.f:30-35), BKPIC (bkpic.f:38-41) and FLUXMB's NDIN/NDOUT
+ * and NNIN/NNOUT (fluxmb.f:30-46); listed surface nodes leave the
+ * atmospheric boundary (IFATM = -1, atmnxt.f:80-102).  A node may appear
+ * once per fixed-head record.  Picard scheme (IOPT = 1) only. */
+int cathy_gpu_flow3d_set_bc(int64_t *handle, const int *kind, const int *nrec,
+                            const double *times, const int *counts,
+                            const int *nodes, const double *values);
+
+/* Seepage faces (the sfbc file, time-invariant): NSF faces, face i holding
+ * COUNTS(i) 1-based NODES listed top to bottom; ISFCVG = 1 adds the exit
+ * points' convergence to the Picard test (inital.f:218-224, SFCHEK).  A
+ * node with head >= 1e-8 m at the start is an exit node held at 0
+ * (sfinit.f:22-45); exit nodes are Dirichlet rows of BCPIC
+ * (bcpic.f:41-53) reset to 0 by SHLPIC (shlpic.f:36-45), their flux SFQ is
+ * back-calculated by BKPIC (bkpic.f:42-49) and leaves through VSFFLW
+ * (fluxmb.f:53-62, masbal.f:69-73); after every Picard iteration EXTALL /
+ * EXTCVG (extall.f:22-47, extcvg.f:20-26) move the exit points, and
+ * back-steps restore them (bkstep.f:56-67).  Replaces, with DUPUIT = 0 and
+ * ISFONE = 0, the seepage-face arguments NSF, NSFNUM, NSFNOD, SFEX, SFEXIT,
+ * SFEXP, SFQ, SFQP, LHSSF, SFCHEK, KSFZER of FLOW3D (flow3d.f:8-35).  The
+ * run starts over at the next init; Picard scheme only. */
+int cathy_gpu_flow3d_set_seepage(int64_t *handle, const int *nsf,
+                                 const int *counts, const int *nodes,
+                                 const int *isfcvg);
+
+/* The seepage faces' SFEX (1 = exit node) and SFQ (m^3/s) by node
+ * position (face by face, top to bottom); null outputs are skipped.
+ * Returns the number of seepage nodes. */
+int cathy_gpu_flow3d_get_seepage(int64_t *handle, int *sfex, double *sfq);
+
+/* Root water uptake (QTRANIE) of the Feddes model, recomputed at the start
+ * of every time step.  VEG_TYPE(NNOD) 1-based types of the surface nodes;
+ * per type PCANA, PCREF, PCWLT, ZROOT (> 0), PZ, OMGC (the soil file's
+ * vegetation lines, datin.f:425-428); Z(N) node elevations.  Every root
+ * zone must end above its column's base.  Replaces
+ *   CALL ETRAN(N,NNOD,NSTR,ATMPOT,Z,PNEW,PNODI,VEG_TYPE,QTRANIE)
+ *     (etran.f:1; cathy_main.f:2929)
+ * whose QTRANIE BCPIC subtracts from every row (bcpic.f:84-86).  Without
+ * it QTRANIE is zero, which is what ETRAN gives for ATMPOT >= 0.  The run
+ * starts over at the next init; Picard scheme only. */
+int cathy_gpu_flow3d_set_vegetation(int64_t *handle, const int *nveg,
+                                    const int *veg_type, const double *pcana,
+                                    const double *pcref, const double *pcwlt,
+                                    const double *zroot, const double *pz,
+                                    const double *omgc, const double *z);
+
 /* Counters of the last advance: Picard its, PCG its, IC(0) breakdown
  * fallbacks, kernel time of the linear solver (ms), total device ms. */
 int cathy_gpu_flow3d_counters(int64_t *handle, double *out6);

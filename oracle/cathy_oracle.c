@@ -912,6 +912,59 @@ int orc_norms(int n, double *pikmax, double *pinf, double *pl2, double *finf,
   return ikmax;
 }
 
+int orc_etran(int n, int nnod, int nstr, const double *atmpot,
+              const double *z, const double *psi, const int *vegtype,
+              const double *pcana, const double *pcref, const double *pcwlt,
+              const double *zroot, const double *pz, const double *omgc,
+              double scf, double *qtranie) {
+  /* SRC/etran.f:15-80.  Returns 0, or -1 if a root zone reaches the base
+   * of its column (the reference would read past the column there). */
+  double *btrani = (double *)calloc((size_t)n, sizeof(double));
+  double *btran = (double *)calloc((size_t)nnod, sizeof(double));
+  double *etp = (double *)calloc((size_t)nnod, sizeof(double));
+  double *omg = (double *)calloc((size_t)nnod, sizeof(double));
+  int rc = 0;
+  for (int k = 0; k < n; ++k) qtranie[k] = 0.0;
+  for (int i = 0; i < nnod && rc == 0; ++i) {
+    int v = vegtype[i];
+    int j = 1;
+    double zsurf = z[i], depth = 0.0;
+    btran[i] = 0.0;
+    omg[i] = 0.0;
+    etp[i] = (atmpot[i] < 0.0) ? -1.0 * scf * atmpot[i] : 0.0;
+    while (depth <= zroot[v]) {
+      if (j > nstr) { rc = -1; break; }
+      int k = (j - 1) * nnod + i;
+      double s1 = pcana[v], s2 = pcana[v] + 1.0e-03;
+      double dz = (j == 1) ? (zsurf - z[k + nnod]) / 2.0
+                           : (z[k - nnod] - z[k + nnod]) / 2.0;
+      double sh2o = psi[k];
+      double gx1 = (sh2o - pcwlt[v]) / (pcref[v] - pcwlt[v]);
+      gx1 = fmin(1.0, fmax(0.0, gx1));
+      double gx2 = 1.0 - (sh2o - s1) / (s2 - s1);
+      gx2 = fmin(1.0, fmax(0.0, gx2));
+      double gx = fmin(gx1, gx2);
+      double beta = (1.0 - depth / zroot[v]) *
+                    exp(-1.0 * pz[v] * depth / zroot[v]);
+      btrani[k] = fmax(0.0, beta * dz * gx);
+      btran[i] = btran[i] + beta * dz;
+      omg[i] = omg[i] + gx * beta * dz;
+      j = j + 1;
+      depth = zsurf - z[(j - 1) * nnod + i];
+    }
+    btran[i] = fmax(0.0, btran[i]);
+    omg[i] = omg[i] / btran[i];
+  }
+  if (rc == 0)
+    for (int i = 0; i < nnod; ++i)
+      for (int j = 1; j <= nstr + 1; ++j) {
+        int k = (j - 1) * nnod + i, v = vegtype[i];
+        qtranie[k] = etp[i] * btrani[k] / btran[i] / fmax(omg[i], omgc[v]);
+      }
+  free(btrani); free(btran); free(etp); free(omg);
+  return rc;
+}
+
 void orc_switch_old(int nnod, int *ifatm, double *atmact,
                     const double *atmpot, double *pnew, double pmin) {
   /* SRC/switch_old.f:22-52. */

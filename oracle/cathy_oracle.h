@@ -205,6 +205,13 @@ double orc_stormb(const double *pnew, const double *ptimep,
 int orc_norms(int n, double *pikmax, double *pinf, double *pl2, double *finf,
               double *fl2, const double *pnew, const double *pold,
               const double *tnoti);
+/* SRC/etran.f:15-80: Feddes root uptake QTRANIE(n); vegtype[nnod] 0-based
+ * into the per-type tables.  Returns -1 if a root zone reaches the base. */
+int orc_etran(int n, int nnod, int nstr, const double *atmpot,
+              const double *z, const double *psi, const int *vegtype,
+              const double *pcana, const double *pcref, const double *pcwlt,
+              const double *zroot, const double *pz, const double *omgc,
+              double scf, double *qtranie);
 /* SRC/switch_old.f:12-53. */
 void orc_switch_old(int nnod, int *ifatm, double *atmact,
                     const double *atmpot, double *pnew, double pmin);

@@ -33,6 +33,14 @@ def main() -> int:
     ap.add_argument("--nprt", type=int, default=1)
     ap.add_argument("--nr", type=int, default=0)
     ap.add_argument("--numvp", type=int, default=1)
+    # bounds of the listed boundary conditions, which update_cathyH leaves at
+    # 1 and a user raises by hand for non-empty nansfdirbc / nansfneubc /
+    # sfbc files (cathy_tools.py:2270-2273), and of the vegetation types
+    ap.add_argument("--npmax", type=int, default=1)
+    ap.add_argument("--nqmax", type=int, default=1)
+    ap.add_argument("--nsfmax", type=int, default=1)
+    ap.add_argument("--nnsfmx", type=int, default=1)
+    ap.add_argument("--maxveg", type=int, default=1)
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     head = open(os.path.join(a.src, "CATHY.H")).readlines()[:109]
@@ -50,10 +58,10 @@ def main() -> int:
         f"      PARAMETER (NP2MAX=1,MAXSTR={a.nstr})\n",
         f"      PARAMETER (NFACEMAX={nfacemax})\n",
         "      PARAMETER (NMAX=NODMAX*(MAXSTR + 1),NTEMAX=3*NTRMAX*MAXSTR)\n",
-        "      PARAMETER (NPMAX=1,NPMAX_TRA=1,NQMAX=1,NSFMAX=1)\n",
-        "      PARAMETER (NNSFMX=1,MAXDIR=NODMAX+NPMAX+NSFMAX*NNSFMX)\n",
+        f"      PARAMETER (NPMAX={a.npmax},NPMAX_TRA=1,NQMAX={a.nqmax},NSFMAX={a.nsfmax})\n",
+        f"      PARAMETER (NNSFMX={a.nnsfmx},MAXDIR=NODMAX+NPMAX+NSFMAX*NNSFMX)\n",
         "      PARAMETER (MAXNUDN=1,MAXNUDT=1,MAXNUDC=1)\n",
-        f"      PARAMETER (MAXZON={a.nzone},MAXTRM={maxtrm},MAXIT=30,MAXVEG=1)\n",
+        f"      PARAMETER (MAXZON={a.nzone},MAXTRM={maxtrm},MAXIT=30,MAXVEG={a.maxveg})\n",
         f"      PARAMETER (NRMAX={a.nr + 1},MAXPRT={a.nprt},MAXVP={a.numvp})\n",
         f"      PARAMETER (N1MAX={a.n1},NTPMAX=N1MAX*NMAX)\n",
         "      PARAMETER (MAXBOT=1,INTBOT=1,MAXQOUT=1)\n",
@@ -78,7 +86,8 @@ def main() -> int:
             os.remove(link)
         os.symlink(path, link)
     sizes = dict(rowmax=a.rows, colmax=a.cols, maxcel=maxcel, nodmax=nodmax, maxstr=a.nstr,
-                 nmax=nodmax * (a.nstr + 1), ntemax=3 * ntrmax * a.nstr, maxzon=a.nzone, maxtrm=maxtrm, n1max=a.n1)
+                 nmax=nodmax * (a.nstr + 1), ntemax=3 * ntrmax * a.nstr, maxzon=a.nzone, maxtrm=maxtrm, n1max=a.n1,
+                 npmax=a.npmax, nqmax=a.nqmax, nsfmax=a.nsfmax, nnsfmx=a.nnsfmx, maxveg=a.maxveg)
     with open(os.path.join(a.out, "sizes.json"), "w") as f:
         json.dump(sizes, f)
     return 0

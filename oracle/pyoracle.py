@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libcathy_ref.so")
 REF_LARGE_SO = os.path.join(HERE, "_ref", "large", "libcathy_ref_large.so")
+REF_BC_SO = os.path.join(HERE, "_ref", "bc", "libcathy_ref_bc.so")
 
 VG_FIELDS = ["PNOT", "N", "M", "RMC", "PSAT", "N1", "M52", "NR", "MM1", "MR", "PSN"]
 VG_NFIELD = len(VG_FIELDS)
@@ -222,6 +223,22 @@ STAT_NAMES = ["NSTEP", "DELTAT", "TIME", "ITER", "NITERT", "BACK", "STORE1", "ST
               "ITRTOT", "ITLIN", "KLSFAI"]
 
 
+def oracle_etran(orc: "Oracle", n, nnod, nstr, atmpot, z, psi, vegtype0, tab: dict, scf):
+    """orc_etran: QTRANIE(n); vegtype0 0-based; tab holds pcana, pcref,
+    pcwlt, zroot, pz, omgc arrays."""
+    q = np.zeros(n)
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    keep = [f(atmpot), f(z), f(psi), np.ascontiguousarray(vegtype0, dtype=np.int32)] + \
+           [f(np.atleast_1d(tab[k])) for k in ("pcana", "pcref", "pcwlt", "zroot", "pz", "omgc")]
+    rc = orc.L.orc_etran(C.c_int(n), C.c_int(nnod), C.c_int(nstr), *[dptr(x) if x.dtype == np.float64 else
+                                                                      x.ctypes.data_as(C.POINTER(C.c_int))
+                                                                      for x in keep],
+                         C.c_double(scf), dptr(q))
+    if rc != 0:
+        raise ValueError("orc_etran: a root zone reaches the base of its column")
+    return q
+
+
 class OracleRun:
     def __init__(self, orc: Oracle, h, n, nnod):
         self.orc, self.h, self.n, self.nnod = orc, h, n, nnod
@@ -267,8 +284,11 @@ class RefLib:
             with open(sz) as f:
                 d = json.load(f)
             self.nmax, self.maxstr, self.maxzon, self.maxtrm = d["nmax"], d["maxstr"], d["maxzon"], d["maxtrm"]
+            self.npmax, self.nqmax = d.get("npmax", 1), d.get("nqmax", 1)
+            self.nsfmax, self.nnsfmx, self.maxveg = d.get("nsfmax", 1), d.get("nnsfmx", 1), d.get("maxveg", 1)
         else:
             self.nmax, self.maxstr, self.maxzon, self.maxtrm = REF_NMAX, REF_MAXSTR, 1, REF_MAXTRM
+            self.npmax = self.nqmax = self.nsfmax = self.nnsfmx = self.maxveg = 1
         self.vg = np.ctypeslib.as_array((C.c_double * (VG_NFIELD * self.nmax)).in_dll(self.L, "vg_"))
         # /CHORD/PMIN,... : PMIN is the first REAL*8 of the block
         self.chord = np.ctypeslib.as_array((C.c_double * 1).in_dll(self.L, "chord_"))

@@ -34,3 +34,56 @@ C  REFATM_CLOSE disconnects the unit again.
       IF (IIN6 .NE. 0) CLOSE(IIN6)
       RETURN
       END
+C
+C  REFUNIT_OPEN connects a boundary-condition file to one of the input
+C  units of COMMON /IOUNIT/ the reference reads its time series from:
+C  KIND 7 = IIN7 (sfbc: SFVONE/SFVNXT), 8 = IIN8 (nansfdirbc: BCONE/BCNXT
+C  for the Dirichlet nodes), 9 = IIN9 (nansfneubc: the same for the
+C  Neumann nodes), 6 = IIN6 (atmbc).  An unset unit (zero) is given the
+C  number 90 + KIND first; IU receives the unit.  IERR as for REFATM_OPEN,
+C  or -2 for another KIND.
+      SUBROUTINE REFUNIT_OPEN(KIND, CODES, NC, IU, IERR)
+      IMPLICIT NONE
+      INCLUDE 'IOUNITS.H'
+      INTEGER KIND, NC, IERR, I, IU
+      INTEGER CODES(*)
+      CHARACTER*4096 FNAME
+      IERR = -1
+      IU = 0
+      IF (NC .LT. 1 .OR. NC .GT. 4096) RETURN
+      FNAME = ' '
+      DO I = 1, NC
+         FNAME(I:I) = CHAR(CODES(I))
+      END DO
+      IERR = -2
+      IF (KIND .EQ. 6) THEN
+         IF (IIN6 .EQ. 0) IIN6 = 96
+         IU = IIN6
+      ELSE IF (KIND .EQ. 7) THEN
+         IF (IIN7 .EQ. 0) IIN7 = 97
+         IU = IIN7
+      ELSE IF (KIND .EQ. 8) THEN
+         IF (IIN8 .EQ. 0) IIN8 = 98
+         IU = IIN8
+      ELSE IF (KIND .EQ. 9) THEN
+         IF (IIN9 .EQ. 0) IIN9 = 99
+         IU = IIN9
+      ELSE
+         RETURN
+      END IF
+      CLOSE(IU)
+      OPEN(IU, FILE=FNAME(1:NC), STATUS='OLD', IOSTAT=IERR)
+      RETURN
+      END
+C
+C  REFUNIT_CLOSE disconnects the unit of KIND again.
+      SUBROUTINE REFUNIT_CLOSE(KIND)
+      IMPLICIT NONE
+      INCLUDE 'IOUNITS.H'
+      INTEGER KIND
+      IF (KIND .EQ. 6 .AND. IIN6 .NE. 0) CLOSE(IIN6)
+      IF (KIND .EQ. 7 .AND. IIN7 .NE. 0) CLOSE(IIN7)
+      IF (KIND .EQ. 8 .AND. IIN8 .NE. 0) CLOSE(IIN8)
+      IF (KIND .EQ. 9 .AND. IIN9 .NE. 0) CLOSE(IIN9)
+      RETURN
+      END

@@ -9,6 +9,15 @@ atmospheric forcing (below), free drainage at the base, no seepage faces,
 no nudging, no ET), calling the reference's FLOW3D, NEUMANN, SWITCH_OLD,
 WEIGHT, CHVELO, STORCAL, TIMUPD and ICVDWT routines directly.
 
+Listed boundary conditions (``dirbc`` / ``neubc``: nansfdirbc / nansfneubc
+files, bcfiles.write_bc) are read by the reference's own BCONE, BCNXT and
+BCBAK (their units opened by REFUNIT_OPEN), NEUMANN gets the real NNEU /
+NNEUC / QINP, and ``veg`` (Problem.veg) sets COMMON /FEDDES/ for the
+reference's ETRAN, called each step as cathy_main.f:2929 does.  That mode
+needs a build whose NPMAX / NQMAX / MAXVEG hold the lists (oracle/_ref/bc)
+and always takes the forcing from an atmbc file (one is written for a
+constant ``rain``), so back-steps run the reference's BKSTEP.
+
 Atmospheric forcing comes one of two ways:
 - ``atmbc`` names an atmbc file (``write_atmbc``).  The reference's own
   ATMONE and ATMNXT then read and interpolate it (the file is opened on
@@ -28,6 +37,8 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import shutil
+import tempfile
 
 import numpy as np
 
@@ -78,7 +89,9 @@ FLOW3D_ARGS = [
 class RefFlowRun:
     """Reference-FLOW3D driven run.  Parameters follow orc_params."""
 
-    def __init__(self, ref: RefLib, mesh, soil, psi0, rain, prm: dict, atmbc: str | None = None):
+    def __init__(self, ref: RefLib, mesh, soil, psi0, rain, prm: dict, atmbc: str | None = None,
+                 dirbc: str | None = None, neubc: str | None = None, veg: dict | None = None,
+                 sfbc: str | None = None, isfcvg: int = 0):
         assert soil["nzone"] <= ref.maxzon, "more zones than the reference build's MAXZON"
         assert mesh.nstr <= ref.maxstr and mesh.n <= ref.nmax, "mesh larger than the reference build"
         self.ref = ref
@@ -165,8 +178,156 @@ class RefFlowRun:
         self.rain = rain
         self.atmbc = atmbc
         self.nterm = nterm
+        self.lists = dirbc is not None or neubc is not None or veg is not None or sfbc is not None
+        self._tmp = None
+        if self.lists:
+            self._tmp = tempfile.mkdtemp(prefix="refbc_")
+            if atmbc is None:
+                atmbc = os.path.join(self._tmp, "atmbc")
+                write_atmbc(atmbc, [0.0], [rain], 1)
+                self.atmbc = atmbc
+            if dirbc is None:
+                dirbc = os.path.join(self._tmp, "nansfdirbc")
+                with open(dirbc, "w") as f:
+                    f.write("0\ttime\n0\t0\n")
+            if neubc is None:
+                neubc = os.path.join(self._tmp, "nansfneubc")
+                with open(neubc, "w") as f:
+                    f.write(f"0\ttime\n{-1 if prm['free_drainage'] else 0}\t0\n")
+            self.bc = {"dir": self._bc_state(ref.npmax), "neu": self._bc_state(ref.nqmax)}
+            d, q = self.bc["dir"], self.bc["neu"]
+            # FLOW3D's view (cathy_main.f:3101-3128): NDIR(2), NDIRC(2), ANP, ANQ,
+            # ACONTP, ACONTQ, PRESC, PTIM, PINP, Q, QTIM, QINP
+            a["NDIR"], a["NDIRC"] = d["nb2c"][1:], d["nbfc"][1:]
+            a["NP"], a["NQ"], a["CONTP"], a["CONTQ"] = d["anbc"], q["anbc"], d["abcnod"], q["abcnod"]
+            a["PRESC"], a["PTIM"], a["PINP"] = d["bc"], d["tim"], d["inp"]
+            a["Q"], a["QTIM"], a["QINP"] = q["bc"], q["tim"], q["inp"]
+            self.units = {}
+            # seepage faces: SFVONE / SFVNXT / SFVREC state (NSFMAX, NNSFMX of the build)
+            self.sfbc, self.isfcvg = sfbc, isfcvg
+            fm, fx = ref.nsfmax, ref.nnsfmx
+            i32 = lambda k: np.zeros(k, dtype=np.int32)
+            self.sfv = dict(sfv=i32(2), sfvnum=i32(2 * fm), sfvnod=i32(2 * fm * fx), sfvtim=np.zeros(2),
+                            ifsf=i32(n + 1), ifsfp=i32(n + 1))
+            files = ((8, dirbc), (9, neubc)) + (((7, sfbc),) if sfbc is not None else ())
+            for kind, path in files:
+                codes = np.frombuffer(os.fsencode(path), dtype=np.uint8).astype(np.int32)
+                iu, ierr = C.c_int(0), C.c_int(0)
+                L.refunit_open_(C.byref(C.c_int(kind)), codes.ctypes.data_as(C.POINTER(C.c_int)),
+                                C.byref(C.c_int(len(codes))), C.byref(iu), C.byref(ierr))
+                if ierr.value != 0:
+                    raise OSError(f"REFUNIT_OPEN({path}) failed: IOSTAT {ierr.value}")
+                self.units[kind] = iu.value
+            self.veg = veg
+            self.qtr_on = veg is not None
+            if veg is not None:
+                self._set_feddes(veg, prm["scf"])
+                self.vegtype = np.ascontiguousarray(veg["veg_type"], dtype=np.int32)
         self._argv = [a[name].ctypes.data_as(C.c_void_p) for name, _ in FLOW3D_ARGS]
         self._init_state(prm)
+
+    # -- listed boundary conditions (reference BCONE / BCNXT / ETRAN) ---------
+    def _bc_state(self, nbcmax):
+        n = self.n
+        i32 = lambda k: np.zeros(k, dtype=np.int32)
+        return dict(nbcmax=nbcmax, nb2c=i32(3), nbfc=i32(3), nbc=i32(3), hti=i32(1), tim=np.zeros(3),
+                    inp=np.zeros(3 * nbcmax + 3), nod=i32(3 * nbcmax + 3), bc=np.zeros(nbcmax + 1),
+                    anbc=i32(1), abcnod=i32(nbcmax + 1), cont=i32(n + 1), cond=np.zeros(n + 1))
+
+    def _set_feddes(self, veg, scf):
+        """COMMON /FEDDES/ SCF, PCREF, PCWLT, PCANA, PZ, OMGC, ZROOT (MAXVEG
+        entries each, SOILCHAR.H)."""
+        mv = self.ref.maxveg
+        nveg = len(np.atleast_1d(veg["pcana"]))
+        assert nveg <= mv, "more vegetation types than the reference build's MAXVEG"
+        com = np.ctypeslib.as_array((C.c_double * (1 + 6 * mv)).in_dll(self.ref.L, "feddes_"))
+        com[:] = 0.0
+        com[0] = scf
+        for j, key in enumerate(("pcref", "pcwlt", "pcana", "pz", "omgc", "zroot")):
+            com[1 + j * mv: 1 + j * mv + nveg] = np.atleast_1d(veg[key])
+
+    def _bc_args(self, st, with_n: bool):
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        head = [C.byref(C.c_int(self.n))] if with_n else []
+        return head + [C.byref(C.c_int(self.nnod)), C.byref(C.c_int(st["nbcmax"])), C.byref(C.c_int(0)),
+                       ip(st["nb2c"]), ip(st["nbfc"]), ip(st["nbc"]), C.byref(C.c_int(self.mesh.nstr)),
+                       ip(st["hti"]), dp(self.a["TIME"])]
+
+    def _bcone(self, key, unit):
+        st = self.bc[key]
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        name = C.create_string_buffer(b" " * 22)
+        self.ref.L.bcone_(name, C.byref(C.c_int(unit)), C.byref(C.c_int(0)), C.byref(C.c_int(0)),
+                          *self._bc_args(st, True), dp(self.a["DELTAT"]), dp(st["tim"]), dp(st["inp"]),
+                          ip(st["nod"]), dp(st["bc"]), ip(st["anbc"]), ip(st["abcnod"]), ip(st["cont"]),
+                          dp(st["cond"]), C.c_size_t(22))
+
+    def _bcnxt(self, key, unit):
+        st = self.bc[key]
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        name = C.create_string_buffer(b" " * 22)
+        self.ref.L.bcnxt_(name, C.byref(C.c_int(unit)), C.byref(C.c_int(0)), C.byref(C.c_int(0)),
+                          *self._bc_args(st, False), dp(st["tim"]), dp(st["inp"]), ip(st["nod"]), dp(st["bc"]),
+                          ip(st["anbc"]), ip(st["abcnod"]), C.c_size_t(22))
+
+    def _etran(self):
+        if not self.qtr_on:
+            return
+        a = self.a
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        self.ref.L.etran_(C.byref(C.c_int(self.n)), C.byref(C.c_int(self.nnod)), C.byref(C.c_int(self.mesh.nstr)),
+                          dp(a["ATMPOT"]), dp(a["Z"]), dp(a["PNEW"]), dp(a["PNODI"]), ip(self.vegtype),
+                          dp(a["QTRANIE"]))
+
+    def _sf_args(self):
+        """SFVNXT / SFVREC state arguments after N (sfvnxt.f:1-3)."""
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        a, v = self.a, self.sfv
+        return [C.byref(C.c_int(self.ref.nsfmax)), C.byref(C.c_int(self.ref.nnsfmx)), ip(a["NSF"]),
+                ip(a["NSFNUM"]), ip(a["NSFNOD"]), ip(v["sfv"]), ip(v["sfvnum"]), ip(v["sfvnod"])]
+
+    def _sfvone(self):
+        """inital.f:208-229: SFVONE, SFQP = 0, SFCHEK, SFINIT (DUPUIT = 0)."""
+        a = self.a
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        v = self.sfv
+        z = np.zeros(self.n); z[:] = a["Z"][:self.n]
+        L = self.ref.L
+        L.sfvone_(C.byref(C.c_int(self.units[7])), C.byref(C.c_int(0)), C.byref(C.c_int(0)), C.byref(C.c_int(0)),
+                  C.byref(C.c_int(self.n)), C.byref(C.c_int(self.ref.nsfmax)), C.byref(C.c_int(self.ref.nnsfmx)),
+                  dp(a["TIME"]), ip(a["NSF"]), ip(a["NSFNUM"]), ip(a["NSFNOD"]), ip(v["sfv"]), ip(v["sfvnum"]),
+                  ip(v["sfvnod"]), dp(v["sfvtim"]), ip(v["ifsf"]), ip(v["ifsfp"]), dp(z))
+        a["SFQP"][:] = 0.0
+        a["SFCHEK"][0] = 1 if (self.isfcvg == 1 and a["NSF"][0] > 0) else 0
+        a["DUPUIT"][0] = 0
+        L.sfinit_(C.byref(C.c_int(self.n)), ip(a["NSF"]), ip(a["NSFNUM"]), ip(a["NSFNOD"]), ip(a["SFEX"]),
+                  ip(a["SFEXP"]), ip(a["SFEXIT"]), ip(a["DUPUIT"]), dp(a["PTIMEP"]), dp(a["PNEW"]), ip(a["SFFLAG"]),
+                  dp(a["DELTAT"]), dp(a["Z"]), ip(a["PUNTDIRSF_NODE"]))
+
+    def _sfv_step(self):
+        """cathy_main.f:2913-2921: SFVNXT past the second record's time,
+        else SFVREC."""
+        a = self.a
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        if a["TIME"][0] > self.sfv["sfvtim"][1]:
+            z = np.zeros(self.n); z[:] = a["Z"][:self.n]
+            self.ref.L.sfvnxt_(C.byref(C.c_int(self.units[7])), C.byref(C.c_int(0)), C.byref(C.c_int(0)),
+                               C.byref(C.c_int(0)), C.byref(C.c_int(self.n)), *self._sf_args(), dp(self.sfv["sfvtim"]),
+                               dp(z))
+        else:
+            self.ref.L.sfvrec_(*self._sf_args())
+
+    def close(self):
+        if self._tmp:
+            for kind in getattr(self, "units", {}):
+                self.ref.L.refunit_close_(C.byref(C.c_int(kind)))
+            shutil.rmtree(self._tmp, ignore_errors=True)
+            self._tmp = None
 
     # -- reference set-up calls --------------------------------------------
     def _tpnodi(self, mesh, soil, a):
@@ -212,6 +373,13 @@ class RefFlowRun:
         a["ITER"][0] = 1
         a["PNEW"][:n] = a["PTIMEP"][:n]
         self.nstep = 1
+        if self.lists:   # inital.f:160-181: BCONE of the Dirichlet and Neumann series
+            a["QPOLD"][:] = 0.0
+            a["QTRANIE"][:] = 0.0
+            self._bcone("dir", self.units[8])
+            self._bcone("neu", self.units[9])
+            if self.sfbc is not None:
+                self._sfvone()
         if self.atmbc is not None:
             self._atmone()
         else:
@@ -244,6 +412,9 @@ class RefFlowRun:
         a = self.a
         ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
         self._nsf = np.zeros(4, dtype=np.int32)
+        if self.lists and self.sfbc is not None:
+            return [ip(a["NP"]), ip(a["NQ"]), ip(a["CONTP"]), ip(a["CONTQ"]), ip(a["NSF"]), ip(a["NSFNUM"]),
+                    ip(a["NSFNOD"])]
         return [ip(a["NP"]), ip(a["NQ"]), ip(a["CONTP"]), ip(a["CONTQ"]), C.byref(C.c_int(0)), ip(self._nsf),
                 ip(self._nsf)]
 
@@ -299,7 +470,10 @@ class RefFlowRun:
         mb = self._mb(); mb["STORE0"] = st1[0]; mb["STORE1"] = st1[0]; mb["STORE2"] = st1[0]
         self.ckrwp = a["CKRW"][:n].copy()
         self._neumann(self.ckrwp)
-        q = a["Q"][:nnod] if self.p["free_drainage"] else np.zeros(0)
+        if self.lists:   # mbinit.f:44-51, then cathy_main.f:2696-2706 under free drainage
+            q = a["Q"][:int(a["NQ"][0])]
+        else:
+            q = a["Q"][:nnod] if self.p["free_drainage"] else np.zeros(0)
         nin = 0.0; nout = 0.0
         for v in q:
             if v > 0.0:
@@ -343,6 +517,15 @@ class RefFlowRun:
                             dp(a["PNODI"]), dp(a["VOLNOD"]))
 
     def _neumann(self, ckrw):
+        if self.lists:   # neumann.f with the Neumann series' own NNEU / NNEUC / QINP
+            st = self.bc["neu"]
+            dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+            ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+            ck = np.zeros(len(self.a["CKRW"])); ck[:self.n] = ckrw[:self.n]
+            self.ref.L.neumann_(dp(self.a["TIME"]), ip(st["nb2c"]), ip(st["nbfc"]), C.byref(C.c_int(self.nnod)),
+                                C.byref(C.c_int(self.mesh.nstr)), dp(ck), dp(self.kznod), dp(self.a["ARENOD"]),
+                                ip(st["abcnod"]), dp(st["inp"]), dp(st["bc"]))
+            return
         if not self.p["free_drainage"]:
             return
         a, nnod, nstr = self.a, self.nnod, self.mesh.nstr
@@ -382,22 +565,37 @@ class RefFlowRun:
                                    "ANP", "SFV")}
         ints["ANQ"] = np.array([a["NQ"][0], 0, 0, 0], dtype=np.int32)
         nsf_arr = i32(64)
+        dir_ = dict(nb2c=ints["NDIR"], nbfc=ints["NDIRC"], nbc=ints["NP"], hti=ints["HTIDIR"], nod=a["CONTP"],
+                    bc=a["PRESC"], tim=never, inp=a["PINP"], anbc=ints["ANP"], abcnod=a["CONTP"])
+        neu_ = dict(nb2c=ints["NNEU"], nbfc=ints["NNEUC"], nbc=ints["NQ"], hti=ints["HTINEU"], nod=a["CONTQ"],
+                    bc=a["Q"], tim=never, inp=a["QINP"], anbc=ints["ANQ"], abcnod=a["CONTQ"])
+        sf = dict(nsf=ints["NSF"], nsfnum=nsf_arr, sfex=nsf_arr, sfexit=nsf_arr, sfexp=nsf_arr, nsfnod=nsf_arr,
+                  sfv=ints["SFV"], sfvnum=nsf_arr, sfvnod=nsf_arr, sfvtim=np.zeros(2))
+        if self.lists:   # the series' own state: BKSTEP runs BCNXT or BCBAK on it
+            dir_, neu_ = self.bc["dir"], self.bc["neu"]
+            if self.sfbc is not None:
+                v = self.sfv
+                sf = dict(nsf=a["NSF"], nsfnum=a["NSFNUM"], sfex=a["SFEX"], sfexit=a["SFEXIT"], sfexp=a["SFEXP"],
+                          nsfnod=a["NSFNOD"], sfv=v["sfv"], sfvnum=v["sfvnum"], sfvnod=v["sfvnod"],
+                          sfvtim=v["sfvtim"])
         self.ref.L.bkstep_(
-            ip(a["IPRT1"]), C.byref(C.c_int(n)), C.byref(C.c_int(self.mesh.nstr)), ip(ints["NDIR"]),
-            ip(ints["NDIRC"]), ip(ints["NP"]), ip(ints["NNEU"]), ip(ints["NNEUC"]), ip(ints["NQ"]),
-            ip(a["HSPATM"]), ip(a["HTIATM"]), ip(self.ieto), ip(ints["HTIDIR"]), ip(ints["HTINEU"]),
+            ip(a["IPRT1"]), C.byref(C.c_int(n)), C.byref(C.c_int(self.mesh.nstr)), ip(dir_["nb2c"]),
+            ip(dir_["nbfc"]), ip(dir_["nbc"]), ip(neu_["nb2c"]), ip(neu_["nbfc"]), ip(neu_["nbc"]),
+            ip(a["HSPATM"]), ip(a["HTIATM"]), ip(self.ieto), ip(dir_["hti"]), ip(neu_["hti"]),
             ip(a["ITER"]), ip(a["NITERT"]), ip(ints["KBACKT"]), ip(ints["KBACK"]), C.byref(C.c_int(nnod)),
-            ip(ints["NSF"]), ip(ints["NUMRES"]), ip(ints["NCELNL"]), ip(a["CONTP"]), ip(a["CONTQ"]),
-            ip(a["IFATM"]), ip(a["IFATMP"]), ip(nsf_arr), ip(nsf_arr), ip(nsf_arr), ip(nsf_arr),
+            ip(sf["nsf"]), ip(ints["NUMRES"]), ip(ints["NCELNL"]), ip(dir_["nod"]), ip(neu_["nod"]),
+            ip(a["IFATM"]), ip(a["IFATMP"]), ip(sf["nsfnum"]), ip(sf["sfex"]), ip(sf["sfexit"]), ip(sf["sfexp"]),
             ip(ints["SURF"]), ip(ints["PONDING"]), ip(ints["PONDP"]), ip(a["DTGMIN"]),
             dp(a["TIME"]), dp(a["TIMEP"]), dp(a["DELTAT"]), dp(a["DTMIN"]), dp(a["DTREDM"]), dp(a["DTREDS"]),
-            dp(a["TETAF"]), dp(a["OVFLNOD"]), dp(np.zeros(nnod + 1)), dp(a["PRESC"]), dp(never), dp(a["PINP"]),
-            dp(a["Q"]), dp(never), dp(a["QINP"]), dp(a["ATMTIM"]), dp(a["ATMINP"]), dp(a["ARENOD"]),
+            dp(a["TETAF"]), dp(a["OVFLNOD"]), dp(np.zeros(nnod + 1)), dp(dir_["bc"]), dp(dir_["tim"]),
+            dp(dir_["inp"]), dp(neu_["bc"]), dp(neu_["tim"]), dp(neu_["inp"]), dp(a["ATMTIM"]), dp(a["ATMINP"]),
+            dp(a["ARENOD"]),
             dp(a["POLD"]), dp(a["PTOLD"]), dp(a["PNEW"]), dp(a["PTNEW"]), dp(a["PTIMEP"]),
             dp(a["ATMACT"]), dp(a["ATMPOT"]), dp(a["ATMOLD"]), dp(a["QPNEW"]), dp(a["QPOLD"]),
-            dp(a["SFQ"]), dp(a["SFQP"]), ip(ints["ANP"]), ip(ints["ANQ"]), ip(a["CONTP"]), ip(a["CONTQ"]),
-            ip(nsf_arr), ip(ints["SFV"]), ip(nsf_arr), ip(nsf_arr), dp(np.zeros(2)), ip(ints["TRANSP"]))
-        assert ints["NSF"][0] == 0 and ints["KBACK"][0] == 1
+            dp(a["SFQ"]), dp(a["SFQP"]), ip(dir_["anbc"]), ip(neu_["anbc"]), ip(dir_["abcnod"]),
+            ip(neu_["abcnod"]),
+            ip(sf["nsfnod"]), ip(sf["sfv"]), ip(sf["sfvnum"]), ip(sf["sfvnod"]), dp(sf["sfvtim"]), ip(ints["TRANSP"]))
+        assert ints["KBACK"][0] == 1
 
     def _bkstep(self):
         """bkstep.f:58-147 restated (constant non-atmospheric BCs); the
@@ -429,19 +627,29 @@ class RefFlowRun:
         if self.finished:
             return self.finished, {}
         a, n, nnod, p = self.a, self.n, self.nnod, self.p
+        if self.lists:   # cathy_main.f:2892-2903: BCNXT of the Dirichlet and Neumann series
+            self._bcnxt("dir", self.units[8])
+            self._bcnxt("neu", self.units[9])
         self._neumann(a["CKRW"])
+        if self.lists and self.sfbc is not None:
+            self._sfv_step()
         if self.atmbc is not None:
             self._atmnxt()
         else:   # ATMNXT on a constant record: ATMACT = ATMPOT for IFATM = 0 nodes
             for i in range(nnod):
                 if a["IFATM"][i] == 0:
                     a["ATMACT"][i] = a["ATMPOT"][i]
+        if self.lists:
+            self._etran()   # cathy_main.f:2929
         self._switch_old()
         self._copy("POLD", "PNEW", n)
         self._weight()
         self._copy("PTOLD", "PTNEW", n)
         backs = 0
         while True:
+            if self.lists:   # cathy_main.f:2983 (label 100): NODDIR starts with the fixed-head nodes
+                k = int(a["NP"][0])
+                a["NODDIR"][:k] = a["CONTP"][:k]
             self._flow3d()
             if not a["BCKSTP"][0]:
                 break
@@ -461,9 +669,16 @@ class RefFlowRun:
                      DSTORE=mb["DSTORE"], CDSTOR=c["CDSTOR"], VIN=mb["VIN"], CVIN=c["CVIN"], VOUT=mb["VOUT"],
                      CVOUT=c["CVOUT"], VINOUT=mb["VIN"] + mb["VOUT"], VTOT=c["VTOT"], ERRAS=mb["ERRAS"],
                      ERREL=mb["ERREL"], CERRAS=c["CERRAS"], CAERAS=c["CAERAS"])
-        for k in ("ADIN", "ADOUT", "NDIN", "NDOUT", "ANIN", "ANOUT", "NNIN", "NNOUT"):
+        for k in ("ADIN", "ADOUT", "NDIN", "NDOUT", "ANIN", "ANOUT", "NNIN", "NNOUT", "SFFLW"):
             mb[k + "P"] = mb[k]
+        if self.lists and self.sfbc is not None:   # cathy_main.f:3293-3299
+            a["SFEXP"][:] = a["SFEX"]
+            a["SFEXIT"][:] = a["SFEX"]
+            a["SFQP"][:] = a["SFQ"]
         self._set_mb(mb)
+        if self.lists:   # cathy_main.f:3762
+            k = int(a["NP"][0])
+            a["QPOLD"][:k] = a["QPNEW"][:k]
         a["IFATMP"][:nnod] = a["IFATM"][:nnod]
         a["ATMOLD"][:nnod] = a["ATMACT"][:nnod]
         self._copy("PTIMEP", "PNEW", n)

@@ -76,6 +76,13 @@ class Problem:
     seed: int = 0
     meta: dict = field(default_factory=dict)
     hspatm: int = 1       # HSPATM of the atmbc file: 1 homogeneous, 0 one value per surface node
+    # listed-node boundary series (bcfiles.BCSeries: nansfdirbc / nansfneubc)
+    # and the vegetation of root uptake (veg_type (nnod,) 1-based; pcana,
+    # pcref, pcwlt, zroot, pz, omgc per type); None: not used
+    bc_dir: object = None
+    bc_neu: object = None
+    veg: dict | None = None
+    seepage: dict | None = None  # faces: 0-based node arrays top to bottom; isfcvg: 0 / 1
 
 
 def sig4(a: np.ndarray) -> np.ndarray:
@@ -123,6 +130,78 @@ def hillslope_small(seed: int = 1, params: Params | None = None, nstr: int = 10,
     p = params or hillslope_small_params()
     return Problem("hillslope_small", m, soil, icvdwt(m, 1.0), np.array([0.0]), np.array([1e-7]), p, seed,
                    dict(cells=cells, dx=dx))
+
+
+def hillslope_bc(seed: int = 1, params: Params | None = None) -> Problem:
+    """The small hillslope with every listed boundary condition of the
+    subsurface step, over 3 h:
+
+    * fixed heads on the downslope face (x = max), three records: t = 0
+      every face node at the hydrostatic head of a water table 1.5 m deep;
+      t = 1800 s the face's surface nodes at -0.3 m replicated down the
+      layers (NODIN2 > 0); t = 5400 s only the face nodes deeper than 1 m,
+      water table 1.2 m deep;
+    * free drainage at the base plus listed Neumann fluxes on the upslope
+      face (x = min): layers 3-8 at 2e-6 m^3/s each and three surface
+      corner nodes at 1e-6 (t = 0), 4e-6 with one node listed twice
+      (t = 2700 s), all zero (t = 6000 s);
+    * root uptake with two vegetation types (upslope half ZROOT 1 m, PZ 1;
+      downslope half ZROOT 0.6 m, PZ 2, other stress heads), SCF = 1, under
+      a forcing that turns from 1e-7 m/s rain to 2e-7 m/s evaporation in
+      the first hour (ATMPOT < 0 drives ETRAN).
+    """
+    from . import bcfiles as B
+    p = params or hillslope_small_params(tmax=10800.0, scf=1.0)
+    base = hillslope_small(seed=seed, params=p)
+    m = base.mesh
+    nnod, nstr = m.nnod, m.nstr
+    xs = m.x[:nnod]
+    zs = np.tile(m.z[:nnod], nstr + 1)
+    down = np.flatnonzero(np.isclose(m.x, xs.max()))          # every layer
+    up_s = np.flatnonzero(np.isclose(xs, xs.min()))           # surface ids
+    layer = np.arange(m.n) // nnod
+    hyd = lambda wt, ids: (zs[ids] - wt) - m.z[ids]
+    deep = down[(zs[down] - m.z[down]) > 1.0]
+    empty = np.zeros(0, dtype=np.int64)
+    bc_dir = B.make_series("dirichlet", [
+        (0.0, 0, empty, [], down, hyd(1.5, down)),
+        (1800.0, len(down[down < nnod]), down[down < nnod], np.full(len(down[down < nnod]), -0.3), empty, []),
+        (5400.0, 0, empty, [], deep, hyd(1.2, deep)),
+    ], nnod, nstr)
+    lat = np.flatnonzero(np.isclose(m.x, xs.min()) & (layer >= 3) & (layer <= 8))
+    corner = up_s[:3]
+    x0 = np.concatenate([lat, corner])
+    q0 = np.concatenate([np.full(len(lat), 2e-6), np.full(3, 1e-6)])
+    x1 = np.concatenate([lat, corner, lat[:1]])
+    q1 = np.concatenate([np.full(len(lat), 4e-6), np.full(3, 1e-6), [4e-6]])
+    bc_neu = B.make_series("neumann", [
+        (0.0, -1, empty, [], x0, q0),
+        (2700.0, -1, empty, [], x1, q1),
+        (6000.0, -1, empty, [], x0, np.zeros(len(x0))),
+    ], nnod, nstr)
+    veg = dict(veg_type=np.where(xs < 0.5 * (xs.min() + xs.max()), 1, 2).astype(np.int32),
+               pcana=np.array([0.0, -0.1]), pcref=np.array([-4.0, -2.0]), pcwlt=np.array([-150.0, -50.0]),
+               zroot=np.array([1.0, 0.6]), pz=np.array([1.0, 2.0]), omgc=np.array([1.0, 0.5]))
+    return Problem("hillslope_bc", m, base.soil, base.psi0, np.array([0.0, 3600.0, 10800.0]),
+                   np.array([1e-7, -2e-7, -2e-7]), p, seed, dict(base.meta), 1, bc_dir, bc_neu, veg)
+
+
+def hillslope_seepage(seed: int = 1, params: Params | None = None, isfcvg: int = 1) -> Problem:
+    """The small hillslope with two seepage faces on its downslope face
+    (the vertical node lines at the 6th and 16th surface node of x = max,
+    11 nodes each, top to bottom), a water table 0.4 m deep, 5e-7 m/s rain
+    and free drainage at the base, for 2 h; ISFCVG = 1 puts the exit points
+    into the Picard convergence test."""
+    p = params or hillslope_small_params(tmax=7200.0)
+    base = hillslope_small(seed=seed, params=p)
+    m = base.mesh
+    nnod, nstr = m.nnod, m.nstr
+    xs = m.x[:nnod]
+    edge = np.flatnonzero(np.isclose(xs, xs.max()))
+    edge = edge[np.argsort(m.y[edge])]
+    faces = [np.array([s + l * nnod for l in range(nstr + 1)]) for s in (edge[5], edge[15])]
+    return Problem("hillslope_seepage", m, base.soil, icvdwt(m, 0.4), np.array([0.0]), np.array([5e-7]), p, seed,
+                   dict(base.meta), 1, seepage=dict(faces=faces, isfcvg=isfcvg))
 
 
 def hillslope_small_member(seed: int, params: Params | None = None) -> Problem:

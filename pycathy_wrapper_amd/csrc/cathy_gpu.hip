@@ -418,6 +418,49 @@ int cathy_gpu_symslv_release(void) {
   });
 }
 
+int cathy_gpu_flow3d_set_bc(int64_t *handle, const int *kind, const int *nrec, const double *times,
+                            const int *counts, const int *nodes, const double *values) {
+  return guarded([&] {
+    const std::shared_ptr<Flow> f = lookup(handle);
+    CG_REQUIRE(kind && nrec && times && counts, "null argument");
+    f->set_bc(*kind, *nrec, times, counts, nodes, values);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_set_seepage(int64_t *handle, const int *nsf, const int *counts, const int *nodes,
+                                 const int *isfcvg) {
+  return guarded([&] {
+    const std::shared_ptr<Flow> f = lookup(handle);
+    CG_REQUIRE(nsf && isfcvg, "null argument");
+    f->set_seepage(*nsf, counts, nodes, *isfcvg);
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_get_seepage(int64_t *handle, int *sfex_out, double *sfq_out) {
+  return guarded([&] {
+    const std::shared_ptr<Flow> f = lookup(handle);
+    const int m = f->nsf_nodes;
+    if (m > 0 && sfex_out) f->sfex.download(sfex_out, m, f->st);
+    if (m > 0 && sfq_out) f->sfq.download(sfq_out, m, f->st);
+    CG_HIP(hipStreamSynchronize(f->st));
+    return m;
+  });
+}
+
+int cathy_gpu_flow3d_set_vegetation(int64_t *handle, const int *nveg, const int *veg_type, const double *pcana,
+                                    const double *pcref, const double *pcwlt, const double *zroot,
+                                    const double *pz, const double *omgc, const double *z) {
+  return guarded([&] {
+    const std::shared_ptr<Flow> f = lookup(handle);
+    CG_REQUIRE(nveg, "null argument");
+    const double *tab[6] = {pcana, pcref, pcwlt, zroot, pz, omgc};
+    f->set_vegetation(*nveg, veg_type, tab, z);
+    return 0;
+  });
+}
+
 int cathy_gpu_flow3d_set_pcg_batches(int64_t *handle, const int *first_pct, const int *followup) {
   return guarded([&] {
     const std::shared_ptr<Flow> f = lookup(handle);

@@ -68,6 +68,65 @@ struct Params {
   double pmin = -5.0, pondh_min = 0.0, scf = 0.0;
 };
 
+// ------------------------------------------------ listed-node BC series --
+// A non-atmospheric Dirichlet or Neumann series (input/nansfdirbc,
+// input/nansfneubc): records of (time, node rows, values) in file order and
+// the three-record window BCONE / BCNXT / BCBAK keep over them (bcone.f,
+// bcnxt.f, bcbak.f).  The window only moves forward; the active record
+// (ANBC, ABCNOD, BC) is the window's second record while the third lies
+// ahead, else the third; BCBAK (a back-step to a time at or before the
+// second record's) takes the first.
+struct BcSeries {
+  std::vector<double> t;      // record times
+  std::vector<int> off;       // record k: entries off[k] .. off[k+1]
+  std::vector<int> row;       // device rows
+  std::vector<int> node;      // CATHY ids (0-based)
+  std::vector<double> val;
+  int next = 0, hti = 0, act = -1;
+  double tim[3] = {0, 0, 0};
+  int rec[3] = {-1, -1, -1};  // record held in each window slot (-1: none)
+  bool set() const { return !t.empty(); }
+  int count(int k) const { return k < 0 ? 0 : off[k + 1] - off[k]; }
+  int max_count() const {
+    int m = 0;
+    for (size_t k = 0; k + 1 < off.size(); ++k) m = std::max(m, off[k + 1] - off[k]);
+    return m;
+  }
+  // bcnxt.f:30-41: shift the window until TIME <= its third time
+  void advance_window(double time) {
+    while (!(time <= tim[2])) {
+      tim[0] = tim[1]; tim[1] = tim[2];
+      rec[0] = rec[1]; rec[1] = rec[2];
+      if (next >= (int)t.size()) { hti = 1; break; }
+      tim[2] = t[next];
+      rec[2] = next++;
+    }
+    act = tim[2] > tim[1] ? rec[1] : rec[2];  // bcnxt.f:42-54
+  }
+  void bcone(double time) {  // bcone.f:20-60
+    next = 0; hti = 0;
+    tim[0] = tim[1] = 0.0;
+    rec[0] = rec[1] = -1;
+    tim[2] = t[0];
+    rec[2] = 0;
+    next = 1;
+    advance_window(time);
+  }
+  void bcnxt(double time) {
+    if (hti) return;
+    advance_window(time);
+  }
+  void bcbak() {  // bcbak.f:14-21
+    if (tim[0] >= tim[1]) return;
+    act = rec[0];
+  }
+  // bkstep.f:73-88: forward again past the window's second time, else back
+  void back(double time) {
+    if (time > tim[1]) bcnxt(time);
+    else bcbak();
+  }
+};
+
 // ------------------------------------------------------------------ Flow --
 struct Flow {
   int n = 0, nt = 0, ntri = 0, nnod = 0, nstr = 0, nnz = 0, ordering = 0, lump = 1;
@@ -107,6 +166,27 @@ struct Flow {
   DBuf<uint8_t> isdir;
   DBuf<int> ifatm, ifatmp;
   DBuf<double> atmpot, atmact, atmold;
+  // listed-node boundary conditions: fixed heads (CONTP/PRESC, QPNEW and
+  // QPOLD by list position), listed Neumann fluxes (CONTQ/Q beyond the base
+  // nodes of free drainage), root uptake (ETRAN's QTRANIE) -- all optional
+  BcSeries bc_dir, bc_neu;
+  DBuf<uint8_t> bcfix;                       // per row: 2 fixed head, 4 seepage exit
+  DBuf<int> dir_rows, neu_rows, atm_off;     // current lists (rows), IFATM = -1 surface ids
+  DBuf<double> dir_vals, neu_vals, qpnew, qpold;
+  std::vector<int> h_dir_rows, h_neu_rows;   // rows marked / fluxes added by the last apply
+  int np_cur = 0, nx_cur = 0, n_atm_off = 0;
+  double nx_in = 0.0, nx_out = 0.0;          // sums of the listed Neumann fluxes (fluxmb.f:39-46)
+  bool veg_on = false;
+  DBuf<int> vegc, permd;
+  DBuf<double> zc, vegtab, qtr;
+  // seepage faces (sfbc): nodes face by face, top to bottom; SFEX / SFEXIT
+  // / SFEXP and SFQ / SFQP by node position (cathy_main.f:1085-1100)
+  std::vector<int> sf_node, sf_off;
+  DBuf<int> sf_rows, sfex, sfexit, sfexp;
+  DBuf<double> sfq, sfqp;
+  int nsf_nodes = 0, isfcvg = 0, ksf = 0;
+  double sfflw_cur = 0.0, sfflwp = 0.0;
+  bool lists_on() const { return bc_dir.set() || bc_neu.set() || nsf_nodes > 0; }
   DBuf<float> F32;
   DBuf<double> F64, dinv;
   DBuf<double> r, p, Bv, z;
@@ -946,8 +1026,8 @@ struct Flow {
     scal.alloc(1);
     dbeta.alloc(1);
     flags.alloc(2);
-    res.alloc(16);
-    h_res.assign(16, 0.0);
+    res.alloc(24);
+    h_res.assign(24, 0.0);
     CG_HIP(hipMemsetAsync(flags.p, 0, sizeof(int) * 2, st));
   }
 
@@ -1259,7 +1339,7 @@ struct Flow {
                        etai_n.p, dckrw.p, detai.p);
     hipLaunchKernelGGL(k_elem_avg, dim3(ge), dim3(kBlock), 0, st, nt, enode.p, ckrw.p, etai_n.p, ckrwe.p, et1e.p);
     hipLaunchKernelGGL(k_bc_rows, dim3(gn), dim3(kBlock), 0, st, n, row_surf.p, qnode.p, ifatm.p, atmact.p,
-                       atmold.p, prm.tetaf, isdir.p, bcflux.p);
+                       atmold.p, prm.tetaf, lists_on() ? bcfix.p : nullptr, isdir.p, bcflux.p);
     AsmArgs a{};
     a.n = n; a.sl = sell(); a.ckrwe = ckrwe.p; a.ptnew = ptnew.p; a.pnew = pnew.p; a.ptimep = ptimep.p;
     a.isdir = isdir.p; a.bcflux = bcflux.p; a.tetaf = prm.tetaf; a.rdt = 1.0 / deltat; a.A = A.p;
@@ -1295,6 +1375,7 @@ struct Flow {
     a.pnew = pnew.p; a.ptimep = ptimep.p; a.swnew = swnew.p; a.swtimep = swtimep.p; a.et2 = et2.p;
     a.isdir = isdir.p; a.bcflux = bcflux.p; a.tetaf = prm.tetaf; a.rdt = 1.0 / deltat; a.A = A.p;
     a.tnoti = tnoti.p; a.xt5 = xt5.p;
+    a.qtr = veg_on ? qtr.p : nullptr;
     return a;
   }
   AsmPlan asm_plan() const { return AsmPlan{plan_off.p, plan_ie.p, plan_io.p, plan_s4.p, plan_w2.p, plan_gw.p}; }
@@ -1308,7 +1389,7 @@ struct Flow {
       hipLaunchKernelGGL(k_elem_avg, dim3(ge), dim3(kBlock), 0, st, nt, enode.p, ckrw.p, et1.p, ckrwe.p,
                          et1e.p);
     hipLaunchKernelGGL(k_bc_rows, dim3(gn), dim3(kBlock), 0, st, n, row_surf.p, qnode.p, ifatm.p, atmact.p,
-                       atmold.p, prm.tetaf, isdir.p, bcflux.p);
+                       atmold.p, prm.tetaf, lists_on() ? bcfix.p : nullptr, isdir.p, bcflux.p);
     const AsmArgs a = asm_args();
     const int ga = grid_for(n, kBlock);
     if (use_plan) {
@@ -1328,6 +1409,7 @@ struct Flow {
     after_solve(niter);
     hipLaunchKernelGGL(k_update_pnew, dim3(gn), dim3(kBlock), 0, st, n, pdiff.p, row_surf.p, ifatm.p, pold.p,
                        pnew.p);
+    shlpic_lists();
   }
   int last_pcg = 0;
   // PCG launch batches: the first covers pcg_first_pct % of the last solve's
@@ -1341,7 +1423,8 @@ struct Flow {
   // MASBAL (masbal.f:37-107) + NORMS (norms.f); result in h_res.
   void masbal_norms() {
     masbal_launch();
-    CG_HIP(hipMemcpyAsync(h_res.data(), res.p, sizeof(double) * 16, hipMemcpyDeviceToHost, st));
+    CG_HIP(hipMemcpyAsync(h_res.data(), res.p, sizeof(double) * (nsf_nodes > 0 ? 24 : 16), hipMemcpyDeviceToHost,
+                          st));
     CG_HIP(hipStreamSynchronize(st));
     masbal_host(h_res.data());
   }
@@ -1361,13 +1444,28 @@ struct Flow {
     hipLaunchKernelGGL(k_flux, dim3(Gf), dim3(kBlock), 0, st, nnod, ifatm.p, atmact.p, qbot.p,
                        prm.free_drainage ? nnod : 0, P(11));
     hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(kBlock), 0, st, G, Gf, np, P(11), res.p);
+    if (np_cur > 0) {  // bkpic.f:38-41 at the fixed-head nodes, fluxmb.f:30-37 -> res[13], res[14]
+      hipLaunchKernelGGL(k_bkpic_rows, dim3(grid_for(np_cur)), dim3(kBlock), 0, st, np_cur, dir_rows.p,
+                         (const int *)nullptr, sell(), A.p, pdiff.p, xt5.p, prm.tetaf, qpold.p, qpnew.p);
+      hipLaunchKernelGGL(k_list_flux, dim3(1), dim3(kBlock), 0, st, np_cur, qpnew.p, (const int *)nullptr,
+                         res.p + 13);
+    }
+    if (nsf_nodes > 0) {  // bkpic.f:42-49 SFQ, fluxmb.f:53-62 SFFLW -> res[16..17], then CONVER's
+      // EXTALL / EXTCVG (after NORMS) -> res[18] = KSF
+      hipLaunchKernelGGL(k_bkpic_rows, dim3(grid_for(nsf_nodes)), dim3(kBlock), 0, st, nsf_nodes, sf_rows.p,
+                         sfex.p, sell(), A.p, pdiff.p, xt5.p, prm.tetaf, sfqp.p, sfq.p);
+      hipLaunchKernelGGL(k_list_flux, dim3(1), dim3(kBlock), 0, st, nsf_nodes, sfq.p, sfex.p, res.p + 16);
+      hipLaunchKernelGGL(k_extall, dim3(1), dim3(kBlock), 0, st, nsf_nodes, sf_rows.p, sfex.p, sfexit.p, sfq.p,
+                         pnew.p, bcfix.p, res.p + 18);
+    }
   }
   // the host part of MASBAL on the reduced values R[16] (k_norms_final)
   void masbal_host(const double *R) {
     dstore = R[0];
     pl2 = R[1]; fl2 = R[2]; finf = R[3]; pinf = R[4]; pikmax = R[5]; ikmax = (int)R[6];
-    const double adin = R[7], adout = R[8], anin = R[9], anout = R[10], nnin = R[11], nnout = R[12];
-    const double ndin = 0.0, ndout = 0.0;
+    const double adin = R[7], adout = R[8], anin = R[9], anout = R[10];
+    const double nnin = R[11] + nx_in, nnout = R[12] + nx_out;  // free drainage, then the listed fluxes
+    const double ndin = np_cur > 0 ? R[13] : 0.0, ndout = np_cur > 0 ? R[14] : 0.0;
     const double deltam = 0.5 * deltat;
     const double vadin = (adin + adinp) * deltam, vadout = (adout + adoutp) * deltam;
     const double vndin = (ndin + ndinp) * deltam, vndout = (ndout + ndoutp) * deltam;
@@ -1375,10 +1473,260 @@ struct Flow {
     const double vnnin = (nnin + nninp) * deltam, vnnout = (nnout + nnoutp) * deltam;
     vin = vadin + vndin + vanin + vnnin;
     vout = vadout + vndout + vanout + vnnout;
+    if (nsf_nodes > 0) {  // masbal.f:69,73: VSFFLW = (SFFLW + SFFLWP) DELTAT/2
+      sfflw_cur = R[16] + R[17];
+      vout = vout + (sfflw_cur + sfflwp) * deltam;
+      ksf = (int)R[18];
+    }
     erras = vin + vout - dstore;
     errel = (vin + vout) != 0.0 ? 100.0 * erras / (vin + vout) : 0.0;
     cur[0] = adin; cur[1] = adout; cur[2] = ndin; cur[3] = ndout;
     cur[4] = anin; cur[5] = anout; cur[6] = nnin; cur[7] = nnout;
+  }
+
+  // ---------------------------------------- listed-node boundary series --
+  // KIND 0: fixed heads (nansfdirbc, PRESC in m), 1: Neumann fluxes beyond
+  // the base nodes of free drainage (nansfneubc, Q in m^3/s).  nodes are
+  // 1-based CATHY ids; records as bcfiles.read_bc expands them.
+  void set_bc(int kind, int nrec, const double *times, const int *counts, const int *nodes, const double *vals) {
+    CG_REQUIRE(kind == 0 || kind == 1, "KIND is 0 (Dirichlet) or 1 (Neumann)");
+    CG_REQUIRE(prm.iopt == 1, "listed boundary conditions need the Picard scheme (IOPT=1)");
+    CG_REQUIRE(nrec >= 1 && times && counts, "a series needs at least one record");
+    CG_REQUIRE(times[0] <= 0.0, "the first record of a series starts at time 0");
+    BcSeries b;
+    b.off.push_back(0);
+    std::vector<char> seen(kind == 0 ? n : 0, 0);
+    int e = 0;
+    for (int k = 0; k < nrec; ++k) {
+      CG_REQUIRE(counts[k] >= 0, "negative record length");
+      CG_REQUIRE(k == 0 || times[k] >= times[k - 1], "record times must not decrease");
+      CG_REQUIRE(counts[k] == 0 || (nodes && vals), "null node or value list");
+      for (int i = 0; i < counts[k]; ++i, ++e) {
+        const int nd = nodes[e] - 1;
+        CG_REQUIRE(nd >= 0 && nd < n, "boundary node id out of 1..N");
+        if (kind == 0) {
+          CG_REQUIRE(!seen[nd], "a fixed-head node is listed twice in one record");
+          seen[nd] = 1;
+        }
+        b.node.push_back(nd);
+        b.row.push_back(perm[nd]);
+        b.val.push_back(vals[e]);
+      }
+      if (kind == 0)
+        for (int i = b.off.back(); i < (int)b.node.size(); ++i) seen[b.node[i]] = 0;
+      b.off.push_back((int)b.node.size());
+      b.t.push_back(times[k]);
+    }
+    (kind == 0 ? bc_dir : bc_neu) = std::move(b);
+    initialised = 0;  // the run starts over at the next flow3d_init
+  }
+  // vegetation of the surface nodes (1-based types) and its Feddes table;
+  // z(N) in CATHY numbering.  Root zones must end above the base.
+  void set_vegetation(int nveg, const int *veg_type, const double *const tab[6], const double *z) {
+    CG_REQUIRE(prm.iopt == 1, "root uptake needs the Picard scheme (IOPT=1)");
+    CG_REQUIRE(nveg >= 1 && veg_type && z, "null argument");
+    for (int i = 0; i < 6; ++i) CG_REQUIRE(tab[i], "null vegetation table");
+    std::vector<int> hv(nnod);
+    for (int s = 0; s < nnod; ++s) {
+      const int v = veg_type[s] - 1;
+      CG_REQUIRE(v >= 0 && v < nveg, "vegetation type out of 1..NVEG");
+      const double zr = tab[3][v];
+      CG_REQUIRE(zr > 0.0, "ZROOT must be > 0");
+      CG_REQUIRE(z[s] - z[(long)nstr * nnod + s] > zr, "a root zone reaches the base of its column (etran.f:67-72)");
+      for (int l = 0; l < nstr; ++l)
+        CG_REQUIRE(z[(long)l * nnod + s] > z[(long)(l + 1) * nnod + s], "z must decrease down each column");
+      hv[s] = v;
+    }
+    std::vector<double> ht(6 * (size_t)nveg);
+    for (int i = 0; i < 6; ++i)
+      for (int v = 0; v < nveg; ++v) ht[(size_t)i * nveg + v] = tab[i][v];
+    vegc.upload(hv, st);
+    vegtab.upload(ht, st);
+    zc.upload(std::vector<double>(z, z + n), st);
+    permd.upload(perm, st);
+    qtr.alloc(n);
+    CG_HIP(hipMemsetAsync(qtr.p, 0, sizeof(double) * n, st));
+    CG_HIP(hipStreamSynchronize(st));
+    n_veg = nveg;
+    veg_on = true;
+    initialised = 0;  // the run starts over at the next flow3d_init
+  }
+  // time-invariant seepage faces: NSF faces of COUNTS(i) 1-based nodes each,
+  // top to bottom; ISFCVG = 1 makes the exit points part of the Picard
+  // convergence test (inital.f:218-224)
+  void set_seepage(int nsf, const int *counts, const int *nodes, int isfcvg_) {
+    CG_REQUIRE(prm.iopt == 1, "seepage faces need the Picard scheme (IOPT=1)");
+    CG_REQUIRE(nsf >= 0 && (nsf == 0 || (counts && nodes)), "null argument");
+    CG_REQUIRE(isfcvg_ == 0 || isfcvg_ == 1, "ISFCVG is 0 or 1");
+    std::vector<int> nd, off{0};
+    std::vector<char> seen(n, 0);
+    int e = 0;
+    for (int f = 0; f < nsf; ++f) {
+      CG_REQUIRE(counts[f] >= 1, "a seepage face needs at least one node");
+      for (int j = 0; j < counts[f]; ++j, ++e) {
+        const int v = nodes[e] - 1;
+        CG_REQUIRE(v >= 0 && v < n, "seepage node id out of 1..N");
+        CG_REQUIRE(!seen[v], "a node lies on two seepage faces or twice on one");
+        seen[v] = 1;
+        nd.push_back(v);
+      }
+      off.push_back((int)nd.size());
+    }
+    sf_node = std::move(nd);
+    sf_off = std::move(off);
+    nsf_nodes = (int)sf_node.size();
+    isfcvg = isfcvg_;
+    initialised = 0;  // the run starts over at the next flow3d_init
+  }
+  int n_veg = 0;
+  VegTab veg_tab() const {
+    const double *b = vegtab.p;
+    return VegTab{b, b + n_veg, b + 2 * n_veg, b + 3 * n_veg, b + 4 * n_veg, b + 5 * n_veg};
+  }
+  // etran.f, once per time step before SWITCH_OLD (cathy_main.f:2929)
+  void etran() {
+    if (!veg_on) return;
+    hipLaunchKernelGGL(k_etran, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, nstr, permd.p, zc.p, pnew.p,
+                       atmpot.p, vegc.p, veg_tab(), prm.scf, qtr.p);
+  }
+  // device buffers of the lists, sized for the longest record
+  std::vector<int> pos_tmp;
+  void alloc_lists() {
+    if (!lists_on()) return;
+    bcfix.alloc(n);
+    CG_HIP(hipMemsetAsync(bcfix.p, 0, n, st));
+    CG_HIP(hipMemsetAsync(qnode.p, 0, sizeof(double) * n, st));
+    const int md = std::max(1, bc_dir.max_count()), mq = std::max(1, bc_neu.max_count());
+    dir_rows.alloc(md); dir_vals.alloc(md); qpnew.alloc(md); qpold.alloc(md);
+    CG_HIP(hipMemsetAsync(qpnew.p, 0, sizeof(double) * md, st));
+    CG_HIP(hipMemsetAsync(qpold.p, 0, sizeof(double) * md, st));
+    neu_rows.alloc(mq); neu_vals.alloc(mq);
+    atm_off.alloc(std::max(1, std::min(nnod, md + mq)));
+    pos_tmp.assign(n, -1);
+    np_cur = nx_cur = n_atm_off = 0;
+    if (nsf_nodes > 0) {
+      std::vector<int> rows(nsf_nodes);
+      for (int i = 0; i < nsf_nodes; ++i) rows[i] = perm[sf_node[i]];
+      sf_rows.upload(rows, st);
+      for (DBuf<int> *b : {&sfex, &sfexit, &sfexp}) b->alloc(nsf_nodes);
+      sfq.alloc(nsf_nodes);
+      sfqp.alloc(nsf_nodes);
+      CG_HIP(hipMemsetAsync(sfq.p, 0, sizeof(double) * nsf_nodes, st));
+      CG_HIP(hipMemsetAsync(sfqp.p, 0, sizeof(double) * nsf_nodes, st));
+    }
+    sfflw_cur = sfflwp = 0.0;
+    ksf = 0;
+  }
+  // sfinit.f:22-45: a seepage node at or above 1e-8 m of head starts as an
+  // exit node with PTIMEP = PNEW = 0 (hp: PNEW = PTIMEP by CATHY id order of
+  // device rows)
+  void sfinit(std::vector<double> &hp) {
+    if (nsf_nodes == 0) return;
+    std::vector<int> e(nsf_nodes, 0);
+    for (int i = 0; i < nsf_nodes; ++i) {
+      const int r = perm[sf_node[i]];
+      if (hp[r] >= 1.0e-8) {
+        e[i] = 1;
+        hp[r] = 0.0;
+      }
+    }
+    sfex.upload(e, st);
+    sfexit.upload(e, st);
+    sfexp.upload(e, st);
+    hipLaunchKernelGGL(k_sync_exit, dim3(grid_for(nsf_nodes)), dim3(kBlock), 0, st, nsf_nodes, sf_rows.p, sfex.p,
+                       bcfix.p);
+    CG_HIP(hipStreamSynchronize(st));
+  }
+  // the active fixed-head record to the device: clear the old rows' bit,
+  // mark the new rows, PRESC by list position
+  void apply_dir() {
+    if (!bc_dir.set()) return;
+    if (np_cur > 0)
+      hipLaunchKernelGGL(k_mark_rows, dim3(grid_for(np_cur)), dim3(kBlock), 0, st, np_cur, dir_rows.p, 2, 0, bcfix.p);
+    const int k = bc_dir.act, m = bc_dir.count(k);
+    if (m > 0) {
+      h_dir_rows.assign(bc_dir.row.begin() + bc_dir.off[k], bc_dir.row.begin() + bc_dir.off[k + 1]);
+      h_dir_vals.assign(bc_dir.val.begin() + bc_dir.off[k], bc_dir.val.begin() + bc_dir.off[k + 1]);
+      dir_rows.upload(h_dir_rows.data(), m, st);
+      dir_vals.upload(h_dir_vals.data(), m, st);
+      hipLaunchKernelGGL(k_mark_rows, dim3(grid_for(m)), dim3(kBlock), 0, st, m, dir_rows.p, 2, 1, bcfix.p);
+    }
+    np_cur = m;
+  }
+  std::vector<double> h_dir_vals, h_neu_vals;
+  // the active Neumann record: Q by list position (neumann.f:25-31 takes the
+  // window's second record for the listed nodes under free drainage), summed
+  // per row in list order; the old rows' fluxes are cleared first
+  void apply_neu() {
+    if (!bc_neu.set()) return;
+    if (nx_cur > 0)
+      hipLaunchKernelGGL(k_set_rows, dim3(grid_for(nx_cur)), dim3(kBlock), 0, st, nx_cur, neu_rows.p,
+                         (const double *)nullptr, 0.0, qnode.p);
+    const int k = bc_neu.act, m = bc_neu.count(k);
+    const int k2 = bc_neu.rec[1], m2 = bc_neu.count(k2);
+    h_neu_rows.clear();
+    h_neu_vals.clear();
+    nx_in = nx_out = 0.0;
+    for (int e = 0; e < m; ++e) {
+      const double v = (prm.free_drainage && e < m2) ? bc_neu.val[bc_neu.off[k2] + e] : bc_neu.val[bc_neu.off[k] + e];
+      const int r = bc_neu.row[bc_neu.off[k] + e];
+      if (v > 0.0) nx_in += v; else nx_out += v;
+      if (pos_tmp[r] < 0) {
+        pos_tmp[r] = (int)h_neu_rows.size();
+        h_neu_rows.push_back(r);
+        h_neu_vals.push_back(v);
+      } else {
+        h_neu_vals[pos_tmp[r]] = h_neu_vals[pos_tmp[r]] + v;
+      }
+    }
+    for (int r : h_neu_rows) pos_tmp[r] = -1;
+    nx_cur = (int)h_neu_rows.size();
+    neu_rows.upload(h_neu_rows.data(), nx_cur, st);
+    neu_vals.upload(h_neu_vals.data(), nx_cur, st);
+  }
+  // surface nodes of the active lists: IFATM = -1 (atmnxt.f:80-102)
+  std::vector<int> h_atm_off;
+  void collect_atm_off() {
+    h_atm_off.clear();
+    auto take = [&](const BcSeries &b) {
+      if (!b.set() || b.act < 0) return;
+      for (int e = b.off[b.act]; e < b.off[b.act + 1]; ++e)
+        if (b.node[e] < nnod) h_atm_off.push_back(b.node[e]);
+    };
+    take(bc_dir);
+    take(bc_neu);
+    for (int v : sf_node)
+      if (v < nnod) h_atm_off.push_back(v);
+    std::sort(h_atm_off.begin(), h_atm_off.end());
+    h_atm_off.erase(std::unique(h_atm_off.begin(), h_atm_off.end()), h_atm_off.end());
+    n_atm_off = (int)h_atm_off.size();
+    if (n_atm_off > (int)atm_off.n) atm_off.alloc(n_atm_off);
+    atm_off.upload(h_atm_off.data(), n_atm_off, st);
+  }
+  void atm_off_apply(int *fa) {
+    if (n_atm_off > 0)
+      hipLaunchKernelGGL(k_set_int, dim3(grid_for(n_atm_off)), dim3(kBlock), 0, st, n_atm_off, atm_off.p, -1, fa);
+  }
+  // NEUMANN (neumann.f) and the listed fluxes into qnode: the listed rows
+  // restart from zero, free drainage writes the base rows, the listed
+  // fluxes are added
+  void neumann_apply(const double *ck) {
+    if (nx_cur > 0)
+      hipLaunchKernelGGL(k_set_rows, dim3(grid_for(nx_cur)), dim3(kBlock), 0, st, nx_cur, neu_rows.p,
+                         (const double *)nullptr, 0.0, qnode.p);
+    free_drainage(ck);
+    if (nx_cur > 0)
+      hipLaunchKernelGGL(k_add_rows, dim3(grid_for(nx_cur)), dim3(kBlock), 0, st, nx_cur, neu_rows.p, neu_vals.p,
+                         qnode.p);
+  }
+  // SHLPIC's PNEW(CONTP) = PRESC (shlpic.f:30-35)
+  void shlpic_lists() {
+    if (np_cur > 0)
+      hipLaunchKernelGGL(k_set_rows, dim3(grid_for(np_cur)), dim3(kBlock), 0, st, np_cur, dir_rows.p, dir_vals.p,
+                         0.0, pnew.p);
+    if (nsf_nodes > 0)
+      hipLaunchKernelGGL(k_zero_exit, dim3(grid_for(nsf_nodes)), dim3(kBlock), 0, st, nsf_nodes, sf_rows.p,
+                         sfex.p, pnew.p);
   }
 
   void switch_old() {
@@ -1416,12 +1764,16 @@ struct Flow {
     const bool itagen = iter < prm.ituns;
     const bool errgmx = pl2 >= prm.ernlmx || pinf >= prm.ernlmx || fl2 >= prm.ernlmx || finf >= prm.ernlmx;
     const bool normcv = prm.l2norm == 0 ? pinf <= prm.toluns : pl2 <= prm.toluns;
-    if (!lsfail && !errgmx && !normcv && itagen) {
+    // seepage exit points: SFCHEK (inital.f:218-224), KSFZER (conver.f:90-98)
+    const bool sfchek = isfcvg == 1 && nsf_nodes > 0, ksfzer = ksf == 0;
+    if ((!lsfail && !errgmx && !normcv && itagen) || (!lsfail && !errgmx && itagen && sfchek && !ksfzer)) {
+      if (nsf_nodes > 0)  // flow3d.f:241-246: SFEXIT = SFEX
+        CG_HIP(hipMemcpyAsync(sfexit.p, sfex.p, sizeof(int) * nsf_nodes, hipMemcpyDeviceToDevice, st));
       iter += 1;
       return true;
     }
     itrtot += iter;
-    if (!lsfail && !errgmx && normcv) {
+    if (!lsfail && !errgmx && normcv && (!sfchek || ksfzer)) {
       noback = 0;
     } else if (dtgmin) {
       bckstp = 1;
@@ -1536,6 +1888,12 @@ struct Flow {
     finished = 0; noback = 0; bckstp = 0;
     store1 = store2 = dstore = cdstor = cvin = cvout = vtot = cerras = caeras = 0.0;
     CG_HIP(hipMemcpyAsync(pnew.p, ptimep.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    // BCONE of the fixed-head and Neumann series at TIME = DELTAT (inital.f:160-181)
+    CG_REQUIRE(!(lists_on() || veg_on) || prm.iopt == 1, "listed boundary conditions and root uptake need IOPT=1");
+    alloc_lists();
+    if (bc_dir.set()) bc_dir.bcone(time);
+    if (bc_neu.set()) bc_neu.bcone(time);
+    if (lists_on()) collect_atm_off();
     // ATMONE (atmone.f:39-163), host side on the surface values
     const int w = arow();
     atminp.assign((size_t)3 * w, 0.0);
@@ -1571,9 +1929,12 @@ struct Flow {
         apot[s] = rt * are[s];
       }
     }
+    sfinit(hp);                    // inital.f:226-229, ahead of ATMONE
     std::vector<double> hpt = hp;  // PTIMEP == PNEW here
+    for (int s : h_atm_off) fa[s] = fap[s] = -1;  // atmone.f:87-109, ahead of the classification
     for (int s = 0; s < nnod; ++s) {
       const int d = perm[s];
+      if (fa[s] == -1) continue;
       if (hp[d] >= prm.pondh_min) { fa[s] = 2; fap[s] = 2; continue; }
       if (hp[d] >= 0.0 && apot[s] > 0.0) fa[s] = 1;
       if (hpt[d] >= 0.0 && aold[s] > 0.0) fap[s] = 1;
@@ -1605,13 +1966,17 @@ struct Flow {
     store2 = store1;
     CG_HIP(hipMemcpyAsync(ckrwp.p, ckrw.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     nninp = nnoutp = 0.0;
+    apply_dir();
+    apply_neu();
+    neumann_apply(ckrw.p);
     if (prm.free_drainage) {
-      free_drainage(ckrw.p);
       std::vector<double> q(nnod);
       qbot.download(q.data(), nnod, st);
       CG_HIP(hipStreamSynchronize(st));
       for (int s = 0; s < nnod; ++s) { if (q[s] > 0.0) nninp += q[s]; else nnoutp += q[s]; }
     }
+    nninp += nx_in;  // mbinit.f:44-51 / cathy_main.f:2696-2706 over the listed fluxes too
+    nnoutp += nx_out;
     // SWTIMEP is a function of PTIMEP only
     hipLaunchKernelGGL(k_sw_kr, dim3(gn), dim3(kBlock), 0, st, n, ptimep.p, soil(), swtimep.p,
                        (double *)nullptr);
@@ -1621,8 +1986,16 @@ struct Flow {
     initialised = 1;
   }
 
-  void bkstep() {  // bkstep.f:58-147 (constant non-atmospheric BCs, NSF=0)
+  void bkstep() {  // bkstep.f:58-147 (NSF=0)
     CG_HIP(hipMemcpyAsync(pnew.p, ptimep.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    if (np_cur > 0) CG_HIP(hipMemcpyAsync(qpnew.p, qpold.p, sizeof(double) * np_cur, hipMemcpyDeviceToDevice, st));
+    if (nsf_nodes > 0) {  // bkstep.f:56-67: SFEX = SFEXIT = SFEXP, SFQ = SFQP
+      CG_HIP(hipMemcpyAsync(sfex.p, sfexp.p, sizeof(int) * nsf_nodes, hipMemcpyDeviceToDevice, st));
+      CG_HIP(hipMemcpyAsync(sfexit.p, sfexp.p, sizeof(int) * nsf_nodes, hipMemcpyDeviceToDevice, st));
+      CG_HIP(hipMemcpyAsync(sfq.p, sfqp.p, sizeof(double) * nsf_nodes, hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_sync_exit, dim3(grid_for(nsf_nodes)), dim3(kBlock), 0, st, nsf_nodes, sf_rows.p, sfex.p,
+                         bcfix.p);
+    }
     CG_HIP(hipMemcpyAsync(ifatm.p, ifatmp.p, sizeof(int) * nnod, hipMemcpyDeviceToDevice, st));
     CG_HIP(hipMemcpyAsync(atmact.p, atmold.p, sizeof(double) * nnod, hipMemcpyDeviceToDevice, st));
     time = timep;
@@ -1631,7 +2004,19 @@ struct Flow {
     time = time + deltat;
     iter = 1;
     nitert = 0;
-    if (time > atmtim[1]) atmnxt(); else atmbak();
+    if (lists_on()) {  // bkstep.f:73-88: BCNXT past the window's second time, else BCBAK
+      if (bc_dir.set()) bc_dir.back(time);
+      if (bc_neu.set()) bc_neu.back(time);
+      apply_dir();
+      apply_neu();
+      collect_atm_off();
+    }
+    if (time > atmtim[1]) {
+      atmnxt();
+      atm_off_apply(ifatm.p);
+    } else {
+      atmbak();
+    }
     switch_old();
     guess_point();
     hipLaunchKernelGGL(k_start_iter, dim3(grid_for(n)), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
@@ -1665,8 +2050,17 @@ struct Flow {
     step_pcg0 = c_pcg;
     step_pic0 = c_picard;
     step_backs = 0;
-    free_drainage(ckrw.p);
+    if (lists_on()) {  // BCNXT of the fixed-head and Neumann series (cathy_main.f:2892-2903)
+      if (bc_dir.set()) bc_dir.bcnxt(time);
+      if (bc_neu.set()) bc_neu.bcnxt(time);
+      apply_dir();
+      apply_neu();
+      collect_atm_off();
+    }
+    neumann_apply(ckrw.p);
     atmnxt();
+    atm_off_apply(ifatm.p);
+    etran();
     switch_old();
     guess_point();
     hipLaunchKernelGGL(k_start_iter, dim3(grid_for(n)), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
@@ -1675,7 +2069,7 @@ struct Flow {
   // a failed attempt: BKSTEP and the free-drainage fluxes of the restored state
   void back_step() {
     bkstep();
-    free_drainage(ckrwp.p);
+    neumann_apply(ckrwp.p);
     ++step_backs;
   }
   // end of an accepted step: CHVELO, STORCAL, the cumulative balance,
@@ -1704,6 +2098,13 @@ struct Flow {
     }
     adinp = cur[0]; adoutp = cur[1]; ndinp = cur[2]; ndoutp = cur[3];
     aninp = cur[4]; anoutp = cur[5]; nninp = cur[6]; nnoutp = cur[7];
+    if (np_cur > 0) CG_HIP(hipMemcpyAsync(qpold.p, qpnew.p, sizeof(double) * np_cur, hipMemcpyDeviceToDevice, st));
+    if (nsf_nodes > 0) {  // cathy_main.f:3282-3299
+      sfflwp = sfflw_cur;
+      CG_HIP(hipMemcpyAsync(sfexp.p, sfex.p, sizeof(int) * nsf_nodes, hipMemcpyDeviceToDevice, st));
+      CG_HIP(hipMemcpyAsync(sfexit.p, sfex.p, sizeof(int) * nsf_nodes, hipMemcpyDeviceToDevice, st));
+      CG_HIP(hipMemcpyAsync(sfqp.p, sfq.p, sizeof(double) * nsf_nodes, hipMemcpyDeviceToDevice, st));
+    }
     CG_HIP(hipMemcpyAsync(ifatmp.p, ifatm.p, sizeof(int) * nnod, hipMemcpyDeviceToDevice, st));
     CG_HIP(hipMemcpyAsync(atmold.p, atmact.p, sizeof(double) * nnod, hipMemcpyDeviceToDevice, st));
     if (grecord) guess_target(st, nnod, gL, gLp, gcolrow.p, ptimep.p, pnew.p, gtarget.p);

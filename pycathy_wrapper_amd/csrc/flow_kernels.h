@@ -159,6 +159,7 @@ struct AsmArgs {
   const double *ptnew, *pnew, *ptimep, *swnew, *swtimep, *et2;
   const uint8_t *isdir;
   const double *bcflux;
+  const double *qtr;  // root uptake QTRANIE per row (null: none)
   double tetaf, rdt;
   double *A, *tnoti, *xt5;
 };
@@ -169,8 +170,27 @@ __global__ void k_sw_kr(int n, const double *psi_in, SoilSoA s, double *sw, doub
 __global__ void k_elem_avg(int nt, const int4 *enode, const double *ckrw, const double *et1, double *ckrwe,
                            double *et1e);
 __global__ void k_bc_rows(int n, const int *row_surf, const double *qnode, const int *ifatm,
-                          const double *atmact, const double *atmold, double tetaf, uint8_t *isdir,
-                          double *bcflux);
+                          const double *atmact, const double *atmold, double tetaf, const uint8_t *bcfix,
+                          uint8_t *isdir, double *bcflux);
+// listed-node boundary conditions (fixed head, Neumann lists, seepage faces)
+__global__ void k_set_rows(int m, const int *rows, const double *vals, double v0, double *x);
+__global__ void k_add_rows(int m, const int *rows, const double *vals, double *x);
+__global__ void k_mark_rows(int m, const int *rows, int bit, int set, uint8_t *bcfix);
+__global__ void k_set_int(int m, const int *idx, int v, int *x);
+__global__ void k_bkpic_rows(int m, const int *rows, const int *mask, Sell sl, const double *A,
+                             const double *pdiff, const double *xt5, double tetaf, const double *qold,
+                             double *qnew);
+__global__ void k_list_flux(int m, const double *q, const int *mask, double *out2);
+__global__ void k_extall(int m, const int *rows, int *sfex, const int *sfexit, double *sfq, double *pnew,
+                         uint8_t *bcfix, double *out);
+__global__ void k_sync_exit(int m, const int *rows, const int *sfex, uint8_t *bcfix);
+__global__ void k_zero_exit(int m, const int *rows, const int *sfex, double *pnew);
+// Feddes root-uptake parameters per vegetation type (SOILCHAR.H /FEDDES/)
+struct VegTab {
+  const double *pcana, *pcref, *pcwlt, *zroot, *pz, *omgc;
+};
+__global__ void k_etran(int nnod, int nstr, const int *perm, const double *zc, const double *pnew,
+                        const double *atmpot, const int *vegc, VegTab vt, double scf, double *qtr);
 template <int MR, bool LUMPED>
 __global__ void k_assemble(AsmArgs a);
 

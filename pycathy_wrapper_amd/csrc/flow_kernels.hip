@@ -64,17 +64,24 @@ __global__ void k_elem_avg(int nt, const int4 *__restrict__ enode,
   et1e[e] = b * 0.25;
 }
 
-// bcpic.f:34-87 classification of each row: Dirichlet (atmospheric nodes
-// with IFATM = 1 or 2) or the flux it receives (Neumann Q, atmospheric
-// TETAF*ATMACT + (1-TETAF)*ATMOLD for IFATM = 0).
+// bcpic.f:34-87 classification of each row: Dirichlet (fixed-head nodes,
+// seepage-face exit nodes and atmospheric nodes with IFATM = 1 or 2) or not,
+// and the flux it receives: the Neumann Q of its node (free drainage and
+// listed nodes, bcpic.f:71-74) plus, at IFATM = 0 surface nodes, TETAF*ATMACT
+// + (1-TETAF)*ATMOLD (bcpic.f:75-80).  bcfix (null: none) holds per row bit
+// 2 = fixed head (the CONTP list), bit 4 = seepage face with SFEX = 1.
+// isdir = 1 marks the Dirichlet rows (NODDIR of bcpic.f), which GRADDP
+// and RESSYM leave out of their norms (solscal-extended.f:1287-1297,
+// 4400-4437: SYMSLV passes them NUMDIR / NODDIR).
 __device__ __forceinline__ void bc_rows_body(Blk bk, int n, const int *__restrict__ row_surf,
                           const double *__restrict__ qnode, const int *__restrict__ ifatm,
                           const double *__restrict__ atmact, const double *__restrict__ atmold,
-                          double tetaf, uint8_t *__restrict__ isdir,
+                          double tetaf, const uint8_t *__restrict__ bcfix, uint8_t *__restrict__ isdir,
                           double *__restrict__ bcflux) {
   int r = bk.b * blockDim.x + threadIdx.x;
   if (r >= n) return;
   uint8_t d = 0;
+  if (bcfix && (bcfix[r] & 6)) d = 1;
   double f = qnode[r];
   int s = row_surf[r];
   if (s >= 0) {
@@ -88,9 +95,19 @@ __device__ __forceinline__ void bc_rows_body(Blk bk, int n, const int *__restric
 __global__ void k_bc_rows(int n, const int *__restrict__ row_surf,
                           const double *__restrict__ qnode, const int *__restrict__ ifatm,
                           const double *__restrict__ atmact, const double *__restrict__ atmold,
-                          double tetaf, uint8_t *__restrict__ isdir,
+                          double tetaf, const uint8_t *__restrict__ bcfix, uint8_t *__restrict__ isdir,
                           double *__restrict__ bcflux) {
-  bc_rows_body(self_blk(), n, row_surf, qnode, ifatm, atmact, atmold, tetaf, isdir, bcflux);
+  bc_rows_body(self_blk(), n, row_surf, qnode, ifatm, atmact, atmold, tetaf, bcfix, isdir, bcflux);
+}
+
+// BCPIC's right-hand side of row r from the assembled value t: Dirichlet
+// rows restart from zero (bcpic.f:36,48,62), every row then takes its
+// Neumann/atmospheric flux (bcpic.f:71-80) and loses its root uptake
+// QTRANIE (bcpic.f:84-86, null: none).
+__device__ __forceinline__ double bc_rhs(const AsmArgs &a, int r, double t) {
+  double v = (a.isdir[r] ? 0.0 : t) + a.bcflux[r];
+  if (a.qtr) v = v - a.qtr[r];
+  return v;
 }
 
 // Upper-pair slot of (k,l) in an element's 10 stiffness values.
@@ -170,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a) {
   }
   t = t - grav;
   a.xt5[r] = t;
-  a.tnoti[r] = a.isdir[r] ? 0.0 : t + a.bcflux[r];
+  a.tnoti[r] = bc_rhs(a, r, t);
   const double tetaf = a.tetaf;
   const double m5 = m4r * a.et2[r];
 #pragma unroll
@@ -323,7 +340,7 @@ __device__ __forceinline__ void assemble_plan_body(Blk bk, AsmArgs a, AsmPlan P,
   t = t - (m2d * rdt) * (a.pnew[r] - a.ptimep[r]) - (m4r * rdt) * (a.swnew[r] - a.swtimep[r]);
   t = t - grav;
   a.xt5[r] = t;
-  a.tnoti[r] = a.isdir[r] ? 0.0 : t + a.bcflux[r];
+  a.tnoti[r] = bc_rhs(a, r, t);
 }
 template <int MR>
 __global__ __launch_bounds__(kBlock) void k_assemble_plan(AsmArgs a, AsmPlan P, const double *__restrict__ ckrw_node,
@@ -1525,6 +1542,169 @@ __global__ void k_free_drainage(int nnod, const int *__restrict__ bot_row,
   qnode[r] = q;
 }
 
+// ------------------------------------ listed-node boundary conditions --
+// x[rows[i]] = vals[i] (vals null: v0).  SHLPIC's PNEW(CONTP) = PRESC and
+// seepage PNEW = 0 (shlpic.f:30-45); the rows of a list are distinct.
+__global__ void k_set_rows(int m, const int *__restrict__ rows, const double *__restrict__ vals, double v0,
+                           double *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) x[rows[i]] = vals ? vals[i] : v0;
+}
+// x[rows[i]] += vals[i] over distinct rows: the listed Neumann fluxes.
+__global__ void k_add_rows(int m, const int *__restrict__ rows, const double *__restrict__ vals,
+                           double *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) x[rows[i]] = x[rows[i]] + vals[i];
+}
+// set (set != 0) or clear `bit` of bcfix at the listed rows.
+__global__ void k_mark_rows(int m, const int *__restrict__ rows, int bit, int set, uint8_t *__restrict__ bcfix) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int r = rows[i];
+  bcfix[r] = set ? (uint8_t)(bcfix[r] | bit) : (uint8_t)(bcfix[r] & ~bit);
+}
+// x[idx[i]] = v: IFATM = -1 at surface nodes that carry another boundary
+// condition (atmone.f:101-117, atmnxt.f:80-102).
+__global__ void k_set_int(int m, const int *__restrict__ idx, int v, int *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) x[idx[i]] = v;
+}
+// bkpic.f:28-49 at listed rows: the back-calculated flux
+// QNEW = (A PDIFF - XT5 - (1-TETAF) QOLD) / TETAF of a fixed-head node
+// (QPNEW) or, where mask = 1, of a seepage-face exit node (SFQ).
+__global__ void k_bkpic_rows(int m, const int *__restrict__ rows, const int *__restrict__ mask, Sell sl,
+                             const double *__restrict__ A, const double *__restrict__ pdiff,
+                             const double *__restrict__ xt5, double tetaf, const double *__restrict__ qold,
+                             double *__restrict__ qnew) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  if (mask && mask[i] != 1) return;
+  const int r = rows[i];
+  const RowPos rp = row_pos(sl, r);
+  const int len = sell_len(sl, rp, r);
+  double v = 0.0;
+  for (int j = 0; j < len; ++j) {
+    const int q = sp(sl.so, r, j);
+    v += A[q] * pdiff[sell_col(sl, rp, q, j)];
+  }
+  v = v - xt5[r];
+  qnew[i] = (v - (1.0 - tetaf) * qold[i]) * (1.0 / tetaf);
+}
+// fluxmb.f:30-37 / :53-62: one work-group sums the positive and the
+// negative entries of q (where mask = 1, if given) into out2[0], out2[1].
+__global__ void k_list_flux(int m, const double *__restrict__ q, const int *__restrict__ mask,
+                            double *__restrict__ out2) {
+  __shared__ double sh[kBlock / 64 + 1];
+  double pos = 0.0, neg = 0.0;
+  for (int i = threadIdx.x; i < m; i += kBlock) {
+    if (mask && mask[i] != 1) continue;
+    const double v = q[i];
+    if (v > 0.0) pos += v; else neg += v;
+  }
+  const double tp = block_sum<kBlock>(pos, sh);
+  __syncthreads();
+  const double tn = block_sum<kBlock>(neg, sh);
+  if (threadIdx.x == 0) {
+    out2[0] = tp;
+    out2[1] = tn;
+  }
+}
+
+// extall.f:22-47 + extcvg.f:20-26, one work-group over the seepage-face
+// nodes (face by face, top to bottom): an exit node (SFEX = 1) whose
+// back-calculated flux SFQ turned inflowing (>= 0) leaves the exit set
+// (SFQ = 0); any other node whose head rose above 1e-8 joins it (PNEW = 0).
+// bcfix bit 4 follows SFEX; out[0] = KSF, the nodes whose SFEX differs
+// from SFEXIT (the previous iteration's).
+__global__ void k_extall(int m, const int *__restrict__ rows, int *__restrict__ sfex,
+                         const int *__restrict__ sfexit, double *__restrict__ sfq, double *__restrict__ pnew,
+                         uint8_t *__restrict__ bcfix, double *__restrict__ out) {
+  __shared__ double sh[kBlock / 64 + 1];
+  double ksf = 0.0;
+  for (int i = threadIdx.x; i < m; i += kBlock) {
+    const int r = rows[i];
+    int e = sfex[i];
+    if (e == 1) {
+      if (sfq[i] >= 0.0) {
+        e = 0;
+        sfq[i] = 0.0;
+      }
+    } else if (pnew[r] > 1.0e-8) {
+      pnew[r] = 0.0;
+      e = 1;
+    }
+    sfex[i] = e;
+    bcfix[r] = e == 1 ? (uint8_t)(bcfix[r] | 4) : (uint8_t)(bcfix[r] & ~4);
+    if (e != sfexit[i]) ksf += 1.0;
+  }
+  const double t = block_sum<kBlock>(ksf, sh);
+  if (threadIdx.x == 0) out[0] = t;
+}
+// bcfix bit 4 from SFEX (after a back-step restores SFEX).
+__global__ void k_sync_exit(int m, const int *__restrict__ rows, const int *__restrict__ sfex,
+                            uint8_t *__restrict__ bcfix) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int r = rows[i];
+  bcfix[r] = sfex[i] == 1 ? (uint8_t)(bcfix[r] | 4) : (uint8_t)(bcfix[r] & ~4);
+}
+// SHLPIC's PNEW = 0 at the exit nodes (shlpic.f:36-45).
+__global__ void k_zero_exit(int m, const int *__restrict__ rows, const int *__restrict__ sfex,
+                            double *__restrict__ pnew) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m && sfex[i] == 1) pnew[rows[i]] = 0.0;
+}
+
+// etran.f:15-80: Feddes root water uptake QTRANIE.  The potential
+// evaporation ETP = -SCF*ATMPOT (ATMPOT < 0, else 0) of a surface column is
+// spread over the root-zone nodes (depth d <= ZROOT of the column's
+// vegetation type) in proportion to BTRANI = max(0, beta dz g), beta =
+// (1 - d/ZROOT) exp(-PZ d/ZROOT), g the water-stress factor of PSI between
+// PCWLT, PCREF and PCANA:
+//   QTRANIE = ETP BTRANI / BTRAN / max(OMG, OMGC),
+// BTRAN = sum beta dz, OMG = sum g beta dz / BTRAN over the root zone.
+// One thread per column; zc and the columns' nodes k = (j-1) NNOD + i are
+// in CATHY numbering, perm maps them to rows.  Every root zone lies above
+// its column's base (checked on the host), so the layer loop ends before
+// the base node.
+__global__ void k_etran(int nnod, int nstr, const int *__restrict__ perm, const double *__restrict__ zc,
+                        const double *__restrict__ pnew, const double *__restrict__ atmpot,
+                        const int *__restrict__ vegc, VegTab vt, double scf, double *__restrict__ qtr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnod) return;
+  const int v = vegc[i];
+  const double zr = vt.zroot[v], pca = vt.pcana[v], pref = vt.pcref[v], pwlt = vt.pcwlt[v], pzv = vt.pz[v];
+  const double ap = atmpot[i];
+  const double etp = ap < 0.0 ? -1.0 * scf * ap : 0.0;
+  const double zsurf = zc[i];
+  double btran = 0.0, omg = 0.0, depth = 0.0;
+  int j = 1;
+  while (depth <= zr && j <= nstr) {
+    const int k = (j - 1) * nnod + i;
+    const double s1 = pca, s2 = pca + 1.0e-03;
+    const double dz = j == 1 ? (zsurf - zc[k + nnod]) / 2.0 : (zc[k - nnod] - zc[k + nnod]) / 2.0;
+    const double sh2o = pnew[perm[k]];
+    const double gx1 = fmin(1.0, fmax(0.0, (sh2o - pwlt) / (pref - pwlt)));
+    const double gx2 = fmin(1.0, fmax(0.0, 1.0 - (sh2o - s1) / (s2 - s1)));
+    const double gx = fmin(gx1, gx2);
+    const double beta = (1.0 - depth / zr) * exp(-1.0 * pzv * depth / zr);
+    qtr[perm[k]] = fmax(0.0, beta * dz * gx);  // BTRANI until the column's totals are known
+    btran = btran + beta * dz;
+    omg = omg + gx * beta * dz;
+    j = j + 1;
+    depth = zsurf - zc[(j - 1) * nnod + i];
+  }
+  const int jroot = j - 1;  // layers 1..jroot carry BTRANI
+  btran = fmax(0.0, btran);
+  omg = omg / btran;
+  const double den = fmax(omg, vt.omgc[v]);
+  for (int l = 1; l <= nstr + 1; ++l) {
+    const int r = perm[(l - 1) * nnod + i];
+    const double b = l <= jroot ? qtr[r] : 0.0;
+    qtr[r] = etp * b / btran / den;
+  }
+}
+
 // storcal.f partial: sum Sw * VOLNOD * PNODI.
 __device__ __forceinline__ void store_body(Blk bk, int n, const double *__restrict__ sw, const double *__restrict__ volnod,
                         const double *__restrict__ pnodi, double *__restrict__ part) {
@@ -1703,7 +1883,7 @@ __global__ void kb_constitutive(const int *start, int nmem, const MemberDev *md)
 }
 __global__ void kb_bc_rows(const int *start, int nmem, const MemberDev *md) {
   CG_BATCH_HEAD
-  bc_rows_body(bk, d.n, d.row_surf, d.qnode, d.ifatm, d.atmact, d.atmold, d.tetaf, d.isdir, d.bcflux);
+  bc_rows_body(bk, d.n, d.row_surf, d.qnode, d.ifatm, d.atmact, d.atmold, d.tetaf, nullptr, d.isdir, d.bcflux);
 }
 template <int MR>
 __global__ __launch_bounds__(kBlock) void kb_assemble_plan(const int *start, int nmem, const MemberDev *md) {

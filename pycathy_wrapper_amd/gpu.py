@@ -106,6 +106,14 @@ class FlowGPU:
         self.h = h
         self.set_params(problem.params)
         self.set_forcing(problem.atm_times, problem.atm_vals)
+        if getattr(problem, "bc_dir", None) is not None:
+            self.set_bc(problem.bc_dir)
+        if getattr(problem, "bc_neu", None) is not None:
+            self.set_bc(problem.bc_neu)
+        if getattr(problem, "veg", None) is not None:
+            self.set_vegetation(**problem.veg)
+        if getattr(problem, "seepage", None) is not None:
+            self.set_seepage(problem.seepage["faces"], problem.seepage.get("isfcvg", 0))
         self.init(problem.psi0)
 
     def set_params(self, p, hspatm: int | None = None):
@@ -126,6 +134,48 @@ class FlowGPU:
             raise ValueError(f"set_forcing: {v.size} values for {len(t)} records of width {width}")
         _check(lib().cathy_gpu_flow3d_set_forcing(C.byref(self.h), _i(len(t)), _p(t), _p(v)), "set_forcing")
 
+    def set_bc(self, series) -> None:
+        """A nansfdirbc / nansfneubc series (bcfiles.BCSeries) for the run;
+        call init again afterwards (the constructor does, for the problem's
+        own series)."""
+        kind = {"dirichlet": 0, "neumann": 1}[series.kind]
+        if kind == 1 and series.free_drainage != bool(self.problem.params.free_drainage):
+            raise ValueError("set_bc: the Neumann series and the free-drainage parameter disagree")
+        t, cnt, nodes, vals = series.explicit(self.nnod)
+        self._keep[f"bc{kind}"] = (t, cnt, nodes, vals)
+        _check(lib().cathy_gpu_flow3d_set_bc(C.byref(self.h), _i(kind), _i(len(t)), _p(t), _p(cnt), _p(nodes),
+                                             _p(vals)), "set_bc")
+
+    def set_seepage(self, faces, isfcvg: int = 0) -> None:
+        """Time-invariant seepage faces: a list of 0-based node arrays, each
+        face top to bottom (bcfiles.read_sfbc gives them); ISFCVG = 1 makes
+        the exit points part of the convergence test.  Call init again
+        afterwards."""
+        counts = np.array([len(f) for f in faces], dtype=np.int32)
+        nodes = (np.concatenate([np.asarray(f, dtype=np.int64) for f in faces]) + 1).astype(np.int32) \
+            if faces else np.zeros(0, dtype=np.int32)
+        self._keep["sf"] = (counts, nodes)
+        _check(lib().cathy_gpu_flow3d_set_seepage(C.byref(self.h), _i(len(faces)), _p(counts), _p(nodes),
+                                                  _i(isfcvg)), "set_seepage")
+
+    def seepage_state(self):
+        """(SFEX, SFQ) of the seepage nodes, face by face, top to bottom."""
+        m = _check(lib().cathy_gpu_flow3d_get_seepage(C.byref(self.h), None, None), "get_seepage")
+        ex, q = np.zeros(m, dtype=np.int32), np.zeros(m)
+        _check(lib().cathy_gpu_flow3d_get_seepage(C.byref(self.h), _p(ex), _p(q)), "get_seepage")
+        return ex, q
+
+    def set_vegetation(self, veg_type, pcana, pcref, pcwlt, zroot, pz, omgc) -> None:
+        """Feddes root uptake (ETRAN): vegetation type per surface node
+        (1-based) and the per-type table; call init again afterwards."""
+        f64 = lambda a: np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)
+        vt = np.ascontiguousarray(veg_type, dtype=np.int32)
+        tab = [f64(a) for a in (pcana, pcref, pcwlt, zroot, pz, omgc)]
+        z = f64(self.problem.mesh.z)
+        self._keep["veg"] = (vt, tab, z)
+        _check(lib().cathy_gpu_flow3d_set_vegetation(C.byref(self.h), _i(len(tab[0])), _p(vt),
+                                                     *[_p(a) for a in tab], _p(z)), "set_vegetation")
+
     def init(self, psi0):
         p = np.ascontiguousarray(psi0, dtype=np.float64)
         _check(lib().cathy_gpu_flow3d_init(C.byref(self.h), _p(p)), "flow3d_init")
@@ -139,6 +189,13 @@ class FlowGPU:
     def psi(self) -> np.ndarray:
         out = np.zeros(self.n)
         _check(lib().cathy_gpu_flow3d_get_state(C.byref(self.h), _p(out), None, None, None), "get_state")
+        return out
+
+    def ifatm(self) -> np.ndarray:
+        """IFATM of the surface nodes (-1 another boundary condition, 0
+        Neumann, 1/2 Dirichlet atmospheric)."""
+        out = np.zeros(self.nnod, dtype=np.int32)
+        _check(lib().cathy_gpu_flow3d_get_state(C.byref(self.h), None, None, None, _p(out)), "get_state")
         return out
 
     def sw(self) -> np.ndarray:

@@ -16,6 +16,15 @@ container, where the reference tree is mounted at /root/reference).
 4. weil_outputs.npz -- text of the weil run's output files as the reference
    wrote them (examples/SSHydro/weill_exemple/output/): the first two output
    times of psi and sw (with sw's header line) and the whole mbeconv.
+5. hillslope_bc_ref.npz -- configs.hillslope_bc (fixed heads, listed
+   Neumann fluxes, root uptake) to its 3 h end by the reference's FLOW3D,
+   BCONE/BCNXT/BCBAK on the written nansfdirbc/nansfneubc files, NEUMANN
+   and ETRAN (oracle/_ref/bc): per-step statistics, psi/Sw/IFATM at
+   selected steps.
+6. hillslope_seepage_ref.npz -- configs.hillslope_seepage (two seepage
+   faces, exit points in the convergence test) by the reference's FLOW3D
+   with SFVONE/SFVREC on the written sfbc file, SFINIT, EXTALL/EXTCVG:
+   per-step statistics, psi/Sw and the faces' SFEX at selected steps.
 3. hillslope_small_newton_ref.npz -- the first 60 steps of the same run
    with the Newton linearisation (IOPT=2, ISOLV=0: ILU(0)-BiCGSTAB).  The
    reference's Newton mass-balance terms are stale (its Newton branch never
@@ -170,6 +179,93 @@ def make_hillslope_small_newton(out: str, nsteps: int = 60, steps_saved=(1, 5, 1
     print(f"hillslope_small newton: {len(stats)} steps")
 
 
+def make_hillslope_bc(out: str, steps_saved=(1, 2, 10, 50, 100, 200, 300, 420, 440, 460, 476, 480)) -> None:
+    """hillslope_bc (fixed heads, listed Neumann fluxes under free drainage,
+    two-type root uptake; configs.hillslope_bc) stepped to its 3 h end by
+    the reference's own FLOW3D with its own BCONE / BCNXT / BCBAK reading
+    the nansfdirbc / nansfneubc files bcfiles.write_bc writes, NEUMANN and
+    ETRAN, built with BC-sized bounds (oracle/_ref/bc)."""
+    import tempfile
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pycathy_wrapper_amd import bcfiles as B
+    from pycathy_wrapper_amd import configs as CF
+    import pyoracle as P
+    import refrun as R
+    pr = CF.hillslope_bc()
+    m = pr.mesh
+    with tempfile.TemporaryDirectory() as d:
+        paths = {k: os.path.join(d, k) for k in ("dir", "neu", "atmbc")}
+        B.write_bc(paths["dir"], pr.bc_dir.file_records(m.nnod, m.nstr))
+        B.write_bc(paths["neu"], pr.bc_neu.file_records(m.nnod, m.nstr))
+        R.write_atmbc(paths["atmbc"], pr.atm_times, pr.atm_vals, 1)
+        rr = R.RefFlowRun(P.RefLib(P.REF_BC_SO), m, pr.soil, pr.psi0, 0.0, pr.params.as_dict(),
+                          atmbc=paths["atmbc"], dirbc=paths["dir"], neubc=paths["neu"], veg=pr.veg)
+        stats, psi, sw, ifatm, saved, keys = [], [], [], [], [], None
+        k = 0
+        while True:
+            rc, s = rr.step()
+            k += 1
+            keys = keys or list(s)
+            stats.append([s[nm] for nm in keys])
+            if k in steps_saved or rc:
+                saved.append(k)
+                psi.append(rr.psi().copy())
+                sw.append(rr.sw().copy())
+                ifatm.append(rr.a["IFATM"][:m.nnod].copy())
+            if rc:
+                break
+        rr.close()
+    np.savez_compressed(out, stat_names=np.array(keys), stats=np.array(stats), steps=np.array(saved),
+                        psi=np.array(psi), sw=np.array(sw), ifatm=np.array(ifatm), rc_last=rc)
+    print(f"hillslope_bc: {k} steps, saved {saved}")
+
+
+def make_hillslope_seepage(out: str, steps_saved=(1, 2, 5, 10, 20, 40, 100, 200, 250, 280, 300)) -> None:
+    """hillslope_seepage (two seepage faces, ISFCVG = 1; configs.
+    hillslope_seepage) to its 2 h end by the reference's FLOW3D with its own
+    SFVONE / SFVREC reading the sfbc file bcfiles.write_sfbc writes, SFINIT,
+    EXTALL / EXTCVG and BKSTEP (oracle/_ref/bc).  The file repeats the
+    faces in a second record at t = 1e30: SFVONE reads the next record's
+    header at once, so the reference needs one beyond the run."""
+    import tempfile
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pycathy_wrapper_amd import bcfiles as B
+    from pycathy_wrapper_amd import configs as CF
+    import pyoracle as P
+    import refrun as R
+    pr = CF.hillslope_seepage()
+    m = pr.mesh
+    faces = pr.seepage["faces"]
+    with tempfile.TemporaryDirectory() as d:
+        sf = os.path.join(d, "sfbc")
+        B.write_sfbc(sf, [B.SFRecord(0.0, faces), B.SFRecord(1e30, faces)])
+        lib = P.RefLib(P.REF_BC_SO)
+        rr = R.RefFlowRun(lib, m, pr.soil, pr.psi0, float(pr.atm_vals[0]), pr.params.as_dict(), sfbc=sf,
+                          isfcvg=pr.seepage["isfcvg"])
+        fm = lib.nsfmax
+        pos = [(j * fm + i) for i, f in enumerate(faces) for j in range(len(f))]
+        stats, psi, sw, sfex, saved, keys = [], [], [], [], [], None
+        k = 0
+        while True:
+            rc, s = rr.step()
+            k += 1
+            keys = keys or list(s)
+            stats.append([s[nm] for nm in keys])
+            if k in steps_saved or rc:
+                saved.append(k)
+                psi.append(rr.psi().copy())
+                sw.append(rr.sw().copy())
+                sfex.append(rr.a["SFEX"][pos].copy())
+            if rc:
+                break
+        rr.close()
+    np.savez_compressed(out, stat_names=np.array(keys), stats=np.array(stats), steps=np.array(saved),
+                        psi=np.array(psi), sw=np.array(sw), sfex=np.array(sfex), rc_last=rc)
+    print(f"hillslope_seepage: {k} steps, saved {saved}")
+
+
 def make_outputs(out: str, nblocks: int = 2) -> None:
     d = os.path.join(REF, "examples/SSHydro/weill_exemple/output")
     texts = {}
@@ -188,3 +284,5 @@ if __name__ == "__main__":
     make_hillslope_small(os.path.join(HERE, "hillslope_small_ref.npz"))
     make_hillslope_small_newton(os.path.join(HERE, "hillslope_small_newton_ref.npz"))
     make_outputs(os.path.join(HERE, "weil_outputs.npz"))
+    make_hillslope_bc(os.path.join(HERE, "hillslope_bc_ref.npz"))
+    make_hillslope_seepage(os.path.join(HERE, "hillslope_seepage_ref.npz"))
