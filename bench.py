@@ -345,43 +345,69 @@ def read_traffic():
         return None
 
 
-def pmc_summary(paths: list[str], kernel: str, copy_bytes: float) -> int:
-    """Average per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of ``kernel`` and of
-    the calibration copy kernel from rocprofv3 --pmc CSV output, and the HBM
-    bytes per launch each implies after scaling by copy_bytes / counter."""
-    acc = {}
+def _pmc_rows(paths: list[str]):
     for pat in paths:
         for f in glob.glob(os.path.join(pat, "**", "*counter_collection.csv"), recursive=True) + \
                  (glob.glob(pat) if pat.endswith(".csv") else []):
             with open(f) as fh:
-                for row in csv.DictReader(fh):
-                    name = row.get("Kernel_Name", "")
-                    key = "spmv" if kernel in name else ("copy" if "k_copy" in name else None)
-                    if key is None:
-                        continue
-                    cn = row.get("Counter_Name", "")
-                    v = float(row.get("Counter_Value", "nan"))
-                    acc.setdefault((key, cn), []).append(v)
+                yield from csv.DictReader(fh)
+
+
+def pmc_summary(paths: list[str], kernel: str, copy_bytes: float, all_kernels: bool = False) -> int:
+    """Average per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of ``kernel`` and of
+    the calibration copy kernel from rocprofv3 --pmc CSV output, and the HBM
+    bytes per launch each implies after scaling by copy_bytes / counter.
+    With all_kernels, the same per-launch figures for every kernel seen."""
+    acc = {}
+    for row in _pmc_rows(paths):
+        name = row.get("Kernel_Name", "")
+        if "k_copy" in name:
+            key = "copy"
+        elif kernel in name:
+            key = "spmv"
+        elif all_kernels:
+            key = name.split("(")[0].replace("void ", "").strip()
+        else:
+            continue
+        cn = row.get("Counter_Name", "")
+        v = float(row.get("Counter_Value", "nan"))
+        acc.setdefault((key, cn), []).append(v)
     out = {}
     for (key, cn), vs in sorted(acc.items()):
-        # SpMV launches queued after PCG convergence exit at once and move
+        # launches queued after PCG convergence exit at once and move
         # (almost) nothing; only the live launches are averaged
-        live = [v for v in vs if v >= 0.05 * max(vs)]
-        out[(key, cn)] = sum(live) / len(live)
-        print(f"{key} {cn}: {len(live)} live of {len(vs)} dispatches, mean {out[(key, cn)]:.1f} KiB")
+        live = [v for v in vs if v >= 0.05 * max(vs)] or vs
+        out[(key, cn)] = (sum(live) / len(live), len(live), len(vs))
+        if key in ("spmv", "copy"):
+            print(f"{key} {cn}: {len(live)} live of {len(vs)} dispatches, mean {out[(key, cn)][0]:.1f} KiB")
     total = 0.0
     ok = True
+    factor = {}
     for cn in ("FETCH_SIZE", "WRITE_SIZE"):
-        if ("spmv", cn) not in out or ("copy", cn) not in out:
+        if ("copy", cn) not in out:
             ok = False
             continue
-        factor = copy_bytes / (out[("copy", cn)] * 1024.0)
-        b = out[("spmv", cn)] * 1024.0 * factor
-        print(f"{cn}_calibration_factor: {factor:.4f}")
+        factor[cn] = copy_bytes / (out[("copy", cn)][0] * 1024.0)
+        if ("spmv", cn) not in out:
+            ok = False
+            continue
+        b = out[("spmv", cn)][0] * 1024.0 * factor[cn]
+        print(f"{cn}_calibration_factor: {factor[cn]:.4f}")
         print(f"{cn}_bytes_per_launch: {b:.0f}")
         total += b
     if ok:
         print(f"traffic_bytes_per_launch: {total:.0f}")
+    if all_kernels and len(factor) == 2:
+        print(f"{'kernel':58s} {'live':>6s} {'fetch MB':>9s} {'write MB':>9s} {'HBM MB':>8s}")
+        names = sorted({k for (k, _) in out if k not in ("copy",)})
+        for k in names:
+            f = out.get((k, "FETCH_SIZE"))
+            w = out.get((k, "WRITE_SIZE"))
+            if not f or not w:
+                continue
+            fb = f[0] * 1024.0 * factor["FETCH_SIZE"] / 1e6
+            wb = w[0] * 1024.0 * factor["WRITE_SIZE"] / 1e6
+            print(f"{('cg::k_spmv_dot2' if k == 'spmv' else k)[:58]:58s} {f[1]:6d} {fb:9.2f} {wb:9.2f} {fb + wb:8.2f}")
     return 0 if ok else 1
 
 
@@ -654,9 +680,13 @@ def main() -> int:
                          "201x201-cell network instead, print its own JSON line and exit")
     ap.add_argument("--pmc-summary", nargs="+", metavar="DIR_OR_CSV",
                     help="summarise rocprofv3 --pmc CSV output (FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--cold-reps", type=int, default=50,
+                    help="SpMV launches timed after a 1 GB cache flush each (0: skip)")
+    ap.add_argument("--pmc-all", action="store_true",
+                    help="with --pmc-summary: per-launch HBM bytes of every kernel, not only the SpMV")
     args = ap.parse_args()
     if args.pmc_summary:
-        return pmc_summary(args.pmc_summary, "k_spmv_dot2", args.calib_mb * 1e6)
+        return pmc_summary(args.pmc_summary, "k_spmv_dot2", args.calib_mb * 1e6, args.pmc_all)
     if args.gpus != 1:
         print(f"bench.py: one GPU only; --gpus {args.gpus} is not supported (members of an ensemble shard "
               f"across GPUs without communication; no multi-GPU run is part of this build)", file=sys.stderr)
@@ -735,6 +765,13 @@ def main() -> int:
     achieved = spmv_bytes / (avg_ms * 1e-3) / 1e9 if spmv_n else 0.0
     log(f"[roofline] k_spmv_dot2: {spmv_n} launches, {avg_ms * 1e3:.2f} us avg, {spmv_bytes / 1e6:.1f} MB "
         f"-> {achieved:.0f} GB/s")
+    # the same SpMV with the caches cleared before each launch (1 GB copy:
+    # A and p come from HBM, not the 256 MB last-level cache) and warm
+    cold_ms, warm_ms = g.bench_spmv(args.cold_reps, 1024.0) if args.cold_reps > 0 else (0.0, 0.0)
+    cold_gbs = spmv_bytes / (cold_ms * 1e-3) / 1e9 if cold_ms > 0 else 0.0
+    if args.cold_reps > 0:
+        log(f"[roofline] k_spmv_dot2 after a 1 GB cache flush: {cold_ms * 1e3:.2f} us avg over {args.cold_reps} "
+            f"-> {cold_gbs:.0f} GB/s ({cold_gbs / HBM_PEAK_GBS:.3f} of peak); warm {warm_ms * 1e3:.2f} us")
     if args.calib_mb > 0:
         ms = gpu.calib_copy(int(args.calib_mb * 1e6), 5)
         log(f"[calib] copy of {args.calib_mb:.0f} MB: {ms * 1e3:.1f} us/launch")
@@ -855,6 +892,8 @@ def main() -> int:
             "avg_us": avg_ms * 1e3,
             "launches": spmv_n,
             "bytes_per_launch": spmv_bytes,
+            "cold": {"avg_us": cold_ms * 1e3, "achieved": cold_gbs, "frac": cold_gbs / HBM_PEAK_GBS,
+                     "launches": args.cold_reps, "flush_mb": 1024, "warm_avg_us": warm_ms * 1e3},
         },
         "cpu_baseline": cpu,
     }

@@ -336,6 +336,14 @@ int cathy_gpu_flow3d_set_vegetation(int64_t *handle, const int *nveg,
                                     const double *zroot, const double *pz,
                                     const double *omgc, const double *z);
 
+/* Measurement only (no counterpart in the reference): the PCG SpMV on the
+ * run's current matrix and search direction, NREP launches each after a
+ * FLUSH_MB copy that evicts the caches; OUT2 = mean ms of those launches
+ * and of a second, warm launch right after each.  The run's state is kept
+ * (the SpMV's outputs are PCG scratch). */
+int cathy_gpu_flow3d_bench_spmv(int64_t *handle, const int *nrep,
+                                const double *flush_mb, double *out2);
+
 /* Counters of the last advance: Picard its, PCG its, IC(0) breakdown
  * fallbacks, kernel time of the linear solver (ms), total device ms. */
 int cathy_gpu_flow3d_counters(int64_t *handle, double *out6);

@@ -461,6 +461,15 @@ int cathy_gpu_flow3d_set_vegetation(int64_t *handle, const int *nveg, const int 
   });
 }
 
+int cathy_gpu_flow3d_bench_spmv(int64_t *handle, const int *nrep, const double *flush_mb, double *out2) {
+  return guarded([&] {
+    const std::shared_ptr<Flow> f = lookup(handle);
+    CG_REQUIRE(nrep && flush_mb && out2 && *flush_mb >= 0.0, "bad argument");
+    f->bench_spmv(*nrep, (size_t)(*flush_mb * 1e6), out2);
+    return 0;
+  });
+}
+
 int cathy_gpu_flow3d_set_pcg_batches(int64_t *handle, const int *first_pct, const int *followup) {
   return guarded([&] {
     const std::shared_ptr<Flow> f = lookup(handle);

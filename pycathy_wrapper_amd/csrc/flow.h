@@ -1148,6 +1148,43 @@ struct Flow {
     else factor_t<double>(F64.p);
   }
 
+  // k_spmv_dot2 on the current matrix and search direction, timed by the
+  // dispatch packet's events: out[0] the mean over nrep launches each after
+  // a copy of flush_bytes through scratch buffers (which evicts the L2s and
+  // the last-level cache, so A and p come from HBM), out[1] the mean of a
+  // second launch right after (warm caches).  Touches only the SpMV's
+  // output and partials, which every PCG iteration recomputes.
+  void bench_spmv(int nrep, size_t flush_bytes, double *out) {
+    CG_REQUIRE(initialised && nrep >= 1, "bench_spmv needs an initialised run and nrep >= 1");
+    const size_t fw = std::max<size_t>(1, flush_bytes / sizeof(double));
+    DBuf<double> fa(fw), fb(fw);
+    CG_HIP(hipMemsetAsync(fa.p, 0, fw * sizeof(double), st));
+    int *done = flags.p;
+    CG_HIP(hipMemsetAsync(done, 0, sizeof(int), st));
+    const int Gs = grid_for(n, kBlock);
+    const FacTab fts{(spmv_transposed && ftab_ok && sell().coded && maxrow <= kUnroll) ? ftab.p : nullptr, ftab_nch};
+    std::vector<hipEvent_t> ev(4 * (size_t)nrep);
+    for (hipEvent_t &e : ev) CG_HIP(hipEventCreate(&e));
+    for (int i = 0; i < nrep; ++i) {
+      hipLaunchKernelGGL(k_copy, dim3((unsigned)((fw + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, fw, fa.p, fb.p);
+      for (int h = 0; h < 2; ++h)
+        hipExtLaunchKernelGGL(k_spmv_dot2, dim3(Gs), dim3(kBlock), 0, st, ev[4 * i + 2 * h], ev[4 * i + 2 * h + 1], 0,
+                              n, sell(), fts, A.p, p.p, r.p, Bv.p, P(1), P(2), done);
+    }
+    CG_HIP(hipStreamSynchronize(st));
+    double cold = 0.0, warm = 0.0;
+    for (int i = 0; i < nrep; ++i) {
+      float a = 0.f, b = 0.f;
+      CG_HIP(hipEventElapsedTime(&a, ev[4 * i], ev[4 * i + 1]));
+      CG_HIP(hipEventElapsedTime(&b, ev[4 * i + 2], ev[4 * i + 3]));
+      cold += a;
+      warm += b;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    out[0] = cold / nrep;
+    out[1] = warm / nrep;
+  }
+
   // SYMSLV on the current A / TNOTI: x = pdiff.  Returns NITER; err out.
   int pcg(double tol, int imax, double *err_out, int expect) {
     const int G = std::min(grid_for(n, kBlock), 1024);

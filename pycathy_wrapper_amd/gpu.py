@@ -158,6 +158,14 @@ class FlowGPU:
         _check(lib().cathy_gpu_flow3d_set_seepage(C.byref(self.h), _i(len(faces)), _p(counts), _p(nodes),
                                                   _i(isfcvg)), "set_seepage")
 
+    def bench_spmv(self, nrep: int = 50, flush_mb: float = 1024.0):
+        """(cold ms, warm ms) of the PCG SpMV on the current matrix: each
+        cold launch follows a flush_mb copy that clears the caches."""
+        out = np.zeros(2)
+        _check(lib().cathy_gpu_flow3d_bench_spmv(C.byref(self.h), _i(nrep), C.byref(C.c_double(flush_mb)), _p(out)),
+               "bench_spmv")
+        return float(out[0]), float(out[1])
+
     def seepage_state(self):
         """(SFEX, SFQ) of the seepage nodes, face by face, top to bottom."""
         m = _check(lib().cathy_gpu_flow3d_get_seepage(C.byref(self.h), None, None), "get_seepage")
