@@ -76,6 +76,34 @@ def build_problem(cells: int, nstr: int, seed: int = 2):
     return CF.hillslope_large(seed=seed, cells=cells, nstr=nstr, params=bench_params())
 
 
+def full_run(pr, g, seconds: float) -> dict:
+    """Member 0 alone from its initial state through the whole 2-day run
+    (or until `seconds` of wall time), timed as one block: the later windows
+    -- the 6 h rain pulse, the drainage after it -- next to the first steps
+    the default line times."""
+    g.init(pr.psi0)
+    n = pic = pcg = backs = 0
+    marks = {}
+    t0 = time.perf_counter()
+    rc, s = 0, {}
+    while True:
+        rc, s = g.advance()
+        c = g.counters()
+        n += 1
+        pic += c["picard"]
+        pcg += c["pcg"]
+        backs += s["BACK"]
+        for hrs in (6, 24, 48):   # wall time and steps when the run passes 6 h, 1 day, 2 days
+            if hrs not in marks and s["TIME"] >= hrs * 3600.0 - 1e-6:
+                marks[hrs] = dict(steps=n, wall_s=time.perf_counter() - t0)
+        if rc or time.perf_counter() - t0 > seconds:
+            break
+    wall = time.perf_counter() - t0
+    return dict(steps=n, wall_s=wall, steps_per_s=n / wall, sim_s=s.get("TIME", 0.0),
+                sim_s_per_wall_s=s.get("TIME", 0.0) / wall, picard=pic, pcg=pcg, back_steps=backs, rc=rc,
+                last_dt=s.get("DELTAT", 0.0), marks_h=marks)
+
+
 def ensemble_run(members, steps: int) -> dict:
     """Steps 1..K of every member at once, one host thread and HIP stream
     per member (the library releases nothing between calls; ctypes drops
@@ -680,6 +708,9 @@ def main() -> int:
                          "201x201-cell network instead, print its own JSON line and exit")
     ap.add_argument("--pmc-summary", nargs="+", metavar="DIR_OR_CSV",
                     help="summarise rocprofv3 --pmc CSV output (FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--full-run", type=float, default=120.0,
+                    help="wall seconds allowed for member 0's whole 2-day run, timed after the default line's "
+                         "legs (0 skips it)")
     ap.add_argument("--cold-reps", type=int, default=50,
                     help="SpMV launches timed after a 1 GB cache flush each (0: skip)")
     ap.add_argument("--pmc-all", action="store_true",
@@ -797,6 +828,13 @@ def main() -> int:
         f"{streams_rate:.2f} member-steps/s")
     for _, gi in others:
         gi.close()
+    full = None
+    if args.full_run > 0:
+        full = full_run(pr, g, args.full_run)
+        log(f"[full run] member 0 alone: {full['steps']} steps, sim 0 -> {full['sim_s']:.0f} s in "
+            f"{full['wall_s']:.2f}s ({full['steps_per_s']:.2f} steps/s, {full['sim_s_per_wall_s']:.0f} sim-s per "
+            f"wall-s), {full['picard']:.0f} Picard, {full['pcg']:.0f} PCG, {full['back_steps']:.0f} back-steps, "
+            f"rc {full['rc']}; passes {full['marks_h']}")
     fp64 = None
     if args.fp64_factor and args.ordering == "column3":
         # the same steps of member 0 with its IC(0) factor in FP64 (the
@@ -878,6 +916,7 @@ def main() -> int:
                               "pcg_iterations": pcg, "back_steps": backs},
             "learned_guess": False,
             "learned": learned,
+            "full_run": full,
             "fp64_factor": fp64,
             "parallelism": "single GPU",
         },

@@ -25,6 +25,9 @@ container, where the reference tree is mounted at /root/reference).
    faces, exit points in the convergence test) by the reference's FLOW3D
    with SFVONE/SFVREC on the written sfbc file, SFINIT, EXTALL/EXTCVG:
    per-step statistics, psi/Sw and the faces' SFEX at selected steps.
+7. hillslope_small_as_written_ref.npz -- hillslope_small under its parm
+   as written (DTMIN = 1 s, ITUNS 10/5/7): the reference's run to its
+   NOBACK stop, every step's statistics and the final psi/Sw.
 3. hillslope_small_newton_ref.npz -- the first 60 steps of the same run
    with the Newton linearisation (IOPT=2, ISOLV=0: ILU(0)-BiCGSTAB).  The
    reference's Newton mass-balance terms are stale (its Newton branch never
@@ -266,6 +269,29 @@ def make_hillslope_seepage(out: str, steps_saved=(1, 2, 5, 10, 20, 40, 100, 200,
     print(f"hillslope_seepage: {k} steps, saved {saved}")
 
 
+def make_hillslope_small_as_written(out: str) -> None:
+    """hillslope_small with the workload's parm as written (DTMIN = 1 s,
+    ITUNS/ITUNS1/ITUNS2 = 10/5/7): the reference's FLOW3D stops with NOBACK
+    (DELTAT at DTMIN and ITUNS reached); every step and the final state."""
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pycathy_wrapper_amd import configs as CF
+    import pyoracle as P
+    import refrun as R
+    pr = CF.hillslope_small(params=CF.Params(toluns=1e-6))
+    rr = R.RefFlowRun(P.RefLib(), pr.mesh, pr.soil, pr.psi0, float(pr.atm_vals[0]), pr.params.as_dict())
+    stats, keys = [], None
+    while True:
+        rc, s = rr.step()
+        keys = keys or list(s)
+        stats.append([s[nm] for nm in keys])
+        if rc:
+            break
+    np.savez_compressed(out, stat_names=np.array(keys), stats=np.array(stats), psi=rr.psi(), sw=rr.sw(),
+                        rc_last=rc)
+    print(f"hillslope_small as written: {len(stats)} steps, rc {rc}")
+
+
 def make_outputs(out: str, nblocks: int = 2) -> None:
     d = os.path.join(REF, "examples/SSHydro/weill_exemple/output")
     texts = {}
@@ -286,3 +312,4 @@ if __name__ == "__main__":
     make_outputs(os.path.join(HERE, "weil_outputs.npz"))
     make_hillslope_bc(os.path.join(HERE, "hillslope_bc_ref.npz"))
     make_hillslope_seepage(os.path.join(HERE, "hillslope_seepage_ref.npz"))
+    make_hillslope_small_as_written(os.path.join(HERE, "hillslope_small_as_written_ref.npz"))

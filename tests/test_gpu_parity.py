@@ -25,7 +25,8 @@ import systems as SY
 
 pytestmark = pytest.mark.gpu
 
-GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hillslope_small_ref.npz")
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "hillslope_small_ref.npz")
 PSI_TOL = 1e-6
 MB_TOL = 1e-8
 MB_KEYS = ("STORE1", "STORE2", "DSTORE", "CDSTOR", "VIN", "CVIN", "VOUT", "CVOUT", "VTOT", "ERRAS", "CERRAS")
@@ -127,6 +128,39 @@ def _full_day(gpu, ordering, golden, exact_pcg):
     finally:
         g.close()
     say(f"ordering {ordering}: {k} steps, worst psi rel {worst:.2e}")
+
+
+@pytest.mark.parametrize("ordering", ["natural", "column3"])
+def test_flow_as_written_stops_with_reference(gpu, ordering):
+    """hillslope_small under its parm as written (DTMIN = 1 s, ITUNS 10/5/7):
+    the reference's own FLOW3D stops with NOBACK at step 16
+    (tests/golden/hillslope_small_as_written_ref.npz); the device takes the
+    same steps and stops at the same one with rc 2 -- why the workload's
+    runs use DTMIN = 0.01 s and ITUNS 30/10/20."""
+    gold = np.load(os.path.join(HERE, "golden", "hillslope_small_as_written_ref.npz"))
+    names = [str(v) for v in gold["stat_names"]]
+    col = {nm: j for j, nm in enumerate(names)}
+    stats = gold["stats"]
+    pr = CF.hillslope_small(params=CF.Params(toluns=1e-6))
+    g = gpu.FlowGPU(pr, gpu.ORDER_NATURAL if ordering == "natural" else gpu.ORDER_COLUMN3)
+    scale = abs(stats[:, col["STORE1"]]).max() * 1e-6
+    try:
+        k = 0
+        while True:
+            rc, s = g.advance()
+            ref = stats[k]
+            k += 1
+            say(f"{ordering} step {k}: dt {s['DELTAT']} iter {s['ITER']:.0f} back {s['BACK']:.0f} rc {rc}")
+            assert s["DELTAT"] == ref[col["DELTAT"]] and s["ITER"] == ref[col["ITER"]], k
+            assert s["BACK"] == ref[col["BACK"]], k
+            for key in MB_KEYS:
+                assert _mb_close(s[key], ref[col[key]], scale), (k, key)
+            if rc:
+                break
+        assert k == len(stats) == 16 and rc == int(gold["rc_last"]) == 2
+        assert _rel_inf(g.psi(), gold["psi"]) <= PSI_TOL
+    finally:
+        g.close()
 
 
 def test_flow_natural_full_day_matches_reference(gpu):
