@@ -68,6 +68,42 @@ module cathy_gpu
       real(c_double), intent(in) :: atmtim(*), atmval(*)
     end function
 
+    ! BCONE/BCNXT/BCBAK record series of the Dirichlet (kind 0: CONTP,
+    ! PRESC) or Neumann (kind 1: CONTQ, Q) nodes, expanded as RDNDBC/READBC
+    integer(c_int) function cathy_gpu_flow3d_set_bc(handle, kind, nrec, times, counts, nodes, values) &
+        bind(C, name="cathy_gpu_flow3d_set_bc")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      integer(c_int), intent(in) :: kind, nrec
+      real(c_double), intent(in) :: times(*), values(*)
+      integer(c_int), intent(in) :: counts(*), nodes(*)
+    end function
+
+    ! ETRAN's vegetation: VEG_TYPE(NNOD) and the soil file's PCANA .. OMGC
+    integer(c_int) function cathy_gpu_flow3d_set_vegetation(handle, nveg, veg_type, pcana, pcref, pcwlt, &
+        zroot, pz, omgc, z) bind(C, name="cathy_gpu_flow3d_set_vegetation")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      integer(c_int), intent(in) :: nveg, veg_type(*)
+      real(c_double), intent(in) :: pcana(*), pcref(*), pcwlt(*), zroot(*), pz(*), omgc(*), z(*)
+    end function
+
+    ! seepage faces: NSF faces of NSFNUM(i) nodes, NSFNOD packed face by face
+    integer(c_int) function cathy_gpu_flow3d_set_seepage(handle, nsf, counts, nodes, isfcvg) &
+        bind(C, name="cathy_gpu_flow3d_set_seepage")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), intent(in) :: handle
+      integer(c_int), intent(in) :: nsf, counts(*), nodes(*), isfcvg
+    end function
+
+    integer(c_int) function cathy_gpu_flow3d_get_seepage(handle, sfex, sfq) &
+        bind(C, name="cathy_gpu_flow3d_get_seepage")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      integer(c_int), intent(out) :: sfex(*)
+      real(c_double), intent(out) :: sfq(*)
+    end function
+
     integer(c_int) function cathy_gpu_flow3d_init(handle, ptimep) bind(C, name="cathy_gpu_flow3d_init")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), intent(in) :: handle
