@@ -18,6 +18,12 @@ needs a build whose NPMAX / NQMAX / MAXVEG hold the lists (oracle/_ref/bc)
 and always takes the forcing from an atmbc file (one is written for a
 constant ``rain``), so back-steps run the reference's BKSTEP.
 
+``surf=True`` runs a coupled (ISIMGR = 2) project's atmospheric switching:
+FLOW3D gets SURF true, so its CONVER calls the reference's SWITCH, and step
+starts and back-steps call ADRSTN (cathy_main.f:2942-2946, bkstep.f:135-139).
+The routing (SURF_FLOWTRA, PONDUPD) is not run: a step that ends with
+PONDING set ends the run with rc 3, before the step that would route.
+
 Atmospheric forcing comes one of two ways:
 - ``atmbc`` names an atmbc file (``write_atmbc``).  The reference's own
   ATMONE and ATMNXT then read and interpolate it (the file is opened on
@@ -91,7 +97,7 @@ class RefFlowRun:
 
     def __init__(self, ref: RefLib, mesh, soil, psi0, rain, prm: dict, atmbc: str | None = None,
                  dirbc: str | None = None, neubc: str | None = None, veg: dict | None = None,
-                 sfbc: str | None = None, isfcvg: int = 0):
+                 sfbc: str | None = None, isfcvg: int = 0, surf: bool = False):
         assert soil["nzone"] <= ref.maxzon, "more zones than the reference build's MAXZON"
         assert mesh.nstr <= ref.maxstr and mesh.n <= ref.nmax, "mesh larger than the reference build"
         self.ref = ref
@@ -178,7 +184,15 @@ class RefFlowRun:
         self.rain = rain
         self.atmbc = atmbc
         self.nterm = nterm
-        self.lists = dirbc is not None or neubc is not None or veg is not None or sfbc is not None
+        self.lists = dirbc is not None or neubc is not None or veg is not None or sfbc is not None or surf
+        # SURF (ISIMGR = 2): CONVER calls SWITCH instead of SWITCH_OLD, and
+        # step starts and back-steps call ADRSTN (cathy_main.f:2942-2946,
+        # bkstep.f:135-139).  The routing itself is not run: a step that
+        # ends with a ponded node stops the run (rc 3).
+        self.surf = bool(surf)
+        a["SURF"][0] = 1 if surf else 0
+        a["PONDING"][0] = 0
+        self.pondp = np.zeros(4, dtype=np.int32)
         self._tmp = None
         if self.lists:
             self._tmp = tempfile.mkdtemp(prefix="refbc_")
@@ -546,6 +560,20 @@ class RefFlowRun:
         self.ref.L.switch_old_(C.byref(C.c_int(self.nnod)), ip(a["IFATM"]), dp(a["ATMACT"]), dp(a["ATMPOT"]),
                                dp(a["PNEW"]))
 
+    def _adrstn(self):
+        a = self.a
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        self.ref.L.adrstn_(C.byref(C.c_int(self.nnod)), ip(a["IFATM"]), dp(a["ATMPOT"]), dp(a["ATMACT"]),
+                           dp(a["PNEW"]))
+
+    def _switch_start(self):
+        """cathy_main.f:2942-2946: SWITCH_OLD, or ADRSTN when coupled."""
+        if self.surf:
+            self._adrstn()
+        else:
+            self._switch_old()
+
     def _flow3d(self):
         self.ref.L.flow3d_(*self._argv)
 
@@ -564,6 +592,8 @@ class RefFlowRun:
                                    "KBACK", "NSF", "NUMRES", "NCELNL", "SURF", "PONDING", "PONDP", "TRANSP",
                                    "ANP", "SFV")}
         ints["ANQ"] = np.array([a["NQ"][0], 0, 0, 0], dtype=np.int32)
+        ints["SURF"][0] = a["SURF"][0]
+        ints["PONDING"], ints["PONDP"] = a["PONDING"], self.pondp
         nsf_arr = i32(64)
         dir_ = dict(nb2c=ints["NDIR"], nbfc=ints["NDIRC"], nbc=ints["NP"], hti=ints["HTIDIR"], nod=a["CONTP"],
                     bc=a["PRESC"], tim=never, inp=a["PINP"], anbc=ints["ANP"], abcnod=a["CONTP"])
@@ -617,6 +647,7 @@ class RefFlowRun:
         a["ITER"][0] = 1
         a["NITERT"][0] = 0
         # a constant record: ATMNXT/ATMBAK leave ATMPOT unchanged
+        assert not self.surf, "the coupled switching needs the reference's BKSTEP (an atmbc file)"
         self._switch_old()
         self._copy("POLD", "PNEW", n)
         self._weight()
@@ -641,7 +672,7 @@ class RefFlowRun:
                     a["ATMACT"][i] = a["ATMPOT"][i]
         if self.lists:
             self._etran()   # cathy_main.f:2929
-        self._switch_old()
+        self._switch_start()
         self._copy("POLD", "PNEW", n)
         self._weight()
         self._copy("PTOLD", "PTNEW", n)
@@ -683,6 +714,12 @@ class RefFlowRun:
         a["ATMOLD"][:nnod] = a["ATMACT"][:nnod]
         self._copy("PTIMEP", "PNEW", n)
         self.ckrwp = a["CKRW"][:n].copy()
+        if self.surf:   # cathy_main.f:3771: PONDP = PONDING
+            self.pondp[0] = a["PONDING"][0]
+            stats["PONDING"] = int(a["PONDING"][0] != 0)
+            if a["PONDING"][0]:   # the next step would route (cathy_main.f:2993), which is not run here
+                self.finished = 3
+                return 3, stats
         if a["NOBACK"][0]:
             self.finished = 2
             return 2, stats

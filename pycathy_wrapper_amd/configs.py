@@ -83,6 +83,10 @@ class Problem:
     bc_neu: object = None
     veg: dict | None = None
     seepage: dict | None = None  # faces: 0-based node arrays top to bottom; isfcvg: 0 / 1
+    # ISIMGR = 2: the coupled atmospheric switching (SWITCH in CONVER, ADRSTN
+    # at step starts) with PONDH_MIN from the parm file; the run stops when a
+    # step ends with a ponded surface node (the routing is not run)
+    surf: bool = False
 
 
 def sig4(a: np.ndarray) -> np.ndarray:

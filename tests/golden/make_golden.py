@@ -33,6 +33,21 @@ container, where the reference tree is mounted at /root/reference).
    reference's Newton mass-balance terms are stale (its Newton branch never
    sets FLOW3D's local SWNEW/SWTIMEP) and are stored as computed.
 
+8. bc_noflow_project/, outlets_project/, weil_project/ -- example projects
+   as pyCATHY lays them out (input/, prepro/: the files project.load_project
+   reads), copied from examplesTmp/SSHydro/bc_noflow,
+   examplesTmp/catchments/outlets and examples/SSHydro/weill_exemple, with
+   what the reference program wrote for them: output/mbeconv as written,
+   output/psi_sw.npz (the psi and sw blocks of output/psi and output/sw,
+   values as printed, with their NSTEP and TIME) and, for weil, the head of
+   output/risul (parameter echo, soil table, initial heads).
+9. bc_noflow_ref.npz, outlets_ref.npz -- those two projects run by the
+   reference's FLOW3D with the coupled switching (SURF true: SWITCH in
+   CONVER, ADRSTN at step starts and in BKSTEP), their own nansfneubc /
+   sfbc / atmbc files read by the reference's BCONE / SFVONE / ATMONE
+   (oracle/_ref/bc): per-step statistics for the steps the stored mbeconv
+   holds, psi/Sw/IFATM at selected steps.
+
 Only data goes into the fixtures; no reference source text.
 """
 from __future__ import annotations
@@ -305,6 +320,98 @@ def make_outputs(out: str, nblocks: int = 2) -> None:
     print("weil outputs:", {k: len(v) for k, v in texts.items()})
 
 
+PROJECTS = {
+    "bc_noflow_project": "examplesTmp/SSHydro/bc_noflow",
+    "outlets_project": "examplesTmp/catchments/outlets",
+    "weil_project": "examples/SSHydro/weill_exemple",
+}
+PROJECT_FILES = ("input/parm", "input/soil", "input/ic", "input/atmbc", "input/dem_parameters",
+                 "input/nansfdirbc", "input/nansfneubc", "input/sfbc", "input/root_map", "prepro/dem",
+                 "prepro/zone")
+
+
+def read_blocks(path: str):
+    """The blocks of a psi / sw output file (detout.f): a header line
+    'NSTEP TIME' then N values; returns [(nstep, time, values)]."""
+    lines = open(path).read().splitlines()
+    heads = [i for i, ln in enumerate(lines) if "NSTEP" in ln and "TIME" in ln]
+    out = []
+    for j, h in enumerate(heads):
+        f = lines[h].split()
+        end = heads[j + 1] if j + 1 < len(heads) else len(lines)
+        vals = np.array(" ".join(lines[h + 1:end]).split(), dtype=np.float64)
+        out.append((int(f[0]), float(f[1]), vals))
+    return out
+
+
+def copy_projects() -> None:
+    """Item 8: the project directories and what the reference wrote."""
+    import shutil
+    for name, src in PROJECTS.items():
+        dst = os.path.join(HERE, name)
+        for rel in PROJECT_FILES:
+            s = os.path.join(REF, src, rel)
+            if os.path.exists(s):
+                os.makedirs(os.path.dirname(os.path.join(dst, rel)), exist_ok=True)
+                shutil.copyfile(s, os.path.join(dst, rel))
+        os.makedirs(os.path.join(dst, "output"), exist_ok=True)
+        if name == "weil_project":
+            lines = open(os.path.join(REF, src, "output/risul")).read().splitlines(keepends=True)
+            end = next(i for i, ln in enumerate(lines) if "TIME STEP:" in ln)
+            with open(os.path.join(dst, "output/risul_head"), "w") as f:
+                f.writelines(lines[:end])
+            continue
+        shutil.copyfile(os.path.join(REF, src, "output/mbeconv"), os.path.join(dst, "output/mbeconv"))
+        arrs = {}
+        for kind in ("psi", "sw"):
+            blocks = read_blocks(os.path.join(REF, src, "output", kind))
+            keep = {}
+            for nstep, t, v in blocks:   # first block of each NSTEP (the last is written twice)
+                keep.setdefault(nstep, (t, v))
+            steps = sorted(keep)
+            arrs[kind + "_nstep"] = np.array(steps)
+            arrs[kind + "_time"] = np.array([keep[k][0] for k in steps])
+            arrs[kind] = np.array([keep[k][1] for k in steps])
+        np.savez_compressed(os.path.join(dst, "output/psi_sw.npz"), **arrs)
+        print(f"{name}: psi blocks at steps {list(arrs['psi_nstep'])}")
+
+
+def make_project_run(name: str, out: str, steps_saved=(1, 2, 5, 10, 50, 100, 200, 300)) -> None:
+    """Item 9: the project run by the reference's FLOW3D with the coupled
+    switching, for as many steps as the stored mbeconv holds."""
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pycathy_wrapper_amd import project as PJ
+    import pyoracle as P
+    import refrun as R
+    root = os.path.join(HERE, name)
+    pr = PJ.load_project(root)
+    m = pr.mesh
+    nmb = sum(1 for ln in open(os.path.join(root, "output/mbeconv")) if not ln.startswith("#"))
+    io = lambda k: os.path.join(root, "input", k)
+    rr = R.RefFlowRun(P.RefLib(P.REF_BC_SO), m, pr.soil, pr.psi0, 0.0, pr.params.as_dict(), atmbc=io("atmbc"),
+                      dirbc=io("nansfdirbc"), neubc=io("nansfneubc"), veg=pr.veg, sfbc=io("sfbc"), surf=pr.surf)
+    stats, psi, sw, ifatm, saved, keys = [], [], [], [], [], None
+    k = 0
+    while True:
+        rc, s = rr.step()
+        k += 1
+        keys = keys or list(s)
+        stats.append([s[nm] for nm in keys])
+        last = rc or k == nmb
+        if k in steps_saved or last:
+            saved.append(k)
+            psi.append(rr.psi().copy())
+            sw.append(rr.sw().copy())
+            ifatm.append(rr.a["IFATM"][:m.nnod].copy())
+        if last:
+            break
+    rr.close()
+    np.savez_compressed(out, stat_names=np.array(keys), stats=np.array(stats), steps=np.array(saved),
+                        psi=np.array(psi), sw=np.array(sw), ifatm=np.array(ifatm), rc_last=rc)
+    print(f"{name}: {k} steps, rc {rc}, saved {saved}")
+
+
 if __name__ == "__main__":
     make_meshes(os.path.join(HERE, "meshes.npz"))
     make_hillslope_small(os.path.join(HERE, "hillslope_small_ref.npz"))
@@ -313,3 +420,7 @@ if __name__ == "__main__":
     make_hillslope_bc(os.path.join(HERE, "hillslope_bc_ref.npz"))
     make_hillslope_seepage(os.path.join(HERE, "hillslope_seepage_ref.npz"))
     make_hillslope_small_as_written(os.path.join(HERE, "hillslope_small_as_written_ref.npz"))
+    copy_projects()
+    make_project_run("bc_noflow_project", os.path.join(HERE, "bc_noflow_ref.npz"))
+    make_project_run("outlets_project", os.path.join(HERE, "outlets_ref.npz"),
+                     steps_saved=(1, 2, 5, 10, 50, 100, 200, 300, 400, 500, 600, 700))

@@ -125,17 +125,18 @@ def test_reader_matches_reference_on_listed_series(tmp_path):
 
 
 def test_reader_matches_reference_on_example_files():
-    """The bc_noflow example's own files (21x21-node DEM, 15 layers)."""
+    """The bc_noflow example's own files (a 20x20-cell DEM whose corner cell
+    is inactive: 440 surface nodes, 15 layers)."""
     _need(REF_SO)
     import pyoracle as P
-    nnod, nstr = 441, 15
+    nnod, nstr = 440, 15
     n = nnod * (nstr + 1)
-    neu = B.read_bc(os.path.join(HERE, "golden", "bc_noflow_nansfneubc"), "neumann", nnod, nstr)
+    neu = B.read_bc(os.path.join(HERE, "golden", "bc_noflow_project", "input", "nansfneubc"), "neumann", nnod, nstr)
     assert [r.time for r in neu.records] == [0.0, 12000.0, 18000.0]
     assert all(len(r.nodes) == 440 and np.all(r.values == 1.2e-6) for r in neu.records)
-    _check_series(P.RefLib(REF_SO).L, os.path.join(HERE, "golden", "bc_noflow_nansfneubc"), neu, n, nnod, nstr,
+    _check_series(P.RefLib(REF_SO).L, os.path.join(HERE, "golden", "bc_noflow_project", "input", "nansfneubc"), neu, n, nnod, nstr,
                   [10.0, 12000.0, 12000.5, 18000.5])
-    dirs = B.read_bc(os.path.join(HERE, "golden", "bc_noflow_nansfdirbc"), "dirichlet", nnod, nstr)
+    dirs = B.read_bc(os.path.join(HERE, "golden", "bc_noflow_project", "input", "nansfdirbc"), "dirichlet", nnod, nstr)
     assert [len(r.nodes) for r in dirs.records] == [0, 0, 0]
 
 
