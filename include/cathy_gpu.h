@@ -135,8 +135,12 @@ int cathy_gpu_flow3d_init(int64_t *handle, const double *ptimep);
  *  [18] CERRAS [19] CAERAS [20] PINF [21] ITRTOT [22] ITLIN [23] KLSFAI
  *  [24] PL2 [25] FINF [26] FL2 [27] IKMAX (1-based) [28] wall-clock ms
  *  of the step (host clock) [29] Picard and [30] PCG iterations of the
- *  step, back-stepped attempts included [31] reserved
- * Returns 0 while the run continues, 1 at TMAX, 2 on NOBACK, <0 on error. */
+ *  step, back-stepped attempts included [31] PONDING at the step's end
+ *  (coupled surface, cathy_gpu_flow3d_set_surface)
+ * Returns 0 while the run continues, 1 at TMAX, 2 on NOBACK, 3 when a
+ * coupled run's step ends with a ponded surface node (the next step would
+ * route it first, which the device does not do; the state is the step's
+ * end), <0 on error. */
 int cathy_gpu_flow3d_advance(int64_t *handle, double *stats);
 
 /* State read-back / overwrite (CATHY node order): PNEW, SW, CKRW (N),
@@ -330,6 +334,20 @@ int cathy_gpu_flow3d_get_seepage(int64_t *handle, int *sfex, double *sfq);
  * whose QTRANIE BCPIC subtracts from every row (bcpic.f:84-86).  Without
  * it QTRANIE is zero, which is what ETRAN gives for ATMPOT >= 0.  The run
  * starts over at the next init; Picard scheme only. */
+/* The coupled surface of ISIMGR = 2 (SURF = 1) or the subsurface-only
+ * run (0, the default).  With SURF = 1 the atmospheric switching is the
+ * coupled one: CONVER calls SWITCH instead of SWITCH_OLD (conver.f:61-71,
+ * switch.f) and step starts and back-steps call ADRSTN (cathy_main.f:
+ * 2942-2946, bkstep.f:135-139); PONDH_MIN (set_params) is the parm file's,
+ * which ATMONE and SWITCH test.  Replaces, for a surface that never ponds,
+ *   CALL SWITCH(NNOD,IFATM,PONDING,TIME,DELTAT,PONDH_MIN,ARENOD,PONDNOD,
+ *               ATMPOT,ATMACT,QTRANIE,PNEW,OVFLNOD)        (conver.f:67)
+ *   CALL ADRSTN(NNOD,IFATM,ATMPOT,ATMACT,PNEW)    (cathy_main.f:2945)
+ * The routing (SURF_FLOWTRA, PONDUPD) is not run: PONDNOD stays zero, and
+ * advance returns 3 after a step that ends with PONDING set.  The run
+ * starts over at the next init. */
+int cathy_gpu_flow3d_set_surface(int64_t *handle, const int *surf);
+
 int cathy_gpu_flow3d_set_vegetation(int64_t *handle, const int *nveg,
                                     const int *veg_type, const double *pcana,
                                     const double *pcref, const double *pcwlt,

@@ -997,6 +997,123 @@ void orc_switch_old(int nnod, int *ifatm, double *atmact,
   }
 }
 
+/* The band of a head h against PONDH_MIN (ph) and PMIN, tested in the
+ * order SWITCH tests them: 1 h >= ph, 2 0 <= h < ph, 3 pmin < h < 0,
+ * 4 h <= pmin; 0 none (NaN). */
+static int sw_band(double h, double ph, double pmin) {
+  if (h >= ph) return 1;
+  if (h >= 0.0 && h < ph) return 2;
+  if (h > pmin && h < 0.0) return 3;
+  if (h <= pmin) return 4;
+  return 0;
+}
+
+int orc_switch(int nnod, int *ifatm, double deltat, double pondh_min,
+               double pmin, const double *arenod, const double *pondnod,
+               const double *atmpot, double *atmact, const double *qtranie,
+               double *pnew, double *ovflnod) {
+  /* SRC/switch.f:27-423. */
+  int ponding = 0;
+  for (int i = 0; i < nnod; ++i) {
+    const int fa = ifatm[i];
+    if (fa == -1) {
+      ovflnod[i] = 0.0;
+      continue;
+    }
+    double dif = atmpot[i] - (atmact[i] - qtranie[i]);
+    if (fabs(dif) < 1.0e-14) dif = 0.0;
+    const double pl = pondnod[i] + (dif * deltat / arenod[i]);
+    const double drain = -pondnod[i] * arenod[i] / deltat;
+    const int rain = atmpot[i] >= 0.0, infil = atmact[i] >= 0.0;
+    if (fa == 2 || fa == 1) {
+      /* guards of cases G and H (IFATM 1, evaporation) */
+      if (fa == 1 && !rain && pnew[i] <= pmin) {
+        if (!infil && atmact[i] < atmpot[i]) { /* H */
+          ifatm[i] = 0;
+          atmact[i] = atmpot[i];
+          pnew[i] = pmin;
+          ovflnod[i] = 0.0;
+        }
+        continue;
+      }
+      /* cases B and F (rainfall, exfiltration) end at PL < 0 with IFATM 1;
+       * cases C and D (IFATM 2, evaporation) zero PNEW in bands 2 and 3 */
+      const int exf_rain = rain && !infil;
+      const int zero_p = (fa == 2 && !rain);
+      int b = sw_band(pl, pondh_min, pmin);
+      if (exf_rain && b >= 3) b = 5;
+      switch (b) {
+        case 1:
+          ponding = 1;
+          ifatm[i] = 2;
+          pnew[i] = pl;
+          ovflnod[i] = dif;
+          break;
+        case 2:
+          ifatm[i] = 1;
+          if (zero_p) pnew[i] = 0.0;
+          ovflnod[i] = dif;
+          break;
+        case 3:
+          ifatm[i] = 0;
+          atmact[i] = atmpot[i];
+          if (zero_p) pnew[i] = 0.0;
+          ovflnod[i] = drain;
+          break;
+        case 4:
+          ifatm[i] = 0;
+          atmact[i] = atmpot[i];
+          ovflnod[i] = drain;
+          break;
+        case 5:
+          ifatm[i] = 1;
+          ovflnod[i] = dif;
+          break;
+        default:
+          break;
+      }
+      continue;
+    }
+    if (fa == 0) { /* cases I and J: the same tests on PNEW */
+      const double over = (pnew[i] - pondnod[i]) * arenod[i] / deltat;
+      switch (sw_band(pnew[i], pondh_min, pmin)) {
+        case 1:
+          ponding = 1;
+          ifatm[i] = 2;
+          ovflnod[i] = over;
+          break;
+        case 2:
+          ifatm[i] = 1;
+          ovflnod[i] = over;
+          break;
+        case 3:
+          ifatm[i] = 0;
+          ovflnod[i] = drain;
+          break;
+        case 4:
+          ifatm[i] = 1;
+          pnew[i] = pmin;
+          ovflnod[i] = drain;
+          break;
+        default:
+          break;
+      }
+    }
+  }
+  return ponding;
+}
+
+void orc_adrstn(int nnod, int *ifatm, const double *atmpot, double *atmact,
+                double *pnew, double pmin) {
+  /* SRC/adrstn.f:22-31. */
+  for (int i = 0; i < nnod; ++i)
+    if (pnew[i] <= pmin && (atmpot[i] > 0.0 || atmact[i] < atmpot[i])) {
+      ifatm[i] = 0;
+      atmact[i] = atmpot[i];
+      pnew[i] = pmin;
+    }
+}
+
 int orc_timupd(double *timep, double *time, double *deltat, double dtmin,
                double dtmax, double tmax, double dtmaga, double dtmagm,
                double dtreds, double dtredm, int iter, int ituns1,

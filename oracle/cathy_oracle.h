@@ -215,6 +215,19 @@ int orc_etran(int n, int nnod, int nstr, const double *atmpot,
 /* SRC/switch_old.f:12-53. */
 void orc_switch_old(int nnod, int *ifatm, double *atmact,
                     const double *atmpot, double *pnew, double pmin);
+/* SRC/switch.f:8-423: the coupled (SURF) switching CONVER calls instead
+ * of SWITCH_OLD.  PL = PONDNOD + (ATMPOT - (ATMACT - QTRANIE))*DELTAT/
+ * ARENOD decides the IFATM 1 / 2 cases, PNEW the IFATM 0 ones.  Arrays
+ * are per surface node.  Returns PONDING. */
+int orc_switch(int nnod, int *ifatm, double deltat, double pondh_min,
+               double pmin, const double *arenod, const double *pondnod,
+               const double *atmpot, double *atmact, const double *qtranie,
+               double *pnew, double *ovflnod);
+/* SRC/adrstn.f:13-33: an 'air dry' surface node under rainfall (or whose
+ * actual flux exceeds the potential) goes back to the Neumann condition
+ * at PNEW = PMIN. */
+void orc_adrstn(int nnod, int *ifatm, const double *atmpot, double *atmact,
+                double *pnew, double pmin);
 /* SRC/timupd.f:8-41.  Returns DTGMIN. */
 int orc_timupd(double *timep, double *time, double *deltat, double dtmin,
                double dtmax, double tmax, double dtmaga, double dtmagm,

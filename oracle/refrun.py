@@ -717,15 +717,15 @@ class RefFlowRun:
         if self.surf:   # cathy_main.f:3771: PONDP = PONDING
             self.pondp[0] = a["PONDING"][0]
             stats["PONDING"] = int(a["PONDING"][0] != 0)
-            if a["PONDING"][0]:   # the next step would route (cathy_main.f:2993), which is not run here
-                self.finished = 3
-                return 3, stats
         if a["NOBACK"][0]:
             self.finished = 2
             return 2, stats
         if abs(a["TIME"][0] - p["tmax"]) <= 0.001 * a["DELTAT"][0]:
             self.finished = 1
             return 1, stats
+        if self.surf and a["PONDING"][0]:   # the next step would route first (cathy_main.f:2993)
+            self.finished = 3
+            return 3, stats
         dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
         ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
         self.ref.L.timupd_(dp(a["TIMEP"]), dp(a["TIME"]), dp(a["DELTAT"]), C.byref(C.c_double(p["dtmin"])),

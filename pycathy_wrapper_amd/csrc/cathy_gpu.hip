@@ -461,6 +461,16 @@ int cathy_gpu_flow3d_set_vegetation(int64_t *handle, const int *nveg, const int 
   });
 }
 
+int cathy_gpu_flow3d_set_surface(int64_t *handle, const int *surf) {
+  return guarded([&] {
+    const std::shared_ptr<Flow> f = lookup(handle);
+    CG_REQUIRE(surf && (*surf == 0 || *surf == 1), "SURF must be 0 or 1");
+    f->surf = *surf == 1;
+    f->initialised = 0;
+    return 0;
+  });
+}
+
 int cathy_gpu_flow3d_bench_spmv(int64_t *handle, const int *nrep, const double *flush_mb, double *out2) {
   return guarded([&] {
     const std::shared_ptr<Flow> f = lookup(handle);

@@ -157,8 +157,9 @@ struct Ensemble {
       CG_REQUIRE(f.initialised, who + "flow3d_init not called");
       CG_REQUIRE(f.ordering == CATHY_GPU_ORDER_COLUMN3, who + "the ensemble needs the column ordering");
       CG_REQUIRE(f.prm.iopt == 1, who + "the ensemble runs the Picard linearisation (IOPT = 1)");
-      CG_REQUIRE(!f.lists_on() && !f.veg_on,
-                 who + "listed boundary series and root uptake run per member (cathy_gpu_flow3d_advance)");
+      CG_REQUIRE(!f.lists_on() && !f.veg_on && !f.surf,
+                 who + "listed boundary series, root uptake and the coupled surface switching run per member "
+                       "(cathy_gpu_flow3d_advance)");
       CG_REQUIRE(f.use_plan && f.maxrow <= kUnroll, who + "the ensemble needs the lumped plan assembly (LUMP = 1)");
       CG_REQUIRE(f.ftab_ok, who + "the IC(0) chain tables do not apply to this mesh");
       CG_REQUIRE(f.f32 == f32, who + "members differ in IC(0) factor precision");

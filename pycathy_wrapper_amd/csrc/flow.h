@@ -177,6 +177,10 @@ struct Flow {
   int np_cur = 0, nx_cur = 0, n_atm_off = 0;
   double nx_in = 0.0, nx_out = 0.0;          // sums of the listed Neumann fluxes (fluxmb.f:39-46)
   bool veg_on = false;
+  // coupled surface (ISIMGR = 2): SWITCH in CONVER and ADRSTN at step
+  // starts and back-steps; pond is SWITCH's PONDING flag on the device
+  bool surf = false;
+  DBuf<int> pond;
   DBuf<int> vegc, permd;
   DBuf<double> zc, vegtab, qtr;
   // seepage faces (sfbc): nodes face by face, top to bottom; SFEX / SFEXIT
@@ -1003,6 +1007,8 @@ struct Flow {
     ckrwe.alloc(nt);
     et1e.alloc(nt);
     ifatm.alloc(nnod); ifatmp.alloc(nnod);
+    pond.alloc(1);
+    hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, 0);
     atmpot.alloc(nnod); atmact.alloc(nnod); atmold.alloc(nnod); qbot.alloc(nnod);
     alloc_solver();
     CG_HIP(hipMemsetAsync(qnode.p, 0, sizeof(double) * n, st));
@@ -1770,6 +1776,21 @@ struct Flow {
     hipLaunchKernelGGL(k_switch_old, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, surf_row.p, ifatm.p,
                        atmact.p, atmpot.p, pnew.p, prm.pmin);
   }
+  // conver.f:61-71: SWITCH_OLD, or SWITCH when the surface is coupled (it
+  // clears PONDING first and sets it where a node ponds)
+  void switch_conv() {
+    if (!surf) { switch_old(); return; }
+    hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, 0);
+    hipLaunchKernelGGL(k_switch_surf, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, surf_row.p, ifatm.p,
+                       atmact.p, atmpot.p, veg_on ? (const double *)qtr.p : (const double *)nullptr, arenod.p,
+                       deltat, prm.pondh_min, prm.pmin, pnew.p, pond.p);
+  }
+  // cathy_main.f:2942-2946 and bkstep.f:135-139: SWITCH_OLD, or ADRSTN
+  void switch_start() {
+    if (!surf) { switch_old(); return; }
+    hipLaunchKernelGGL(k_adrstn, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, surf_row.p, ifatm.p,
+                       atmpot.p, atmact.p, pnew.p, prm.pmin);
+  }
 
   // FLOW3D, Picard branch (flow3d.f:93-294).
   void flow3d() {
@@ -1780,7 +1801,7 @@ struct Flow {
       else picard_iteration(err);
       c_picard += 1;
       masbal_norms();
-      if (wants_switch()) switch_old();
+      if (wants_switch()) switch_conv();
       if (picard_continues()) {
         hipLaunchKernelGGL(k_next_iter, dim3(gn), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
                            pold.p, ptnew.p, ptold.p);
@@ -1915,6 +1936,8 @@ struct Flow {
     std::vector<double> h(n);
     for (int i = 0; i < n; ++i) h[perm[i]] = ptimep_ref[i];
     ptimep.upload(h, st);
+    // inital.f:201-205: PONDING starts clear (IPOND = 0)
+    hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, 0);
     // INIT1 (init1.f:37-60)
     deltat = prm.deltat;
     if (deltat > prm.dtmax) deltat = prm.dtmax;
@@ -2054,7 +2077,9 @@ struct Flow {
     } else {
       atmbak();
     }
-    switch_old();
+    switch_start();
+    if (surf)  // bkstep.f:148-149: PONDING = PONDP, which is clear (a ponded step ends the run)
+      hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, 0);
     guess_point();
     hipLaunchKernelGGL(k_start_iter, dim3(grid_for(n)), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
                        pold.p, ptnew.p, ptold.p);
@@ -2098,7 +2123,7 @@ struct Flow {
     atmnxt();
     atm_off_apply(ifatm.p);
     etran();
-    switch_old();
+    switch_start();
     guess_point();
     hipLaunchKernelGGL(k_start_iter, dim3(grid_for(n)), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
                        pold.p, ptnew.p, ptold.p);
@@ -2153,6 +2178,8 @@ struct Flow {
     CG_HIP(hipMemcpyAsync(ckrwp.p, ckrw.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_sw_kr, dim3(grid_for(n)), dim3(kBlock), 0, st, n, ptimep.p, soil(), swtimep.p,
                        (double *)nullptr);
+    int ponding = 0;
+    if (surf) pond.download(&ponding, 1, st);
     CG_HIP(hipStreamSynchronize(st));
     last_counters[0] = c_picard - pic0;
     last_counters[1] = c_pcg - pcg0;
@@ -2161,9 +2188,12 @@ struct Flow {
       stv[28] = last_counters[4];  // wall ms from the step's start to its end
       stv[29] = last_counters[0];  // Picard iterations of the step, back-stepped attempts included
       stv[30] = last_counters[1];  // PCG iterations likewise
+      stv[31] = ponding;           // PONDING at the step's end (coupled surface)
     }
     if (noback) return finished = 2;
     if (std::fabs(time - prm.tmax) <= 0.001 * deltat) return finished = 1;
+    // a ponded surface: the next step would route first (cathy_main.f:2993)
+    if (ponding) return finished = 3;
     dtgmin = timupd();
     nstep += 1;
     iter = 1;

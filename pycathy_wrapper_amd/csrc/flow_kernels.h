@@ -296,6 +296,12 @@ __global__ void k_norms(int n, const int *refid, const double *pnew, const doubl
 __global__ void k_flux(int nnod, const int *ifatm, const double *atmact, const double *qbot, int nq,
                        double *part6);
 __global__ void k_norms_final(int nb, int nbf, NormPart part, const double *part6, double *res);
+__global__ void k_switch_surf(int nnod, const int *surf_row, int *ifatm, double *atmact, const double *atmpot,
+                              const double *qtr, const double *arenod, double deltat, double pondh_min, double pmin,
+                              double *pnew, int *ponding);
+__global__ void k_adrstn(int nnod, const int *surf_row, int *ifatm, const double *atmpot, double *atmact,
+                         double *pnew, double pmin);
+__global__ void k_put_int(int *x, int v);
 __global__ void k_switch_old(int nnod, const int *surf_row, int *ifatm, double *atmact, const double *atmpot,
                              double *pnew, double pmin);
 __global__ void k_next_iter(int n, double tetaf, const double *pnew, const double *ptimep, double *pold,

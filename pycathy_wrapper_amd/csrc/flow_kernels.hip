@@ -1483,6 +1483,90 @@ __global__ void k_switch_old(int nnod, const int *__restrict__ surf_row, int *__
   switch_old_body(self_blk(), nnod, surf_row, ifatm, atmact, atmpot, pnew, pmin);
 }
 
+// switch.f:27-423: the switching CONVER calls when the surface is coupled
+// (SURF).  The routing does not run on the device, so no surface node holds
+// ponded water (PONDNOD = 0) and the head a node would pond to is
+// PL = ATMDIF*DELTAT/ARENOD with ATMDIF = ATMPOT - (ATMACT - QTRANIE); the
+// IFATM 0 cases test PNEW itself.  Bands, in SWITCH's order: PL >= PONDH_MIN
+// (the node ponds: IFATM 2, PNEW = PL, *ponding = 1), 0 <= PL < PONDH_MIN
+// (IFATM 1), PMIN < PL < 0 and PL <= PMIN (IFATM 0, ATMACT = ATMPOT);
+// rainfall with exfiltration ends every PL < 0 at IFATM 1; a ponded node
+// under evaporation zeroes PNEW in the middle two bands; an IFATM 1 node at
+// or below PMIN under evaporation stays unless it exfiltrates faster than
+// the potential rate.  OVFLNOD, which only the routing reads, is not kept.
+__global__ void k_switch_surf(int nnod, const int *__restrict__ surf_row, int *__restrict__ ifatm,
+                              double *__restrict__ atmact, const double *__restrict__ atmpot,
+                              const double *__restrict__ qtr, const double *__restrict__ arenod, double deltat,
+                              double pondh_min, double pmin, double *__restrict__ pnew, int *__restrict__ ponding) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnod) return;
+  const int fa = ifatm[i];
+  if (fa == -1) return;
+  const int r = surf_row[i];
+  const double ap = atmpot[i], aa = atmact[i], pn = pnew[r];
+  auto band = [&](double h) {
+    if (h >= pondh_min) return 1;
+    if (h >= 0.0 && h < pondh_min) return 2;
+    if (h > pmin && h < 0.0) return 3;
+    if (h <= pmin) return 4;
+    return 0;
+  };
+  if (fa == 0) {
+    switch (band(pn)) {
+      case 1: ifatm[i] = 2; *ponding = 1; break;
+      case 2: ifatm[i] = 1; break;
+      case 4: ifatm[i] = 1; pnew[r] = pmin; break;
+      default: break;
+    }
+    return;
+  }
+  if (fa != 1 && fa != 2) return;
+  const bool rain = ap >= 0.0, infil = aa >= 0.0;
+  if (fa == 1 && !rain && pn <= pmin) {  // cases G and H
+    if (!infil && aa < ap) {
+      ifatm[i] = 0;
+      atmact[i] = ap;
+      pnew[r] = pmin;
+    }
+    return;
+  }
+  double dif = ap - (aa - (qtr ? qtr[r] : 0.0));
+  if (fabs(dif) < 1.0e-14) dif = 0.0;
+  const double pl = 0.0 + (dif * deltat / arenod[i]);
+  const bool zero_p = fa == 2 && !rain;
+  int b = band(pl);
+  if (rain && !infil && b >= 3) b = 5;  // cases B and F
+  switch (b) {
+    case 1: ifatm[i] = 2; pnew[r] = pl; *ponding = 1; break;
+    case 2: ifatm[i] = 1; if (zero_p) pnew[r] = 0.0; break;
+    case 3: ifatm[i] = 0; atmact[i] = ap; if (zero_p) pnew[r] = 0.0; break;
+    case 4: ifatm[i] = 0; atmact[i] = ap; break;
+    case 5: ifatm[i] = 1; break;
+    default: break;
+  }
+}
+
+// adrstn.f:22-31: an 'air dry' surface node under rainfall, or whose actual
+// flux is below the potential one, returns to the Neumann condition at
+// PNEW = PMIN (step starts and back-steps of a coupled run).
+__global__ void k_adrstn(int nnod, const int *__restrict__ surf_row, int *__restrict__ ifatm,
+                         const double *__restrict__ atmpot, double *__restrict__ atmact, double *__restrict__ pnew,
+                         double pmin) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnod) return;
+  const int r = surf_row[i];
+  if (pnew[r] <= pmin && (atmpot[i] > 0.0 || atmact[i] < atmpot[i])) {
+    ifatm[i] = 0;
+    atmact[i] = atmpot[i];
+    pnew[r] = pmin;
+  }
+}
+
+// *x = v (one thread).
+__global__ void k_put_int(int *__restrict__ x, int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *x = v;
+}
+
 // flow3d.f:235-240 / cathy_main.f:2976-2978: POLD = PNEW, PTOLD = PTNEW,
 // PTNEW = TETAF*PNEW + (1-TETAF)*PTIMEP (weight.f).
 __device__ __forceinline__ void next_iter_body(Blk bk, int n, double tetaf, const double *__restrict__ pnew,

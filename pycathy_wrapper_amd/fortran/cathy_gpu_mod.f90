@@ -104,6 +104,14 @@ module cathy_gpu
       real(c_double), intent(out) :: sfq(*)
     end function
 
+    ! SURF = 1: the coupled switching of ISIMGR = 2 (SWITCH in CONVER, ADRSTN)
+    integer(c_int) function cathy_gpu_flow3d_set_surface(handle, surf) &
+        bind(C, name="cathy_gpu_flow3d_set_surface")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), intent(in) :: handle
+      integer(c_int), intent(in) :: surf
+    end function
+
     integer(c_int) function cathy_gpu_flow3d_init(handle, ptimep) bind(C, name="cathy_gpu_flow3d_init")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), intent(in) :: handle

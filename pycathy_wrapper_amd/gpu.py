@@ -25,9 +25,9 @@ ORDER_COLUMN3 = 1
 
 STAT_NAMES = ["NSTEP", "DELTAT", "TIME", "ITER", "NITERT", "BACK", "STORE1", "STORE2", "DSTORE", "CDSTOR",
               "VIN", "CVIN", "VOUT", "CVOUT", "VINOUT", "VTOT", "ERRAS", "ERREL", "CERRAS", "CAERAS", "PINF",
-              "ITRTOT", "ITLIN", "KLSFAI", "PL2", "FINF", "FL2", "IKMAX", "PICARD", "PCG"]
+              "ITRTOT", "ITLIN", "KLSFAI", "PL2", "FINF", "FL2", "IKMAX", "PICARD", "PCG", "PONDING"]
 # stats(32) slots of STAT_NAMES; slot 28 (the step's wall-clock ms) is kept apart
-_STAT_SLOTS = list(range(28)) + [29, 30]
+_STAT_SLOTS = list(range(28)) + [29, 30, 31]
 
 
 def _stat_dict(st) -> dict:
@@ -114,6 +114,8 @@ class FlowGPU:
             self.set_vegetation(**problem.veg)
         if getattr(problem, "seepage", None) is not None:
             self.set_seepage(problem.seepage["faces"], problem.seepage.get("isfcvg", 0))
+        if getattr(problem, "surf", False):
+            self.set_surface(True)
         self.init(problem.psi0)
 
     def set_params(self, p, hspatm: int | None = None):
@@ -183,6 +185,12 @@ class FlowGPU:
         self._keep["veg"] = (vt, tab, z)
         _check(lib().cathy_gpu_flow3d_set_vegetation(C.byref(self.h), _i(len(tab[0])), _p(vt),
                                                      *[_p(a) for a in tab], _p(z)), "set_vegetation")
+
+    def set_surface(self, surf: bool) -> None:
+        """The coupled surface of ISIMGR = 2 (SWITCH in CONVER, ADRSTN at
+        step starts; advance returns 3 after a step that ends ponded) or the
+        subsurface-only switching; call init again afterwards."""
+        _check(lib().cathy_gpu_flow3d_set_surface(C.byref(self.h), _i(1 if surf else 0)), "set_surface")
 
     def init(self, psi0):
         p = np.ascontiguousarray(psi0, dtype=np.float64)

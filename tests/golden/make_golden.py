@@ -41,12 +41,13 @@ container, where the reference tree is mounted at /root/reference).
    output/psi_sw.npz (the psi and sw blocks of output/psi and output/sw,
    values as printed, with their NSTEP and TIME) and, for weil, the head of
    output/risul (parameter echo, soil table, initial heads).
-9. bc_noflow_ref.npz, outlets_ref.npz -- those two projects run by the
+9. bc_noflow_ref.npz, outlets_ref.npz, weil_ref.npz -- those projects run by the
    reference's FLOW3D with the coupled switching (SURF true: SWITCH in
    CONVER, ADRSTN at step starts and in BKSTEP), their own nansfneubc /
    sfbc / atmbc files read by the reference's BCONE / SFVONE / ATMONE
    (oracle/_ref/bc): per-step statistics for the steps the stored mbeconv
-   holds, psi/Sw/IFATM at selected steps.
+   holds, psi/Sw/IFATM at selected steps.  weil ponds in its first step
+   (its surface starts saturated), so its run stops there with rc 3.
 
 Only data goes into the fixtures; no reference source text.
 """
@@ -387,7 +388,8 @@ def make_project_run(name: str, out: str, steps_saved=(1, 2, 5, 10, 50, 100, 200
     root = os.path.join(HERE, name)
     pr = PJ.load_project(root)
     m = pr.mesh
-    nmb = sum(1 for ln in open(os.path.join(root, "output/mbeconv")) if not ln.startswith("#"))
+    mbe = os.path.join(root, "output/mbeconv")
+    nmb = sum(1 for ln in open(mbe) if not ln.startswith("#")) if os.path.exists(mbe) else 10 ** 9
     io = lambda k: os.path.join(root, "input", k)
     rr = R.RefFlowRun(P.RefLib(P.REF_BC_SO), m, pr.soil, pr.psi0, 0.0, pr.params.as_dict(), atmbc=io("atmbc"),
                       dirbc=io("nansfdirbc"), neubc=io("nansfneubc"), veg=pr.veg, sfbc=io("sfbc"), surf=pr.surf)
@@ -424,3 +426,4 @@ if __name__ == "__main__":
     make_project_run("bc_noflow_project", os.path.join(HERE, "bc_noflow_ref.npz"))
     make_project_run("outlets_project", os.path.join(HERE, "outlets_ref.npz"),
                      steps_saved=(1, 2, 5, 10, 50, 100, 200, 300, 400, 500, 600, 700))
+    make_project_run("weil_project", os.path.join(HERE, "weil_ref.npz"))

@@ -23,7 +23,9 @@ hold).
    the mesh and the coupled switching reproduce the reference program end
    to end on its own example inputs, with no ponding (OVL. FLUX is zero
    throughout both runs' hgatmsf).  Where oracle/_ref/bc is built, the
-   bc_noflow golden is re-derived and must be identical.
+   bc_noflow golden is re-derived and must be identical.  weil_exemple
+   ponds in its first step: that step equals the stored mbeconv's first
+   row and the run stops (rc 3) where the reference would start routing.
 4. Refusals: a coupled project that starts ponded (IPOND != 0; inital.f:201
    sets PONDING, so step 1 would route); ISIMGR = 3; NLRELX != 0.
 """
@@ -168,6 +170,28 @@ def test_coupled_runs_equal_reference_program_outputs(name):
                 assert np.all(np.abs(gold[kind][i] - b) <= 5.0000001e-7 * np.abs(b) + 1e-300), (ns, kind)
             hit += 1
     assert hit >= 1
+
+
+def test_weil_first_step_equals_reference_then_ponds():
+    """weil_exemple starts saturated to the surface (ICVHE), so ATMONE makes
+    every surface node IFATM 2 under PONDH_MIN = 0 and the first step's
+    SWITCH ponds them: the reference program routes from step 2 on.  Its
+    step 1 (eight back-steps, DELTAT 5 s down to 0.01953125 s) is the coupled
+    FLOW3D's alone, and the golden equals the stored mbeconv's first row;
+    the run stops there with rc 3."""
+    gold = np.load(os.path.join(GOLD, "weil_ref.npz"))
+    col = {nm: j for j, nm in enumerate(str(v) for v in gold["stat_names"])}
+    assert len(gold["stats"]) == 1 and int(gold["rc_last"]) == 3
+    st = gold["stats"][0]
+    assert st[col["PONDING"]] == 1 and st[col["BACK"]] == 8
+    text = str(np.load(os.path.join(GOLD, "weil_outputs.npz"))["mbeconv"])
+    row = [ln for ln in text.splitlines() if not ln.startswith("#")][0].split()
+    for c, key in MBE_COLS.items():
+        if key == "ITER":
+            assert int(st[col[key]]) == int(row[c])
+        else:
+            assert _printed_close(st[col[key]], row[c]), (key, st[col[key]], row[c])
+    assert np.any(gold["ifatm"][0] == 2)   # the ponded nodes
 
 
 def test_bc_noflow_golden_rederived():
