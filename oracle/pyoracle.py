@@ -283,10 +283,12 @@ class RefLib:
             import json
             with open(sz) as f:
                 d = json.load(f)
+            self.sizes = d
             self.nmax, self.maxstr, self.maxzon, self.maxtrm = d["nmax"], d["maxstr"], d["maxzon"], d["maxtrm"]
             self.npmax, self.nqmax = d.get("npmax", 1), d.get("nqmax", 1)
             self.nsfmax, self.nnsfmx, self.maxveg = d.get("nsfmax", 1), d.get("nnsfmx", 1), d.get("maxveg", 1)
         else:
+            self.sizes = {}
             self.nmax, self.maxstr, self.maxzon, self.maxtrm = REF_NMAX, REF_MAXSTR, 1, REF_MAXTRM
             self.npmax = self.nqmax = self.nsfmax = self.nnsfmx = self.maxveg = 1
         self.vg = np.ctypeslib.as_array((C.c_double * (VG_NFIELD * self.nmax)).in_dll(self.L, "vg_"))

@@ -87,3 +87,35 @@ C  REFUNIT_CLOSE disconnects the unit of KIND again.
       IF (KIND .EQ. 9 .AND. IIN9 .NE. 0) CLOSE(IIN9)
       RETURN
       END
+C
+C  REFUNIT_OPENN connects unit IU to a file: MODE 0 an existing file to
+C  read, MODE 1 a file to write (replaced).  CODES / NC / IERR as for
+C  REFATM_OPEN.  Used to give DATIN its input units (IIN1 .. IIN39) and
+C  the output units it writes its echo to.
+      SUBROUTINE REFUNIT_OPENN(IU, CODES, NC, MODE, IERR)
+      IMPLICIT NONE
+      INTEGER IU, NC, MODE, IERR, I
+      INTEGER CODES(*)
+      CHARACTER*4096 FNAME
+      IERR = -1
+      IF (NC .LT. 1 .OR. NC .GT. 4096) RETURN
+      FNAME = ' '
+      DO I = 1, NC
+         FNAME(I:I) = CHAR(CODES(I))
+      END DO
+      CLOSE(IU)
+      IF (MODE .EQ. 0) THEN
+         OPEN(IU, FILE=FNAME(1:NC), STATUS='OLD', IOSTAT=IERR)
+      ELSE
+         OPEN(IU, FILE=FNAME(1:NC), STATUS='REPLACE', IOSTAT=IERR)
+      END IF
+      RETURN
+      END
+C
+C  REFUNIT_CLOSEN disconnects unit IU.
+      SUBROUTINE REFUNIT_CLOSEN(IU)
+      IMPLICIT NONE
+      INTEGER IU
+      CLOSE(IU)
+      RETURN
+      END

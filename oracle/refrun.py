@@ -97,7 +97,7 @@ class RefFlowRun:
 
     def __init__(self, ref: RefLib, mesh, soil, psi0, rain, prm: dict, atmbc: str | None = None,
                  dirbc: str | None = None, neubc: str | None = None, veg: dict | None = None,
-                 sfbc: str | None = None, isfcvg: int = 0, surf: bool = False):
+                 sfbc: str | None = None, isfcvg: int = 0, surf: bool = False, routing=None):
         assert soil["nzone"] <= ref.maxzon, "more zones than the reference build's MAXZON"
         assert mesh.nstr <= ref.maxstr and mesh.n <= ref.nmax, "mesh larger than the reference build"
         self.ref = ref
@@ -193,6 +193,17 @@ class RefFlowRun:
         a["SURF"][0] = 1 if surf else 0
         a["PONDING"][0] = 0
         self.pondp = np.zeros(4, dtype=np.int32)
+        # ``routing`` (refproject.RefProject: the project read by DATIN, whose
+        # COMMON /RIVERNETWORK/ and /SURFWATER/ it filled) adds the routing
+        # block of the time loop: a step that starts ponded routes the
+        # overland flow (SURF_FLOWTRA) and PONDUPD sets the ponded nodes'
+        # heads and fluxes (cathy_main.f:2993-3058), back-steps restore the
+        # routing state (:3145-3157, bkstep.f:148-157) and accepted steps
+        # advance it (:3770-3782).
+        self.rp = routing
+        if routing is not None:
+            assert surf, "the routing block needs the coupled switching"
+            self._init_routing()
         self._tmp = None
         if self.lists:
             self._tmp = tempfile.mkdtemp(prefix="refbc_")
@@ -560,6 +571,70 @@ class RefFlowRun:
         self.ref.L.switch_old_(C.byref(C.c_int(self.nnod)), ip(a["IFATM"]), dp(a["ATMACT"]), dp(a["ATMPOT"]),
                                dp(a["PNEW"]))
 
+    def _init_routing(self):
+        """INIT_SR (init_sr.f) on a clean /SURFWATER/ state, and the time
+        loop's own routing arrays (cathy_main.f: OVFLCEL, PONDCEL, OVFLP)."""
+        rp, nnod = self.rp, self.nnod
+        sw = rp.surfwater()
+        for k, v in sw.items():
+            if not k.startswith("ELEVATION") and k not in ("HRES", "ARES", "H_SOGLIA", "H_FONDO"):
+                v[:] = 0.0
+        i32 = lambda k: np.zeros(k, dtype=np.int32)
+        self.swc = sw
+        self.r = dict(ovflcel=np.zeros(rp.maxcel + 16), pondcel=np.zeros(rp.maxcel + 16),
+                      ovflp=np.zeros(nnod + 16), celtype=i32(rp.maxcel + 16), cells_r=i32(rp.maxcel + 16),
+                      nsurf=i32(1), nsurft=i32(1), nsurft_tb=i32(1), transp=i32(1),
+                      dummy_n=np.zeros(max(self.n, rp.maxcel) + 16))
+        self.a["OVFLNOD"][:] = 0.0
+        self.a["PONDNOD"][:] = 0.0
+        self.routed = 0
+
+    def _route_block(self):
+        """cathy_main.f:2993-3058 (TRANSP false): save the routing state, route
+        (SURF_FLOWTRA) and update the surface conditions (PONDUPD)."""
+        a, rp, r, sw = self.a, self.rp, self.r, self.swc
+        sw["AK_MAX_SAV"][0] = sw["AK_MAX"][0]
+        for k in ("Q_IN_KK_SN", "Q_OUT_KK_SN_1", "Q_OUT_KK_SN_2", "VOLUME_KK_SN"):
+            sw[k + "_SAV"][:] = sw[k]
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+        I = lambda v: C.byref(C.c_int(v))
+        g = rp.a
+        dn = r["dummy_n"]
+        self.ref.L.surf_flowtra_(
+            ip(g["NCELL"]), I(self.nnod), ip(g["NROW"]), ip(g["NCOL"]), ip(g["NTRI"]), ip(g["DOSTEP"]), I(0),
+            ip(g["NCELL_COARSE"]), ip(g["NCELNL"]), ip(g["CELL"]), ip(g["INDEX"]), ip(g["INDEX_WITH_LAKES"]),
+            ip(g["TIPO_R"]), ip(g["RESERVR"]), dp(g["DEM_MAP"]), ip(g["LAKES_MAP"]), ip(g["CELLCOL"]),
+            ip(g["CELLROW"]), ip(r["celtype"]), ip(r["cells_r"]), ip(g["TP2D"]), ip(g["TRIANG"]),
+            dp(a["TIME"]), dp(a["DELTAT"]), dp(a["ARENOD"]), dp(a["OVFLNOD"]), dp(a["PONDNOD"]),
+            dp(r["ovflcel"]), dp(r["pondcel"]), dp(dn), dp(dn), dp(dn), dp(dn), ip(g["N_HA"]), ip(r["nsurf"]),
+            ip(r["nsurft"]), ip(r["nsurft_tb"]), ip(r["transp"]), dp(dn), dp(dn), dp(dn))
+        self.ref.L.pondupd_(I(self.nnod), ip(a["PONDING"]), ip(a["IFATM"]), dp(a["PONDH_MIN"]), dp(a["DELTAT"]),
+                            dp(a["PONDNOD"]), dp(a["ARENOD"]), dp(a["ATMPOT"]), dp(a["ATMACT"]), dp(a["PNEW"]))
+        self.routed += 1
+
+    def _route_backstep(self):
+        """cathy_main.f:3145-3157: the attempt's saved routing state becomes
+        the one BKSTEP restores; the next-level arrays are cleared."""
+        sw = self.swc
+        sw["AK_MAX_P"][0] = sw["AK_MAX_SAV"][0]
+        for k in ("Q_IN_KK_SN", "Q_OUT_KK_SN_1", "Q_OUT_KK_SN_2", "VOLUME_KK_SN"):
+            sw[k + "_P"][:] = sw[k + "_SAV"]
+        for k in ("Q_IN_KKP1_SN", "Q_OUT_KKP1_SN_1", "Q_OUT_KKP1_SN_2", "VOLUME_KKP1_SN"):
+            sw[k][:] = 0.0
+
+    def _route_step_end(self):
+        """cathy_main.f:3223-3226 and 3770-3782: no ponded head where the
+        surface is unsaturated; OVFLP, and the routing state advanced."""
+        a, sw, nnod = self.a, self.swc, self.nnod
+        pn = a["PNEW"][:nnod]
+        a["PONDNOD"][:nnod][pn <= 0.0] = 0.0
+        self.r["ovflp"][:nnod] = a["OVFLNOD"][:nnod]
+        for kk, kp in (("Q_IN_KK_SN", "Q_IN_KKP1_SN"), ("Q_OUT_KK_SN_1", "Q_OUT_KKP1_SN_1"),
+                       ("Q_OUT_KK_SN_2", "Q_OUT_KKP1_SN_2"), ("VOLUME_KK_SN", "VOLUME_KKP1_SN")):
+            sw[kk][:] = sw[kp]
+            sw[kp][:] = 0.0
+
     def _adrstn(self):
         a = self.a
         dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
@@ -617,7 +692,8 @@ class RefFlowRun:
             ip(a["IFATM"]), ip(a["IFATMP"]), ip(sf["nsfnum"]), ip(sf["sfex"]), ip(sf["sfexit"]), ip(sf["sfexp"]),
             ip(ints["SURF"]), ip(ints["PONDING"]), ip(ints["PONDP"]), ip(a["DTGMIN"]),
             dp(a["TIME"]), dp(a["TIMEP"]), dp(a["DELTAT"]), dp(a["DTMIN"]), dp(a["DTREDM"]), dp(a["DTREDS"]),
-            dp(a["TETAF"]), dp(a["OVFLNOD"]), dp(np.zeros(nnod + 1)), dp(dir_["bc"]), dp(dir_["tim"]),
+            dp(a["TETAF"]), dp(a["OVFLNOD"]), dp(self.r["ovflp"] if self.rp is not None else np.zeros(nnod + 1)),
+            dp(dir_["bc"]), dp(dir_["tim"]),
             dp(dir_["inp"]), dp(neu_["bc"]), dp(neu_["tim"]), dp(neu_["inp"]), dp(a["ATMTIM"]), dp(a["ATMINP"]),
             dp(a["ARENOD"]),
             dp(a["POLD"]), dp(a["PTOLD"]), dp(a["PNEW"]), dp(a["PTNEW"]), dp(a["PTIMEP"]),
@@ -681,12 +757,19 @@ class RefFlowRun:
             if self.lists:   # cathy_main.f:2983 (label 100): NODDIR starts with the fixed-head nodes
                 k = int(a["NP"][0])
                 a["NODDIR"][:k] = a["CONTP"][:k]
+            if self.rp is not None and a["PONDING"][0]:
+                self._route_block()
+            a["POLD"][:nnod] = a["PNEW"][:nnod]   # cathy_main.f:3099
             self._flow3d()
             if not a["BCKSTP"][0]:
                 break
+            if self.rp is not None:
+                self._route_backstep()
             self._bkstep()
             self._neumann(self.ckrwp)
             backs += 1
+        if self.rp is not None:
+            self._route_step_end()
         self._chvelo("PNEW")
         mb = self._mb()
         st1 = np.zeros(1); st2 = np.array([mb["STORE2"]])
@@ -723,7 +806,7 @@ class RefFlowRun:
         if abs(a["TIME"][0] - p["tmax"]) <= 0.001 * a["DELTAT"][0]:
             self.finished = 1
             return 1, stats
-        if self.surf and a["PONDING"][0]:   # the next step would route first (cathy_main.f:2993)
+        if self.surf and self.rp is None and a["PONDING"][0]:   # the next step would route first (cathy_main.f:2993)
             self.finished = 3
             return 3, stats
         dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))

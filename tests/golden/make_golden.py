@@ -46,8 +46,11 @@ container, where the reference tree is mounted at /root/reference).
    CONVER, ADRSTN at step starts and in BKSTEP), their own nansfneubc /
    sfbc / atmbc files read by the reference's BCONE / SFVONE / ATMONE
    (oracle/_ref/bc): per-step statistics for the steps the stored mbeconv
-   holds, psi/Sw/IFATM at selected steps.  weil ponds in its first step
-   (its surface starts saturated), so its run stops there with rc 3.
+   holds, psi/Sw/IFATM/PONDNOD at selected steps.  weil ponds from its
+   first step on (its surface starts saturated): its run takes the whole
+   routing block of the time loop (oracle/refproject.py reads the project
+   with the reference's DATIN, whose drainage network SURF_FLOWTRA routes)
+   to TMAX = 7200 s.
 
 Only data goes into the fixtures; no reference source text.
 """
@@ -329,6 +332,13 @@ PROJECTS = {
 PROJECT_FILES = ("input/parm", "input/soil", "input/ic", "input/atmbc", "input/dem_parameters",
                  "input/nansfdirbc", "input/nansfneubc", "input/sfbc", "input/root_map", "prepro/dem",
                  "prepro/zone")
+# the drainage network and reservoir files DATIN reads for ISIMGR = 2
+# (weil only: the project whose surface ponds)
+ROUTING_FILES = ("input/posizione_serb", "input/livelli_iniz_s", "prepro/lakes_map", "prepro/qoi_a",
+                 "prepro/dtm_w_1", "prepro/dtm_w_2", "prepro/dtm_p_outflow_1", "prepro/dtm_p_outflow_2",
+                 "prepro/dtm_local_slope_1", "prepro/dtm_local_slope_2", "prepro/dtm_epl_1", "prepro/dtm_epl_2",
+                 "prepro/dtm_kSs1_sf_1", "prepro/dtm_kSs1_sf_2", "prepro/dtm_Ws1_sf_1", "prepro/dtm_Ws1_sf_2",
+                 "prepro/dtm_b1_sf", "prepro/dtm_y1_sf", "prepro/dtm_nrc")
 
 
 def read_blocks(path: str):
@@ -350,7 +360,7 @@ def copy_projects() -> None:
     import shutil
     for name, src in PROJECTS.items():
         dst = os.path.join(HERE, name)
-        for rel in PROJECT_FILES:
+        for rel in PROJECT_FILES + (ROUTING_FILES if name == "weil_project" else ()):
             s = os.path.join(REF, src, rel)
             if os.path.exists(s):
                 os.makedirs(os.path.dirname(os.path.join(dst, rel)), exist_ok=True)
@@ -377,7 +387,7 @@ def copy_projects() -> None:
         print(f"{name}: psi blocks at steps {list(arrs['psi_nstep'])}")
 
 
-def make_project_run(name: str, out: str, steps_saved=(1, 2, 5, 10, 50, 100, 200, 300)) -> None:
+def make_project_run(name: str, out: str, steps_saved=(1, 2, 5, 10, 50, 100, 200, 300), nsteps=10 ** 9) -> None:
     """Item 9: the project run by the reference's FLOW3D with the coupled
     switching, for as many steps as the stored mbeconv holds."""
     sys.path.insert(0, ROOT)
@@ -385,15 +395,20 @@ def make_project_run(name: str, out: str, steps_saved=(1, 2, 5, 10, 50, 100, 200
     from pycathy_wrapper_amd import project as PJ
     import pyoracle as P
     import refrun as R
+    import refproject as RP
     root = os.path.join(HERE, name)
     pr = PJ.load_project(root)
     m = pr.mesh
     mbe = os.path.join(root, "output/mbeconv")
-    nmb = sum(1 for ln in open(mbe) if not ln.startswith("#")) if os.path.exists(mbe) else 10 ** 9
+    nmb = sum(1 for ln in open(mbe) if not ln.startswith("#")) if os.path.exists(mbe) else nsteps
     io = lambda k: os.path.join(root, "input", k)
-    rr = R.RefFlowRun(P.RefLib(P.REF_BC_SO), m, pr.soil, pr.psi0, 0.0, pr.params.as_dict(), atmbc=io("atmbc"),
-                      dirbc=io("nansfdirbc"), neubc=io("nansfneubc"), veg=pr.veg, sfbc=io("sfbc"), surf=pr.surf)
-    stats, psi, sw, ifatm, saved, keys = [], [], [], [], [], None
+    ref = P.RefLib(P.REF_BC_SO)
+    # the routing block where the project holds its drainage network
+    routing = RP.RefProject(ref, root) if os.path.exists(os.path.join(root, "prepro", "qoi_a")) else None
+    rr = R.RefFlowRun(ref, m, pr.soil, pr.psi0, 0.0, pr.params.as_dict(), atmbc=io("atmbc"),
+                      dirbc=io("nansfdirbc"), neubc=io("nansfneubc"), veg=pr.veg, sfbc=io("sfbc"), surf=pr.surf,
+                      routing=routing)
+    stats, psi, sw, ifatm, pond, saved, keys = [], [], [], [], [], [], None
     k = 0
     while True:
         rc, s = rr.step()
@@ -406,11 +421,15 @@ def make_project_run(name: str, out: str, steps_saved=(1, 2, 5, 10, 50, 100, 200
             psi.append(rr.psi().copy())
             sw.append(rr.sw().copy())
             ifatm.append(rr.a["IFATM"][:m.nnod].copy())
+            pond.append(rr.a["PONDNOD"][:m.nnod].copy())
         if last:
             break
     rr.close()
+    if routing is not None:
+        routing.close()
     np.savez_compressed(out, stat_names=np.array(keys), stats=np.array(stats), steps=np.array(saved),
-                        psi=np.array(psi), sw=np.array(sw), ifatm=np.array(ifatm), rc_last=rc)
+                        psi=np.array(psi), sw=np.array(sw), ifatm=np.array(ifatm), pondnod=np.array(pond),
+                        rc_last=rc, routed=rr.routed if routing is not None else 0)
     print(f"{name}: {k} steps, rc {rc}, saved {saved}")
 
 
@@ -426,4 +445,5 @@ if __name__ == "__main__":
     make_project_run("bc_noflow_project", os.path.join(HERE, "bc_noflow_ref.npz"))
     make_project_run("outlets_project", os.path.join(HERE, "outlets_ref.npz"),
                      steps_saved=(1, 2, 5, 10, 50, 100, 200, 300, 400, 500, 600, 700))
-    make_project_run("weil_project", os.path.join(HERE, "weil_ref.npz"))
+    make_project_run("weil_project", os.path.join(HERE, "weil_ref.npz"),
+                     steps_saved=(1, 2, 10, 50, 100, 179, 235))

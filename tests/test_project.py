@@ -24,8 +24,9 @@ hold).
    to end on its own example inputs, with no ponding (OVL. FLUX is zero
    throughout both runs' hgatmsf).  Where oracle/_ref/bc is built, the
    bc_noflow golden is re-derived and must be identical.  weil_exemple
-   ponds in its first step: that step equals the stored mbeconv's first
-   row and the run stops (rc 3) where the reference would start routing.
+   ponds from its first step on: with the routing block of the time loop
+   (the reference's SURF_FLOWTRA and PONDUPD on the project as its DATIN
+   reads it) its 235 steps equal the stored mbeconv, psi and sw.
 4. Refusals: a coupled project that starts ponded (IPOND != 0; inital.f:201
    sets PONDING, so step 1 would route); ISIMGR = 3; NLRELX != 0.
 """
@@ -172,26 +173,49 @@ def test_coupled_runs_equal_reference_program_outputs(name):
     assert hit >= 1
 
 
-def test_weil_first_step_equals_reference_then_ponds():
+def _blocks(text: str):
+    """(NSTEP, values) of each block of a psi / sw output text."""
+    lines = text.splitlines()
+    heads = [i for i, ln in enumerate(lines) if "NSTEP" in ln and "TIME" in ln]
+    out = []
+    for j, h in enumerate(heads):
+        end = heads[j + 1] if j + 1 < len(heads) else len(lines)
+        out.append((int(lines[h].split()[0]), np.array(" ".join(lines[h + 1:end]).split(), dtype=np.float64)))
+    return out
+
+
+def test_weil_coupled_run_equals_reference_program():
     """weil_exemple starts saturated to the surface (ICVHE), so ATMONE makes
-    every surface node IFATM 2 under PONDH_MIN = 0 and the first step's
-    SWITCH ponds them: the reference program routes from step 2 on.  Its
-    step 1 (eight back-steps, DELTAT 5 s down to 0.01953125 s) is the coupled
-    FLOW3D's alone, and the golden equals the stored mbeconv's first row;
-    the run stops there with rc 3."""
+    every surface node IFATM 2 under PONDH_MIN = 0, the first step's SWITCH
+    ponds them and every later step starts with the routing block: the
+    overland flow routed through the project's drainage network
+    (SURF_FLOWTRA) and the ponded heads set by PONDUPD.  The reference's
+    routines run that loop here on the project as DATIN reads it
+    (weil_ref.npz: 235 steps to TMAX = 7200 s, 234 of them routed), and it
+    equals the reference program's stored mbeconv at every step and its
+    psi / sw at step 179, to the digits printed."""
     gold = np.load(os.path.join(GOLD, "weil_ref.npz"))
     col = {nm: j for j, nm in enumerate(str(v) for v in gold["stat_names"])}
-    assert len(gold["stats"]) == 1 and int(gold["rc_last"]) == 3
-    st = gold["stats"][0]
-    assert st[col["PONDING"]] == 1 and st[col["BACK"]] == 8
-    text = str(np.load(os.path.join(GOLD, "weil_outputs.npz"))["mbeconv"])
-    row = [ln for ln in text.splitlines() if not ln.startswith("#")][0].split()
-    for c, key in MBE_COLS.items():
-        if key == "ITER":
-            assert int(st[col[key]]) == int(row[c])
-        else:
-            assert _printed_close(st[col[key]], row[c]), (key, st[col[key]], row[c])
-    assert np.any(gold["ifatm"][0] == 2)   # the ponded nodes
+    stats = gold["stats"]
+    assert int(gold["rc_last"]) == 1 and int(gold["routed"]) == len(stats) - 1
+    out = np.load(os.path.join(GOLD, "weil_outputs.npz"))
+    rows = [ln.split() for ln in str(out["mbeconv"]).splitlines() if not ln.startswith("#") and ln.strip()]
+    assert len(rows) == len(stats) == 235
+    for k, row in enumerate(rows):
+        for c, key in MBE_COLS.items():
+            v = stats[k, col[key]]
+            if key == "ITER":
+                assert int(v) == int(row[c]), (k + 1, key)
+            else:
+                assert _printed_close(v, row[c]), (k + 1, key, v, row[c])
+    assert np.all(stats[:, col["PONDING"]] == 1)
+    steps = list(gold["steps"])
+    for kind in ("psi", "sw"):
+        blocks = dict(_blocks(str(out[kind])))
+        b = blocks[179]
+        a = gold[kind][steps.index(179)]
+        assert np.all(np.abs(a - b) <= 5.0000001e-7 * np.abs(b) + 1e-300), kind
+    assert np.any(gold["pondnod"][-1] > 0.0)
 
 
 def test_bc_noflow_golden_rederived():
