@@ -348,6 +348,40 @@ int cathy_gpu_flow3d_get_seepage(int64_t *handle, int *sfex, double *sfq);
  * starts over at the next init. */
 int cathy_gpu_flow3d_set_surface(int64_t *handle, const int *surf);
 
+/* The routing block of a coupled run (SURF = 1 with ponding): links a
+ * routing handle (cathy_gpu_route_create: the project's drainage network,
+ * no reservoirs, ALTEZZE cell types 0) to the flow handle.  Each attempt
+ * that starts with PONDING set then runs, on the device, what
+ * cathy_main.f:2993-3058 runs before FLOW3D:
+ *   CALL SURF_FLOWTRA(NCELL,NNOD,NROW,NCOL,NTRI,DOSTEP,NUMRES,...,
+ *                     TIME,DELTAT,ARENOD,OVFLNOD,PONDNOD,...)
+ *     (surf_flowtra.f:9-16: NOD_CELL, TRANSFER_F3D_SURF, ROUTE + ALTEZZE
+ *      substeps of CU <= 1, TRANSFER_SURF_F3D, CELL_NOD)
+ *   CALL PONDUPD(NNOD,PONDING,IFATM,PONDH_MIN,DELTAT,PONDNOD,ARENOD,
+ *                ATMPOT,ATMACT,PNEW)                    (pondupd.f:5)
+ * with the routing state saved for back-steps and advanced at each
+ * accepted step (cathy_main.f:3145-3157, 3770-3782; bkstep.f:148-157), and
+ * SWITCH takes the ponded heads into account; advance no longer stops at
+ * a ponded step.  DATIN's arrays (datin.f:6-20), 1-based: CELL(5,
+ * NCELL_COARSE), INDCEL and INDCELWL (INDEX / INDEX_WITH_LAKES) and
+ * DEM_MAP with element (I, J) at (I-1) + (J-1)*NROW (I the DEM row from
+ * the north), TRIANG(4, NTRI).  DOSTEP = 1, no lakes.  The run starts
+ * over at the next init. */
+int cathy_gpu_flow3d_set_routing(int64_t *handle, const int64_t *route,
+                                 const int *nrow, const int *ncol,
+                                 const double *delta_x, const double *delta_y,
+                                 const int *ncell_coarse, const int *cell,
+                                 const int *indcel, const int *indcelwl,
+                                 const double *dem_map, const int *ntri,
+                                 const int *triang);
+
+/* The routing block's state: PONDNOD(NNOD) and OVFLNOD(NNOD) by surface
+ * node, H_WATER_KKP1_SN(NROW*NCOL) by I_BASIN - 1 (null outputs skipped);
+ * info[2]: routing blocks run, PONDING at the last step's end. */
+int cathy_gpu_flow3d_routing_state(int64_t *handle, double *pondnod,
+                                   double *ovflnod, double *h_water,
+                                   int *info);
+
 int cathy_gpu_flow3d_set_vegetation(int64_t *handle, const int *nveg,
                                     const int *veg_type, const double *pcana,
                                     const double *pcref, const double *pcwlt,

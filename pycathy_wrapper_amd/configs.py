@@ -87,6 +87,8 @@ class Problem:
     # at step starts) with PONDH_MIN from the parm file; the run stops when a
     # step ends with a ponded surface node (the routing is not run)
     surf: bool = False
+    # the drainage network of a coupled run (project.read_routing), or None
+    routing: dict | None = None
 
 
 def sig4(a: np.ndarray) -> np.ndarray:

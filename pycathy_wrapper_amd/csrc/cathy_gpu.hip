@@ -471,6 +471,39 @@ int cathy_gpu_flow3d_set_surface(int64_t *handle, const int *surf) {
   });
 }
 
+int cathy_gpu_flow3d_set_routing(int64_t *handle, const int64_t *route, const int *nrow, const int *ncol,
+                                 const double *delta_x, const double *delta_y, const int *ncell_coarse,
+                                 const int *cell, const int *indcel, const int *indcelwl, const double *dem_map,
+                                 const int *ntri, const int *triang) {
+  return guarded([&] {
+    const std::shared_ptr<Flow> f = lookup(handle);
+    CG_REQUIRE(route && nrow && ncol && delta_x && delta_y && ncell_coarse && ntri, "null argument");
+    const RouteLink link = route_link(route);
+    f->set_routing(link, *nrow, *ncol, *delta_x, *delta_y, *ncell_coarse, cell, indcel, indcelwl, dem_map, *ntri,
+                   triang);
+    f->initialised = 0;
+    return 0;
+  });
+}
+
+int cathy_gpu_flow3d_routing_state(int64_t *handle, double *pondnod, double *ovflnod, double *h_water, int *info) {
+  return guarded([&] {
+    const std::shared_ptr<Flow> f = lookup(handle);
+    CG_REQUIRE(f->rt, "no routing linked (cathy_gpu_flow3d_set_routing)");
+    Flow::Routing &R = *f->rt;
+    if (pondnod) R.pondnod.download(pondnod, f->nnod, f->st);
+    if (ovflnod) R.ovflnod.download(ovflnod, f->nnod, f->st);
+    if (h_water) CG_HIP(hipMemcpyAsync(h_water, R.state.p + 9 * (size_t)R.ngrid, sizeof(double) * R.ngrid,
+                                       hipMemcpyDeviceToHost, f->st));
+    CG_HIP(hipStreamSynchronize(f->st));
+    if (info) {
+      info[0] = R.routed;
+      info[1] = R.pondp;
+    }
+    return 0;
+  });
+}
+
 int cathy_gpu_flow3d_bench_spmv(int64_t *handle, const int *nrep, const double *flush_mb, double *out2) {
   return guarded([&] {
     const std::shared_ptr<Flow> f = lookup(handle);

@@ -22,6 +22,7 @@
 #include "cathy_gpu.h"
 #include "flow_kernels.h"
 #include "guess.h"
+#include "route_link.h"
 
 namespace cg {
 
@@ -181,6 +182,30 @@ struct Flow {
   // starts and back-steps; pond is SWITCH's PONDING flag on the device
   bool surf = false;
   DBuf<int> pond;
+  // The routing block of a coupled run (cathy_main.f:2993-3058): the
+  // overland flux OVFLNOD to cells (NOD_CELL), to routing order
+  // (TRANSFER_F3D_SURF), ROUTE + ALTEZZE substeps on the linked routing
+  // handle, the depths back to cells (TRANSFER_SURF_F3D) and nodes
+  // (CELL_NOD) as PONDNOD, then PONDUPD; with the back-step and step-end
+  // copies of the routing state (cathy_main.f:3145-3157, 3770-3782,
+  // bkstep.f:148-157).  DOSTEP = 1, no lakes or reservoirs.
+  struct Routing {
+    RouteLink link;
+    int nrow = 0, ncol = 0, ngrid = 0, ncoarse = 0, npair = 0;
+    double dx = 0.0, dy = 0.0;
+    DBuf<int> corner, target, pair_dst, pair_src, csrc, nptr, nlist, err;
+    DBuf<double> ovflnod, ovflp, pondnod, ovflcel, pondcel, coarse;
+    // [10][ngrid]: SURFACE_WATER_SN, Q_IN_KK_SN, Q_OUT_KK_SN_1/2, VOLUME_KK_SN,
+    // Q_IN_KKP1_SN, Q_OUT_KKP1_SN_1/2, VOLUME_KKP1_SN, H_WATER_KKP1_SN
+    DBuf<double> state;
+    DBuf<double> sav, pst;  // [4][ngrid]: the _SAV and _P copies of the KK arrays
+    DBuf<double> cu_ak;     // CU_MAX, AK_MAX
+    double ak_max = 0.0, ak_max_sav = 0.0, ak_max_p = 0.0;
+    bool ak_fresh = false;  // AK_MAX on the device is newer (routed this step)
+    int pond_host = 0, pondp = 0;
+    int routed = 0;         // routing blocks run (all attempts)
+  };
+  std::unique_ptr<Routing> rt;
   DBuf<int> vegc, permd;
   DBuf<double> zc, vegtab, qtr;
   // seepage faces (sfbc): nodes face by face, top to bottom; SFEX / SFEXIT
@@ -1772,6 +1797,160 @@ struct Flow {
                          sfex.p, pnew.p);
   }
 
+  // ---- routing block ----
+  // CELL(5, NCOARSE), INDCEL / INDCELWL / DEM_MAP (I, J) at (I-1) + (J-1)*NROW
+  // (I the DEM row from the north), TRIANG(4, NTRI): 1-based as DATIN and
+  // TRIANGOLI give them.
+  void set_routing(const RouteLink &link, int nrow_, int ncol_, double dx_, double dy_, int ncoarse_,
+                   const int *cell, const int *indcel, const int *indcelwl, const double *dem_map, int ntri_,
+                   const int *triang) {
+    CG_REQUIRE(link.numres == 0, "the routing network has reservoirs; the coupled step routes none");
+    CG_REQUIRE(!link.typed, "the routing network has ALTEZZE cell types; the time loop passes none");
+    CG_REQUIRE(nrow_ >= 1 && ncol_ >= 1 && link.nrow == nrow_ && link.ncol == ncol_ && link.ngrid == nrow_ * ncol_,
+               "the routing network's grid is not NROW x NCOL");
+    CG_REQUIRE(ntri_ == ntri, "NTRI differs from the mesh's");
+    CG_REQUIRE(cell && indcel && indcelwl && dem_map && triang && ncoarse_ >= 1, "null argument");
+    auto R = std::make_unique<Routing>();
+    R->link = link;
+    R->nrow = nrow_; R->ncol = ncol_; R->ngrid = nrow_ * ncol_;
+    R->dx = dx_; R->dy = dy_;
+    const int nr = nrow_, nc = ncol_, ng = R->ngrid;
+    auto at = [&](const int *a, int i, int j) { return a[(size_t)(i - 1) + (size_t)(j - 1) * nr]; };
+    // NOD_CELL (nod_cell.f:44-71) and CELL_NOD's coarse cells (cell_nod.f:22-48): the DEM's active
+    // cells row by row, in TRIANGOLI's order
+    std::vector<int> corner, target, csrc;
+    for (int i = 1; i <= nr; ++i)
+      for (int j = 1; j <= nc; ++j) {
+        if (dem_map[(size_t)(i - 1) + (size_t)(j - 1) * nr] == 0.0) continue;
+        const int c = (int)target.size();
+        CG_REQUIRE(c < ncoarse_, "more active DEM cells than NCELL_COARSE");
+        for (int k = 0; k < 4; ++k) {
+          const int v = cell[5 * (size_t)c + k];
+          CG_REQUIRE(v >= 1 && v <= nnod, "CELL holds a node outside 1..NNOD");
+          corner.push_back(v - 1);
+        }
+        const int t = at(indcelwl, i, j);
+        CG_REQUIRE(t >= 1 && t <= ng, "an active DEM cell has no INDEX_WITH_LAKES number");
+        target.push_back(t - 1);
+        csrc.push_back(t - 1);
+      }
+    CG_REQUIRE((int)target.size() == ncoarse_, "NCELL_COARSE differs from the DEM's active cells");
+    R->ncoarse = ncoarse_;
+    // TRANSFER_F3D_SURF / TRANSFER_SURF_F3D (transfer_f3d_surf.f:42-63): cell I_BASIN_SUB of the
+    // flow grid and I_BASIN_SURF of the routing order, where both numberings hold the cell
+    std::vector<int> pdst, psrc;
+    for (int icol = 1; icol <= nc; ++icol)
+      for (int irow = 1; irow <= nr; ++irow) {
+        const int sub = nc * (nr - irow) + icol, surfi = (icol - 1) * nr + irow;
+        const int i = nr - irow + 1, j = icol;
+        if (at(indcel, i, j) != 0 && at(indcelwl, i, j) != 0) {
+          pdst.push_back(surfi - 1);
+          psrc.push_back(sub - 1);
+        }
+      }
+    R->npair = (int)pdst.size();
+    // CELL_NOD (cell_nod.f:50-68): each node's triangles' coarse cells, in triangle order
+    std::vector<int> cnt(nnod + 1, 0);
+    for (int k = 1; k <= ntri_; ++k)
+      for (int ii = 0; ii < 3; ++ii) {
+        const int v = triang[4 * (size_t)(k - 1) + ii];
+        CG_REQUIRE(v >= 1 && v <= nnod, "TRIANG holds a node outside 1..NNOD");
+        CG_REQUIRE(k / 2 + k % 2 <= ncoarse_, "a triangle beyond the coarse cells");
+        cnt[v] += 1;
+      }
+    std::vector<int> ptr(nnod + 1, 0), list((size_t)3 * ntri_);
+    for (int s = 0; s < nnod; ++s) {
+      CG_REQUIRE(cnt[s + 1] > 0, "a surface node in no triangle");
+      ptr[s + 1] = ptr[s] + cnt[s + 1];
+    }
+    std::vector<int> fillp(ptr.begin(), ptr.end() - 1);
+    for (int k = 1; k <= ntri_; ++k)
+      for (int ii = 0; ii < 3; ++ii) {
+        const int v = triang[4 * (size_t)(k - 1) + ii] - 1;
+        list[fillp[v]++] = k / 2 + k % 2 - 1;
+      }
+    R->corner.upload(corner, st); R->target.upload(target, st); R->csrc.upload(csrc, st);
+    R->pair_dst.upload(pdst, st); R->pair_src.upload(psrc, st);
+    R->nptr.upload(ptr, st); R->nlist.upload(list, st);
+    for (DBuf<double> *b : {&R->ovflnod, &R->ovflp, &R->pondnod}) b->alloc(nnod);
+    R->ovflcel.alloc(ng); R->pondcel.alloc(ng); R->coarse.alloc(ncoarse_);
+    R->state.alloc((size_t)10 * ng); R->sav.alloc((size_t)4 * ng); R->pst.alloc((size_t)4 * ng);
+    R->cu_ak.alloc(2); R->err.alloc(1);
+    CG_HIP(hipStreamSynchronize(st));
+    rt = std::move(R);
+    routing_reset();
+  }
+  // INIT_SR (init_sr.f:13-38) and the time loop's own arrays: all clear
+  void routing_reset() {
+    Routing &R = *rt;
+    auto zero = [&](DBuf<double> &b) {
+      hipLaunchKernelGGL(k_fill, dim3(grid_for((int)b.n)), dim3(kBlock), 0, st, b.n, 0.0, b.p);
+    };
+    for (DBuf<double> *b : {&R.ovflnod, &R.ovflp, &R.pondnod, &R.ovflcel, &R.pondcel, &R.coarse, &R.state, &R.sav,
+                            &R.pst, &R.cu_ak})
+      zero(*b);
+    hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, R.err.p, 0);
+    R.ak_max = R.ak_max_sav = R.ak_max_p = 0.0;
+    R.ak_fresh = false;
+    R.pond_host = R.pondp = 0;
+    R.routed = 0;
+  }
+  // cathy_main.f:2993-3058 (TRANSP false): SURF_FLOWTRA (surf_flowtra.f:48-190) then PONDUPD
+  void route_block() {
+    Routing &R = *rt;
+    const size_t g = R.ngrid;
+    R.ak_max_sav = R.ak_max;
+    CG_HIP(hipMemcpyAsync(R.sav.p, R.state.p + g, sizeof(double) * 4 * g, hipMemcpyDeviceToDevice, st));
+    const int gs = grid_for(nnod);
+    hipLaunchKernelGGL(k_nod_cell_div, dim3(gs), dim3(kBlock), 0, st, nnod, arenod.p, R.ovflnod.p);
+    hipLaunchKernelGGL(k_nod_cell, dim3(grid_for(R.ncoarse)), dim3(kBlock), 0, st, R.ncoarse, R.corner.p,
+                       R.target.p, R.dx, R.dy, (const double *)R.ovflnod.p, R.ovflcel.p);
+    if (R.npair)
+      hipLaunchKernelGGL(k_gather_pairs, dim3(grid_for(R.npair)), dim3(kBlock), 0, st, R.npair, R.pair_dst.p,
+                         R.pair_src.p, (const double *)R.ovflcel.p, R.state.p);
+    // surf_flowtra.f:82-92: substeps of at most unit Courant number
+    const double cu_max = R.ak_max * deltat, cutrgt = 1.0;
+    int nsurf = 1;
+    double dtsurf = deltat;
+    if (cu_max > cutrgt) {
+      dtsurf = cutrgt / R.ak_max;
+      nsurf = (int)(deltat / dtsurf) + 1;
+      dtsurf = deltat / nsurf;
+    }
+    hipLaunchKernelGGL(k_put_double, dim3(1), dim3(64), 0, st, R.cu_ak.p + 1, R.ak_max);
+    for (int i = 1; i <= nsurf; ++i) {
+      route_substep(R.link, st, dtsurf, R.dx, R.dy, R.state.p, R.cu_ak.p, R.err.p);
+      if (nsurf > 1 && i < nsurf) route_advance_state();
+    }
+    R.ak_fresh = true;
+    if (R.npair)
+      hipLaunchKernelGGL(k_gather_pairs, dim3(grid_for(R.npair)), dim3(kBlock), 0, st, R.npair, R.pair_src.p,
+                         R.pair_dst.p, (const double *)(R.state.p + 9 * g), R.pondcel.p);
+    hipLaunchKernelGGL(k_cell_coarse, dim3(grid_for(R.ncoarse)), dim3(kBlock), 0, st, R.ncoarse, R.csrc.p,
+                       (const double *)R.pondcel.p, R.coarse.p);
+    hipLaunchKernelGGL(k_cell_nod, dim3(gs), dim3(kBlock), 0, st, nnod, R.nptr.p, R.nlist.p,
+                       (const double *)R.coarse.p, R.pondnod.p);
+    hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, 0);
+    hipLaunchKernelGGL(k_pondupd, dim3(gs), dim3(kBlock), 0, st, nnod, surf_row.p, ifatm.p, prm.pondh_min,
+                       1.0 / deltat, (const double *)R.pondnod.p, arenod.p, atmpot.p, atmact.p, pnew.p, pond.p);
+    R.routed += 1;
+  }
+  // KK = KKP1, KKP1 = 0 (surf_flowtra.f:150-158, cathy_main.f:3773-3780)
+  void route_advance_state() {
+    Routing &R = *rt;
+    const size_t g = R.ngrid;
+    CG_HIP(hipMemcpyAsync(R.state.p + g, R.state.p + 5 * g, sizeof(double) * 4 * g, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((int)(4 * g))), dim3(kBlock), 0, st, 4 * g, 0.0, R.state.p + 5 * g);
+  }
+  // cathy_main.f:3145-3157: the attempt's saved state is the one BKSTEP restores
+  void route_backstep() {
+    Routing &R = *rt;
+    const size_t g = R.ngrid;
+    R.ak_max_p = R.ak_max_sav;
+    CG_HIP(hipMemcpyAsync(R.pst.p, R.sav.p, sizeof(double) * 4 * g, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((int)(4 * g))), dim3(kBlock), 0, st, 4 * g, 0.0, R.state.p + 5 * g);
+  }
+
   void switch_old() {
     hipLaunchKernelGGL(k_switch_old, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, surf_row.p, ifatm.p,
                        atmact.p, atmpot.p, pnew.p, prm.pmin);
@@ -1783,7 +1962,9 @@ struct Flow {
     hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, 0);
     hipLaunchKernelGGL(k_switch_surf, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, surf_row.p, ifatm.p,
                        atmact.p, atmpot.p, veg_on ? (const double *)qtr.p : (const double *)nullptr, arenod.p,
-                       deltat, prm.pondh_min, prm.pmin, pnew.p, pond.p);
+                       deltat, prm.pondh_min, prm.pmin, pnew.p, pond.p,
+                       rt ? (const double *)rt->pondnod.p : (const double *)nullptr,
+                       rt ? rt->ovflnod.p : (double *)nullptr);
   }
   // cathy_main.f:2942-2946 and bkstep.f:135-139: SWITCH_OLD, or ADRSTN
   void switch_start() {
@@ -1936,8 +2117,9 @@ struct Flow {
     std::vector<double> h(n);
     for (int i = 0; i < n; ++i) h[perm[i]] = ptimep_ref[i];
     ptimep.upload(h, st);
-    // inital.f:201-205: PONDING starts clear (IPOND = 0)
+    // inital.f:201-205: PONDING starts clear (IPOND = 0); INIT_SR (init_sr.f)
     hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, 0);
+    if (rt) routing_reset();
     // INIT1 (init1.f:37-60)
     deltat = prm.deltat;
     if (deltat > prm.dtmax) deltat = prm.dtmax;
@@ -2078,8 +2260,18 @@ struct Flow {
       atmbak();
     }
     switch_start();
-    if (surf)  // bkstep.f:148-149: PONDING = PONDP, which is clear (a ponded step ends the run)
+    if (surf && rt) {  // bkstep.f:148-157: PONDING = PONDP, OVFLNOD = OVFLP, the routing state of _P
+      Routing &R = *rt;
+      R.pond_host = R.pondp;
+      hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, R.pondp);
+      CG_HIP(hipMemcpyAsync(R.ovflnod.p, R.ovflp.p, sizeof(double) * nnod, hipMemcpyDeviceToDevice, st));
+      R.ak_max = R.ak_max_p;
+      R.ak_fresh = false;
+      CG_HIP(hipMemcpyAsync(R.state.p + R.ngrid, R.pst.p, sizeof(double) * 4 * R.ngrid, hipMemcpyDeviceToDevice,
+                            st));
+    } else if (surf) {  // PONDING = PONDP, which is clear (a ponded step ends a run without routing)
       hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, 0);
+    }
     guess_point();
     hipLaunchKernelGGL(k_start_iter, dim3(grid_for(n)), dim3(kBlock), 0, st, n, prm.tetaf, pnew.p, ptimep.p,
                        pold.p, ptnew.p, ptold.p);
@@ -2091,8 +2283,14 @@ struct Flow {
     if (finished) return finished;
     step_begin();
     while (true) {
+      if (surf && rt) {  // label 100 of cathy_main.f: route a ponded surface, then POLD = PNEW on it
+        if (rt->pond_host) route_block();
+        hipLaunchKernelGGL(k_copy_surf, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, surf_row.p, pnew.p,
+                           pold.p);
+      }
       flow3d();
       if (!bckstp) break;
+      if (surf && rt) route_backstep();
       back_step();
     }
     return step_end(stv);
@@ -2179,8 +2377,24 @@ struct Flow {
     hipLaunchKernelGGL(k_sw_kr, dim3(grid_for(n)), dim3(kBlock), 0, st, n, ptimep.p, soil(), swtimep.p,
                        (double *)nullptr);
     int ponding = 0;
+    double ak_new = 0.0;
     if (surf) pond.download(&ponding, 1, st);
+    if (surf && rt) {  // cathy_main.f:3223-3225 and 3770-3782
+      Routing &R = *rt;
+      hipLaunchKernelGGL(k_pond_reset, dim3(grid_for(nnod)), dim3(kBlock), 0, st, nnod, surf_row.p, pnew.p,
+                         R.pondnod.p);
+      CG_HIP(hipMemcpyAsync(R.ovflp.p, R.ovflnod.p, sizeof(double) * nnod, hipMemcpyDeviceToDevice, st));
+      route_advance_state();
+      if (R.ak_fresh)
+        CG_HIP(hipMemcpyAsync(&ak_new, R.cu_ak.p + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    }
     CG_HIP(hipStreamSynchronize(st));
+    if (surf && rt) {
+      Routing &R = *rt;
+      R.pondp = ponding;
+      R.pond_host = ponding;
+      if (R.ak_fresh) { R.ak_max = ak_new; R.ak_fresh = false; }
+    }
     last_counters[0] = c_picard - pic0;
     last_counters[1] = c_pcg - pcg0;
     last_counters[4] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2192,8 +2406,9 @@ struct Flow {
     }
     if (noback) return finished = 2;
     if (std::fabs(time - prm.tmax) <= 0.001 * deltat) return finished = 1;
-    // a ponded surface: the next step would route first (cathy_main.f:2993)
-    if (ponding) return finished = 3;
+    // a ponded surface without the routing linked: the next step would
+    // route first (cathy_main.f:2993)
+    if (ponding && !rt) return finished = 3;
     dtgmin = timupd();
     nstep += 1;
     iter = 1;

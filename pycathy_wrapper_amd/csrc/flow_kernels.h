@@ -298,7 +298,19 @@ __global__ void k_flux(int nnod, const int *ifatm, const double *atmact, const d
 __global__ void k_norms_final(int nb, int nbf, NormPart part, const double *part6, double *res);
 __global__ void k_switch_surf(int nnod, const int *surf_row, int *ifatm, double *atmact, const double *atmpot,
                               const double *qtr, const double *arenod, double deltat, double pondh_min, double pmin,
-                              double *pnew, int *ponding);
+                              double *pnew, int *ponding, const double *pondnod, double *ovflnod);
+__global__ void k_nod_cell_div(int nnod, const double *arenod, double *ovflnod);
+__global__ void k_nod_cell(int ncell, const int *corner, const int *target, double dx, double dy,
+                           const double *nodval, double *cellval);
+__global__ void k_gather_pairs(int m, const int *dst, const int *src, const double *from, double *to);
+__global__ void k_cell_coarse(int ncell, const int *src, const double *cellval, double *coarse);
+__global__ void k_cell_nod(int nnod, const int *ptr, const int *list, const double *coarse, double *pondnod);
+__global__ void k_pondupd(int nnod, const int *surf_row, const int *ifatm, double pondh_min, double dtr,
+                          const double *pondnod, const double *arenod, const double *atmpot, double *atmact,
+                          double *pnew, int *ponding);
+__global__ void k_pond_reset(int nnod, const int *surf_row, const double *pnew, double *pondnod);
+__global__ void k_copy_surf(int nnod, const int *surf_row, const double *pnew, double *pold);
+__global__ void k_put_double(double *x, double v);
 __global__ void k_adrstn(int nnod, const int *surf_row, int *ifatm, const double *atmpot, double *atmact,
                          double *pnew, double pmin);
 __global__ void k_put_int(int *x, int v);

@@ -1484,26 +1484,36 @@ __global__ void k_switch_old(int nnod, const int *__restrict__ surf_row, int *__
 }
 
 // switch.f:27-423: the switching CONVER calls when the surface is coupled
-// (SURF).  The routing does not run on the device, so no surface node holds
-// ponded water (PONDNOD = 0) and the head a node would pond to is
-// PL = ATMDIF*DELTAT/ARENOD with ATMDIF = ATMPOT - (ATMACT - QTRANIE); the
-// IFATM 0 cases test PNEW itself.  Bands, in SWITCH's order: PL >= PONDH_MIN
-// (the node ponds: IFATM 2, PNEW = PL, *ponding = 1), 0 <= PL < PONDH_MIN
-// (IFATM 1), PMIN < PL < 0 and PL <= PMIN (IFATM 0, ATMACT = ATMPOT);
-// rainfall with exfiltration ends every PL < 0 at IFATM 1; a ponded node
-// under evaporation zeroes PNEW in the middle two bands; an IFATM 1 node at
-// or below PMIN under evaporation stays unless it exfiltrates faster than
-// the potential rate.  OVFLNOD, which only the routing reads, is not kept.
+// (SURF).  PL = PONDNOD + ATMDIF*DELTAT/ARENOD is the head a node would pond
+// to, with ATMDIF = ATMPOT - (ATMACT - QTRANIE); the IFATM 0 cases test
+// PNEW itself.  Bands, in SWITCH's order: PL >= PONDH_MIN (the node ponds:
+// IFATM 2, PNEW = PL, *ponding = 1), 0 <= PL < PONDH_MIN (IFATM 1),
+// PMIN < PL < 0 and PL <= PMIN (IFATM 0, ATMACT = ATMPOT); rainfall with
+// exfiltration ends every PL < 0 at IFATM 1; a ponded node under
+// evaporation zeroes PNEW in the middle two bands; an IFATM 1 node at or
+// below PMIN under evaporation stays unless it exfiltrates faster than the
+// potential rate.  OVFLNOD (the overland flux the routing takes) is
+// written where SWITCH writes it.  pondnod / ovflnod null: no routing (no
+// ponded water, OVFLNOD not kept).
 __global__ void k_switch_surf(int nnod, const int *__restrict__ surf_row, int *__restrict__ ifatm,
                               double *__restrict__ atmact, const double *__restrict__ atmpot,
                               const double *__restrict__ qtr, const double *__restrict__ arenod, double deltat,
-                              double pondh_min, double pmin, double *__restrict__ pnew, int *__restrict__ ponding) {
+                              double pondh_min, double pmin, double *__restrict__ pnew, int *__restrict__ ponding,
+                              const double *__restrict__ pondnod, double *__restrict__ ovflnod) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnod) return;
   const int fa = ifatm[i];
-  if (fa == -1) return;
+  if (fa == -1) {
+    if (ovflnod) ovflnod[i] = 0.0;
+    return;
+  }
   const int r = surf_row[i];
-  const double ap = atmpot[i], aa = atmact[i], pn = pnew[r];
+  const double ap = atmpot[i], aa = atmact[i], pn = pnew[r], ar = arenod[i];
+  const double pond = pondnod ? pondnod[i] : 0.0;
+  double dif = ap - (aa - (qtr ? qtr[r] : 0.0));
+  if (fabs(dif) < 1.0e-14) dif = 0.0;
+  const double pl = pond + (dif * deltat / ar);
+  const double drain = -pond * ar / deltat;
   auto band = [&](double h) {
     if (h >= pondh_min) return 1;
     if (h >= 0.0 && h < pondh_min) return 2;
@@ -1511,39 +1521,45 @@ __global__ void k_switch_surf(int nnod, const int *__restrict__ surf_row, int *_
     if (h <= pmin) return 4;
     return 0;
   };
+  double ov = 0.0;
+  bool set_ov = true;
   if (fa == 0) {
+    const double over = (pn - pond) * ar / deltat;
     switch (band(pn)) {
-      case 1: ifatm[i] = 2; *ponding = 1; break;
-      case 2: ifatm[i] = 1; break;
-      case 4: ifatm[i] = 1; pnew[r] = pmin; break;
-      default: break;
+      case 1: ifatm[i] = 2; *ponding = 1; ov = over; break;
+      case 2: ifatm[i] = 1; ov = over; break;
+      case 3: ov = drain; break;
+      case 4: ifatm[i] = 1; pnew[r] = pmin; ov = drain; break;
+      default: set_ov = false; break;
     }
-    return;
-  }
-  if (fa != 1 && fa != 2) return;
-  const bool rain = ap >= 0.0, infil = aa >= 0.0;
-  if (fa == 1 && !rain && pn <= pmin) {  // cases G and H
-    if (!infil && aa < ap) {
-      ifatm[i] = 0;
-      atmact[i] = ap;
-      pnew[r] = pmin;
+  } else if (fa == 1 || fa == 2) {
+    const bool rain = ap >= 0.0, infil = aa >= 0.0;
+    if (fa == 1 && !rain && pn <= pmin) {  // cases G and H
+      set_ov = false;
+      if (!infil && aa < ap) {
+        ifatm[i] = 0;
+        atmact[i] = ap;
+        pnew[r] = pmin;
+        ov = 0.0;
+        set_ov = true;
+      }
+    } else {
+      const bool zero_p = fa == 2 && !rain;
+      int b = band(pl);
+      if (rain && !infil && b >= 3) b = 5;  // cases B and F
+      switch (b) {
+        case 1: ifatm[i] = 2; pnew[r] = pl; *ponding = 1; ov = dif; break;
+        case 2: ifatm[i] = 1; if (zero_p) pnew[r] = 0.0; ov = dif; break;
+        case 3: ifatm[i] = 0; atmact[i] = ap; if (zero_p) pnew[r] = 0.0; ov = drain; break;
+        case 4: ifatm[i] = 0; atmact[i] = ap; ov = drain; break;
+        case 5: ifatm[i] = 1; ov = dif; break;
+        default: set_ov = false; break;
+      }
     }
-    return;
+  } else {
+    set_ov = false;
   }
-  double dif = ap - (aa - (qtr ? qtr[r] : 0.0));
-  if (fabs(dif) < 1.0e-14) dif = 0.0;
-  const double pl = 0.0 + (dif * deltat / arenod[i]);
-  const bool zero_p = fa == 2 && !rain;
-  int b = band(pl);
-  if (rain && !infil && b >= 3) b = 5;  // cases B and F
-  switch (b) {
-    case 1: ifatm[i] = 2; pnew[r] = pl; *ponding = 1; break;
-    case 2: ifatm[i] = 1; if (zero_p) pnew[r] = 0.0; break;
-    case 3: ifatm[i] = 0; atmact[i] = ap; if (zero_p) pnew[r] = 0.0; break;
-    case 4: ifatm[i] = 0; atmact[i] = ap; break;
-    case 5: ifatm[i] = 1; break;
-    default: break;
-  }
+  if (ovflnod && set_ov) ovflnod[i] = ov;
 }
 
 // adrstn.f:22-31: an 'air dry' surface node under rainfall, or whose actual
@@ -1560,6 +1576,86 @@ __global__ void k_adrstn(int nnod, const int *__restrict__ surf_row, int *__rest
     atmact[i] = atmpot[i];
     pnew[r] = pmin;
   }
+}
+
+// ---- the routing block of a coupled step (cathy_main.f:2993-3058) ----
+// nod_cell.f:24-27: OVFLNOD(K) = OVFLNOD(K)/ARENOD(K), in place.
+__global__ void k_nod_cell_div(int nnod, const double *__restrict__ arenod, double *__restrict__ ovflnod) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nnod) ovflnod[s] = ovflnod[s] / arenod[s];
+}
+// nod_cell.f:30-71 with DOSTEP = 1: each active DEM cell (in TRIANGOLI's
+// order) takes the mean of its four corner nodes' values times its area,
+// stored at its INDEX_WITH_LAKES position.
+__global__ void k_nod_cell(int ncell, const int *__restrict__ corner, const int *__restrict__ target, double dx,
+                           double dy, const double *__restrict__ nodval, double *__restrict__ cellval) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  double v = 0.0;
+  for (int k = 0; k < 4; ++k) v = v + nodval[corner[4 * c + k]];
+  v = v * 0.25;
+  cellval[target[c]] = v * dx * dy;
+}
+// to[dst[i]] = from[src[i]]: TRANSFER_F3D_SURF (cell to routing order) and
+// TRANSFER_SURF_F3D (back), for grids without lakes.
+__global__ void k_gather_pairs(int m, const int *__restrict__ dst, const int *__restrict__ src,
+                               const double *__restrict__ from, double *__restrict__ to) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) to[dst[i]] = from[src[i]];
+}
+// cell_nod.f:22-48 with DOSTEP = 1: the coarse cells' values (src -1: none).
+__global__ void k_cell_coarse(int ncell, const int *__restrict__ src, const double *__restrict__ cellval,
+                              double *__restrict__ coarse) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  double v = 0.0;
+  if (src[c] >= 0) v = v + cellval[src[c]];
+  coarse[c] = v / 1.0;
+}
+// cell_nod.f:50-68: PONDNOD of a node is the mean of its triangles' cells,
+// summed in triangle order.
+__global__ void k_cell_nod(int nnod, const int *__restrict__ ptr, const int *__restrict__ list,
+                           const double *__restrict__ coarse, double *__restrict__ pondnod) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnod) return;
+  double v = 0.0;
+  for (int j = ptr[s]; j < ptr[s + 1]; ++j) v = v + coarse[list[j]];
+  pondnod[s] = v / (double)(ptr[s + 1] - ptr[s]);
+}
+// pondupd.f:19-35: a node whose ponded head reaches PONDH_MIN keeps it as
+// its head (IFATM 1, 2) or adds it to its flux (IFATM 0); PONDING is set
+// where one does (the caller clears it first).
+__global__ void k_pondupd(int nnod, const int *__restrict__ surf_row, const int *__restrict__ ifatm,
+                          double pondh_min, double dtr, const double *__restrict__ pondnod,
+                          const double *__restrict__ arenod, const double *__restrict__ atmpot,
+                          double *__restrict__ atmact, double *__restrict__ pnew, int *__restrict__ ponding) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnod) return;
+  const int fa = ifatm[s];
+  if (fa == -1 || !(pondnod[s] >= pondh_min)) return;
+  if (fa == 1 || fa == 2) {
+    pnew[surf_row[s]] = pondnod[s];
+    *ponding = 1;
+  } else if (fa == 0) {
+    *ponding = 1;
+    atmact[s] = atmpot[s] + pondnod[s] * arenod[s] * dtr;
+  }
+}
+// cathy_main.f:3223-3225: no ponded head where the surface is unsaturated.
+__global__ void k_pond_reset(int nnod, const int *__restrict__ surf_row, const double *__restrict__ pnew,
+                             double *__restrict__ pondnod) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nnod && pnew[surf_row[s]] <= 0.0) pondnod[s] = 0.0;
+}
+// cathy_main.f:3099: POLD(1..NNOD) = PNEW(1..NNOD).
+__global__ void k_copy_surf(int nnod, const int *__restrict__ surf_row, const double *__restrict__ pnew,
+                            double *__restrict__ pold) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nnod) pold[surf_row[s]] = pnew[surf_row[s]];
+}
+// *x = v (one thread).
+__global__ void k_put_double(double *__restrict__ x, double v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *x = v;
 }
 
 // *x = v (one thread).

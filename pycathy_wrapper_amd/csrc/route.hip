@@ -28,6 +28,7 @@
 
 #include "cathy_gpu.h"
 #include "common.h"
+#include "route_link.h"
 
 namespace cg {
 namespace {
@@ -354,6 +355,33 @@ inline void basin_ij(int ib, int nrow, int &i, int &j) {
 }
 
 }  // namespace
+
+RouteLink route_link(const int64_t *handle) {
+  std::shared_ptr<Route> r = route_lookup(handle);
+  RouteLink l;
+  l.route = r.get();
+  l.nrow = r->nrow;
+  l.ncol = r->ncol;
+  l.ngrid = r->ngrid;
+  l.ncell = r->ncell;
+  l.numres = r->numres;
+  l.typed = r->typed;
+  l.keep = std::static_pointer_cast<void>(r);
+  return l;
+}
+
+void route_substep(const RouteLink &l, hipStream_t st, double deltat, double dx, double dy, double *state,
+                   double *cu_ak, int *err) {
+  Route *r = static_cast<Route *>(l.route);
+  CG_REQUIRE(r->numres == 0, "the coupled step routes networks without reservoirs");
+  std::lock_guard<std::mutex> lk(r->mu);
+  const size_t g = r->ngrid;
+  double *S = state;
+  hipLaunchKernelGGL(k_route, dim3(1), dim3(kRouteBlock), 0, st, r->dev(), deltat, dx, dy, S, S + g, S + 2 * g,
+                     S + 3 * g, S + 4 * g, S + 5 * g, S + 6 * g, S + 7 * g, S + 8 * g, S + 9 * g, cu_ak,
+                     (const double *)nullptr, (double *)nullptr, err);
+  CG_HIP(hipGetLastError());
+}
 }  // namespace cg
 
 using namespace cg;

@@ -116,6 +116,9 @@ class FlowGPU:
             self.set_seepage(problem.seepage["faces"], problem.seepage.get("isfcvg", 0))
         if getattr(problem, "surf", False):
             self.set_surface(True)
+        self.route = None
+        if getattr(problem, "routing", None) is not None:
+            self.set_routing(problem.routing)
         self.init(problem.psi0)
 
     def set_params(self, p, hspatm: int | None = None):
@@ -191,6 +194,41 @@ class FlowGPU:
         step starts; advance returns 3 after a step that ends ponded) or the
         subsurface-only switching; call init again afterwards."""
         _check(lib().cathy_gpu_flow3d_set_surface(C.byref(self.h), _i(1 if surf else 0)), "set_surface")
+
+    def set_routing(self, routing: dict) -> None:
+        """The routing block of a coupled run (project.read_routing): the
+        drainage network goes to a routing handle (routing.RouteGPU) linked
+        to this flow handle; call init again afterwards."""
+        from .routing import RouteGPU
+        m = self.problem.mesh
+        if self.route is not None:
+            self.route.close()
+        self.route = RouteGPU(routing["net"])
+        nr, nc = routing["nrow"], routing["ncol"]
+        colmaj = lambda g: np.ascontiguousarray(np.asarray(g).T.reshape(-1))   # (I, J) at (I-1) + (J-1)*NROW
+        cell = np.zeros((len(routing["cell"]), 5), dtype=np.int32)
+        cell[:, :4] = routing["cell"] + 1
+        tri = np.ascontiguousarray(m.triang, dtype=np.int32).copy()
+        tri[:, :3] += 1
+        keep = dict(cell=np.ascontiguousarray(cell.reshape(-1)), indcel=colmaj(routing["index"]).astype(np.int32),
+                    indcelwl=colmaj(routing["index_wl"]).astype(np.int32),
+                    dem=colmaj(routing["dem"]).astype(np.float64), tri=np.ascontiguousarray(tri.reshape(-1)))
+        self._keep["routing"] = keep
+        _check(lib().cathy_gpu_flow3d_set_routing(
+            C.byref(self.h), C.byref(self.route.h), _i(nr), _i(nc), C.byref(C.c_double(routing["dx"])),
+            C.byref(C.c_double(routing["dy"])), _i(len(cell)), _p(keep["cell"]), _p(keep["indcel"]),
+            _p(keep["indcelwl"]), _p(keep["dem"]), _i(m.ntri), _p(keep["tri"])), "set_routing")
+
+    def routing_state(self):
+        """(PONDNOD, OVFLNOD, H_WATER_KKP1_SN, routing blocks run)."""
+        m = self.problem.mesh
+        r = self.problem.routing
+        pond, ovfl = np.zeros(m.nnod), np.zeros(m.nnod)
+        hw = np.zeros(r["nrow"] * r["ncol"])
+        info = np.zeros(2, dtype=np.int32)
+        _check(lib().cathy_gpu_flow3d_routing_state(C.byref(self.h), _p(pond), _p(ovfl), _p(hw), _p(info)),
+               "routing_state")
+        return pond, ovfl, hw, int(info[0])
 
     def init(self, psi0):
         p = np.ascontiguousarray(psi0, dtype=np.float64)
@@ -330,6 +368,10 @@ class FlowGPU:
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             lib().cathy_gpu_flow3d_destroy(C.byref(self.h))
+            self.h = C.c_int64(0)
+        if getattr(self, "route", None) is not None:
+            self.route.close()
+            self.route = None
 
     def __del__(self):
         try:

@@ -99,6 +99,24 @@ def surface_from_dem(dem: np.ndarray, dx: float, dy: float, zone: np.ndarray | N
     return nnod, ntri, x, y, z, tri.astype(np.int32)
 
 
+def dem_cells(dem: np.ndarray) -> np.ndarray:
+    """CELL(1:4, ICELL) of triangoli.f:88-94 (DOSTEP = 1): the corner nodes
+    (0-based) of each active DEM cell, row by row, in the order NW, SW, SE,
+    NE -- the node numbering of ``surface_from_dem``."""
+    dem = np.asarray(dem, dtype=np.float64)
+    nrow, ncol = dem.shape
+    active = dem != 0.0
+    nodi = np.zeros((nrow + 1, ncol + 1), dtype=bool)
+    nodi[:-1, :-1] |= active
+    nodi[1:, :-1] |= active
+    nodi[1:, 1:] |= active
+    nodi[:-1, 1:] |= active
+    ids = np.full((nrow + 1, ncol + 1), -1, dtype=np.int64)
+    ids[nodi] = np.arange(int(nodi.sum()))
+    ci, cj = np.nonzero(active)
+    return np.stack([ids[ci, cj], ids[ci + 1, cj], ids[ci + 1, cj + 1], ids[ci, cj + 1]], axis=1).astype(np.int32)
+
+
 def _ordered_scatter_add(size: int, idx: np.ndarray, vals: np.ndarray) -> np.ndarray:
     """out[idx[k]] += vals[k] in the order k = 0, 1, ... (np.add.at is
     unbuffered and sequential, which keeps the reference's summation order)."""

@@ -26,11 +26,14 @@ paths relative to the project directory):
 A project with ISIMGR = 2 (surface routing coupled) runs with the coupled
 atmospheric switching -- SWITCH in CONVER, ADRSTN at step starts and
 back-steps (conver.f:61-71, cathy_main.f:2942-2946, bkstep.f:135-139) --
-and PONDH_MIN from the parm file, as long as no surface node ponds: the
-routing (SURF_FLOWTRA, PONDUPD) is not run, so a step that ends ponded
-stops the run (status 3), and a project that starts ponded (IPOND != 0) is
-refused.  ``subsurface_only`` runs such a project as ISIMGR = 1 instead
-(SWITCH_OLD, PONDH_MIN = 1e10, datin.f:120).
+and PONDH_MIN from the parm file.  Where the project holds its drainage
+network (prepro/qoi_a, the dtm_* rasters, lakes_map and posizione_serb:
+``read_routing``), a ponded surface is routed as the reference's time
+loop routes it (``Problem.routing``, cathy_gpu_flow3d_set_routing);
+without one, a step that ends ponded stops the run (status 3).  A project
+that starts ponded (IPOND != 0) is refused.  ``subsurface_only`` runs
+such a project as ISIMGR = 1 instead (SWITCH_OLD, PONDH_MIN = 1e10,
+datin.f:120).
 """
 from __future__ import annotations
 
@@ -237,6 +240,61 @@ def read_dem_parameters(path: str) -> dict:
                 ivert=int(ivert), isp=int(isp), base=base, zratio=np.array(toks[10:10 + nstr]))
 
 
+ROUTING_FILES = {"qoi_a": "prepro/qoi_a", "lakes_map": "prepro/lakes_map",
+                 "posizione_serb": "input/posizione_serb",
+                 "w1": "prepro/dtm_w_1", "w2": "prepro/dtm_w_2", "pout1": "prepro/dtm_p_outflow_1",
+                 "pout2": "prepro/dtm_p_outflow_2", "slope1": "prepro/dtm_local_slope_1",
+                 "slope2": "prepro/dtm_local_slope_2", "epl1": "prepro/dtm_epl_1", "epl2": "prepro/dtm_epl_2",
+                 "kss1": "prepro/dtm_kSs1_sf_1", "kss2": "prepro/dtm_kSs1_sf_2", "ws1": "prepro/dtm_Ws1_sf_1",
+                 "ws2": "prepro/dtm_Ws1_sf_2", "b1": "prepro/dtm_b1_sf", "y1": "prepro/dtm_y1_sf",
+                 "nrc": "prepro/dtm_nrc"}
+
+
+def read_routing(root: str, dem: np.ndarray, zone: np.ndarray, dx: float, dy: float, dostep: int) -> dict:
+    """The drainage network and cell numbering DATIN reads and builds for
+    ISIMGR = 2 (datin.f:216-225, 287-370): the reservoir count of
+    input/posizione_serb (IIN17; none are supported), prepro/lakes_map
+    (RAST_INPUT_LZ; no lakes), QOI_SN from prepro/qoi_a (IIN23: a header
+    line, then NCELL values), the thirteen DTM rasters (RAST_INPUT_REAL /
+    _INT store RAST(ICOL, IROW) with IROW = 1 the raster's last row: the
+    ``routing.Network`` layout), INDEX / INDEX_WITH_LAKES (INDEX_DEM: the
+    grid position (I-1)*NCOL + J of a cell with ZONE /= 0 and no lake, else
+    0) and CELL (TRIANGOLI, ``mesh.dem_cells``)."""
+    from .routing import Network
+    f = {k: os.path.join(root, v) for k, v in ROUTING_FILES.items()}
+    for k, p in f.items():
+        if not os.path.exists(p):
+            raise FileNotFoundError(f"project {root}: {ROUTING_FILES[k]} missing (the drainage network)")
+    if dostep != 1:
+        raise ValueError(f"DOSTEP = {dostep}: the coupled step takes the full-resolution surface only")
+    nres = int(_reader(f["posizione_serb"]).read_int(1)[0])
+    if nres != 0:
+        raise ValueError(f"{nres} reservoirs in input/posizione_serb: the coupled step routes none")
+    lakes = read_raster(f["lakes_map"])[0].astype(np.int32)
+    if np.any(lakes > 0):
+        raise ValueError("prepro/lakes_map holds lakes: the coupled step routes none")
+    nrow, ncol = dem.shape
+    index_wl = np.where(zone != 0, np.arange(1, nrow * ncol + 1).reshape(nrow, ncol), 0).astype(np.int32)
+    index = np.where(lakes <= 0, index_wl, 0).astype(np.int32)
+    ncell = int((zone != 0).sum())
+    with open(f["qoi_a"]) as fh:
+        toks = fh.read().split("\n", 1)[1].split()
+    if len(toks) < ncell:
+        raise ValueError(f"prepro/qoi_a: {len(toks)} values for NCELL = {ncell}")
+    qoi = np.array(toks[:ncell], dtype=np.int64).astype(np.int32)
+    grids = {}
+    for k in ("w1", "w2", "slope1", "slope2", "epl1", "epl2", "kss1", "kss2", "ws1", "ws2", "b1", "y1", "nrc",
+              "pout1", "pout2"):
+        r, hdr = read_raster(f[k])
+        if r.shape != (nrow, ncol):
+            raise ValueError(f"{ROUTING_FILES[k]}: {r.shape} for a {nrow} x {ncol} DEM")
+        r = r[::-1].copy()
+        grids[k] = r.astype(np.int32) if k.startswith("pout") else r
+    net = Network(nrow=nrow, ncol=ncol, qoi=qoi, dx=dx, dy=dy, **grids)
+    return dict(nrow=nrow, ncol=ncol, dx=dx, dy=dy, net=net, index=index, index_wl=index_wl, lakes=lakes,
+                cell=M.dem_cells(dem), dem=dem, ncell=ncell)
+
+
 def _exists(*parts) -> str | None:
     p = os.path.join(*parts)
     return p if os.path.exists(p) else None
@@ -307,8 +365,12 @@ def load_project(root: str, subsurface_only: bool = False, files: dict | None = 
         if any(r.faces for r in recs):
             seep = dict(faces=B.seepage_faces(recs), isfcvg=parm["isfcvg"])
     veg = dict(veg_type=veg_type, **soil["veg"])
+    # the drainage network of a coupled run, where the project holds one
+    routing = None
+    if surf and os.path.exists(os.path.join(root, "prepro", "qoi_a")):
+        routing = read_routing(root, dem, zone, dp["dx"], dp["dy"], dp["dostep"])
     meta = dict(project=os.path.abspath(root), parm=parm, isimgr=parm["isimgr"], subsurface_only=subsurface_only,
                 ic=dict(indp=ic["indp"], ipond=ic["ipond"], wtposition=ic["wtposition"]), nveg=nveg)
     return CF.Problem(os.path.basename(os.path.abspath(root)), m, soil["table"], psi0, at, av, prm, 0, meta, hspatm,
                       bc_dir if has(bc_dir) else None, bc_neu if has(bc_neu) else None,
-                      veg if parm["iopt"] == 1 else None, seep, surf)
+                      veg if parm["iopt"] == 1 else None, seep, surf, routing)

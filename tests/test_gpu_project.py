@@ -14,9 +14,13 @@ a catchment with inactive cells) must equal
   * and so the reference program's own output files for these projects
     (output/mbeconv to the printed digits, output/psi and output/sw blocks
     to their 7 printed digits), within the same 1e-8 / 1e-6 slack.
-weil_exemple starts saturated to the surface and ponds in its first step:
-the device takes that step as the reference did (eight back-steps, the
-stored mbeconv's first row) and returns 3, where the reference would route.
+weil_exemple starts saturated to the surface and ponds from its first step
+on: with its drainage network linked (cathy_gpu_flow3d_set_routing) the
+device routes the ponded surface before each step as the reference's time
+loop does and must equal the reference's run (tests/golden/weil_ref.npz:
+the reference's SURF_FLOWTRA and PONDUPD on the project as its DATIN reads
+it) and the reference program's stored mbeconv, psi and sw; without the
+network it takes the first step and returns 3.
 """
 from __future__ import annotations
 
@@ -113,26 +117,98 @@ def test_project_run_matches_reference(name, ordering):
     say(f"{name} {ordering}: {k} steps, worst psi rel {worst:.2e}")
 
 
-def test_weil_ponds_in_its_first_step():
+def _blocks(text: str):
+    lines = text.splitlines()
+    heads = [i for i, ln in enumerate(lines) if "NSTEP" in ln and "TIME" in ln]
+    out = {}
+    for j, h in enumerate(heads):
+        end = heads[j + 1] if j + 1 < len(heads) else len(lines)
+        out[int(lines[h].split()[0])] = np.array(" ".join(lines[h + 1:end]).split(), dtype=np.float64)
+    return out
+
+
+@pytest.mark.parametrize("ordering", ["natural", "column3"])
+def test_weil_coupled_run_with_routing_matches_reference(ordering):
+    """weil_exemple as the reference program runs it: ponded from step 1,
+    the overland flow routed through its drainage network before every
+    later step (234 routing blocks), to TMAX = 7200 s."""
+    from pycathy_wrapper_amd import gpu as G
+    gold = np.load(os.path.join(GOLD, "weil_ref.npz"))
+    col = {nm: j for j, nm in enumerate(str(v) for v in gold["stat_names"])}
+    stats = gold["stats"]
+    saved = list(gold["steps"])
+    out = np.load(os.path.join(GOLD, "weil_outputs.npz"))
+    rows = [ln.split() for ln in str(out["mbeconv"]).splitlines() if not ln.startswith("#") and ln.strip()]
+    psi179, sw179 = _blocks(str(out["psi"]))[179], _blocks(str(out["sw"]))[179]
+    pr = PJ.load_project(os.path.join(GOLD, "weil_project"))
+    assert pr.surf and pr.routing is not None
+    say(f"weil, {ordering}: {len(stats)} reference steps")
+    g = G.FlowGPU(pr, G.ORDER_NATURAL if ordering == "natural" else G.ORDER_COLUMN3)
+    scale = abs(stats[:, col["STORE1"]]).max() * 1e-6
+    worst = 0.0
+    try:
+        assert np.all(g.ifatm() == 2)   # atmone.f:122-126: PNEW >= PONDH_MIN = 0 all over the surface
+        for k in range(1, len(stats) + 1):
+            rc, s = g.advance()
+            ref = stats[k - 1]
+            if k % 50 == 0:
+                say(f"step {k}: t {s['TIME']:.1f} dt {s['DELTAT']:.4g} iter {s['ITER']:.0f} back {s['BACK']:.0f}")
+            assert s["DELTAT"] == ref[col["DELTAT"]], (k, s["DELTAT"], ref[col["DELTAT"]])
+            assert s["TIME"] == ref[col["TIME"]], k
+            assert s["ITER"] == ref[col["ITER"]], (k, s["ITER"], ref[col["ITER"]])
+            assert s["BACK"] == ref[col["BACK"]], k
+            assert s["PONDING"] == 1, k
+            if ordering == "natural":
+                assert s["NITERT"] == ref[col["NITERT"]], (k, s["NITERT"], ref[col["NITERT"]])
+            for key in MB_KEYS:
+                assert abs(s[key] - ref[col[key]]) <= 1e-8 * max(abs(ref[col[key]]), scale), \
+                    (k, key, s[key], ref[col[key]])
+            row = rows[k - 1]
+            for c, key in MBE_COLS.items():
+                if key == "ITER":
+                    assert int(s[key]) == int(row[c]), (k, key)
+                else:
+                    assert _printed(s[key], row[c], 1e-8 * max(abs(s[key]), scale)), (k, key, s[key], row[c])
+            if k in saved:
+                i = saved.index(k)
+                e = _rel_inf(g.psi(), gold["psi"][i])
+                worst = max(worst, e)
+                assert e <= PSI_TOL, (k, e)
+                assert _rel_inf(g.sw(), gold["sw"][i]) <= PSI_TOL, k
+                assert np.array_equal(g.ifatm(), gold["ifatm"][i]), k
+                pond, _, _, nrouted = g.routing_state()
+                assert nrouted >= k - 1
+                assert np.max(np.abs(pond - gold["pondnod"][i])) <= PSI_TOL * max(np.abs(gold["pondnod"][i]).max(),
+                                                                                  1e-3), k
+            if k == 179:
+                for val, b in ((g.psi(), psi179), (g.sw(), sw179)):
+                    assert np.all(np.abs(val - b) <= 5.0000001e-7 * np.abs(b) + PSI_TOL * np.abs(b).max())
+            if rc:
+                break
+        assert k == len(stats) and rc == int(gold["rc_last"]) == 1, (k, rc)
+        assert g.routing_state()[3] == int(gold["routed"])
+    finally:
+        g.close()
+    say(f"weil {ordering}: {k} steps, worst psi rel {worst:.2e}")
+
+
+def test_weil_without_its_network_stops_where_it_ponds():
+    """Without the drainage network the device cannot route: the coupled
+    run stops (3) after the first step, which equals the reference's."""
     from pycathy_wrapper_amd import gpu as G
     gold = np.load(os.path.join(GOLD, "weil_ref.npz"))
     col = {nm: j for j, nm in enumerate(str(v) for v in gold["stat_names"])}
     ref = gold["stats"][0]
     pr = PJ.load_project(os.path.join(GOLD, "weil_project"))
+    pr.routing = None
     g = G.FlowGPU(pr, G.ORDER_NATURAL)
     try:
-        assert np.all(g.ifatm() == 2)   # atmone.f:122-126: PNEW >= PONDH_MIN = 0 everywhere on the surface
         rc, s = g.advance()
         assert rc == 3 and s["PONDING"] == 1
         for key in ("DELTAT", "TIME", "ITER", "BACK", "NITERT"):
             assert s[key] == ref[col[key]], (key, s[key], ref[col[key]])
-        scale = abs(ref[col["STORE1"]]) * 1e-6
-        for key in MB_KEYS:
-            assert abs(s[key] - ref[col[key]]) <= 1e-8 * max(abs(ref[col[key]]), scale), key
         assert _rel_inf(g.psi(), gold["psi"][0]) <= PSI_TOL
-        assert np.array_equal(g.ifatm(), gold["ifatm"][0])
-        rc2, _ = g.advance()
-        assert rc2 == 3   # the run stays stopped
+        assert g.advance()[0] == 3   # the run stays stopped
     finally:
         g.close()
     # the same project as ISIMGR = 1 takes SWITCH_OLD and does not stop there

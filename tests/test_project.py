@@ -218,6 +218,63 @@ def test_weil_coupled_run_equals_reference_program():
     assert np.any(gold["pondnod"][-1] > 0.0)
 
 
+def test_reader_matches_reference_datin():
+    """project.load_project on weil_exemple against the reference's own
+    DATIN reading the same files (oracle/refproject.py): every parm and
+    soil value DATIN returns, the surface mesh (TRIANG, X, Y, Z), the
+    vegetation types, and the drainage network and cell numbering of the
+    routing block (INDEX, INDEX_WITH_LAKES, CELL, QOI_SN, the DTM grids as
+    COMMON /RIVERNETWORK/ holds them, DELTA_X / DELTA_Y)."""
+    if not os.path.exists(REF_BC_SO):
+        pytest.skip("oracle/_ref/bc not built (make -C oracle ref-bc)")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as P
+    import refproject as RP
+    root = os.path.join(GOLD, "weil_project")
+    rp = RP.RefProject(P.RefLib(P.REF_BC_SO), root)
+    try:
+        pr = PJ.load_project(root)
+        m, prm, parm = pr.mesh, pr.params, pr.meta["parm"]
+        for key in ("DELTAT", "DTMIN", "DTMAX", "TMAX", "DTMAGA", "DTMAGM", "DTREDS", "DTREDM", "TOLUNS",
+                    "TOLSWI", "ERNLMX", "TOLCG", "TETAF", "PONDH_MIN"):
+            assert getattr(prm, key.lower()) == rp.scalar(key), key
+        for key in ("ITUNS", "ITUNS1", "ITUNS2", "ITMXCG", "LUMP", "IOPT", "L2NORM"):
+            assert getattr(prm, key.lower()) == rp.scalar(key), key
+        for key in ("ISIMGR", "KSLOPE", "ISFCVG", "NLRELX", "IPRT1", "NPRT"):
+            assert parm[key.lower()] == rp.scalar(key), key
+        assert (rp.scalar("NNOD"), rp.scalar("NTRI"), rp.scalar("NSTR"), rp.scalar("N"), rp.scalar("NT")) == \
+            (m.nnod, m.ntri, m.nstr, m.n, m.nt)
+        ms = rp.ref.maxstr
+        for key in ("PERMX", "PERMY", "PERMZ", "ELSTOR", "POROS", "VGNCELL", "VGRMCCELL", "VGPSATCELL"):
+            tab = rp.a[key][:ms * rp.scalar("NZONE")].reshape(-1, ms).T[:m.nstr]
+            mine = pr.soil[{"VGNCELL": "vgn", "VGRMCCELL": "vgrmc", "VGPSATCELL": "vgpsat"}.get(key, key.lower())]
+            assert np.array_equal(mine, tab), key
+        tri = rp.a["TRIANG"][:4 * m.ntri].reshape(m.ntri, 4)
+        assert np.array_equal(np.sort(tri[:, :3], axis=1) - 1, m.triang[:, :3]) and np.array_equal(tri[:, 3],
+                                                                                                    m.triang[:, 3])
+        for key, v in (("X", m.x), ("Y", m.y), ("Z", m.z)):
+            assert np.array_equal(rp.a[key][:m.nnod], v[:m.nnod]), key
+        assert np.array_equal(rp.a["VEG_TYPE"][:m.nnod], pr.veg["veg_type"])
+        r = pr.routing
+        assert r is not None and (r["nrow"], r["ncol"]) == (rp.scalar("NROW"), rp.scalar("NCOL"))
+        assert np.array_equal(r["index"], rp.grid("INDEX")) and np.array_equal(r["index_wl"],
+                                                                                rp.grid("INDEX_WITH_LAKES"))
+        nc = rp.scalar("NCELL_COARSE")
+        assert np.array_equal(r["cell"] + 1, rp.a["CELL"][:5 * nc].reshape(nc, 5)[:, :4])
+        rn = rp.rivernetwork()
+        net = r["net"]
+        assert (net.dx, net.dy) == (rn["DELTA_X"], rn["DELTA_Y"]) and r["ncell"] == rp.scalar("NCELL")
+        assert np.array_equal(net.qoi, rn["QOI_SN"])
+        names = dict(w1="DTM_W_1", w2="DTM_W_2", slope1="DTM_LOCAL_SLOPE_1", slope2="DTM_LOCAL_SLOPE_2",
+                     epl1="DTM_EPL_1", epl2="DTM_EPL_2", kss1="DTM_KSS1_SF_1", kss2="DTM_KSS1_SF_2",
+                     ws1="DTM_WS1_SF_1", ws2="DTM_WS1_SF_2", b1="DTM_B1_SF", y1="DTM_Y1_SF", nrc="DTM_NRC",
+                     pout1="DTM_P_OUTFLOW_1", pout2="DTM_P_OUTFLOW_2")
+        for k, name in names.items():   # Network holds them as stored: row J from the south
+            assert np.array_equal(getattr(net, k), rn[name][::-1]), k
+    finally:
+        rp.close()
+
+
 def test_bc_noflow_golden_rederived():
     if not os.path.exists(REF_BC_SO):
         pytest.skip("oracle/_ref/bc not built (make -C oracle ref-bc)")
