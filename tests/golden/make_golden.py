@@ -33,15 +33,19 @@ container, where the reference tree is mounted at /root/reference).
    reference's Newton mass-balance terms are stale (its Newton branch never
    sets FLOW3D's local SWNEW/SWTIMEP) and are stored as computed.
 
-8. bc_noflow_project/, outlets_project/, weil_project/ -- example projects
-   as pyCATHY lays them out (input/, prepro/: the files project.load_project
-   reads), copied from examplesTmp/SSHydro/bc_noflow,
-   examplesTmp/catchments/outlets and examples/SSHydro/weill_exemple, with
+8. bc_noflow_project/, outlets_project/, weil_project/,
+   spatial_atmbc_project/, soil3d_project/ -- example projects as pyCATHY
+   lays them out (input/, prepro/: the files project.load_project reads),
+   copied from examplesTmp/SSHydro/bc_noflow, examplesTmp/catchments/outlets,
+   examples/SSHydro/weill_exemple, examples/SSHydro/atmbc_spatially_from_weill
+   (per-node rainfall, HSPATM = 0) and examples/SSHydro/soil3d_from_weill
+   (400 soil zones), with
    what the reference program wrote for them: output/mbeconv as written,
    output/psi_sw.npz (the psi and sw blocks of output/psi and output/sw,
    values as printed, with their NSTEP and TIME) and, for weil, the head of
    output/risul (parameter echo, soil table, initial heads).
-9. bc_noflow_ref.npz, outlets_ref.npz, weil_ref.npz -- those projects run by the
+9. bc_noflow_ref.npz, outlets_ref.npz, weil_ref.npz, spatial_atmbc_ref.npz,
+   soil3d_ref.npz -- those projects run by the
    reference's FLOW3D with the coupled switching (SURF true: SWITCH in
    CONVER, ADRSTN at step starts and in BKSTEP), their own nansfneubc /
    sfbc / atmbc files read by the reference's BCONE / SFVONE / ATMONE
@@ -328,12 +332,15 @@ PROJECTS = {
     "bc_noflow_project": "examplesTmp/SSHydro/bc_noflow",
     "outlets_project": "examplesTmp/catchments/outlets",
     "weil_project": "examples/SSHydro/weill_exemple",
+    "spatial_atmbc_project": "examples/SSHydro/atmbc_spatially_from_weill",
+    "soil3d_project": "examples/SSHydro/soil3d_from_weill",
 }
+ROUTED = ("weil_project", "spatial_atmbc_project", "soil3d_project")
 PROJECT_FILES = ("input/parm", "input/soil", "input/ic", "input/atmbc", "input/dem_parameters",
                  "input/nansfdirbc", "input/nansfneubc", "input/sfbc", "input/root_map", "prepro/dem",
                  "prepro/zone")
 # the drainage network and reservoir files DATIN reads for ISIMGR = 2
-# (weil only: the project whose surface ponds)
+# (the projects whose surface ponds)
 ROUTING_FILES = ("input/posizione_serb", "input/livelli_iniz_s", "prepro/lakes_map", "prepro/qoi_a",
                  "prepro/dtm_w_1", "prepro/dtm_w_2", "prepro/dtm_p_outflow_1", "prepro/dtm_p_outflow_2",
                  "prepro/dtm_local_slope_1", "prepro/dtm_local_slope_2", "prepro/dtm_epl_1", "prepro/dtm_epl_2",
@@ -360,7 +367,7 @@ def copy_projects() -> None:
     import shutil
     for name, src in PROJECTS.items():
         dst = os.path.join(HERE, name)
-        for rel in PROJECT_FILES + (ROUTING_FILES if name == "weil_project" else ()):
+        for rel in PROJECT_FILES + (ROUTING_FILES if name in ROUTED else ()):
             s = os.path.join(REF, src, rel)
             if os.path.exists(s):
                 os.makedirs(os.path.dirname(os.path.join(dst, rel)), exist_ok=True)
@@ -447,3 +454,6 @@ if __name__ == "__main__":
                      steps_saved=(1, 2, 5, 10, 50, 100, 200, 300, 400, 500, 600, 700))
     make_project_run("weil_project", os.path.join(HERE, "weil_ref.npz"),
                      steps_saved=(1, 2, 10, 50, 100, 179, 235))
+    make_project_run("spatial_atmbc_project", os.path.join(HERE, "spatial_atmbc_ref.npz"),
+                     steps_saved=(1, 2, 10, 100, 500, 1027))
+    make_project_run("soil3d_project", os.path.join(HERE, "soil3d_ref.npz"), steps_saved=(1, 2, 10, 20, 40))
