@@ -417,7 +417,10 @@ int cathy_gpu_flow3d_counters(int64_t *handle, double *out6);
  * at (I-1) + (J-1)*LDC (LDC = COLMAX >= NCOL): P_OUTFLOW_1/2, W_1/2,
  * LOCAL_SLOPE_1/2, EPL_1/2, KSS1_SF_1/2, WS1_SF_1/2, B1_SF, Y1_SF, NRC.
  * Every routed branch (W /= 0; the last cell's cardinal branch excepted)
- * must drain to a grid cell listed later in QOI_SN, else -1. */
+ * must drain to a grid cell listed later in QOI_SN, or to one QOI_SN does
+ * not list (a no-data cell: its inflow then enters only ALTEZZE), else -1.
+ * The downstream I_BASIN is formed as route.f:86-88 forms it, so a branch
+ * past the grid's edge names the grid cell that index falls on. */
 int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol,
                            const int *ldc, const int *ncell, const int *qoi_sn,
                            const int *p_outflow_1, const int *p_outflow_2,

@@ -69,6 +69,11 @@ struct RouteDev {
   const double *prm;        // [kNParam][ncell]
   int numres;
   ResDev res;
+  // branches draining to a cell QOI_SN does not route (a no-data cell, or
+  // where the outlet's diagonal branch lands past the grid's edge): pairs
+  // (source grid index * 2 + branch, target grid index), in QOI_SN order
+  int nsink;
+  const int *sink;
 };
 
 // The flow-independent factors of MC for one branch (mc.f:40-49), in the
@@ -262,6 +267,16 @@ __global__ __launch_bounds__(kRouteBlock) void k_route(
     kbeg = kend;
     kend = knext;
   }
+  // route.f:133-137, 244-246: the outflows into unrouted cells, in the
+  // order the reference adds them
+  if (d.nsink) {
+    if (threadIdx.x == 0)
+      for (int u = 0; u < d.nsink; ++u) {
+        const int src = d.sink[2 * u], tgt = d.sink[2 * u + 1];
+        qin[tgt] = qin[tgt] + qout[src & 1][src >> 1];
+      }
+    __syncthreads();
+  }
   // ALTEZZE (altezze.f:33-105) over every grid cell; reservoir and lake
   // depths only where the caller's type array asks for them
   for (int c = threadIdx.x; c < d.ngrid; c += blockDim.x) {
@@ -306,7 +321,8 @@ struct Route {
   int nrow = 0, ncol = 0, ncell = 0, ngrid = 0, nlev = 0, maxpred = 0;
   std::mutex mu;  // steps and reservoir set-up on one handle share its device buffers
   hipStream_t st = nullptr;
-  DBuf<int> lev_ptr, cell, key, pred;
+  DBuf<int> lev_ptr, cell, key, pred, sink;
+  int nsink = 0;
   DBuf<double> prm;
   // per-step state [member][ngrid]: 5 inputs, 4 in/out (volume_kkp1 keeps
   // reservoir cells), 1 output; cu_ak; pool levels [member][numres] x 2
@@ -329,7 +345,8 @@ struct Route {
   RouteDev dev() const {
     return RouteDev{ncell, ngrid, nlev, lev_ptr.p, cell.p, key.p, pred.p, prm.p, numres,
                     ResDev{maxres, r_nha.p, r_hres.p, r_ares.p, r_soglia.p, r_fondo.p, r_rid.p, r_rflag.p,
-                           typed ? r_alt.p : nullptr, r_elev.p}};
+                           typed ? r_alt.p : nullptr, r_elev.p},
+                    nsink, sink.p};
   }
 };
 
@@ -414,7 +431,7 @@ int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol, co
                                    kss1_sf_2, ws1_sf_1, ws1_sf_2, b1_sf, y1_sf, nrc};
     // feeders of each cell in QOI_SN order, branch 1 before 2 (route.f:133-137, 220-221)
     std::vector<std::vector<int>> feeders(n);
-    std::vector<int> level(n, 0);
+    std::vector<int> level(n, 0), sinks;
     int nlev = 1;
     for (int p = 0; p < n; ++p) {
       const int ib = qoi_sn[p];
@@ -428,9 +445,16 @@ int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol, co
         const int di = (int)lroundf((float)(code - 5) / 3.0f);  // NINT((P-5)/3.)
         const int dj = code - 5 - 3 * di;
         const int i2 = i + di, j2 = j + dj;
-        if (i2 < 1 || i2 > nc || j2 < 1 || j2 > nr)
-          throw Error(-1, "cell " + std::to_string(ib) + " drains off the grid");
-        const int t = pos[(i2 - 1) * nr + j2 - 1];
+        // the downstream I_BASIN as route.f forms it (:86-88), also past the
+        // grid's edge, where it names another cell of the grid
+        const int64_t tb = (int64_t)(i2 - 1) * nr + j2;
+        if (tb < 1 || tb > ngrid) throw Error(-1, "cell " + std::to_string(ib) + " drains off the grid");
+        const int t = pos[tb - 1];
+        if (t < 0) {  // a cell QOI_SN does not route: its inflow only enters ALTEZZE
+          sinks.push_back(((i - 1) * nr + j - 1) * 2 + br);
+          sinks.push_back((int)tb - 1);
+          continue;
+        }
         if (t <= p)
           throw Error(-1, "cell " + std::to_string(ib) + " drains to a cell not after it in QOI_SN");
         feeders[t].push_back(((i - 1) * nr + j - 1) * 2 + br);
@@ -497,6 +521,8 @@ int cathy_gpu_route_create(int64_t *handle, const int *nrow, const int *ncol, co
     r->key.upload(key, r->st);
     r->pred.upload(pred, r->st);
     r->prm.upload(prm, r->st);
+    r->nsink = (int)sinks.size() / 2;
+    if (r->nsink) r->sink.upload(sinks, r->st);
     CG_HIP(hipStreamSynchronize(r->st));
     std::lock_guard<std::mutex> lk(g_route_mu);
     *handle = g_route_next++;

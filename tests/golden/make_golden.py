@@ -38,14 +38,15 @@ container, where the reference tree is mounted at /root/reference).
    lays them out (input/, prepro/: the files project.load_project reads),
    copied from examplesTmp/SSHydro/bc_noflow, examplesTmp/catchments/outlets,
    examples/SSHydro/weill_exemple, examples/SSHydro/atmbc_spatially_from_weill
-   (per-node rainfall, HSPATM = 0) and examples/SSHydro/soil3d_from_weill
-   (400 soil zones), with
+   (per-node rainfall, HSPATM = 0), examples/SSHydro/soil3d_from_weill
+   (400 soil zones) and examples/SSHydro/atmbc_spatially_from_weill_withnodata
+   (a DEM with no-data cells, 326 surface nodes), with
    what the reference program wrote for them: output/mbeconv as written,
    output/psi_sw.npz (the psi and sw blocks of output/psi and output/sw,
    values as printed, with their NSTEP and TIME) and, for weil, the head of
    output/risul (parameter echo, soil table, initial heads).
 9. bc_noflow_ref.npz, outlets_ref.npz, weil_ref.npz, spatial_atmbc_ref.npz,
-   soil3d_ref.npz -- those projects run by the
+   soil3d_ref.npz, nodata_ref.npz -- those projects run by the
    reference's FLOW3D with the coupled switching (SURF true: SWITCH in
    CONVER, ADRSTN at step starts and in BKSTEP), their own nansfneubc /
    sfbc / atmbc files read by the reference's BCONE / SFVONE / ATMONE
@@ -334,8 +335,9 @@ PROJECTS = {
     "weil_project": "examples/SSHydro/weill_exemple",
     "spatial_atmbc_project": "examples/SSHydro/atmbc_spatially_from_weill",
     "soil3d_project": "examples/SSHydro/soil3d_from_weill",
+    "nodata_project": "examples/SSHydro/atmbc_spatially_from_weill_withnodata",
 }
-ROUTED = ("weil_project", "spatial_atmbc_project", "soil3d_project")
+ROUTED = ("weil_project", "spatial_atmbc_project", "soil3d_project", "nodata_project")
 PROJECT_FILES = ("input/parm", "input/soil", "input/ic", "input/atmbc", "input/dem_parameters",
                  "input/nansfdirbc", "input/nansfneubc", "input/sfbc", "input/root_map", "prepro/dem",
                  "prepro/zone")
@@ -385,7 +387,8 @@ def copy_projects() -> None:
             blocks = read_blocks(os.path.join(REF, src, "output", kind))
             keep = {}
             for nstep, t, v in blocks:   # first block of each NSTEP (the last is written twice)
-                keep.setdefault(nstep, (t, v))
+                if len(v) == len(blocks[0][2]):   # a block cut short (a run stopped while writing) is left out
+                    keep.setdefault(nstep, (t, v))
             steps = sorted(keep)
             arrs[kind + "_nstep"] = np.array(steps)
             arrs[kind + "_time"] = np.array([keep[k][0] for k in steps])
@@ -457,3 +460,4 @@ if __name__ == "__main__":
     make_project_run("spatial_atmbc_project", os.path.join(HERE, "spatial_atmbc_ref.npz"),
                      steps_saved=(1, 2, 10, 100, 500, 1027))
     make_project_run("soil3d_project", os.path.join(HERE, "soil3d_ref.npz"), steps_saved=(1, 2, 10, 20, 40))
+    make_project_run("nodata_project", os.path.join(HERE, "nodata_ref.npz"), steps_saved=(1, 2, 10, 50, 100, 180))

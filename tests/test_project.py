@@ -17,9 +17,10 @@ hold).
    (SWITCH in CONVER, ADRSTN at step starts and back-steps; oracle/refrun.py
    with surf=True, golden bc_noflow_ref.npz / outlets_ref.npz), and on
    atmbc_spatially_from_weill (per-node rainfall, 1027 steps) and
-   soil3d_from_weill (400 soil zones, the 40 steps its output holds) with
-   the routing block too (spatial_atmbc_ref.npz, soil3d_ref.npz: both pond
-   from their first step), equals the
+   soil3d_from_weill (400 soil zones, the 40 steps its output holds) and
+   atmbc_spatially_from_weill_withnodata (a DEM with no-data cells, 180
+   steps, ponded at 15 of them) with the routing block too
+   (spatial_atmbc_ref.npz, soil3d_ref.npz, nodata_ref.npz), equals the
    reference program's own stored mbeconv at every stored step -- DELTAT,
    TIME, the Picard count NLIN, STORE1/2, DSTORE, VIN, VOUT, the mass-balance
    error and their cumulative sums -- to the digits printed, and psi / Sw
@@ -147,7 +148,7 @@ def _mbeconv(name):
     return rows
 
 
-@pytest.mark.parametrize("name", ["bc_noflow", "outlets", "spatial_atmbc", "soil3d"])
+@pytest.mark.parametrize("name", ["bc_noflow", "outlets", "spatial_atmbc", "soil3d", "nodata"])
 def test_coupled_runs_equal_reference_program_outputs(name):
     gold = np.load(os.path.join(GOLD, f"{name}_ref.npz"))
     col = {nm: j for j, nm in enumerate(str(v) for v in gold["stat_names"])}
@@ -162,11 +163,16 @@ def test_coupled_runs_equal_reference_program_outputs(name):
                 assert int(v) == int(row[c]), (k + 1, key)
             else:
                 assert _printed_close(v, row[c]), (k + 1, key, v, row[c])
-        # bc_noflow and outlets never pond (their routing never runs); the
-        # other two pond from their first step on and route every later one
-        assert stats[k, col["PONDING"]] == (0 if name in ("bc_noflow", "outlets") else 1)
-    if name not in ("bc_noflow", "outlets"):
+        # bc_noflow and outlets never pond (their routing never runs);
+        # spatial_atmbc and soil3d pond from their first step on and route
+        # every later one; nodata ponds now and then
+        if name in ("bc_noflow", "outlets", "spatial_atmbc", "soil3d"):
+            assert stats[k, col["PONDING"]] == (0 if name in ("bc_noflow", "outlets") else 1)
+    if name in ("spatial_atmbc", "soil3d"):
         assert int(gold["routed"]) == len(stats) - 1
+    if name == "nodata":
+        npond = int(stats[:-1, col["PONDING"]].sum())
+        assert 0 < npond < len(stats) - 1 and int(gold["routed"]) >= npond
     out = np.load(os.path.join(GOLD, f"{name}_project", "output", "psi_sw.npz"))
     steps = list(gold["steps"])
     hit = 0
