@@ -77,7 +77,7 @@ class Network:
 
 
 def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0, reservoir: bool = False,
-                      masked: float = 0.0) -> Network:
+                      masked: float = 0.0, sinks: bool = False) -> Network:
     """A drainage network on z = s*((I-1) + (J-1))*dx + noise (|noise| <
     s*dx/4, so every cell but (1, 1) has a lower cardinal neighbour and no
     two cells tie).  Each cell sends its water down its steepest cardinal
@@ -94,7 +94,10 @@ def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0, res
     With ``masked`` > 0: that fraction of the grid, the highest cells, lies
     outside the catchment (ZONE = 0 in the reference's DEM): those cells are
     left out of QOI_SN and route nothing, so NCELL < NROW*NCOL.  No cell
-    drains into them, since every drainage path descends."""
+    drains into them, since every drainage path descends -- unless
+    ``sinks``: then every routed cell with an outside diagonal neighbour
+    sends 30% of its water there (route.f adds such outflow to the cell's
+    inflow, which only ALTEZZE uses)."""
     rng = np.random.default_rng(seed)
     s = 0.02
     I = np.arange(1, ncol + 1)[None, :]
@@ -128,6 +131,21 @@ def synthetic_network(nrow: int, ncol: int, dx: float = 10.0, seed: int = 0, res
         w2.ravel(order="C")[out] = 0.0
         order = order[nmask:]
     qoi = ib.ravel(order="C")[order].astype(np.int32)
+    if sinks and nmask:
+        routed = np.zeros(nrow * ncol, dtype=bool)
+        routed[qoi - 1] = True
+        for jj in range(nrow):
+            for ii in range(ncol):
+                if not routed[ii * nrow + jj] or ii * nrow + jj + 1 == qoi[-1]:
+                    continue
+                for di, dj in DIAGONAL:
+                    i2, j2 = ii + di, jj + dj
+                    if 0 <= i2 < ncol and 0 <= j2 < nrow and not routed[i2 * nrow + j2]:
+                        pout2[jj, ii] = 5 + 3 * di + dj
+                        w2[jj, ii] = 0.3
+                        w1[jj, ii] = 0.7
+                        sl2[jj, ii] = max(sl2[jj, ii], 1e-3)
+                        break
     f = lambda lo, hi: rng.uniform(lo, hi, (nrow, ncol))
     res = None
     if reservoir:

@@ -39,16 +39,19 @@ def _compare(net, dev_st, orc_states, cu_d, ak_d, cu_o, ak_o, tag):
         assert cu_d[m] == pytest.approx(cu_o[m], rel=TOL) and ak_d[m] == pytest.approx(ak_o[m], rel=TOL), (tag, m)
 
 
-@pytest.mark.parametrize("reservoir,typed,masked", [(False, False, 0.0), (True, False, 0.0), (True, True, 0.0),
-                                                    (False, False, 0.3)],
-                         ids=["channels", "reservoir", "reservoir-typed-altezze", "masked-catchment"])
-def test_route_substeps_match_oracle(reservoir, typed, masked):
+@pytest.mark.parametrize("reservoir,typed,masked,sinks", [(False, False, 0.0, False), (True, False, 0.0, False),
+                                                          (True, True, 0.0, False), (False, False, 0.3, False),
+                                                          (False, False, 0.3, True)],
+                         ids=["channels", "reservoir", "reservoir-typed-altezze", "masked-catchment", "sinks"])
+def test_route_substeps_match_oracle(reservoir, typed, masked, sinks):
     """The 20x20 network of the reference pin, one member, six substeps
     (each substep's new levels feed the next); with a reservoir whose pool
     rises through its spill height.  ALTEZZE takes the reference's all-zero
     cell types, or TIPO_R ("typed"); "masked" leaves 30% of the grid outside
-    the catchment (NCELL < NROW*NCOL)."""
-    net = RT.synthetic_network(20, 20, dx=10.0, seed=2, reservoir=reservoir, masked=masked)
+    the catchment (NCELL < NROW*NCOL); "sinks" besides sends part of the
+    water of the cells beside it diagonally into cells outside the catchment,
+    which the device adds after the levels, in QOI_SN order."""
+    net = RT.synthetic_network(20, 20, dx=10.0, seed=2, reservoir=reservoir, masked=masked, sinks=sinks)
     import pyoracle as P
     orc = P.Oracle()
     celtype = net.reservoirs["tipo_r"] if typed else None

@@ -155,10 +155,11 @@ def net():
 
 
 @pytest.mark.skipif(not has_ref(), reason="oracle/_ref not built (needs the reference sources)")
-@pytest.mark.parametrize("reservoir,typed,masked", [(False, False, 0.0), (True, False, 0.0), (True, True, 0.0),
-                                                    (False, False, 0.3)],
-                         ids=["channels", "reservoir", "reservoir-typed-altezze", "masked-catchment"])
-def test_route_altezze_bit_identical(oracle, ref, reservoir, typed, masked):
+@pytest.mark.parametrize("reservoir,typed,masked,sinks", [(False, False, 0.0, False), (True, False, 0.0, False),
+                                                          (True, True, 0.0, False), (False, False, 0.3, False),
+                                                          (False, False, 0.3, True)],
+                         ids=["channels", "reservoir", "reservoir-typed-altezze", "masked-catchment", "sinks"])
+def test_route_altezze_bit_identical(oracle, ref, reservoir, typed, masked, sinks):
     """Six substeps (dt = 30 s): the reference's ROUTE + ALTEZZE and the
     oracle give bit-identical outflows, inflows, volumes, depths, CU_MAX and
     AK_MAX.  Both branches are routed in most cells, and the last cell's
@@ -168,8 +169,12 @@ def test_route_altezze_bit_identical(oracle, ref, reservoir, typed, masked):
     reservoir outlet and lake cells take the pool's depth.  With a reservoir
     (SERBATOIO, CALCOLI): its pool level too, which rises through the spill
     height, so the outlet passes from holding its inflow to releasing it.
-    "masked": 30% of the grid lies outside the catchment, NCELL < NROW*NCOL."""
-    net = RT.synthetic_network(REF_ROWMAX, REF_COLMAX, dx=10.0, seed=2, reservoir=reservoir, masked=masked)
+    "masked": 30% of the grid lies outside the catchment, NCELL < NROW*NCOL;
+    "sinks": besides, routed cells beside the masked ones send 30% of their
+    water diagonally into them, where ROUTE adds it to inflows that only
+    ALTEZZE reads."""
+    net = RT.synthetic_network(REF_ROWMAX, REF_COLMAX, dx=10.0, seed=2, reservoir=reservoir, masked=masked,
+                               sinks=sinks)
     rr = RefRouting(ref, net)
     st = initial_state(net)
     assert (net.w2.ravel() > 0).sum() > net.ncell // 2
@@ -190,6 +195,10 @@ def test_route_altezze_bit_identical(oracle, ref, reservoir, typed, masked):
         ak = ak_o
         advance(st)
     assert st["q_in_kk"].max() > 0.0
+    if sinks:
+        routed = np.zeros(net.ngrid, dtype=bool)
+        routed[net.qoi - 1] = True
+        assert st["q_in_kk"][~routed].max() > 0.0 and st["volume_kk"][~routed].max() > 0.0
     if reservoir:
         assert not spills[0] and spills[-1], spills
 
