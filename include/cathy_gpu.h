@@ -236,6 +236,37 @@ int cathy_gpu_enkf_update(const int *nens, int64_t *handles, const double *w,
  * Results are those of cathy_gpu_enkf_update (the same kernel). */
 int cathy_gpu_enkf_update_noalloc(const int *nens, int64_t *handles,
                                   const double *w, const double *inflation);
+/* Localised EnKF analysis of the member states in place (pyCATHY DA/enkf.py:
+ * 225-343, enkf_analysis_localized_with_inflation: P_xo[states] * L before
+ * the update, :307-313; state inflation :318-320).  handles: nens flow
+ * handles on one mesh and ordering (2 <= nens <= 64); *nobs = m
+ * observations; s: S = HX - mean(HX), s[j*nens + k]; invd: COV^-1 D (or
+ * D / diag(R) in the Sakov form), same layout.  The localisation is
+ * either loc, the dense L(i, j) at loc[i*m + j] over the N state rows in
+ * CATHY order (the reference's L argument), or, with loc NULL, the
+ * Gaspari-Cohn weights of DA/localisation.py:136-188 formed on the device
+ * from node_xy (N x 2, CATHY order), obs_xy (m x 2) and *radius.  Member
+ * x_k of each row becomes x_k + sum_j (sum_l x'_l S(j,l))/(nens-1) L(i,j)
+ * invD(j,k), then inflation about the analysis mean (*inflation, 1 =
+ * none); each run continues from it (cathy_gpu_flow3d_adopt_state).  The
+ * augmented state's parameters are the caller's (unlocalised, as in the
+ * reference). */
+int cathy_gpu_enkf_update_localized(const int *nens, int64_t *handles,
+                                    const int *nobs, const double *s,
+                                    const double *invd, const double *loc,
+                                    const double *node_xy,
+                                    const double *obs_xy,
+                                    const double *radius,
+                                    const double *inflation);
+/* Particle-filter resampling of the member states in place (pyCATHY
+ * DA/pf.py:98-109 ensemble[:, indices], then the state jitter of
+ * :129-130).  idx: nens 0-based member indices (systematic_resample's
+ * output; the identity to jitter only); member j's PNEW becomes member
+ * idx[j]'s, plus *jitter_std times a standard normal drawn per (CATHY row,
+ * member j) from Philox4x32-10 under *seed with the Box-Muller transform
+ * (0: exact copies).  Each run continues from its new state. */
+int cathy_gpu_pf_resample(const int *nens, int64_t *handles, const int *idx,
+                          const double *jitter_std, const int64_t *seed);
 /* Continue a run from its current PNEW (after set_pnew or an analysis):
  * PTIMEP = PNEW, Sw and kr at it, storage baseline reset. */
 int cathy_gpu_flow3d_adopt_state(int64_t *handle);

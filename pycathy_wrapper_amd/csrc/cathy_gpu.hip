@@ -64,9 +64,12 @@ static unsigned char *scratch_ptr() {
 // analysis mean), with W and the state pointer tables staged in device
 // memory the caller provides (dw: m*m doubles, dx / dxa: m pointers each);
 // every member then continues from its analysis state.
-static void enkf_apply(int m, int64_t *handles, const double *w, double inflation, double *dw,
-                       const double **dx, double **dxa) {
-  std::array<std::shared_ptr<Flow>, kEnkfMax> fl;
+using Members = std::array<std::shared_ptr<Flow>, kEnkfMax>;
+
+// The members of an analysis: initialised, distinct, on one mesh and node
+// ordering; their queued work has ended when this returns.
+static Members analysis_members(int m, int64_t *handles) {
+  Members fl;
   for (int j = 0; j < m; ++j) fl[j] = lookup(&handles[j]);
   Flow *f0 = fl[0].get();
   for (int j = 0; j < m; ++j) {
@@ -75,9 +78,19 @@ static void enkf_apply(int m, int64_t *handles, const double *w, double inflatio
     CG_REQUIRE(fl[j]->n == f0->n && fl[j]->perm == f0->perm,
                "members differ in mesh or node ordering; their states do not align");
   }
-  for (int j = 0; j < m * m; ++j) CG_REQUIRE(std::isfinite(w[j]), "non-finite analysis weight");
   // every member's queued work ends before its state is read
   for (int j = 0; j < m; ++j) CG_HIP(hipStreamSynchronize(fl[j]->st));
+  return fl;
+}
+
+// register width of the member loops: the smallest of 8, 16, 32, 64 >= m
+static int member_width(int m) { return m <= 8 ? 8 : m <= 16 ? 16 : m <= 32 ? 32 : 64; }
+
+static void enkf_apply(int m, int64_t *handles, const double *w, double inflation, double *dw,
+                       const double **dx, double **dxa) {
+  for (int j = 0; j < m * m; ++j) CG_REQUIRE(std::isfinite(w[j]), "non-finite analysis weight");
+  Members fl = analysis_members(m, handles);
+  Flow *f0 = fl[0].get();
   std::array<const double *, kEnkfMax> hx;
   std::array<double *, kEnkfMax> hxa;
   for (int j = 0; j < m; ++j) {
@@ -290,6 +303,101 @@ int cathy_gpu_enkf_update_noalloc(const int *nens, int64_t *handles, const doubl
     const double **dx = reinterpret_cast<const double **>(base + kScratchW);
     double **dxa = reinterpret_cast<double **>(base + kScratchW + kScratchTab);
     enkf_apply(m, handles, w, *inflation, dw, dx, dxa);
+    return 0;
+  });
+}
+
+int cathy_gpu_enkf_update_localized(const int *nens, int64_t *handles, const int *nobs, const double *s,
+                                    const double *invd, const double *loc, const double *node_xy,
+                                    const double *obs_xy, const double *radius, const double *inflation) {
+  return guarded([&] {
+    CG_REQUIRE(nens && handles && nobs && s && invd && inflation, "null argument");
+    const int m = *nens, mo = *nobs;
+    CG_REQUIRE(m >= 2 && m <= kEnkfMax, "ensemble size out of range (2..64)");
+    CG_REQUIRE(mo >= 1, "at least one observation");
+    CG_REQUIRE(loc || (node_xy && obs_xy && radius),
+               "give the localisation matrix, or node and observation positions with a radius");
+    if (!loc) CG_REQUIRE(std::isfinite(*radius) && *radius > 0.0, "the localisation radius must be positive");
+    CG_REQUIRE(std::isfinite(*inflation), "non-finite inflation");
+    for (size_t j = 0; j < (size_t)mo * m; ++j)
+      CG_REQUIRE(std::isfinite(s[j]) && std::isfinite(invd[j]), "non-finite observation term");
+    Members fl = analysis_members(m, handles);
+    Flow *f0 = fl[0].get();
+    const int n = f0->n;
+    hipStream_t st = f0->st;
+    std::array<const double *, kEnkfMax> hx;
+    std::array<double *, kEnkfMax> hxa;
+    for (int j = 0; j < m; ++j) hx[j] = hxa[j] = fl[j]->pnew.p;
+    DBuf<const double *> dx(m);
+    DBuf<double *> dxa(m);
+    DBuf<double> ds((size_t)mo * m), dinv((size_t)mo * m), dloc, dnxy, doxy;
+    DBuf<int> dip(n);
+    dx.upload(hx.data(), m, st);
+    dxa.upload(hxa.data(), m, st);
+    ds.upload(s, (size_t)mo * m, st);
+    dinv.upload(invd, (size_t)mo * m, st);
+    dip.upload(f0->iperm.data(), n, st);
+    if (loc) {
+      dloc.alloc((size_t)n * mo);
+      dloc.upload(loc, (size_t)n * mo, st);
+    } else {
+      dnxy.alloc((size_t)2 * n);
+      doxy.alloc((size_t)2 * mo);
+      dnxy.upload(node_xy, (size_t)2 * n, st);
+      doxy.upload(obs_xy, (size_t)2 * mo, st);
+    }
+    const LocArgs a{n, m, mo, dx.p, dxa.p, ds.p, dinv.p, dip.p, dloc.p, dnxy.p, doxy.p,
+                    loc ? 0.0 : *radius, *inflation};
+    const dim3 g(grid_for(n)), b(kBlock);
+    switch (member_width(m)) {
+      case 8: hipLaunchKernelGGL(k_enkf_localized<8>, g, b, 0, st, a); break;
+      case 16: hipLaunchKernelGGL(k_enkf_localized<16>, g, b, 0, st, a); break;
+      case 32: hipLaunchKernelGGL(k_enkf_localized<32>, g, b, 0, st, a); break;
+      default: hipLaunchKernelGGL(k_enkf_localized<64>, g, b, 0, st, a); break;
+    }
+    CG_HIP(hipGetLastError());
+    CG_HIP(hipStreamSynchronize(st));
+    for (int j = 0; j < m; ++j) fl[j]->adopt_state();
+    return 0;
+  });
+}
+
+int cathy_gpu_pf_resample(const int *nens, int64_t *handles, const int *idx, const double *jitter_std,
+                          const int64_t *seed) {
+  return guarded([&] {
+    CG_REQUIRE(nens && handles && idx && jitter_std && seed, "null argument");
+    const int m = *nens;
+    CG_REQUIRE(m >= 2 && m <= kEnkfMax, "ensemble size out of range (2..64)");
+    for (int j = 0; j < m; ++j) CG_REQUIRE(idx[j] >= 0 && idx[j] < m, "resampled member index out of range");
+    CG_REQUIRE(std::isfinite(*jitter_std) && *jitter_std >= 0.0, "the jitter deviation must be finite and >= 0");
+    Members fl = analysis_members(m, handles);
+    Flow *f0 = fl[0].get();
+    const int n = f0->n;
+    hipStream_t st = f0->st;
+    std::array<const double *, kEnkfMax> hs;
+    std::array<double *, kEnkfMax> hd;
+    for (int j = 0; j < m; ++j) {
+      hs[j] = fl[idx[j]]->pnew.p;
+      hd[j] = fl[j]->pnew.p;
+    }
+    DBuf<const double *> dsrc(m);
+    DBuf<double *> ddst(m);
+    DBuf<int> dip(n);
+    dsrc.upload(hs.data(), m, st);
+    ddst.upload(hd.data(), m, st);
+    dip.upload(f0->iperm.data(), n, st);
+    const dim3 g(grid_for(n)), b(kBlock);
+    const double sg = *jitter_std;
+    const uint64_t sd = (uint64_t)*seed;
+    switch (member_width(m)) {
+      case 8: hipLaunchKernelGGL(k_pf_resample<8>, g, b, 0, st, n, m, dsrc.p, dip.p, sg, sd, ddst.p); break;
+      case 16: hipLaunchKernelGGL(k_pf_resample<16>, g, b, 0, st, n, m, dsrc.p, dip.p, sg, sd, ddst.p); break;
+      case 32: hipLaunchKernelGGL(k_pf_resample<32>, g, b, 0, st, n, m, dsrc.p, dip.p, sg, sd, ddst.p); break;
+      default: hipLaunchKernelGGL(k_pf_resample<64>, g, b, 0, st, n, m, dsrc.p, dip.p, sg, sd, ddst.p); break;
+    }
+    CG_HIP(hipGetLastError());
+    CG_HIP(hipStreamSynchronize(st));
+    for (int j = 0; j < m; ++j) fl[j]->adopt_state();
     return 0;
   });
 }

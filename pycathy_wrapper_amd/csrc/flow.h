@@ -2606,7 +2606,36 @@ struct Flow {
     store1 = device_sum_store();
     store2 = store1;
     dtprev = 0.0;
+    mbinit_fluxes();
     CG_HIP(hipStreamSynchronize(st));
+  }
+  // MBINIT (mbinit.f:28-58) at an adopted state: the previous step's
+  // boundary fluxes become those a run started from this state begins
+  // with -- no Dirichlet fluxes, the atmospheric fluxes ATMOLD at the
+  // IFATMP = 0 nodes, and the Neumann fluxes Q (free drainage at the
+  // state's kr, then the listed ones).  The cumulative totals go on.
+  void mbinit_fluxes() {
+    adinp = adoutp = ndinp = ndoutp = 0.0;
+    aninp = anoutp = 0.0;
+    std::vector<double> aold(nnod);
+    std::vector<int> fap(nnod);
+    atmold.download(aold.data(), nnod, st);
+    ifatmp.download(fap.data(), nnod, st);
+    CG_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < nnod; ++s)
+      if (fap[s] == 0) {
+        if (aold[s] > 0.0) aninp += aold[s]; else anoutp += aold[s];
+      }
+    nninp = nnoutp = 0.0;
+    neumann_apply(ckrw.p);
+    if (prm.free_drainage) {
+      std::vector<double> q(nnod);
+      qbot.download(q.data(), nnod, st);
+      CG_HIP(hipStreamSynchronize(st));
+      for (int s = 0; s < nnod; ++s) { if (q[s] > 0.0) nninp += q[s]; else nnoutp += q[s]; }
+    }
+    nninp += nx_in;
+    nnoutp += nx_out;
   }
 
   void get_state(double *pn, double *swo, double *ck, int *fa) {

@@ -429,6 +429,24 @@ __global__ void k_sw_only(int n, const double *psi, SoilSoA s, double *sw);
 constexpr int kEnkfMax = 64;  // members per analysis launch
 __global__ void k_enkf_update(int n, int nens, const double *const *X, const double *W, double alpha,
                               double *const *Xa);
+// the localised analysis: states X/Xa (device order, one pointer per
+// member), S and COV^-1 D (m x nens, row-major), device row -> CATHY row,
+// and either the dense localisation (CATHY rows x m) or node / observation
+// (x, y) positions with the Gaspari-Cohn radius
+struct LocArgs {
+  int n, nens, m;
+  const double *const *X;
+  double *const *Xa;
+  const double *s, *invd;
+  const int *iperm;
+  const double *loc, *node_xy, *obs_xy;
+  double radius, alpha;
+};
+template <int NE>
+__global__ void k_enkf_localized(LocArgs a);
+template <int NE>
+__global__ void k_pf_resample(int n, int nens, const double *const *src, const int *iperm, double sigma,
+                              uint64_t seed, double *const *dst);
 __global__ void k_vel3d(int nt, const int4 *enode, const double *pnew, const double *ckrw, const double *vt,
                         double *uu, double *vv, double *ww);
 __global__ void k_vnod3d(int n, const int *inc_ptr, const int *inc, const double *uu, const double *vv,

@@ -1986,6 +1986,141 @@ __global__ void k_enkf_update(int n, int nens, const double *const *__restrict__
   for (int j = 0; j < nens; ++j) Xa[j][i] = alpha == 1.0 ? xa[j] : meana + alpha * (xa[j] - meana);
 }
 
+// Gaspari-Cohn weight of distance d for radius L (DA/localisation.py:
+// 136-151): the fifth-order piecewise rational function in z = |d|/L, in
+// the reference's nesting; 0 from z = 2 on.
+__device__ __forceinline__ double gaspari_cohn(double d, double L) {
+  const double z = fabs(d) / L;
+  if (z <= 1.0) return (((-0.25 * z + 0.5) * z + 0.625) * z - 5.0 / 3.0) * (z * z) + 1.0;
+  if (z <= 2.0) return ((((z / 12.0 - 0.5) * z + 0.625) * z + 5.0 / 3.0) * z - 5.0) * z + 4.0 - 2.0 / (3.0 * z);
+  return 0.0;
+}
+
+// Localised analysis of the states (enkf.py:304-321), one thread per state
+// row (device order; ri its CATHY row):
+//   c_j = (sum_k x'_k S(j,k)) / (N-1),   x_a,k = x_k + sum_j c_j L(ri,j) invD(j,k),
+// S(j,k) at s[j*nens + k], invD likewise; L(ri,j) from loc[ri*m + j] or,
+// without loc, the Gaspari-Cohn weight of the (x, y) distance from node ri
+// to observation j.  Then inflation about the analysis mean.  Member
+// values stay in registers (NE >= nens, the rest masked); every row of
+// every member is read before the thread writes, so the update is in place.
+template <int NE>
+__global__ __launch_bounds__(256) void k_enkf_localized(LocArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int nens = a.nens;
+  const int ri = a.iperm[i];
+  double xp[NE], dx[NE];
+  double mean = 0.0;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    xp[k] = k < nens ? a.X[k][i] : 0.0;
+    mean += xp[k];
+  }
+  mean /= nens;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    xp[k] = k < nens ? xp[k] - mean : 0.0;
+    dx[k] = 0.0;
+  }
+  double px = 0.0, py = 0.0;
+  if (!a.loc) {
+    px = a.node_xy[2 * (size_t)ri];
+    py = a.node_xy[2 * (size_t)ri + 1];
+  }
+  const double nm1 = (double)(nens - 1);
+  for (int j = 0; j < a.m; ++j) {
+    const double *sj = a.s + (size_t)j * nens, *dj = a.invd + (size_t)j * nens;
+    double c = 0.0;
+#pragma unroll
+    for (int k = 0; k < NE; ++k)
+      if (k < nens) c = fma(xp[k], sj[k], c);
+    c = c / nm1;
+    double l;
+    if (a.loc) {
+      l = a.loc[(size_t)ri * a.m + j];
+    } else {
+      const double ex = a.obs_xy[2 * j] - px, ey = a.obs_xy[2 * j + 1] - py;
+      l = gaspari_cohn(sqrt(ex * ex + ey * ey), a.radius);
+    }
+    const double cl = c * l;
+#pragma unroll
+    for (int k = 0; k < NE; ++k)
+      if (k < nens) dx[k] = fma(cl, dj[k], dx[k]);
+  }
+  double meana = 0.0;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    dx[k] = k < nens ? a.X[k][i] + dx[k] : 0.0;  // x_a
+    meana += dx[k];
+  }
+  meana /= nens;
+#pragma unroll
+  for (int k = 0; k < NE; ++k)
+    if (k < nens) a.Xa[k][i] = a.alpha == 1.0 ? dx[k] : meana + a.alpha * (dx[k] - meana);
+}
+template __global__ void k_enkf_localized<8>(LocArgs);
+template __global__ void k_enkf_localized<16>(LocArgs);
+template __global__ void k_enkf_localized<32>(LocArgs);
+template __global__ void k_enkf_localized<64>(LocArgs);
+
+// Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers:
+// as easy as 1, 2, 3", SC 2011): ten rounds of two 32x32 -> 64-bit
+// multiplies with the key bumped by the Weyl constants between rounds.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+    c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k.x, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k.y, (uint32_t)p0);
+    k.x += 0x9E3779B9u;
+    k.y += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// One standard normal per (CATHY row, member, seed): Philox of counter
+// (row, member, 0, 0) under the 64-bit seed, two 32-bit words to uniforms
+// u1 in (0, 1], u2 in [0, 1), then Box-Muller sqrt(-2 ln u1) cos(2 pi u2).
+__device__ __forceinline__ double jitter_normal(uint32_t row, uint32_t member, uint64_t seed) {
+  const uint4 r = philox4x32_10(make_uint4(row, member, 0u, 0u), make_uint2((uint32_t)seed, (uint32_t)(seed >> 32)));
+  const double u1 = ((double)r.x + 1.0) * 2.3283064365386963e-10;  // 2^-32
+  const double u2 = (double)r.y * 2.3283064365386963e-10;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// Particle-filter resampling (pf.py:98-109) with jitter (pf.py:129-130):
+// member j's row i becomes member idx[j]'s (src[j] = its state) plus
+// sigma * N(0,1) from jitter_normal(CATHY row, j, seed) when sigma > 0.
+// All members' rows are read before any is written (in place).
+template <int NE>
+__global__ __launch_bounds__(256) void k_pf_resample(int n, int nens, const double *const *__restrict__ src,
+                                                     const int *__restrict__ iperm, double sigma, uint64_t seed,
+                                                     double *const *__restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v[NE];
+#pragma unroll
+  for (int j = 0; j < NE; ++j) v[j] = j < nens ? src[j][i] : 0.0;
+#pragma unroll
+  for (int j = 0; j < NE; ++j)
+    if (j < nens) dst[j][i] = v[j];
+  if (sigma > 0.0) {  // the thread's own rows, just written
+    const uint32_t ri = (uint32_t)iperm[i];
+    for (int j = 0; j < nens; ++j) {
+      double *d = dst[j];
+      d[i] = d[i] + sigma * jitter_normal(ri, (uint32_t)j, seed);
+    }
+  }
+}
+template __global__ void k_pf_resample<8>(int, int, const double *const *, const int *, double, uint64_t,
+                                          double *const *);
+template __global__ void k_pf_resample<16>(int, int, const double *const *, const int *, double, uint64_t,
+                                           double *const *);
+template __global__ void k_pf_resample<32>(int, int, const double *const *, const int *, double, uint64_t,
+                                           double *const *);
+template __global__ void k_pf_resample<64>(int, int, const double *const *, const int *, double, uint64_t,
+                                           double *const *);
+
 // vel3d.f:28-52 per element: kr_e = mean nodal kr (nodelt.f), then
 //   UU = (sum psi_j b_j) (-kr_e VOLUR IVOL) Kx, VV likewise with c / Ky,
 //   WW = ((sum psi_j d_j) (-kr_e VOLUR IVOL) - kr_e) Kz.

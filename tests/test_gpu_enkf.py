@@ -19,14 +19,15 @@ from pycathy_wrapper_amd import enkf as EK
 pytestmark = pytest.mark.gpu
 
 
-def _members(n, steps):
-    """n members advanced `steps` steps of 10 s each; also the largest
-    |mass-balance error| of those steps, per member."""
+def _members(n, steps, ordering=None, extra=0):
+    """n members advanced `steps` steps of 10 s each (the next `extra` steps
+    are 10 s too); also the largest |mass-balance error| of those steps,
+    per member."""
     from pycathy_wrapper_amd import gpu as G
     fl, err = [], []
     for sd in range(n):
-        f = G.FlowGPU(CF.hillslope_small_member(sd), G.ORDER_COLUMN3)
-        f.set_dt_schedule([10.0] * steps)  # every member at the same time
+        f = G.FlowGPU(CF.hillslope_small_member(sd), G.ORDER_COLUMN3 if ordering is None else ordering)
+        f.set_dt_schedule([10.0] * (steps + extra))  # every member at the same time
         e = 0.0
         for _ in range(steps):
             rc, st = f.advance()
@@ -80,3 +81,153 @@ def test_enkf_update_rejects_members_that_do_not_align():
     finally:
         a.close()
         b.close()
+
+
+def _oracle_from(problem, psi0):
+    """The oracle's run of a member's problem started from psi0, in steps of
+    10 s (a run restarted from an analysis, as pyCATHY's DA restarts each
+    member)."""
+    import dataclasses
+    import pyoracle as P
+    pr = dataclasses.replace(problem, psi0=np.ascontiguousarray(psi0))
+    m = pr.mesh
+    prm = pr.params.as_dict()
+    prm.update(deltat=10.0, dtmax=10.0)
+    return P.Oracle().run_create(dict(tetra=m.tetra, x=m.x, y=m.y, z=m.z, arenod=m.arenod, nnod=m.nnod,
+                                      nstr=m.nstr, ntri=m.ntri), pr.soil, pr.psi0, pr.atm_times, pr.atm_vals,
+                                 pr.hspatm, P.OrcParams(**prm))
+
+
+@pytest.mark.parametrize("mode,ordering,sakov,inflation", [("gaspari-cohn", "column3", False, 1.0),
+                                                           ("gaspari-cohn", "natural", True, 1.15),
+                                                           ("dense", "column3", False, 1.1)])
+def test_localised_update_matches_oracle(mode, ordering, sakov, inflation):
+    """cathy_gpu_enkf_update_localized against the restatement of
+    enkf_analysis_localized_with_inflation with the Gaspari-Cohn matrix of
+    DA/localisation.py (or a dense random L), six members; the device forms
+    the weights per row from node and observation positions.  Same
+    tolerance as the plain analysis.  The forecast then continues as the
+    oracle's run restarted from the analysis state does: the first step of
+    each member against it, psi to 1e-6 relative and the mass balance
+    (boundary fluxes re-evaluated at the analysis state, as MBINIT does at
+    a restart) to 1e-8 of the member's storage."""
+    from pycathy_wrapper_amd import gpu as G
+    fl, err = _members(6, 8, G.ORDER_COLUMN3 if ordering == "column3" else G.ORDER_NATURAL, extra=2)
+    try:
+        X = np.stack([f.psi() for f in fl], axis=1)
+        mesh = fl[0].problem.mesh
+        nodes = np.arange(7, X.shape[0], 350)[:10]
+        HX = EK.observe(fl, nodes, "psi")
+        rng = np.random.default_rng(21)
+        y = HX.mean(axis=1) + 0.02 * rng.standard_normal(len(nodes))
+        R = np.eye(len(nodes)) * (0.5 ** 2 if sakov else 0.01 ** 2)
+        xy = np.stack([mesh.x, mesh.y], axis=1)
+        radius = 0.15 * float(np.ptp(mesh.x))
+        if mode == "dense":
+            L = rng.uniform(0.0, 1.0, (X.shape[0], len(nodes)))
+            Pa = EK.enkf_analysis_localized_with_inflation(y, R, fl, np.zeros((0, 6)), HX, L=L, sakov=sakov,
+                                                           inflate_states=inflation)
+        else:
+            L = EO.build_localization_matrix(xy[nodes], xy, radius)
+            Pa = EK.enkf_analysis_localized_with_inflation(y, R, fl, np.zeros((0, 6)), HX, obs_xy=xy[nodes],
+                                                           radius=radius, sakov=sakov, inflate_states=inflation)
+            assert 0.05 < np.mean(L > 0) < 0.95  # the localisation cuts some rows off
+        assert Pa.shape == (0, 6)
+        Xa = np.stack([f.psi() for f in fl], axis=1)
+        Xo, _ = EO.enkf_analysis_localized_with_inflation(y, R, X, np.zeros((0, 6)), HX, L=L, sakov=sakov,
+                                                          inflate_states=inflation)
+        assert np.max(np.abs(Xa - Xo)) <= 1e-11 * np.max(np.abs(X))
+        assert np.max(np.abs(Xa - X)) > 1e-6
+        if mode != "dense" and inflation == 1.0:
+            far = np.all(L == 0.0, axis=1)
+            assert far.any() and np.array_equal(Xa[far], X[far])  # rows beyond 2 radii keep their states
+        scale = 1e-6 * mesh.volnod.sum()
+        for k, f in enumerate(fl):
+            run = _oracle_from(f.problem, Xa[:, k])
+            try:
+                rc_o, so = run.step()
+                rc_g, sg = f.advance()
+                assert rc_g == rc_o == 0
+                assert sg["DELTAT"] == so["DELTAT"] and sg["ITER"] == so["ITER"], k
+                assert np.max(np.abs(f.psi() - run.psi())) <= 1e-6 * np.max(np.abs(run.psi()))
+                for key in ("DSTORE", "VIN", "VOUT", "ERRAS"):
+                    assert abs(sg[key] - so[key]) <= 1e-8 * max(abs(so[key]), scale), (k, key)
+            finally:
+                run.close()
+            assert f.advance()[0] == 0  # (the oracle's next step size follows its own step control)
+    finally:
+        for f in fl:
+            f.close()
+
+
+@pytest.mark.parametrize("ordering", ["column3", "natural"])
+def test_pf_resample_copies_members_and_adds_the_generators_jitter(ordering):
+    """cathy_gpu_pf_resample: member j takes member idx[j]'s state bit for
+    bit; with jitter, plus sigma times the normal the restated generator
+    (Philox4x32-10 + Box-Muller, pinned on its known answers) draws for
+    that CATHY node and member -- equal up to the last bits of log/cos."""
+    from pycathy_wrapper_amd import gpu as G
+    fl, _ = _members(6, 4, G.ORDER_COLUMN3 if ordering == "column3" else G.ORDER_NATURAL)
+    try:
+        X = np.stack([f.psi() for f in fl], axis=1)
+        idx = np.array([4, 4, 0, 2, 2, 5])
+        EK.pf_resample(fl, idx, 0.0)
+        X1 = np.stack([f.psi() for f in fl], axis=1)
+        assert np.array_equal(X1, X[:, idx])
+        sigma, seed = 0.05, 987654321012
+        EK.pf_resample(fl, np.arange(6), sigma, seed)
+        X2 = np.stack([f.psi() for f in fl], axis=1)
+        z = EO.jitter_normals(np.arange(X.shape[0]), 6, seed)
+        assert np.max(np.abs((X2 - X1) - sigma * z)) <= 1e-13
+        with pytest.raises(G.CathyGpuError, match="out of range"):
+            EK.pf_resample(fl, np.array([0, 1, 2, 3, 4, 6]))
+        for f in fl:
+            rc, _ = f.advance()
+            assert rc == 0
+    finally:
+        for f in fl:
+            f.close()
+
+
+@pytest.mark.parametrize("threshold", [0.0, 0.9], ids=["weighted", "resampled"])
+def test_particle_filter_analysis_on_resident_members(threshold):
+    """pf.py's particle_filter_analysis on device-resident members, with the
+    weighted EnKF update: against the restatement with the same draws (the
+    uniform u, the data perturbations, and the device generator's state
+    jitter).  "weighted": no resampling, so the update runs with unequal
+    weights; "resampled": N_eff below 0.9 N, resampled first."""
+    fl, err = _members(6, 6)
+    try:
+        X = np.stack([f.psi() for f in fl], axis=1)
+        nodes = np.arange(11, X.shape[0], 500)[:8]
+        HX = EK.observe(fl, nodes, "psi")
+        sd = float(np.std(HX, axis=1).mean())
+        y = HX.mean(axis=1) + 0.5 * sd
+        R = np.eye(len(nodes)) * sd ** 2
+        P = np.random.default_rng(3).standard_normal((2, 6))
+        rng = np.random.default_rng(5)
+        out = EK.particle_filter_analysis(y, R, P, fl, HX, use_enkf_update=True, resample_threshold=threshold,
+                                          jitter_std_param=0.01, jitter_std_state=0.002, alpha=1.1, seed=77,
+                                          u=0.4, rng=rng)
+        assert out["resampled"] == (threshold > 0)
+        assert out["weights"].max() < 0.9 or threshold > 0
+        rng2 = np.random.default_rng(5)
+        e = rng2.standard_normal((len(nodes), 6))
+        pn = rng2.standard_normal((2, 6))
+        zs = EO.jitter_normals(np.arange(X.shape[0]), 6, 77)
+        Xo, Po, w, ne, res, idx = EO.particle_filter_analysis(y, R, P, X, HX, u=0.4, state_noise=zs,
+                                                              param_noise=pn, data_noise=e, use_enkf_update=True,
+                                                              resample_threshold=threshold, jitter_std_param=0.01,
+                                                              jitter_std_state=0.002, alpha=1.1)
+        assert res == out["resampled"] and np.array_equal(out["indices"], idx)
+        assert np.allclose(out["weights"], w, rtol=1e-13)
+        Xa = np.stack([f.psi() for f in fl], axis=1)
+        assert np.max(np.abs(Xa - Xo)) <= 1e-11 * np.max(np.abs(X))
+        assert np.max(np.abs(Xa - X[:, idx] - 0.002 * zs)) > 1e-6   # the weighted update moved the states
+        assert np.allclose(out["Analysisparam"], Po, rtol=1e-12, atol=1e-12)
+        for f, er in zip(fl, err):
+            rc, st = f.advance()
+            assert rc == 0
+    finally:
+        for f in fl:
+            f.close()
