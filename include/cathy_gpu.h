@@ -258,6 +258,20 @@ int cathy_gpu_enkf_update_localized(const int *nens, int64_t *handles,
                                     const double *obs_xy,
                                     const double *radius,
                                     const double *inflation);
+/* Domain-localised analysis (pyCATHY DA/cathy_DA.py:671-803 with
+ * localisation 'zones' / 'veg_map' / 'nodes', DA/localisation.py:12-48):
+ * one analysis per zone, each over the state rows of its zone with the
+ * observations in it, all in one launch.  s and invd as for
+ * cathy_gpu_enkf_update_localized, each observation's rows formed from its
+ * own zone's observations (COV of that zone alone); row_zone: the zone of
+ * each of the N state rows (CATHY order), obs_zone: the zone of each of
+ * the m observations.  Row i moves by the observations j with obs_zone[j]
+ * == row_zone[i] only; rows of a zone without observations keep their
+ * states. */
+int cathy_gpu_enkf_update_zones(const int *nens, int64_t *handles,
+                                const int *nobs, const double *s,
+                                const double *invd, const int *row_zone,
+                                const int *obs_zone, const double *inflation);
 /* Particle-filter resampling of the member states in place (pyCATHY
  * DA/pf.py:98-109 ensemble[:, indices], then the state jitter of
  * :129-130).  idx: nens 0-based member indices (systematic_resample's

@@ -243,3 +243,30 @@ def jitter_normals(rows, nens, seed):
     u1 = (w[0].astype(np.float64) + 1.0) * 2.0 ** -32
     u2 = w[1].astype(np.float64) * 2.0 ** -32
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def zone_analysis(data, data_cov, ensemble, predict_obs, row_zone, obs_zone, sakov=False, inflate_states=1.0,
+                  skip_zone=-9999):
+    """Domain localisation as DA/cathy_DA.py:700-803 runs it: for every zone
+    of the mask but skip_zone (:701-702) that holds observations (:735-736),
+    the analysis of that zone's state rows with its observations alone
+    (:760-771, here enkf_analysis); rows of other zones keep the forecast
+    (:790-803)."""
+    X = np.asarray(ensemble, dtype=np.float64)
+    out = X.copy()
+    HX = np.asarray(predict_obs, dtype=np.float64)
+    y = np.asarray(data, dtype=np.float64).reshape(-1)
+    R = np.asarray(data_cov, dtype=np.float64)
+    rz = np.asarray(row_zone)
+    oz = np.asarray(obs_zone)
+    for z in np.unique(rz):
+        if z == skip_zone:
+            continue
+        sel = np.flatnonzero(oz == z)
+        if sel.size == 0:
+            continue
+        rows = np.flatnonzero(rz == z)
+        Xa, _ = enkf_analysis(y[sel], R[np.ix_(sel, sel)], np.zeros((0, X.shape[1])), X[rows], HX[sel], sakov=sakov,
+                              inflate_states=inflate_states)
+        out[rows] = Xa
+    return out

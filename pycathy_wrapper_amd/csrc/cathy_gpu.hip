@@ -307,57 +307,87 @@ int cathy_gpu_enkf_update_noalloc(const int *nens, int64_t *handles, const doubl
   });
 }
 
+}  // extern "C"
+
+namespace cg {
+// the localised analyses: one of loc (dense), row_zone + obs_zone (zones),
+// node_xy + obs_xy + radius (Gaspari-Cohn)
+static void enkf_localized_apply(int m, int64_t *handles, int mo, const double *s, const double *invd,
+                                 const double *loc, const double *node_xy, const double *obs_xy, double radius,
+                                 const int *row_zone, const int *obs_zone, double inflation) {
+  CG_REQUIRE(m >= 2 && m <= kEnkfMax, "ensemble size out of range (2..64)");
+  CG_REQUIRE(mo >= 1, "at least one observation");
+  CG_REQUIRE(std::isfinite(inflation), "non-finite inflation");
+  for (size_t j = 0; j < (size_t)mo * m; ++j)
+    CG_REQUIRE(std::isfinite(s[j]) && std::isfinite(invd[j]), "non-finite observation term");
+  Members fl = analysis_members(m, handles);
+  Flow *f0 = fl[0].get();
+  const int n = f0->n;
+  hipStream_t st = f0->st;
+  std::array<const double *, kEnkfMax> hx;
+  std::array<double *, kEnkfMax> hxa;
+  for (int j = 0; j < m; ++j) hx[j] = hxa[j] = fl[j]->pnew.p;
+  DBuf<const double *> dx(m);
+  DBuf<double *> dxa(m);
+  DBuf<double> ds((size_t)mo * m), dinv((size_t)mo * m), dloc, dnxy, doxy;
+  DBuf<int> dip(n), drz, doz;
+  dx.upload(hx.data(), m, st);
+  dxa.upload(hxa.data(), m, st);
+  ds.upload(s, (size_t)mo * m, st);
+  dinv.upload(invd, (size_t)mo * m, st);
+  dip.upload(f0->iperm.data(), n, st);
+  if (loc) {
+    dloc.alloc((size_t)n * mo);
+    dloc.upload(loc, (size_t)n * mo, st);
+  } else if (row_zone) {
+    drz.alloc(n);
+    doz.alloc(mo);
+    drz.upload(row_zone, n, st);
+    doz.upload(obs_zone, mo, st);
+  } else {
+    dnxy.alloc((size_t)2 * n);
+    doxy.alloc((size_t)2 * mo);
+    dnxy.upload(node_xy, (size_t)2 * n, st);
+    doxy.upload(obs_xy, (size_t)2 * mo, st);
+  }
+  const LocArgs a{n, m, mo, dx.p, dxa.p, ds.p, dinv.p, dip.p, dloc.p, dnxy.p, doxy.p, drz.p, doz.p,
+                  radius, inflation};
+  const dim3 g(grid_for(n)), b(kBlock);
+  switch (member_width(m)) {
+    case 8: hipLaunchKernelGGL(k_enkf_localized<8>, g, b, 0, st, a); break;
+    case 16: hipLaunchKernelGGL(k_enkf_localized<16>, g, b, 0, st, a); break;
+    case 32: hipLaunchKernelGGL(k_enkf_localized<32>, g, b, 0, st, a); break;
+    default: hipLaunchKernelGGL(k_enkf_localized<64>, g, b, 0, st, a); break;
+  }
+  CG_HIP(hipGetLastError());
+  CG_HIP(hipStreamSynchronize(st));
+  for (int j = 0; j < m; ++j) fl[j]->adopt_state();
+}
+}  // namespace cg
+
+extern "C" {
+
 int cathy_gpu_enkf_update_localized(const int *nens, int64_t *handles, const int *nobs, const double *s,
                                     const double *invd, const double *loc, const double *node_xy,
                                     const double *obs_xy, const double *radius, const double *inflation) {
   return guarded([&] {
     CG_REQUIRE(nens && handles && nobs && s && invd && inflation, "null argument");
-    const int m = *nens, mo = *nobs;
-    CG_REQUIRE(m >= 2 && m <= kEnkfMax, "ensemble size out of range (2..64)");
-    CG_REQUIRE(mo >= 1, "at least one observation");
     CG_REQUIRE(loc || (node_xy && obs_xy && radius),
                "give the localisation matrix, or node and observation positions with a radius");
     if (!loc) CG_REQUIRE(std::isfinite(*radius) && *radius > 0.0, "the localisation radius must be positive");
-    CG_REQUIRE(std::isfinite(*inflation), "non-finite inflation");
-    for (size_t j = 0; j < (size_t)mo * m; ++j)
-      CG_REQUIRE(std::isfinite(s[j]) && std::isfinite(invd[j]), "non-finite observation term");
-    Members fl = analysis_members(m, handles);
-    Flow *f0 = fl[0].get();
-    const int n = f0->n;
-    hipStream_t st = f0->st;
-    std::array<const double *, kEnkfMax> hx;
-    std::array<double *, kEnkfMax> hxa;
-    for (int j = 0; j < m; ++j) hx[j] = hxa[j] = fl[j]->pnew.p;
-    DBuf<const double *> dx(m);
-    DBuf<double *> dxa(m);
-    DBuf<double> ds((size_t)mo * m), dinv((size_t)mo * m), dloc, dnxy, doxy;
-    DBuf<int> dip(n);
-    dx.upload(hx.data(), m, st);
-    dxa.upload(hxa.data(), m, st);
-    ds.upload(s, (size_t)mo * m, st);
-    dinv.upload(invd, (size_t)mo * m, st);
-    dip.upload(f0->iperm.data(), n, st);
-    if (loc) {
-      dloc.alloc((size_t)n * mo);
-      dloc.upload(loc, (size_t)n * mo, st);
-    } else {
-      dnxy.alloc((size_t)2 * n);
-      doxy.alloc((size_t)2 * mo);
-      dnxy.upload(node_xy, (size_t)2 * n, st);
-      doxy.upload(obs_xy, (size_t)2 * mo, st);
-    }
-    const LocArgs a{n, m, mo, dx.p, dxa.p, ds.p, dinv.p, dip.p, dloc.p, dnxy.p, doxy.p,
-                    loc ? 0.0 : *radius, *inflation};
-    const dim3 g(grid_for(n)), b(kBlock);
-    switch (member_width(m)) {
-      case 8: hipLaunchKernelGGL(k_enkf_localized<8>, g, b, 0, st, a); break;
-      case 16: hipLaunchKernelGGL(k_enkf_localized<16>, g, b, 0, st, a); break;
-      case 32: hipLaunchKernelGGL(k_enkf_localized<32>, g, b, 0, st, a); break;
-      default: hipLaunchKernelGGL(k_enkf_localized<64>, g, b, 0, st, a); break;
-    }
-    CG_HIP(hipGetLastError());
-    CG_HIP(hipStreamSynchronize(st));
-    for (int j = 0; j < m; ++j) fl[j]->adopt_state();
+    enkf_localized_apply(*nens, handles, *nobs, s, invd, loc, node_xy, obs_xy, loc ? 0.0 : *radius, nullptr,
+                         nullptr, *inflation);
+    return 0;
+  });
+}
+
+int cathy_gpu_enkf_update_zones(const int *nens, int64_t *handles, const int *nobs, const double *s,
+                                const double *invd, const int *row_zone, const int *obs_zone,
+                                const double *inflation) {
+  return guarded([&] {
+    CG_REQUIRE(nens && handles && nobs && s && invd && row_zone && obs_zone && inflation, "null argument");
+    enkf_localized_apply(*nens, handles, *nobs, s, invd, nullptr, nullptr, nullptr, 0.0, row_zone, obs_zone,
+                         *inflation);
     return 0;
   });
 }

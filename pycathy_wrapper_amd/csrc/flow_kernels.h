@@ -440,6 +440,7 @@ struct LocArgs {
   const double *s, *invd;
   const int *iperm;
   const double *loc, *node_xy, *obs_xy;
+  const int *row_zone, *obs_zone;  // zone localisation: L(i,j) = [zone of row i == zone of observation j]
   double radius, alpha;
 };
 template <int NE>

@@ -1999,9 +1999,10 @@ __device__ __forceinline__ double gaspari_cohn(double d, double L) {
 // Localised analysis of the states (enkf.py:304-321), one thread per state
 // row (device order; ri its CATHY row):
 //   c_j = (sum_k x'_k S(j,k)) / (N-1),   x_a,k = x_k + sum_j c_j L(ri,j) invD(j,k),
-// S(j,k) at s[j*nens + k], invD likewise; L(ri,j) from loc[ri*m + j] or,
-// without loc, the Gaspari-Cohn weight of the (x, y) distance from node ri
-// to observation j.  Then inflation about the analysis mean.  Member
+// S(j,k) at s[j*nens + k], invD likewise; L(ri,j) from loc[ri*m + j], or
+// 1 where row ri and observation j lie in one zone and 0 elsewhere, or the
+// Gaspari-Cohn weight of the (x, y) distance from node ri to observation
+// j.  Then inflation about the analysis mean.  Member
 // values stay in registers (NE >= nens, the rest masked); every row of
 // every member is read before the thread writes, so the update is in place.
 template <int NE>
@@ -2024,11 +2025,13 @@ __global__ __launch_bounds__(256) void k_enkf_localized(LocArgs a) {
     dx[k] = 0.0;
   }
   double px = 0.0, py = 0.0;
-  if (!a.loc) {
+  const int zi = a.row_zone ? a.row_zone[ri] : 0;
+  if (!a.loc && !a.row_zone) {
     px = a.node_xy[2 * (size_t)ri];
     py = a.node_xy[2 * (size_t)ri + 1];
   }
   const double nm1 = (double)(nens - 1);
+  bool hit = false;  // zones: an observation in the row's zone
   for (int j = 0; j < a.m; ++j) {
     const double *sj = a.s + (size_t)j * nens, *dj = a.invd + (size_t)j * nens;
     double c = 0.0;
@@ -2039,6 +2042,10 @@ __global__ __launch_bounds__(256) void k_enkf_localized(LocArgs a) {
     double l;
     if (a.loc) {
       l = a.loc[(size_t)ri * a.m + j];
+    } else if (a.row_zone) {
+      const bool in = a.obs_zone[j] == zi;
+      hit = hit || in;
+      l = in ? 1.0 : 0.0;
     } else {
       const double ex = a.obs_xy[2 * j] - px, ey = a.obs_xy[2 * j + 1] - py;
       l = gaspari_cohn(sqrt(ex * ex + ey * ey), a.radius);
@@ -2048,6 +2055,7 @@ __global__ __launch_bounds__(256) void k_enkf_localized(LocArgs a) {
     for (int k = 0; k < NE; ++k)
       if (k < nens) dx[k] = fma(cl, dj[k], dx[k]);
   }
+  if (a.row_zone && !hit) return;  // a zone with no observation is not analysed
   double meana = 0.0;
 #pragma unroll
   for (int k = 0; k < NE; ++k) {

@@ -180,6 +180,47 @@ def enkf_analysis_localized_with_inflation(data, data_cov, flows, param, predict
     return Pa
 
 
+def zone_analysis(data, data_cov, flows, predict_obs, surface_zone, obs_nodes, param=None, sakov: bool = False,
+                  inflation: float = 1.0, skip_zone: int = -9999):
+    """pyCATHY's domain localisation (cathy_DA.py:671-803): one analysis per
+    zone of the surface map (veg_map, zones, or one zone per node), each
+    over the 3D nodes under that zone's surface nodes
+    (localisation.py:12-48) and the observations at them; zone
+    ``skip_zone`` is not analysed.  All zones' states move in one device
+    launch (cathy_gpu_enkf_update_zones).  Returns the analysis parameters
+    of each zone with observations, {zone: [n_par x N]}."""
+    N = len(flows)
+    HX = np.asarray(predict_obs, dtype=np.float64)
+    m = HX.shape[0]
+    y = np.asarray(data, dtype=np.float64).reshape(-1)
+    R = np.asarray(data_cov, dtype=np.float64)
+    nnod = flows[0].nnod
+    sz = np.asarray(surface_zone).reshape(-1)
+    if sz.size != nnod:
+        raise ValueError("surface_zone needs one zone per surface node")
+    row_zone = np.ascontiguousarray(sz[np.arange(flows[0].n) % nnod], dtype=np.int32)
+    oz = sz[np.asarray(obs_nodes, dtype=np.int64) % nnod].astype(np.int32)
+    none = np.int32(np.iinfo(np.int32).min)                    # matches no row
+    obs_zone = np.full(m, none, dtype=np.int32)
+    S = np.zeros((m, N))
+    invD = np.zeros((m, N))
+    P = np.asarray(param, dtype=np.float64).reshape(-1, N) if param is not None and np.size(param) else None
+    out = {}
+    for z in np.unique(oz):
+        if z == skip_zone:
+            continue
+        sel = np.flatnonzero(oz == z)
+        Sz, Dz = observation_terms(y[sel], R[np.ix_(sel, sel)], HX[sel], sakov)
+        S[sel], invD[sel], obs_zone[sel] = Sz, Dz, z
+        if P is not None:
+            out[int(z)] = P + ((P - P.mean(axis=1, keepdims=True)) @ Sz.T / (N - 1)) @ Dz
+    h = np.array([f.h.value for f in flows], dtype=np.int64)
+    S, invD = np.ascontiguousarray(S), np.ascontiguousarray(invD)
+    G._check(G.lib().cathy_gpu_enkf_update_zones(G._i(N), G._p(h), G._i(m), G._p(S), G._p(invD), G._p(row_zone),
+                                                  G._p(obs_zone), C.byref(C.c_double(inflation))), "enkf_update_zones")
+    return out
+
+
 # ------------------------------------------------------ particle filter --
 
 def particle_weights(data, data_cov, predict_obs):

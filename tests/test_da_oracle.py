@@ -275,3 +275,43 @@ def test_jitter_normals_are_standard_normal():
     assert abs(np.mean(z[:, 0] * z[:, 1])) < 0.005                    # members independent
     z2 = EO.jitter_normals(np.arange(1000), 4, seed=12346)
     assert not np.allclose(z2, z[:1000])
+
+
+# ------------------------------------------------------- domain localisation
+
+@pytest.mark.parametrize("sakov", [False, True])
+def test_zone_analysis_host_terms_and_zone_indicator(sakov):
+    """The product's single-launch form of the per-zone analyses: each
+    observation's (S, COV^-1 D) rows from its own zone's observations, and
+    L(i, j) = [zone of row i == zone of observation j], rows of zones
+    without observations untouched; evaluated here in numpy against the
+    restated per-zone loop.  A zone with one observation pins the
+    scalar-gain case."""
+    rng = np.random.default_rng(31)
+    n, N = 60, 8
+    X = rng.standard_normal((n, N)) - 1.0
+    row_zone = np.repeat([3, 5, -9999, 7, 9], 12)
+    obs = np.array([2, 7, 15, 20, 27, 38, 40, 44])              # zones 3, 3, 5, 5, -9999, 7, 7, 7
+    HX = X[obs] + 0.05 * rng.standard_normal((obs.size, N))
+    y = HX.mean(axis=1) + 0.3 * rng.standard_normal(obs.size)
+    R = np.diag(rng.uniform(0.05, 0.2, obs.size))
+    oz = row_zone[obs]
+    Xo = EO.zone_analysis(y, R, X, HX, row_zone, oz, sakov=sakov, inflate_states=1.2)
+    S = np.zeros((obs.size, N)); D = np.zeros((obs.size, N)); ozd = np.full(obs.size, -2 ** 31)
+    for z in np.unique(oz):
+        if z == -9999:
+            continue
+        sel = np.flatnonzero(oz == z)
+        S[sel], D[sel] = EK.observation_terms(y[sel], R[np.ix_(sel, sel)], HX[sel], sakov)
+        ozd[sel] = z
+    L = (row_zone[:, None] == ozd[None, :]).astype(float)
+    Xp = X - X.mean(axis=1, keepdims=True)
+    Xa = X + ((Xp @ S.T / (N - 1)) * L) @ D
+    mu = Xa.mean(axis=1, keepdims=True)
+    Xa = mu + 1.2 * (Xa - mu)
+    hit = L.any(axis=1)
+    Xa[~hit] = X[~hit]
+    assert np.allclose(Xa, Xo, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(Xo[row_zone == -9999], X[row_zone == -9999])
+    assert np.array_equal(Xo[row_zone == 9], X[row_zone == 9])      # no observation in zone 9
+    assert not np.allclose(Xo[row_zone == 3], X[row_zone == 3])

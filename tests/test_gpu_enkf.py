@@ -231,3 +231,39 @@ def test_particle_filter_analysis_on_resident_members(threshold):
     finally:
         for f in fl:
             f.close()
+
+
+@pytest.mark.parametrize("ordering", ["column3", "natural"])
+def test_zone_analysis_matches_oracle(ordering):
+    """cathy_gpu_enkf_update_zones: pyCATHY's domain localisation (one
+    analysis per zone of a surface map, zone -9999 left out), all zones in
+    one launch, against the restated per-zone loop of cathy_DA.py; rows of
+    the left-out zone and of a zone without observations keep their states
+    bit for bit."""
+    from pycathy_wrapper_amd import gpu as G
+    fl, _ = _members(6, 6, G.ORDER_COLUMN3 if ordering == "column3" else G.ORDER_NATURAL)
+    try:
+        X = np.stack([f.psi() for f in fl], axis=1)
+        mesh = fl[0].problem.mesh
+        nnod = mesh.nnod
+        xs = mesh.x[:nnod]
+        q = np.quantile(xs, [0.25, 0.5, 0.75])
+        surf_zone = np.where(xs < q[0], 1, np.where(xs < q[1], 2, np.where(xs < q[2], -9999, 4))).astype(np.int32)
+        obs = np.array([s for s in range(5, X.shape[0], 97) if surf_zone[s % nnod] != 4][:12])
+        HX = EK.observe(fl, obs, "psi")
+        rng = np.random.default_rng(8)
+        y = HX.mean(axis=1) + 0.02 * rng.standard_normal(obs.size)
+        R = np.eye(obs.size) * 0.02 ** 2
+        EK.zone_analysis(y, R, fl, HX, surf_zone, obs, inflation=1.05)
+        Xa = np.stack([f.psi() for f in fl], axis=1)
+        row_zone = surf_zone[np.arange(X.shape[0]) % nnod]
+        Xo = EO.zone_analysis(y, R, X, HX, row_zone, row_zone[obs], inflate_states=1.05)
+        assert np.max(np.abs(Xa - Xo)) <= 1e-11 * np.max(np.abs(X))
+        for z in (-9999, 4):
+            assert np.array_equal(Xa[row_zone == z], X[row_zone == z])
+        assert np.max(np.abs(Xa - X)) > 1e-6
+        for f in fl:
+            assert f.advance()[0] == 0
+    finally:
+        for f in fl:
+            f.close()
