@@ -36,6 +36,28 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::cg::Error(-1, msg);                               \
   } while (0)
 
+// Pinned host buffer (hipHostMalloc): device-to-host copies into it are
+// plain DMA, with no staging through a bounce buffer and no host copy.
+template <typename T>
+struct Pinned {
+  T *p = nullptr;
+  size_t n = 0;
+  Pinned() = default;
+  Pinned(const Pinned &) = delete;
+  Pinned &operator=(const Pinned &) = delete;
+  void alloc(size_t count) {
+    release();
+    CG_HIP(hipHostMalloc((void **)&p, sizeof(T) * std::max<size_t>(count, 1)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~Pinned() { release(); }
+};
+
 // Owning device buffer.
 template <typename T>
 struct DBuf {

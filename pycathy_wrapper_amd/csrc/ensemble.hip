@@ -46,24 +46,6 @@ int shape_groups(const Flow &f, int sh) {
   }
 }
 
-// pinned host staging (the per-round tables go up with one copy each)
-template <typename T>
-struct Pinned {
-  T *p = nullptr;
-  size_t n = 0;
-  void alloc(size_t count) {
-    release();
-    CG_HIP(hipHostMalloc((void **)&p, sizeof(T) * std::max<size_t>(count, 1)));
-    n = count;
-  }
-  void release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  ~Pinned() { release(); }
-};
-
 struct Ensemble {
   std::vector<std::shared_ptr<Flow>> mem;
   int nm = 0;

@@ -225,6 +225,16 @@ struct Flow {
   DBuf<int> flags;  // [0] PCG done, [1] IC(0) breakdown count
   DBuf<double> res;
   std::vector<double> h_res;
+  // what the host polls, in pinned memory: the solver's done flag and
+  // scalars, MASBAL's reduced values, single sums
+  struct HostBox {
+    int done;
+    PcgScal pcg;
+    BicgScal bicg;
+    double res[24];
+    double v;
+  };
+  Pinned<HostBox> hbox;
 
   // host time / balance state (cathy_main.f)
   Params prm;
@@ -1059,6 +1069,7 @@ struct Flow {
     flags.alloc(2);
     res.alloc(24);
     h_res.assign(24, 0.0);
+    if (!hbox.p) hbox.alloc(1);
     CG_HIP(hipMemsetAsync(flags.p, 0, sizeof(int) * 2, st));
   }
 
@@ -1253,9 +1264,11 @@ struct Flow {
         hipLaunchKernelGGL(k_pcg_beta, dim3(Gr), dim3(kBlock), 0, st, n, Gr, P(4), P(3), z.p, p.p, scal.p, done);
       }
       launched += batch;
-      CG_HIP(hipMemcpyAsync(&h_done, done, sizeof(int), hipMemcpyDeviceToHost, st));
-      CG_HIP(hipMemcpyAsync(&hs, scal.p, sizeof(PcgScal), hipMemcpyDeviceToHost, st));
+      CG_HIP(hipMemcpyAsync(&hbox.p->done, done, sizeof(int), hipMemcpyDeviceToHost, st));
+      CG_HIP(hipMemcpyAsync(&hbox.p->pcg, scal.p, sizeof(PcgScal), hipMemcpyDeviceToHost, st));
       CG_HIP(hipStreamSynchronize(st));
+      h_done = hbox.p->done;
+      hs = hbox.p->pcg;
       if (prof) {
         // launches after convergence exit at once; count only the live ones
         const int executed = h_done ? hs.niter : hs.niter - 1;  // bodies run so far
@@ -1386,9 +1399,11 @@ struct Flow {
         hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(kBlock), 0, st, n, Gr, P(18), P(1), bscal.p, done);
       }
       launched += batch;
-      CG_HIP(hipMemcpyAsync(&h_done, done, sizeof(int), hipMemcpyDeviceToHost, st));
-      CG_HIP(hipMemcpyAsync(&hs, bscal.p, sizeof(BicgScal), hipMemcpyDeviceToHost, st));
+      CG_HIP(hipMemcpyAsync(&hbox.p->done, done, sizeof(int), hipMemcpyDeviceToHost, st));
+      CG_HIP(hipMemcpyAsync(&hbox.p->bicg, bscal.p, sizeof(BicgScal), hipMemcpyDeviceToHost, st));
       CG_HIP(hipStreamSynchronize(st));
+      h_done = hbox.p->done;
+      hs = hbox.p->bicg;
       if (h_done) break;
       CG_REQUIRE(launched <= imax + 3, "BiCGSTAB did not stop");
       batch = std::max(2, std::min(8, imax + 1 - hs.niter + 1));
@@ -1491,9 +1506,10 @@ struct Flow {
   // MASBAL (masbal.f:37-107) + NORMS (norms.f); result in h_res.
   void masbal_norms() {
     masbal_launch();
-    CG_HIP(hipMemcpyAsync(h_res.data(), res.p, sizeof(double) * (nsf_nodes > 0 ? 24 : 16), hipMemcpyDeviceToHost,
-                          st));
+    const int nr = nsf_nodes > 0 ? 24 : 16;
+    CG_HIP(hipMemcpyAsync(hbox.p->res, res.p, sizeof(double) * nr, hipMemcpyDeviceToHost, st));
     CG_HIP(hipStreamSynchronize(st));
+    std::copy(hbox.p->res, hbox.p->res + nr, h_res.begin());
     masbal_host(h_res.data());
   }
   void masbal_launch() {
@@ -2102,10 +2118,9 @@ struct Flow {
     const int G = red_grid(n);
     hipLaunchKernelGGL(k_store, dim3(G), dim3(kBlock), 0, st, n, sw.p, volnod.p, pnodi.p, P(17));
     hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, st, G, P(17), res.p + 15);
-    double v = 0;
-    CG_HIP(hipMemcpyAsync(&v, res.p + 15, sizeof(double), hipMemcpyDeviceToHost, st));
+    CG_HIP(hipMemcpyAsync(&hbox.p->v, res.p + 15, sizeof(double), hipMemcpyDeviceToHost, st));
     CG_HIP(hipStreamSynchronize(st));
-    return v;
+    return hbox.p->v;
   }
 
   // -------------------------------------------------- run initialisation --
