@@ -14,8 +14,7 @@ Also restated: that localised analysis itself (enkf.py:225-343) with the
 Gaspari-Cohn localisation matrix of DA/localisation.py:136-188, and the
 particle filter of DA/pf.py (weights, systematic resampling, the weighted
 EnKF update, additive jitter), with every random draw of the reference
-passed in as an argument.  pf.py's multiplicative log-K jitter
-(use_log_K, :136-144) is not restated.
+passed in as an argument.
 """
 from __future__ import annotations
 
@@ -181,13 +180,15 @@ def weighted_enkf_update(data, data_cov, param, ensemble, observation, weights, 
 
 def particle_filter_analysis(data, data_cov, param, ensemble, observation, u, state_noise=None, param_noise=None,
                              data_noise=None, use_enkf_update=False, resample_threshold=0.5, jitter_std_param=0.01,
-                             jitter_std_state=0.005, alpha=1.0):
+                             jitter_std_state=0.005, alpha=1.0, use_log_K=False):
     """DA/pf.py:3-195 with its random draws given: weights and N_eff
     (:64-81), systematic resampling of states, parameters and predicted
     observations when N_eff < threshold * N (:98-109; u is the uniform
     draw), the weighted EnKF update (:115-119, gain 1 as that call passes
     it), additive jitter of states and parameters (:125-147;
-    state_noise / param_noise are the standard normal draws).  Returns
+    state_noise / param_noise are the standard normal draws; with use_log_K
+    the parameters' second half, K, takes the jitter multiplicatively in
+    log space, :136-144).  Returns
     (states, params, weights, n_eff, resampled, indices)."""
     X = np.array(ensemble, dtype=np.float64)
     N = X.shape[1]
@@ -209,7 +210,13 @@ def particle_filter_analysis(data, data_cov, param, ensemble, observation, u, st
         if jitter_std_state > 0:
             X = X + state_noise * jitter_std_state
         if jitter_std_param > 0:
-            P = P + param_noise * jitter_std_param
+            if use_log_K:  # :136-144: additive on the first half (porosity), multiplicative on K
+                nc = P.shape[0] // 2
+                P = P.copy()
+                P[:nc] = P[:nc] + param_noise[:nc] * jitter_std_param
+                P[nc:] = P[nc:] * np.exp(param_noise[nc:] * jitter_std_param)
+            else:
+                P = P + param_noise * jitter_std_param
     return X, P, w, n_eff, resampled, idx
 
 

@@ -281,7 +281,7 @@ def weighted_update_weights(data, data_cov, predict_obs, weights, data_noise, al
 def particle_filter_analysis(data, data_cov, param, flows, predict_obs, use_enkf_update: bool = False,
                              resample_threshold: float = 0.5, jitter_std_param: float = 0.01,
                              jitter_std_state: float = 0.005, alpha: float = 1.0, seed: int = 0, u=None,
-                             rng=None) -> dict:
+                             rng=None, use_log_K: bool = False) -> dict:
     """pyCATHY's particle_filter_analysis (pf.py:3-195) with the states
     resident in the members: weights and N_eff on the host; when N_eff <
     threshold * N, systematic resampling of the states on the device
@@ -289,7 +289,8 @@ def particle_filter_analysis(data, data_cov, param, flows, predict_obs, use_enkf
     the optional weighted EnKF update (enkf_update with
     weighted_update_weights, gain 1 as pf.py passes it); state jitter on
     the device (Philox draws under ``seed``), parameter jitter on the host
-    (``rng`` normal draws, np.random by default).  Returns the reference's
+    (``rng`` normal draws, np.random by default; ``use_log_K``: the second
+    half of the parameters, K, jittered multiplicatively).  Returns the reference's
     dict without 'Analysis' (the states are the members')."""
     rng = rng or np.random
     N = len(flows)
@@ -318,6 +319,13 @@ def particle_filter_analysis(data, data_cov, param, flows, predict_obs, use_enkf
     elif resampled or sigma > 0:
         pf_resample(flows, idx, sigma, seed)
     if jitter and jitter_std_param > 0:
-        P = P + rng.standard_normal(P.shape) * jitter_std_param
+        z = rng.standard_normal(P.shape)
+        if use_log_K:  # pf.py:136-144: porosity (first half) additive, K (second half) multiplicative
+            nc = P.shape[0] // 2
+            P = P.copy()
+            P[:nc] = P[:nc] + z[:nc] * jitter_std_param
+            P[nc:] = P[nc:] * np.exp(z[nc:] * jitter_std_param)
+        else:
+            P = P + z * jitter_std_param
     return {"Analysisparam": P, "weights": w, "n_eff": n_eff, "resampled": resampled, "observation": HX,
             "indices": idx}

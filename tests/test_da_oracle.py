@@ -315,3 +315,21 @@ def test_zone_analysis_host_terms_and_zone_indicator(sakov):
     assert np.array_equal(Xo[row_zone == -9999], X[row_zone == -9999])
     assert np.array_equal(Xo[row_zone == 9], X[row_zone == 9])      # no observation in zone 9
     assert not np.allclose(Xo[row_zone == 3], X[row_zone == 3])
+
+
+def test_particle_filter_log_k_jitter():
+    """use_log_K (pf.py:136-144): porosity (first half) moves additively, K
+    (second half) multiplicatively, so K stays positive and its log moves
+    by the draw times the deviation."""
+    rng = np.random.default_rng(13)
+    N = 6
+    X = rng.standard_normal((10, N))
+    P = np.vstack([rng.uniform(0.3, 0.5, (2, N)), 10.0 ** rng.uniform(-6, -4, (2, N))])
+    HX = X[[1, 4]]
+    y = HX.mean(axis=1)
+    pn = rng.standard_normal(P.shape)
+    _, Pj, *_ = EO.particle_filter_analysis(y, np.eye(2) * 1e6, P, X, HX, u=0.5, state_noise=np.zeros_like(X),
+                                            param_noise=pn, jitter_std_param=0.05, jitter_std_state=0.0,
+                                            use_log_K=True)
+    assert np.allclose(Pj[:2] - P[:2], 0.05 * pn[:2], rtol=0, atol=1e-15)
+    assert np.all(Pj[2:] > 0) and np.allclose(np.log(Pj[2:] / P[2:]), 0.05 * pn[2:], rtol=1e-12, atol=1e-14)
