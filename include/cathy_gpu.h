@@ -272,6 +272,13 @@ int cathy_gpu_enkf_update_zones(const int *nens, int64_t *handles,
                                 const int *nobs, const double *s,
                                 const double *invd, const int *row_zone,
                                 const int *obs_zone, const double *inflation);
+/* Gram matrix of the member state perturbations, gram[i*nens + j] =
+ * sum over the N state rows of (x_i - xbar)(x_j - xbar), xbar the row's
+ * member mean: X'^T X', the N_ens x N_ens matrix through which the dual
+ * analysis's second step (pyCATHY DA/enkf.py:345-361, the states as
+ * observations of themselves) reduces to member-space algebra.  Fixed
+ * summation order (deterministic); states are not changed. */
+int cathy_gpu_enkf_gram(const int *nens, int64_t *handles, double *gram);
 /* Particle-filter resampling of the member states in place (pyCATHY
  * DA/pf.py:98-109 ensemble[:, indices], then the state jitter of
  * :129-130).  idx: nens 0-based member indices (systematic_resample's

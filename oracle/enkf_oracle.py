@@ -277,3 +277,42 @@ def zone_analysis(data, data_cov, ensemble, predict_obs, row_zone, obs_zone, sak
                               inflate_states=inflate_states)
         out[rows] = Xa
     return out
+
+
+def _analysis_2d(data, data_cov, param, ensemble, predict_obs, sakov=False):
+    """enkf_analysis (enkf.py:16-224) with per-member data allowed, as its
+    innovation (data.T - predict_obs.T).T forms it (:122-125); returns
+    (COV, analysis states, analysis params)."""
+    X = np.asarray(ensemble, dtype=np.float64)
+    N = X.shape[1]
+    n = X.shape[0]
+    P = np.asarray(param, dtype=np.float64).reshape(-1, N) if np.size(param) else np.zeros((0, N))
+    HX = np.asarray(predict_obs, dtype=np.float64)
+    y = np.asarray(data, dtype=np.float64)
+    R = np.asarray(data_cov, dtype=np.float64)
+    A = np.vstack([X, P])
+    Ap = A - A.mean(axis=1, keepdims=True)
+    D = (y.T - HX.T).T
+    S = HX - HX.mean(axis=1, keepdims=True)
+    if sakov:
+        COV = R
+        invD = D / np.diag(R)[:, None]
+    else:
+        COV = (S @ S.T) / (N - 1) + R
+        invD = np.linalg.solve(COV, D)
+    Aa = A + ((Ap @ S.T) / (N - 1)) @ invD
+    return COV, Aa[:n], Aa[n:]
+
+
+def enkf_dual_analysis(data, data_cov, param, ensemble, predict_obs, sakov=False):
+    """DA/enkf.py:345-361 literally: the analysis of the states; then a
+    second analysis of the forecast with the first analysis's states as
+    per-member data, the forecast states as their own predicted
+    observations, and R = mean(diag(COV of the first step)) I (N_state x
+    N_state).  Returns the second analysis (states, params).  Its
+    covariance is N_state x N_state, so only small meshes run here."""
+    COV1, Xa1, _ = _analysis_2d(np.asarray(data).reshape(-1), data_cov, param, ensemble, predict_obs, sakov)
+    X = np.asarray(ensemble, dtype=np.float64)
+    R2 = np.diag([np.mean(np.diag(COV1))] * len(Xa1))
+    _, Xa2, Pa2 = _analysis_2d(Xa1, R2, param, X, X, sakov)
+    return Xa2, Pa2

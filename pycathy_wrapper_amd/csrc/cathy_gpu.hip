@@ -392,6 +392,38 @@ int cathy_gpu_enkf_update_zones(const int *nens, int64_t *handles, const int *no
   });
 }
 
+int cathy_gpu_enkf_gram(const int *nens, int64_t *handles, double *gram) {
+  return guarded([&] {
+    CG_REQUIRE(nens && handles && gram, "null argument");
+    const int m = *nens;
+    CG_REQUIRE(m >= 2 && m <= kEnkfMax, "ensemble size out of range (2..64)");
+    Members fl = analysis_members(m, handles);
+    Flow *f0 = fl[0].get();
+    const int n = f0->n, ne = member_width(m);
+    hipStream_t st = f0->st;
+    std::array<const double *, kEnkfMax> hx;
+    for (int j = 0; j < m; ++j) hx[j] = fl[j]->pnew.p;
+    const int nb = std::max(1, std::min(kGramGroups, (n + kGramRows - 1) / kGramRows));
+    DBuf<const double *> dx(m);
+    DBuf<double> part((size_t)nb * ne * ne), g((size_t)ne * ne);
+    dx.upload(hx.data(), m, st);
+    switch (ne) {
+      case 8: hipLaunchKernelGGL(k_enkf_gram<8>, dim3(nb), dim3(256), 0, st, n, m, dx.p, part.p); break;
+      case 16: hipLaunchKernelGGL(k_enkf_gram<16>, dim3(nb), dim3(256), 0, st, n, m, dx.p, part.p); break;
+      case 32: hipLaunchKernelGGL(k_enkf_gram<32>, dim3(nb), dim3(256), 0, st, n, m, dx.p, part.p); break;
+      default: hipLaunchKernelGGL(k_enkf_gram<64>, dim3(nb), dim3(256), 0, st, n, m, dx.p, part.p); break;
+    }
+    hipLaunchKernelGGL(k_enkf_gram_sum, dim3((ne * ne + 255) / 256), dim3(256), 0, st, nb, ne * ne, part.p, g.p);
+    CG_HIP(hipGetLastError());
+    std::vector<double> h((size_t)ne * ne);
+    g.download(h.data(), h.size(), st);
+    CG_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) gram[(size_t)i * m + j] = h[(size_t)i * ne + j];
+    return 0;
+  });
+}
+
 int cathy_gpu_pf_resample(const int *nens, int64_t *handles, const int *idx, const double *jitter_std,
                           const int64_t *seed) {
   return guarded([&] {

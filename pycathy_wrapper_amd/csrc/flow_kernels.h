@@ -445,6 +445,11 @@ struct LocArgs {
 };
 template <int NE>
 __global__ void k_enkf_localized(LocArgs a);
+constexpr int kGramRows = 32;    // rows staged per work-group round
+constexpr int kGramGroups = 512;  // work-groups (one partial Gram matrix each)
+template <int NE>
+__global__ void k_enkf_gram(int n, int nens, const double *const *X, double *part);
+__global__ void k_enkf_gram_sum(int nb, int nn, const double *part, double *g);
 template <int NE>
 __global__ void k_pf_resample(int n, int nens, const double *const *src, const int *iperm, double sigma,
                               uint64_t seed, double *const *dst);

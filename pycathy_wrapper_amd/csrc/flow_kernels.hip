@@ -2072,6 +2072,66 @@ template __global__ void k_enkf_localized<16>(LocArgs);
 template __global__ void k_enkf_localized<32>(LocArgs);
 template __global__ void k_enkf_localized<64>(LocArgs);
 
+// Gram matrix of the member perturbations, G(i,j) = sum_rows x'_i x'_j
+// (x' = x - row mean), for the dual analysis.  Each work-group stages
+// kGramRows rows of all members in LDS with their row means, then thread t
+// accumulates the pairs p = t, t + 256, ... of the NE x NE matrix over its
+// rows; one partial matrix per work-group (part[b * NE * NE + p]), summed
+// in work-group order by k_enkf_gram_sum.
+template <int NE>
+__global__ __launch_bounds__(256) void k_enkf_gram(int n, int nens, const double *const *__restrict__ X,
+                                                   double *__restrict__ part) {
+  constexpr int R = kGramRows;
+  constexpr int NP = NE * NE / 256 > 0 ? NE * NE / 256 : 1;
+  __shared__ double xs[R * NE];
+  __shared__ double mu[R];
+  double acc[NP];
+#pragma unroll
+  for (int u = 0; u < NP; ++u) acc[u] = 0.0;
+  for (int r0 = blockIdx.x * R; r0 < n; r0 += gridDim.x * R) {
+    for (int e = threadIdx.x; e < R * NE; e += blockDim.x) {
+      const int r = e / NE, k = e % NE, row = r0 + r;
+      xs[e] = (row < n && k < nens) ? X[k][row] : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x < R) {
+      double m = 0.0;
+      for (int k = 0; k < nens; ++k) m += xs[threadIdx.x * NE + k];
+      mu[threadIdx.x] = m / nens;
+    }
+    __syncthreads();
+    for (int r = 0; r < R; ++r) {
+      if (r0 + r >= n) break;
+      const double m = mu[r];
+#pragma unroll
+      for (int u = 0; u < NP; ++u) {
+        const int p = threadIdx.x + 256 * u;
+        const int i = p / NE, j = p % NE;
+        if (p < NE * NE && i < nens && j < nens)
+          acc[u] = fma(xs[r * NE + i] - m, xs[r * NE + j] - m, acc[u]);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < NP; ++u) {
+    const int p = threadIdx.x + 256 * u;
+    if (p < NE * NE) part[(size_t)blockIdx.x * NE * NE + p] = acc[u];
+  }
+}
+template __global__ void k_enkf_gram<8>(int, int, const double *const *, double *);
+template __global__ void k_enkf_gram<16>(int, int, const double *const *, double *);
+template __global__ void k_enkf_gram<32>(int, int, const double *const *, double *);
+template __global__ void k_enkf_gram<64>(int, int, const double *const *, double *);
+
+__global__ void k_enkf_gram_sum(int nb, int nn, const double *__restrict__ part, double *__restrict__ g) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nn) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[(size_t)b * nn + p];
+  g[p] = s;
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers:
 // as easy as 1, 2, 3", SC 2011): ten rounds of two 32x32 -> 64-bit
 // multiplies with the key bumped by the Weyl constants between rounds.

@@ -329,3 +329,49 @@ def particle_filter_analysis(data, data_cov, param, flows, predict_obs, use_enkf
             P = P + z * jitter_std_param
     return {"Analysisparam": P, "weights": w, "n_eff": n_eff, "resampled": resampled, "observation": HX,
             "indices": idx}
+
+
+# -------------------------------------------------------- dual analysis --
+
+def state_gram(flows) -> np.ndarray:
+    """G = X'^T X' over the members' resident states (cathy_gpu_enkf_gram)."""
+    N = len(flows)
+    h = np.array([f.h.value for f in flows], dtype=np.int64)
+    g = np.zeros((N, N))
+    G._check(G.lib().cathy_gpu_enkf_gram(G._i(N), G._p(h), G._p(g)), "enkf_gram")
+    return g
+
+
+def dual_weights(data, data_cov, predict_obs, gram, sakov: bool = False) -> np.ndarray:
+    """W2 (N x N) of pyCATHY's enkf_dual_analysis (enkf.py:345-361) from the
+    observations and the states' Gram matrix G = X'^T X'.  The first step is
+    X + X' W1 (W1 = S^T COV^-1 D/(N-1)); the second takes the first's states
+    as per-member data of the forecast states themselves, with R2 = c I,
+    c = mean(diag(COV1)), so S2 = X' and D2 = X' W1.  By the push-through
+    identity S2^T (S2 S2^T/(N-1) + c I)^-1 = (G/(N-1) + c I)^-1 S2^T:
+        W2 = (G/(N-1) + c I)^-1 G W1 / (N-1)     (Sakov: G W1 / (c (N-1))),
+    N x N algebra instead of the reference's N_state x N_state covariance.
+    The dual's result is X + X' W2, parameters P + P' W2."""
+    S, invD = observation_terms(data, data_cov, predict_obs, sakov)
+    m, N = S.shape
+    R = np.asarray(data_cov, dtype=np.float64)
+    cov1 = R if sakov else S @ S.T / (N - 1) + R
+    c = float(np.mean(np.diag(cov1)))
+    W1 = S.T @ invD / (N - 1)
+    Gm = np.asarray(gram, dtype=np.float64)
+    if sakov:
+        return Gm @ W1 / (c * (N - 1))
+    return np.linalg.solve(Gm / (N - 1) + c * np.eye(N), Gm @ W1) / (N - 1)
+
+
+def enkf_dual_analysis(data, data_cov, param, flows, predict_obs, sakov: bool = False):
+    """pyCATHY's enkf_dual_analysis on resident members: the states' Gram
+    matrix on the device, W2 on the host, X + X' W2 on the device in place.
+    Returns the analysis parameters [n_par x N]."""
+    N = len(flows)
+    W2 = dual_weights(data, data_cov, predict_obs, state_gram(flows), sakov)
+    P = np.asarray(param, dtype=np.float64).reshape(-1, N) if param is not None and np.size(param) else \
+        np.zeros((0, N))
+    Pa = P + (P - P.mean(axis=1, keepdims=True)) @ W2
+    enkf_update(flows, W2)
+    return Pa

@@ -333,3 +333,29 @@ def test_particle_filter_log_k_jitter():
                                             use_log_K=True)
     assert np.allclose(Pj[:2] - P[:2], 0.05 * pn[:2], rtol=0, atol=1e-15)
     assert np.all(Pj[2:] > 0) and np.allclose(np.log(Pj[2:] / P[2:]), 0.05 * pn[2:], rtol=1e-12, atol=1e-14)
+
+
+# ------------------------------------------------------------ dual analysis
+
+@pytest.mark.parametrize("sakov", [False, True])
+def test_dual_analysis_member_space_form(sakov):
+    """The product's N x N form of enkf_dual_analysis (Gram matrix and the
+    push-through identity) against the literal two-step restatement with
+    its N_state x N_state covariance."""
+    X, P = _ens(21, n=45, N=9, npar=2)
+    rng = np.random.default_rng(4)
+    obs = [3, 12, 30, 41]
+    HX = X[obs] + 0.05 * rng.standard_normal((4, 9))
+    y = HX.mean(axis=1) + 0.2
+    R = np.diag(rng.uniform(0.05, 0.3, 4))
+    Xo, Po = EO.enkf_dual_analysis(y, R, P, X, HX, sakov=sakov)
+    Xp = X - X.mean(axis=1, keepdims=True)
+    W2 = EK.dual_weights(y, R, HX, Xp.T @ Xp, sakov)
+    assert np.allclose(X + Xp @ W2, Xo, rtol=1e-10, atol=1e-10)
+    assert np.allclose(P + (P - P.mean(axis=1, keepdims=True)) @ W2, Po, rtol=1e-10, atol=1e-10)
+    # the 2D-data step is enkf_analysis for one data vector
+    _, Xs, Ps = EO._analysis_2d(y, R, P, X, HX, sakov)
+    Xa1, Pa1 = EO.enkf_analysis(y, R, P, X, HX, sakov=sakov)
+    assert np.allclose(Xs, Xa1, rtol=1e-13, atol=1e-13) and np.allclose(Ps, Pa1, rtol=1e-13, atol=1e-13)
+    # the first step alone differs: the second step is a real update
+    assert not np.allclose(Xa1, Xo, rtol=1e-6, atol=1e-6)

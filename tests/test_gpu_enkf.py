@@ -267,3 +267,35 @@ def test_zone_analysis_matches_oracle(ordering):
     finally:
         for f in fl:
             f.close()
+
+
+@pytest.mark.parametrize("sakov", [False, True])
+def test_dual_analysis_matches_oracle(sakov):
+    """cathy_gpu_enkf_gram (X'^T X' over the resident states) against numpy,
+    and pyCATHY's enkf_dual_analysis on the device (Gram matrix, N x N
+    weights, the member update) against its literal restatement, whose
+    second step solves the 4,851 x 4,851 state covariance."""
+    fl, _ = _members(6, 6)
+    try:
+        X = np.stack([f.psi() for f in fl], axis=1)
+        Xp = X - X.mean(axis=1, keepdims=True)
+        g = EK.state_gram(fl)
+        assert np.allclose(g, Xp.T @ Xp, rtol=1e-12, atol=1e-12 * np.abs(Xp.T @ Xp).max())
+        assert np.array_equal(np.stack([f.psi() for f in fl], axis=1), X)  # states untouched
+        nodes = np.arange(9, X.shape[0], 450)[:9]
+        HX = EK.observe(fl, nodes, "psi")
+        rng = np.random.default_rng(14)
+        y = HX.mean(axis=1) + 0.02 * rng.standard_normal(nodes.size)
+        R = np.eye(nodes.size) * (0.5 ** 2 if sakov else 0.02 ** 2)
+        P = rng.standard_normal((2, 6))
+        Pa = EK.enkf_dual_analysis(y, R, P, fl, HX, sakov=sakov)
+        Xa = np.stack([f.psi() for f in fl], axis=1)
+        Xo, Po = EO.enkf_dual_analysis(y, R, P, X, HX, sakov=sakov)
+        assert np.max(np.abs(Xa - Xo)) <= 1e-10 * np.max(np.abs(X))
+        assert np.allclose(Pa, Po, rtol=1e-10, atol=1e-10)
+        assert np.max(np.abs(Xa - X)) > 1e-6
+        for f in fl:
+            assert f.advance()[0] == 0
+    finally:
+        for f in fl:
+            f.close()
