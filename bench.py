@@ -381,6 +381,58 @@ def _pmc_rows(paths: list[str]):
                 yield from csv.DictReader(fh)
 
 
+def trace_summary(paths: list[str], skip: float = 1.0 / 3.0) -> int:
+    """Where the GPU's time goes in a rocprofv3 --kernel-trace CSV: after the
+    first `skip` of the dispatches (set-up), every instant is shared equally
+    among the kernels running then (so concurrent streams are not counted
+    twice); idle time is split by the kernel pair around each gap."""
+    import collections
+    import csv
+    import re
+
+    def short(name):
+        name = re.sub(r"^void ", "", name).split("(")[0]
+        return re.sub(r"cg::", "", name)
+
+    ev = []
+    for p in paths:
+        with open(p, newline="") as fh:
+            for r in csv.DictReader(fh):
+                ev.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), short(r["Kernel_Name"])))
+    ev.sort()
+    ev = ev[int(len(ev) * skip):]
+    pts = sorted([(s, 1, i) for i, (s, _, _) in enumerate(ev)] + [(e, -1, i) for i, (_, e, _) in enumerate(ev)])
+    active, share, conc, gaps = set(), collections.Counter(), collections.Counter(), collections.Counter()
+    last, last_end_kernel, idle = pts[0][0], None, 0
+    for t, typ, i in pts:
+        dt = t - last
+        if dt > 0:
+            if active:
+                for j in active:
+                    share[ev[j][2]] += dt / len(active)
+                conc[len(active)] += dt
+            else:
+                idle += dt
+                gaps[(last_end_kernel, ev[i][2] if typ == 1 else None)] += dt
+        last = t
+        if typ == 1:
+            active.add(i)
+        else:
+            active.discard(i)
+            last_end_kernel = ev[i][2]
+    tot = last - pts[0][0]
+    print(f"# window {tot / 1e6:.1f} ms after the first {skip:.0%} of {len(ev) / (1 - skip):.0f} dispatches; "
+          f"GPU idle {idle / 1e6:.1f} ms ({idle / tot:.1%}); time with k kernels running: " +
+          ", ".join(f"{k}: {v / tot:.1%}" for k, v in sorted(conc.items())))
+    print(f"{'kernel':60s} {'shared_ms':>10s} {'share':>7s}")
+    for k, v in share.most_common(25):
+        print(f"{k[:60]:60s} {v / 1e6:10.2f} {v / tot:7.1%}")
+    print(f"# idle gaps by the kernels around them (ended -> started)")
+    for (a, b), v in gaps.most_common(8):
+        print(f"{str(a)[:40]:40s} -> {str(b)[:40]:40s} {v / 1e6:8.2f} ms")
+    return 0
+
+
 def pmc_summary(paths: list[str], kernel: str, copy_bytes: float, all_kernels: bool = False) -> int:
     """Average per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of ``kernel`` and of
     the calibration copy kernel from rocprofv3 --pmc CSV output, and the HBM
@@ -713,11 +765,18 @@ def main() -> int:
                          "legs (0 skips it)")
     ap.add_argument("--cold-reps", type=int, default=50,
                     help="SpMV launches timed after a 1 GB cache flush each (0: skip)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the ensemble leg's timed steps (for profiling; prints no JSON line)")
+    ap.add_argument("--trace-summary", nargs="+", metavar="CSV",
+                    help="summarise rocprofv3 --kernel-trace CSV output (GPU time shared among concurrent "
+                         "kernels, idle gaps) and exit")
     ap.add_argument("--pmc-all", action="store_true",
                     help="with --pmc-summary: per-launch HBM bytes of every kernel, not only the SpMV")
     args = ap.parse_args()
     if args.pmc_summary:
         return pmc_summary(args.pmc_summary, "k_spmv_dot2", args.calib_mb * 1e6, args.pmc_all)
+    if args.trace_summary:
+        return trace_summary(args.trace_summary)
     if args.gpus != 1:
         print(f"bench.py: one GPU only; --gpus {args.gpus} is not supported (members of an ensemble shard "
               f"across GPUs without communication; no multi-GPU run is part of this build)", file=sys.stderr)
@@ -766,6 +825,10 @@ def main() -> int:
         others.append((pi, gi))
         log(f"[setup] member {i} (seed {2 + i}) built, warmed up and reset in {time.perf_counter() - t0:.1f}s")
 
+    if args.ensemble_only:  # for profiling the ensemble: its timed leg alone, no JSON line
+        ens = grouped_run([(pr, g)] + others, args.steps, args.group_rows)
+        log(f"[ensemble-only] {sum(r['steps'] for r in ens['members'])} member-steps in {ens['wall']:.3f}s")
+        return 0
     g.profile(True)
     dts = []
     steps = picard = pcg = backs = 0
