@@ -42,6 +42,8 @@ extern "C" {
 /* preconditioner orderings (cathy_gpu_flow3d_create, *ordering) */
 #define CATHY_GPU_ORDER_NATURAL 0 /* the reference's IC(0): exact INCLDP order */
 #define CATHY_GPU_ORDER_COLUMN3 1 /* IC(0) in 3-colour column order (fast) */
+#define CATHY_GPU_ORDER_COLUMN3C 2 /* the same classes, each soil column's rows contiguous
+                                    (chain-major; same results to rounding) */
 
 /* Human-readable text of the last error on this thread. */
 const char *cathy_gpu_last_error(void);

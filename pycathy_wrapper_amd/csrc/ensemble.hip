@@ -137,7 +137,7 @@ struct Ensemble {
       const Flow &f = *mem[j];
       const std::string who = "member " + std::to_string(j) + ": ";
       CG_REQUIRE(f.initialised, who + "flow3d_init not called");
-      CG_REQUIRE(f.ordering == CATHY_GPU_ORDER_COLUMN3, who + "the ensemble needs the column ordering");
+      CG_REQUIRE(f.column_order(), who + "the ensemble needs the column ordering");
       CG_REQUIRE(f.prm.iopt == 1, who + "the ensemble runs the Picard linearisation (IOPT = 1)");
       CG_REQUIRE(!f.lists_on() && !f.veg_on && !f.surf,
                  who + "listed boundary series, root uptake and the coupled surface switching run per member "

@@ -131,6 +131,9 @@ struct BcSeries {
 // ------------------------------------------------------------------ Flow --
 struct Flow {
   int n = 0, nt = 0, ntri = 0, nnod = 0, nstr = 0, nnz = 0, ordering = 0, lump = 1;
+  // the 3-colour column ordering, level-major or chain-major rows
+  bool column_order() const { return ordering == CATHY_GPU_ORDER_COLUMN3 || ordering == CATHY_GPU_ORDER_COLUMN3C; }
+  bool chain_major() const { return ordering == CATHY_GPU_ORDER_COLUMN3C; }
   bool f32 = true;  // IC(0) factor precision
   // PCG SpMV reads the lower triangle through the transpose (column
   // ordering with valid chain tables): one member alone runs faster, since
@@ -310,6 +313,7 @@ struct Flow {
     s.width = cwidth;
     s.nplane = nstr + 1;
     s.ncolr = cncolr;
+    s.clay = chain_major() ? 1 : 0;
     for (int c = 0; c < 4; ++c) {
       s.cbase[c] = ccbase[c];
       s.cnc[c] = ccnc[c];
@@ -327,7 +331,7 @@ struct Flow {
   int ccbase[4] = {0, 0, 0, 0}, ccnc[4] = {1, 1, 1, 1}, ccgoff[4] = {0, 0, 0, 0};
   void build_codes() {
     coded = 0;
-    if (ordering != CATHY_GPU_ORDER_COLUMN3) return;
+    if (!column_order()) return;
     const int ncl = (int)sched.cols.size();
     if (ncl < 1 || ncl > 4 || nnod >= (1 << 27) || maxrow > 255) return;
     for (int c = 0; c < ncl; ++c) {
@@ -344,8 +348,13 @@ struct Flow {
       for (int i = 1; i < ncl; ++i)
         if (r >= ccbase[i]) c = i;
       const int off = r - ccbase[c];
-      k = off / ccnc[c];
-      t = off - k * ccnc[c];
+      if (chain_major()) {
+        t = off / np;
+        k = off - t * np;
+      } else {
+        k = off / ccnc[c];
+        t = off - k * ccnc[c];
+      }
     };
     for (int r = 0; r < n; ++r) {
       int c, k, t;
@@ -419,10 +428,11 @@ struct Flow {
     CG_REQUIRE((long)nnod * (nstr + 1) == n, "N must equal NNOD*(NSTR+1) (gen3d.f numbering)");
     CG_REQUIRE((long)3 * ntri * nstr == nt, "NT must equal 3*NTRI*NSTR (gen3d.f)");
     CG_REQUIRE(nzone >= 1 && maxstr >= nstr, "NZONE/MAXSTR");
-    CG_REQUIRE(ordering == CATHY_GPU_ORDER_NATURAL || ordering == CATHY_GPU_ORDER_COLUMN3, "ORDERING");
+    CG_REQUIRE(ordering == CATHY_GPU_ORDER_NATURAL || ordering == CATHY_GPU_ORDER_COLUMN3 ||
+                   ordering == CATHY_GPU_ORDER_COLUMN3C, "ORDERING");
     CG_REQUIRE((long)n <= (long)kMaxRedBlocks * kBlock, "N above the per-launch row limit");
-    CG_REQUIRE(ordering != CATHY_GPU_ORDER_COLUMN3 || (nstr + 1) <= 64, "column ordering needs NSTR < 64");
-    f32 = ordering == CATHY_GPU_ORDER_COLUMN3;
+    CG_REQUIRE(!column_order() || (nstr + 1) <= 64, "column ordering needs NSTR < 64");
+    f32 = column_order();
     CG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     CG_HIP(hipEventCreate(&ev0));
     CG_HIP(hipEventCreate(&ev1));
@@ -621,17 +631,26 @@ struct Flow {
     for (int s = 0; s < nnod; ++s) rank[s] = cnt[c[s]]++;
     std::vector<long> base(ncolr + 1, 0);
     for (int k = 0; k < ncolr; ++k) base[k + 1] = base[k] + (long)cnt[k] * (nstr + 1);
+    // level-major: a colour's rows plane by plane, its chains side by
+    // side; chain-major: each chain's nstr+1 rows together
+    const bool cm = chain_major();
     for (int s = 0; s < nnod; ++s)
       for (int k = 0; k <= nstr; ++k) {
         const long ref = (long)k * nnod + s;
-        const long dev = base[c[s]] + (long)k * cnt[c[s]] + rank[s];
+        const long dev = cm ? base[c[s]] + (long)rank[s] * (nstr + 1) + k
+                            : base[c[s]] + (long)k * cnt[c[s]] + rank[s];
         perm[ref] = (int)dev;
         iperm[dev] = (int)ref;
       }
     int g0 = 0;
     for (int k = 0; k < ncolr; ++k)
       if (cnt[k]) {
-        sched.cols.push_back(ColDesc{(int)base[k], cnt[k], nstr + 1, g0});
+        ColDesc C{(int)base[k], cnt[k], nstr + 1, g0, 0, 0, 1, cnt[k]};
+        if (cm) {
+          C.st = nstr + 1;
+          C.sk = 1;
+        }
+        sched.cols.push_back(C);
         g0 += cnt[k];
       }
   }
@@ -642,7 +661,7 @@ struct Flow {
   void build_factor_tables(const std::vector<int> &hrp, const std::vector<int> &hcol, const std::vector<int> &hsd,
                            const std::vector<int> &htpos) {
     ftab_ok = false;
-    if (ordering != CATHY_GPU_ORDER_COLUMN3 || sched.cols.empty()) return;
+    if (!column_order() || sched.cols.empty()) return;
     int nch = 0;
     for (const ColDesc &C : sched.cols) nch = std::max(nch, C.g0 + C.nc);
     constexpr int NB = kFacWords * 4;
@@ -651,8 +670,8 @@ struct Flow {
       const int *f = std::lower_bound(b, e, col);
       return (f != e && *f == col) ? (int)(f - b) : -1;
     };
-    // the bytes row r (plane k of a colour with nc chains) needs
-    auto row_bytes = [&](int r, int k, int nc, uint8_t *out) -> bool {
+    // the bytes row r (plane k of a colour whose planes lie sk rows apart) needs
+    auto row_bytes = [&](int r, int k, int sk, uint8_t *out) -> bool {
       std::fill(out, out + NB, (uint8_t)0xFF);
       const int jd = hsd[r], jp = k > 0 ? jd - 1 : jd;
       if (jd > 16) return false;
@@ -664,7 +683,7 @@ struct Flow {
         }
       }
       if (k > 0) {
-        const int pred = r - nc;
+        const int pred = r - sk;
         if (hcol[hrp[r] + jd - 1] != pred) return false;
         for (int a = 0; a < jp; ++a) {
           const int b = pos_in(pred, hcol[hrp[r] + a]);
@@ -683,14 +702,14 @@ struct Flow {
           if (fac_class(k, C.len) == cls) krep = k;
         if (krep < 0) continue;
         for (int t = 0; t < C.nc; ++t)
-          if (!row_bytes(C.base + t + krep * C.nc, krep, C.nc, &tb[((size_t)cls * nch + C.g0 + t) * NB])) return;
+          if (!row_bytes(col_row(C, t, krep), krep, C.sk, &tb[((size_t)cls * nch + C.g0 + t) * NB])) return;
       }
     }
     for (const ColDesc &C : sched.cols)
       for (int k = 0; k < C.len; ++k) {
         const int cls = fac_class(k, C.len);
         for (int t = 0; t < C.nc; ++t) {
-          if (!row_bytes(C.base + t + k * C.nc, k, C.nc, tmp.data())) return;
+          if (!row_bytes(col_row(C, t, k), k, C.sk, tmp.data())) return;
           if (std::memcmp(tmp.data(), &tb[((size_t)cls * nch + C.g0 + t) * NB], NB) != 0) return;
         }
       }
@@ -781,7 +800,7 @@ struct Flow {
       C.nf = C.nb = 0;
       for (int k = 0; k < C.len; ++k)
         for (int t = 0; t < C.nc; ++t) {
-          const int r = C.base + t + k * C.nc;
+          const int r = col_row(C, t, k);
           C.nf = std::max(C.nf, hsd[r] - (k > 0 ? 1 : 0));
           C.nb = std::max(C.nb, hlen[r] - hsd[r] - 1 - (k < C.len - 1 ? 1 : 0));
         }
@@ -841,18 +860,18 @@ struct Flow {
     // every level row must have its same-column neighbours where the column
     // kernels expect them: predecessor = last lower slot, successor = first
     // upper slot
-    if (ordering == CATHY_GPU_ORDER_COLUMN3)
+    if (column_order())
       for (const ColDesc &C : sched.cols)
         for (int k = 0; k < C.len; ++k)
           for (int t = 0; t < C.nc; ++t) {
-            const int r = C.base + t + k * C.nc;
-            if (k > 0) CG_REQUIRE(hsd[r] >= 1 && hsc[spos(r, hsd[r] - 1)] == r - C.nc, "column predecessor slot");
+            const int r = col_row(C, t, k);
+            if (k > 0) CG_REQUIRE(hsd[r] >= 1 && hsc[spos(r, hsd[r] - 1)] == r - C.sk, "column predecessor slot");
             if (k + 1 < C.len)
-              CG_REQUIRE(hsd[r] + 1 < hlen[r] && hsc[spos(r, hsd[r] + 1)] == r + C.nc, "column successor slot");
+              CG_REQUIRE(hsd[r] + 1 < hlen[r] && hsc[spos(r, hsd[r] + 1)] == r + C.sk, "column successor slot");
             for (int j = 0; j < hlen[r]; ++j) {  // no coupling inside a colour class but the chain
               const int c = hsc[spos(r, j)];
               if (c >= C.base && c < C.base + C.nc * C.len && c != r)
-                CG_REQUIRE(c == r - C.nc || c == r + C.nc, "colour class not independent");
+                CG_REQUIRE(c == r - C.sk || c == r + C.sk, "colour class not independent");
             }
           }
     h_rowptr = std::move(hrp);
@@ -1076,7 +1095,7 @@ struct Flow {
   // IC(0) factor precision of the column ordering (FP32 by default; the
   // natural ordering keeps the reference's FP64).  Reallocates the factor.
   void set_factor_fp64(bool on) {
-    CG_REQUIRE(on || ordering == CATHY_GPU_ORDER_COLUMN3, "the natural ordering keeps the FP64 factor");
+    CG_REQUIRE(on || column_order(), "the natural ordering keeps the FP64 factor");
     if (on == !f32) return;
     CG_HIP(hipStreamSynchronize(st));
     if (on) {
@@ -1131,7 +1150,7 @@ struct Flow {
   void precond_apply_t(const FT *F, const double *tin, double *out, const int *done) {
     // z = (L L^T)^-1 t   (proddp.f)
     const Sell sl = sell();
-    if (ordering == CATHY_GPU_ORDER_COLUMN3) {
+    if (column_order()) {
       // the last colour couples to no later colour: its two sweeps share
       // one launch
       const size_t nc = sched.cols.size();
@@ -1161,7 +1180,7 @@ struct Flow {
   template <typename FT>
   void factor_t(FT *F) {
     const Sell sl = sell();
-    if (ordering == CATHY_GPU_ORDER_COLUMN3) {
+    if (column_order()) {
       const FacTab ft{ftab.p, ftab_nch};
       for (const ColDesc &C : sched.cols) {
         const dim3 g(grid_for(C.nc, kChains)), b(256);
@@ -1306,7 +1325,7 @@ struct Flow {
   template <typename FT>
   void ilu_fwd_t(const FT *F, const double *tin, double *out, const int *done) {
     const Sell sl = sell();
-    if (ordering == CATHY_GPU_ORDER_COLUMN3) {
+    if (column_order()) {
       for (const ColDesc &C : sched.cols) launch_fwd_col3<FT>(C, sl, F, dinv.p, tin, out, done);
     } else {
       for (const ListLevel &L : sched.lv)
@@ -1317,7 +1336,7 @@ struct Flow {
   template <typename FT>
   void ilu_bwd_t(const FT *F, const double *xin, double *out, const int *done) {
     const Sell sl = sell();
-    if (ordering == CATHY_GPU_ORDER_COLUMN3) {
+    if (column_order()) {
       for (size_t c = sched.cols.size(); c-- > 0;) launch_bwd_col3<FT>(sched.cols[c], sl, F, ones.p, xin, out, done);
     } else {
       for (size_t l = sched.lv.size(); l-- > 0;) {
@@ -1338,7 +1357,7 @@ struct Flow {
   template <typename FT>
   void ilu_factor_t(FT *F) {
     const Sell sl = sell();
-    if (ordering == CATHY_GPU_ORDER_COLUMN3) {
+    if (column_order()) {
       for (const ColDesc &C : sched.cols)
         hipLaunchKernelGGL(k_ilu_factor_col3<FT>, dim3(grid_for(C.nc, kChains)), dim3(256), ilu_smem(C), st, C,
                            sl, pairs(), stpos.p, A.p, isdir.p, F, dinv.p, flags.p + 1);

@@ -22,6 +22,7 @@ constexpr int kUnroll = 16;     // row slots handled by the unrolled gather loop
 
 // Column ids come from one of two sources.  Explicit: col[slot].  Coded
 // (extruded meshes in the column ordering): row r = base_c + k nc_c + t
+// (level-major) or base_c + t nplane + k (chain-major, clay = 1)
 // (colour c, node plane k, rank t; chain g = goff_c + t) and every slot's
 // column is (c', k + dk, t'), where the code (c', dk, t') depends only on
 // the chain g and the plane type pt (0 surface, 1 interior, 2 base):
@@ -38,6 +39,7 @@ struct Sell {
   const uint32_t *code;
   const uint8_t *clen, *cdiag;
   int coded, nchain, width, nplane, ncolr;
+  int clay;  // row layout of the colour classes: 0 level-major, 1 chain-major
   int cbase[4], cnc[4], cgoff[4];
 };
 
@@ -63,8 +65,9 @@ __device__ __forceinline__ RowPos row_pos(const Sell &sl, int r) {
   const int b = pick4(c, sl.cbase[0], sl.cbase[1], sl.cbase[2], sl.cbase[3]);
   const int nc = pick4(c, sl.cnc[0], sl.cnc[1], sl.cnc[2], sl.cnc[3]);
   const int off = r - b;
-  p.k = off / nc;
-  p.g = pick4(c, sl.cgoff[0], sl.cgoff[1], sl.cgoff[2], sl.cgoff[3]) + (off - p.k * nc);
+  const int tq = sl.clay ? off / sl.nplane : 0;
+  p.k = sl.clay ? off - tq * sl.nplane : off / nc;
+  p.g = pick4(c, sl.cgoff[0], sl.cgoff[1], sl.cgoff[2], sl.cgoff[3]) + (sl.clay ? tq : off - p.k * nc);
   p.pt = p.k == 0 ? 0 : (p.k == sl.nplane - 1 ? 2 : 1);
   return p;
 }
@@ -94,7 +97,7 @@ __device__ __forceinline__ int code_col(const Sell &sl, int k, uint32_t w) {
   nc = c2 == 2 ? sl.cnc[2] : nc;
   b = c2 == 3 ? sl.cbase[3] : b;
   nc = c2 == 3 ? sl.cnc[3] : nc;
-  return b + (k + dk) * nc + t2;
+  return sl.clay ? b + t2 * sl.nplane + (k + dk) : b + (k + dk) * nc + t2;
 }
 
 // column of slot j (SELL position q) of the row at p / r
@@ -129,11 +132,34 @@ __device__ __forceinline__ int sell_diag(const Sell &sl, const RowPos &p, int r)
 }
 
 // One colour class of the column ordering: chains t < nc, chain t = rows
-// base + t + k*nc for k < len (k = node plane, 0 = surface).
+// base + t*st + k*sk for k < len (k = node plane, 0 = surface): level-major
+// (st = 1, sk = nc) or chain-major (st = len, sk = 1).
 struct ColDesc {
   int base, nc, len, g0;  // g0: chain id of t = 0
   int nf, nb;             // most earlier- / later-colour slots of a row (gathered by the sweeps)
+  int st, sk;             // row strides between chains and between planes
 };
+__device__ __host__ __forceinline__ int col_row(const ColDesc &C, int t, int k) { return C.base + t * C.st + k * C.sk; }
+// A column work-group holds kChains chains, one row per thread (or a loop
+// over them): element i of its LDS arrays is chain t0 + t_local, plane k.
+// Level-major classes put the chains side by side (i = k*kChains +
+// t_local, consecutive threads on consecutive chains, as the rows lie);
+// chain-major ones one chain after the other (i = t_local*len + k).
+__device__ __forceinline__ void chain_elem(const ColDesc &C, int t0, int i, int &k, int &t) {
+  if (C.sk == 1) {
+    const int tl = i / C.len;
+    k = i - tl * C.len;
+    t = t0 + tl;
+  } else {
+    k = i / kChains;
+    t = t0 + (i - k * kChains);
+  }
+}
+// LDS index of (local chain tl, plane k) and the step between planes
+__device__ __forceinline__ int chain_lds(const ColDesc &C, int tl, int k) {
+  return C.sk == 1 ? tl * C.len + k : k * kChains + tl;
+}
+__device__ __forceinline__ int chain_step(const ColDesc &C) { return C.sk == 1 ? 1 : kChains; }
 
 // One level of the natural-order schedule: rows lvl_rows[off .. off+cnt).
 struct ListLevel {

@@ -384,11 +384,12 @@ __device__ __forceinline__ void factor_chains(const ColDesc &C, int t0, double *
                                               const double *__restrict__ A, FT *__restrict__ F,
                                               double *__restrict__ dinv, int *nfall) {
   const int tot = C.len * kChains;
+  const int ls = chain_step(C);
   if (threadIdx.x < kChains && t0 + threadIdx.x < C.nc) {
     double e = 1.0;
     bool pdir = true;
     for (int k = 0; k < C.len; ++k) {
-      const int i = k * kChains + threadIdx.x;
+      const int i = chain_lds(C, threadIdx.x, k);
       const int pd = Pd[i];
       if (pd < 0) {
         pdir = true;
@@ -408,9 +409,10 @@ __device__ __forceinline__ void factor_chains(const ColDesc &C, int t0, double *
   }
   __syncthreads();
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int k = i / kChains, t = t0 + (i % kChains);
+    int k, t;
+    chain_elem(C, t0, i, k, t);
     if (t >= C.nc) continue;
-    const int r = C.base + t + k * C.nc;
+    const int r = col_row(C, t, k);
     const int pd = Pd[i];
     if (pd < 0) {
       if (k > 0) {
@@ -425,7 +427,7 @@ __device__ __forceinline__ void factor_chains(const ColDesc &C, int t0, double *
     F[pd] = FT(d);
     dinv[r] = 1.0 / d;
     if (k > 0) {
-      const double l = Pd[i - kChains] >= 0 ? Qv[i] / sqrt(Sv[i - kChains]) : 0.0;
+      const double l = Pd[i - ls] >= 0 ? Qv[i] / sqrt(Sv[i - ls]) : 0.0;
       F[Pl[i]] = FT(l);
       F[Pt[i]] = FT(l);
     }
@@ -454,9 +456,10 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, Pair
   int *Pt = Pl + tot;                // its transpose slot
   const int t0 = xcd_group() * kChains;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int k = i / kChains, t = t0 + (i % kChains);
+    int k, t;
+    chain_elem(C, t0, i, k, t);
     if (t >= C.nc) continue;
-    const int r = C.base + t + k * C.nc;
+    const int r = col_row(C, t, k);
     const RowPos rp = chain_pos(sl, C.g0 + t, k);
     const int jd = sell_diag(sl, rp, r);
     const int jp = k > 0 ? jd - 1 : jd;
@@ -488,7 +491,8 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3(ColDesc C, Sell sl, Pair
   }
   __syncthreads();
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int k = i / kChains, t = t0 + (i % kChains);
+    int k, t;
+    chain_elem(C, t0, i, k, t);
     if (t >= C.nc || k == 0) continue;
     if (Pd[i] < 0) continue;
     const int q = Pl[i];
@@ -522,13 +526,15 @@ __device__ __forceinline__ void ic_factor_col3t_body(Blk bk, double *smem, ColDe
   int *Pl = Pd + tot;                // slot of the predecessor entry
   int *Pt = Pl + tot;                // its transpose slot
   const int t0 = bk.b * kChains;
-  const int i = threadIdx.x, k = i / kChains, t = t0 + (i % kChains);
-  const bool live = k < C.len && t < C.nc;
+  const int i = threadIdx.x;
+  int k, t;
+  chain_elem(C, t0, i, k, t);
+  const bool live = i < tot && t < C.nc;
   double Lr[NWc];
   uint32_t wpre[4];
   int r = 0, base = 0, jd = 0, jp = 0;
   if (live) {
-    r = C.base + t + k * C.nc;
+    r = col_row(C, t, k);
     const int g = C.g0 + t;
     const RowPos rp = chain_pos(sl, g, k);
     jd = sell_diag(sl, rp, r);
@@ -604,7 +610,7 @@ __device__ __forceinline__ void ic_factor_col3t_body(Blk bk, double *smem, ColDe
       Pd[i] = base + 64 * jd;
     }
     if (k > 0) {
-      const int pred = r - C.nc;
+      const int pred = r - C.sk;
       Pl[i] = base + 64 * (jd - 1);
       Pt[i] = sl.so[pred >> 6] + (pred & 63) + 64 * tab_byte(wt, jd - 1);
     } else {
@@ -616,7 +622,7 @@ __device__ __forceinline__ void ic_factor_col3t_body(Blk bk, double *smem, ColDe
   // the predecessor's numerator: its own earlier-colour entries are final
   // after the barrier
   if (live && k > 0 && Pd[i] >= 0) {
-    const int pred = r - C.nc;
+    const int pred = r - C.sk;
     const int sp0 = sl.so[pred >> 6] + (pred & 63);
     FT G2[NWc];
 #pragma unroll
@@ -644,20 +650,21 @@ __global__ __launch_bounds__(256) void k_ic_factor_col3t(ColDesc C, Sell sl, Fac
 
 // The chain recurrence v_k = b_k - a_k v_(k-1) (DOWN: k ascending) or
 // v_k = b_k - a_k v_(k+1) (up), v = 0 outside the chain, for all chains of
-// a work-group at once: element i = k*kChains + c of the LDS arrays b, a,
-// one element per thread (len*kChains <= blockDim.x).  A Hillis-Steele
+// a work-group at once: element i of the LDS arrays b, a is (chain, plane
+// k), one element per thread (len*kChains <= blockDim.x; ON false for
+// threads past them), planes LS elements apart.  A Hillis-Steele
 // scan of the affine maps: after the round of stride s, (b_k, a_k) is the
 // composition of the maps k .. k-2s+1, so after ceil(log2 len) rounds
 // b_k = v_k.  Composition: b_k - a_k (b_j - a_j v) = (b_k - a_k b_j) -
 // (-a_k a_j) v.
 template <bool DOWN>
-__device__ __forceinline__ void chain_scan(double *b, double *a, int len) {
-  const int i = threadIdx.x, k = i / kChains;
+__device__ __forceinline__ void chain_scan(double *b, double *a, int len, bool on, int k, int ls) {
+  const int i = threadIdx.x;
   for (int s = 1; s < len; s <<= 1) {
-    const bool act = k < len && (DOWN ? k >= s : k + s < len);
+    const bool act = on && (DOWN ? k >= s : k + s < len);
     double nb = 0.0, na = 0.0;
     if (act) {
-      const int j = DOWN ? i - s * kChains : i + s * kChains;
+      const int j = DOWN ? i - s * ls : i + s * ls;
       const double ai = a[i];
       nb = fma(-ai, b[j], b[i]);
       na = -ai * a[j];
@@ -688,9 +695,10 @@ __device__ __forceinline__ void fwd_col3_body(Blk bk, double *smem, ColDesc C, S
   const int t0 = bk.b * kChains;
   const int kl = C.len - 1;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int k = i / kChains, t = t0 + (i % kChains);
+    int k, t;
+    chain_elem(C, t0, i, k, t);
     if (t >= C.nc) continue;
-    const int r = C.base + t + k * C.nc;
+    const int r = col_row(C, t, k);
     const RowPos rp = chain_pos(sl, C.g0 + t, k);
     const int jd = sell_diag(sl, rp, r);
     const int jp = k > 0 ? jd - 1 : jd;
@@ -731,15 +739,18 @@ __device__ __forceinline__ void fwd_col3_body(Blk bk, double *smem, ColDesc C, S
   __syncthreads();
   if (tot <= (int)blockDim.x) {
     // chains of at most blockDim.x / kChains levels: log-depth scans
-    const int i = threadIdx.x, k = i / kChains, t = t0 + (i % kChains);
-    const bool store = k < C.len && t < C.nc;
-    chain_scan<true>(T, Lp, C.len);
+    const int i = threadIdx.x;
+    int k, t;
+    chain_elem(C, t0, i, k, t);
+    const bool on = i < tot;
+    const bool store = on && t < C.nc;
+    chain_scan<true>(T, Lp, C.len, on, k, chain_step(C));
     if (LAST) {
-      if (k < C.len) T[i] *= D[i];
+      if (on) T[i] *= D[i];
       __syncthreads();
-      chain_scan<false>(T, U, C.len);
+      chain_scan<false>(T, U, C.len, on, k, chain_step(C));
     }
-    if (store) y[C.base + t + k * C.nc] = T[i];
+    if (store) y[col_row(C, t, k)] = T[i];
     return;
   }
   if (threadIdx.x < kChains) {
@@ -747,18 +758,18 @@ __device__ __forceinline__ void fwd_col3_body(Blk bk, double *smem, ColDesc C, S
     if (t < C.nc) {
       double yp = 0.0;
       for (int k = 0; k < C.len; ++k) {
-        const int i = k * kChains + threadIdx.x;
+        const int i = chain_lds(C, threadIdx.x, k);
         const double v = T[i] - Lp[i] * yp;
         if (LAST) T[i] = v * D[i];
-        else y[C.base + t + k * C.nc] = v;
+        else y[col_row(C, t, k)] = v;
         yp = v;
       }
       if (LAST) {
         double xn = 0.0;
         for (int k = kl; k >= 0; --k) {
-          const int i = k * kChains + threadIdx.x;
+          const int i = chain_lds(C, threadIdx.x, k);
           const double v = T[i] - U[i] * xn;
-          y[C.base + t + k * C.nc] = v;
+          y[col_row(C, t, k)] = v;
           xn = v;
         }
       }
@@ -785,9 +796,10 @@ __device__ __forceinline__ void bwd_col3_body(Blk bk, double *smem, ColDesc C, S
   const int t0 = bk.b * kChains;
   const int kl = C.len - 1;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int k = i / kChains, t = t0 + (i % kChains);
+    int k, t;
+    chain_elem(C, t0, i, k, t);
     if (t >= C.nc) continue;
-    const int r = C.base + t + k * C.nc;
+    const int r = col_row(C, t, k);
     const RowPos rp = chain_pos(sl, C.g0 + t, k);
     const int jd = sell_diag(sl, rp, r);
     const int js = k < kl ? jd + 1 : jd;
@@ -825,9 +837,12 @@ __device__ __forceinline__ void bwd_col3_body(Blk bk, double *smem, ColDesc C, S
   }
   __syncthreads();
   if (tot <= (int)blockDim.x) {
-    const int i = threadIdx.x, k = i / kChains, t = t0 + (i % kChains);
-    chain_scan<false>(T, U, C.len);
-    if (k < C.len && t < C.nc) x[C.base + t + k * C.nc] = T[i];
+    const int i = threadIdx.x;
+    int k, t;
+    chain_elem(C, t0, i, k, t);
+    const bool on = i < tot;
+    chain_scan<false>(T, U, C.len, on, k, chain_step(C));
+    if (on && t < C.nc) x[col_row(C, t, k)] = T[i];
     return;
   }
   if (threadIdx.x < kChains) {
@@ -835,9 +850,9 @@ __device__ __forceinline__ void bwd_col3_body(Blk bk, double *smem, ColDesc C, S
     if (t < C.nc) {
       double xn = 0.0;
       for (int k = kl; k >= 0; --k) {
-        const int i = k * kChains + threadIdx.x;
+        const int i = chain_lds(C, threadIdx.x, k);
         const double v = T[i] - U[i] * xn;
-        x[C.base + t + k * C.nc] = v;
+        x[col_row(C, t, k)] = v;
         xn = v;
       }
     }

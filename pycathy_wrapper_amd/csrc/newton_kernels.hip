@@ -155,9 +155,10 @@ __global__ __launch_bounds__(256) void k_ilu_factor_col3(ColDesc C, Sell sl, Pai
   int *Pl = Pd + tot, *Pt = Pl + tot;
   const int t0 = xcd_group() * kChains;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int k = i / kChains, t = t0 + (i % kChains);
+    int k, t;
+    chain_elem(C, t0, i, k, t);
     if (t >= C.nc) continue;
-    const int r = C.base + t + k * C.nc;
+    const int r = col_row(C, t, k);
     const RowPos rp = chain_pos(sl, C.g0 + t, k);
     const int jd = sell_diag(sl, rp, r);
     const int jp = k > 0 ? jd - 1 : jd;
@@ -194,7 +195,8 @@ __global__ __launch_bounds__(256) void k_ilu_factor_col3(ColDesc C, Sell sl, Pai
   }
   __syncthreads();
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int k = i / kChains, t = t0 + (i % kChains);
+    int k, t;
+    chain_elem(C, t0, i, k, t);
     if (t >= C.nc || k == 0) continue;
     if (Pd[i] < 0) continue;
     const int q = Pl[i];
@@ -210,8 +212,8 @@ __global__ __launch_bounds__(256) void k_ilu_factor_col3(ColDesc C, Sell sl, Pai
       double dprev = 1.0;
       bool pdir = true;
       for (int k = 0; k < C.len; ++k) {
-        const int i = k * kChains + threadIdx.x;
-        const int r = C.base + t + k * C.nc;
+        const int i = chain_lds(C, threadIdx.x, k);
+        const int r = col_row(C, t, k);
         const int pd = Pd[i];
         if (pd < 0) {
           if (k > 0) {

@@ -60,13 +60,15 @@ def _member(size, seed, iopt=1, dry=False):
     return pr
 
 
-def test_lockstep_mixed_ensemble_matches_solo_runs_and_oracle(oracle):
+@pytest.mark.parametrize("layout", ["level-major", "chain-major"])
+def test_lockstep_mixed_ensemble_matches_solo_runs_and_oracle(oracle, layout):
     from pycathy_wrapper_amd import gpu as G
     K = 3
     plan = [("S", 100), ("S", 101), ("S", 102), ("S", 103), ("M", 132), ("M", 133), ("L", 156)]
     probs = [_member(sz, sd) for sz, sd in plan]
     say("lockstep mixed ensemble: " + ", ".join(f"{sz}{sd} ({p.mesh.n} nodes)" for (sz, sd), p in zip(plan, probs)))
-    flows = [G.FlowGPU(p, G.ORDER_COLUMN3) for p in probs]
+    order = G.ORDER_COLUMN3 if layout == "level-major" else G.ORDER_COLUMN3C
+    flows = [G.FlowGPU(p, order) for p in probs]
     ens = G.EnsembleGPU(flows)
     try:
         rc, taken, stats = ens.advance(K)
@@ -82,7 +84,7 @@ def test_lockstep_mixed_ensemble_matches_solo_runs_and_oracle(oracle):
     # each member alone: the same steps, bit for bit
     for j, (pr, (sz, sd)) in enumerate(zip(probs, plan)):
         say(f"solo {sz}{sd}")
-        g = G.FlowGPU(pr, G.ORDER_COLUMN3)
+        g = G.FlowGPU(pr, order)
         try:
             for s in range(K):
                 rc_g, sg = g.advance()
