@@ -112,6 +112,29 @@ module cathy_gpu
       integer(c_int), intent(in) :: surf
     end function
 
+    ! the routing block of a coupled run: a route handle and DATIN's cell
+    ! numbering (CELL(5,NCELL_COARSE), INDEX, INDEX_WITH_LAKES, DEM_MAP,
+    ! TRIANG(4,NTRI)); the flow step then runs SURF_FLOWTRA and PONDUPD
+    integer(c_int) function cathy_gpu_flow3d_set_routing(handle, route, nrow, ncol, delta_x, delta_y, &
+        ncell_coarse, cell, indcel, indcelwl, dem_map, ntri, triang) &
+        bind(C, name="cathy_gpu_flow3d_set_routing")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle, route
+      integer(c_int), intent(in) :: nrow, ncol, ncell_coarse, ntri
+      real(c_double), intent(in) :: delta_x, delta_y
+      integer(c_int), intent(in) :: cell(*), indcel(*), indcelwl(*), triang(*)
+      real(c_double), intent(in) :: dem_map(*)
+    end function
+
+    ! PONDNOD(NNOD), OVFLNOD(NNOD), H_WATER_KKP1_SN(NROW*NCOL); INFO(2)
+    integer(c_int) function cathy_gpu_flow3d_routing_state(handle, pondnod, ovflnod, h_water, info) &
+        bind(C, name="cathy_gpu_flow3d_routing_state")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      real(c_double), intent(out) :: pondnod(*), ovflnod(*), h_water(*)
+      integer(c_int), intent(out) :: info(2)
+    end function
+
     integer(c_int) function cathy_gpu_flow3d_init(handle, ptimep) bind(C, name="cathy_gpu_flow3d_init")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), intent(in) :: handle
