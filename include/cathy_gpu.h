@@ -44,6 +44,9 @@ extern "C" {
 #define CATHY_GPU_ORDER_COLUMN3 1 /* IC(0) in 3-colour column order (fast) */
 #define CATHY_GPU_ORDER_COLUMN3C 2 /* the same classes, each soil column's rows contiguous
                                     (chain-major; same results to rounding) */
+#define CATHY_GPU_ORDER_COLUMN3T 3 /* the same classes, rows in tiles of 8 soil columns,
+                                    each tile plane by plane (same results to
+                                    rounding; COLUMN3 is the fastest) */
 
 /* Human-readable text of the last error on this thread. */
 const char *cathy_gpu_last_error(void);

@@ -131,9 +131,15 @@ struct BcSeries {
 // ------------------------------------------------------------------ Flow --
 struct Flow {
   int n = 0, nt = 0, ntri = 0, nnod = 0, nstr = 0, nnz = 0, ordering = 0, lump = 1;
-  // the 3-colour column ordering, level-major or chain-major rows
-  bool column_order() const { return ordering == CATHY_GPU_ORDER_COLUMN3 || ordering == CATHY_GPU_ORDER_COLUMN3C; }
-  bool chain_major() const { return ordering == CATHY_GPU_ORDER_COLUMN3C; }
+  // the 3-colour column ordering; its rows in tiles of row_tile() chains
+  // per colour (0: one tile per colour, plane by plane)
+  bool column_order() const {
+    return ordering == CATHY_GPU_ORDER_COLUMN3 || ordering == CATHY_GPU_ORDER_COLUMN3C ||
+           ordering == CATHY_GPU_ORDER_COLUMN3T;
+  }
+  int row_tile() const {
+    return ordering == CATHY_GPU_ORDER_COLUMN3C ? 1 : (ordering == CATHY_GPU_ORDER_COLUMN3T ? kChains : 0);
+  }
   bool f32 = true;  // IC(0) factor precision
   // PCG SpMV reads the lower triangle through the transpose (column
   // ordering with valid chain tables): one member alone runs faster, since
@@ -313,7 +319,7 @@ struct Flow {
     s.width = cwidth;
     s.nplane = nstr + 1;
     s.ncolr = cncolr;
-    s.clay = chain_major() ? 1 : 0;
+    s.ctw = row_tile();
     for (int c = 0; c < 4; ++c) {
       s.cbase[c] = ccbase[c];
       s.cnc[c] = ccnc[c];
@@ -348,9 +354,8 @@ struct Flow {
       for (int i = 1; i < ncl; ++i)
         if (r >= ccbase[i]) c = i;
       const int off = r - ccbase[c];
-      if (chain_major()) {
-        t = off / np;
-        k = off - t * np;
+      if (row_tile()) {
+        tile_decode(off, ccnc[c], row_tile(), np, k, t);
       } else {
         k = off / ccnc[c];
         t = off - k * ccnc[c];
@@ -428,8 +433,7 @@ struct Flow {
     CG_REQUIRE((long)nnod * (nstr + 1) == n, "N must equal NNOD*(NSTR+1) (gen3d.f numbering)");
     CG_REQUIRE((long)3 * ntri * nstr == nt, "NT must equal 3*NTRI*NSTR (gen3d.f)");
     CG_REQUIRE(nzone >= 1 && maxstr >= nstr, "NZONE/MAXSTR");
-    CG_REQUIRE(ordering == CATHY_GPU_ORDER_NATURAL || ordering == CATHY_GPU_ORDER_COLUMN3 ||
-                   ordering == CATHY_GPU_ORDER_COLUMN3C, "ORDERING");
+    CG_REQUIRE(ordering == CATHY_GPU_ORDER_NATURAL || column_order(), "ORDERING");
     CG_REQUIRE((long)n <= (long)kMaxRedBlocks * kBlock, "N above the per-launch row limit");
     CG_REQUIRE(!column_order() || (nstr + 1) <= 64, "column ordering needs NSTR < 64");
     f32 = column_order();
@@ -631,25 +635,23 @@ struct Flow {
     for (int s = 0; s < nnod; ++s) rank[s] = cnt[c[s]]++;
     std::vector<long> base(ncolr + 1, 0);
     for (int k = 0; k < ncolr; ++k) base[k + 1] = base[k] + (long)cnt[k] * (nstr + 1);
-    // level-major: a colour's rows plane by plane, its chains side by
-    // side; chain-major: each chain's nstr+1 rows together
-    const bool cm = chain_major();
+    // a colour's rows plane by plane (level-major), or in tiles of
+    // row_tile() chains, each tile plane by plane
+    std::vector<ColDesc> cd(ncolr);
+    for (int k = 0; k < ncolr; ++k)
+      cd[k] = ColDesc{(int)base[k], std::max(cnt[k], 1), nstr + 1, 0, 0, 0, row_tile() ? row_tile() : std::max(cnt[k], 1)};
     for (int s = 0; s < nnod; ++s)
       for (int k = 0; k <= nstr; ++k) {
         const long ref = (long)k * nnod + s;
-        const long dev = cm ? base[c[s]] + (long)rank[s] * (nstr + 1) + k
-                            : base[c[s]] + (long)k * cnt[c[s]] + rank[s];
+        const long dev = col_row(cd[c[s]], rank[s], k);
         perm[ref] = (int)dev;
         iperm[dev] = (int)ref;
       }
     int g0 = 0;
     for (int k = 0; k < ncolr; ++k)
       if (cnt[k]) {
-        ColDesc C{(int)base[k], cnt[k], nstr + 1, g0, 0, 0, 1, cnt[k]};
-        if (cm) {
-          C.st = nstr + 1;
-          C.sk = 1;
-        }
+        ColDesc C = cd[k];
+        C.g0 = g0;
         sched.cols.push_back(C);
         g0 += cnt[k];
       }
@@ -702,14 +704,14 @@ struct Flow {
           if (fac_class(k, C.len) == cls) krep = k;
         if (krep < 0) continue;
         for (int t = 0; t < C.nc; ++t)
-          if (!row_bytes(col_row(C, t, krep), krep, C.sk, &tb[((size_t)cls * nch + C.g0 + t) * NB])) return;
+          if (!row_bytes(col_row(C, t, krep), krep, col_w(C, t), &tb[((size_t)cls * nch + C.g0 + t) * NB])) return;
       }
     }
     for (const ColDesc &C : sched.cols)
       for (int k = 0; k < C.len; ++k) {
         const int cls = fac_class(k, C.len);
         for (int t = 0; t < C.nc; ++t) {
-          if (!row_bytes(col_row(C, t, k), k, C.sk, tmp.data())) return;
+          if (!row_bytes(col_row(C, t, k), k, col_w(C, t), tmp.data())) return;
           if (std::memcmp(tmp.data(), &tb[((size_t)cls * nch + C.g0 + t) * NB], NB) != 0) return;
         }
       }
@@ -864,14 +866,14 @@ struct Flow {
       for (const ColDesc &C : sched.cols)
         for (int k = 0; k < C.len; ++k)
           for (int t = 0; t < C.nc; ++t) {
-            const int r = col_row(C, t, k);
-            if (k > 0) CG_REQUIRE(hsd[r] >= 1 && hsc[spos(r, hsd[r] - 1)] == r - C.sk, "column predecessor slot");
+            const int r = col_row(C, t, k), w = col_w(C, t);
+            if (k > 0) CG_REQUIRE(hsd[r] >= 1 && hsc[spos(r, hsd[r] - 1)] == r - w, "column predecessor slot");
             if (k + 1 < C.len)
-              CG_REQUIRE(hsd[r] + 1 < hlen[r] && hsc[spos(r, hsd[r] + 1)] == r + C.sk, "column successor slot");
+              CG_REQUIRE(hsd[r] + 1 < hlen[r] && hsc[spos(r, hsd[r] + 1)] == r + w, "column successor slot");
             for (int j = 0; j < hlen[r]; ++j) {  // no coupling inside a colour class but the chain
               const int c = hsc[spos(r, j)];
               if (c >= C.base && c < C.base + C.nc * C.len && c != r)
-                CG_REQUIRE(c == r - C.sk || c == r + C.sk, "colour class not independent");
+                CG_REQUIRE(c == r - w || c == r + w, "colour class not independent");
             }
           }
     h_rowptr = std::move(hrp);

@@ -21,8 +21,10 @@ constexpr int kChains = 8;      // chains per work-group in the column kernels (
 constexpr int kUnroll = 16;     // row slots handled by the unrolled gather loops
 
 // Column ids come from one of two sources.  Explicit: col[slot].  Coded
-// (extruded meshes in the column ordering): row r = base_c + k nc_c + t
-// (level-major) or base_c + t nplane + k (chain-major, clay = 1)
+// (extruded meshes in the column ordering): the rows of colour c lie in
+// tiles of TW chains (TW = nc_c: level-major, the whole colour plane by
+// plane; TW = 1: chain-major; else TW chains plane by plane): row = base_c
+// + tile*TW*nplane + k*w + (t - tile*TW), w = the tile's width
 // (colour c, node plane k, rank t; chain g = goff_c + t) and every slot's
 // column is (c', k + dk, t'), where the code (c', dk, t') depends only on
 // the chain g and the plane type pt (0 surface, 1 interior, 2 base):
@@ -39,7 +41,7 @@ struct Sell {
   const uint32_t *code;
   const uint8_t *clen, *cdiag;
   int coded, nchain, width, nplane, ncolr;
-  int clay;  // row layout of the colour classes: 0 level-major, 1 chain-major
+  int ctw;   // chains per row tile of every colour class (0: level-major, one tile per class)
   int cbase[4], cnc[4], cgoff[4];
 };
 
@@ -55,6 +57,22 @@ __device__ __forceinline__ T pick4(int i, T a0, T a1, T a2, T a3) {
   return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
 }
 
+// Plane k and rank t of the row at offset off of a colour class with nc
+// chains of np planes, in tiles of tw chains.
+__device__ __host__ __forceinline__ void tile_decode(int off, int nc, int tw, int np, int &k, int &t) {
+  const int tile = off / (tw * np);
+  const int rem = off - tile * tw * np;
+  const int w = nc - tile * tw < tw ? nc - tile * tw : tw;
+  k = rem / w;
+  t = tile * tw + (rem - k * w);
+}
+// Offset of (rank t, plane k) in such a class.
+__device__ __host__ __forceinline__ int tile_offset(int t, int k, int nc, int tw, int np) {
+  const int tile = t / tw;
+  const int w = nc - tile * tw < tw ? nc - tile * tw : tw;
+  return tile * tw * np + k * w + (t - tile * tw);
+}
+
 __device__ __forceinline__ RowPos row_pos(const Sell &sl, int r) {
   RowPos p{0, 0, 0};
   if (!sl.coded) return p;
@@ -65,9 +83,14 @@ __device__ __forceinline__ RowPos row_pos(const Sell &sl, int r) {
   const int b = pick4(c, sl.cbase[0], sl.cbase[1], sl.cbase[2], sl.cbase[3]);
   const int nc = pick4(c, sl.cnc[0], sl.cnc[1], sl.cnc[2], sl.cnc[3]);
   const int off = r - b;
-  const int tq = sl.clay ? off / sl.nplane : 0;
-  p.k = sl.clay ? off - tq * sl.nplane : off / nc;
-  p.g = pick4(c, sl.cgoff[0], sl.cgoff[1], sl.cgoff[2], sl.cgoff[3]) + (sl.clay ? tq : off - p.k * nc);
+  int t;
+  if (sl.ctw) {
+    tile_decode(off, nc, sl.ctw, sl.nplane, p.k, t);
+  } else {
+    p.k = off / nc;
+    t = off - p.k * nc;
+  }
+  p.g = pick4(c, sl.cgoff[0], sl.cgoff[1], sl.cgoff[2], sl.cgoff[3]) + t;
   p.pt = p.k == 0 ? 0 : (p.k == sl.nplane - 1 ? 2 : 1);
   return p;
 }
@@ -97,7 +120,7 @@ __device__ __forceinline__ int code_col(const Sell &sl, int k, uint32_t w) {
   nc = c2 == 2 ? sl.cnc[2] : nc;
   b = c2 == 3 ? sl.cbase[3] : b;
   nc = c2 == 3 ? sl.cnc[3] : nc;
-  return sl.clay ? b + t2 * sl.nplane + (k + dk) : b + (k + dk) * nc + t2;
+  return sl.ctw ? b + tile_offset(t2, k + dk, nc, sl.ctw, sl.nplane) : b + (k + dk) * nc + t2;
 }
 
 // column of slot j (SELL position q) of the row at p / r
@@ -131,22 +154,30 @@ __device__ __forceinline__ int sell_diag(const Sell &sl, const RowPos &p, int r)
   return sl.coded ? (int)sl.cdiag[p.pt * sl.nchain + p.g] : sl.diag[r];
 }
 
-// One colour class of the column ordering: chains t < nc, chain t = rows
-// base + t*st + k*sk for k < len (k = node plane, 0 = surface): level-major
-// (st = 1, sk = nc) or chain-major (st = len, sk = 1).
+// One colour class of the column ordering: chains t < nc of len planes
+// (k = node plane, 0 = surface), in row tiles of tw chains: tw = nc
+// (level-major, row base + k*nc + t), tw = 1 (chain-major, base + t*len +
+// k) or tw = kChains (each work-group's rows one contiguous block).
 struct ColDesc {
   int base, nc, len, g0;  // g0: chain id of t = 0
   int nf, nb;             // most earlier- / later-colour slots of a row (gathered by the sweeps)
-  int st, sk;             // row strides between chains and between planes
+  int tw;                 // chains per row tile
 };
-__device__ __host__ __forceinline__ int col_row(const ColDesc &C, int t, int k) { return C.base + t * C.st + k * C.sk; }
+// rows between planes k and k+1 of chain t (its tile's width)
+__device__ __host__ __forceinline__ int col_w(const ColDesc &C, int t) {
+  const int t1 = (t / C.tw) * C.tw;
+  return C.nc - t1 < C.tw ? C.nc - t1 : C.tw;
+}
+__device__ __host__ __forceinline__ int col_row(const ColDesc &C, int t, int k) {
+  return C.tw >= C.nc ? C.base + k * C.nc + t : C.base + tile_offset(t, k, C.nc, C.tw, C.len);
+}
 // A column work-group holds kChains chains, one row per thread (or a loop
 // over them): element i of its LDS arrays is chain t0 + t_local, plane k.
-// Level-major classes put the chains side by side (i = k*kChains +
-// t_local, consecutive threads on consecutive chains, as the rows lie);
-// chain-major ones one chain after the other (i = t_local*len + k).
+// Tiles of several chains put them side by side (i = k*kChains + t_local,
+// consecutive threads on consecutive chains, as the rows lie); chain-major
+// classes one chain after the other (i = t_local*len + k).
 __device__ __forceinline__ void chain_elem(const ColDesc &C, int t0, int i, int &k, int &t) {
-  if (C.sk == 1) {
+  if (C.tw == 1) {
     const int tl = i / C.len;
     k = i - tl * C.len;
     t = t0 + tl;
@@ -157,9 +188,9 @@ __device__ __forceinline__ void chain_elem(const ColDesc &C, int t0, int i, int 
 }
 // LDS index of (local chain tl, plane k) and the step between planes
 __device__ __forceinline__ int chain_lds(const ColDesc &C, int tl, int k) {
-  return C.sk == 1 ? tl * C.len + k : k * kChains + tl;
+  return C.tw == 1 ? tl * C.len + k : k * kChains + tl;
 }
-__device__ __forceinline__ int chain_step(const ColDesc &C) { return C.sk == 1 ? 1 : kChains; }
+__device__ __forceinline__ int chain_step(const ColDesc &C) { return C.tw == 1 ? 1 : kChains; }
 
 // One level of the natural-order schedule: rows lvl_rows[off .. off+cnt).
 struct ListLevel {

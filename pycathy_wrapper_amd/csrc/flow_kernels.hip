@@ -610,7 +610,7 @@ __device__ __forceinline__ void ic_factor_col3t_body(Blk bk, double *smem, ColDe
       Pd[i] = base + 64 * jd;
     }
     if (k > 0) {
-      const int pred = r - C.sk;
+      const int pred = r - col_w(C, t);
       Pl[i] = base + 64 * (jd - 1);
       Pt[i] = sl.so[pred >> 6] + (pred & 63) + 64 * tab_byte(wt, jd - 1);
     } else {
@@ -622,7 +622,7 @@ __device__ __forceinline__ void ic_factor_col3t_body(Blk bk, double *smem, ColDe
   // the predecessor's numerator: its own earlier-colour entries are final
   // after the barrier
   if (live && k > 0 && Pd[i] >= 0) {
-    const int pred = r - C.sk;
+    const int pred = r - col_w(C, t);
     const int sp0 = sl.so[pred >> 6] + (pred & 63);
     FT G2[NWc];
 #pragma unroll

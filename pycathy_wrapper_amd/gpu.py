@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "libcathy_gpu.so")
 ORDER_NATURAL = 0
 ORDER_COLUMN3 = 1
 ORDER_COLUMN3C = 2  # the column ordering with each soil column's rows contiguous
+ORDER_COLUMN3T = 3  # the column ordering with rows in tiles of 8 soil columns
 
 STAT_NAMES = ["NSTEP", "DELTAT", "TIME", "ITER", "NITERT", "BACK", "STORE1", "STORE2", "DSTORE", "CDSTOR",
               "VIN", "CVIN", "VOUT", "CVOUT", "VINOUT", "VTOT", "ERRAS", "ERREL", "CERRAS", "CAERAS", "PINF",

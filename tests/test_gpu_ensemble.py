@@ -60,14 +60,14 @@ def _member(size, seed, iopt=1, dry=False):
     return pr
 
 
-@pytest.mark.parametrize("layout", ["level-major", "chain-major"])
+@pytest.mark.parametrize("layout", ["level-major", "chain-major", "tiled"])
 def test_lockstep_mixed_ensemble_matches_solo_runs_and_oracle(oracle, layout):
     from pycathy_wrapper_amd import gpu as G
     K = 3
     plan = [("S", 100), ("S", 101), ("S", 102), ("S", 103), ("M", 132), ("M", 133), ("L", 156)]
     probs = [_member(sz, sd) for sz, sd in plan]
     say("lockstep mixed ensemble: " + ", ".join(f"{sz}{sd} ({p.mesh.n} nodes)" for (sz, sd), p in zip(plan, probs)))
-    order = G.ORDER_COLUMN3 if layout == "level-major" else G.ORDER_COLUMN3C
+    order = {"level-major": G.ORDER_COLUMN3, "chain-major": G.ORDER_COLUMN3C, "tiled": G.ORDER_COLUMN3T}[layout]
     flows = [G.FlowGPU(p, order) for p in probs]
     ens = G.EnsembleGPU(flows)
     try:

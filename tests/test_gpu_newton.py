@@ -81,12 +81,13 @@ def _oracle_run(oracle, pr):
                              P.OrcParams(**pr.params.as_dict()))
 
 
-@pytest.mark.parametrize("ordering", ["natural", "column3", "column3c"])
+@pytest.mark.parametrize("ordering", ["natural", "column3", "column3c", "column3t"])
 def test_newton_heterogeneous_matches_oracle(gpu, oracle, ordering):
     """41x41 nodes, 15 graded layers, per-cell soil: 6 Newton steps
-    (column3c: the column ordering's chain-major row layout)."""
+    (column3c / column3t: the column ordering's chain-major / tiled rows)."""
     pr = CF.hillslope_large(cells=40, nstr=15, params=CF.hillslope_small_params(tmax=172800.0, iopt=2))
-    order = {"natural": gpu.ORDER_NATURAL, "column3": gpu.ORDER_COLUMN3, "column3c": gpu.ORDER_COLUMN3C}[ordering]
+    order = {"natural": gpu.ORDER_NATURAL, "column3": gpu.ORDER_COLUMN3, "column3c": gpu.ORDER_COLUMN3C,
+             "column3t": gpu.ORDER_COLUMN3T}[ordering]
     g = gpu.FlowGPU(pr, order)
     run = _oracle_run(oracle, pr)
     scale = 1e-6 * pr.mesh.volnod.sum()

@@ -170,12 +170,13 @@ def test_flow_natural_full_day_matches_reference(gpu):
     _full_day(gpu, gpu.ORDER_NATURAL, np.load(GOLD), exact_pcg=True)
 
 
-@pytest.mark.parametrize("layout", ["level-major", "chain-major"])
+@pytest.mark.parametrize("layout", ["level-major", "chain-major", "tiled"])
 def test_flow_column3_full_day_matches_reference(gpu, layout):
     """The same day with the 3-colour column ordering (FP32 factor), its
-    rows plane by plane or each soil column's rows together."""
-    _full_day(gpu, gpu.ORDER_COLUMN3 if layout == "level-major" else gpu.ORDER_COLUMN3C, np.load(GOLD),
-              exact_pcg=False)
+    rows plane by plane, each soil column's rows together, or in tiles of 8
+    columns."""
+    order = {"level-major": gpu.ORDER_COLUMN3, "chain-major": gpu.ORDER_COLUMN3C, "tiled": gpu.ORDER_COLUMN3T}
+    _full_day(gpu, order[layout], np.load(GOLD), exact_pcg=False)
 
 
 def _oracle_run(oracle, pr):
@@ -186,15 +187,16 @@ def _oracle_run(oracle, pr):
                              P.OrcParams(**pr.params.as_dict()))
 
 
-@pytest.mark.parametrize("ordering", ["natural", "column3", "column3_fp64", "column3c", "column3c_fp64"])
+@pytest.mark.parametrize("ordering", ["natural", "column3", "column3_fp64", "column3c", "column3c_fp64", "column3t"])
 def test_flow_heterogeneous_matches_oracle(gpu, oracle, ordering):
     """41x41 nodes, 15 graded layers, per-cell lognormal Ks and +-20% vG
     parameters, rain then drainage: 8 steps against the oracle (column3_fp64:
-    the column ordering with its IC(0) factor in FP64; column3c: its
-    chain-major row layout)."""
+    the column ordering with its IC(0) factor in FP64; column3c / column3t:
+    its chain-major / tiled row layouts)."""
     pr = CF.hillslope_large(cells=40, nstr=15, params=CF.hillslope_small_params(tmax=172800.0))
     order = gpu.ORDER_NATURAL if ordering == "natural" else (
-        gpu.ORDER_COLUMN3C if ordering.startswith("column3c") else gpu.ORDER_COLUMN3)
+        gpu.ORDER_COLUMN3C if ordering.startswith("column3c") else
+        gpu.ORDER_COLUMN3T if ordering.startswith("column3t") else gpu.ORDER_COLUMN3)
     g = gpu.FlowGPU(pr, order)
     if ordering.endswith("_fp64"):
         g.factor_fp64(True)
@@ -273,7 +275,7 @@ def test_flow_spatial_forcing_matches_oracle(gpu, oracle, ordering, ieto):
         run.close()
 
 
-@pytest.mark.parametrize("layout", ["level-major", "chain-major"])
+@pytest.mark.parametrize("layout", ["level-major", "chain-major", "tiled"])
 def test_flow_deep_columns_match_oracle(gpu, oracle, layout):
     """Columns of 36 nodes (35 layers): more levels than the column kernels'
     one-row-per-thread forms hold (32), so the factor takes the pair-list
@@ -281,7 +283,8 @@ def test_flow_deep_columns_match_oracle(gpu, oracle, layout):
     heterogeneous 13x13-node hillslope against the oracle."""
     pr = CF.hillslope_large(cells=12, nstr=35, params=CF.hillslope_small_params(tmax=172800.0))
     assert pr.mesh.nstr + 1 > 32
-    g = gpu.FlowGPU(pr, gpu.ORDER_COLUMN3 if layout == "level-major" else gpu.ORDER_COLUMN3C)
+    g = gpu.FlowGPU(pr, {"level-major": gpu.ORDER_COLUMN3, "chain-major": gpu.ORDER_COLUMN3C,
+                         "tiled": gpu.ORDER_COLUMN3T}[layout])
     run = _oracle_run(oracle, pr)
     scale = 1e-6 * pr.mesh.volnod.sum()
     try:
