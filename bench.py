@@ -891,6 +891,7 @@ def main() -> int:
         f"{streams_rate:.2f} member-steps/s")
     for _, gi in others:
         gi.close()
+    psi32 = g.psi()  # member 0 after its K steps (FP32 factor), before the full run moves it on
     full = None
     if args.full_run > 0:
         full = full_run(pr, g, args.full_run)
@@ -903,7 +904,6 @@ def main() -> int:
         # the same steps of member 0 with its IC(0) factor in FP64 (the
         # column ordering's factor is FP32 otherwise)
         import numpy as np
-        psi32 = g.psi()
         g.factor_fp64(True)
         g.init(pr.psi0)
         n64 = pic64 = pcg64 = 0
