@@ -432,6 +432,13 @@ int cathy_gpu_flow3d_set_routing(int64_t *handle, const int64_t *route,
                                  const double *dem_map, const int *ntri,
                                  const int *triang);
 
+/* Initial ponding heads PONDNOD(NNOD) of a coupled run (IPOND = 1 or 2;
+ * datin.f:394-403 reads them and sets PTIMEP = PONDNOD at ponded surface
+ * nodes, which the caller's initial head carries).  With them the run starts
+ * with PONDING set (inital.f:201-207), so its first step routes; they need
+ * set_surface and set_routing.  Applied at every init; NULL clears them. */
+int cathy_gpu_flow3d_set_initial_ponding(int64_t *handle, const double *pondnod);
+
 /* The routing block's state: PONDNOD(NNOD) and OVFLNOD(NNOD) by surface
  * node, H_WATER_KKP1_SN(NROW*NCOL) by I_BASIN - 1 (null outputs skipped);
  * info[2]: routing blocks run, PONDING at the last step's end. */

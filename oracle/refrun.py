@@ -97,7 +97,7 @@ class RefFlowRun:
 
     def __init__(self, ref: RefLib, mesh, soil, psi0, rain, prm: dict, atmbc: str | None = None,
                  dirbc: str | None = None, neubc: str | None = None, veg: dict | None = None,
-                 sfbc: str | None = None, isfcvg: int = 0, surf: bool = False, routing=None):
+                 sfbc: str | None = None, isfcvg: int = 0, surf: bool = False, routing=None, pond0=None):
         assert soil["nzone"] <= ref.maxzon, "more zones than the reference build's MAXZON"
         assert mesh.nstr <= ref.maxstr and mesh.n <= ref.nmax, "mesh larger than the reference build"
         self.ref = ref
@@ -204,6 +204,13 @@ class RefFlowRun:
         if routing is not None:
             assert surf, "the routing block needs the coupled switching"
             self._init_routing()
+        # ``pond0``: initial ponding heads (IPOND 1/2, datin.f:394-403); the
+        # run then starts with PONDING set (inital.f:201-207)
+        if pond0 is not None:
+            assert routing is not None, "initial ponding routes from the first step"
+            a["PONDNOD"][:self.nnod] = pond0
+            a["PONDING"][0] = 1
+            self.pondp[0] = 1
         self._tmp = None
         if self.lists:
             self._tmp = tempfile.mkdtemp(prefix="refbc_")

@@ -236,6 +236,13 @@ int cathy_gpu_flow3d_set_spmv_transposed(int64_t *handle, const int *on, int *ac
   });
 }
 
+int cathy_gpu_flow3d_set_initial_ponding(int64_t *handle, const double *pondnod) {
+  return guarded([&] {
+    lookup(handle)->set_initial_ponding(pondnod);
+    return 0;
+  });
+}
+
 int cathy_gpu_flow3d_set_factor_fp64(int64_t *handle, const int *on) {
   return guarded([&] {
     const std::shared_ptr<Flow> f = lookup(handle);

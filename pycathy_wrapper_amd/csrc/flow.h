@@ -1917,6 +1917,17 @@ struct Flow {
     rt = std::move(R);
     routing_reset();
   }
+  // Initial ponding heads (IPOND = 1 or 2, datin.f:394-403): with them the
+  // run starts ponded (inital.f:201-207), so its first step routes.
+  std::vector<double> pond0;
+  void set_initial_ponding(const double *p) {
+    if (!p) {
+      pond0.clear();
+      return;
+    }
+    for (int s = 0; s < nnod; ++s) CG_REQUIRE(std::isfinite(p[s]), "non-finite ponding head");
+    pond0.assign(p, p + nnod);
+  }
   // INIT_SR (init_sr.f:13-38) and the time loop's own arrays: all clear
   void routing_reset() {
     Routing &R = *rt;
@@ -2156,6 +2167,12 @@ struct Flow {
     // inital.f:201-205: PONDING starts clear (IPOND = 0); INIT_SR (init_sr.f)
     hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, 0);
     if (rt) routing_reset();
+    if (!pond0.empty()) {  // PONDING from the start, PONDNOD as DATIN read it
+      CG_REQUIRE(surf && rt, "initial ponding needs the coupled surface with its routing (set_surface, set_routing)");
+      rt->pondnod.upload(pond0.data(), nnod, st);
+      rt->pond_host = rt->pondp = 1;
+      hipLaunchKernelGGL(k_put_int, dim3(1), dim3(64), 0, st, pond.p, 1);
+    }
     // INIT1 (init1.f:37-60)
     deltat = prm.deltat;
     if (deltat > prm.dtmax) deltat = prm.dtmax;

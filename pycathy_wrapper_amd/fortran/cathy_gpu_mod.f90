@@ -126,6 +126,14 @@ module cathy_gpu
       real(c_double), intent(in) :: dem_map(*)
     end function
 
+    ! initial ponding heads PONDNOD(NNOD) (IPOND 1/2): the run starts ponded
+    integer(c_int) function cathy_gpu_flow3d_set_initial_ponding(handle, pondnod) &
+        bind(C, name="cathy_gpu_flow3d_set_initial_ponding")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), intent(in) :: handle
+      real(c_double), intent(in) :: pondnod(*)
+    end function
+
     ! PONDNOD(NNOD), OVFLNOD(NNOD), H_WATER_KKP1_SN(NROW*NCOL); INFO(2)
     integer(c_int) function cathy_gpu_flow3d_routing_state(handle, pondnod, ovflnod, h_water, info) &
         bind(C, name="cathy_gpu_flow3d_routing_state")

@@ -121,6 +121,8 @@ class FlowGPU:
         self.route = None
         if getattr(problem, "routing", None) is not None:
             self.set_routing(problem.routing)
+            if problem.routing.get("pond0") is not None:
+                self.set_initial_ponding(problem.routing["pond0"])
         self.init(problem.psi0)
 
     def set_params(self, p, hspatm: int | None = None):
@@ -220,6 +222,19 @@ class FlowGPU:
             C.byref(self.h), C.byref(self.route.h), _i(nr), _i(nc), C.byref(C.c_double(routing["dx"])),
             C.byref(C.c_double(routing["dy"])), _i(len(cell)), _p(keep["cell"]), _p(keep["indcel"]),
             _p(keep["indcelwl"]), _p(keep["dem"]), _i(m.ntri), _p(keep["tri"])), "set_routing")
+
+    def set_initial_ponding(self, pondnod) -> None:
+        """Initial ponding heads PONDNOD (IPOND 1 or 2): the run starts
+        ponded and its first step routes (needs the routing linked); None
+        clears them.  Takes effect at the next init."""
+        if pondnod is None:
+            _check(lib().cathy_gpu_flow3d_set_initial_ponding(C.byref(self.h), None), "set_initial_ponding")
+            return
+        p = np.ascontiguousarray(pondnod, dtype=np.float64)
+        if p.shape != (self.nnod,):
+            raise ValueError("one ponding head per surface node")
+        self._keep["pond0"] = p
+        _check(lib().cathy_gpu_flow3d_set_initial_ponding(C.byref(self.h), _p(p)), "set_initial_ponding")
 
     def routing_state(self):
         """(PONDNOD, OVFLNOD, H_WATER_KKP1_SN, routing blocks run)."""

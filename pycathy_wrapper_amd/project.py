@@ -31,7 +31,9 @@ network (prepro/qoi_a, the dtm_* rasters, lakes_map and posizione_serb:
 ``read_routing``), a ponded surface is routed as the reference's time
 loop routes it (``Problem.routing``, cathy_gpu_flow3d_set_routing);
 without one, a step that ends ponded stops the run (status 3).  A project
-that starts ponded (IPOND != 0) is refused.  ``subsurface_only`` runs
+that starts ponded (IPOND != 0) starts with its ponding heads and routes
+from the first step (``Problem.routing["pond0"]``); without a drainage
+network it is refused.  ``subsurface_only`` runs
 such a project as ISIMGR = 1 instead (SWITCH_OLD, PONDH_MIN = 1e10,
 datin.f:120).
 """
@@ -340,9 +342,10 @@ def load_project(root: str, subsurface_only: bool = False, files: dict | None = 
     soil = read_soil(fl["soil"], dp["nstr"], dp["nzone"], nveg)
     ic = read_ic(fl["ic"], m.n, m.nnod)
     surf = parm["isimgr"] >= 2 and not subsurface_only
-    if surf and ic["ipond"] != 0:   # inital.f:201-205: PONDING from the start, so step 1 routes
-        raise ValueError(f"project {root}: initial ponding (IPOND = {ic['ipond']}) needs the surface routing, "
-                         f"which the device step does not run")
+    ponded = surf and ic["ipond"] != 0   # inital.f:201-205: PONDING from the start, so step 1 routes
+    if ponded and not os.path.exists(os.path.join(root, "prepro", "qoi_a")):
+        raise ValueError(f"project {root}: initial ponding (IPOND = {ic['ipond']}) routes from the first step, "
+                         f"and the project has no drainage network (prepro/qoi_a)")
     psi0 = initial_head(ic, m, surf)
     hspatm, ieto, at, av = read_atmbc(fl["atmbc"], m.nnod)
     bc_neu = B.read_bc(fl["nansfneubc"], "neumann", m.nnod, m.nstr) if fl.get("nansfneubc") else None
@@ -369,6 +372,8 @@ def load_project(root: str, subsurface_only: bool = False, files: dict | None = 
     routing = None
     if surf and os.path.exists(os.path.join(root, "prepro", "qoi_a")):
         routing = read_routing(root, dem, zone, dp["dx"], dp["dy"], dp["dostep"])
+        if ponded:   # PONDNOD as DATIN read it (datin.f:394-403)
+            routing["pond0"] = np.asarray(ic["pondnod"], dtype=np.float64).copy()
     meta = dict(project=os.path.abspath(root), parm=parm, isimgr=parm["isimgr"], subsurface_only=subsurface_only,
                 ic=dict(indp=ic["indp"], ipond=ic["ipond"], wtposition=ic["wtposition"]), nveg=nveg)
     return CF.Problem(os.path.basename(os.path.abspath(root)), m, soil["table"], psi0, at, av, prm, 0, meta, hspatm,

@@ -63,6 +63,7 @@ from __future__ import annotations
 
 import os
 import re
+import shutil
 import sys
 
 import numpy as np
@@ -397,6 +398,18 @@ def copy_projects() -> None:
         print(f"{name}: psi blocks at steps {list(arrs['psi_nstep'])}")
 
 
+def make_ponded_project() -> None:
+    """weil_project started ponded: its input/ic set to INDP 2, IPOND 1 with
+    a 2 cm ponding head everywhere (datin.f:377-403, icvhe.f: hydrostatic
+    with the water table 2 cm above the surface), all else unchanged."""
+    src, dst = os.path.join(HERE, "weil_project"), os.path.join(HERE, "ponded_project")
+    if os.path.exists(dst):
+        shutil.rmtree(dst)
+    shutil.copytree(src, dst, ignore=shutil.ignore_patterns("output"))
+    with open(os.path.join(dst, "input", "ic"), "w") as fh:
+        fh.write("2 1 INDP IPOND\n0.02   PONDNOD\n")
+
+
 def make_project_run(name: str, out: str, steps_saved=(1, 2, 5, 10, 50, 100, 200, 300), nsteps=10 ** 9) -> None:
     """Item 9: the project run by the reference's FLOW3D with the coupled
     switching, for as many steps as the stored mbeconv holds."""
@@ -415,9 +428,10 @@ def make_project_run(name: str, out: str, steps_saved=(1, 2, 5, 10, 50, 100, 200
     ref = P.RefLib(P.REF_BC_SO)
     # the routing block where the project holds its drainage network
     routing = RP.RefProject(ref, root) if os.path.exists(os.path.join(root, "prepro", "qoi_a")) else None
+    pond0 = pr.routing.get("pond0") if pr.routing is not None else None
     rr = R.RefFlowRun(ref, m, pr.soil, pr.psi0, 0.0, pr.params.as_dict(), atmbc=io("atmbc"),
                       dirbc=io("nansfdirbc"), neubc=io("nansfneubc"), veg=pr.veg, sfbc=io("sfbc"), surf=pr.surf,
-                      routing=routing)
+                      routing=routing, pond0=pond0)
     stats, psi, sw, ifatm, pond, saved, keys = [], [], [], [], [], [], None
     k = 0
     while True:
@@ -457,6 +471,9 @@ if __name__ == "__main__":
                      steps_saved=(1, 2, 5, 10, 50, 100, 200, 300, 400, 500, 600, 700))
     make_project_run("weil_project", os.path.join(HERE, "weil_ref.npz"),
                      steps_saved=(1, 2, 10, 50, 100, 179, 235))
+    make_ponded_project()
+    make_project_run("ponded_project", os.path.join(HERE, "ponded_ref.npz"), steps_saved=(1, 2, 5, 10, 20, 40, 60),
+                     nsteps=60)
     make_project_run("spatial_atmbc_project", os.path.join(HERE, "spatial_atmbc_ref.npz"),
                      steps_saved=(1, 2, 10, 100, 500, 1027))
     make_project_run("soil3d_project", os.path.join(HERE, "soil3d_ref.npz"), steps_saved=(1, 2, 10, 20, 40))

@@ -219,6 +219,52 @@ def test_routed_project_matches_reference(name, ordering):
     say(f"{name} {ordering}: {k} steps, worst psi rel {worst:.2e}")
 
 
+@pytest.mark.parametrize("ordering", ["natural", "column3"])
+def test_ponded_start_matches_reference(ordering):
+    """weil_exemple started ponded (input/ic: INDP 2, IPOND 1, a 2 cm
+    ponding head everywhere; tests/golden/ponded_project): the run starts
+    with PONDING set (inital.f:201-207) and routes before its first step.
+    60 steps against the reference's run of the same files
+    (tests/golden/ponded_ref.npz), as for the other routed projects."""
+    from pycathy_wrapper_amd import gpu as G
+    gold = np.load(os.path.join(GOLD, "ponded_ref.npz"))
+    col = {nm: j for j, nm in enumerate(str(v) for v in gold["stat_names"])}
+    stats = gold["stats"]
+    saved = list(gold["steps"])
+    pr = PJ.load_project(os.path.join(GOLD, "ponded_project"))
+    assert pr.routing is not None and np.all(pr.routing["pond0"] == 0.02)
+    g = G.FlowGPU(pr, G.ORDER_NATURAL if ordering == "natural" else G.ORDER_COLUMN3)
+    scale = abs(stats[:, col["STORE1"]]).max() * (1e-6 if ordering == "natural" else 1e-4)
+    try:
+        for k in range(1, len(stats) + 1):
+            rc, s = g.advance()
+            ref = stats[k - 1]
+            assert s["DELTAT"] == ref[col["DELTAT"]], (k, s["DELTAT"], ref[col["DELTAT"]])
+            assert s["ITER"] == ref[col["ITER"]] and s["BACK"] == ref[col["BACK"]], k
+            assert s["PONDING"] == ref[col["PONDING"]], k
+            if ordering == "natural":
+                assert s["NITERT"] == ref[col["NITERT"]], k
+            for key in MB_KEYS:
+                assert abs(s[key] - ref[col[key]]) <= 1e-8 * max(abs(ref[col[key]]), scale), \
+                    (k, key, s[key], ref[col[key]])
+            if k in saved:
+                i = saved.index(k)
+                assert _rel_inf(g.psi(), gold["psi"][i]) <= PSI_TOL, k
+                assert _rel_inf(g.sw(), gold["sw"][i]) <= PSI_TOL, k
+                assert np.array_equal(g.ifatm(), gold["ifatm"][i]), k
+                pond = g.routing_state()[0]
+                assert np.max(np.abs(pond - gold["pondnod"][i])) <= PSI_TOL * max(np.abs(gold["pondnod"][i]).max(),
+                                                                                  1e-3), k
+            if k == 1:
+                assert g.routing_state()[3] == 1 + s["BACK"]   # the first step routed
+            if rc:
+                break
+        assert k == len(stats)
+        assert g.routing_state()[3] == int(gold["routed"])
+    finally:
+        g.close()
+
+
 def test_weil_without_its_network_stops_where_it_ponds():
     """Without the drainage network the device cannot route: the coupled
     run stops (3) after the first step, which equals the reference's."""

@@ -333,3 +333,39 @@ def test_project_refusals(tmp_path):
     (root / "input" / "parm").write_text("\n".join(parm) + "\n")
     with pytest.raises(ValueError, match="NLRELX"):
         PJ.load_project(str(root))
+
+
+def test_ponded_start_reader_and_reference():
+    """The project started ponded (INDP 2, IPOND 1, 2 cm): the reader keeps
+    PONDNOD as the reference's DATIN reads it and forms the initial head as
+    ICVHE does (water table 2 cm above the surface); the reference harness,
+    started with PONDING set, re-derives the first steps of its golden run
+    (tests/golden/ponded_ref.npz)."""
+    if not os.path.exists(REF_BC_SO):
+        pytest.skip("oracle/_ref/bc not built (make -C oracle ref-bc)")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as P
+    import refproject as RP
+    import refrun as R
+    root = os.path.join(GOLD, "ponded_project")
+    pr = PJ.load_project(root)
+    m = pr.mesh
+    rp = RP.RefProject(P.RefLib(P.REF_BC_SO), root)
+    try:
+        assert rp.scalar("IPOND") == 1 and rp.scalar("INDP") == 2
+        assert np.array_equal(pr.routing["pond0"], rp.a["PONDNOD"][:m.nnod])
+        assert np.allclose(pr.psi0[:m.nnod], 0.02, rtol=0, atol=1e-12)
+        assert np.allclose(pr.psi0, np.tile(m.z[:m.nnod] + 0.02, m.nstr + 1) - m.z, rtol=0, atol=1e-12)
+        io = lambda k: os.path.join(root, "input", k)
+        gold = np.load(os.path.join(GOLD, "ponded_ref.npz"))
+        names = [str(v) for v in gold["stat_names"]]
+        rr = R.RefFlowRun(rp.ref, m, pr.soil, pr.psi0, 0.0, pr.params.as_dict(), atmbc=io("atmbc"),
+                          dirbc=io("nansfdirbc"), neubc=io("nansfneubc"), veg=pr.veg, sfbc=io("sfbc"), surf=True,
+                          routing=rp, pond0=pr.routing["pond0"])
+        for k in range(3):
+            rc, st = rr.step()
+            assert [st[nm] for nm in names] == list(gold["stats"][k]), k
+        assert rr.routed >= 3
+        rr.close()
+    finally:
+        rp.close()
