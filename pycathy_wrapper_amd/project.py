@@ -210,8 +210,10 @@ def read_atmbc(path: str, nnod: int):
 
 def read_raster(path: str):
     """A GRASS ASCII raster as the reference's RAST_INPUT_* read it: the six
-    header lines (north, south, east, west, rows, cols), then rows x cols
-    values, top row first."""
+    header lines (north, south, east, west, rows, cols), then one list-
+    directed READ per row, top row first (rast_input_dem.f:48-50): each row
+    starts on a new line, takes cols values (across lines if need be) and
+    drops what its last line holds beyond them."""
     with open(path) as f:
         lines = f.read().splitlines()
     hdr = {}
@@ -219,10 +221,18 @@ def read_raster(path: str):
         k, v = ln.split(":", 1)
         hdr[k.strip()] = float(v)
     nrow, ncol = int(hdr["rows"]), int(hdr["cols"])
-    vals = np.array(" ".join(lines[6:]).split(), dtype=np.float64)
-    if vals.size < nrow * ncol:
-        raise ValueError(f"{path}: {vals.size} values for a {nrow} x {ncol} raster")
-    return vals[: nrow * ncol].reshape(nrow, ncol), hdr
+    out = np.empty((nrow, ncol))
+    li = 6
+    for r in range(nrow):
+        vals = []
+        while len(vals) < ncol:
+            if li >= len(lines):
+                raise ValueError(f"{path}: the file ends at row {r + 1} of the {nrow} x {ncol} its header "
+                                 f"declares (the reference's read runs past the end)")
+            vals.extend(lines[li].split())
+            li += 1
+        out[r] = np.array(vals[:ncol], dtype=np.float64)
+    return out, hdr
 
 
 def read_dem_parameters(path: str) -> dict:
@@ -329,6 +339,9 @@ def load_project(root: str, subsurface_only: bool = False, files: dict | None = 
     dp = read_dem_parameters(fl["dem_parameters"])
     dem, hdr = read_raster(fl["dem"])
     zone = read_raster(fl["zone"])[0].astype(np.int32) if fl.get("zone") else np.ones(dem.shape, np.int32)
+    if zone.shape != dem.shape:
+        raise ValueError(f"project {root}: the zone raster is {zone.shape[0]} x {zone.shape[1]}, the DEM "
+                         f"{dem.shape[0]} x {dem.shape[1]}")
     m = M.build_mesh(dem, dp["dx"], dp["dy"], dp["nstr"], dp["zratio"], dp["base"], zone=zone, ivert=dp["ivert"],
                      west=hdr["west"], south=hdr["south"], factor=dp["factor"])
     # vegetation per surface node: TRIANGOLI's cell-to-node rule on the map

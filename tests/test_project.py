@@ -369,3 +369,18 @@ def test_ponded_start_reader_and_reference():
         rr.close()
     finally:
         rp.close()
+
+
+def test_raster_rows_read_as_the_reference_reads_them(tmp_path):
+    """RAST_INPUT_* read one row per list-directed READ: a row starts on a
+    new line, may continue over the next ones, and drops what its last line
+    holds beyond the row; a file that ends early is an error (the reference
+    reads past its end)."""
+    hdr = "north: 0\nsouth: 0\neast: 0\nwest: 0\nrows: 2\ncols: 3\n"
+    p = tmp_path / "r"
+    p.write_text(hdr + "1 2 3 99 98\n4\n5 6 97\n")
+    r, h = PJ.read_raster(str(p))
+    assert r.tolist() == [[1, 2, 3], [4, 5, 6]] and h["rows"] == 2
+    p.write_text(hdr + "1 2 3 4 5 6\n")
+    with pytest.raises(ValueError, match="ends at row 2"):
+        PJ.read_raster(str(p))
