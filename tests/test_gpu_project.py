@@ -143,14 +143,15 @@ def _stored(name):
     return rows, dict(zip(out["psi_nstep"].tolist(), out["psi"])), dict(zip(out["sw_nstep"].tolist(), out["sw"]))
 
 
-@pytest.mark.parametrize("name", ["weil", "spatial_atmbc", "soil3d", "nodata"])
+@pytest.mark.parametrize("name", ["weil", "spatial_atmbc", "soil3d", "nodata", "camporese"])
 @pytest.mark.parametrize("ordering", ["natural", "column3"])
 def test_routed_project_matches_reference(name, ordering):
     """Projects that pond, run as the reference program ran them with the
     routing block before every step that starts ponded: weil_exemple (235
     steps to TMAX = 7200 s), atmbc_spatially_from_weill (per-node rainfall,
     1027 steps to 86400 s), soil3d_from_weill (400 soil zones, the 40 steps
-    its stored output holds), all ponded from their first step on, and
+    its stored output holds), meshing_from_Camporese (235 steps), all
+    ponded from their first step on, and
     atmbc_spatially_from_weill_withnodata (a DEM with no-data cells, the 180
     steps its output holds, ponded at 15 step ends)."""
     from pycathy_wrapper_amd import gpu as G
@@ -219,20 +220,30 @@ def test_routed_project_matches_reference(name, ordering):
     say(f"{name} {ordering}: {k} steps, worst psi rel {worst:.2e}")
 
 
+@pytest.mark.parametrize("name", ["ponded", "mesh10x20"])
 @pytest.mark.parametrize("ordering", ["natural", "column3"])
-def test_ponded_start_matches_reference(ordering):
-    """weil_exemple started ponded (input/ic: INDP 2, IPOND 1, a 2 cm
-    ponding head everywhere; tests/golden/ponded_project): the run starts
-    with PONDING set (inital.f:201-207) and routes before its first step.
-    60 steps against the reference's run of the same files
-    (tests/golden/ponded_ref.npz), as for the other routed projects."""
+def test_project_matches_reference_harness(name, ordering):
+    """Routed runs pinned on the reference's routines alone (no stored
+    output of the program belongs to these inputs): weil_exemple started
+    ponded (input/ic: INDP 2, IPOND 1, a 2 cm ponding head everywhere;
+    tests/golden/ponded_project), which starts with PONDING set
+    (inital.f:201-207) and routes before its first step, 60 steps; and
+    meshing_from_weill, a 10 x 20 DEM (NROW != NCOL through the cell
+    numbering and the drainage network), 195 steps to its NOBACK stop.
+    Against the reference's run of the same files
+    (tests/golden/{ponded,mesh10x20}_ref.npz), as for the other routed
+    projects."""
     from pycathy_wrapper_amd import gpu as G
-    gold = np.load(os.path.join(GOLD, "ponded_ref.npz"))
+    gold = np.load(os.path.join(GOLD, f"{name}_ref.npz"))
     col = {nm: j for j, nm in enumerate(str(v) for v in gold["stat_names"])}
     stats = gold["stats"]
     saved = list(gold["steps"])
-    pr = PJ.load_project(os.path.join(GOLD, "ponded_project"))
-    assert pr.routing is not None and np.all(pr.routing["pond0"] == 0.02)
+    pr = PJ.load_project(os.path.join(GOLD, f"{name}_project"))
+    assert pr.routing is not None
+    if name == "ponded":
+        assert np.all(pr.routing["pond0"] == 0.02)
+    else:
+        assert (pr.routing["nrow"], pr.routing["ncol"]) == (10, 20)
     g = G.FlowGPU(pr, G.ORDER_NATURAL if ordering == "natural" else G.ORDER_COLUMN3)
     scale = abs(stats[:, col["STORE1"]]).max() * (1e-6 if ordering == "natural" else 1e-4)
     try:
@@ -255,11 +266,11 @@ def test_ponded_start_matches_reference(ordering):
                 pond = g.routing_state()[0]
                 assert np.max(np.abs(pond - gold["pondnod"][i])) <= PSI_TOL * max(np.abs(gold["pondnod"][i]).max(),
                                                                                   1e-3), k
-            if k == 1:
+            if k == 1 and name == "ponded":
                 assert g.routing_state()[3] == 1 + s["BACK"]   # the first step routed
             if rc:
                 break
-        assert k == len(stats)
+        assert k == len(stats) and rc == int(gold["rc_last"]), (k, rc)
         assert g.routing_state()[3] == int(gold["routed"])
     finally:
         g.close()

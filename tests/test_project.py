@@ -148,7 +148,7 @@ def _mbeconv(name):
     return rows
 
 
-@pytest.mark.parametrize("name", ["bc_noflow", "outlets", "spatial_atmbc", "soil3d", "nodata"])
+@pytest.mark.parametrize("name", ["bc_noflow", "outlets", "spatial_atmbc", "soil3d", "nodata", "camporese"])
 def test_coupled_runs_equal_reference_program_outputs(name):
     gold = np.load(os.path.join(GOLD, f"{name}_ref.npz"))
     col = {nm: j for j, nm in enumerate(str(v) for v in gold["stat_names"])}
@@ -166,9 +166,9 @@ def test_coupled_runs_equal_reference_program_outputs(name):
         # bc_noflow and outlets never pond (their routing never runs);
         # spatial_atmbc and soil3d pond from their first step on and route
         # every later one; nodata ponds now and then
-        if name in ("bc_noflow", "outlets", "spatial_atmbc", "soil3d"):
+        if name in ("bc_noflow", "outlets", "spatial_atmbc", "soil3d", "camporese"):
             assert stats[k, col["PONDING"]] == (0 if name in ("bc_noflow", "outlets") else 1)
-    if name in ("spatial_atmbc", "soil3d"):
+    if name in ("spatial_atmbc", "soil3d", "camporese"):
         assert int(gold["routed"]) == len(stats) - 1
     if name == "nodata":
         npond = int(stats[:-1, col["PONDING"]].sum())
@@ -232,8 +232,10 @@ def test_weil_coupled_run_equals_reference_program():
     assert np.any(gold["pondnod"][-1] > 0.0)
 
 
-def test_reader_matches_reference_datin():
-    """project.load_project on weil_exemple against the reference's own
+@pytest.mark.parametrize("name", ["weil_project", "mesh10x20_project", "camporese_project"])
+def test_reader_matches_reference_datin(name):
+    """project.load_project on weil_exemple, meshing_from_weill (a 10 x 20
+    DEM) and meshing_from_Camporese against the reference's own
     DATIN reading the same files (oracle/refproject.py): every parm and
     soil value DATIN returns, the surface mesh (TRIANG, X, Y, Z), the
     vegetation types, and the drainage network and cell numbering of the
@@ -244,7 +246,7 @@ def test_reader_matches_reference_datin():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as P
     import refproject as RP
-    root = os.path.join(GOLD, "weil_project")
+    root = os.path.join(GOLD, name)
     rp = RP.RefProject(P.RefLib(P.REF_BC_SO), root)
     try:
         pr = PJ.load_project(root)
