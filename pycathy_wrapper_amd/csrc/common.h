@@ -18,6 +18,9 @@ constexpr int kBlock = 256;
 // Upper bound on work-groups of reduction kernels: partial sums are
 // written per work-group and summed in a fixed order (deterministic).
 constexpr int kMaxRedBlocks = 8192;
+// Work-groups of the PCG vector kernels (alpha, dot, beta): a grid-stride
+// loop over the rows, one partial sum per group.
+constexpr int kVecGroups = 512;
 
 struct Error : std::runtime_error {
   int code;

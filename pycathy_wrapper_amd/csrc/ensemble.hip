@@ -30,7 +30,7 @@ namespace {
 constexpr int kMaxMembers = 512;
 
 // launch shapes of the batched kernels: groups per member
-enum Shape { kShRows = 0, kShRed, kShNnod, kShFlux, kShOne, kShCol0, kNShape = kShCol0 + 4 };
+enum Shape { kShRows = 0, kShRed, kShNnod, kShFlux, kShOne, kShVec, kShCol0, kNShape = kShCol0 + 4 };
 
 int shape_groups(const Flow &f, int sh) {
   switch (sh) {
@@ -39,6 +39,7 @@ int shape_groups(const Flow &f, int sh) {
     case kShNnod: return grid_for(f.nnod);
     case kShFlux: return grid_for(f.nnod, kBlock, kMaxRedBlocks);
     case kShOne: return 1;
+    case kShVec: return std::min(grid_for(f.n), kVecGroups);
     default: {
       const int c = sh - kShCol0;
       return c < (int)f.sched.cols.size() ? grid_for(f.sched.cols[c].nc, kChains) : 0;
@@ -193,6 +194,7 @@ struct Ensemble {
     d.nq = f.prm.free_drainage ? f.nnod : 0;
     d.g_rows = grid_for(f.n);
     d.g_red = std::min(grid_for(f.n), 1024);
+    d.g_vec = std::min(grid_for(f.n), kVecGroups);
     d.g_flux = grid_for(f.nnod, kBlock, kMaxRedBlocks);
     d.tetaf = f.prm.tetaf;
     d.rdt = 1.0 / f.deltat;
@@ -289,10 +291,10 @@ struct Ensemble {
   // one PCG iteration for every member still iterating (graddp.f:1325-1377)
   void pcg_iteration() {
     launch(kb_spmv_dot2, kShRows, 0);
-    launch(kb_pcg_alpha, kShRed, 0);
+    launch(kb_pcg_alpha, kShVec, 0);
     precond(kVecR, kVecZ, 1);
-    launch(kb_dot, kShRed, 0);
-    launch(kb_pcg_beta, kShRed, 0);
+    launch(kb_dot, kShVec, 0);
+    launch(kb_pcg_beta, kShVec, 0);
   }
   // the Picard iteration up to the PCG loop: CHPIC0, BCPIC rows, the fused
   // assembly (picard.f:74-165), IC(0), x0 = M^-1 b, r, p = M^-1 r

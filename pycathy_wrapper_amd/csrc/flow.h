@@ -1259,7 +1259,7 @@ struct Flow {
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kBlock), 0, st, G, P(0), scal.p, done, tol, imax);
     precond_apply(r.p, p.p, nullptr);  // p = M^-1 r
     const int Gs = grid_for(n, kBlock);  // SpMV: one thread per row
-    const int Gr = std::min(grid_for(n, kBlock), 1024);  // vector-kernel groups
+    const int Gr = std::min(grid_for(n, kBlock), kVecGroups);
     // the SpMV reads the lower triangle through the transpose where the chain
     // tables hold every row's transposed slots (column ordering, coded rows)
     const FacTab fts{(spmv_transposed && ftab_ok && sell().coded && maxrow <= kUnroll) ? ftab.p : nullptr, ftab_nch};

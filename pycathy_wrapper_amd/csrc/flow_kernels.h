@@ -397,7 +397,8 @@ __global__ void k_fill(size_t n, double v, double *dst);
 // its own launches would, bit for bit.
 struct MemberDev {
   int n, nnod, nq;             // rows, surface nodes, base nodes with free drainage (0: none)
-  int g_rows, g_red, g_flux;   // groups of the member's own row / vector / flux launches
+  int g_rows, g_red, g_flux;   // groups of the member's own row / reduction / flux launches
+  int g_vec;                   // groups of its PCG vector launches (alpha, dot, beta)
   double tetaf, rdt, tol, pmin;
   int imax, ncol;
   Sell sl;

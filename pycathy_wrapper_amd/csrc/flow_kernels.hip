@@ -2355,7 +2355,7 @@ __global__ void kb_dot(const int *start, int nmem, const MemberDev *md) {
 }
 __global__ void kb_pcg_beta(const int *start, int nmem, const MemberDev *md) {
   CG_BATCH_HEAD
-  pcg_beta_body(bk, d.n, d.g_red, slot_part(d, 4), slot_part(d, 3), d.z, d.p, d.scal, d.flags);
+  pcg_beta_body(bk, d.n, d.g_vec, slot_part(d, 4), slot_part(d, 3), d.z, d.p, d.scal, d.flags);
 }
 // out[3m..3m+2] = done flag, iterations, last relative residual of member m
 __global__ void kb_poll(const int *start, int nmem, const MemberDev *md, double *out) {
